@@ -1,0 +1,184 @@
+/*
+ * svfsi_gpu — MI355X-native fluid Newton step for svFSIplus (C ABI).
+ *
+ * The two reference entry points this library replaces on the device:
+ *
+ *   void fluid::construct_fluid(ComMod& com_mod, const mshType& lM,
+ *                               const Array<double>& Ag, const Array<double>& Yg)
+ *        Code/Source/solver/fluid.cpp:464  (declared in fluid.h)
+ *   void fsi_linear_solver::fsils_solve(FSILS_lhsType& lhs, FSILS_lsType& ls,
+ *        const int dof, Array<double>& Ri, Array<double>& Val,
+ *        const consts::PreconditionerType prec, const Vector<int>& incL,
+ *        const Vector<double>& res)
+ *        Code/Source/liner_solver/fsils_api.hpp:60-61, solve.cpp:50
+ *
+ * All arrays keep the reference layouts (column-major Array<T>,
+ * Code/Source/solver/Array.h:374-388):
+ *   x(3,nNo)  IEN(4,nEl)  Ag/Yg/R(4,nNo)  Val(16,nnz) with block j at
+ *   Val[16j .. 16j+15], row-major inside the 4x4 block (spar_mul.cpp:232-235).
+ *   rowPtr(nNo+1) is the exclusive-end CSR of com_mod.rowPtr; colPtr(nnz)
+ *   holds sorted column node ids per row (lhsa.cpp:366-379).
+ *
+ * Pointers named d_* are device (HBM) pointers; the others are host pointers.
+ * Every function returns SVG_OK (0) or an SVG_ERR_* code; svg_last_error()
+ * gives the message (the reference throws std::runtime_error on the same
+ * conditions). A stream argument may be NULL for the default stream.
+ * Calls on one svg_lhs / svg_msh are not re-entrant (as in the reference).
+ */
+#ifndef SVFSI_GPU_H
+#define SVFSI_GPU_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SVG_OK 0
+#define SVG_ERR_ARG 1          /* bad argument / size                                  */
+#define SVG_ERR_HIP 2          /* HIP runtime error (allocation, launch)               */
+#define SVG_ERR_JACOBIAN 3     /* fluid.cpp:612-621  zero element Jacobian             */
+#define SVG_ERR_SINGULAR 4     /* ns_solver.cpp:386-391 "Singular matrix detected"     */
+#define SVG_ERR_ILLCOND 5      /* ns_solver.cpp:464-472 ill-conditioned LHS            */
+#define SVG_ERR_LSTYPE 6       /* solve.cpp:179 "LS_type not defined"                  */
+#define SVG_ERR_NEU_RES 7      /* solve.cpp:97 "res is required for Neu surfaces"      */
+#define SVG_ERR_UNSUPPORTED 8  /* a reference feature this library does not provide   */
+
+/* Linear solver types (fils_struct.hpp:71-77) and preconditioners (consts.h:426-440). */
+#define SVG_LS_BICGS 795
+#define SVG_LS_NS 796
+#define SVG_LS_GMRES 797
+#define SVG_LS_CG 798
+#define SVG_PREC_NONE 700
+#define SVG_PREC_FSILS 701
+
+/* FSILS face groups (fils_struct.hpp:60-64). */
+#define SVG_BC_DIR 0
+#define SVG_BC_NEU 1
+
+/* Node flags written by the pipe generator. */
+#define SVG_NODE_INLET 1
+#define SVG_NODE_WALL 2
+#define SVG_NODE_OUTLET 4
+
+const char* svg_last_error(void);
+
+/* ------------------------------------------------------------------------ */
+/* Synthetic workload generator (setup, not timed).                          */
+/* Straight pipe of radius `radius`, length `length` along z: an n x n hex   */
+/* cross-section (n even) mapped onto the disc by the elliptical map         */
+/* (s,t) -> (s sqrt(1-t^2/2), t sqrt(1-s^2/2)), nz hexes axially, 6 tets per */
+/* hex (Kuhn split mirrored per quadrant), interior nodes jittered by        */
+/* U(-jitter, jitter) * h per coordinate from a splitmix64 stream seeded by  */
+/* `seed`. Optional stenosis: radius scaled by 1 - depth*(1-cos(2pi(z-z0)/len))/2 */
+/* on z0 <= z <= z0+len.                                                      */
+typedef struct {
+  int n, nz;
+  double radius, length, jitter;
+  unsigned long long seed;
+  double sten_depth, sten_z0, sten_len;
+} svg_pipe_spec;
+
+int svg_pipe_counts(const svg_pipe_spec* sp, int* nNo, int* nEl);
+int svg_pipe_generate(const svg_pipe_spec* sp, double* x, int* IEN, int* flags);
+
+/* Block-CSR pattern of a mesh (what lhsa builds, lhsa.cpp:153-380). */
+int svg_csr_pattern_count(int nNo, int nEl, int eNoN, const int* IEN, int* nnz);
+int svg_csr_pattern(int nNo, int nEl, int eNoN, const int* IEN, int* rowPtr, int* colPtr);
+/* Greedy element colouring (no two elements of one colour share a node). */
+int svg_color_elements(int nNo, int nEl, int eNoN, const int* IEN, int* color, int* nColors);
+
+/* ------------------------------------------------------------------------ */
+/* Device-resident FSILS lhs.                                                 */
+/* Replaces fsils_lhs_create (liner_solver/lhs.cpp:57-120, one rank: map is   */
+/* the identity, rowPtr pairs are [rowPtr(a), rowPtr(a+1)-1], diagPtr found   */
+/* by search) and fsils_bc_create (liner_solver/bc.cpp:45-130).               */
+typedef struct svg_lhs svg_lhs;
+
+int svg_lhs_create(int nNo, int nnz, const int* rowPtr, const int* colPtr, int nFaces, svg_lhs** lhs);
+int svg_lhs_bc_create(svg_lhs* lhs, int faIn, int nNo, int dof, int bGrp, const int* gNodes, const double* val);
+int svg_lhs_free(svg_lhs* lhs);
+
+/* ------------------------------------------------------------------------ */
+/* Element assembly: construct_fluid (fluid.cpp:464-708) for TET4 P1-P1 VMS  */
+/* with constant viscosity, one fluid domain, fixed mesh (mvMsh false).       */
+typedef struct svg_msh svg_msh;
+
+/* Uploads x and IEN, colours the elements and builds the element -> block
+ * slot map over `lhs`'s pattern (the binary search of do_assem, lhsa.cpp:123-134). */
+int svg_msh_create(const svg_lhs* lhs, int nEl, const int* IEN, int nNo, const double* x, svg_msh** msh);
+int svg_msh_free(svg_msh* msh);
+int svg_msh_num_colors(const svg_msh* msh, int* nColors);
+
+typedef struct {
+  double rho;      /* fluid_density                                   */
+  double mu;       /* viscosity (viscType_Const, fluid.cpp:2158-2162)  */
+  double f[3];     /* body force f_x, f_y, f_z                         */
+  double K_darcy;  /* inverse_darcy_permeability                       */
+  double af, am, gam; /* generalized-alpha (initialize.cpp:430-463)    */
+  double dt;
+  double qmTET4;   /* quadrature modifier, default (5+3 sqrt5)/20      */
+} svg_fluid_params;
+
+/* R(4,nNo) += element residuals, Val(16,nnz) += element tangents (the
+ * caller zeroes them first, as ls_alloc does). d_Bf may be NULL (zero). */
+int svg_construct_fluid(svg_msh* msh, const svg_fluid_params* p, const double* d_Ag, const double* d_Yg,
+                        const double* d_Bf, double* d_R, double* d_Val, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* FSILS solve (solve.cpp:50-193). Mirrors FSILS_subLsType / FSILS_lsType    */
+/* (fils_struct.hpp:216-283).                                                */
+typedef struct {
+  int suc, mItr, sD, itr, cM, cN, cD;
+  double absTol, relTol, iNorm, fNorm, dB, callD;
+} svg_subls;
+
+typedef struct {
+  int LS_type;
+  int Resm, Resc;
+  svg_subls GM, CG, RI;
+} svg_ls;
+
+/* fsils_ls_create defaults (ls.cpp:40-120) without the optional arguments. */
+int svg_ls_create(svg_ls* ls, int LS_type);
+
+/* Ri(dof,nNo): residual in, Newton increment out. Val(dof*dof,nnz) is scaled
+ * in place by the diagonal preconditioner, as in the reference. dof = 4.
+ * incL(nIncL)/res(nIncL) have the reference meaning (res may be NULL when no
+ * Neumann face is registered). Synchronous: returns when Ri holds the result
+ * and ls holds the iteration record. */
+int svg_fsils_solve(svg_lhs* lhs, svg_ls* ls, int dof, double* d_Ri, double* d_Val, int prec, int nIncL,
+                    const int* incL, const double* res, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Kernels of the solve, exposed for unit parity tests.                      */
+/* fsils_spar_mul_vv (spar_mul.cpp:191-258), dof in {1..4}.                  */
+int svg_spar_mul_vv(const svg_lhs* lhs, int dof, const double* d_K, const double* d_U, double* d_KU, void* stream);
+/* precond_diag (precond.cpp:122-257) with every registered face included. */
+int svg_precond_diag(svg_lhs* lhs, int dof, double* d_Val, double* d_R, double* d_W, void* stream);
+/* fsils_dot_v / fsi_ls_normv (dot.cpp:69-117, norm.cpp:67-112); blocking. */
+int svg_dot_v(int dof, int nNo, const double* d_U, const double* d_V, double* out, void* stream);
+int svg_norm_v(int dof, int nNo, const double* d_U, double* out, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Generalized-alpha pieces of a Newton iteration (pic.cpp).                 */
+/* pici (pic.cpp:486-574): Ag = (1-am) Ao + am An, Yg = (1-af) Yo + af Yn.   */
+int svg_pici(int nNo, double am, double af, const double* d_Ao, const double* d_An, const double* d_Yo,
+             const double* d_Yn, double* d_Ag, double* d_Yg, void* stream);
+/* picc update (pic.cpp:155-167): An -= R, Yn -= gam dt R. */
+int svg_picc_update(int nNo, double gam, double dt, const double* d_R, double* d_An, double* d_Yn, void* stream);
+/* d[0..n) = v (a fill kernel, safe under graph capture). */
+int svg_fill(double* d, size_t n, double v, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Kernel timing with HIP events on the launching stream.                    */
+/* When enabled, every launch of the named kernels is bracketed by events;   */
+/* svg_prof_get returns the launch count and summed duration (ms).           */
+int svg_prof_enable(int on);
+int svg_prof_reset(void);
+int svg_prof_get(const char* name, int* count, double* total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,260 @@
+"""Python binding of the CPU oracle (test infrastructure only).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this package, and only as the checker / CPU baseline. The product library
+(svfsiplus_amd) never imports or links it.
+
+liboracle.so is the C restatement in this directory; _ref/libsvref.so (if
+built in this container) is the reference's own code compiled from its
+sources, see ref.mk.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+REF_PATH = HERE / "_ref" / "libsvref.so"
+
+LS_NS, LS_GMRES, LS_CG = 796, 797, 798
+PREC_FSILS = 701
+
+
+class FluidParams(C.Structure):
+    _fields_ = [("rho", C.c_double), ("mu", C.c_double), ("f", C.c_double * 3), ("K_darcy", C.c_double),
+                ("af", C.c_double), ("am", C.c_double), ("gam", C.c_double), ("dt", C.c_double),
+                ("qmTET4", C.c_double)]
+
+
+class SubLs(C.Structure):
+    _fields_ = [("suc", C.c_int), ("mItr", C.c_int), ("sD", C.c_int), ("itr", C.c_int), ("cM", C.c_int),
+                ("cN", C.c_int), ("cD", C.c_int), ("absTol", C.c_double), ("relTol", C.c_double),
+                ("iNorm", C.c_double), ("fNorm", C.c_double), ("dB", C.c_double), ("callD", C.c_double)]
+
+
+class Ls(C.Structure):
+    _fields_ = [("LS_type", C.c_int), ("Resm", C.c_int), ("Resc", C.c_int), ("GM", SubLs), ("CG", SubLs),
+                ("RI", SubLs)]
+
+
+class Face(C.Structure):
+    _fields_ = [("nNo", C.c_int), ("dof", C.c_int), ("bGrp", C.c_int), ("incFlag", C.c_int),
+                ("coupledFlag", C.c_int), ("glob", C.c_void_p), ("val", C.c_void_p)]
+
+
+class Lhs(C.Structure):
+    _fields_ = [("nNo", C.c_int), ("nnz", C.c_int), ("rowPtr2", C.c_void_p), ("colPtr", C.c_void_p),
+                ("diagPtr", C.c_void_p), ("nFaces", C.c_int), ("face", C.POINTER(Face))]
+
+
+_lib = None
+_ref = None
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+        lib = C.CDLL(str(LIB_PATH))
+        lib.orc_last_error.restype = C.c_char_p
+        lib.orc_fsils_last_error.restype = C.c_char_p
+        lib.orc_dot_v.restype = C.c_double
+        lib.orc_dot_s.restype = C.c_double
+        lib.orc_norm_v.restype = C.c_double
+        lib.orc_norm_s.restype = C.c_double
+        for fn in ("orc_pici", "orc_picc_update"):
+            getattr(lib, fn).restype = None
+        _lib = lib
+    return _lib
+
+
+def ref_available() -> bool:
+    return REF_PATH.exists()
+
+
+def load_ref():
+    global _ref
+    if _ref is None:
+        lib = C.CDLL(str(REF_PATH))
+        lib.ref_last_error.restype = C.c_char_p
+        _ref = lib
+    return _ref
+
+
+def _p(a: np.ndarray) -> int:
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data
+
+
+def fluid_params(phys) -> FluidParams:
+    p = FluidParams()
+    p.rho, p.mu = phys.rho, phys.mu
+    p.f[0], p.f[1], p.f[2] = phys.f
+    p.K_darcy = phys.K_darcy
+    p.af, p.am, p.gam, p.dt = phys.af, phys.am, phys.gam, phys.dt
+    p.qmTET4 = phys.qmTET4
+    return p
+
+
+def construct_fluid(pipe, phys, Ag, Yg, Bf=None):
+    """Oracle construct_fluid: returns (R (nNo,4), Val (nnz,16)) from zero."""
+    lib = load()
+    R = np.zeros((pipe.nNo, 4))
+    Val = np.zeros((pipe.nnz, 16))
+    x = np.ascontiguousarray(pipe.x)
+    IEN = np.ascontiguousarray(pipe.IEN, dtype=np.int32)
+    Ag = np.ascontiguousarray(Ag, dtype=np.float64)
+    Yg = np.ascontiguousarray(Yg, dtype=np.float64)
+    bf = None if Bf is None else np.ascontiguousarray(Bf, dtype=np.float64)
+    prm = fluid_params(phys)
+    rc = lib.orc_construct_fluid(C.c_int(pipe.nEl), C.c_void_p(_p(IEN)), C.c_void_p(_p(x)),
+                                 C.c_void_p(None if bf is None else _p(bf)), C.c_void_p(_p(Ag)), C.c_void_p(_p(Yg)),
+                                 C.byref(prm), C.c_void_p(_p(pipe.rowPtr)), C.c_void_p(_p(pipe.colPtr)),
+                                 C.c_void_p(_p(R)), C.c_void_p(_p(Val)))
+    if rc:
+        raise RuntimeError(lib.orc_last_error().decode())
+    return R, Val
+
+
+def ref_construct_fluid(pipe, phys, Ag, Yg, Bf=None):
+    """The reference's own construct_fluid (oracle/_ref)."""
+    lib = load_ref()
+    R = np.zeros((pipe.nNo, 4))
+    Val = np.zeros((pipe.nnz, 16))
+    prm = np.array([phys.rho, phys.mu, phys.f[0], phys.f[1], phys.f[2], phys.K_darcy, phys.af, phys.am, phys.gam,
+                    phys.dt])
+    x = np.ascontiguousarray(pipe.x)
+    IEN = np.ascontiguousarray(pipe.IEN, dtype=np.int32)
+    Ag = np.ascontiguousarray(Ag, dtype=np.float64)
+    Yg = np.ascontiguousarray(Yg, dtype=np.float64)
+    bf = None if Bf is None else np.ascontiguousarray(Bf, dtype=np.float64)
+    rc = lib.ref_construct_fluid(C.c_int(pipe.nEl), C.c_void_p(_p(IEN)), C.c_int(pipe.nNo), C.c_void_p(_p(x)),
+                                 C.c_void_p(None if bf is None else _p(bf)), C.c_void_p(_p(Ag)), C.c_void_p(_p(Yg)),
+                                 C.c_void_p(_p(prm)), C.c_int(pipe.nnz), C.c_void_p(_p(pipe.rowPtr)),
+                                 C.c_void_p(_p(pipe.colPtr)), C.c_void_p(_p(R)), C.c_void_p(_p(Val)))
+    if rc:
+        raise RuntimeError(lib.ref_last_error().decode())
+    return R, Val
+
+
+class OracleLhs:
+    """Holds the numpy arrays behind an orc_lhs."""
+
+    def __init__(self, pipe, faces=None):
+        self.rowPtr2 = np.ascontiguousarray(pipe.rowPtr2())
+        self.colPtr = np.ascontiguousarray(pipe.colPtr, dtype=np.int32)
+        self.diagPtr = np.ascontiguousarray(pipe.diagPtr())
+        faces = pipe.faces if faces is None else faces
+        self._glob = [np.ascontiguousarray(f[1], dtype=np.int32) for f in faces]
+        self._val = [np.zeros((len(f[1]), 3)) for f in faces]
+        self.faces = (Face * max(1, len(faces)))()
+        for i in range(len(faces)):
+            self.faces[i] = Face(len(self._glob[i]), 3, 0, 1, 0, _p(self._glob[i]), _p(self._val[i]))
+        self.lhs = Lhs(pipe.nNo, pipe.nnz, _p(self.rowPtr2), _p(self.colPtr), _p(self.diagPtr), len(faces),
+                       C.cast(self.faces, C.POINTER(Face)))
+
+
+def ls_create(ls_type, relTol=-1.0, absTol=-1.0, maxItr=-1, dimKry=-1, relTolIn=None, absTolIn=None, maxItrIn=None):
+    lib = load()
+    ls = Ls()
+    rti = (C.c_double * 2)(*relTolIn) if relTolIn else None
+    ati = (C.c_double * 2)(*absTolIn) if absTolIn else None
+    mti = (C.c_int * 2)(*maxItrIn) if maxItrIn else None
+    lib.orc_ls_create(C.byref(ls), C.c_int(ls_type), C.c_double(relTol), C.c_double(absTol), C.c_int(maxItr),
+                      C.c_int(dimKry), rti, ati, mti)
+    return ls
+
+
+def fsils_solve(pipe, ls, R, Val, prec=PREC_FSILS, faces=None):
+    """Oracle fsils_solve on copies; returns (increment (nNo,4), scaled Val, ls)."""
+    lib = load()
+    L = OracleLhs(pipe, faces)
+    Ri = np.ascontiguousarray(R, dtype=np.float64).copy()
+    V = np.ascontiguousarray(Val, dtype=np.float64).copy()
+    nf = L.lhs.nFaces
+    incL = np.ones(max(1, nf), dtype=np.int32)
+    res = np.zeros(max(1, nf))
+    rc = lib.orc_fsils_solve(C.byref(L.lhs), C.byref(ls), C.c_int(4), C.c_void_p(_p(Ri)), C.c_void_p(_p(V)),
+                             C.c_int(prec), C.c_int(nf), C.c_void_p(_p(incL)), C.c_void_p(_p(res)))
+    if rc:
+        raise RuntimeError(lib.orc_fsils_last_error().decode())
+    return Ri, V, ls
+
+
+def ref_fsils_solve(pipe, ls_type, ls_prm, R, Val, prec=PREC_FSILS, faces=None):
+    """The reference's own fsils_solve; ls_prm = (RI.relTol, RI.absTol, RI.mItr, RI.sD, GM.relTol, GM.absTol,
+    GM.mItr, GM.sD, CG.relTol, CG.absTol, CG.mItr). Returns (increment, scaled Val, out dict)."""
+    lib = load_ref()
+    faces = pipe.faces if faces is None else faces
+    nn = np.array([len(f[1]) for f in faces], dtype=np.int32)
+    glob = np.ascontiguousarray(np.concatenate([f[1] for f in faces]), dtype=np.int32)
+    val = np.zeros((glob.shape[0], 3))
+    Ri = np.ascontiguousarray(R, dtype=np.float64).copy()
+    V = np.ascontiguousarray(Val, dtype=np.float64).copy()
+    prm = np.array(ls_prm, dtype=np.float64)
+    out = np.zeros(9)
+    rc = lib.ref_fsils_solve(C.c_int(pipe.nNo), C.c_int(pipe.nnz), C.c_void_p(_p(pipe.rowPtr)),
+                             C.c_void_p(_p(pipe.colPtr)), C.c_int(len(faces)), C.c_void_p(_p(nn)),
+                             C.c_void_p(_p(glob)), C.c_void_p(_p(val)), C.c_int(ls_type), C.c_void_p(_p(prm)),
+                             C.c_int(4), C.c_void_p(_p(Ri)), C.c_void_p(_p(V)), C.c_int(prec), C.c_void_p(_p(out)))
+    if rc:
+        raise RuntimeError(lib.ref_last_error().decode())
+    keys = ["itr", "iNorm", "fNorm", "suc", "dB", "GM_itr", "CG_itr", "Resm", "Resc"]
+    return Ri, V, dict(zip(keys, out.tolist()))
+
+
+def ls_params(ls) -> tuple:
+    return (ls.RI.relTol, ls.RI.absTol, ls.RI.mItr, ls.RI.sD, ls.GM.relTol, ls.GM.absTol, ls.GM.mItr, ls.GM.sD,
+            ls.CG.relTol, ls.CG.absTol, ls.CG.mItr)
+
+
+def spar_mul_vv(pipe, dof, K, U):
+    lib = load()
+    L = OracleLhs(pipe, [])
+    K = np.ascontiguousarray(K, dtype=np.float64)
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    KU = np.zeros((pipe.nNo, dof))
+    lib.orc_spar_mul_vv(C.byref(L.lhs), C.c_int(dof), C.c_void_p(_p(K)), C.c_void_p(_p(U)), C.c_void_p(_p(KU)))
+    return KU
+
+
+def precond_diag(pipe, Val, R, faces=None):
+    lib = load()
+    L = OracleLhs(pipe, faces)
+    V = np.ascontiguousarray(Val, dtype=np.float64).copy()
+    Rr = np.ascontiguousarray(R, dtype=np.float64).copy()
+    W = np.zeros((pipe.nNo, 4))
+    lib.orc_precond_diag(C.byref(L.lhs), C.c_int(4), C.c_void_p(_p(V)), C.c_void_p(_p(Rr)), C.c_void_p(_p(W)))
+    return V, Rr, W
+
+
+def pici(phys, Ao, An, Yo, Yn):
+    lib = load()
+    nNo = Ao.shape[0]
+    Ag = np.zeros_like(Ao)
+    Yg = np.zeros_like(Ao)
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (Ao, An, Yo, Yn)]
+    lib.orc_pici(C.c_int(nNo), C.c_double(phys.am), C.c_double(phys.af), *[C.c_void_p(_p(a)) for a in arrs],
+                 C.c_void_p(_p(Ag)), C.c_void_p(_p(Yg)))
+    return Ag, Yg
+
+
+def picc_update(phys, R, An, Yn):
+    lib = load()
+    R = np.ascontiguousarray(R, dtype=np.float64)
+    lib.orc_picc_update(C.c_int(R.shape[0]), C.c_double(phys.gam), C.c_double(phys.dt), C.c_void_p(_p(R)),
+                        C.c_void_p(_p(An)), C.c_void_p(_p(Yn)))
+
+
+def ge(A, B):
+    lib = load()
+    A = np.asfortranarray(A, dtype=np.float64)
+    Bv = np.ascontiguousarray(B, dtype=np.float64).copy()
+    N = A.shape[0]
+    ok = lib.orc_ge(C.c_int(N), C.c_int(N), C.c_void_p(A.ctypes.data), C.c_void_p(_p(Bv)))
+    return bool(ok), Bv

@@ -1,0 +1,726 @@
+/*
+ * Oracle (test infrastructure only): restatement of the reference's FSILS
+ * linear-solver path for one MPI rank (nTasks == 1: every halo sum and
+ * allreduce is the identity, lhs.map is the identity, mynNo == nNo).
+ *
+ *   fsils_spar_mul_{ss,sv,vs,vv}   liner_solver/spar_mul.cpp:46-258
+ *   fsils_{nc_}dot_{s,v}           liner_solver/dot.cpp:48-171
+ *   fsi_ls_norm{s,v}               liner_solver/norm.cpp:50-112
+ *   omp_{mul,sum}_{s,v}            liner_solver/omp_la.cpp:39-144
+ *   precond_diag/pre_mul/pos_mul   liner_solver/precond.cpp:46-257, 549-624
+ *   ge                             liner_solver/ge.cpp:40-139
+ *   gmres (inner, dof=3)           liner_solver/gmres.cpp:91-280
+ *   gmres_v                        liner_solver/gmres.cpp:450-629
+ *   cgrad::schur                   liner_solver/cgrad.cpp:50-160
+ *   ns_solver + depart + bc_pre    liner_solver/ns_solver.cpp:56-499
+ *   fsils_ls_create                liner_solver/ls.cpp:40-120
+ *   fsils_solve                    liner_solver/solve.cpp:50-193
+ *
+ * Coupled (resistance) faces are not modelled: add_bc_mul and the face.nS
+ * terms are identities when no face has coupledFlag set, which is the case
+ * for the Dirichlet + zero-traction pipe configurations. A coupled face makes
+ * orc_fsils_solve return an error instead of computing something different.
+ */
+#include "oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+extern const char* orc_last_error(void);
+static char g_err2[256];
+
+#define RS(i) (lhs->rowPtr2[2 * (i)])
+#define RE(i) (lhs->rowPtr2[2 * (i) + 1])
+
+void orc_spar_mul_ss(const orc_lhs* lhs, const double* K, const double* U, double* KU)
+{
+  int nNo = lhs->nNo;
+  for (int i = 0; i < nNo; i++) KU[i] = 0.0;
+  for (int i = 0; i < nNo; i++)
+    for (int j = RS(i); j <= RE(i); j++) KU[i] = KU[i] + K[j] * U[lhs->colPtr[j]];
+}
+
+void orc_spar_mul_sv(const orc_lhs* lhs, int dof, const double* K, const double* U, double* KU)
+{
+  int nNo = lhs->nNo;
+  for (int i = 0; i < dof * nNo; i++) KU[i] = 0.0;
+  for (int i = 0; i < nNo; i++)
+    for (int j = RS(i); j <= RE(i); j++) {
+      int col = lhs->colPtr[j];
+      for (int m = 0; m < dof; m++) KU[m + dof * i] = KU[m + dof * i] + K[m + dof * j] * U[col];
+    }
+}
+
+void orc_spar_mul_vs(const orc_lhs* lhs, int dof, const double* K, const double* U, double* KU)
+{
+  int nNo = lhs->nNo;
+  for (int i = 0; i < nNo; i++) KU[i] = 0.0;
+  for (int i = 0; i < nNo; i++)
+    for (int j = RS(i); j <= RE(i); j++) {
+      int col = lhs->colPtr[j];
+      if (dof == 3) {
+        KU[i] = KU[i] + K[0 + 3 * j] * U[0 + 3 * col] + K[1 + 3 * j] * U[1 + 3 * col] + K[2 + 3 * j] * U[2 + 3 * col];
+      } else if (dof == 2) {
+        KU[i] = KU[i] + K[0 + 2 * j] * U[0 + 2 * col] + K[1 + 2 * j] * U[1 + 2 * col];
+      } else if (dof == 4) {
+        KU[i] = KU[i] + K[0 + 4 * j] * U[0 + 4 * col] + K[1 + 4 * j] * U[1 + 4 * col] +
+                K[2 + 4 * j] * U[2 + 4 * col] + K[3 + 4 * j] * U[3 + 4 * col];
+      } else {
+        double sum = 0.0;
+        for (int m = 0; m < dof; m++) sum += K[m + dof * j] * U[m + dof * col];
+        KU[i] = KU[i] + sum;
+      }
+    }
+}
+
+void orc_spar_mul_vv(const orc_lhs* lhs, int dof, const double* K, const double* U, double* KU)
+{
+  int nNo = lhs->nNo;
+  int d2 = dof * dof;
+  for (int i = 0; i < dof * nNo; i++) KU[i] = 0.0;
+  for (int i = 0; i < nNo; i++)
+    for (int j = RS(i); j <= RE(i); j++) {
+      int col = lhs->colPtr[j];
+      const double* k = K + (size_t)d2 * j;
+      const double* u = U + (size_t)dof * col;
+      if (dof == 4) {
+        KU[0 + 4 * i] = KU[0 + 4 * i] + k[0] * u[0] + k[1] * u[1] + k[2] * u[2] + k[3] * u[3];
+        KU[1 + 4 * i] = KU[1 + 4 * i] + k[4] * u[0] + k[5] * u[1] + k[6] * u[2] + k[7] * u[3];
+        KU[2 + 4 * i] = KU[2 + 4 * i] + k[8] * u[0] + k[9] * u[1] + k[10] * u[2] + k[11] * u[3];
+        KU[3 + 4 * i] = KU[3 + 4 * i] + k[12] * u[0] + k[13] * u[1] + k[14] * u[2] + k[15] * u[3];
+      } else if (dof == 3) {
+        KU[0 + 3 * i] = KU[0 + 3 * i] + k[0] * u[0] + k[1] * u[1] + k[2] * u[2];
+        KU[1 + 3 * i] = KU[1 + 3 * i] + k[3] * u[0] + k[4] * u[1] + k[5] * u[2];
+        KU[2 + 3 * i] = KU[2 + 3 * i] + k[6] * u[0] + k[7] * u[1] + k[8] * u[2];
+      } else if (dof == 2) {
+        KU[0 + 2 * i] = KU[0 + 2 * i] + k[0] * u[0] + k[1] * u[1];
+        KU[1 + 2 * i] = KU[1 + 2 * i] + k[2] * u[0] + k[3] * u[1];
+      } else if (dof == 1) {
+        KU[i] = KU[i] + k[0] * u[0];
+      } else {
+        for (int l = 0; l < dof; l++) {
+          double sum = 0.0;
+          for (int c = 0; c < dof; c++) sum += k[l * dof + c] * u[c];
+          KU[l + dof * i] = KU[l + dof * i] + sum;
+        }
+      }
+    }
+}
+
+double orc_dot_s(int nNo, const double* U, const double* V)
+{
+  double r = 0.0;
+  for (int i = 0; i < nNo; i++) r = r + U[i] * V[i];
+  return r;
+}
+
+double orc_dot_v(int dof, int nNo, const double* U, const double* V)
+{
+  double r = 0.0;
+  if (dof == 4) {
+    for (int i = 0; i < nNo; i++)
+      r = r + U[4 * i] * V[4 * i] + U[4 * i + 1] * V[4 * i + 1] + U[4 * i + 2] * V[4 * i + 2] + U[4 * i + 3] * V[4 * i + 3];
+  } else if (dof == 3) {
+    for (int i = 0; i < nNo; i++) r = r + U[3 * i] * V[3 * i] + U[3 * i + 1] * V[3 * i + 1] + U[3 * i + 2] * V[3 * i + 2];
+  } else if (dof == 2) {
+    for (int i = 0; i < nNo; i++) r = r + U[2 * i] * V[2 * i] + U[2 * i + 1] * V[2 * i + 1];
+  } else if (dof == 1) {
+    for (int i = 0; i < nNo; i++) r = r + U[i] * V[i];
+  } else {
+    /* The reference's default branch sits behind a misspelled case label
+     * (dot.cpp:96, "defualt:") and is unreachable; the result stays 0. */
+  }
+  return r;
+}
+
+double orc_norm_s(int nNo, const double* U)
+{
+  double r = 0.0;
+  for (int i = 0; i < nNo; i++) r = r + U[i] * U[i];
+  return sqrt(r);
+}
+
+double orc_norm_v(int dof, int nNo, const double* U)
+{
+  double r = 0.0;
+  if (dof == 4) {
+    for (int i = 0; i < nNo; i++)
+      r = r + U[4 * i] * U[4 * i] + U[4 * i + 1] * U[4 * i + 1] + U[4 * i + 2] * U[4 * i + 2] + U[4 * i + 3] * U[4 * i + 3];
+  } else if (dof == 3) {
+    for (int i = 0; i < nNo; i++) r = r + U[3 * i] * U[3 * i] + U[3 * i + 1] * U[3 * i + 1] + U[3 * i + 2] * U[3 * i + 2];
+  } else if (dof == 2) {
+    for (int i = 0; i < nNo; i++) r = r + U[2 * i] * U[2 * i] + U[2 * i + 1] * U[2 * i + 1];
+  } else if (dof == 1) {
+    for (int i = 0; i < nNo; i++) r = r + U[i] * U[i];
+  } else {
+    for (int i = 0; i < nNo; i++)
+      for (int j = 0; j < dof; j++) r = r + U[j + dof * i] * U[j + dof * i];
+  }
+  return sqrt(r);
+}
+
+static void omp_sum(int n, double r, double* U, const double* V)
+{
+  for (int i = 0; i < n; i++) U[i] = U[i] + r * V[i];
+}
+
+static void omp_mul(int n, double r, double* U)
+{
+  for (int i = 0; i < n; i++) U[i] = r * U[i];
+}
+
+void orc_precond_diag(const orc_lhs* lhs, int dof, double* Val, double* R, double* W)
+{
+  int nNo = lhs->nNo;
+  int d2 = dof * dof;
+  for (int Ac = 0; Ac < nNo; Ac++) {
+    int d = lhs->diagPtr[Ac];
+    for (int i = 0; i < dof; i++) W[i + dof * Ac] = Val[i * dof + i + d2 * d];
+  }
+  for (int i = 0; i < dof * nNo; i++)
+    if (W[i] == 0.0) W[i] = 1.0;
+  for (int i = 0; i < dof * nNo; i++) W[i] = 1.0 / sqrt(fabs(W[i]));
+  for (int f = 0; f < lhs->nFaces; f++) {
+    const orc_face* face = &lhs->face[f];
+    if (!face->incFlag) continue;
+    int n = face->dof < dof ? face->dof : dof;
+    if (face->bGrp == 0) {
+      for (int a = 0; a < face->nNo; a++) {
+        int Ac = face->glob[a];
+        for (int i = 0; i < n; i++) W[i + dof * Ac] = W[i + dof * Ac] * face->val[i + face->dof * a];
+      }
+    }
+  }
+  /* pre_mul: Val(4i+c, j) *= W(i, row) (precond.cpp:549-624) */
+  for (int Ac = 0; Ac < nNo; Ac++)
+    for (int i = 0; i < dof; i++)
+      for (int j = RS(Ac); j <= RE(Ac); j++)
+        for (int c = 0; c < dof; c++) Val[i * dof + c + d2 * j] = Val[i * dof + c + d2 * j] * W[i + dof * Ac];
+  for (int i = 0; i < dof * nNo; i++) R[i] = W[i] * R[i];
+  /* pos_mul: Val(4r+i, j) *= W(i, col) (precond.cpp:46-111) */
+  for (int Ac = 0; Ac < nNo; Ac++)
+    for (int j = RS(Ac); j <= RE(Ac); j++) {
+      int a = lhs->colPtr[j];
+      for (int r = 0; r < dof; r++)
+        for (int i = 0; i < dof; i++) Val[r * dof + i + d2 * j] = Val[r * dof + i + d2 * j] * W[i + dof * a];
+    }
+}
+
+int orc_ge(int N, int ldA, const double* A, double* B)
+{
+  /* Returns 1 on success (the reference's true), 0 on a singular system. */
+  double tol = DBL_TRUE_MIN;
+  double eps = DBL_EPSILON;
+  double* W = (double*)malloc(sizeof(double) * (N > 0 ? N : 1));
+  for (int i = 0; i < N; i++) {
+    if (fabs(A[i + ldA * i]) < tol) {
+      for (int k = 0; k < ldA; k++) B[k] = 0.0;
+      free(W);
+      return 0;
+    }
+    W[i] = 1.0 / sqrt(fabs(A[i + ldA * i]));
+  }
+  int ldC = N;
+  double* C = (double*)malloc(sizeof(double) * (N > 0 ? N : 1) * (N + 1));
+#define CC(i, j) C[(i) + ldC * (j)]
+  for (int i = 0; i < N; i++) {
+    for (int j = 0; j < N; j++) CC(i, j) = W[i] * W[j] * A[i + ldA * j];
+    CC(i, N) = W[i] * B[i];
+  }
+  int ok = 1;
+  if (N <= 0) {
+    ok = 0;
+  } else if (N == 1) {
+    B[0] = CC(0, 1) / CC(0, 0);
+    B[0] = B[0] * W[0];
+  } else if (N == 2) {
+    double pivot = CC(0, 0) * CC(1, 1) - CC(1, 0) * CC(0, 1);
+    if (fabs(pivot) < eps) {
+      for (int k = 0; k < ldA; k++) B[k] = 0.0;
+      ok = 0;
+    } else {
+      B[0] = (CC(0, 2) * CC(1, 1) - CC(1, 2) * CC(0, 1)) / pivot;
+      B[1] = (CC(1, 2) * CC(0, 0) - CC(0, 2) * CC(1, 0)) / pivot;
+      B[0] = W[0] * B[0];
+      B[1] = W[1] * B[1];
+    }
+  } else {
+    for (int m = 0; m < N - 1 && ok; m++) {
+      int ipv = m;
+      double pivot = fabs(CC(m, m));
+      for (int i = m + 1; i < N; i++)
+        if (fabs(CC(i, m)) > pivot) { ipv = i; pivot = fabs(CC(i, m)); }
+      if (fabs(pivot) < eps) {
+        for (int k = 0; k < ldA; k++) B[k] = 0.0;
+        ok = 0;
+        break;
+      }
+      if (ipv != m)
+        for (int j = m; j < N + 1; j++) { double s = CC(m, j); CC(m, j) = CC(ipv, j); CC(ipv, j) = s; }
+      for (int i = m + 1; i < N; i++) {
+        double s = CC(i, m) / CC(m, m);
+        CC(i, m) = 0.0;
+        for (int j = m + 1; j < N + 1; j++) CC(i, j) = CC(i, j) - s * CC(m, j);
+      }
+    }
+    if (ok) {
+      for (int j = N - 1; j >= 0; j--) {
+        for (int i = j + 1; i < N; i++) CC(j, N) = CC(j, N) - CC(j, i) * CC(i, N);
+        CC(j, N) = CC(j, N) / CC(j, j);
+      }
+      for (int i = 0; i < N; i++) B[i] = W[i] * CC(i, N);
+    }
+  }
+#undef CC
+  free(C);
+  free(W);
+  return ok;
+}
+
+/* Arnoldi cycle shared by gmres and gmres_v: returns last_i, sets *suc. */
+static int arnoldi_cycle(const orc_lhs* lhs, orc_subls* ls, int dof, const double* Val, double* u,
+                         double* h, double* c, double* s, double* err, double eps)
+{
+  int nNo = lhs->nNo;
+  size_t n = (size_t)dof * nNo;
+  int sD = ls->sD;
+  int last_i = 0;
+#define H(i, j) h[(i) + (sD + 1) * (j)]
+  for (int i = 0; i < sD; i++) {
+    ls->itr = ls->itr + 1;
+    last_i = i;
+    orc_spar_mul_vv(lhs, dof, Val, u + n * i, u + n * (i + 1));
+    for (int j = 0; j <= i + 1; j++) H(j, i) = orc_dot_v(dof, nNo, u + n * j, u + n * (i + 1));
+    for (int j = 0; j <= i; j++) {
+      omp_sum((int)n, -H(j, i), u + n * (i + 1), u + n * j);
+      H(i + 1, i) = H(i + 1, i) - H(j, i) * H(j, i);
+    }
+    H(i + 1, i) = sqrt(fabs(H(i + 1, i)));
+    omp_mul((int)n, 1.0 / H(i + 1, i), u + n * (i + 1));
+    for (int j = 0; j <= i - 1; j++) {
+      double tmp = c[j] * H(j, i) + s[j] * H(j + 1, i);
+      H(j + 1, i) = -s[j] * H(j, i) + c[j] * H(j + 1, i);
+      H(j, i) = tmp;
+    }
+    double tmp = sqrt(H(i, i) * H(i, i) + H(i + 1, i) * H(i + 1, i));
+    c[i] = H(i, i) / tmp;
+    s[i] = H(i + 1, i) / tmp;
+    H(i, i) = tmp;
+    H(i + 1, i) = 0.0;
+    err[i + 1] = -s[i] * err[i];
+    err[i] = c[i] * err[i];
+    if (fabs(err[i + 1]) < eps) {
+      ls->suc = 1;
+      break;
+    }
+  }
+  return last_i;
+}
+
+static void back_solve_update(int dof, int nNo, int sD, int last_i, const double* h, const double* err,
+                              double* y, const double* u, double* X)
+{
+  size_t n = (size_t)dof * nNo;
+  for (int i = 0; i <= last_i; i++) y[i] = err[i];
+  for (int j = last_i; j >= 0; j--) {
+    for (int k = j + 1; k <= last_i; k++) y[j] = y[j] - H(j, k) * y[k];
+    y[j] = y[j] / H(j, j);
+  }
+  for (int j = 0; j <= last_i; j++) omp_sum((int)n, y[j], X, u + n * j);
+}
+
+void orc_gmres_v(const orc_lhs* lhs, orc_subls* ls, int dof, const double* Val, double* R)
+{
+  int nNo = lhs->nNo;
+  size_t n = (size_t)dof * nNo;
+  int sD = ls->sD;
+  double* h = (double*)calloc((size_t)(sD + 1) * sD, sizeof(double));
+  double* X = (double*)calloc(n, sizeof(double));
+  double* u = (double*)calloc(n * (sD + 1), sizeof(double));
+  double* y = (double*)calloc(sD, sizeof(double));
+  double* c = (double*)calloc(sD, sizeof(double));
+  double* s = (double*)calloc(sD, sizeof(double));
+  double* err = (double*)calloc(sD + 1, sizeof(double));
+
+  ls->suc = 0;
+  double eps = orc_norm_v(dof, nNo, R);
+  ls->iNorm = eps;
+  ls->fNorm = eps;
+  eps = fmax(ls->absTol, ls->relTol * eps);
+  ls->itr = 0;
+  int last_i = 0;
+  if (ls->iNorm <= ls->absTol) {
+    ls->callD = DBL_EPSILON;
+    ls->dB = 0.0;
+    goto done;
+  }
+  for (int l = 0; l < ls->mItr; l++) {
+    ls->dB = ls->fNorm;
+    ls->itr = ls->itr + 1;
+    orc_spar_mul_vv(lhs, dof, Val, X, u);
+    for (size_t k = 0; k < n; k++) u[k] = R[k] - u[k];
+    err[0] = orc_norm_v(dof, nNo, u);
+    for (size_t k = 0; k < n; k++) u[k] = u[k] / err[0];
+    last_i = arnoldi_cycle(lhs, ls, dof, Val, u, h, c, s, err, eps);
+    if (last_i >= sD) last_i = sD - 1;
+    back_solve_update(dof, nNo, sD, last_i, h, err, y, u, X);
+    ls->fNorm = fabs(err[last_i + 1]);
+    if (ls->suc) break;
+  }
+  memcpy(R, X, n * sizeof(double));
+  ls->dB = 10.0 * log(ls->fNorm / ls->dB);
+done:
+  free(h); free(X); free(u); free(y); free(c); free(s); free(err);
+}
+
+void orc_gmres(const orc_lhs* lhs, orc_subls* ls, int dof, const double* Val, const double* R, double* X)
+{
+  int nNo = lhs->nNo;
+  size_t n = (size_t)dof * nNo;
+  int sD = ls->sD;
+  double* h = (double*)calloc((size_t)(sD + 1) * sD, sizeof(double));
+  double* u = (double*)calloc(n * (sD + 1), sizeof(double));
+  double* y = (double*)calloc(sD, sizeof(double));
+  double* c = (double*)calloc(sD, sizeof(double));
+  double* s = (double*)calloc(sD, sizeof(double));
+  double* err = (double*)calloc(sD + 1, sizeof(double));
+  ls->suc = 0;
+  double eps = 0.0;
+  int last_i = 0;
+  for (size_t k = 0; k < n; k++) X[k] = 0.0;
+  for (int l = 0; l < ls->mItr; l++) {
+    if (l == 0) {
+      memcpy(u, R, n * sizeof(double));
+    } else {
+      orc_spar_mul_vv(lhs, dof, Val, X, u);
+      ls->itr = ls->itr + 1;
+      for (size_t k = 0; k < n; k++) u[k] = R[k] - u[k];
+    }
+    err[0] = orc_norm_v(dof, nNo, u);
+    if (l == 0) {
+      eps = err[0];
+      if (eps <= ls->absTol) {
+        ls->callD = DBL_EPSILON;
+        ls->dB = 0.0;
+        goto done;
+      }
+      ls->iNorm = eps;
+      ls->fNorm = eps;
+      eps = fmax(ls->absTol, ls->relTol * eps);
+    }
+    ls->dB = ls->fNorm;
+    for (size_t k = 0; k < n; k++) u[k] = u[k] / err[0];
+    last_i = arnoldi_cycle(lhs, ls, dof, Val, u, h, c, s, err, eps);
+    if (last_i >= sD) last_i = sD - 1;
+    back_solve_update(dof, nNo, sD, last_i, h, err, y, u, X);
+    ls->fNorm = fabs(err[last_i + 1]);
+    if (ls->suc) break;
+  }
+  ls->dB = 10.0 * log(ls->fNorm / ls->dB);
+done:
+  free(h); free(u); free(y); free(c); free(s); free(err);
+}
+#undef H
+
+void orc_schur(const orc_lhs* lhs, orc_subls* ls, int dof, const double* D, const double* G, const double* L,
+               double* R)
+{
+  int nNo = lhs->nNo;
+  double* X = (double*)calloc(nNo, sizeof(double));
+  double* P = (double*)calloc(nNo, sizeof(double));
+  double* SP = (double*)calloc(nNo, sizeof(double));
+  double* DGP = (double*)calloc(nNo, sizeof(double));
+  double* GP = (double*)calloc((size_t)dof * nNo, sizeof(double));
+  ls->suc = 0;
+  ls->iNorm = orc_norm_s(nNo, R);
+  double eps = pow(fmax(ls->absTol, ls->relTol * ls->iNorm), 2.0);
+  double errO = ls->iNorm * ls->iNorm;
+  double err = errO;
+  memcpy(P, R, nNo * sizeof(double));
+  int last_i = 0;
+  for (int i = 0; i < ls->mItr; i++) {
+    last_i = i;
+    if (err < eps) {
+      ls->suc = 1;
+      break;
+    }
+    errO = err;
+    orc_spar_mul_sv(lhs, dof, G, P, GP);
+    orc_spar_mul_vs(lhs, dof, D, GP, DGP);
+    orc_spar_mul_ss(lhs, L, P, SP);
+    omp_sum(nNo, -1.0, SP, DGP);
+    double alpha = errO / orc_dot_s(nNo, P, SP);
+    omp_sum(nNo, alpha, X, P);
+    omp_sum(nNo, -alpha, R, SP);
+    err = orc_norm_s(nNo, R);
+    err = err * err;
+    double c1 = errO / err;
+    omp_sum(nNo, c1, P, R);
+    double c2 = err / errO;
+    omp_mul(nNo, c2, P);
+  }
+  memcpy(R, X, nNo * sizeof(double));
+  ls->fNorm = sqrt(err);
+  ls->itr = ls->itr + last_i;
+  if (errO < DBL_EPSILON) ls->dB = 0.0; else ls->dB = 5.0 * log(err / errO);
+  free(X); free(P); free(SP); free(DGP); free(GP);
+}
+
+int orc_ns_solver(const orc_lhs* lhs, orc_ls* ls, int dof, const double* Val, double* Ri)
+{
+  const int nNo = lhs->nNo, nnz = lhs->nnz;
+  const int nsd = dof - 1;
+  const int iBmax = ls->RI.mItr;
+  const int nB = 2 * iBmax;
+  double* Rc = calloc(nNo, sizeof(double));
+  double* Rci = calloc(nNo, sizeof(double));
+  double* tmp = calloc((size_t)nB * nB + nB, sizeof(double));
+  double* B = calloc(nB, sizeof(double));
+  double* xB = calloc(nB, sizeof(double));
+  double* oldxB = calloc(nB, sizeof(double));
+  double* Rm = calloc((size_t)nsd * nNo, sizeof(double));
+  double* Rmi = calloc((size_t)nsd * nNo, sizeof(double));
+  double* A = calloc((size_t)nB * nB, sizeof(double));
+  double* P = calloc((size_t)nNo * iBmax, sizeof(double));
+  double* MP = calloc((size_t)nNo * nB, sizeof(double));
+  double* U = calloc((size_t)nsd * nNo * iBmax, sizeof(double));
+  double* MU = calloc((size_t)nsd * nNo * nB, sizeof(double));
+  double* Gt = calloc((size_t)nsd * nnz, sizeof(double));
+  double* mK = calloc((size_t)nsd * nsd * nnz, sizeof(double));
+  double* mG = calloc((size_t)nsd * nnz, sizeof(double));
+  double* mD = calloc((size_t)nsd * nnz, sizeof(double));
+  double* mL = calloc(nnz, sizeof(double));
+  double* Ut = calloc((size_t)nsd * nNo, sizeof(double));
+  int rc = 0;
+  size_t nv = (size_t)nsd * nNo;
+
+  for (int i = 0; i < nsd; i++)
+    for (int j = 0; j < nNo; j++) Rmi[i + nsd * j] = Ri[i + dof * j];
+  for (int j = 0; j < nNo; j++) Rci[j] = Ri[dof - 1 + dof * j];
+  memcpy(Rm, Rmi, nv * sizeof(double));
+  memcpy(Rc, Rci, nNo * sizeof(double));
+
+  double eps = sqrt(pow(orc_norm_v(nsd, nNo, Rm), 2.0) + pow(orc_norm_s(nNo, Rc), 2.0));
+  ls->RI.iNorm = eps;
+  ls->RI.fNorm = eps * eps;
+  ls->CG.callD = 0.0;
+  ls->GM.callD = 0.0;
+  ls->CG.itr = 0;
+  ls->GM.itr = 0;
+  ls->RI.suc = 0;
+  eps = fmax(ls->RI.absTol, ls->RI.relTol * eps);
+
+  /* depart (ns_solver.cpp:91-159), nsd == 3 */
+  for (int i = 0; i < nnz; i++) {
+    const double* t = Val + 16 * (size_t)i;
+    mK[0 + 9 * i] = t[0]; mK[1 + 9 * i] = t[1]; mK[2 + 9 * i] = t[2];
+    mK[3 + 9 * i] = t[4]; mK[4 + 9 * i] = t[5]; mK[5 + 9 * i] = t[6];
+    mK[6 + 9 * i] = t[8]; mK[7 + 9 * i] = t[9]; mK[8 + 9 * i] = t[10];
+    mG[0 + 3 * i] = t[3]; mG[1 + 3 * i] = t[7]; mG[2 + 3 * i] = t[11];
+    mD[0 + 3 * i] = t[12]; mD[1 + 3 * i] = t[13]; mD[2 + 3 * i] = t[14];
+    mL[i] = t[15];
+  }
+  for (int i = 0; i < nNo; i++)
+    for (int j = RS(i); j <= RE(i); j++) {
+      int k = lhs->colPtr[j];
+      for (int l = RS(k); l <= RE(k); l++)
+        if (lhs->colPtr[l] == i) {
+          for (int m = 0; m < nsd; m++) Gt[m + nsd * l] = -mG[m + nsd * j];
+          break;
+        }
+    }
+
+  int iBB = 0, i_count = 0;
+  for (int i = 0; i < ls->RI.mItr; i++) {
+    int iB = 2 * i;
+    iBB = 2 * i + 1;
+    ls->RI.dB = ls->RI.fNorm;
+    i_count = i;
+    double* Ui = U + nv * i;
+    double* Pi = P + (size_t)nNo * i;
+    orc_gmres(lhs, &ls->GM, nsd, mK, Rm, Ui);
+    orc_spar_mul_vs(lhs, nsd, mD, Ui, Pi);
+    for (int k = 0; k < nNo; k++) Pi[k] = Rc[k] - Pi[k];
+    orc_schur(lhs, &ls->CG, nsd, Gt, mG, mL, Pi);
+    double* MUiB = MU + nv * iB;
+    double* MUiBB = MU + nv * iBB;
+    orc_spar_mul_sv(lhs, nsd, mG, Pi, MUiB);
+    for (size_t k = 0; k < nv; k++) MUiBB[k] = Rm[k] - MUiB[k];
+    /* gmres writes X in place; its right-hand side is MU(:,:,iBB). */
+    memcpy(Ut, MUiBB, nv * sizeof(double));
+    orc_gmres(lhs, &ls->GM, nsd, mK, Ut, Ui);
+    orc_spar_mul_vv(lhs, nsd, mK, Ui, MUiBB);
+    orc_spar_mul_ss(lhs, mL, Pi, MP + (size_t)nNo * iB);
+    orc_spar_mul_vs(lhs, nsd, mD, Ui, MP + (size_t)nNo * iBB);
+
+    int c = 0;
+    for (int k = iB; k <= iBB; k++) {
+      for (int j = 0; j <= k; j++) {
+        tmp[c] = orc_dot_v(nsd, nNo, MU + nv * j, MU + nv * k) +
+                 orc_dot_s(nNo, MP + (size_t)nNo * j, MP + (size_t)nNo * k);
+        c = c + 1;
+      }
+      tmp[c] = orc_dot_v(nsd, nNo, MU + nv * k, Rmi) + orc_dot_s(nNo, MP + (size_t)nNo * k, Rci);
+      c = c + 1;
+    }
+    c = 0;
+    for (int k = iB; k <= iBB; k++) {
+      for (int j = 0; j <= k; j++) {
+        A[j + nB * k] = tmp[c];
+        A[k + nB * j] = tmp[c];
+        c = c + 1;
+      }
+      B[k] = tmp[c];
+      c = c + 1;
+    }
+    memcpy(xB, B, nB * sizeof(double));
+    if (orc_ge(iBB + 1, nB, A, xB)) {
+      memcpy(oldxB, xB, nB * sizeof(double));
+    } else {
+      snprintf(g_err2, sizeof g_err2, "FSILS: Singular matrix detected");
+      rc = 4;
+      goto done;
+    }
+    double sum = 0.0;
+    for (int q = 0; q <= iBB; q++) sum += xB[q] * B[q];
+    ls->RI.fNorm = pow(ls->RI.iNorm, 2.0) - sum;
+    if (ls->RI.fNorm < eps * eps) {
+      ls->RI.suc = 1;
+      break;
+    }
+    for (size_t k = 0; k < nv; k++) Rm[k] = Rmi[k] - xB[0] * MU[k];
+    for (int k = 0; k < nNo; k++) Rc[k] = Rci[k] - xB[0] * MP[k];
+    for (int j = 1; j <= iBB; j++) {
+      for (size_t k = 0; k < nv; k++) Rm[k] = Rm[k] - xB[j] * MU[nv * j + k];
+      for (int k = 0; k < nNo; k++) Rc[k] = Rc[k] - xB[j] * MP[(size_t)nNo * j + k];
+    }
+  }
+  if (i_count >= ls->RI.mItr) {
+    ls->RI.itr = ls->RI.mItr;
+  } else {
+    ls->RI.itr = i_count;
+    for (int k = 0; k < nNo; k++) Rc[k] = Rci[k] - xB[0] * MP[k];
+    for (int j = 1; j <= iBB; j++)
+      for (int k = 0; k < nNo; k++) Rc[k] = Rc[k] - xB[j] * MP[(size_t)nNo * j + k];
+  }
+  ls->Resc = (int)(100.0 * pow(orc_norm_s(nNo, Rc), 2.0) / ls->RI.fNorm);
+  ls->Resm = 100 - ls->Resc;
+
+  for (size_t k = 0; k < nv; k++) Rmi[k] = xB[1] * U[k];
+  for (int k = 0; k < nNo; k++) Rci[k] = xB[0] * P[k];
+  for (int i = 1; i <= ls->RI.itr; i++) {
+    int iB = 2 * i, iBB2 = 2 * i + 1;
+    for (size_t k = 0; k < nv; k++) Rmi[k] = Rmi[k] + xB[iBB2] * U[nv * i + k];
+    for (int k = 0; k < nNo; k++) Rci[k] = Rci[k] + xB[iB] * P[(size_t)nNo * i + k];
+  }
+  ls->RI.dB = 5.0 * log(ls->RI.fNorm / ls->RI.dB);
+  if (ls->Resc < 0.0 || ls->Resm < 0.0) {
+    ls->Resc = 0;
+    ls->Resm = 0;
+    ls->RI.dB = 0;
+    ls->RI.fNorm = 0.0;
+    snprintf(g_err2, sizeof g_err2,
+             "FSILS: unexpected behavior in FSILS (likely due to the ill-conditioned LHS matrix)");
+    rc = 5;
+    goto done;
+  }
+  ls->RI.fNorm = sqrt(ls->RI.fNorm);
+  for (int i = 0; i < nsd; i++)
+    for (int j = 0; j < nNo; j++) Ri[i + dof * j] = Rmi[i + nsd * j];
+  for (int j = 0; j < nNo; j++) Ri[dof - 1 + dof * j] = Rci[j];
+done:
+  free(Rc); free(Rci); free(tmp); free(B); free(xB); free(oldxB); free(Rm); free(Rmi); free(A);
+  free(P); free(MP); free(U); free(MU); free(Gt); free(mK); free(mG); free(mD); free(mL); free(Ut);
+  return rc;
+}
+
+void orc_ls_create(orc_ls* ls, int LS_type, double relTol, double absTol, int maxItr, int dimKry,
+                   const double relTolIn[2], const double absTolIn[2], const int maxItrIn[2])
+{
+  memset(ls, 0, sizeof *ls);
+  ls->LS_type = LS_type;
+  switch (LS_type) {
+    case ORC_LS_NS:
+      ls->RI.relTol = 0.4; ls->GM.relTol = 1.E-2; ls->CG.relTol = 0.2;
+      ls->RI.mItr = 10; ls->GM.mItr = 2; ls->CG.mItr = 500;
+      ls->GM.sD = 100; ls->RI.sD = 100;
+      break;
+    case ORC_LS_GMRES:
+      ls->RI.relTol = 0.1; ls->RI.mItr = 4; ls->RI.sD = 250;
+      break;
+    case ORC_LS_CG:
+      ls->RI.relTol = 1.E-2; ls->RI.mItr = 1000;
+      break;
+    default:
+      break;
+  }
+  ls->RI.absTol = 1.E-10;
+  ls->GM.absTol = 1.E-10;
+  ls->CG.absTol = 1.E-10;
+  /* Optional arguments are "present" when positive here. */
+  if (relTol > 0) ls->RI.relTol = relTol;
+  if (absTol > 0) ls->RI.absTol = absTol;
+  if (maxItr > 0) ls->RI.mItr = maxItr;
+  if (dimKry > 0) { ls->RI.sD = dimKry; ls->GM.sD = dimKry; }
+  if (relTolIn) { ls->GM.relTol = relTolIn[0]; ls->CG.relTol = relTolIn[1]; }
+  if (absTolIn) { ls->GM.absTol = absTolIn[0]; ls->CG.absTol = absTolIn[1]; }
+  if (maxItrIn) { ls->GM.mItr = maxItrIn[0]; ls->CG.mItr = maxItrIn[1]; }
+}
+
+int orc_fsils_solve(orc_lhs* lhs, orc_ls* ls, int dof, double* Ri, double* Val, int prec, int nIncL,
+                    const int* incL, const double* res)
+{
+  int nNo = lhs->nNo;
+  if (lhs->nFaces != 0) {
+    for (int f = 0; f < lhs->nFaces; f++) lhs->face[f].incFlag = 1;
+    if (nIncL != 0)
+      for (int f = 0; f < lhs->nFaces; f++)
+        if (incL[f] == 0) lhs->face[f].incFlag = 0;
+    int flag = 0;
+    for (int f = 0; f < lhs->nFaces; f++)
+      if (lhs->face[f].bGrp == 1) { flag = 1; break; }
+    if (res == NULL && flag) {
+      snprintf(g_err2, sizeof g_err2, "[fsils_solve] res is required for Neu surfaces");
+      return 7;
+    }
+    for (int f = 0; f < lhs->nFaces; f++) {
+      orc_face* face = &lhs->face[f];
+      face->coupledFlag = 0;
+      if (!face->incFlag) continue;
+      if (face->bGrp == 1 && res[f] != 0.0) {
+        snprintf(g_err2, sizeof g_err2, "coupled Neumann faces are not supported by the oracle");
+        return 8;
+      }
+    }
+  }
+  size_t n = (size_t)dof * nNo;
+  double* R = malloc(n * sizeof(double));
+  double* Wc = malloc(n * sizeof(double));
+  memcpy(R, Ri, n * sizeof(double));
+  if (prec == 701) {
+    orc_precond_diag(lhs, dof, Val, R, Wc);
+  } else {
+    for (size_t k = 0; k < n; k++) Wc[k] = 1.0;
+  }
+  int rc = 0;
+  if (ls->LS_type == ORC_LS_NS) {
+    rc = orc_ns_solver(lhs, ls, dof, Val, R);
+  } else if (ls->LS_type == ORC_LS_GMRES) {
+    orc_gmres_v(lhs, &ls->RI, dof, Val, R);
+  } else {
+    snprintf(g_err2, sizeof g_err2, "FSILS: LS_type not defined");
+    rc = 6;
+  }
+  if (rc == 0) {
+    for (size_t k = 0; k < n; k++) R[k] = Wc[k] * R[k];
+    memcpy(Ri, R, n * sizeof(double));
+  }
+  free(R);
+  free(Wc);
+  return rc;
+}
+
+const char* orc_fsils_last_error(void) { return g_err2; }

@@ -1,0 +1,179 @@
+// Reference harness (test infrastructure only, never shipped with the
+// product): drives the reference's own construct_fluid and fsils_solve,
+// compiled from the sources under /root/reference by oracle/ref.mk, through
+// a plain C interface so that tests can compare the oracle restatement (and,
+// via committed golden fixtures, the HIP path) with the real reference.
+//
+// Objects are built the way the reference's own driver builds them for a
+// single-rank run:
+//   mesh shape functions   nn::select_ele + fs::init_fs_msh (read_msh.cpp)
+//   equation coefficients  initialize.cpp:424-463 (values passed in)
+//   FSILS lhs / faces / ls fsils_lhs_create, fsils_bc_create, fsils_ls_create
+//                          then the <LS> overrides of read_files.cpp:2056-2090
+#include "ComMod.h"
+#include "FsilsLinearAlgebra.h"
+#include "fluid.h"
+#include "fs.h"
+#include "fsils_api.hpp"
+#include "lhs.h"
+#include "nn.h"
+
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+static std::string g_ref_err;
+
+extern "C" const char* ref_last_error(void) { return g_ref_err.c_str(); }
+
+// prm: rho, mu, fx, fy, fz, K_darcy, af, am, gam, dt
+extern "C" int ref_construct_fluid(int nEl, const int* IEN, int nNo, const double* x, const double* Bf,
+                                   const double* Ag, const double* Yg, const double* prm, int nnz,
+                                   const int* rowPtr, const int* colPtr, double* R, double* Val)
+{
+  using namespace consts;
+  try {
+    ComMod com_mod;
+    com_mod.nsd = 3;
+    com_mod.nsymd = 6;
+    com_mod.dof = 4;
+    com_mod.tDof = 4;
+    com_mod.tnNo = nNo;
+    com_mod.cEq = 0;
+    com_mod.mvMsh = false;
+    com_mod.dt = prm[9];
+    com_mod.x.resize(3, nNo);
+    com_mod.Bf.resize(3, nNo);
+    for (int a = 0; a < nNo; a++)
+      for (int i = 0; i < 3; i++) {
+        com_mod.x(i, a) = x[i + 3 * a];
+        com_mod.Bf(i, a) = Bf ? Bf[i + 3 * a] : 0.0;
+      }
+    com_mod.rowPtr.resize(nNo + 1);
+    com_mod.colPtr.resize(nnz);
+    for (int i = 0; i <= nNo; i++) com_mod.rowPtr(i) = rowPtr[i];
+    for (int i = 0; i < nnz; i++) com_mod.colPtr(i) = colPtr[i];
+    com_mod.R.resize(4, nNo);
+    com_mod.Val.resize(16, nnz);
+    std::memcpy(com_mod.R.data(), R, sizeof(double) * 4 * nNo);
+    std::memcpy(com_mod.Val.data(), Val, sizeof(double) * 16 * (size_t)nnz);
+
+    com_mod.eq.resize(1);
+    auto& eq = com_mod.eq[0];
+    eq.af = prm[6];
+    eq.am = prm[7];
+    eq.gam = prm[8];
+    eq.dof = 4;
+    eq.nDmn = 1;
+    eq.dmn.resize(1);
+    auto& dmn = eq.dmn[0];
+    dmn.Id = -1;
+    dmn.phys = EquationType::phys_fluid;
+    dmn.prop[PhysicalProperyType::fluid_density] = prm[0];
+    dmn.prop[PhysicalProperyType::f_x] = prm[2];
+    dmn.prop[PhysicalProperyType::f_y] = prm[3];
+    dmn.prop[PhysicalProperyType::f_z] = prm[4];
+    dmn.prop[PhysicalProperyType::inverse_darcy_permeability] = prm[5];
+    dmn.fluid_visc.viscType = FluidViscosityModelType::viscType_Const;
+    dmn.fluid_visc.mu_i = prm[1];
+    FsilsLinearAlgebra la;
+    eq.linear_algebra = &la;
+
+    mshType msh;
+    msh.nEl = nEl;
+    msh.gnEl = nEl;
+    msh.eNoN = 4;
+    msh.nFs = 1;
+    msh.IEN.resize(4, nEl);
+    for (int e = 0; e < nEl; e++)
+      for (int a = 0; a < 4; a++) msh.IEN(a, e) = IEN[a + 4 * e];
+    nn::select_ele(com_mod, msh);
+    fs::init_fs_msh(com_mod, msh);
+
+    fluid::construct_fluid(com_mod, msh, Array<double>(4, nNo, const_cast<double*>(Ag)),
+                           Array<double>(4, nNo, const_cast<double*>(Yg)));
+    std::memcpy(R, com_mod.R.data(), sizeof(double) * 4 * nNo);
+    std::memcpy(Val, com_mod.Val.data(), sizeof(double) * 16 * (size_t)nnz);
+    eq.linear_algebra = nullptr;
+  } catch (const std::exception& ex) {
+    g_ref_err = ex.what();
+    return 1;
+  }
+  return 0;
+}
+
+// ls_prm: RI.relTol, RI.absTol, RI.mItr, RI.sD, GM.relTol, GM.absTol, GM.mItr, GM.sD,
+//         CG.relTol, CG.absTol, CG.mItr
+// out:    RI.itr, RI.iNorm, RI.fNorm, RI.suc, RI.dB, GM.itr, CG.itr, Resm, Resc
+extern "C" int ref_fsils_solve(int nNo, int nnz, const int* rowPtr, const int* colPtr, int nFaces,
+                               const int* faceNNo, const int* faceGlob, const double* faceVal, int LS_type,
+                               const double* ls_prm, int dof, double* Ri, double* Val, int prec, double* out)
+{
+  using namespace fsi_linear_solver;
+  try {
+    FSILS_commuType commu;
+    commu.foC = true;
+    commu.masF = true;
+    commu.master = 0;
+    commu.task = 0;
+    commu.tF = 1;
+    commu.nTasks = 1;
+    FSILS_lhsType lhs;
+    Vector<int> gNodes(nNo), vRow(nNo + 1), vCol(nnz);
+    for (int a = 0; a < nNo; a++) gNodes(a) = a;
+    for (int a = 0; a <= nNo; a++) vRow(a) = rowPtr[a];
+    for (int i = 0; i < nnz; i++) vCol(i) = colPtr[i];
+    fsils_lhs_create(lhs, commu, nNo, nNo, nnz, gNodes, vRow, vCol, nFaces);
+    int off = 0;
+    for (int f = 0; f < nFaces; f++) {
+      int n = faceNNo[f];
+      Vector<int> g(n);
+      Array<double> v(3, n);
+      for (int a = 0; a < n; a++) {
+        g(a) = faceGlob[off + a];
+        for (int i = 0; i < 3; i++) v(i, a) = faceVal[3 * (off + a) + i];
+      }
+      fsils_bc_create(lhs, f, n, 3, BcType::BC_TYPE_Dir, g, v);
+      off += n;
+    }
+    FSILS_lsType ls;
+    fsils_ls_create(ls, static_cast<LinearSolverType>(LS_type));
+    ls.RI.relTol = ls_prm[0];
+    ls.RI.absTol = ls_prm[1];
+    ls.RI.mItr = (int)ls_prm[2];
+    ls.RI.sD = (int)ls_prm[3];
+    ls.GM.relTol = ls_prm[4];
+    ls.GM.absTol = ls_prm[5];
+    ls.GM.mItr = (int)ls_prm[6];
+    ls.GM.sD = (int)ls_prm[7];
+    ls.CG.relTol = ls_prm[8];
+    ls.CG.absTol = ls_prm[9];
+    ls.CG.mItr = (int)ls_prm[10];
+
+    Array<double> aRi(dof, nNo), aVal(dof * dof, nnz);
+    std::memcpy(aRi.data(), Ri, sizeof(double) * dof * nNo);
+    std::memcpy(aVal.data(), Val, sizeof(double) * dof * dof * (size_t)nnz);
+    Vector<int> incL(nFaces);
+    Vector<double> res(nFaces);
+    for (int f = 0; f < nFaces; f++) incL(f) = 1;
+    fsils_solve(lhs, ls, dof, aRi, aVal,
+                prec == 701 ? consts::PreconditionerType::PREC_FSILS : consts::PreconditionerType::PREC_NONE,
+                incL, res);
+    std::memcpy(Ri, aRi.data(), sizeof(double) * dof * nNo);
+    std::memcpy(Val, aVal.data(), sizeof(double) * dof * dof * (size_t)nnz);
+    out[0] = ls.RI.itr;
+    out[1] = ls.RI.iNorm;
+    out[2] = ls.RI.fNorm;
+    out[3] = ls.RI.suc ? 1.0 : 0.0;
+    out[4] = ls.RI.dB;
+    out[5] = ls.GM.itr;
+    out[6] = ls.CG.itr;
+    out[7] = ls.Resm;
+    out[8] = ls.Resc;
+  } catch (const std::exception& ex) {
+    g_ref_err = ex.what();
+    return 1;
+  }
+  return 0;
+}

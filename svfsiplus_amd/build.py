@@ -1,0 +1,99 @@
+"""Builds the native library in-tree: svfsiplus_amd/libsvgpu.so (gfx950).
+
+    python -m svfsiplus_amd.build            # library only
+    python -m svfsiplus_amd.build --oracle   # also the CPU oracle (test infrastructure)
+
+hipcc cross-compiles for gfx950 without a GPU present. Every HIP source is
+built with packed-FP32 instruction selection disabled.
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+CSRC = PKG / "csrc"
+LIB = PKG / "libsvgpu.so"
+
+HIP_SOURCES = ["svg_runtime.hip", "svg_fsils.hip", "svg_assembly.hip"]
+CPP_SOURCES = ["host_setup.cpp"]
+
+ARCH = os.environ.get("SVG_OFFLOAD_ARCH", "gfx950")
+
+
+def _hipcc() -> str:
+    for cand in (shutil.which("hipcc"), "/opt/rocm/bin/hipcc"):
+        if cand and Path(cand).exists():
+            return cand
+    raise RuntimeError("hipcc not found; the HIP library cannot be built")
+
+
+def _run(cmd: list[str]) -> None:
+    r = subprocess.run(cmd, cwd=str(ROOT), capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("build step failed: " + " ".join(cmd[:3]) + " ...")
+
+
+def build_library(verbose: bool = False) -> Path:
+    hipcc = _hipcc()
+    objdir = PKG / "_build"
+    objdir.mkdir(exist_ok=True)
+    common = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
+              f"-I{ROOT / 'include'}", f"-I{CSRC}"]
+    hip_flags = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+    objs = []
+    for src in HIP_SOURCES:
+        obj = objdir / (src + ".o")
+        cmd = [hipcc, *common, *hip_flags, "-x", "hip", "-c", str(CSRC / src), "-o", str(obj)]
+        if verbose:
+            print(" ".join(cmd))
+        _run(cmd)
+        objs.append(str(obj))
+    for src in CPP_SOURCES:
+        obj = objdir / (src + ".o")
+        cmd = ["g++", *common, "-c", str(CSRC / src), "-o", str(obj)]
+        if verbose:
+            print(" ".join(cmd))
+        _run(cmd)
+        objs.append(str(obj))
+    tmp = LIB.with_suffix(".so.tmp")
+    cmd = [hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *objs]
+    _run(cmd)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_oracle() -> Path:
+    oracle = ROOT / "oracle"
+    _run(["make", "-s", "-C", str(oracle)])
+    return oracle / "liboracle.so"
+
+
+def build_reference_harness() -> Path | None:
+    """Compiles oracle/_ref/libsvref.so from the reference sources (only
+    where /root/reference exists; the GPU box uses the prebuilt file)."""
+    if not Path("/root/reference/Code/Source/solver/fluid.cpp").exists():
+        return None
+    _run(["make", "-s", "-j8", "-f", str(ROOT / "oracle" / "ref.mk")])
+    return ROOT / "oracle" / "_ref" / "libsvref.so"
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--oracle", action="store_true")
+    ap.add_argument("-v", "--verbose", action="store_true")
+    a = ap.parse_args()
+    print(build_library(a.verbose))
+    if a.oracle:
+        print(build_oracle())
+        print(build_reference_harness())
+
+
+if __name__ == "__main__":
+    main()

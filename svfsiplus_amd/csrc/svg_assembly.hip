@@ -1,0 +1,374 @@
+// TET4 P1-P1 VMS fluid assembly on the device: construct_fluid
+// (Code/Source/solver/fluid.cpp:464-708) with fluid_3d_m (:1697-2139),
+// fluid_3d_c (:1389-1689) and do_assem (lhsa.cpp:97-141) fused into one
+// kernel, one element per lane, launched once per element colour so that the
+// read-modify-write of Val and R blocks never races.
+//
+// Specialisations, each exact for this element and material (the values
+// dropped are signed zeros, which leave every sum unchanged):
+//   * TET4 shape-function gradients are constant, so Nx, Jac, ksix, ux, px,
+//     divU and es are computed once per element (the reference recomputes the
+//     same arithmetic for every Gauss point; lShpF, fluid.cpp:609-626);
+//   * second derivatives Nwxx vanish for TET4 (gn_nxx solves K X = 0), so
+//     uxx, d2u2, es_x and mu_x are zero;
+//   * constant viscosity gives mu_g = 0 (fluid.cpp:2158-2162), so the esNx
+//     terms of the tangent and the viscosity-gradient terms of updu vanish and
+//     updu(i,j,b) = delta_ij * (-rho uNx_b - mu K N_b);
+//   * all four TET4 Gauss weights are 1/24, so w and wl are per element.
+// Per-entry summation over Gauss points keeps the reference order g = 0..3;
+// element contributions are summed into Val in colour order instead of
+// element order (the only intended difference: rounding of the sums).
+#include "svg_device.h"
+
+#include <cfloat>
+#include <climits>
+#include <cmath>
+
+namespace svg {
+
+struct FluidConst {
+  double rho, mu, f[3], K;
+  double T1;    // af * gam * dt
+  double amd;   // am / T1
+  double kT;    // 4 (ctM/dt)^2 + (K mu / rho)^2
+  double mr2;   // (mu/rho)^2
+  double wg;    // Gauss weight (1/24)
+  double N[4][4];  // N[g][a]
+};
+
+__device__ __forceinline__ bool is_zero_dev(double v)
+{
+  double a = fabs(v);
+  double nrm = fmax(a, DBL_EPSILON);
+  return (a / nrm) < 10.0 * DBL_EPSILON;
+}
+
+__global__ __launch_bounds__(128) void k_fluid_tet4(int q0, int qn, const int4* __restrict__ ien,
+                                                     const int* __restrict__ slot, int nElPad,
+                                                     const double* __restrict__ X, const double* __restrict__ Ag,
+                                                     const double* __restrict__ Yg, const double* __restrict__ Bf,
+                                                     FluidConst C, double* __restrict__ R, double* __restrict__ Val,
+                                                     int* __restrict__ err)
+{
+  int q = q0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= q0 + qn) return;
+  const int4 nd4 = ien[q];
+  const int nd[4] = {nd4.x, nd4.y, nd4.z, nd4.w};
+
+  double xl[3][4], al[3][4], yl[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; a++) {
+    const double* xp = X + 3 * (size_t)nd[a];
+    xl[0][a] = xp[0];
+    xl[1][a] = xp[1];
+    xl[2][a] = xp[2];
+    const double4 ya = *reinterpret_cast<const double4*>(Yg + 4 * (size_t)nd[a]);
+    yl[0][a] = ya.x;
+    yl[1][a] = ya.y;
+    yl[2][a] = ya.z;
+    yl[3][a] = ya.w;
+    const double4 aa = *reinterpret_cast<const double4*>(Ag + 4 * (size_t)nd[a]);
+    al[0][a] = aa.x;
+    al[1][a] = aa.y;
+    al[2][a] = aa.z;
+    if (Bf) {
+      const double* bp = Bf + 3 * (size_t)nd[a];
+      al[0][a] = al[0][a] - bp[0];
+      al[1][a] = al[1][a] - bp[1];
+      al[2][a] = al[2][a] - bp[2];
+    } else {
+      al[0][a] = al[0][a] - 0.0;
+      al[1][a] = al[1][a] - 0.0;
+      al[2][a] = al[2][a] - 0.0;
+    }
+  }
+
+  // ---- gnn, nn.cpp:503-540 (TET4 parent gradients are 0/1/-1: xXi = x_j - x_3)
+  double xXi[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) xXi[i][j] = xl[i][j] - xl[i][3];
+  const double Jac = xXi[0][0] * xXi[1][1] * xXi[2][2] + xXi[0][1] * xXi[1][2] * xXi[2][0] +
+                     xXi[0][2] * xXi[1][0] * xXi[2][1] - xXi[0][0] * xXi[1][2] * xXi[2][1] -
+                     xXi[0][1] * xXi[1][0] * xXi[2][2] - xXi[0][2] * xXi[1][1] * xXi[2][0];
+  if (is_zero_dev(Jac)) {
+    atomicMin(err, q);
+    return;
+  }
+  double xiX[3][3];
+  xiX[0][0] = (xXi[1][1] * xXi[2][2] - xXi[1][2] * xXi[2][1]) / Jac;
+  xiX[0][1] = (xXi[2][1] * xXi[0][2] - xXi[2][2] * xXi[0][1]) / Jac;
+  xiX[0][2] = (xXi[0][1] * xXi[1][2] - xXi[0][2] * xXi[1][1]) / Jac;
+  xiX[1][0] = (xXi[1][2] * xXi[2][0] - xXi[1][0] * xXi[2][2]) / Jac;
+  xiX[1][1] = (xXi[2][2] * xXi[0][0] - xXi[2][0] * xXi[0][2]) / Jac;
+  xiX[1][2] = (xXi[0][2] * xXi[1][0] - xXi[0][0] * xXi[1][2]) / Jac;
+  xiX[2][0] = (xXi[1][0] * xXi[2][1] - xXi[1][1] * xXi[2][0]) / Jac;
+  xiX[2][1] = (xXi[2][0] * xXi[0][1] - xXi[2][1] * xXi[0][0]) / Jac;
+  xiX[2][2] = (xXi[0][0] * xXi[1][1] - xXi[0][1] * xXi[1][0]) / Jac;
+  double Kxi[3][3];
+  Kxi[0][0] = xiX[0][0] * xiX[0][0] + xiX[1][0] * xiX[1][0] + xiX[2][0] * xiX[2][0];
+  Kxi[0][1] = xiX[0][1] * xiX[0][0] + xiX[1][1] * xiX[1][0] + xiX[2][1] * xiX[2][0];
+  Kxi[0][2] = xiX[0][2] * xiX[0][0] + xiX[1][2] * xiX[1][0] + xiX[2][2] * xiX[2][0];
+  Kxi[1][1] = xiX[0][1] * xiX[0][1] + xiX[1][1] * xiX[1][1] + xiX[2][1] * xiX[2][1];
+  Kxi[1][2] = xiX[0][1] * xiX[0][2] + xiX[1][1] * xiX[1][2] + xiX[2][1] * xiX[2][2];
+  Kxi[2][2] = xiX[0][2] * xiX[0][2] + xiX[1][2] * xiX[1][2] + xiX[2][2] * xiX[2][2];
+  Kxi[1][0] = Kxi[0][1];
+  Kxi[2][0] = Kxi[0][2];
+  Kxi[2][1] = Kxi[1][2];
+  double Nx[3][4];
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    Nx[i][0] = xiX[0][i];
+    Nx[i][1] = xiX[1][i];
+    Nx[i][2] = xiX[2][i];
+    Nx[i][3] = -xiX[0][i] - xiX[1][i] - xiX[2][i];
+  }
+
+  const double rho = C.rho, mu = C.mu, K = C.K;
+  const double w = C.wg * Jac;
+  const double wl = w * C.T1;
+  const double wr = w * rho;
+  const double muK = mu * K;
+
+  // Element-constant kinematics (P1 gradients).
+  double ux[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+      ux[i][j] = Nx[i][0] * yl[j][0] + Nx[i][1] * yl[j][1] + Nx[i][2] * yl[j][2] + Nx[i][3] * yl[j][3];
+  const double divU = ux[0][0] + ux[1][1] + ux[2][2];
+  double px[3];
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+    px[i] = Nx[i][0] * yl[3][0] + Nx[i][1] * yl[3][1] + Nx[i][2] * yl[3][2] + Nx[i][3] * yl[3][3];
+  double es[3][3];
+  es[0][0] = ux[0][0] + ux[0][0];
+  es[1][1] = ux[1][1] + ux[1][1];
+  es[2][2] = ux[2][2] + ux[2][2];
+  es[1][0] = ux[1][0] + ux[0][1];
+  es[2][1] = ux[2][1] + ux[1][2];
+  es[0][2] = ux[0][2] + ux[2][0];
+  es[0][1] = es[1][0];
+  es[1][2] = es[2][1];
+  es[2][0] = es[0][2];
+  const double kSsum = Kxi[0][0] * Kxi[0][0] + Kxi[1][0] * Kxi[1][0] + Kxi[2][0] * Kxi[2][0] +
+                       Kxi[0][1] * Kxi[0][1] + Kxi[1][1] * Kxi[1][1] + Kxi[2][1] * Kxi[2][1] +
+                       Kxi[0][2] * Kxi[0][2] + Kxi[1][2] * Kxi[1][2] + Kxi[2][2] * Kxi[2][2];
+  const double kS = 36.0 * kSsum * C.mr2;
+  const double trK = Kxi[0][0] + Kxi[1][1] + Kxi[2][2];
+
+  double lR[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int a = 0; a < 4; a++) lR[i][a] = 0.0;
+
+  // Per-Gauss-point scalars kept for the tangent loop.
+  double gTauM[4], gTauC[4], gTauB[4], gUNx[4][4], gUpNx[4][4];
+
+#pragma unroll
+  for (int g = 0; g < 4; g++) {
+    const double* N = C.N[g];
+    double ud[3], u[3], p;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      ud[i] = -C.f[i];
+      u[i] = 0.0;
+    }
+    p = 0.0;
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        ud[i] = ud[i] + N[a] * al[i][a];
+        u[i] = u[i] + N[a] * yl[i][a];
+      }
+      p = p + N[a] * yl[3][a];
+    }
+    const double kU = u[0] * u[0] * Kxi[0][0] + u[1] * u[0] * Kxi[1][0] + u[2] * u[0] * Kxi[2][0] +
+                      u[0] * u[1] * Kxi[0][1] + u[1] * u[1] * Kxi[1][1] + u[2] * u[1] * Kxi[2][1] +
+                      u[0] * u[2] * Kxi[0][2] + u[1] * u[2] * Kxi[1][2] + u[2] * u[2] * Kxi[2][2];
+    const double tauM = 1.0 / (rho * sqrt(C.kT + kU + kS));
+    double up[3];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      double rV = ud[j] + u[0] * ux[0][j] + u[1] * ux[1][j] + u[2] * ux[2][j];
+      up[j] = -tauM * (rho * rV + px[j] + muK * u[j]);
+    }
+    const double tauC = 1.0 / (tauM * trK);
+    double tauB = up[0] * up[0] * Kxi[0][0] + up[1] * up[0] * Kxi[1][0] + up[2] * up[0] * Kxi[2][0] +
+                  up[0] * up[1] * Kxi[0][1] + up[1] * up[1] * Kxi[1][1] + up[2] * up[1] * Kxi[2][1] +
+                  up[0] * up[2] * Kxi[0][2] + up[1] * up[2] * Kxi[1][2] + up[2] * up[2] * Kxi[2][2];
+    if (is_zero_dev(tauB)) tauB = DBL_EPSILON;
+    tauB = rho / sqrt(tauB);
+    double ua[3] = {u[0] + up[0], u[1] + up[1], u[2] + up[2]};
+    const double pa = p - tauC * divU;
+    double rV2[3], rV3[3];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      rV2[j] = tauB * (up[0] * ux[0][j] + up[1] * ux[1][j] + up[2] * ux[2][j]);
+      rV3[j] = ud[j] + ua[0] * ux[0][j] + ua[1] * ux[1][j] + ua[2] * ux[2][j];
+    }
+    double rM[3][3];
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        rM[i][j] = mu * es[i][j] - rho * up[j] * ua[i] + rV2[j] * up[i];
+        if (i == j) rM[i][j] = rM[i][j] - pa;
+      }
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        lR[i][a] = lR[i][a] + wr * N[a] * rV3[i] + w * (Nx[0][a] * rM[0][i] + Nx[1][a] * rM[1][i] + Nx[2][a] * rM[2][i]);
+      }
+      gUNx[g][a] = u[0] * Nx[0][a] + u[1] * Nx[1][a] + u[2] * Nx[2][a];
+      gUpNx[g][a] = up[0] * Nx[0][a] + up[1] * Nx[1][a] + up[2] * Nx[2][a];
+    }
+    // Brinkman term (fluid.cpp:2132-2138)
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+      for (int i = 0; i < 3; i++) lR[i][a] = lR[i][a] + muK * w * N[a] * (u[i] + up[i]);
+    gTauM[g] = tauM;
+    gTauC[g] = tauC;
+    gTauB[g] = tauB;
+  }
+  // Continuity residual (fluid_3d_c, fluid.cpp:1655-1658), Gauss points in order.
+#pragma unroll
+  for (int g = 0; g < 4; g++)
+#pragma unroll
+    for (int a = 0; a < 4; a++) lR[3][a] = lR[3][a] + w * (C.N[g][a] * divU - gUpNx[g][a]);
+
+  // ---- R scatter (do_assem, lhsa.cpp:113-115)
+#pragma unroll
+  for (int a = 0; a < 4; a++) {
+    double4* rp = reinterpret_cast<double4*>(R + 4 * (size_t)nd[a]);
+    double4 r = *rp;
+    r.x = r.x + lR[0][a];
+    r.y = r.y + lR[1][a];
+    r.z = r.z + lR[2][a];
+    r.w = r.w + lR[3][a];
+    *rp = r;
+  }
+
+  // ---- tangent blocks, one (a,b) pair at a time, accumulated over g in order
+  const double rho_amd = rho * C.amd;
+#pragma unroll 1
+  for (int ab = 0; ab < 16; ab++) {
+    const int a = ab >> 2, b = ab & 3;
+    const int s = slot[(size_t)(b * 4 + a) * nElPad + q];
+    double Nxa[3], Nxb[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      // dynamic a/b index into register arrays: select through a small switch
+      Nxa[i] = (a == 0) ? Nx[i][0] : (a == 1) ? Nx[i][1] : (a == 2) ? Nx[i][2] : Nx[i][3];
+      Nxb[i] = (b == 0) ? Nx[i][0] : (b == 1) ? Nx[i][1] : (b == 2) ? Nx[i][2] : Nx[i][3];
+    }
+    const double NxNx = Nxa[0] * Nxb[0] + Nxa[1] * Nxb[1] + Nxa[2] * Nxb[2];
+    double acc[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) acc[k] = 0.0;
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+      const double Na = C.N[g][a], Nb = C.N[g][b];
+      const double uNxa = (a == 0) ? gUNx[g][0] : (a == 1) ? gUNx[g][1] : (a == 2) ? gUNx[g][2] : gUNx[g][3];
+      const double uNxb = (b == 0) ? gUNx[g][0] : (b == 1) ? gUNx[g][1] : (b == 2) ? gUNx[g][2] : gUNx[g][3];
+      const double upNxa = (a == 0) ? gUpNx[g][0] : (a == 1) ? gUpNx[g][1] : (a == 2) ? gUpNx[g][2] : gUpNx[g][3];
+      const double upNxb = (b == 0) ? gUpNx[g][0] : (b == 1) ? gUpNx[g][1] : (b == 2) ? gUpNx[g][2] : gUpNx[g][3];
+      const double tauM = gTauM[g], tauC = gTauC[g], tauB = gTauB[g];
+      const double uaNxa = uNxa + upNxa;
+      const double T1ub = -rho * uNxb - muK * Nb;  // updu(i,i,b)
+      const double rtu = rho * tauM * uaNxa;
+      // momentum, fluid.cpp:2058-2113
+      const double T1 = mu * NxNx + rho_amd * Nb * (Na + rho * tauM * uaNxa) + rho * Na * (uNxb + upNxb) +
+                        tauB * upNxa * upNxb;
+#pragma unroll
+      for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+          if (i == j) {
+            const double T2 = (mu + tauC) * (Nxa[i] * Nxb[i]) - rtu * T1ub;
+            acc[4 * i + j] = acc[4 * i + j] + wl * (T2 + T1);
+            acc[4 * i + j] = acc[4 * i + j] + muK * wl * Nb * Na;
+          } else {
+            const double T2 = mu * (Nxa[j] * Nxb[i]) + tauC * (Nxa[i] * Nxb[j]);
+            acc[4 * i + j] = acc[4 * i + j] + wl * T2;
+          }
+        }
+      // pressure columns, fluid.cpp:2117-2130
+#pragma unroll
+      for (int i = 0; i < 3; i++) acc[4 * i + 3] = acc[4 * i + 3] - wl * (Nxa[i] * Nb - Nxb[i] * rtu);
+      // continuity rows, fluid.cpp:1662-1688
+      const double T1c = rho_amd * Nb;
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        const double T2 = Nxa[j] * (T1ub - T1c);
+        acc[12 + j] = acc[12 + j] + wl * (Na * Nxb[j] - tauM * T2);
+      }
+      acc[15] = acc[15] + wl * tauM * NxNx;
+    }
+    double2* vp = reinterpret_cast<double2*>(Val + 16 * (size_t)s);
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      double2 v = vp[k];
+      v.x = v.x + acc[2 * k];
+      v.y = v.y + acc[2 * k + 1];
+      vp[k] = v;
+    }
+  }
+}
+
+__global__ void k_set_int(int* p, int v) { *p = v; }
+
+}  // namespace svg
+
+using namespace svg;
+
+extern "C" int svg_construct_fluid(svg_msh* M, const svg_fluid_params* p, const double* Ag, const double* Yg,
+                                   const double* Bf, double* R, double* Val, void* stream)
+{
+  if (!M || !p || !Ag || !Yg || !R || !Val) return set_error(SVG_ERR_ARG, "svg_construct_fluid: null argument");
+  hipStream_t s = (hipStream_t)stream;
+  FluidConst C;
+  C.rho = p->rho;
+  C.mu = p->mu;
+  for (int i = 0; i < 3; i++) C.f[i] = p->f[i];
+  C.K = p->K_darcy;
+  C.T1 = p->af * p->gam * p->dt;
+  C.amd = p->am / C.T1;
+  C.kT = 4.0 * pow(1.0 / p->dt, 2.0);
+  C.kT = C.kT + pow(p->K_darcy * p->mu / p->rho, 2.0);
+  C.mr2 = pow(p->mu / p->rho, 2.0);
+  C.wg = 1.0 / 24.0;
+  // TET4 Gauss points and shape functions (nn_elem_gip.h:501-517, nn_elem_gnn.h:547-569).
+  double sq = p->qmTET4 > 0 ? p->qmTET4 : (5.0 + 3.0 * sqrt(5.0)) / 20.0;
+  double t = (1 - sq) / 3.0;
+  double xi[4][3] = {{sq, t, t}, {t, sq, t}, {t, t, sq}, {t, t, t}};
+  for (int g = 0; g < 4; g++) {
+    C.N[g][0] = xi[g][0];
+    C.N[g][1] = xi[g][1];
+    C.N[g][2] = xi[g][2];
+    C.N[g][3] = 1.0 - xi[g][0] - xi[g][1] - xi[g][2];
+  }
+  launch("set_int", k_set_int, dim3(1), dim3(1), 0, s, M->d_err, INT_MAX);
+  const int bs = 128;
+  for (int c = 0; c < M->nColors; c++) {
+    int q0 = M->colorOff[c], qn = M->colorOff[c + 1] - q0;
+    if (qn <= 0) continue;
+    launch("fluid_tet4", k_fluid_tet4, dim3(blocks_for(qn, bs)), dim3(bs), 0, s, q0, qn,
+           reinterpret_cast<const int4*>(M->d_ien), (const int*)M->d_slot, M->nElPad, (const double*)M->d_x, Ag, Yg,
+           Bf, C, R, Val, M->d_err);
+  }
+  SVG_HIP(hipGetLastError());
+  SVG_HIP(hipMemcpyAsync(M->h_err, M->d_err, sizeof(int), hipMemcpyDeviceToHost, s));
+  SVG_HIP(hipStreamSynchronize(s));
+  if (*M->h_err != INT_MAX) {
+    return set_error(SVG_ERR_JACOBIAN, "[construct_fluid] Jacobian for element (colour-sorted index " +
+                                           std::to_string(*M->h_err) + ") is < 0.");
+  }
+  return SVG_OK;
+}

@@ -1,0 +1,122 @@
+// Device-side declarations shared by the HIP translation units: the opaque
+// handle types behind the C ABI, error/launch helpers and event timing.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "svg_internal.h"
+
+namespace svg {
+
+// Returns SVG_ERR_HIP with a message if `e` is not hipSuccess.
+int hip_check(hipError_t e, const char* what);
+
+#define SVG_HIP(call)                                   \
+  do {                                                  \
+    hipError_t _e = (call);                             \
+    if (_e != hipSuccess) return ::svg::hip_check(_e, #call); \
+  } while (0)
+
+// Event timing of named kernels on the launching stream.
+int prof_begin(const char* name, hipStream_t s);
+void prof_end(int slot, hipStream_t s);
+
+template <typename K, typename... Args>
+inline void launch(const char* name, K kernel, dim3 grid, dim3 block, size_t shmem, hipStream_t s, Args... args)
+{
+  int slot = prof_begin(name, s);
+  hipLaunchKernelGGL(kernel, grid, block, shmem, s, args...);
+  prof_end(slot, s);
+}
+
+inline unsigned blocks_for(size_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
+
+template <typename T>
+int dev_alloc(T** p, size_t count)
+{
+  *p = nullptr;
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+  if (e != hipSuccess) return hip_check(e, "hipMalloc");
+  return SVG_OK;
+}
+
+template <typename T>
+void dev_free(T*& p)
+{
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+// Device state of the GMRES cycle (one per solve workspace).
+struct GmState {
+  int done;     // |err(i+1)| < eps reached in the current cycle
+  int last_i;   // last Arnoldi column of the cycle
+  int itr;      // ls.itr (SpMV count + restarts)
+  int suc;
+  int sD;
+  int divzero;  // err(0) == 0 at a restart (Array division by zero)
+  double eps;
+  double scale;  // 1 / h(i+1,i) for the current normalisation
+  double iNorm, fNorm;
+};
+
+// Krylov workspace kept resident between solves on one lhs.
+struct KrylovWork {
+  int n = 0;        // dof * nNo
+  int sD = 0;
+  size_t ld = 0;    // stride between basis vectors (elements)
+  double* u = nullptr;        // (sD+1) basis vectors
+  double* X = nullptr;        // solution accumulator
+  double* W = nullptr;        // preconditioner diagonal (dof,nNo)
+  double* h = nullptr;        // (sD+1) x sD Hessenberg, column-major
+  double* cs = nullptr;       // Givens cosines (sD)
+  double* sn = nullptr;       // Givens sines (sD)
+  double* err = nullptr;      // (sD+1)
+  double* coef = nullptr;     // projection coefficients of the current column (sD+2)
+  double* y = nullptr;        // (sD)
+  double* partial = nullptr;  // reduction partials (nblk x (sD+2))
+  int nblk = 0;
+  GmState* st = nullptr;
+  GmState* h_st = nullptr;    // pinned host mirror
+  double* scal = nullptr;     // scratch scalars for standalone dots
+};
+
+}  // namespace svg
+
+struct svg_face_rec {
+  int nNo = 0, dof = 0, bGrp = SVG_BC_DIR;
+  bool created = false, incFlag = false, coupledFlag = false;
+  double res = 0.0;
+  std::vector<int> glob;
+  std::vector<double> val;
+};
+
+struct svg_lhs {
+  int nNo = 0, nnz = 0, nFaces = 0;
+  std::vector<int> rowPtr, colPtr, diagPtr;
+  int* d_rowPtr = nullptr;   // nNo+1, exclusive end
+  int* d_colPtr = nullptr;
+  int* d_diagPtr = nullptr;
+  int* d_trans = nullptr;    // block index of the transposed block (lazy, NS solver)
+  std::vector<svg_face_rec> face;
+  double* d_mask = nullptr;  // (4,nNo): product of Dirichlet face values per dof
+  std::vector<int> maskKey;  // incFlag pattern the mask was built for
+  bool maskBuilt = false;
+  svg::KrylovWork kw;
+};
+
+struct svg_msh {
+  const svg_lhs* lhs = nullptr;
+  int nEl = 0, nNo = 0, nColors = 0;
+  std::vector<int> colorOff;   // nColors+1 offsets into the colour-sorted order
+  double* d_x = nullptr;       // (3,nNo)
+  int* d_ien = nullptr;        // (4, nEl) in colour-sorted order (int4 per element)
+  int* d_slot = nullptr;       // 16 slots per element, SoA: slot[p*nElPad + e]
+  int* d_err = nullptr;        // zero-Jacobian flag (element index + 1)
+  int* h_err = nullptr;        // pinned mirror
+  int nElPad = 0;
+};

@@ -1,0 +1,354 @@
+// Runtime of the C ABI: errors, event timing, the device-resident lhs and
+// mesh handles, and the element-wise generalized-alpha kernels.
+#include "svg_device.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+
+namespace svg {
+
+static thread_local std::string g_err;
+
+int set_error(int code, const std::string& msg)
+{
+  g_err = msg;
+  return code;
+}
+
+int hip_check(hipError_t e, const char* what)
+{
+  return set_error(SVG_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// ---------------------------------------------------------------- timing --
+struct ProfAcc {
+  int count = 0;
+  double ms = 0.0;
+};
+struct ProfPending {
+  std::string name;
+  hipEvent_t a, b;
+};
+static std::mutex g_prof_mu;
+static bool g_prof_on = false;
+static std::map<std::string, ProfAcc> g_prof;
+static std::vector<ProfPending> g_pending;
+static std::vector<hipEvent_t> g_event_pool;
+
+static hipEvent_t take_event()
+{
+  if (!g_event_pool.empty()) {
+    hipEvent_t e = g_event_pool.back();
+    g_event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+int prof_begin(const char* name, hipStream_t s)
+{
+  if (!g_prof_on) return -1;
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  ProfPending p;
+  p.name = name;
+  p.a = take_event();
+  p.b = take_event();
+  if (!p.a || !p.b) return -1;
+  (void)hipEventRecord(p.a, s);
+  g_pending.push_back(p);
+  return (int)g_pending.size() - 1;
+}
+
+void prof_end(int slot, hipStream_t s)
+{
+  if (slot < 0) return;
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  (void)hipEventRecord(g_pending[slot].b, s);
+}
+
+static void prof_flush()
+{
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  for (auto& p : g_pending) {
+    float ms = 0.f;
+    if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      auto& acc = g_prof[p.name];
+      acc.count++;
+      acc.ms += ms;
+    }
+    g_event_pool.push_back(p.a);
+    g_event_pool.push_back(p.b);
+  }
+  g_pending.clear();
+}
+
+// ------------------------------------------------------- element kernels --
+__global__ void k_pici(int n, double c0, double c1, double c2, double c3, const double* __restrict__ Ao,
+                       const double* __restrict__ An, const double* __restrict__ Yo, const double* __restrict__ Yn,
+                       double* __restrict__ Ag, double* __restrict__ Yg)
+{
+#pragma clang fp contract(off)
+  size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (size_t)n) return;
+  Ag[k] = Ao[k] * c0 + An[k] * c1;
+  Yg[k] = Yo[k] * c2 + Yn[k] * c3;
+}
+
+__global__ void k_picc(int n, double coef, const double* __restrict__ R, double* __restrict__ An,
+                       double* __restrict__ Yn)
+{
+#pragma clang fp contract(off)
+  size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (size_t)n) return;
+  double r = R[k];
+  An[k] = An[k] - r;
+  Yn[k] = Yn[k] - r * coef;
+}
+
+__global__ void k_fill(size_t n, double v, double* __restrict__ d)
+{
+  size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; k < n; k += stride) d[k] = v;
+}
+
+}  // namespace svg
+
+using namespace svg;
+
+extern "C" {
+
+const char* svg_last_error(void) { return svg::g_err.c_str(); }
+
+int svg_prof_enable(int on)
+{
+  g_prof_on = (on != 0);
+  return SVG_OK;
+}
+
+int svg_prof_reset(void)
+{
+  prof_flush();
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  g_prof.clear();
+  return SVG_OK;
+}
+
+int svg_prof_get(const char* name, int* count, double* total_ms)
+{
+  prof_flush();
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  auto it = g_prof.find(name ? name : "");
+  *count = it == g_prof.end() ? 0 : it->second.count;
+  *total_ms = it == g_prof.end() ? 0.0 : it->second.ms;
+  return SVG_OK;
+}
+
+int svg_pici(int nNo, double am, double af, const double* Ao, const double* An, const double* Yo, const double* Yn,
+             double* Ag, double* Yg, void* stream)
+{
+  if (nNo < 0) return set_error(SVG_ERR_ARG, "svg_pici: nNo < 0");
+  if (nNo == 0) return SVG_OK;
+  int n = 4 * nNo;
+  launch("pici", k_pici, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream, n, 1.0 - am, am, 1.0 - af,
+         af, Ao, An, Yo, Yn, Ag, Yg);
+  SVG_HIP(hipGetLastError());
+  return SVG_OK;
+}
+
+int svg_picc_update(int nNo, double gam, double dt, const double* R, double* An, double* Yn, void* stream)
+{
+  if (nNo < 0) return set_error(SVG_ERR_ARG, "svg_picc_update: nNo < 0");
+  if (nNo == 0) return SVG_OK;
+  int n = 4 * nNo;
+  launch("picc", k_picc, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream, n, gam * dt, R, An, Yn);
+  SVG_HIP(hipGetLastError());
+  return SVG_OK;
+}
+
+int svg_fill(double* d, size_t n, double v, void* stream)
+{
+  if (n == 0) return SVG_OK;
+  if (!d) return set_error(SVG_ERR_ARG, "svg_fill: null pointer");
+  unsigned nb = (unsigned)std::min<size_t>(blocks_for(n, 256), 8192);
+  launch("fill", k_fill, dim3(nb), dim3(256), 0, (hipStream_t)stream, n, v, d);
+  SVG_HIP(hipGetLastError());
+  return SVG_OK;
+}
+
+// ------------------------------------------------------------------ lhs --
+int svg_lhs_create(int nNo, int nnz, const int* rowPtr, const int* colPtr, int nFaces, svg_lhs** out)
+{
+  if (!out || !rowPtr || !colPtr || nNo <= 0 || nnz <= 0 || nFaces < 0)
+    return set_error(SVG_ERR_ARG, "svg_lhs_create: bad argument");
+  if ((long long)nnz * 16 > 0x7fffffffLL) return set_error(SVG_ERR_ARG, "svg_lhs_create: 16*nnz overflows int");
+  if (rowPtr[0] != 0 || rowPtr[nNo] != nnz) return set_error(SVG_ERR_ARG, "svg_lhs_create: rowPtr does not span nnz");
+  auto* L = new svg_lhs();
+  L->nNo = nNo;
+  L->nnz = nnz;
+  L->nFaces = nFaces;
+  L->rowPtr.assign(rowPtr, rowPtr + nNo + 1);
+  L->colPtr.assign(colPtr, colPtr + nnz);
+  L->diagPtr.assign(nNo, -1);
+  for (int a = 0; a < nNo; a++) {
+    if (rowPtr[a + 1] < rowPtr[a]) {
+      delete L;
+      return set_error(SVG_ERR_ARG, "svg_lhs_create: rowPtr not monotone");
+    }
+    for (int i = rowPtr[a]; i < rowPtr[a + 1]; i++) {
+      if (colPtr[i] < 0 || colPtr[i] >= nNo) {
+        delete L;
+        return set_error(SVG_ERR_ARG, "svg_lhs_create: colPtr entry out of range");
+      }
+      if (i > rowPtr[a] && colPtr[i] <= colPtr[i - 1]) {
+        delete L;
+        return set_error(SVG_ERR_ARG, "svg_lhs_create: columns not strictly increasing within a row");
+      }
+      if (colPtr[i] == a) L->diagPtr[a] = i;
+    }
+    if (L->diagPtr[a] < 0) {
+      delete L;
+      return set_error(SVG_ERR_ARG, "svg_lhs_create: row without a diagonal block");
+    }
+  }
+  L->face.resize(nFaces);
+  int rc = SVG_OK;
+  if ((rc = dev_alloc(&L->d_rowPtr, nNo + 1)) || (rc = dev_alloc(&L->d_colPtr, nnz)) ||
+      (rc = dev_alloc(&L->d_diagPtr, nNo)) || (rc = dev_alloc(&L->d_mask, (size_t)4 * nNo))) {
+    svg_lhs_free(L);
+    return rc;
+  }
+  hipError_t e = hipMemcpy(L->d_rowPtr, rowPtr, sizeof(int) * (nNo + 1), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(L->d_colPtr, colPtr, sizeof(int) * nnz, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(L->d_diagPtr, L->diagPtr.data(), sizeof(int) * nNo, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    svg_lhs_free(L);
+    return hip_check(e, "svg_lhs_create: upload");
+  }
+  *out = L;
+  return SVG_OK;
+}
+
+int svg_lhs_bc_create(svg_lhs* L, int faIn, int nNo, int dof, int bGrp, const int* gNodes, const double* val)
+{
+  if (!L) return set_error(SVG_ERR_ARG, "svg_lhs_bc_create: null lhs");
+  if (faIn >= L->nFaces)
+    return set_error(SVG_ERR_ARG, "FSILS: faIn is exceeding lhs structure maximum number of faces (" +
+                                      std::to_string(L->nFaces) + ") is less than " + std::to_string(faIn) + ".");
+  if (faIn <= -1) return set_error(SVG_ERR_ARG, "FSILS: faIn is smaller than zero");
+  if (nNo < 0 || dof <= 0 || (nNo > 0 && !gNodes)) return set_error(SVG_ERR_ARG, "svg_lhs_bc_create: bad face");
+  auto& f = L->face[faIn];
+  f.created = true;
+  f.nNo = nNo;
+  f.dof = dof;
+  f.bGrp = bGrp;
+  f.glob.assign(gNodes, gNodes + nNo);
+  for (int a = 0; a < nNo; a++)
+    if (f.glob[a] < 0 || f.glob[a] >= L->nNo) return set_error(SVG_ERR_ARG, "svg_lhs_bc_create: node out of range");
+  f.val.assign((size_t)dof * nNo, 0.0);
+  if (val) std::memcpy(f.val.data(), val, sizeof(double) * dof * nNo);
+  L->maskBuilt = false;
+  return SVG_OK;
+}
+
+int svg_lhs_free(svg_lhs* L)
+{
+  if (!L) return SVG_OK;
+  dev_free(L->d_rowPtr);
+  dev_free(L->d_colPtr);
+  dev_free(L->d_diagPtr);
+  dev_free(L->d_trans);
+  dev_free(L->d_mask);
+  auto& k = L->kw;
+  dev_free(k.u);
+  dev_free(k.X);
+  dev_free(k.W);
+  dev_free(k.h);
+  dev_free(k.cs);
+  dev_free(k.sn);
+  dev_free(k.err);
+  dev_free(k.coef);
+  dev_free(k.y);
+  dev_free(k.partial);
+  dev_free(k.st);
+  dev_free(k.scal);
+  if (k.h_st) (void)hipHostFree(k.h_st);
+  delete L;
+  return SVG_OK;
+}
+
+// ----------------------------------------------------------------- mesh --
+int svg_msh_create(const svg_lhs* L, int nEl, const int* IEN, int nNo, const double* x, svg_msh** out)
+{
+  if (!L || !IEN || !x || !out || nEl <= 0) return set_error(SVG_ERR_ARG, "svg_msh_create: bad argument");
+  if (nNo != L->nNo) return set_error(SVG_ERR_ARG, "svg_msh_create: nNo differs from the lhs");
+  std::vector<int> color, slots;
+  int nColors = 0;
+  int rc = color_elements(nNo, nEl, 4, IEN, color, &nColors);
+  if (rc) return rc;
+  rc = slot_map(nEl, 4, IEN, L->rowPtr.data(), L->colPtr.data(), slots);
+  if (rc) return rc;
+  // Colour-sorted order (stable: element order inside each colour).
+  std::vector<int> off(nColors + 1, 0), order(nEl);
+  for (int e = 0; e < nEl; e++) off[color[e] + 1]++;
+  for (int c = 0; c < nColors; c++) off[c + 1] += off[c];
+  {
+    std::vector<int> fill(off.begin(), off.end() - 1);
+    for (int e = 0; e < nEl; e++) order[fill[color[e]]++] = e;
+  }
+  auto* M = new svg_msh();
+  M->lhs = L;
+  M->nEl = nEl;
+  M->nNo = nNo;
+  M->nColors = nColors;
+  M->colorOff = off;
+  M->nElPad = (nEl + 63) & ~63;
+  std::vector<int> ien((size_t)4 * nEl), sl((size_t)16 * M->nElPad, 0);
+  for (int q = 0; q < nEl; q++) {
+    int e = order[q];
+    for (int a = 0; a < 4; a++) ien[4 * (size_t)q + a] = IEN[4 * (size_t)e + a];
+    for (int p = 0; p < 16; p++) sl[(size_t)p * M->nElPad + q] = slots[16 * (size_t)e + p];
+  }
+  if ((rc = dev_alloc(&M->d_x, (size_t)3 * nNo)) || (rc = dev_alloc(&M->d_ien, (size_t)4 * nEl)) ||
+      (rc = dev_alloc(&M->d_slot, (size_t)16 * M->nElPad)) || (rc = dev_alloc(&M->d_err, 1))) {
+    svg_msh_free(M);
+    return rc;
+  }
+  hipError_t e = hipMemcpy(M->d_x, x, sizeof(double) * 3 * nNo, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(M->d_ien, ien.data(), sizeof(int) * ien.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(M->d_slot, sl.data(), sizeof(int) * sl.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(M->d_err, 0, sizeof(int));
+  if (e == hipSuccess) e = hipHostMalloc((void**)&M->h_err, sizeof(int), hipHostMallocDefault);
+  if (e != hipSuccess) {
+    svg_msh_free(M);
+    return hip_check(e, "svg_msh_create: upload");
+  }
+  *out = M;
+  return SVG_OK;
+}
+
+int svg_msh_free(svg_msh* M)
+{
+  if (!M) return SVG_OK;
+  dev_free(M->d_x);
+  dev_free(M->d_ien);
+  dev_free(M->d_slot);
+  dev_free(M->d_err);
+  if (M->h_err) (void)hipHostFree(M->h_err);
+  delete M;
+  return SVG_OK;
+}
+
+int svg_msh_num_colors(const svg_msh* M, int* nColors)
+{
+  if (!M || !nColors) return set_error(SVG_ERR_ARG, "svg_msh_num_colors: null argument");
+  *nColors = M->nColors;
+  return SVG_OK;
+}
+
+}  // extern "C"
