@@ -258,3 +258,42 @@ def ge(A, B):
     N = A.shape[0]
     ok = lib.orc_ge(C.c_int(N), C.c_int(N), C.c_void_p(A.ctypes.data), C.c_void_p(_p(Bv)))
     return bool(ok), Bv
+
+
+def newton_time_step(pipe, phys, make_ls, minItr=3, maxItr=3, tol=0.5, state=None):
+    """One time step of the oracle Newton loop (main.cpp:371-617 for a single
+    fluid equation): pici -> construct_fluid -> fsils_solve -> picc, with the
+    convergence test of pic.cpp:236-281. make_ls() returns a fresh Ls.
+    Returns (An, Yn, history)."""
+    import sys as _sys
+    from svfsiplus_amd.workload import dirichlet_state
+    Ao, Yo, An, Yn = dirichlet_state(pipe, phys) if state is None else state
+    An = An.copy()
+    Yn = Yn.copy()
+    eps = _sys.float_info.epsilon
+
+    def is_zero(v):
+        a = abs(v)
+        return a / max(a, eps) < 10.0 * eps
+
+    itr, eq_iNorm, pNorm = 0, 0.0, np.finfo(np.float64).max
+    hist = []
+    while True:
+        itr += 1
+        Ag, Yg = pici(phys, Ao, An, Yo, Yn)
+        R, V = construct_fluid(pipe, phys, Ag, Yg)
+        ls = make_ls()
+        dx, _, ls = fsils_solve(pipe, ls, R, V)
+        picc_update(phys, dx, An, Yn)
+        iN = ls.RI.iNorm
+        if is_zero(iN):
+            iN = eps
+        if is_zero(eq_iNorm):
+            eq_iNorm = iN
+        if itr == 1:
+            pNorm = iN / eq_iNorm
+        r1 = iN / eq_iNorm
+        hist.append(dict(itr=itr, ls_itr=ls.RI.itr, iNorm=ls.RI.iNorm, fNorm=ls.RI.fNorm, suc=ls.RI.suc, r1=r1))
+        if itr >= maxItr or ((r1 <= tol or r1 <= tol * pNorm) and itr >= minItr):
+            break
+    return An, Yn, hist

@@ -260,7 +260,7 @@ __global__ __launch_bounds__(128) void k_fluid_tet4(int q0, int qn, const int4* 
 #pragma unroll 1
   for (int ab = 0; ab < 16; ab++) {
     const int a = ab >> 2, b = ab & 3;
-    const int s = slot[(size_t)(b * 4 + a) * nElPad + q];
+    const int s = slot[(size_t)(a * 4 + b) * nElPad + q];
     double Nxa[3], Nxb[3];
 #pragma unroll
     for (int i = 0; i < 3; i++) {

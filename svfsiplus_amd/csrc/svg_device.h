@@ -82,6 +82,9 @@ struct KrylovWork {
   int nblk = 0;
   GmState* st = nullptr;
   GmState* h_st = nullptr;    // pinned host mirror
+  int* h_done = nullptr;      // mapped host flag written by the device tail
+  int* d_done = nullptr;      // device view of h_done
+  hipEvent_t ev[8] = {};      // per-step completion events (look-ahead ring)
   double* scal = nullptr;     // scratch scalars for standalone dots
 };
 

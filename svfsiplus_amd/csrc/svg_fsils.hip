@@ -35,6 +35,7 @@ constexpr int RED_BS = 256;   // threads per reduction block (4 wavefronts)
 constexpr int RED_EPT = 4;    // elements per thread
 constexpr int RED_SPAN = RED_BS * RED_EPT;
 constexpr int FIN_BS = 1024;  // finaliser block (16 wavefronts)
+constexpr int LOOKAHEAD = 3;  // Arnoldi steps enqueued ahead of the host's convergence check
 
 __device__ __forceinline__ double wave_sum(double v)
 {
@@ -146,7 +147,8 @@ struct FinArgs {
 __global__ __launch_bounds__(FIN_BS) void k_finalize(FinArgs A, const double* __restrict__ partial, GmState* st,
                                                      double* __restrict__ h, double* __restrict__ cs,
                                                      double* __restrict__ sn, double* __restrict__ err,
-                                                     double* __restrict__ coef, double* __restrict__ out)
+                                                     double* __restrict__ coef, double* __restrict__ out,
+                                                     volatile int* host_done)
 {
 #pragma clang fp contract(off)
   __shared__ double sums[512];
@@ -212,6 +214,10 @@ __global__ __launch_bounds__(FIN_BS) void k_finalize(FinArgs A, const double* __
   if (fabs(err[i + 1]) < st->eps) {
     st->suc = 1;
     st->done = 1;
+    if (host_done) {
+      *host_done = 1;
+      __threadfence_system();
+    }
   }
 }
 
@@ -344,6 +350,11 @@ static int ensure_work(svg_lhs* L, int dof, int sD)
       (rc = dev_alloc(&k.scal, 8)))
     return rc;
   SVG_HIP(hipHostMalloc((void**)&k.h_st, sizeof(GmState), hipHostMallocDefault));
+  if (!k.h_done) {
+    SVG_HIP(hipHostMalloc((void**)&k.h_done, sizeof(int), hipHostMallocMapped | hipHostMallocCoherent));
+    SVG_HIP(hipHostGetDevicePointer((void**)&k.d_done, k.h_done, 0));
+    for (auto& e : k.ev) SVG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
   SVG_HIP(hipMemset(k.h, 0, sizeof(double) * (sD + 1) * sD));
   SVG_HIP(hipMemset(k.st, 0, sizeof(GmState)));
   return SVG_OK;
@@ -359,7 +370,7 @@ static DotScratch g_dot;
 
 static int dot_device(int n, const double* w, const double* base, size_t ld, int m, double* partial, int ldp,
                       double* out, GmState* st, FinArgs fa, double* h, double* cs, double* sn, double* err,
-                      double* coef, hipStream_t s)
+                      double* coef, hipStream_t s, int* host_done = nullptr)
 {
   int nblk = (n + RED_SPAN - 1) / RED_SPAN;
   if (nblk < 1) nblk = 1;
@@ -368,7 +379,8 @@ static int dot_device(int n, const double* w, const double* base, size_t ld, int
   fa.m = m;
   fa.nblk = nblk;
   fa.ldp = ldp;
-  launch("finalize", k_finalize, dim3(1), dim3(FIN_BS), 0, s, fa, (const double*)partial, st, h, cs, sn, err, coef, out);
+  launch("finalize", k_finalize, dim3(1), dim3(FIN_BS), 0, s, fa, (const double*)partial, st, h, cs, sn, err, coef, out,
+         (volatile int*)host_done);
   return SVG_OK;
 }
 
@@ -410,14 +422,22 @@ static int gmres_v_dev(svg_lhs* L, svg_subls* ls, int dof, const double* Val, do
     fa.mode = FIN_RESTART;
     dot_device(n, k.u, k.u, 0, 1, k.partial, sD + 2, k.scal, k.st, fa, k.h, k.cs, k.sn, k.err, k.coef, s);
     launch("div", k_div, dim3(nbv), dim3(256), 0, s, n, k.u, (const GmState*)k.st);
+    *k.h_done = 0;
     for (int i = 0; i < sD; i++) {
+      // Keep at most LOOKAHEAD steps in flight: wait for step i-LOOKAHEAD, then
+      // stop enqueuing once the device has flagged convergence.
+      if (i >= LOOKAHEAD) {
+        SVG_HIP(hipEventSynchronize(k.ev[(i - LOOKAHEAD) % 8]));
+        if (*(volatile int*)k.h_done) break;
+      }
       spmv<4, 0>(L, Val, k.u + k.ld * i, k.u + k.ld * (i + 1), nullptr, k.st, s);
       fa.mode = FIN_ARNOLDI;
       fa.i = i;
       dot_device(n, k.u + k.ld * (i + 1), k.u, k.ld, i + 2, k.partial, sD + 2, k.scal, k.st, fa, k.h, k.cs, k.sn,
-                 k.err, k.coef, s);
+                 k.err, k.coef, s, k.d_done);
       launch("gs_update", k_gs_update, dim3(nbv), dim3(256), 0, s, n, k.u, k.ld, i, (const double*)k.coef,
              (const GmState*)k.st);
+      SVG_HIP(hipEventRecord(k.ev[i % 8], s));
     }
     launch("backsolve", k_backsolve, dim3(1), dim3(64), 0, s, sD, (const double*)k.h, (const double*)k.err, k.y, k.st);
     launch("xupdate", k_xupdate, dim3(nbv), dim3(256), 0, s, n, k.X, (const double*)k.u, k.ld, (const double*)k.y,

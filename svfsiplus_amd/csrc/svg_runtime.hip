@@ -278,6 +278,9 @@ int svg_lhs_free(svg_lhs* L)
   dev_free(k.st);
   dev_free(k.scal);
   if (k.h_st) (void)hipHostFree(k.h_st);
+  if (k.h_done) (void)hipHostFree(k.h_done);
+  for (auto& e : k.ev)
+    if (e) (void)hipEventDestroy(e);
   delete L;
   return SVG_OK;
 }

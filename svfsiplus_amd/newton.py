@@ -1,0 +1,219 @@
+"""Host-side driver of the fluid Newton step on the device.
+
+Mirrors the reference's Newton loop for one fluid equation
+(Code/Source/solver/main.cpp:371-617):
+
+    pici   (pic.cpp:486-574)          Ag, Yg from Ao/An, Yo/Yn
+    ls_alloc                          R = 0, Val = 0
+    construct_fluid (fluid.cpp:464)   element assembly          -> svg_construct_fluid
+    ls_solve -> fsils_solve (solve.cpp:50)                       -> svg_fsils_solve
+    picc   (pic.cpp:74-330)           An -= R, Yn -= gam dt R, Newton convergence test
+
+Device arrays are torch tensors (HBM storage and the stream); all arithmetic
+runs in libsvgpu.so. There is no CPU fallback: without the library or a GPU
+every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import sys
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from .workload import Physics, Pipe, dirichlet_state
+
+
+def _ptr(t) -> int:
+    return t.data_ptr()
+
+
+def _stream():
+    import torch
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class DeviceLhs:
+    """Device-resident FSILS lhs (fsils_lhs_create + fsils_bc_create)."""
+
+    def __init__(self, pipe: Pipe, faces=None):
+        lib = _lib.load()
+        faces = pipe.faces if faces is None else faces
+        h = C.c_void_p()
+        _lib.check(lib.svg_lhs_create(pipe.nNo, pipe.nnz, pipe.rowPtr.ctypes.data, pipe.colPtr.ctypes.data,
+                                      len(faces), C.byref(h)))
+        self.h = h
+        self.nNo, self.nnz, self.nFaces = pipe.nNo, pipe.nnz, len(faces)
+        for i, (_, nodes) in enumerate(faces):
+            g = np.ascontiguousarray(nodes, dtype=np.int32)
+            val = np.zeros((len(g), 3))
+            _lib.check(lib.svg_lhs_bc_create(h, i, len(g), 3, _lib.BC_DIR, g.ctypes.data, val.ctypes.data))
+
+    def free(self):
+        if self.h:
+            _lib.load().svg_lhs_free(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class DeviceMesh:
+    """Device-resident element data (coordinates, connectivity, colours, slot map)."""
+
+    def __init__(self, lhs: DeviceLhs, pipe: Pipe):
+        lib = _lib.load()
+        h = C.c_void_p()
+        IEN = np.ascontiguousarray(pipe.IEN, dtype=np.int32)
+        x = np.ascontiguousarray(pipe.x)
+        _lib.check(lib.svg_msh_create(lhs.h, pipe.nEl, IEN.ctypes.data, pipe.nNo, x.ctypes.data, C.byref(h)))
+        self.h = h
+        nc = C.c_int()
+        _lib.check(lib.svg_msh_num_colors(h, C.byref(nc)))
+        self.nColors = nc.value
+
+    def free(self):
+        if self.h:
+            _lib.load().svg_msh_free(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def ls_create(ls_type: int, relTol=None, absTol=None, maxItr=None, dimKry=None, ns_gm=None, ns_cg=None) -> _lib.Ls:
+    """fsils_ls_create defaults (ls.cpp:40-120) followed by the <LS> overrides
+    of read_ls (read_files.cpp:2056-2090). ns_gm/ns_cg = (max_iterations, tolerance)."""
+    ls = _lib.Ls()
+    _lib.check(_lib.load().svg_ls_create(C.byref(ls), ls_type))
+    if maxItr is not None:
+        ls.RI.mItr = maxItr
+    if relTol is not None:
+        ls.RI.relTol = relTol
+    if absTol is not None:
+        ls.RI.absTol = absTol
+    if dimKry is not None:
+        ls.RI.sD = dimKry
+    if ls_type == _lib.LS_NS:
+        gm = ns_gm or (1000, 1e-2)
+        cg = ns_cg or (1000, 1e-2)
+        ls.GM.mItr, ls.GM.relTol = gm
+        ls.CG.mItr, ls.CG.relTol = cg
+        ls.GM.absTol = ls.RI.absTol
+        ls.CG.absTol = ls.RI.absTol
+        ls.GM.sD = ls.RI.sD
+    return ls
+
+
+def construct_fluid(msh: DeviceMesh, phys: Physics, Ag, Yg, R, Val, Bf=None):
+    prm = phys.fluid_params()
+    _lib.check(_lib.load().svg_construct_fluid(msh.h, C.byref(prm), _ptr(Ag), _ptr(Yg),
+                                               None if Bf is None else _ptr(Bf), _ptr(R), _ptr(Val), _stream()))
+
+
+def fsils_solve(lhs: DeviceLhs, ls: _lib.Ls, dof: int, Ri, Val, prec: int = _lib.PREC_FSILS, incL=None, res=None):
+    """Reference signature: fsils_solve(lhs, ls, dof, Ri, Val, prec, incL, res) (solve.cpp:50)."""
+    nf = lhs.nFaces
+    inc = np.ones(max(1, nf), dtype=np.int32) if incL is None else np.ascontiguousarray(incL, dtype=np.int32)
+    rs = None if res is None else np.ascontiguousarray(res, dtype=np.float64)
+    _lib.check(_lib.load().svg_fsils_solve(lhs.h, C.byref(ls), dof, _ptr(Ri), _ptr(Val), prec, nf, inc.ctypes.data,
+                                           None if rs is None else rs.ctypes.data, _stream()))
+    return ls
+
+
+def is_zero(v: float) -> bool:
+    """utils::is_zero (utils.cpp:170-189) with value2 = 0."""
+    a = abs(v)
+    eps = sys.float_info.epsilon
+    return a / max(a, eps) < 10.0 * eps
+
+
+@dataclass
+class EqState:
+    """The convergence bookkeeping of eqType used by picc."""
+    minItr: int = 3
+    maxItr: int = 3
+    tol: float = 0.5
+    itr: int = 0
+    iNorm: float = 0.0
+    pNorm: float = float(np.finfo(np.float64).max)
+    ok: bool = False
+
+
+class FluidNewton:
+    """One fluid equation on one pipe, all state resident in HBM."""
+
+    def __init__(self, pipe: Pipe, phys: Physics, ls: _lib.Ls, eq: EqState | None = None, device="cuda"):
+        import torch
+        if not torch.cuda.is_available():
+            raise RuntimeError("FluidNewton needs a GPU (the HIP library has no CPU path)")
+        self.torch = torch
+        self.pipe, self.phys, self.ls = pipe, phys, ls
+        self.eq = eq or EqState()
+        self.lhs = DeviceLhs(pipe)
+        self.msh = DeviceMesh(self.lhs, pipe)
+        dev = torch.device(device)
+        f64 = torch.float64
+        n = pipe.nNo
+        Ao, Yo, An, Yn = dirichlet_state(pipe, phys)
+        self.Ao = torch.tensor(Ao, dtype=f64, device=dev)
+        self.Yo = torch.tensor(Yo, dtype=f64, device=dev)
+        self.An = torch.tensor(An, dtype=f64, device=dev)
+        self.Yn = torch.tensor(Yn, dtype=f64, device=dev)
+        self.Ag = torch.empty((n, 4), dtype=f64, device=dev)
+        self.Yg = torch.empty((n, 4), dtype=f64, device=dev)
+        self.R = torch.empty((n, 4), dtype=f64, device=dev)
+        self.Val = torch.empty((pipe.nnz, 16), dtype=f64, device=dev)
+        self.history = []
+
+    def reset(self):
+        Ao, Yo, An, Yn = dirichlet_state(self.pipe, self.phys)
+        for t, a in ((self.Ao, Ao), (self.Yo, Yo), (self.An, An), (self.Yn, Yn)):
+            t.copy_(self.torch.from_numpy(a))
+        self.eq = EqState(self.eq.minItr, self.eq.maxItr, self.eq.tol)
+        self.history = []
+
+    def newton_iteration(self):
+        lib, ph, eq, ls = _lib.load(), self.phys, self.eq, self.ls
+        s = _stream()
+        eq.itr += 1
+        _lib.check(lib.svg_pici(self.pipe.nNo, ph.am, ph.af, _ptr(self.Ao), _ptr(self.An), _ptr(self.Yo),
+                                _ptr(self.Yn), _ptr(self.Ag), _ptr(self.Yg), s))
+        _lib.check(lib.svg_fill(_ptr(self.R), self.R.numel(), 0.0, s))
+        _lib.check(lib.svg_fill(_ptr(self.Val), self.Val.numel(), 0.0, s))
+        construct_fluid(self.msh, ph, self.Ag, self.Yg, self.R, self.Val)
+        fsils_solve(self.lhs, ls, 4, self.R, self.Val)
+        _lib.check(lib.svg_picc_update(self.pipe.nNo, ph.gam, ph.dt, _ptr(self.R), _ptr(self.An), _ptr(self.Yn), s))
+        # Newton convergence, pic.cpp:236-281
+        eps = np.finfo(np.float64).eps
+        if is_zero(ls.RI.iNorm):
+            ls.RI.iNorm = eps
+        if is_zero(eq.iNorm):
+            eq.iNorm = ls.RI.iNorm
+        if eq.itr == 1:
+            eq.pNorm = ls.RI.iNorm / eq.iNorm
+        r1 = ls.RI.iNorm / eq.iNorm
+        l1 = eq.itr >= eq.maxItr
+        l2 = r1 <= eq.tol
+        l3 = r1 <= eq.tol * eq.pNorm
+        l4 = eq.itr >= eq.minItr
+        if l1 or ((l2 or l3) and l4):
+            eq.ok = True
+        self.history.append(dict(itr=eq.itr, ls_itr=ls.RI.itr, iNorm=ls.RI.iNorm, fNorm=ls.RI.fNorm, suc=ls.RI.suc,
+                                 r1=r1))
+        return eq.ok
+
+    def time_step(self):
+        """One time step: Newton iterations until picc reports convergence."""
+        self.eq.itr = 0
+        self.eq.ok = False
+        while not self.newton_iteration():
+            pass
+        return self.history

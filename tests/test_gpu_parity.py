@@ -1,0 +1,155 @@
+"""GPU parity of the HIP path against the CPU oracle (same seeded inputs).
+
+Bars: bit-exact for the block SpMV and the diagonal preconditioner (same
+arithmetic, same order); <= 1e-12 norm-wise relative for the assembled Val and
+R (only the summation order of element contributions differs); GMRES
+iteration counts within +-1 of the oracle's and solutions within the solver
+tolerance; Newton iteration records matching the oracle's.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from svfsiplus_amd import _lib, workload as W
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def pipe4():
+    return W.make_pipe(4)
+
+
+@pytest.fixture(scope="module")
+def phys():
+    return W.Physics()
+
+
+def _dev(torch, a):
+    return torch.tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda")
+
+
+def _state(pipe, phys, seed=1):
+    rng = np.random.default_rng(seed)
+    Ao, Yo, An, Yn = W.dirichlet_state(pipe, phys)
+    Ag, Yg = oracle.pici(phys, Ao, An, Yo, Yn)
+    Yg = Yg + 0.5 * rng.standard_normal(Yg.shape)
+    Ag = Ag + 10.0 * rng.standard_normal(Ag.shape)
+    return Ag, Yg
+
+
+def test_spar_mul_vv_bitwise(torch_cuda, pipe4):
+    from svfsiplus_amd.newton import DeviceLhs, _stream
+    torch = torch_cuda
+    rng = np.random.default_rng(7)
+    L = DeviceLhs(pipe4)
+    for dof in (4, 3, 2, 1):
+        K = rng.standard_normal((pipe4.nnz, dof * dof))
+        U = rng.standard_normal((pipe4.nNo, dof))
+        ref = oracle.spar_mul_vv(pipe4, dof, K, U)
+        dK, dU = _dev(torch, K), _dev(torch, U)
+        dKU = torch.empty((pipe4.nNo, dof), dtype=torch.float64, device="cuda")
+        _lib.call("svg_spar_mul_vv", L.h, dof, dK.data_ptr(), dU.data_ptr(), dKU.data_ptr(), _stream())
+        torch.cuda.synchronize()
+        got = dKU.cpu().numpy()
+        print("spmv dof", dof, "max diff", np.abs(got - ref).max())
+        assert np.array_equal(got, ref), f"dof={dof}"
+
+
+def test_precond_diag_bitwise(torch_cuda, pipe4, phys):
+    from svfsiplus_amd.newton import DeviceLhs, _stream
+    torch = torch_cuda
+    Ag, Yg = _state(pipe4, phys)
+    R, V = oracle.construct_fluid(pipe4, phys, Ag, Yg)
+    V2, R2, W2 = oracle.precond_diag(pipe4, V, R)
+    L = DeviceLhs(pipe4)
+    dV, dR = _dev(torch, V), _dev(torch, R)
+    dW = torch.empty((pipe4.nNo, 4), dtype=torch.float64, device="cuda")
+    _lib.call("svg_precond_diag", L.h, 4, dV.data_ptr(), dR.data_ptr(), dW.data_ptr(), _stream())
+    torch.cuda.synchronize()
+    assert np.array_equal(dW.cpu().numpy(), W2)
+    assert np.array_equal(dR.cpu().numpy(), R2)
+    assert np.array_equal(dV.cpu().numpy(), V2)
+
+
+def test_dot_norm(torch_cuda, pipe4):
+    from svfsiplus_amd.newton import _stream
+    import ctypes as C
+    torch = torch_cuda
+    rng = np.random.default_rng(3)
+    for n in (1, 63, 1000, 4 * pipe4.nNo, 300001):
+        U = rng.standard_normal(n)
+        V = rng.standard_normal(n)
+        dU, dV = _dev(torch, U), _dev(torch, V)
+        out = C.c_double()
+        _lib.call("svg_dot_v", 1, n, dU.data_ptr(), dV.data_ptr(), C.byref(out), _stream())
+        exact = float(np.dot(U.astype(np.longdouble), V.astype(np.longdouble)))
+        assert abs(out.value - exact) <= 1e-14 * np.abs(U * V).sum() + 1e-300
+        _lib.call("svg_norm_v", 1, n, dU.data_ptr(), C.byref(out), _stream())
+        assert abs(out.value - np.sqrt(np.sum(U.astype(np.longdouble) ** 2))) <= 1e-14 * out.value
+
+
+@pytest.mark.parametrize("n", [2, 4, 8])
+def test_construct_fluid(torch_cuda, phys, n):
+    from svfsiplus_amd.newton import DeviceLhs, DeviceMesh, construct_fluid
+    torch = torch_cuda
+    pipe = W.make_pipe(n)
+    Ag, Yg = _state(pipe, phys, seed=n)
+    R0, V0 = oracle.construct_fluid(pipe, phys, Ag, Yg)
+    L = DeviceLhs(pipe)
+    M = DeviceMesh(L, pipe)
+    dR = torch.zeros((pipe.nNo, 4), dtype=torch.float64, device="cuda")
+    dV = torch.zeros((pipe.nnz, 16), dtype=torch.float64, device="cuda")
+    construct_fluid(M, phys, _dev(torch, Ag), _dev(torch, Yg), dR, dV)
+    torch.cuda.synchronize()
+    R1, V1 = dR.cpu().numpy(), dV.cpu().numpy()
+    eR = np.linalg.norm(R1 - R0) / np.linalg.norm(R0)
+    eV = np.linalg.norm(V1 - V0) / np.linalg.norm(V0)
+    mR = np.abs(R1 - R0).max() / np.abs(R0).max()
+    mV = np.abs(V1 - V0).max() / np.abs(V0).max()
+    print(f"n={n} colours={M.nColors} |dR|/|R|={eR:.2e} |dV|/|V|={eV:.2e} max {mR:.2e} {mV:.2e}")
+    assert eR <= 1e-12 and eV <= 1e-12 and mR <= 1e-12 and mV <= 1e-12
+
+
+@pytest.mark.parametrize("n", [4, 8])
+def test_gmres_solve(torch_cuda, phys, n):
+    from svfsiplus_amd.newton import DeviceLhs, fsils_solve, ls_create
+    torch = torch_cuda
+    pipe = W.make_pipe(n)
+    Ao, Yo, An, Yn = W.dirichlet_state(pipe, phys)
+    Ag, Yg = oracle.pici(phys, Ao, An, Yo, Yn)
+    R, V = oracle.construct_fluid(pipe, phys, Ag, Yg)
+    tol = 1e-6
+    ols = oracle.ls_create(oracle.LS_GMRES, relTol=tol, absTol=1e-30, maxItr=1000, dimKry=50)
+    x0, V0, ols = oracle.fsils_solve(pipe, ols, R, V)
+    L = DeviceLhs(pipe)
+    ls = ls_create(_lib.LS_GMRES, relTol=tol, absTol=1e-30, maxItr=1000, dimKry=50)
+    dR, dV = _dev(torch, R), _dev(torch, V)
+    fsils_solve(L, ls, 4, dR, dV)
+    x1 = dR.cpu().numpy()
+    rel = np.linalg.norm(x1 - x0) / np.linalg.norm(x0)
+    print(f"n={n} itr gpu={ls.RI.itr} oracle={ols.RI.itr} iNorm {ls.RI.iNorm:.17g} {ols.RI.iNorm:.17g} "
+          f"fNorm {ls.RI.fNorm:.3e} {ols.RI.fNorm:.3e} rel={rel:.2e}")
+    assert abs(ls.RI.itr - ols.RI.itr) <= 1
+    assert ls.RI.suc == ols.RI.suc == 1
+    assert abs(ls.RI.iNorm - ols.RI.iNorm) <= 1e-12 * ols.RI.iNorm
+    assert np.array_equal(dV.cpu().numpy(), V0)
+    assert rel <= 10 * tol
+
+
+def test_newton_time_step(torch_cuda, phys):
+    from svfsiplus_amd.newton import FluidNewton, ls_create
+    pipe = W.make_pipe(4)
+    tol = 1e-6
+    An0, Yn0, hist0 = oracle.newton_time_step(
+        pipe, phys, lambda: oracle.ls_create(oracle.LS_GMRES, relTol=tol, absTol=1e-30, maxItr=1000, dimKry=50))
+    fn = FluidNewton(pipe, phys, ls_create(_lib.LS_GMRES, relTol=tol, absTol=1e-30, maxItr=1000, dimKry=50))
+    hist1 = fn.time_step()
+    for h0, h1 in zip(hist0, hist1):
+        print("oracle", h0, "\ngpu   ", h1)
+    assert len(hist0) == len(hist1) == 3
+    for h0, h1 in zip(hist0, hist1):
+        assert abs(h0["ls_itr"] - h1["ls_itr"]) <= 1
+        assert abs(h0["iNorm"] - h1["iNorm"]) <= 1e-8 * h0["iNorm"]
+    Yn1 = fn.Yn.cpu().numpy()
+    assert np.linalg.norm(Yn1 - Yn0) <= 1e-6 * np.linalg.norm(Yn0)
