@@ -86,6 +86,7 @@ struct KrylovWork {
   int* d_done = nullptr;      // device view of h_done
   hipEvent_t ev[8] = {};      // per-step completion events (look-ahead ring)
   double* scal = nullptr;     // scratch scalars for standalone dots
+  unsigned* ticket = nullptr; // last-block ticket of the fused reductions
 };
 
 }  // namespace svg

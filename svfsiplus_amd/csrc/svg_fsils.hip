@@ -34,7 +34,6 @@ namespace svg {
 constexpr int RED_BS = 256;   // threads per reduction block (4 wavefronts)
 constexpr int RED_EPT = 4;    // elements per thread
 constexpr int RED_SPAN = RED_BS * RED_EPT;
-constexpr int FIN_BS = 1024;  // finaliser block (16 wavefronts)
 constexpr int LOOKAHEAD = 3;  // Arnoldi steps enqueued ahead of the host's convergence check
 
 __device__ __forceinline__ double wave_sum(double v)
@@ -46,6 +45,16 @@ __device__ __forceinline__ double wave_sum(double v)
 }
 
 // ------------------------------------------------------------------ SpMV --
+// Blocks are dispatched round-robin over the 8 XCDs; remap so that each XCD
+// walks one contiguous band of rows and its L2 keeps the gathered U entries.
+__device__ __forceinline__ int xcd_block(int b, int nb)
+{
+  const int per = (nb + 7) >> 3;
+  const int xcd = b & 7, idx = b >> 3;
+  const int r = xcd * per + idx;
+  return r < nb ? r : -1;
+}
+
 template <int D, int MODE>
 __global__ __launch_bounds__(256) void k_spmv(int nNo, const int* __restrict__ rowPtr, const int* __restrict__ colPtr,
                                               const double* __restrict__ K, const double* __restrict__ U,
@@ -54,7 +63,13 @@ __global__ __launch_bounds__(256) void k_spmv(int nNo, const int* __restrict__ r
 {
 #pragma clang fp contract(off)
   if (st && st->done) return;
-  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  int bx = blockIdx.x;
+  if (gridDim.x >= 64) {
+    // gridDim is a multiple of 8 for the remapped launches
+    bx = xcd_block(blockIdx.x, gridDim.x);
+    if (bx < 0) return;
+  }
+  const int gid = bx * blockDim.x + threadIdx.x;
   const int row = gid / D;
   const int i = gid - row * D;
   if (row >= nNo) return;
@@ -63,7 +78,32 @@ __global__ __launch_bounds__(256) void k_spmv(int nNo, const int* __restrict__ r
   if constexpr (D == 4) {
     const double2* Kb = reinterpret_cast<const double2*>(K) + 2 * i;
     const double2* Ub = reinterpret_cast<const double2*>(U);
-    for (int j = s; j < e; j++) {
+    int j = s;
+    // four blocks per trip: all loads issued first, the adds keep the CSR order
+    for (; j + 4 <= e; j += 4) {
+      int c[4];
+      double2 k0[4], k1[4], u0[4], u1[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) c[q] = __builtin_nontemporal_load(colPtr + j + q);
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const double* kp = reinterpret_cast<const double*>(Kb + 8 * (size_t)(j + q));
+        k0[q].x = __builtin_nontemporal_load(kp + 0);
+        k0[q].y = __builtin_nontemporal_load(kp + 1);
+        k1[q].x = __builtin_nontemporal_load(kp + 2);
+        k1[q].y = __builtin_nontemporal_load(kp + 3);
+        u0[q] = Ub[2 * (size_t)c[q]];
+        u1[q] = Ub[2 * (size_t)c[q] + 1];
+      }
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        double t = acc + k0[q].x * u0[q].x;
+        t = t + k0[q].y * u0[q].y;
+        t = t + k1[q].x * u1[q].x;
+        acc = t + k1[q].y * u1[q].y;
+      }
+    }
+    for (; j < e; j++) {
       const int col = colPtr[j];
       const double2 k0 = Kb[8 * (size_t)j], k1 = Kb[8 * (size_t)j + 1];
       const double2 u0 = Ub[2 * (size_t)col], u1 = Ub[2 * (size_t)col + 1];
@@ -94,20 +134,48 @@ static void spmv(const svg_lhs* L, const double* K, const double* U, double* KU,
                  hipStream_t s)
 {
   size_t nt = (size_t)D * L->nNo;
-  launch("spmv_bcsr", k_spmv<D, MODE>, dim3(blocks_for(nt, 256)), dim3(256), 0, s, L->nNo,
-         (const int*)L->d_rowPtr, (const int*)L->d_colPtr, K, U, KU, B, st);
+  unsigned nb = blocks_for(nt, 256);
+  if (nb >= 64) nb = (nb + 7) & ~7u;
+  launch("spmv_bcsr", k_spmv<D, MODE>, dim3(nb), dim3(256), 0, s, L->nNo, (const int*)L->d_rowPtr,
+         (const int*)L->d_colPtr, K, U, KU, B, st);
 }
 
 // ------------------------------------------------------------ reductions --
-// partial[b*ldp + j] = sum over block b's elements of base[j*ld + k] * w[k], j < m.
+enum FinMode { FIN_DOT = 0, FIN_NORM_R = 1, FIN_RESTART = 2, FIN_ARNOLDI = 3 };
+
+struct FinArgs {
+  int mode;
+  int i;   // Arnoldi column
+  int sD;
+  double absTol, relTol;
+};
+
+constexpr int MAXM = 512;  // sums per reduction (Krylov dimension + 2)
+
+// Dots of w against m vectors base + j*ld (j < m), reduced deterministically:
+// every block writes its partial sums (transposed, partial[j*nblk + b]); the
+// last block to finish (atomic ticket) sums the partials of every column in a
+// fixed order and runs the tail selected by A.mode:
+//   FIN_DOT      out[0] = sum
+//   FIN_NORM_R   iNorm = fNorm = sqrt(sum), eps = max(absTol, relTol iNorm)   (gmres.cpp:482-487)
+//   FIN_RESTART  err(0) = sqrt(sum), itr += 1                                 (gmres.cpp:501-531)
+//   FIN_ARNOLDI  Hessenberg column i, Pythagorean h(i+1,i), Givens, err, stop (gmres.cpp:550-592)
 __global__ __launch_bounds__(RED_BS) void k_mdot(int n, const double* __restrict__ w, const double* __restrict__ base,
-                                                 size_t ld, int m, double* __restrict__ partial, int ldp,
-                                                 const GmState* __restrict__ st)
+                                                 size_t ld, int m, double* __restrict__ partial,
+                                                 unsigned* __restrict__ ticket, FinArgs A, GmState* st,
+                                                 double* __restrict__ h, double* __restrict__ cs,
+                                                 double* __restrict__ sn, double* __restrict__ err,
+                                                 double* __restrict__ coef, double* __restrict__ out,
+                                                 volatile int* host_done)
 {
 #pragma clang fp contract(off)
   extern __shared__ double lds[];  // [4 waves][m]
-  if (st && st->done) return;
+  __shared__ double sums[MAXM];
+  __shared__ double lh[MAXM], lcs[MAXM], lsn[MAXM];
+  __shared__ int amLast;
+  if (A.mode == FIN_ARNOLDI && st->done) return;
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int nblk = gridDim.x;
   const size_t k0 = (size_t)blockIdx.x * RED_SPAN + t;
   double wr[RED_EPT];
 #pragma unroll
@@ -129,96 +197,104 @@ __global__ __launch_bounds__(RED_BS) void k_mdot(int n, const double* __restrict
   }
   __syncthreads();
   for (int j = t; j < m; j += RED_BS)
-    partial[(size_t)blockIdx.x * ldp + j] = ((lds[j] + lds[m + j]) + lds[2 * m + j]) + lds[3 * m + j];
-}
-
-enum FinMode { FIN_DOT = 0, FIN_NORM_R = 1, FIN_RESTART = 2, FIN_ARNOLDI = 3 };
-
-struct FinArgs {
-  int mode;
-  int m;        // number of sums
-  int nblk;     // partial rows
-  int ldp;
-  int i;        // Arnoldi column
-  int sD;
-  double absTol, relTol;
-};
-
-__global__ __launch_bounds__(FIN_BS) void k_finalize(FinArgs A, const double* __restrict__ partial, GmState* st,
-                                                     double* __restrict__ h, double* __restrict__ cs,
-                                                     double* __restrict__ sn, double* __restrict__ err,
-                                                     double* __restrict__ coef, double* __restrict__ out,
-                                                     volatile int* host_done)
-{
-#pragma clang fp contract(off)
-  __shared__ double sums[512];
-  if (A.mode == FIN_ARNOLDI && st->done) return;
-  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-  for (int j = wv; j < A.m; j += FIN_BS / 64) {
+    partial[(size_t)j * nblk + blockIdx.x] = ((lds[j] + lds[m + j]) + lds[2 * m + j]) + lds[3 * m + j];
+  __threadfence();
+  __syncthreads();
+  if (t == 0) amLast = (atomicAdd(ticket, 1u) == (unsigned)(nblk - 1));
+  __syncthreads();
+  if (!amLast) return;
+  __threadfence();
+  // ---- last block: fixed-order sums of the partials of every column
+  for (int j = wv; j < m; j += RED_BS / 64) {
+    const double* pj = partial + (size_t)j * nblk;
     double acc = 0.0;
-    for (int b = lane; b < A.nblk; b += 64) acc = acc + partial[(size_t)b * A.ldp + j];
+    for (int b = lane; b < nblk; b += 64) acc = acc + pj[b];
     acc = wave_sum(acc);
     if (lane == 0) sums[j] = acc;
   }
+  if (t == 0) *ticket = 0u;
+  if (A.mode == FIN_ARNOLDI) {
+    for (int j = t; j < A.i; j += RED_BS) {
+      lcs[j] = cs[j];
+      lsn[j] = sn[j];
+    }
+  }
   __syncthreads();
-  if (t != 0) return;
   if (A.mode == FIN_DOT) {
-    out[0] = sums[0];
+    if (t == 0) out[0] = sums[0];
     return;
   }
   if (A.mode == FIN_NORM_R) {
-    double nrm = sqrt(sums[0]);
-    st->iNorm = nrm;
-    st->fNorm = nrm;
-    st->eps = fmax(A.absTol, A.relTol * nrm);
-    st->itr = 0;
-    st->suc = 0;
-    st->done = 0;
-    st->divzero = 0;
-    st->last_i = 0;
+    if (t == 0) {
+      double nrm = sqrt(sums[0]);
+      st->iNorm = nrm;
+      st->fNorm = nrm;
+      st->eps = fmax(A.absTol, A.relTol * nrm);
+      st->itr = 0;
+      st->suc = 0;
+      st->done = 0;
+      st->divzero = 0;
+      st->last_i = 0;
+    }
     return;
   }
   if (A.mode == FIN_RESTART) {
-    err[0] = sqrt(sums[0]);
-    st->itr = st->itr + 1;
-    st->done = 0;
-    st->scale = err[0];
-    if (err[0] == 0.0) {
-      st->divzero = 1;
-      st->done = 1;
+    if (t == 0) {
+      double e0 = sqrt(sums[0]);
+      err[0] = e0;
+      st->itr = st->itr + 1;
+      st->done = 0;
+      st->scale = e0;
+      if (e0 == 0.0) {
+        st->divzero = 1;
+        st->done = 1;
+      }
     }
     return;
   }
-  // FIN_ARNOLDI, gmres.cpp:550-592
+  // FIN_ARNOLDI (serial part on LDS by one lane, then parallel stores)
   const int i = A.i, ldh = A.sD + 1;
-  double* hc = h + (size_t)ldh * i;
-  for (int j = 0; j <= i + 1; j++) hc[j] = sums[j];
-  for (int j = 0; j <= i; j++) coef[j] = hc[j];
-  for (int j = 0; j <= i; j++) hc[i + 1] = hc[i + 1] - hc[j] * hc[j];
-  hc[i + 1] = sqrt(fabs(hc[i + 1]));
-  st->scale = 1.0 / hc[i + 1];
-  for (int j = 0; j <= i - 1; j++) {
-    double tmp = cs[j] * hc[j] + sn[j] * hc[j + 1];
-    hc[j + 1] = -sn[j] * hc[j] + cs[j] * hc[j + 1];
-    hc[j] = tmp;
-  }
-  double tmp = sqrt(hc[i] * hc[i] + hc[i + 1] * hc[i + 1]);
-  cs[i] = hc[i] / tmp;
-  sn[i] = hc[i + 1] / tmp;
-  hc[i] = tmp;
-  hc[i + 1] = 0.0;
-  err[i + 1] = -sn[i] * err[i];
-  err[i] = cs[i] * err[i];
-  st->itr = st->itr + 1;
-  st->last_i = i;
-  if (fabs(err[i + 1]) < st->eps) {
-    st->suc = 1;
-    st->done = 1;
-    if (host_done) {
-      *host_done = 1;
-      __threadfence_system();
+  __shared__ double lerr[2];
+  __shared__ int conv;
+  if (t == 0) {
+    for (int j = 0; j <= i + 1; j++) lh[j] = sums[j];
+    double hh = lh[i + 1];
+    for (int j = 0; j <= i; j++) hh = hh - lh[j] * lh[j];
+    hh = sqrt(fabs(hh));
+    lh[i + 1] = hh;
+    st->scale = 1.0 / hh;
+    for (int j = 0; j <= i - 1; j++) {
+      double tmp = lcs[j] * lh[j] + lsn[j] * lh[j + 1];
+      lh[j + 1] = -lsn[j] * lh[j] + lcs[j] * lh[j + 1];
+      lh[j] = tmp;
+    }
+    double tmp = sqrt(lh[i] * lh[i] + lh[i + 1] * lh[i + 1]);
+    double c = lh[i] / tmp, sv = lh[i + 1] / tmp;
+    cs[i] = c;
+    sn[i] = sv;
+    lh[i] = tmp;
+    lh[i + 1] = 0.0;
+    double ei = err[i];
+    lerr[1] = -sv * ei;
+    lerr[0] = c * ei;
+    err[i + 1] = lerr[1];
+    err[i] = lerr[0];
+    st->itr = st->itr + 1;
+    st->last_i = i;
+    conv = fabs(lerr[1]) < st->eps;
+    if (conv) {
+      st->suc = 1;
+      st->done = 1;
+      if (host_done) {
+        *host_done = 1;
+        __threadfence_system();
+      }
     }
   }
+  __syncthreads();
+  double* hc = h + (size_t)ldh * i;
+  for (int j = t; j <= i + 1; j += RED_BS) hc[j] = lh[j];
+  for (int j = t; j <= i; j += RED_BS) coef[j] = sums[j];
 }
 
 // u_{i+1} = (u_{i+1} - sum_j coef_j u_j) * scale, sequentially in j (omp_sum_v, omp_mul_v)
@@ -243,21 +319,37 @@ __global__ __launch_bounds__(256) void k_div(int n, double* __restrict__ u, cons
   u[k] = u[k] / st->scale;
 }
 
-// Back substitution, gmres.cpp:595-608.
-__global__ void k_backsolve(int sD, const double* __restrict__ h, const double* __restrict__ err,
-                            double* __restrict__ y, GmState* st)
+// Back substitution, gmres.cpp:595-608; the triangle is staged in LDS when it fits.
+constexpr int BS_LDS = 6144;  // doubles of the staged Hessenberg (48 KB)
+__global__ __launch_bounds__(256) void k_backsolve(int sD, const double* __restrict__ h, const double* __restrict__ err,
+                                                   double* __restrict__ y, GmState* st)
 {
 #pragma clang fp contract(off)
-  if (threadIdx.x != 0) return;
+  __shared__ double lh[BS_LDS];
+  __shared__ double ly[MAXM];
   int last_i = st->last_i;
   if (last_i >= sD) last_i = sD - 1;
   const int ldh = sD + 1;
-  for (int i = 0; i <= last_i; i++) y[i] = err[i];
-  for (int j = last_i; j >= 0; j--) {
-    for (int k = j + 1; k <= last_i; k++) y[j] = y[j] - h[j + (size_t)ldh * k] * y[k];
-    y[j] = y[j] / h[j + (size_t)ldh * j];
+  const int nc = last_i + 1;
+  const bool staged = (size_t)nc * nc <= (size_t)BS_LDS;
+  if (staged) {
+    for (int q = threadIdx.x; q < nc * nc; q += blockDim.x) {
+      int r = q % nc, c = q / nc;
+      lh[q] = h[r + (size_t)ldh * c];
+    }
   }
-  st->fNorm = fabs(err[last_i + 1]);
+  for (int q = threadIdx.x; q < nc; q += blockDim.x) ly[q] = err[q];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int j = last_i; j >= 0; j--) {
+      double yj = ly[j];
+      for (int k = j + 1; k <= last_i; k++) yj = yj - (staged ? lh[j + nc * k] : h[j + (size_t)ldh * k]) * ly[k];
+      ly[j] = yj / (staged ? lh[j + nc * j] : h[j + (size_t)ldh * j]);
+    }
+    st->fNorm = fabs(err[last_i + 1]);
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < nc; q += blockDim.x) y[q] = ly[q];
 }
 
 // X += sum_j y_j u_j, sequentially in j (gmres.cpp:610-612)
@@ -336,6 +428,7 @@ static int ensure_work(svg_lhs* L, int dof, int sD)
   dev_free(k.partial);
   dev_free(k.st);
   dev_free(k.scal);
+  dev_free(k.ticket);
   if (k.h_st) (void)hipHostFree(k.h_st);
   k.h_st = nullptr;
   k.n = n;
@@ -347,7 +440,7 @@ static int ensure_work(svg_lhs* L, int dof, int sD)
       (rc = dev_alloc(&k.h, (size_t)(sD + 1) * sD)) || (rc = dev_alloc(&k.cs, sD)) || (rc = dev_alloc(&k.sn, sD)) ||
       (rc = dev_alloc(&k.err, sD + 1)) || (rc = dev_alloc(&k.coef, sD + 2)) || (rc = dev_alloc(&k.y, sD)) ||
       (rc = dev_alloc(&k.partial, (size_t)k.nblk * (sD + 2))) || (rc = dev_alloc(&k.st, 1)) ||
-      (rc = dev_alloc(&k.scal, 8)))
+      (rc = dev_alloc(&k.scal, 8)) || (rc = dev_alloc(&k.ticket, 1)))
     return rc;
   SVG_HIP(hipHostMalloc((void**)&k.h_st, sizeof(GmState), hipHostMallocDefault));
   if (!k.h_done) {
@@ -357,6 +450,7 @@ static int ensure_work(svg_lhs* L, int dof, int sD)
   }
   SVG_HIP(hipMemset(k.h, 0, sizeof(double) * (sD + 1) * sD));
   SVG_HIP(hipMemset(k.st, 0, sizeof(GmState)));
+  SVG_HIP(hipMemset(k.ticket, 0, sizeof(unsigned)));
   return SVG_OK;
 }
 
@@ -365,22 +459,19 @@ struct DotScratch {
   double* partial = nullptr;
   int cap = 0;
   double* out = nullptr;
+  unsigned* ticket = nullptr;
 };
 static DotScratch g_dot;
 
-static int dot_device(int n, const double* w, const double* base, size_t ld, int m, double* partial, int ldp,
-                      double* out, GmState* st, FinArgs fa, double* h, double* cs, double* sn, double* err,
-                      double* coef, hipStream_t s, int* host_done = nullptr)
+static int dot_device(int n, const double* w, const double* base, size_t ld, int m, double* partial,
+                      unsigned* ticket, double* out, GmState* st, FinArgs fa, double* h, double* cs, double* sn,
+                      double* err, double* coef, hipStream_t s, int* host_done = nullptr)
 {
   int nblk = (n + RED_SPAN - 1) / RED_SPAN;
   if (nblk < 1) nblk = 1;
-  launch("mdot", k_mdot, dim3(nblk), dim3(RED_BS), (size_t)4 * m * sizeof(double), s, n, w, base, ld, m, partial, ldp,
-         (const GmState*)(fa.mode == FIN_ARNOLDI ? st : nullptr));
-  fa.m = m;
-  fa.nblk = nblk;
-  fa.ldp = ldp;
-  launch("finalize", k_finalize, dim3(1), dim3(FIN_BS), 0, s, fa, (const double*)partial, st, h, cs, sn, err, coef, out,
-         (volatile int*)host_done);
+  if (m > MAXM) return set_error(SVG_ERR_ARG, "reduction: too many sums");
+  launch("mdot", k_mdot, dim3(nblk), dim3(RED_BS), (size_t)4 * m * sizeof(double), s, n, w, base, ld, m, partial,
+         ticket, fa, st, h, cs, sn, err, coef, out, (volatile int*)host_done);
   return SVG_OK;
 }
 
@@ -400,7 +491,7 @@ static int gmres_v_dev(svg_lhs* L, svg_subls* ls, int dof, const double* Val, do
   fa.absTol = ls->absTol;
   fa.relTol = ls->relTol;
   fa.mode = FIN_NORM_R;
-  dot_device(n, R, R, 0, 1, k.partial, sD + 2, k.scal, k.st, fa, k.h, k.cs, k.sn, k.err, k.coef, s);
+  dot_device(n, R, R, 0, 1, k.partial, k.ticket, k.scal, k.st, fa, k.h, k.cs, k.sn, k.err, k.coef, s);
   SVG_HIP(hipMemcpyAsync(k.h_st, k.st, sizeof(GmState), hipMemcpyDeviceToHost, s));
   SVG_HIP(hipStreamSynchronize(s));
   ls->suc = 0;
@@ -420,7 +511,7 @@ static int gmres_v_dev(svg_lhs* L, svg_subls* ls, int dof, const double* Val, do
     // u0 = R - A X; err0 = |u0|; u0 /= err0
     spmv<4, 1>(L, Val, k.X, k.u, R, nullptr, s);
     fa.mode = FIN_RESTART;
-    dot_device(n, k.u, k.u, 0, 1, k.partial, sD + 2, k.scal, k.st, fa, k.h, k.cs, k.sn, k.err, k.coef, s);
+    dot_device(n, k.u, k.u, 0, 1, k.partial, k.ticket, k.scal, k.st, fa, k.h, k.cs, k.sn, k.err, k.coef, s);
     launch("div", k_div, dim3(nbv), dim3(256), 0, s, n, k.u, (const GmState*)k.st);
     *k.h_done = 0;
     for (int i = 0; i < sD; i++) {
@@ -433,13 +524,13 @@ static int gmres_v_dev(svg_lhs* L, svg_subls* ls, int dof, const double* Val, do
       spmv<4, 0>(L, Val, k.u + k.ld * i, k.u + k.ld * (i + 1), nullptr, k.st, s);
       fa.mode = FIN_ARNOLDI;
       fa.i = i;
-      dot_device(n, k.u + k.ld * (i + 1), k.u, k.ld, i + 2, k.partial, sD + 2, k.scal, k.st, fa, k.h, k.cs, k.sn,
+      dot_device(n, k.u + k.ld * (i + 1), k.u, k.ld, i + 2, k.partial, k.ticket, k.scal, k.st, fa, k.h, k.cs, k.sn,
                  k.err, k.coef, s, k.d_done);
       launch("gs_update", k_gs_update, dim3(nbv), dim3(256), 0, s, n, k.u, k.ld, i, (const double*)k.coef,
              (const GmState*)k.st);
       SVG_HIP(hipEventRecord(k.ev[i % 8], s));
     }
-    launch("backsolve", k_backsolve, dim3(1), dim3(64), 0, s, sD, (const double*)k.h, (const double*)k.err, k.y, k.st);
+    launch("backsolve", k_backsolve, dim3(1), dim3(256), 0, s, sD, (const double*)k.h, (const double*)k.err, k.y, k.st);
     launch("xupdate", k_xupdate, dim3(nbv), dim3(256), 0, s, n, k.X, (const double*)k.u, k.ld, (const double*)k.y,
            (const GmState*)k.st);
     SVG_HIP(hipGetLastError());
@@ -572,11 +663,14 @@ static int dot_api(int n, const double* U, const double* V, double* out, hipStre
   }
   if (!g_dot.out) {
     int rc = dev_alloc(&g_dot.out, 1);
+    if (!rc) rc = dev_alloc(&g_dot.ticket, 1);
     if (rc) return rc;
+    SVG_HIP(hipMemset(g_dot.ticket, 0, sizeof(unsigned)));
   }
   FinArgs fa{};
   fa.mode = FIN_DOT;
-  dot_device(n, V, U, 0, 1, g_dot.partial, 1, g_dot.out, nullptr, fa, nullptr, nullptr, nullptr, nullptr, nullptr, s);
+  dot_device(n, V, U, 0, 1, g_dot.partial, g_dot.ticket, g_dot.out, nullptr, fa, nullptr, nullptr, nullptr, nullptr,
+             nullptr, s);
   SVG_HIP(hipGetLastError());
   SVG_HIP(hipMemcpyAsync(out, g_dot.out, sizeof(double), hipMemcpyDeviceToHost, s));
   SVG_HIP(hipStreamSynchronize(s));

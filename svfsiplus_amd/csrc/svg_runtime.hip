@@ -277,6 +277,7 @@ int svg_lhs_free(svg_lhs* L)
   dev_free(k.partial);
   dev_free(k.st);
   dev_free(k.scal);
+  dev_free(k.ticket);
   if (k.h_st) (void)hipHostFree(k.h_st);
   if (k.h_done) (void)hipHostFree(k.h_done);
   for (auto& e : k.ev)

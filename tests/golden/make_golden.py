@@ -1,0 +1,62 @@
+"""Writes the golden fixtures of tests/golden/ from the reference's own code.
+
+Runs in the build container only (where /root/reference exists and
+oracle/_ref/libsvref.so has been compiled from the reference sources by
+oracle/ref.mk). Each fixture is data: the seeded mesh, the inputs, and what
+the reference's construct_fluid and fsils_solve returned for them.
+
+    python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+import oracle  # noqa: E402
+from svfsiplus_amd import workload as W  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+# (name, n, nz, state seed, GMRES tol, NS tol)
+CASES = [("pipe_n2", 2, 6, 11, 1e-6, 1e-3), ("pipe_n4", 4, 8, 12, 1e-6, 1e-3)]
+
+
+def make(name, n, nz, seed, gm_tol, ns_tol):
+    p = W.make_pipe(n, nz)
+    ph = W.Physics()
+    rng = np.random.default_rng(seed)
+    Ao, Yo, An, Yn = W.dirichlet_state(p, ph)
+    Ag, Yg = oracle.pici(ph, Ao, An, Yo, Yn)
+    # a perturbed state exercises every term of the VMS tangent
+    Agp = Ag + 5.0 * rng.standard_normal(Ag.shape)
+    Ygp = Yg + 0.5 * rng.standard_normal(Yg.shape)
+    R_p, V_p = oracle.ref_construct_fluid(p, ph, Agp, Ygp)
+    # the first Newton iteration's system (rest state + Dirichlet inflow)
+    R0, V0 = oracle.ref_construct_fluid(p, ph, Ag, Yg)
+    gls = oracle.ls_create(oracle.LS_GMRES, relTol=gm_tol, absTol=1e-30, maxItr=1000, dimKry=50)
+    xg, Vg, og = oracle.ref_fsils_solve(p, oracle.LS_GMRES, oracle.ls_params(gls), R0, V0)
+    nls = oracle.ls_create(oracle.LS_NS, relTol=ns_tol, absTol=1e-30, maxItr=1000, dimKry=50,
+                           relTolIn=(1e-2, 1e-2), absTolIn=(1e-30, 1e-30), maxItrIn=(1000, 1000))
+    xn, Vn, on = oracle.ref_fsils_solve(p, oracle.LS_NS, oracle.ls_params(nls), R0, V0)
+    np.savez_compressed(
+        os.path.join(OUT, name + ".npz"),
+        n=n, nz=nz, x=p.x, IEN=p.IEN, rowPtr=p.rowPtr, colPtr=p.colPtr, inlet=p.faces[0][1], wall=p.faces[1][1],
+        phys=np.array([ph.rho, ph.mu, ph.dt, ph.ro_inf, ph.peak]),
+        Ag_p=Agp, Yg_p=Ygp, R_p=R_p, Val_p=V_p,
+        Ag0=Ag, Yg0=Yg, R0=R0, Val0=V0,
+        gm_tol=gm_tol, gm_x=xg, gm_Val=Vg, gm_out=np.array([og[k] for k in ("itr", "iNorm", "fNorm", "suc")]),
+        ns_tol=ns_tol, ns_x=xn,
+        ns_out=np.array([on[k] for k in ("itr", "iNorm", "fNorm", "suc", "GM_itr", "CG_itr", "Resm", "Resc")]))
+    print(name, p.nNo, p.nEl, p.nnz, "gmres itr", og["itr"], "ns itr", on["itr"], on["GM_itr"], on["CG_itr"])
+
+
+if __name__ == "__main__":
+    if not oracle.ref_available():
+        sys.exit("oracle/_ref/libsvref.so is not built (make -f oracle/ref.mk)")
+    for c in CASES:
+        make(*c)

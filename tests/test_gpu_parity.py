@@ -10,6 +10,8 @@ import numpy as np
 import pytest
 
 import oracle
+import sys, os
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
 from svfsiplus_amd import _lib, workload as W
 
 pytestmark = pytest.mark.gpu
@@ -153,3 +155,31 @@ def test_newton_time_step(torch_cuda, phys):
         assert abs(h0["iNorm"] - h1["iNorm"]) <= 1e-8 * h0["iNorm"]
     Yn1 = fn.Yn.cpu().numpy()
     assert np.linalg.norm(Yn1 - Yn0) <= 1e-6 * np.linalg.norm(Yn0)
+
+
+@pytest.mark.parametrize("name", ["pipe_n2", "pipe_n4"])
+def test_gpu_against_reference_golden(torch_cuda, name):
+    """The HIP path against the reference's own outputs (fixtures written from the reference build)."""
+    import _golden as G
+    from svfsiplus_amd.newton import DeviceLhs, DeviceMesh, construct_fluid, fsils_solve, ls_create
+    torch = torch_cuda
+    p, ph, d = G.load(name)
+    L = DeviceLhs(p)
+    M = DeviceMesh(L, p)
+    dR = torch.zeros((p.nNo, 4), dtype=torch.float64, device="cuda")
+    dV = torch.zeros((p.nnz, 16), dtype=torch.float64, device="cuda")
+    construct_fluid(M, ph, _dev(torch, d["Ag_p"]), _dev(torch, d["Yg_p"]), dR, dV)
+    torch.cuda.synchronize()
+    for got, ref in ((dR.cpu().numpy(), d["R_p"]), (dV.cpu().numpy(), d["Val_p"])):
+        assert np.linalg.norm(got - ref) <= 1e-12 * np.linalg.norm(ref)
+        assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+    tol = float(d["gm_tol"])
+    ls = ls_create(_lib.LS_GMRES, relTol=tol, absTol=1e-30, maxItr=1000, dimKry=50)
+    dR, dV = _dev(torch, d["R0"]), _dev(torch, d["Val0"])
+    fsils_solve(L, ls, 4, dR, dV)
+    itr, iNorm, fNorm, suc = d["gm_out"].tolist()
+    x = dR.cpu().numpy()
+    print(name, "gmres itr gpu", ls.RI.itr, "ref", itr, "rel", np.linalg.norm(x - d["gm_x"]) / np.linalg.norm(d["gm_x"]))
+    assert abs(ls.RI.itr - itr) <= 1 and ls.RI.suc == suc
+    assert np.array_equal(dV.cpu().numpy(), d["gm_Val"])
+    assert np.linalg.norm(x - d["gm_x"]) <= 10 * tol * np.linalg.norm(d["gm_x"])
