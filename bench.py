@@ -30,8 +30,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-KERNELS = ["spmv_bcsr", "mdot", "finalize", "gs_update", "div", "backsolve", "xupdate", "pc_w", "pc_val",
-           "fluid_tet4", "pici", "picc", "fill", "wc_scale", "set_int"]
+KERNELS = ["spmv_bcsr", "mdot", "mdot_fin", "gs_update", "div", "backsolve", "xupdate", "pc_w", "pc_val",
+           "fluid_tet4", "pici", "picc", "fill", "wc_scale", "set_int", "spmv_tune"]
 
 
 def kernel_table(lib):
@@ -83,6 +83,57 @@ def cpu_baseline(pipe, phys, gpu_hist, budget_s=30.0):
     }
 
 
+def kernel_only(a, fn, pipe, lib):
+    """SpMV launches on the first Newton system of the workload (assembled on the device),
+    and a torch device copy of the SpMV's algorithmic byte count for counter calibration."""
+    import ctypes as C
+    import torch
+    from svfsiplus_amd import _lib
+    from svfsiplus_amd.newton import _stream, construct_fluid
+    ph = fn.phys
+    _lib.check(lib.svg_pici(pipe.nNo, ph.am, ph.af, fn.Ao.data_ptr(), fn.An.data_ptr(), fn.Yo.data_ptr(),
+                            fn.Yn.data_ptr(), fn.Ag.data_ptr(), fn.Yg.data_ptr(), _stream()))
+    fn.R.zero_()
+    fn.Val.zero_()
+    construct_fluid(fn.msh, ph, fn.Ag, fn.Yg, fn.R, fn.Val)
+    U = torch.randn((pipe.nNo, 4), dtype=torch.float64, device="cuda")
+    KU = torch.empty_like(U)
+    nbytes = pipe.nnz * (16 * 8 + 4) + pipe.nNo * (4 + 4 * 8 + 4 * 8)
+    src = torch.empty(nbytes // 2 // 8, dtype=torch.float64, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    nv = C.c_int()
+    _lib.check(lib.svg_spmv_num_variants(C.byref(nv)))
+    variants = {}
+    for v in range(nv.value):
+        us = C.c_double()
+        _lib.check(lib.svg_spmv_time_variant(fn.lhs.h, v, fn.Val.data_ptr(), U.data_ptr(), KU.data_ptr(), a.launches,
+                                             C.byref(us), _stream()))
+        variants[v] = round(us.value, 2)
+    best = C.c_int()
+    _lib.check(lib.svg_spmv_tune(fn.lhs.h, fn.Val.data_ptr(), C.byref(best), _stream()))
+    lib.svg_prof_reset()
+    lib.svg_prof_enable(1)
+    for _ in range(a.launches):
+        _lib.check(lib.svg_spar_mul_vv(fn.lhs.h, 4, fn.Val.data_ptr(), U.data_ptr(), KU.data_ptr(), _stream()))
+    torch.cuda.synchronize()
+    lib.svg_prof_enable(0)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(a.launches):
+        dst.copy_(src)
+    ev1.record()
+    torch.cuda.synchronize()
+    cnt, ms = C.c_int(), C.c_double()
+    lib.svg_prof_get(b"spmv_bcsr", C.byref(cnt), C.byref(ms))
+    avg = ms.value / cnt.value * 1e-3
+    copy_avg = ev0.elapsed_time(ev1) / a.launches * 1e-3
+    print(json.dumps({"kernel": "spmv_bcsr", "launches": cnt.value, "bytes_per_launch": nbytes,
+                      "avg_us": round(avg * 1e6, 2), "achieved_GBs": round(nbytes / avg / 1e9, 1),
+                      "variant_avg_us": variants, "tuned_variant": best.value,
+                      "copy_bytes_per_launch": 2 * src.numel() * 8, "copy_avg_us": round(copy_avg * 1e6, 2),
+                      "copy_GBs": round(2 * src.numel() * 8 / copy_avg / 1e9, 1)}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,6 +144,10 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-prof", action="store_true", help="time without per-kernel events")
+    ap.add_argument("--kernel-only", choices=["spmv"], default=None,
+                    help="launch only the block SpMV on the assembled system (for counter runs), "
+                         "plus a device copy of the same byte count as the calibration kernel")
+    ap.add_argument("--launches", type=int, default=20)
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only (the multi-GPU sharding is a design, not built); "
@@ -114,6 +169,10 @@ def main():
     t_setup = time.perf_counter() - t_setup
     print(f"# setup {t_setup:.1f}s: nNo={pipe.nNo} nEl={pipe.nEl} nnzb={pipe.nnz} colours={fn.msh.nColors}",
           file=sys.stderr, flush=True)
+
+    if a.kernel_only:
+        kernel_only(a, fn, pipe, lib)
+        return
 
     for w in range(a.warmup):
         fn.reset()

@@ -154,6 +154,15 @@ int svg_fsils_solve(svg_lhs* lhs, svg_ls* ls, int dof, double* d_Ri, double* d_V
 /* Kernels of the solve, exposed for unit parity tests.                      */
 /* fsils_spar_mul_vv (spar_mul.cpp:191-258), dof in {1..4}.                  */
 int svg_spar_mul_vv(const svg_lhs* lhs, int dof, const double* d_K, const double* d_U, double* d_KU, void* stream);
+/* Autotuner of the 4x4 block SpMV (no counterpart in the reference): all
+ * launch variants (unroll depth, non-temporal matrix streaming, per-XCD row
+ * bands) compute bit-identical products; svg_spmv_tune times each on d_K over
+ * lhs's pattern and keeps the fastest for this lhs (run automatically at the
+ * first GMRES solve). */
+int svg_spmv_num_variants(int* count);
+int svg_spmv_time_variant(const svg_lhs* lhs, int variant, const double* d_K, const double* d_U, double* d_KU,
+                          int launches, double* avg_us, void* stream);
+int svg_spmv_tune(svg_lhs* lhs, const double* d_K, int* best_variant, void* stream);
 /* precond_diag (precond.cpp:122-257) with every registered face included. */
 int svg_precond_diag(svg_lhs* lhs, int dof, double* d_Val, double* d_R, double* d_W, void* stream);
 /* fsils_dot_v / fsi_ls_normv (dot.cpp:69-117, norm.cpp:67-112); blocking. */

@@ -73,6 +73,19 @@ def load():
     return _lib
 
 
+class device_order:
+    """Context manager: oracle dot products and norms in the HIP library's
+    reduction order (orc_set_dot_order(1)), for bit-level checks of the device path."""
+
+    def __enter__(self):
+        load().orc_set_dot_order(1)
+        return self
+
+    def __exit__(self, *exc):
+        load().orc_set_dot_order(0)
+        return False
+
+
 def ref_available() -> bool:
     return REF_PATH.exists()
 

@@ -112,6 +112,9 @@ double orc_dot_v(int dof, int nNo, const double* U, const double* V);
 double orc_dot_s(int nNo, const double* U, const double* V);
 double orc_norm_v(int dof, int nNo, const double* U);
 double orc_norm_s(int nNo, const double* U);
+/* 0: the reference's serial sums (default); 1: the HIP library's reduction order. */
+void orc_set_dot_order(int order);
+int orc_get_dot_order(void);
 void orc_precond_diag(const orc_lhs* lhs, int dof, double* Val, double* R, double* W);
 int  orc_ge(int N, int ldA, const double* A, double* B);
 void orc_gmres_v(const orc_lhs* lhs, orc_subls* ls, int dof, const double* Val, double* R);

@@ -110,8 +110,57 @@ void orc_spar_mul_vv(const orc_lhs* lhs, int dof, const double* K, const double*
     }
 }
 
+/* Reduction order of dot products and norms.
+ *   0: the reference's serial left-to-right sums (dot.cpp, norm.cpp).
+ *   1: the HIP library's fixed two-pass order: 1024-element spans per
+ *      workgroup, one product per lane, butterfly sums inside each 64-lane
+ *      wavefront, the 16 wavefront sums added in order; then the span partials
+ *      summed per lane (span index = lane mod 64, ascending) and a final
+ *      butterfly. Used to check the device path bit for bit; the reference
+ *      order remains the default. */
+static int g_dot_order = 0;
+void orc_set_dot_order(int order) { g_dot_order = order; }
+int orc_get_dot_order(void) { return g_dot_order; }
+
+static double butterfly64(double* v)
+{
+  double t[64];
+  for (int o = 32; o > 0; o >>= 1) {
+    for (int l = 0; l < 64; l++) t[l] = v[l] + v[l ^ o];
+    memcpy(v, t, sizeof t);
+  }
+  return v[0];
+}
+
+static double device_dot(const double* a, const double* b, long n)
+{
+  long nblk = (n + 1023) / 1024;
+  if (nblk < 1) nblk = 1;
+  double* parts = (double*)malloc(sizeof(double) * nblk);
+  for (long bk = 0; bk < nblk; bk++) {
+    double acc[1024];
+    for (int t = 0; t < 1024; t++) {
+      long k = bk * 1024 + t;
+      double v = k < n ? a[k] : 0.0, w = k < n ? b[k] : 0.0;
+      acc[t] = 0.0 + v * w;
+    }
+    double s = butterfly64(acc);
+    for (int q = 1; q < 16; q++) s = s + butterfly64(acc + 64 * q);
+    parts[bk] = s;
+  }
+  double lanes[64];
+  for (int l = 0; l < 64; l++) {
+    double s = 0.0;
+    for (long bk = l; bk < nblk; bk += 64) s = s + parts[bk];
+    lanes[l] = s;
+  }
+  free(parts);
+  return butterfly64(lanes);
+}
+
 double orc_dot_s(int nNo, const double* U, const double* V)
 {
+  if (g_dot_order) return device_dot(U, V, nNo);
   double r = 0.0;
   for (int i = 0; i < nNo; i++) r = r + U[i] * V[i];
   return r;
@@ -119,6 +168,7 @@ double orc_dot_s(int nNo, const double* U, const double* V)
 
 double orc_dot_v(int dof, int nNo, const double* U, const double* V)
 {
+  if (g_dot_order) return device_dot(U, V, (long)dof * nNo);
   double r = 0.0;
   if (dof == 4) {
     for (int i = 0; i < nNo; i++)
@@ -138,6 +188,7 @@ double orc_dot_v(int dof, int nNo, const double* U, const double* V)
 
 double orc_norm_s(int nNo, const double* U)
 {
+  if (g_dot_order) return sqrt(device_dot(U, U, nNo));
   double r = 0.0;
   for (int i = 0; i < nNo; i++) r = r + U[i] * U[i];
   return sqrt(r);
@@ -145,6 +196,7 @@ double orc_norm_s(int nNo, const double* U)
 
 double orc_norm_v(int dof, int nNo, const double* U)
 {
+  if (g_dot_order) return sqrt(device_dot(U, U, (long)dof * nNo));
   double r = 0.0;
   if (dof == 4) {
     for (int i = 0; i < nNo; i++)

@@ -53,15 +53,30 @@ void dev_free(T*& p)
 
 // Device state of the GMRES cycle (one per solve workspace).
 struct GmState {
-  int done;     // |err(i+1)| < eps reached in the current cycle
+  int done;     // |err(i+1)| < eps reached in the current cycle (first field: kernels skip on it)
   int last_i;   // last Arnoldi column of the cycle
-  int itr;      // ls.itr (SpMV count + restarts)
+  int itr;      // SpMVs + restarts counted in this call
   int suc;
   int sD;
   int divzero;  // err(0) == 0 at a restart (Array division by zero)
+  int early;    // inner GMRES: |R| <= absTol on entry, X stays 0 (gmres.cpp:146-150)
+  int pad;
   double eps;
-  double scale;  // 1 / h(i+1,i) for the current normalisation
+  double scale;  // err(0) after a restart, 1 / h(i+1,i) after an Arnoldi step
   double iNorm, fNorm;
+};
+
+// Device state of the Schur-complement CG (cgrad.cpp:50-160). errO is double
+// buffered by trip parity: trip i reads errO[i&1] and its last kernel writes
+// errO[(i+1)&1], so no kernel writes a value its own workgroups read.
+struct CgState {
+  int stop;     // err < eps seen (or mItr reached): later trips are skipped
+  int last_i;
+  int suc;
+  int pad;
+  double eps, err, iNorm;
+  double errO[2];
+  double errO_last;  // errO of the last executed trip (for dB)
 };
 
 // Krylov workspace kept resident between solves on one lhs.
@@ -89,6 +104,24 @@ struct KrylovWork {
   unsigned* ticket = nullptr; // last-block ticket of the fused reductions
 };
 
+// Workspace of the bipartitioned NS solver (ns_solver.cpp:167-499).
+struct NsWork {
+  int nNo = 0, cap = 0;     // capacity in outer iterations
+  size_t ld3 = 0, ld1 = 0;  // strides of the (3,nNo) and (nNo) vectors
+  double *mK = nullptr, *mG = nullptr, *mD = nullptr, *mL = nullptr, *Gt = nullptr;
+  double *Rm = nullptr, *Rmi = nullptr, *Rc = nullptr, *Rci = nullptr;
+  double *U = nullptr, *MU = nullptr, *P = nullptr, *MP = nullptr;  // cap, 2cap, cap, 2cap vectors
+  double *cgX = nullptr, *cgP = nullptr, *cgSP = nullptr, *cgGP = nullptr;
+  CgState* cg = nullptr;
+  CgState* h_cg = nullptr;
+  double* coef = nullptr;   // xB on the device
+  double* dots = nullptr;   // Gram entries
+  double* h_dots = nullptr;
+  double* partial = nullptr;
+  int partialCap = 0;
+  unsigned* ticket = nullptr;
+};
+
 }  // namespace svg
 
 struct svg_face_rec {
@@ -110,7 +143,10 @@ struct svg_lhs {
   double* d_mask = nullptr;  // (4,nNo): product of Dirichlet face values per dof
   std::vector<int> maskKey;  // incFlag pattern the mask was built for
   bool maskBuilt = false;
-  svg::KrylovWork kw;
+  int spmvVariant = -1;  // 4x4 SpMV launch variant picked by the autotuner (-1: not tuned)
+  svg::KrylovWork kw;    // gmres_v on the 4x4 system
+  svg::KrylovWork kw3;   // inner GMRES of the NS solver on the 3x3 momentum block
+  svg::NsWork ns;
 };
 
 struct svg_msh {

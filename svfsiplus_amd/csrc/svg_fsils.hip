@@ -24,6 +24,7 @@
 //     without host round trips; kernels after convergence exit at once.
 #include "svg_device.h"
 
+#include <algorithm>
 #include <chrono>
 #include <cfloat>
 #include <cmath>
@@ -31,8 +32,9 @@
 
 namespace svg {
 
-constexpr int RED_BS = 256;   // threads per reduction block (4 wavefronts)
-constexpr int RED_EPT = 4;    // elements per thread
+constexpr int RED_BS = 1024;  // threads per reduction workgroup (16 wavefronts), one element each
+constexpr int RED_EPT = 1;    // elements per thread
+constexpr int RED_NW = RED_BS / 64;
 constexpr int RED_SPAN = RED_BS * RED_EPT;
 constexpr int LOOKAHEAD = 3;  // Arnoldi steps enqueued ahead of the host's convergence check
 
@@ -55,73 +57,86 @@ __device__ __forceinline__ int xcd_block(int b, int nb)
   return r < nb ? r : -1;
 }
 
-template <int D, int MODE>
+// KU = K U (MODE 0) or KU = B - K U (MODE 1) for BR x BC blocks stored
+// row-major (K[BR*BC*j + BC*i + c]): vv is BR = BC = dof, sv is BR = dof,
+// BC = 1, vs is BR = 1, BC = dof, ss is 1 x 1 (spar_mul.cpp:46-258). One lane
+// per (row, i) accumulates the reference's left-to-right sum; UNR blocks'
+// loads are issued before their adds, NT streams the matrix with non-temporal
+// loads, xcd remaps workgroups to contiguous row bands per XCD. Every variant
+// performs the same additions in the same order.
+template <int BR, int BC, int MODE, int UNR, bool NT>
 __global__ __launch_bounds__(256) void k_spmv(int nNo, const int* __restrict__ rowPtr, const int* __restrict__ colPtr,
                                               const double* __restrict__ K, const double* __restrict__ U,
                                               double* __restrict__ KU, const double* __restrict__ B,
-                                              const GmState* __restrict__ st)
+                                              const int* __restrict__ skip, int xcd)
 {
 #pragma clang fp contract(off)
-  if (st && st->done) return;
+  if (skip && *skip) return;
   int bx = blockIdx.x;
-  if (gridDim.x >= 64) {
-    // gridDim is a multiple of 8 for the remapped launches
+  if (xcd) {
     bx = xcd_block(blockIdx.x, gridDim.x);
     if (bx < 0) return;
   }
   const int gid = bx * blockDim.x + threadIdx.x;
-  const int row = gid / D;
-  const int i = gid - row * D;
+  const int row = gid / BR;
+  const int i = gid - row * BR;
   if (row >= nNo) return;
   const int s = rowPtr[row], e = rowPtr[row + 1];
   double acc = 0.0;
-  if constexpr (D == 4) {
-    const double2* Kb = reinterpret_cast<const double2*>(K) + 2 * i;
-    const double2* Ub = reinterpret_cast<const double2*>(U);
-    int j = s;
-    // four blocks per trip: all loads issued first, the adds keep the CSR order
-    for (; j + 4 <= e; j += 4) {
-      int c[4];
-      double2 k0[4], k1[4], u0[4], u1[4];
+  int j = s;
+  for (; j + UNR <= e; j += UNR) {
+    int c[UNR];
+    double kv[UNR][BC], uv[UNR][BC];
 #pragma unroll
-      for (int q = 0; q < 4; q++) c[q] = __builtin_nontemporal_load(colPtr + j + q);
+    for (int q = 0; q < UNR; q++) c[q] = NT ? __builtin_nontemporal_load(colPtr + j + q) : colPtr[j + q];
 #pragma unroll
-      for (int q = 0; q < 4; q++) {
-        const double* kp = reinterpret_cast<const double*>(Kb + 8 * (size_t)(j + q));
-        k0[q].x = __builtin_nontemporal_load(kp + 0);
-        k0[q].y = __builtin_nontemporal_load(kp + 1);
-        k1[q].x = __builtin_nontemporal_load(kp + 2);
-        k1[q].y = __builtin_nontemporal_load(kp + 3);
-        u0[q] = Ub[2 * (size_t)c[q]];
-        u1[q] = Ub[2 * (size_t)c[q] + 1];
-      }
+    for (int q = 0; q < UNR; q++) {
+      const double* kp = K + (size_t)BR * BC * (j + q) + BC * i;
+      if constexpr (BC == 4) {
+        double2 a0, a1;
+        if (NT) {
+          a0.x = __builtin_nontemporal_load(kp + 0);
+          a0.y = __builtin_nontemporal_load(kp + 1);
+          a1.x = __builtin_nontemporal_load(kp + 2);
+          a1.y = __builtin_nontemporal_load(kp + 3);
+        } else {
+          a0 = reinterpret_cast<const double2*>(kp)[0];
+          a1 = reinterpret_cast<const double2*>(kp)[1];
+        }
+        kv[q][0] = a0.x;
+        kv[q][1] = a0.y;
+        kv[q][2] = a1.x;
+        kv[q][3] = a1.y;
+        const double2 u0 = reinterpret_cast<const double2*>(U + (size_t)BC * c[q])[0];
+        const double2 u1 = reinterpret_cast<const double2*>(U + (size_t)BC * c[q])[1];
+        uv[q][0] = u0.x;
+        uv[q][1] = u0.y;
+        uv[q][2] = u1.x;
+        uv[q][3] = u1.y;
+      } else {
 #pragma unroll
-      for (int q = 0; q < 4; q++) {
-        double t = acc + k0[q].x * u0[q].x;
-        t = t + k0[q].y * u0[q].y;
-        t = t + k1[q].x * u1[q].x;
-        acc = t + k1[q].y * u1[q].y;
+        for (int cc = 0; cc < BC; cc++) {
+          kv[q][cc] = NT ? __builtin_nontemporal_load(kp + cc) : kp[cc];
+          uv[q][cc] = U[(size_t)BC * c[q] + cc];
+        }
       }
     }
-    for (; j < e; j++) {
-      const int col = colPtr[j];
-      const double2 k0 = Kb[8 * (size_t)j], k1 = Kb[8 * (size_t)j + 1];
-      const double2 u0 = Ub[2 * (size_t)col], u1 = Ub[2 * (size_t)col + 1];
-      double t = acc + k0.x * u0.x;
-      t = t + k0.y * u0.y;
-      t = t + k1.x * u1.x;
-      acc = t + k1.y * u1.y;
-    }
-  } else {
-    for (int j = s; j < e; j++) {
-      const int col = colPtr[j];
-      const double* k = K + (size_t)D * D * j + D * i;
-      const double* u = U + (size_t)D * col;
-      double t = acc + k[0] * u[0];
 #pragma unroll
-      for (int c = 1; c < D; c++) t = t + k[c] * u[c];
+    for (int q = 0; q < UNR; q++) {
+      double t = acc + kv[q][0] * uv[q][0];
+#pragma unroll
+      for (int cc = 1; cc < BC; cc++) t = t + kv[q][cc] * uv[q][cc];
       acc = t;
     }
+  }
+  for (; j < e; j++) {
+    const int col = colPtr[j];
+    const double* k = K + (size_t)BR * BC * j + BC * i;
+    const double* u = U + (size_t)BC * col;
+    double t = acc + k[0] * u[0];
+#pragma unroll
+    for (int cc = 1; cc < BC; cc++) t = t + k[cc] * u[cc];
+    acc = t;
   }
   if (MODE == 0)
     KU[gid] = acc;
@@ -129,15 +144,165 @@ __global__ __launch_bounds__(256) void k_spmv(int nNo, const int* __restrict__ r
     KU[gid] = B[gid] - acc;
 }
 
-template <int D, int MODE>
-static void spmv(const svg_lhs* L, const double* K, const double* U, double* KU, const double* B, const GmState* st,
-                 hipStream_t s)
+// Launch variants of the 4x4 SpMV chosen by the autotuner (svg_spmv_tune).
+struct SpmvVariant {
+  int unroll;
+  bool nt;
+  bool xcd;
+};
+static const SpmvVariant kSpmvVariants[] = {{1, false, false}, {2, false, false}, {4, false, false},
+                                            {1, false, true},  {2, false, true},  {4, false, true},
+                                            {2, true, false},  {4, true, false},  {2, true, true},
+                                            {4, true, true},   {8, false, false}, {8, false, true}};
+constexpr int kNumSpmvVariants = sizeof(kSpmvVariants) / sizeof(kSpmvVariants[0]);
+constexpr int kDefaultSpmvVariant = 0;
+
+template <int BR, int BC, int MODE, int UNR, bool NT>
+static void spmv_launch(const svg_lhs* L, const double* K, const double* U, double* KU, const double* B,
+                        const int* skip, hipStream_t s, const char* name, bool xcd)
 {
-  size_t nt = (size_t)D * L->nNo;
+  size_t nt = (size_t)BR * L->nNo;
   unsigned nb = blocks_for(nt, 256);
-  if (nb >= 64) nb = (nb + 7) & ~7u;
-  launch("spmv_bcsr", k_spmv<D, MODE>, dim3(nb), dim3(256), 0, s, L->nNo, (const int*)L->d_rowPtr,
-         (const int*)L->d_colPtr, K, U, KU, B, st);
+  const int use_xcd = (xcd && nb >= 64) ? 1 : 0;
+  if (use_xcd) nb = (nb + 7) & ~7u;
+  launch(name, k_spmv<BR, BC, MODE, UNR, NT>, dim3(nb), dim3(256), 0, s, L->nNo, (const int*)L->d_rowPtr,
+         (const int*)L->d_colPtr, K, U, KU, B, skip, use_xcd);
+}
+
+template <int BR, int BC, int MODE>
+static void spmv_variant(int v, const svg_lhs* L, const double* K, const double* U, double* KU, const double* B,
+                         const int* skip, hipStream_t s, const char* name)
+{
+  const SpmvVariant& sv = kSpmvVariants[v];
+  switch (sv.unroll * 2 + (sv.nt ? 1 : 0)) {
+    case 2: spmv_launch<BR, BC, MODE, 1, false>(L, K, U, KU, B, skip, s, name, sv.xcd); break;
+    case 4: spmv_launch<BR, BC, MODE, 2, false>(L, K, U, KU, B, skip, s, name, sv.xcd); break;
+    case 5: spmv_launch<BR, BC, MODE, 2, true>(L, K, U, KU, B, skip, s, name, sv.xcd); break;
+    case 8: spmv_launch<BR, BC, MODE, 4, false>(L, K, U, KU, B, skip, s, name, sv.xcd); break;
+    case 9: spmv_launch<BR, BC, MODE, 4, true>(L, K, U, KU, B, skip, s, name, sv.xcd); break;
+    default: spmv_launch<BR, BC, MODE, 8, false>(L, K, U, KU, B, skip, s, name, sv.xcd); break;
+  }
+}
+
+template <int BR, int BC, int MODE>
+static void spmv(const svg_lhs* L, const double* K, const double* U, double* KU, const double* B, const int* skip,
+                 hipStream_t s, const char* name = "spmv_bcsr")
+{
+  if (BR == 4 && BC == 4) {
+    spmv_variant<BR, BC, MODE>(L->spmvVariant >= 0 ? L->spmvVariant : kDefaultSpmvVariant, L, K, U, KU, B, skip, s,
+                               name);
+  } else {
+    spmv_launch<BR, BC, MODE, 2, false>(L, K, U, KU, B, skip, s, name, true);
+  }
+}
+
+// Arnoldi step on the 4x4 system: W = K U (as k_spmv, same order) fused with
+// the classical Gram-Schmidt dots <base_j, W>, j < m (base_{m-1} is W itself):
+// workgroup b owns elements [1024 b, 1024 b + 1024) = rows [256 b, 256 b + 256),
+// one element per lane, so the partial sums follow the same fixed order as
+// k_mdot and the finaliser completes them.
+template <int UNR, bool NT>
+__global__ __launch_bounds__(1024) void k_spmv_dots(int nNo, const int* __restrict__ rowPtr,
+                                                    const int* __restrict__ colPtr, const double* __restrict__ K,
+                                                    const double* __restrict__ U, double* __restrict__ W,
+                                                    const double* __restrict__ base, size_t ld, int m,
+                                                    double* __restrict__ partial, const int* __restrict__ skip, int xcd)
+{
+#pragma clang fp contract(off)
+  extern __shared__ double lds[];  // [16 waves][m]
+  if (skip && *skip) return;
+  int bx = blockIdx.x;
+  if (xcd) {
+    bx = xcd_block(blockIdx.x, gridDim.x);
+    if (bx < 0) return;
+  }
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const size_t k = (size_t)bx * 1024 + t;
+  const int row = (int)(k >> 2), i = (int)(k & 3);
+  double w = 0.0;
+  if (row < nNo) {
+    const int s = rowPtr[row], e = rowPtr[row + 1];
+    double acc = 0.0;
+    int j = s;
+    for (; j + UNR <= e; j += UNR) {
+      int c[UNR];
+      double2 a0[UNR], a1[UNR], u0[UNR], u1[UNR];
+#pragma unroll
+      for (int q = 0; q < UNR; q++) c[q] = NT ? __builtin_nontemporal_load(colPtr + j + q) : colPtr[j + q];
+#pragma unroll
+      for (int q = 0; q < UNR; q++) {
+        const double* kp = K + 16 * (size_t)(j + q) + 4 * i;
+        if (NT) {
+          a0[q].x = __builtin_nontemporal_load(kp + 0);
+          a0[q].y = __builtin_nontemporal_load(kp + 1);
+          a1[q].x = __builtin_nontemporal_load(kp + 2);
+          a1[q].y = __builtin_nontemporal_load(kp + 3);
+        } else {
+          a0[q] = reinterpret_cast<const double2*>(kp)[0];
+          a1[q] = reinterpret_cast<const double2*>(kp)[1];
+        }
+        u0[q] = reinterpret_cast<const double2*>(U + 4 * (size_t)c[q])[0];
+        u1[q] = reinterpret_cast<const double2*>(U + 4 * (size_t)c[q])[1];
+      }
+#pragma unroll
+      for (int q = 0; q < UNR; q++) {
+        double tt = acc + a0[q].x * u0[q].x;
+        tt = tt + a0[q].y * u0[q].y;
+        tt = tt + a1[q].x * u1[q].x;
+        acc = tt + a1[q].y * u1[q].y;
+      }
+    }
+    for (; j < e; j++) {
+      const int col = colPtr[j];
+      const double* kp = K + 16 * (size_t)j + 4 * i;
+      const double* u = U + 4 * (size_t)col;
+      double tt = acc + kp[0] * u[0];
+      tt = tt + kp[1] * u[1];
+      tt = tt + kp[2] * u[2];
+      acc = tt + kp[3] * u[3];
+    }
+    w = acc;
+    W[k] = w;
+  }
+  for (int jj = 0; jj < m; jj++) {
+    double v = (jj == m - 1) ? w : (row < nNo ? base[(size_t)jj * ld + k] : 0.0);
+    double prod = 0.0 + v * w;
+    prod = wave_sum(prod);
+    if (lane == 0) lds[wv * m + jj] = prod;
+  }
+  __syncthreads();
+  const int nblk = gridDim.x;
+  for (int jj = t; jj < m; jj += 1024) {
+    double v = lds[jj];
+    for (int q = 1; q < 16; q++) v = v + lds[q * m + jj];
+    partial[(size_t)jj * nblk + bx] = v;
+  }
+}
+
+template <int UNR, bool NT>
+static unsigned spmv_dots_launch(const svg_lhs* L, const double* K, const double* U, double* W, const double* base,
+                                 size_t ld, int m, double* partial, const int* skip, hipStream_t s, bool xcd)
+{
+  unsigned nb = blocks_for((size_t)4 * L->nNo, 1024);
+  const int use_xcd = (xcd && nb >= 64) ? 1 : 0;
+  if (use_xcd) nb = (nb + 7) & ~7u;
+  launch("spmv_bcsr", k_spmv_dots<UNR, NT>, dim3(nb), dim3(1024), (size_t)16 * m * sizeof(double), s, L->nNo,
+         (const int*)L->d_rowPtr, (const int*)L->d_colPtr, K, U, W, base, ld, m, partial, skip, use_xcd);
+  return nb;
+}
+
+static unsigned spmv_dots(const svg_lhs* L, const double* K, const double* U, double* W, const double* base,
+                          size_t ld, int m, double* partial, const int* skip, hipStream_t s)
+{
+  const SpmvVariant& sv = kSpmvVariants[L->spmvVariant >= 0 ? L->spmvVariant : kDefaultSpmvVariant];
+  switch (sv.unroll * 2 + (sv.nt ? 1 : 0)) {
+    case 2: return spmv_dots_launch<1, false>(L, K, U, W, base, ld, m, partial, skip, s, sv.xcd);
+    case 4: return spmv_dots_launch<2, false>(L, K, U, W, base, ld, m, partial, skip, s, sv.xcd);
+    case 5: return spmv_dots_launch<2, true>(L, K, U, W, base, ld, m, partial, skip, s, sv.xcd);
+    case 8: return spmv_dots_launch<4, false>(L, K, U, W, base, ld, m, partial, skip, s, sv.xcd);
+    case 9: return spmv_dots_launch<4, true>(L, K, U, W, base, ld, m, partial, skip, s, sv.xcd);
+    default: return spmv_dots_launch<8, false>(L, K, U, W, base, ld, m, partial, skip, s, sv.xcd);
+  }
 }
 
 // ------------------------------------------------------------ reductions --
@@ -145,35 +310,26 @@ enum FinMode { FIN_DOT = 0, FIN_NORM_R = 1, FIN_RESTART = 2, FIN_ARNOLDI = 3 };
 
 struct FinArgs {
   int mode;
-  int i;   // Arnoldi column
+  int i;      // Arnoldi column
   int sD;
+  int inc;    // FIN_RESTART: itr increment (the first cycle of the inner GMRES adds none)
+  int first;  // FIN_RESTART: first cycle of the inner GMRES sets iNorm and eps from err(0)
   double absTol, relTol;
 };
 
 constexpr int MAXM = 512;  // sums per reduction (Krylov dimension + 2)
 
-// Dots of w against m vectors base + j*ld (j < m), reduced deterministically:
-// every block writes its partial sums (transposed, partial[j*nblk + b]); the
-// last block to finish (atomic ticket) sums the partials of every column in a
-// fixed order and runs the tail selected by A.mode:
-//   FIN_DOT      out[0] = sum
-//   FIN_NORM_R   iNorm = fNorm = sqrt(sum), eps = max(absTol, relTol iNorm)   (gmres.cpp:482-487)
-//   FIN_RESTART  err(0) = sqrt(sum), itr += 1                                 (gmres.cpp:501-531)
-//   FIN_ARNOLDI  Hessenberg column i, Pythagorean h(i+1,i), Givens, err, stop (gmres.cpp:550-592)
+// Dots of w against m vectors base + j*ld (j < m), pass 1: each workgroup
+// sums its 1024-element span (256 lanes x 4 elements at stride 256, in order),
+// butterflies inside each wavefront and adds the 4 wavefront sums in order;
+// partial[j*nblk + b].
 __global__ __launch_bounds__(RED_BS) void k_mdot(int n, const double* __restrict__ w, const double* __restrict__ base,
                                                  size_t ld, int m, double* __restrict__ partial,
-                                                 unsigned* __restrict__ ticket, FinArgs A, GmState* st,
-                                                 double* __restrict__ h, double* __restrict__ cs,
-                                                 double* __restrict__ sn, double* __restrict__ err,
-                                                 double* __restrict__ coef, double* __restrict__ out,
-                                                 volatile int* host_done)
+                                                 const int* __restrict__ skip)
 {
 #pragma clang fp contract(off)
-  extern __shared__ double lds[];  // [4 waves][m]
-  __shared__ double sums[MAXM];
-  __shared__ double lh[MAXM], lcs[MAXM], lsn[MAXM];
-  __shared__ int amLast;
-  if (A.mode == FIN_ARNOLDI && st->done) return;
+  extern __shared__ double lds[];  // [16 waves][m]
+  if (skip && *skip) return;
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   const int nblk = gridDim.x;
   const size_t k0 = (size_t)blockIdx.x * RED_SPAN + t;
@@ -196,25 +352,50 @@ __global__ __launch_bounds__(RED_BS) void k_mdot(int n, const double* __restrict
     if (lane == 0) lds[wv * m + j] = acc;
   }
   __syncthreads();
-  for (int j = t; j < m; j += RED_BS)
-    partial[(size_t)j * nblk + blockIdx.x] = ((lds[j] + lds[m + j]) + lds[2 * m + j]) + lds[3 * m + j];
-  __threadfence();
-  __syncthreads();
-  if (t == 0) amLast = (atomicAdd(ticket, 1u) == (unsigned)(nblk - 1));
-  __syncthreads();
-  if (!amLast) return;
-  __threadfence();
-  // ---- last block: fixed-order sums of the partials of every column
-  for (int j = wv; j < m; j += RED_BS / 64) {
+  for (int j = t; j < m; j += RED_BS) {
+    double v = lds[j];
+    for (int q = 1; q < RED_NW; q++) v = v + lds[q * m + j];
+    partial[(size_t)j * nblk + blockIdx.x] = v;
+  }
+}
+
+constexpr int FIN_BS = 1024;  // finaliser: one workgroup of 16 wavefronts
+
+// Pass 2 (one workgroup): the partials of every column summed in a fixed
+// order (per lane at stride 64, then a butterfly), then the tail selected by
+// A.mode:
+//   FIN_DOT      out[0] = sum
+//   FIN_NORM_R   iNorm = fNorm = sqrt(sum), eps = max(absTol, relTol iNorm)   (gmres.cpp:482-487)
+//   FIN_RESTART  err(0) = sqrt(sum), itr += inc; the inner GMRES's first
+//                cycle also sets iNorm and eps from err(0) (gmres.cpp:125-148)
+//   FIN_ARNOLDI  Hessenberg column i, Pythagorean h(i+1,i), Givens, err, stop (gmres.cpp:550-592)
+__global__ __launch_bounds__(FIN_BS) void k_fin(FinArgs A, const double* __restrict__ partial, int nblk, int m,
+                                                GmState* st, double* __restrict__ h, double* __restrict__ cs,
+                                                double* __restrict__ sn, double* __restrict__ err,
+                                                double* __restrict__ coef, double* __restrict__ out,
+                                                volatile int* host_done)
+{
+#pragma clang fp contract(off)
+  __shared__ double sums[MAXM];
+  __shared__ double lh[MAXM], lcs[MAXM], lsn[MAXM];
+  __shared__ double pre_err, pre_eps;
+  __shared__ int pre_itr;
+  if (A.mode == FIN_ARNOLDI && st->done) return;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  if (A.mode == FIN_ARNOLDI) {
+    if (t == 64) pre_err = err[A.i];
+    if (t == 128) pre_eps = st->eps;
+    if (t == 192) pre_itr = st->itr;
+  }
+  for (int j = wv; j < m; j += FIN_BS / 64) {
     const double* pj = partial + (size_t)j * nblk;
     double acc = 0.0;
     for (int b = lane; b < nblk; b += 64) acc = acc + pj[b];
     acc = wave_sum(acc);
     if (lane == 0) sums[j] = acc;
   }
-  if (t == 0) *ticket = 0u;
   if (A.mode == FIN_ARNOLDI) {
-    for (int j = t; j < A.i; j += RED_BS) {
+    for (int j = t; j < A.i; j += FIN_BS) {
       lcs[j] = cs[j];
       lsn[j] = sn[j];
     }
@@ -234,6 +415,7 @@ __global__ __launch_bounds__(RED_BS) void k_mdot(int n, const double* __restrict
       st->suc = 0;
       st->done = 0;
       st->divzero = 0;
+      st->early = 0;
       st->last_i = 0;
     }
     return;
@@ -242,9 +424,20 @@ __global__ __launch_bounds__(RED_BS) void k_mdot(int n, const double* __restrict
     if (t == 0) {
       double e0 = sqrt(sums[0]);
       err[0] = e0;
-      st->itr = st->itr + 1;
+      st->itr = st->itr + A.inc;
       st->done = 0;
       st->scale = e0;
+      if (A.first) {
+        st->early = 0;
+        if (e0 <= A.absTol) {
+          st->early = 1;
+          st->done = 1;
+          return;
+        }
+        st->iNorm = e0;
+        st->fNorm = e0;
+        st->eps = fmax(A.absTol, A.relTol * e0);
+      }
       if (e0 == 0.0) {
         st->divzero = 1;
         st->done = 1;
@@ -252,10 +445,8 @@ __global__ __launch_bounds__(RED_BS) void k_mdot(int n, const double* __restrict
     }
     return;
   }
-  // FIN_ARNOLDI (serial part on LDS by one lane, then parallel stores)
+  // FIN_ARNOLDI: serial part on LDS by one lane, then parallel stores
   const int i = A.i, ldh = A.sD + 1;
-  __shared__ double lerr[2];
-  __shared__ int conv;
   if (t == 0) {
     for (int j = 0; j <= i + 1; j++) lh[j] = sums[j];
     double hh = lh[i + 1];
@@ -274,15 +465,13 @@ __global__ __launch_bounds__(RED_BS) void k_mdot(int n, const double* __restrict
     sn[i] = sv;
     lh[i] = tmp;
     lh[i + 1] = 0.0;
-    double ei = err[i];
-    lerr[1] = -sv * ei;
-    lerr[0] = c * ei;
-    err[i + 1] = lerr[1];
-    err[i] = lerr[0];
-    st->itr = st->itr + 1;
+    double ei = pre_err;
+    double e1 = -sv * ei;
+    err[i + 1] = e1;
+    err[i] = c * ei;
+    st->itr = pre_itr + 1;
     st->last_i = i;
-    conv = fabs(lerr[1]) < st->eps;
-    if (conv) {
+    if (fabs(e1) < pre_eps) {
       st->suc = 1;
       st->done = 1;
       if (host_done) {
@@ -293,8 +482,8 @@ __global__ __launch_bounds__(RED_BS) void k_mdot(int n, const double* __restrict
   }
   __syncthreads();
   double* hc = h + (size_t)ldh * i;
-  for (int j = t; j <= i + 1; j += RED_BS) hc[j] = lh[j];
-  for (int j = t; j <= i; j += RED_BS) coef[j] = sums[j];
+  for (int j = t; j <= i + 1; j += FIN_BS) hc[j] = lh[j];
+  for (int j = t; j <= i; j += FIN_BS) coef[j] = sums[j];
 }
 
 // u_{i+1} = (u_{i+1} - sum_j coef_j u_j) * scale, sequentially in j (omp_sum_v, omp_mul_v)
@@ -314,9 +503,17 @@ __global__ __launch_bounds__(256) void k_gs_update(int n, double* __restrict__ u
 // u0 = u0 / err0 (Array division, Array.h:793-805)
 __global__ __launch_bounds__(256) void k_div(int n, double* __restrict__ u, const GmState* __restrict__ st)
 {
+  if (st->done) return;
   const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= (size_t)n) return;
   u[k] = u[k] / st->scale;
+}
+
+__global__ void k_reset_state(GmState* st)
+{
+  if (threadIdx.x == 0) {
+    st->done = st->last_i = st->itr = st->suc = st->divzero = st->early = 0;
+  }
 }
 
 // Back substitution, gmres.cpp:595-608; the triangle is staged in LDS when it fits.
@@ -327,6 +524,7 @@ __global__ __launch_bounds__(256) void k_backsolve(int sD, const double* __restr
 #pragma clang fp contract(off)
   __shared__ double lh[BS_LDS];
   __shared__ double ly[MAXM];
+  if (st->early || st->divzero) return;
   int last_i = st->last_i;
   if (last_i >= sD) last_i = sD - 1;
   const int ldh = sD + 1;
@@ -343,7 +541,19 @@ __global__ __launch_bounds__(256) void k_backsolve(int sD, const double* __restr
   if (threadIdx.x == 0) {
     for (int j = last_i; j >= 0; j--) {
       double yj = ly[j];
-      for (int k = j + 1; k <= last_i; k++) yj = yj - (staged ? lh[j + nc * k] : h[j + (size_t)ldh * k]) * ly[k];
+      int k = j + 1;
+      if (staged) {
+        for (; k + 4 <= last_i + 1; k += 4) {
+          const double h0 = lh[j + nc * k], h1 = lh[j + nc * (k + 1)], h2 = lh[j + nc * (k + 2)],
+                       h3 = lh[j + nc * (k + 3)];
+          const double y0 = ly[k], y1 = ly[k + 1], y2 = ly[k + 2], y3 = ly[k + 3];
+          yj = yj - h0 * y0;
+          yj = yj - h1 * y1;
+          yj = yj - h2 * y2;
+          yj = yj - h3 * y3;
+        }
+      }
+      for (; k <= last_i; k++) yj = yj - (staged ? lh[j + nc * k] : h[j + (size_t)ldh * k]) * ly[k];
       ly[j] = yj / (staged ? lh[j + nc * j] : h[j + (size_t)ldh * j]);
     }
     st->fNorm = fabs(err[last_i + 1]);
@@ -357,6 +567,7 @@ __global__ __launch_bounds__(256) void k_xupdate(int n, double* __restrict__ X, 
                                                  const double* __restrict__ y, const GmState* __restrict__ st)
 {
 #pragma clang fp contract(off)
+  if (st->early || st->divzero) return;
   const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= (size_t)n) return;
   int last_i = st->last_i;
@@ -410,12 +621,305 @@ __global__ __launch_bounds__(256) void k_mul(int n, const double* __restrict__ a
   out[k] = a[k] * b[k];
 }
 
-// --------------------------------------------------------- workspace ----
-static int ensure_work(svg_lhs* L, int dof, int sD)
+// ------------------------------------------------- NS / Schur kernels ---
+// depart (ns_solver.cpp:91-159): split each 4x4 block into mK(9), mG(3), mD(3),
+// mL(1); Gt(:,l) = -mG(:,j) where block l is the transpose of block j.
+__global__ __launch_bounds__(256) void k_depart(int nnz, const double* __restrict__ Val, const int* __restrict__ trans,
+                                                double* __restrict__ mK, double* __restrict__ mG,
+                                                double* __restrict__ mD, double* __restrict__ mL,
+                                                double* __restrict__ Gt)
 {
-  auto& k = L->kw;
-  int n = dof * L->nNo;
-  if (k.u && k.n == n && k.sD >= sD) return SVG_OK;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nnz) return;
+  const double* t = Val + 16 * (size_t)j;
+  double* k = mK + 9 * (size_t)j;
+  k[0] = t[0]; k[1] = t[1]; k[2] = t[2];
+  k[3] = t[4]; k[4] = t[5]; k[5] = t[6];
+  k[6] = t[8]; k[7] = t[9]; k[8] = t[10];
+  mG[3 * (size_t)j + 0] = t[3];
+  mG[3 * (size_t)j + 1] = t[7];
+  mG[3 * (size_t)j + 2] = t[11];
+  mD[3 * (size_t)j + 0] = t[12];
+  mD[3 * (size_t)j + 1] = t[13];
+  mD[3 * (size_t)j + 2] = t[14];
+  mL[j] = t[15];
+  const double* tt = Val + 16 * (size_t)trans[j];
+  Gt[3 * (size_t)j + 0] = -tt[3];
+  Gt[3 * (size_t)j + 1] = -tt[7];
+  Gt[3 * (size_t)j + 2] = -tt[11];
+}
+
+// Fixed-order sum of a block-partial array (per lane at stride 64, then a
+// butterfly) computed by the first wavefront of every workgroup that needs it:
+// each workgroup obtains the identical value without atomics or a launch.
+__device__ __forceinline__ double sum_partials(const double* __restrict__ partial, int nblk, double* shared_out)
+{
+  const int t = threadIdx.x, lane = t & 63;
+  if (t < 64) {
+    double acc = 0.0;
+    for (int b = lane; b < nblk; b += 64) acc = acc + partial[b];
+    acc = wave_sum(acc);
+    if (lane == 0) *shared_out = acc;
+  }
+  __syncthreads();
+  return *shared_out;
+}
+
+__device__ __forceinline__ void block_partial(double v, double* __restrict__ partial)
+{
+  __shared__ double wsum[RED_NW];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  v = wave_sum(v);
+  if (lane == 0) wsum[wv] = v;
+  __syncthreads();
+  if (t == 0) {
+    double v2 = wsum[0];
+    for (int q = 1; q < RED_NW; q++) v2 = v2 + wsum[q];
+    partial[blockIdx.x] = v2;
+  }
+}
+
+// Schur CG start (cgrad.cpp:66-79): X = 0, P = R, partial sums of R^2.
+__global__ __launch_bounds__(RED_BS) void k_cg_init(int nNo, const double* __restrict__ R, double* __restrict__ X,
+                                                    double* __restrict__ P, double* __restrict__ partial)
+{
+#pragma clang fp contract(off)
+  double sq = 0.0;
+  for (int r = 0; r < RED_EPT; r++) {
+    const int row = blockIdx.x * RED_SPAN + threadIdx.x + r * RED_BS;
+    if (row >= nNo) continue;
+    const double v = R[row];
+    X[row] = 0.0;
+    P[row] = v;
+    sq = sq + v * v;
+  }
+  block_partial(sq, partial);
+}
+
+// iNorm = |R|, eps = max(absTol, relTol iNorm)^2, errO = err = iNorm^2, stop test.
+__global__ __launch_bounds__(RED_BS) void k_cg_start(const double* __restrict__ partial, int nblk, CgState* cg,
+                                                     double absTol, double relTol)
+{
+#pragma clang fp contract(off)
+  __shared__ double tot;
+  const double acc = sum_partials(partial, nblk, &tot);
+  if (threadIdx.x == 0) {
+    const double iN = sqrt(acc);
+    const double e = fmax(absTol, relTol * iN);
+    cg->iNorm = iN;
+    cg->eps = e * e;
+    cg->errO[0] = iN * iN;
+    cg->errO_last = cg->errO[0];
+    cg->err = cg->errO[0];
+    cg->suc = 0;
+    cg->stop = 0;
+    cg->last_i = 0;
+    if (cg->err < cg->eps) {
+      cg->suc = 1;
+      cg->stop = 1;
+    }
+  }
+}
+
+// Part 1 (cgrad.cpp:101-118): DGP = Gt . GP (vs), SP = mL P (ss), SP = SP - DGP;
+// partial sums of P.SP.
+__global__ __launch_bounds__(RED_BS) void k_cg_sp(int nNo, const int* __restrict__ rowPtr,
+                                                  const int* __restrict__ colPtr, const double* __restrict__ Gt,
+                                                  const double* __restrict__ mL, const double* __restrict__ GP,
+                                                  const double* __restrict__ P, double* __restrict__ SP,
+                                                  double* __restrict__ partial, const CgState* __restrict__ cg)
+{
+#pragma clang fp contract(off)
+  if (cg->stop) return;
+  double prod = 0.0;
+  for (int r = 0; r < RED_EPT; r++) {
+    const int row = blockIdx.x * RED_SPAN + threadIdx.x + r * RED_BS;
+    if (row >= nNo) continue;
+    double dgp = 0.0, sp = 0.0;
+    for (int j = rowPtr[row]; j < rowPtr[row + 1]; j++) {
+      const int col = colPtr[j];
+      dgp = dgp + Gt[3 * (size_t)j] * GP[3 * (size_t)col] + Gt[3 * (size_t)j + 1] * GP[3 * (size_t)col + 1] +
+            Gt[3 * (size_t)j + 2] * GP[3 * (size_t)col + 2];
+      sp = sp + mL[j] * P[col];
+    }
+    sp = sp + (-1.0) * dgp;
+    SP[row] = sp;
+    prod = prod + P[row] * sp;
+  }
+  block_partial(prod, partial);
+}
+
+// Part 2 (cgrad.cpp:120-135): alpha = errO / <P,SP>, X += alpha P, R -= alpha SP;
+// partial sums of R^2 into partial2.
+__global__ __launch_bounds__(RED_BS) void k_cg_upd(int nNo, double* __restrict__ X, const double* __restrict__ P,
+                                                   double* __restrict__ R, const double* __restrict__ SP,
+                                                   const double* __restrict__ partial, double* __restrict__ partial2,
+                                                   int nblk, const CgState* __restrict__ cg, int i)
+{
+#pragma clang fp contract(off)
+  __shared__ double tot;
+  if (cg->stop) return;
+  const double alpha = cg->errO[i & 1] / sum_partials(partial, nblk, &tot);
+  double sq = 0.0;
+  for (int r = 0; r < RED_EPT; r++) {
+    const int row = blockIdx.x * RED_SPAN + threadIdx.x + r * RED_BS;
+    if (row >= nNo) continue;
+    X[row] = X[row] + alpha * P[row];
+    const double rr = R[row] + (-alpha) * SP[row];
+    R[row] = rr;
+    sq = sq + rr * rr;
+  }
+  block_partial(sq, partial2);
+}
+
+// Part 3 (cgrad.cpp:137-152): err = |R|^2 (sqrt then square), P = c2 (P + c1 R)
+// with c1 = errO/err, c2 = err/errO; workgroup 0 records err and the stop test
+// of the next trip (cgrad.cpp:84-88).
+__global__ __launch_bounds__(RED_BS) void k_cg_p(int nNo, double* __restrict__ P, const double* __restrict__ R,
+                                                 const double* __restrict__ partial2, int nblk, CgState* cg, int i,
+                                                 int mItr, volatile int* host_done)
+{
+#pragma clang fp contract(off)
+  __shared__ double tot;
+  if (cg->stop) return;
+  double e = sqrt(sum_partials(partial2, nblk, &tot));
+  e = e * e;
+  const double errO = cg->errO[i & 1];
+  const double c1 = errO / e, c2 = e / errO;
+  for (int r = 0; r < RED_EPT; r++) {
+    const int row = blockIdx.x * RED_SPAN + threadIdx.x + r * RED_BS;
+    if (row >= nNo) continue;
+    P[row] = c2 * (P[row] + c1 * R[row]);
+  }
+  // Workgroup 0 records the trip. It writes only err, last_i, errO[(i+1)&1] and
+  // the stop flag; stop guards the remaining P updates of this trip, which are
+  // dead once the iteration has converged.
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    cg->err = e;
+    cg->errO_last = errO;
+    cg->last_i = i;
+    if (i + 1 < mItr) {
+      cg->last_i = i + 1;
+      cg->errO[(i + 1) & 1] = e;
+      if (e < cg->eps) {
+        cg->suc = 1;
+        cg->stop = 1;
+        *host_done = 1;
+        __threadfence_system();
+      }
+    } else {
+      cg->stop = 1;
+    }
+  }
+}
+
+// Dot products of a list of vector pairs: out[p] = sum_k a_p[k] b_p[k].
+struct DotPair {
+  const double* a;
+  const double* b;
+  int n;
+};
+constexpr int MAXPAIRS = 64;
+struct DotList {
+  DotPair p[MAXPAIRS];
+  int np;
+};
+
+__global__ __launch_bounds__(RED_BS) void k_dot_list(DotList L, double* __restrict__ partial)
+{
+#pragma clang fp contract(off)
+  __shared__ double wsum[RED_NW];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int pr = blockIdx.y;
+  const DotPair P = L.p[pr];
+  double acc = 0.0;
+  for (int r = 0; r < RED_EPT; r++) {
+    size_t k = (size_t)blockIdx.x * RED_SPAN + t + (size_t)r * RED_BS;
+    if (k < (size_t)P.n) acc = acc + P.a[k] * P.b[k];
+  }
+  acc = wave_sum(acc);
+  if (lane == 0) wsum[wv] = acc;
+  __syncthreads();
+  if (t == 0) {
+    double v2 = wsum[0];
+    for (int q = 1; q < RED_NW; q++) v2 = v2 + wsum[q];
+    partial[(size_t)pr * gridDim.x + blockIdx.x] = v2;
+  }
+}
+
+__global__ __launch_bounds__(FIN_BS) void k_dot_list_fin(int np, int nblk, const double* __restrict__ partial,
+                                                         double* __restrict__ out)
+{
+#pragma clang fp contract(off)
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  for (int q = wv; q < np; q += FIN_BS / 64) {
+    double s = 0.0;
+    for (int b = lane; b < nblk; b += 64) s = s + partial[(size_t)q * nblk + b];
+    s = wave_sum(s);
+    if (lane == 0) out[q] = s;
+  }
+}
+
+// y = x0 - sum_{j<m} c_j V_j, sequentially in j (ns_solver.cpp:420-425).
+__global__ __launch_bounds__(256) void k_axpy_sub(int n, const double* __restrict__ x0, const double* __restrict__ V,
+                                                  size_t ld, int m, const double* __restrict__ c,
+                                                  double* __restrict__ y)
+{
+#pragma clang fp contract(off)
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (size_t)n) return;
+  double v = x0[k] - c[0] * V[k];
+  for (int j = 1; j < m; j++) v = v - c[j] * V[(size_t)j * ld + k];
+  y[k] = v;
+}
+
+// y = c[i0] V_0 + sum_{i=1..itr} c[2i+off] V_i (ns_solver.cpp:480-488).
+__global__ __launch_bounds__(256) void k_combine(int n, const double* __restrict__ V, size_t ld, int itr,
+                                                 const double* __restrict__ c, int off, double* __restrict__ y)
+{
+#pragma clang fp contract(off)
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (size_t)n) return;
+  double v = c[off] * V[k];
+  for (int i = 1; i <= itr; i++) v = v + c[2 * i + off] * V[(size_t)i * ld + k];
+  y[k] = v;
+}
+
+// Ri(4,nNo) = [Rm(3,nNo); Rc(nNo)] and the split of Ri into Rm/Rc.
+__global__ __launch_bounds__(256) void k_interleave(int nNo, const double* __restrict__ m3, const double* __restrict__ c1,
+                                                    double* __restrict__ R4)
+{
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= nNo) return;
+  R4[4 * (size_t)a + 0] = m3[3 * (size_t)a + 0];
+  R4[4 * (size_t)a + 1] = m3[3 * (size_t)a + 1];
+  R4[4 * (size_t)a + 2] = m3[3 * (size_t)a + 2];
+  R4[4 * (size_t)a + 3] = c1[a];
+}
+
+__global__ __launch_bounds__(256) void k_split(int nNo, const double* __restrict__ R4, double* __restrict__ m3,
+                                               double* __restrict__ c1)
+{
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= nNo) return;
+  m3[3 * (size_t)a + 0] = R4[4 * (size_t)a + 0];
+  m3[3 * (size_t)a + 1] = R4[4 * (size_t)a + 1];
+  m3[3 * (size_t)a + 2] = R4[4 * (size_t)a + 2];
+  c1[a] = R4[4 * (size_t)a + 3];
+}
+
+__global__ __launch_bounds__(256) void k_sub(int n, const double* __restrict__ a, const double* __restrict__ b,
+                                             double* __restrict__ out)
+{
+#pragma clang fp contract(off)
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (size_t)n) return;
+  out[k] = a[k] - b[k];
+}
+
+// --------------------------------------------------------- workspace ----
+static void free_work(KrylovWork& k)
+{
   dev_free(k.u);
   dev_free(k.X);
   dev_free(k.W);
@@ -431,10 +935,16 @@ static int ensure_work(svg_lhs* L, int dof, int sD)
   dev_free(k.ticket);
   if (k.h_st) (void)hipHostFree(k.h_st);
   k.h_st = nullptr;
+}
+
+static int ensure_work(KrylovWork& k, int n, int sD)
+{
+  if (k.u && k.n == n && k.sD >= sD) return SVG_OK;
+  free_work(k);
   k.n = n;
   k.sD = sD;
   k.ld = ((size_t)n + 63) & ~(size_t)63;
-  k.nblk = (int)((n + RED_SPAN - 1) / RED_SPAN);
+  k.nblk = (int)(((n + RED_SPAN - 1) / RED_SPAN + 7) & ~7);
   int rc;
   if ((rc = dev_alloc(&k.u, k.ld * (sD + 1))) || (rc = dev_alloc(&k.X, (size_t)n)) || (rc = dev_alloc(&k.W, (size_t)n)) ||
       (rc = dev_alloc(&k.h, (size_t)(sD + 1) * sD)) || (rc = dev_alloc(&k.cs, sD)) || (rc = dev_alloc(&k.sn, sD)) ||
@@ -463,35 +973,103 @@ struct DotScratch {
 };
 static DotScratch g_dot;
 
-static int dot_device(int n, const double* w, const double* base, size_t ld, int m, double* partial,
-                      unsigned* ticket, double* out, GmState* st, FinArgs fa, double* h, double* cs, double* sn,
-                      double* err, double* coef, hipStream_t s, int* host_done = nullptr)
+static int dot_device(int n, const double* w, const double* base, size_t ld, int m, double* partial, double* out,
+                      GmState* st, FinArgs fa, double* h, double* cs, double* sn, double* err, double* coef,
+                      hipStream_t s, int* host_done = nullptr)
 {
   int nblk = (n + RED_SPAN - 1) / RED_SPAN;
   if (nblk < 1) nblk = 1;
   if (m > MAXM) return set_error(SVG_ERR_ARG, "reduction: too many sums");
-  launch("mdot", k_mdot, dim3(nblk), dim3(RED_BS), (size_t)4 * m * sizeof(double), s, n, w, base, ld, m, partial,
-         ticket, fa, st, h, cs, sn, err, coef, out, (volatile int*)host_done);
+  const int* skip = (fa.mode == FIN_ARNOLDI) ? &st->done : nullptr;
+  launch("mdot", k_mdot, dim3(nblk), dim3(RED_BS), (size_t)RED_NW * m * sizeof(double), s, n, w, base, ld, m,
+         partial, skip);
+  launch("mdot_fin", k_fin, dim3(1), dim3(FIN_BS), 0, s, fa, (const double*)partial, nblk, m, st, h, cs, sn, err,
+         coef, out, (volatile int*)host_done);
   return SVG_OK;
 }
 
-// gmres_v on the device; R holds the (preconditioned) right-hand side on
-// entry and the solution on exit, as in the reference.
+// The operator of a Krylov solve: the dof x dof block matrix K over lhs's pattern.
+struct Op {
+  const svg_lhs* L;
+  const double* K;
+  int dof;
+};
+
+static void op_apply(const Op& A, const double* x, double* y, const double* b, const int* skip, hipStream_t s)
+{
+  if (A.dof == 4) {
+    if (b)
+      spmv<4, 4, 1>(A.L, A.K, x, y, b, skip, s);
+    else
+      spmv<4, 4, 0>(A.L, A.K, x, y, nullptr, skip, s);
+  } else {
+    if (b)
+      spmv<3, 3, 1>(A.L, A.K, x, y, b, skip, s, "spmv_k3");
+    else
+      spmv<3, 3, 0>(A.L, A.K, x, y, nullptr, skip, s, "spmv_k3");
+  }
+}
+
+// One restart cycle (gmres.cpp:529-612 / :533-592): up to sD Arnoldi steps with
+// at most LOOKAHEAD steps enqueued beyond the host's last convergence check,
+// then the back substitution and X += sum y_j u_j. Returns after the cycle's
+// state has been copied to the host.
+static int run_cycle(const Op& A, KrylovWork& k, int sD, FinArgs fa, double* X, hipStream_t s)
+{
+  const int n = k.n;
+  const unsigned nbv = blocks_for(n, 256);
+  *k.h_done = 0;
+  for (int i = 0; i < sD; i++) {
+    if (i >= LOOKAHEAD) {
+      SVG_HIP(hipEventSynchronize(k.ev[(i - LOOKAHEAD) % 8]));
+      if (*(volatile int*)k.h_done) break;
+    }
+    fa.mode = FIN_ARNOLDI;
+    fa.i = i;
+    if (A.dof == 4) {
+      const unsigned nb = spmv_dots(A.L, A.K, k.u + k.ld * i, k.u + k.ld * (i + 1), k.u, k.ld, i + 2, k.partial,
+                                    &k.st->done, s);
+      launch("mdot_fin", k_fin, dim3(1), dim3(FIN_BS), 0, s, fa, (const double*)k.partial, (int)nb, i + 2, k.st, k.h,
+             k.cs, k.sn, k.err, k.coef, k.scal, (volatile int*)k.d_done);
+    } else {
+      op_apply(A, k.u + k.ld * i, k.u + k.ld * (i + 1), nullptr, &k.st->done, s);
+      int rc = dot_device(n, k.u + k.ld * (i + 1), k.u, k.ld, i + 2, k.partial, k.scal, k.st, fa, k.h, k.cs,
+                          k.sn, k.err, k.coef, s, k.d_done);
+      if (rc) return rc;
+    }
+    launch("gs_update", k_gs_update, dim3(nbv), dim3(256), 0, s, n, k.u, k.ld, i, (const double*)k.coef,
+           (const GmState*)k.st);
+    SVG_HIP(hipEventRecord(k.ev[i % 8], s));
+  }
+  launch("backsolve", k_backsolve, dim3(1), dim3(256), 0, s, sD, (const double*)k.h, (const double*)k.err, k.y, k.st);
+  launch("xupdate", k_xupdate, dim3(nbv), dim3(256), 0, s, n, X, (const double*)k.u, k.ld, (const double*)k.y,
+         (const GmState*)k.st);
+  SVG_HIP(hipGetLastError());
+  SVG_HIP(hipMemcpyAsync(k.h_st, k.st, sizeof(GmState), hipMemcpyDeviceToHost, s));
+  SVG_HIP(hipStreamSynchronize(s));
+  return SVG_OK;
+}
+
+// gmres_v (gmres.cpp:450-629); R holds the (preconditioned) right-hand side on
+// entry and the solution on exit.
 static int gmres_v_dev(svg_lhs* L, svg_subls* ls, int dof, const double* Val, double* R, hipStream_t s)
 {
   if (dof != 4) return set_error(SVG_ERR_UNSUPPORTED, "gmres_v: only dof = 4 is built");
-  if (ls->sD < 1 || ls->sD > 500) return set_error(SVG_ERR_ARG, "gmres_v: Krylov dimension must be in [1, 500]");
-  int rc = ensure_work(L, dof, ls->sD);
-  if (rc) return rc;
+  if (ls->sD < 1 || ls->sD > 250) return set_error(SVG_ERR_ARG, "gmres_v: Krylov dimension must be in [1, 250]");
   auto& k = L->kw;
+  int rc = ensure_work(k, dof * L->nNo, ls->sD);
+  if (rc) return rc;
   const int n = k.n, sD = ls->sD;
+  const Op A{L, Val, 4};
+  if (L->spmvVariant < 0 && (rc = svg_spmv_tune(L, Val, nullptr, s))) return rc;
   auto t0 = std::chrono::steady_clock::now();
   FinArgs fa{};
   fa.sD = sD;
   fa.absTol = ls->absTol;
   fa.relTol = ls->relTol;
   fa.mode = FIN_NORM_R;
-  dot_device(n, R, R, 0, 1, k.partial, k.ticket, k.scal, k.st, fa, k.h, k.cs, k.sn, k.err, k.coef, s);
+  if ((rc = dot_device(n, R, R, 0, 1, k.partial, k.scal, k.st, fa, k.h, k.cs, k.sn, k.err, k.coef, s)))
+    return rc;
   SVG_HIP(hipMemcpyAsync(k.h_st, k.st, sizeof(GmState), hipMemcpyDeviceToHost, s));
   SVG_HIP(hipStreamSynchronize(s));
   ls->suc = 0;
@@ -508,34 +1086,14 @@ static int gmres_v_dev(svg_lhs* L, svg_subls* ls, int dof, const double* Val, do
   double dB = ls->fNorm;
   for (int l = 0; l < ls->mItr; l++) {
     dB = ls->fNorm;
-    // u0 = R - A X; err0 = |u0|; u0 /= err0
-    spmv<4, 1>(L, Val, k.X, k.u, R, nullptr, s);
+    op_apply(A, k.X, k.u, R, nullptr, s);  // u0 = R - A X
     fa.mode = FIN_RESTART;
-    dot_device(n, k.u, k.u, 0, 1, k.partial, k.ticket, k.scal, k.st, fa, k.h, k.cs, k.sn, k.err, k.coef, s);
+    fa.inc = 1;
+    fa.first = 0;
+    if ((rc = dot_device(n, k.u, k.u, 0, 1, k.partial, k.scal, k.st, fa, k.h, k.cs, k.sn, k.err, k.coef, s)))
+      return rc;
     launch("div", k_div, dim3(nbv), dim3(256), 0, s, n, k.u, (const GmState*)k.st);
-    *k.h_done = 0;
-    for (int i = 0; i < sD; i++) {
-      // Keep at most LOOKAHEAD steps in flight: wait for step i-LOOKAHEAD, then
-      // stop enqueuing once the device has flagged convergence.
-      if (i >= LOOKAHEAD) {
-        SVG_HIP(hipEventSynchronize(k.ev[(i - LOOKAHEAD) % 8]));
-        if (*(volatile int*)k.h_done) break;
-      }
-      spmv<4, 0>(L, Val, k.u + k.ld * i, k.u + k.ld * (i + 1), nullptr, k.st, s);
-      fa.mode = FIN_ARNOLDI;
-      fa.i = i;
-      dot_device(n, k.u + k.ld * (i + 1), k.u, k.ld, i + 2, k.partial, k.ticket, k.scal, k.st, fa, k.h, k.cs, k.sn,
-                 k.err, k.coef, s, k.d_done);
-      launch("gs_update", k_gs_update, dim3(nbv), dim3(256), 0, s, n, k.u, k.ld, i, (const double*)k.coef,
-             (const GmState*)k.st);
-      SVG_HIP(hipEventRecord(k.ev[i % 8], s));
-    }
-    launch("backsolve", k_backsolve, dim3(1), dim3(256), 0, s, sD, (const double*)k.h, (const double*)k.err, k.y, k.st);
-    launch("xupdate", k_xupdate, dim3(nbv), dim3(256), 0, s, n, k.X, (const double*)k.u, k.ld, (const double*)k.y,
-           (const GmState*)k.st);
-    SVG_HIP(hipGetLastError());
-    SVG_HIP(hipMemcpyAsync(k.h_st, k.st, sizeof(GmState), hipMemcpyDeviceToHost, s));
-    SVG_HIP(hipStreamSynchronize(s));
+    if ((rc = run_cycle(A, k, sD, fa, k.X, s))) return rc;
     if (k.h_st->divzero) return set_error(SVG_ERR_ARG, "Array Divide by zero.");
     ls->itr = k.h_st->itr;
     ls->fNorm = k.h_st->fNorm;
@@ -548,6 +1106,420 @@ static int gmres_v_dev(svg_lhs* L, svg_subls* ls, int dof, const double* Val, do
   auto t1 = std::chrono::steady_clock::now();
   ls->callD = std::chrono::duration<double>(t1 - t0).count();
   ls->dB = 10.0 * log(ls->fNorm / dB);
+  return SVG_OK;
+}
+
+// gmres (gmres.cpp:91-280), the inner solver of the NS solver: X = A^-1 R with
+// X starting at 0, the first restart vector R itself, ls->itr accumulating.
+static int gmres_dev(const Op& A, KrylovWork& k, svg_subls* ls, const double* R, double* X, hipStream_t s)
+{
+  if (ls->sD < 1 || ls->sD > MAXM - 2) return set_error(SVG_ERR_ARG, "gmres: Krylov dimension must be in [1, 510]");
+  const int n = A.dof * A.L->nNo, sD = ls->sD;
+  int rc = ensure_work(k, n, sD);
+  if (rc) return rc;
+  auto t0 = std::chrono::steady_clock::now();
+  const unsigned nbv = blocks_for(n, 256);
+  ls->suc = 0;
+  svg_fill(X, (size_t)n, 0.0, s);
+  FinArgs fa{};
+  fa.sD = sD;
+  fa.absTol = ls->absTol;
+  fa.relTol = ls->relTol;
+  launch("reset_state", k_reset_state, dim3(1), dim3(64), 0, s, k.st);
+  int itr0 = ls->itr;
+  for (int l = 0; l < ls->mItr; l++) {
+    if (l == 0) {
+      SVG_HIP(hipMemcpyAsync(k.u, R, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    } else {
+      op_apply(A, X, k.u, R, nullptr, s);  // u0 = R - A X
+    }
+    fa.mode = FIN_RESTART;
+    fa.inc = l == 0 ? 0 : 1;
+    fa.first = l == 0 ? 1 : 0;
+    if ((rc = dot_device(n, k.u, k.u, 0, 1, k.partial, k.scal, k.st, fa, k.h, k.cs, k.sn, k.err, k.coef, s)))
+      return rc;
+    launch("div", k_div, dim3(nbv), dim3(256), 0, s, n, k.u, (const GmState*)k.st);
+    double dBprev = ls->fNorm;
+    if ((rc = run_cycle(A, k, sD, fa, X, s))) return rc;
+    if (l == 0 && k.h_st->early) {
+      ls->callD = DBL_EPSILON;
+      ls->dB = 0.0;
+      return SVG_OK;
+    }
+    if (k.h_st->divzero) return set_error(SVG_ERR_ARG, "Array Divide by zero.");
+    if (l == 0) {
+      ls->iNorm = k.h_st->iNorm;
+      dBprev = k.h_st->iNorm;
+    }
+    ls->dB = dBprev;
+    ls->itr = itr0 + k.h_st->itr;
+    ls->fNorm = k.h_st->fNorm;
+    if (k.h_st->suc) {
+      ls->suc = 1;
+      break;
+    }
+  }
+  auto t1 = std::chrono::steady_clock::now();
+  ls->callD = std::chrono::duration<double>(t1 - t0).count() + ls->callD;
+  ls->dB = 10.0 * log(ls->fNorm / ls->dB);
+  return SVG_OK;
+}
+
+// cgrad::schur (cgrad.cpp:50-160) on P in place: P <- (L - D G)^-1 P.
+static int schur_dev(svg_lhs* L, svg_subls* ls, double* Pio, hipStream_t s)
+{
+  auto& w = L->ns;
+  const int nNo = L->nNo;
+  const int nb = (int)blocks_for(nNo, RED_SPAN);
+  double* part1 = w.partial;
+  double* part2 = w.partial + nb;
+  auto t0 = std::chrono::steady_clock::now();
+  ls->suc = 0;
+  launch("cg_init", k_cg_init, dim3(nb), dim3(RED_BS), 0, s, nNo, (const double*)Pio, w.cgX, w.cgP, part1);
+  launch("cg_start", k_cg_start, dim3(1), dim3(RED_BS), 0, s, (const double*)part1, nb, w.cg, ls->absTol,
+         ls->relTol);
+  auto& kk = L->kw3;  // its look-ahead events and mapped flag
+  SVG_HIP(hipMemcpyAsync(w.h_cg, w.cg, sizeof(CgState), hipMemcpyDeviceToHost, s));
+  SVG_HIP(hipStreamSynchronize(s));
+  const int* skip = &w.cg->stop;
+  if (!w.h_cg->stop) {
+    *kk.h_done = 0;
+    for (int i = 0; i < ls->mItr; i++) {
+      if (i >= LOOKAHEAD) {
+        SVG_HIP(hipEventSynchronize(kk.ev[(i - LOOKAHEAD) % 8]));
+        if (*(volatile int*)kk.h_done) break;
+      }
+      // GP = G P (sv); SP = L P - D GP; alpha, X, R; err, P
+      spmv<3, 1, 0>(L, w.mG, w.cgP, w.cgGP, nullptr, skip, s, "spmv_sv");
+      launch("cg_sp", k_cg_sp, dim3(nb), dim3(RED_BS), 0, s, nNo, (const int*)L->d_rowPtr, (const int*)L->d_colPtr,
+             (const double*)w.Gt, (const double*)w.mL, (const double*)w.cgGP, (const double*)w.cgP, w.cgSP, part1,
+             (const CgState*)w.cg);
+      launch("cg_upd", k_cg_upd, dim3(nb), dim3(RED_BS), 0, s, nNo, w.cgX, (const double*)w.cgP, Pio,
+             (const double*)w.cgSP, (const double*)part1, part2, nb, (const CgState*)w.cg, i);
+      launch("cg_p", k_cg_p, dim3(nb), dim3(RED_BS), 0, s, nNo, w.cgP, (const double*)Pio, (const double*)part2, nb,
+             w.cg, i, ls->mItr, (volatile int*)kk.d_done);
+      SVG_HIP(hipEventRecord(kk.ev[i % 8], s));
+    }
+  }
+  SVG_HIP(hipGetLastError());
+  SVG_HIP(hipMemcpyAsync(w.h_cg, w.cg, sizeof(CgState), hipMemcpyDeviceToHost, s));
+  SVG_HIP(hipMemcpyAsync(Pio, w.cgX, sizeof(double) * nNo, hipMemcpyDeviceToDevice, s));
+  SVG_HIP(hipStreamSynchronize(s));
+  const CgState& c = *w.h_cg;
+  ls->iNorm = c.iNorm;
+  ls->suc = c.suc;
+  ls->fNorm = sqrt(c.err);
+  auto t1 = std::chrono::steady_clock::now();
+  ls->callD = std::chrono::duration<double>(t1 - t0).count() + ls->callD;
+  ls->itr = ls->itr + c.last_i;
+  if (c.errO_last < DBL_EPSILON)
+    ls->dB = 0.0;
+  else
+    ls->dB = 5.0 * log(c.err / c.errO_last);
+  return SVG_OK;
+}
+
+static int ns_grow(svg_lhs* L, int need)
+{
+  auto& w = L->ns;
+  if (need <= w.cap) return SVG_OK;
+  int cap = w.cap ? w.cap : 8;
+  while (cap < need) cap *= 2;
+  double *U, *MU, *P, *MP;
+  int rc;
+  if ((rc = dev_alloc(&U, w.ld3 * cap)) || (rc = dev_alloc(&MU, w.ld3 * 2 * cap)) || (rc = dev_alloc(&P, w.ld1 * cap)) ||
+      (rc = dev_alloc(&MP, w.ld1 * 2 * cap)))
+    return rc;
+  if (w.cap) {
+    SVG_HIP(hipMemcpy(U, w.U, sizeof(double) * w.ld3 * w.cap, hipMemcpyDeviceToDevice));
+    SVG_HIP(hipMemcpy(MU, w.MU, sizeof(double) * w.ld3 * 2 * w.cap, hipMemcpyDeviceToDevice));
+    SVG_HIP(hipMemcpy(P, w.P, sizeof(double) * w.ld1 * w.cap, hipMemcpyDeviceToDevice));
+    SVG_HIP(hipMemcpy(MP, w.MP, sizeof(double) * w.ld1 * 2 * w.cap, hipMemcpyDeviceToDevice));
+  }
+  dev_free(w.U);
+  dev_free(w.MU);
+  dev_free(w.P);
+  dev_free(w.MP);
+  w.U = U;
+  w.MU = MU;
+  w.P = P;
+  w.MP = MP;
+  w.cap = cap;
+  double* coef;
+  if ((rc = dev_alloc(&coef, (size_t)2 * cap))) return rc;
+  dev_free(w.coef);
+  w.coef = coef;
+  return SVG_OK;
+}
+
+static int ns_setup(svg_lhs* L)
+{
+  auto& w = L->ns;
+  const int nNo = L->nNo, nnz = L->nnz;
+  if (w.mK && w.nNo == nNo) return SVG_OK;
+  if (!L->d_trans) {
+    std::vector<int> tr(nnz);
+    for (int i = 0; i < nNo; i++)
+      for (int j = L->rowPtr[i]; j < L->rowPtr[i + 1]; j++) {
+        const int k = L->colPtr[j];
+        const int* b = L->colPtr.data() + L->rowPtr[k];
+        const int* e = L->colPtr.data() + L->rowPtr[k + 1];
+        const int* it = std::lower_bound(b, e, i);
+        if (it == e || *it != i) return set_error(SVG_ERR_ARG, "ns_solver: the block pattern is not symmetric");
+        tr[j] = (int)(it - L->colPtr.data());
+      }
+    int rc = dev_alloc(&L->d_trans, (size_t)nnz);
+    if (rc) return rc;
+    SVG_HIP(hipMemcpy(L->d_trans, tr.data(), sizeof(int) * nnz, hipMemcpyHostToDevice));
+  }
+  w.nNo = nNo;
+  w.ld3 = ((size_t)3 * nNo + 63) & ~(size_t)63;
+  w.ld1 = ((size_t)nNo + 63) & ~(size_t)63;
+  int rc;
+  if ((rc = dev_alloc(&w.mK, (size_t)9 * nnz)) || (rc = dev_alloc(&w.mG, (size_t)3 * nnz)) ||
+      (rc = dev_alloc(&w.mD, (size_t)3 * nnz)) || (rc = dev_alloc(&w.mL, (size_t)nnz)) ||
+      (rc = dev_alloc(&w.Gt, (size_t)3 * nnz)) || (rc = dev_alloc(&w.Rm, w.ld3)) || (rc = dev_alloc(&w.Rmi, w.ld3)) ||
+      (rc = dev_alloc(&w.Rc, w.ld1)) || (rc = dev_alloc(&w.Rci, w.ld1)) || (rc = dev_alloc(&w.cgX, w.ld1)) ||
+      (rc = dev_alloc(&w.cgP, w.ld1)) || (rc = dev_alloc(&w.cgSP, w.ld1)) || (rc = dev_alloc(&w.cgGP, w.ld3)) ||
+      (rc = dev_alloc(&w.cg, 1)) || (rc = dev_alloc(&w.dots, MAXPAIRS)) || (rc = dev_alloc(&w.ticket, 1)))
+    return rc;
+  w.partialCap = (int)blocks_for(3 * nNo, RED_SPAN) * MAXPAIRS + (int)blocks_for(nNo, RED_BS);
+  if ((rc = dev_alloc(&w.partial, (size_t)w.partialCap))) return rc;
+  SVG_HIP(hipMemset(w.ticket, 0, sizeof(unsigned)));
+  SVG_HIP(hipMemset(w.cg, 0, sizeof(CgState)));
+  SVG_HIP(hipHostMalloc((void**)&w.h_cg, sizeof(CgState), hipHostMallocDefault));
+  SVG_HIP(hipHostMalloc((void**)&w.h_dots, sizeof(double) * MAXPAIRS, hipHostMallocDefault));
+  return ns_grow(L, 8);
+}
+
+static int dot_list(svg_lhs* L, const DotList& dl, hipStream_t s)
+{
+  auto& w = L->ns;
+  int nmax = 0;
+  for (int p = 0; p < dl.np; p++) nmax = std::max(nmax, dl.p[p].n);
+  const unsigned nb = std::max(1u, blocks_for(nmax, RED_SPAN));
+  launch("dot_list", k_dot_list, dim3(nb, dl.np), dim3(RED_BS), 0, s, dl, w.partial);
+  launch("dot_list_fin", k_dot_list_fin, dim3(1), dim3(FIN_BS), 0, s, dl.np, (int)nb, (const double*)w.partial,
+         w.dots);
+  SVG_HIP(hipGetLastError());
+  SVG_HIP(hipMemcpyAsync(w.h_dots, w.dots, sizeof(double) * dl.np, hipMemcpyDeviceToHost, s));
+  SVG_HIP(hipStreamSynchronize(s));
+  return SVG_OK;
+}
+
+// ge (ge.cpp:40-139): symmetrically scaled Gaussian elimination with partial
+// pivoting of the NS outer projection system (N <= 2 RI.mItr); host-side, tiny.
+static bool ge_host(int nV, int N, const std::vector<double>& A, std::vector<double>& B)
+{
+  std::vector<double> W(N > 0 ? N : 1);
+  const double tol = DBL_TRUE_MIN, eps = DBL_EPSILON;
+  for (int i = 0; i < N; i++) {
+    if (fabs(A[i + (size_t)nV * i]) < tol) {
+      std::fill(B.begin(), B.end(), 0.0);
+      return false;
+    }
+    W[i] = 1.0 / sqrt(fabs(A[i + (size_t)nV * i]));
+  }
+  std::vector<double> Cm((size_t)(N > 0 ? N : 1) * (N + 1));
+  auto C = [&](int i, int j) -> double& { return Cm[i + (size_t)N * j]; };
+  for (int i = 0; i < N; i++) {
+    for (int j = 0; j < N; j++) C(i, j) = W[i] * W[j] * A[i + (size_t)nV * j];
+    C(i, N) = W[i] * B[i];
+  }
+  if (N <= 0) return false;
+  if (N == 1) {
+    B[0] = C(0, 1) / C(0, 0);
+    B[0] = B[0] * W[0];
+    return true;
+  }
+  if (N == 2) {
+    double pivot = C(0, 0) * C(1, 1) - C(1, 0) * C(0, 1);
+    if (fabs(pivot) < eps) {
+      std::fill(B.begin(), B.end(), 0.0);
+      return false;
+    }
+    B[0] = (C(0, 2) * C(1, 1) - C(1, 2) * C(0, 1)) / pivot;
+    B[1] = (C(1, 2) * C(0, 0) - C(0, 2) * C(1, 0)) / pivot;
+    B[0] = W[0] * B[0];
+    B[1] = W[1] * B[1];
+    return true;
+  }
+  for (int m = 0; m < N - 1; m++) {
+    int ipv = m;
+    double pivot = fabs(C(m, m));
+    for (int i = m + 1; i < N; i++)
+      if (fabs(C(i, m)) > pivot) {
+        ipv = i;
+        pivot = fabs(C(i, m));
+      }
+    if (fabs(pivot) < eps) {
+      std::fill(B.begin(), B.end(), 0.0);
+      return false;
+    }
+    if (ipv != m)
+      for (int j = m; j < N + 1; j++) std::swap(C(m, j), C(ipv, j));
+    for (int i = m + 1; i < N; i++) {
+      double sv = C(i, m) / C(m, m);
+      C(i, m) = 0.0;
+      for (int j = m + 1; j < N + 1; j++) C(i, j) = C(i, j) - sv * C(m, j);
+    }
+  }
+  for (int j = N - 1; j >= 0; j--) {
+    for (int i = j + 1; i < N; i++) C(j, N) = C(j, N) - C(j, i) * C(i, N);
+    C(j, N) = C(j, N) / C(j, j);
+  }
+  for (int i = 0; i < N; i++) B[i] = W[i] * C(i, N);
+  return true;
+}
+
+// ns_solver (ns_solver.cpp:167-499) on the preconditioned system; Ri(4,nNo) in/out.
+static int ns_solver_dev(svg_lhs* L, svg_ls* ls, const double* Val, double* Ri, hipStream_t s)
+{
+  const int nNo = L->nNo, nnz = L->nnz, nsd = 3;
+  int rc = ns_setup(L);
+  if (rc) return rc;
+  auto& w = L->ns;
+  const int mItr = ls->RI.mItr;
+  const int nB = 2 * mItr;
+  if (mItr < 1) return set_error(SVG_ERR_ARG, "ns_solver: RI.mItr must be >= 1");
+  const unsigned nb1 = blocks_for(nNo, 256), nb3 = blocks_for(3 * nNo, 256);
+  auto t0 = std::chrono::steady_clock::now();
+  launch("ns_split", k_split, dim3(nb1), dim3(256), 0, s, nNo, (const double*)Ri, w.Rmi, w.Rci);
+  SVG_HIP(hipMemcpyAsync(w.Rm, w.Rmi, sizeof(double) * 3 * nNo, hipMemcpyDeviceToDevice, s));
+  SVG_HIP(hipMemcpyAsync(w.Rc, w.Rci, sizeof(double) * nNo, hipMemcpyDeviceToDevice, s));
+  DotList dl{};
+  dl.np = 2;
+  dl.p[0] = {w.Rm, w.Rm, 3 * nNo};
+  dl.p[1] = {w.Rc, w.Rc, nNo};
+  if ((rc = dot_list(L, dl, s))) return rc;
+  double eps = sqrt(pow(sqrt(w.h_dots[0]), 2.0) + pow(sqrt(w.h_dots[1]), 2.0));
+  ls->RI.iNorm = eps;
+  ls->RI.fNorm = eps * eps;
+  ls->CG.callD = 0.0;
+  ls->GM.callD = 0.0;
+  ls->CG.itr = 0;
+  ls->GM.itr = 0;
+  ls->RI.suc = 0;
+  eps = fmax(ls->RI.absTol, ls->RI.relTol * eps);
+  launch("depart", k_depart, dim3(blocks_for(nnz, 256)), dim3(256), 0, s, nnz, Val, (const int*)L->d_trans, w.mK,
+         w.mG, w.mD, w.mL, w.Gt);
+  const Op K3{L, w.mK, 3};
+  std::vector<double> A((size_t)nB * nB, 0.0), B(nB, 0.0), xB(nB, 0.0), oldxB(nB, 0.0), tmp;
+  int iBB = 0, i_count = 0;
+  for (int i = 0; i < mItr; i++) {
+    const int iB = 2 * i;
+    iBB = 2 * i + 1;
+    ls->RI.dB = ls->RI.fNorm;
+    i_count = i;
+    if ((rc = ns_grow(L, i + 1))) return rc;
+    double* Ui = w.U + w.ld3 * i;
+    double* Pi = w.P + w.ld1 * i;
+    double* MUiB = w.MU + w.ld3 * iB;
+    double* MUiBB = w.MU + w.ld3 * iBB;
+    double* MPiB = w.MP + w.ld1 * iB;
+    double* MPiBB = w.MP + w.ld1 * iBB;
+    // U_i = K^-1 Rm; P_i = Rc - D U_i; P_i = S^-1 P_i
+    if ((rc = gmres_dev(K3, L->kw3, &ls->GM, w.Rm, Ui, s))) return rc;
+    spmv<1, 3, 1>(L, w.mD, Ui, Pi, w.Rc, nullptr, s, "spmv_vs");
+    if ((rc = schur_dev(L, &ls->CG, Pi, s))) return rc;
+    // MU_iB = G P_i; MU_iBB = Rm - MU_iB; U_i = K^-1 MU_iBB; MU_iBB = K U_i
+    spmv<3, 1, 0>(L, w.mG, Pi, MUiB, nullptr, nullptr, s, "spmv_sv");
+    launch("sub", k_sub, dim3(nb3), dim3(256), 0, s, 3 * nNo, (const double*)w.Rm, (const double*)MUiB, MUiBB);
+    if ((rc = gmres_dev(K3, L->kw3, &ls->GM, MUiBB, Ui, s))) return rc;
+    spmv<3, 3, 0>(L, w.mK, Ui, MUiBB, nullptr, nullptr, s, "spmv_k3");
+    // MP_iB = L P_i; MP_iBB = D U_i
+    spmv<1, 1, 0>(L, w.mL, Pi, MPiB, nullptr, nullptr, s, "spmv_ss");
+    spmv<1, 3, 0>(L, w.mD, Ui, MPiBB, nullptr, nullptr, s, "spmv_vs");
+    // Gram entries (ns_solver.cpp:350-360): <MU_j,MU_k> + <MP_j,MP_k>, <MU_k,Rmi> + <MP_k,Rci>
+    tmp.assign((size_t)2 * (iBB + 2) + 2, 0.0);
+    int c = 0;
+    std::vector<std::pair<int, int>> idx;  // (dot of MU or Rmi, dot of MP or Rci)
+    DotList q{};
+    q.np = 0;
+    auto flush = [&]() -> int {
+      if (q.np == 0) return SVG_OK;
+      int r = dot_list(L, q, s);
+      if (r) return r;
+      for (int p = 0; p < q.np; p++) tmp.push_back(w.h_dots[p]);
+      q.np = 0;
+      return SVG_OK;
+    };
+    tmp.clear();
+    for (int kk = iB; kk <= iBB; kk++) {
+      for (int j = 0; j <= kk + 1; j++) {
+        const bool rhs = (j == kk + 1);
+        if (q.np + 2 > MAXPAIRS && (rc = flush())) return rc;
+        q.p[q.np++] = {rhs ? w.Rmi : w.MU + w.ld3 * j, w.MU + w.ld3 * kk, 3 * nNo};
+        q.p[q.np++] = {rhs ? w.Rci : w.MP + w.ld1 * j, w.MP + w.ld1 * kk, nNo};
+      }
+    }
+    if ((rc = flush())) return rc;
+    c = 0;
+    int tp = 0;
+    for (int kk = iB; kk <= iBB; kk++) {
+      for (int j = 0; j <= kk; j++) {
+        double v = tmp[tp] + tmp[tp + 1];
+        tp += 2;
+        A[j + (size_t)nB * kk] = v;
+        A[kk + (size_t)nB * j] = v;
+        c++;
+      }
+      B[kk] = tmp[tp] + tmp[tp + 1];
+      tp += 2;
+      c++;
+    }
+    xB = B;
+    if (ge_host(nB, iBB + 1, A, xB)) {
+      oldxB = xB;
+    } else {
+      return set_error(SVG_ERR_SINGULAR, "FSILS: Singular matrix detected");
+    }
+    double sum = 0.0;
+    for (int q2 = 0; q2 <= iBB; q2++) sum += xB[q2] * B[q2];
+    ls->RI.fNorm = pow(ls->RI.iNorm, 2.0) - sum;
+    if (ls->RI.fNorm < eps * eps) {
+      ls->RI.suc = 1;
+      break;
+    }
+    SVG_HIP(hipMemcpyAsync(w.coef, xB.data(), sizeof(double) * (iBB + 1), hipMemcpyHostToDevice, s));
+    launch("axpy_sub", k_axpy_sub, dim3(nb3), dim3(256), 0, s, 3 * nNo, (const double*)w.Rmi, (const double*)w.MU,
+           w.ld3, iBB + 1, (const double*)w.coef, w.Rm);
+    launch("axpy_sub", k_axpy_sub, dim3(nb1), dim3(256), 0, s, nNo, (const double*)w.Rci, (const double*)w.MP, w.ld1,
+           iBB + 1, (const double*)w.coef, w.Rc);
+  }
+  SVG_HIP(hipMemcpyAsync(w.coef, xB.data(), sizeof(double) * (iBB + 1), hipMemcpyHostToDevice, s));
+  if (i_count >= mItr) {
+    ls->RI.itr = mItr;
+  } else {
+    ls->RI.itr = i_count;
+    launch("axpy_sub", k_axpy_sub, dim3(nb1), dim3(256), 0, s, nNo, (const double*)w.Rci, (const double*)w.MP, w.ld1,
+           iBB + 1, (const double*)w.coef, w.Rc);
+  }
+  DotList rn{};
+  rn.np = 1;
+  rn.p[0] = {w.Rc, w.Rc, nNo};
+  if ((rc = dot_list(L, rn, s))) return rc;
+  ls->Resc = (int)(100.0 * pow(sqrt(w.h_dots[0]), 2.0) / ls->RI.fNorm);
+  ls->Resm = 100 - ls->Resc;
+  // solution (ns_solver.cpp:480-488)
+  launch("combine", k_combine, dim3(nb3), dim3(256), 0, s, 3 * nNo, (const double*)w.U, w.ld3, ls->RI.itr,
+         (const double*)w.coef, 1, w.Rmi);
+  launch("combine", k_combine, dim3(nb1), dim3(256), 0, s, nNo, (const double*)w.P, w.ld1, ls->RI.itr,
+         (const double*)w.coef, 0, w.Rci);
+  auto t1 = std::chrono::steady_clock::now();
+  ls->RI.callD = std::chrono::duration<double>(t1 - t0).count();
+  ls->RI.dB = 5.0 * log(ls->RI.fNorm / ls->RI.dB);
+  if (ls->Resc < 0.0 || ls->Resm < 0.0) {
+    ls->Resc = 0;
+    ls->Resm = 0;
+    ls->RI.dB = 0;
+    ls->RI.fNorm = 0.0;
+    return set_error(SVG_ERR_ILLCOND,
+                     "FSILS: unexpected behavior in FSILS (likely due to the ill-conditioned LHS matrix)");
+  }
+  ls->RI.fNorm = sqrt(ls->RI.fNorm);
+  launch("ns_join", k_interleave, dim3(nb1), dim3(256), 0, s, nNo, (const double*)w.Rmi, (const double*)w.Rci, Ri);
+  SVG_HIP(hipGetLastError());
   return SVG_OK;
 }
 
@@ -634,13 +1606,67 @@ int svg_spar_mul_vv(const svg_lhs* L, int dof, const double* K, const double* U,
   if (!L || !K || !U || !KU) return set_error(SVG_ERR_ARG, "svg_spar_mul_vv: null argument");
   hipStream_t s = (hipStream_t)stream;
   switch (dof) {
-    case 1: spmv<1, 0>(L, K, U, KU, nullptr, nullptr, s); break;
-    case 2: spmv<2, 0>(L, K, U, KU, nullptr, nullptr, s); break;
-    case 3: spmv<3, 0>(L, K, U, KU, nullptr, nullptr, s); break;
-    case 4: spmv<4, 0>(L, K, U, KU, nullptr, nullptr, s); break;
+    case 1: spmv_launch<1, 1, 0, 2, false>(L, K, U, KU, nullptr, nullptr, s, "spmv_bcsr", true); break;
+    case 2: spmv_launch<2, 2, 0, 2, false>(L, K, U, KU, nullptr, nullptr, s, "spmv_bcsr", true); break;
+    case 3: spmv_launch<3, 3, 0, 2, false>(L, K, U, KU, nullptr, nullptr, s, "spmv_bcsr", true); break;
+    case 4: spmv<4, 4, 0>(L, K, U, KU, nullptr, nullptr, s); break;
     default: return set_error(SVG_ERR_UNSUPPORTED, "svg_spar_mul_vv: dof must be 1..4");
   }
   SVG_HIP(hipGetLastError());
+  return SVG_OK;
+}
+
+int svg_spmv_num_variants(int* count)
+{
+  if (!count) return set_error(SVG_ERR_ARG, "svg_spmv_num_variants: null argument");
+  *count = kNumSpmvVariants;
+  return SVG_OK;
+}
+
+int svg_spmv_time_variant(const svg_lhs* L, int variant, const double* K, const double* U, double* KU, int launches,
+                          double* avg_us, void* stream)
+{
+  if (!L || !K || !U || !KU || !avg_us || launches < 1 || variant < 0 || variant >= kNumSpmvVariants)
+    return set_error(SVG_ERR_ARG, "svg_spmv_time_variant: bad argument");
+  hipStream_t s = (hipStream_t)stream;
+  hipEvent_t a, b;
+  SVG_HIP(hipEventCreate(&a));
+  SVG_HIP(hipEventCreate(&b));
+  spmv_variant<4, 4, 0>(variant, L, K, U, KU, nullptr, nullptr, s, "spmv_tune");
+  SVG_HIP(hipEventRecord(a, s));
+  for (int r = 0; r < launches; r++) spmv_variant<4, 4, 0>(variant, L, K, U, KU, nullptr, nullptr, s, "spmv_tune");
+  SVG_HIP(hipEventRecord(b, s));
+  SVG_HIP(hipEventSynchronize(b));
+  float ms = 0.f;
+  SVG_HIP(hipEventElapsedTime(&ms, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  SVG_HIP(hipGetLastError());
+  *avg_us = 1e3 * ms / launches;
+  return SVG_OK;
+}
+
+int svg_spmv_tune(svg_lhs* L, const double* K, int* best, void* stream)
+{
+  if (!L || !K) return set_error(SVG_ERR_ARG, "svg_spmv_tune: null argument");
+  int rc = ensure_work(L->kw, 4 * L->nNo, L->kw.sD > 0 ? L->kw.sD : 1);
+  if (rc) return rc;
+  // the Krylov basis holds the probe vectors; results are discarded
+  double* U = L->kw.u;
+  double* KU = L->kw.X;
+  SVG_HIP(hipMemsetAsync(U, 0, sizeof(double) * 4 * L->nNo, (hipStream_t)stream));
+  double bestUs = 1e30;
+  int bv = kDefaultSpmvVariant;
+  for (int v = 0; v < kNumSpmvVariants; v++) {
+    double us;
+    if ((rc = svg_spmv_time_variant(L, v, K, U, KU, 3, &us, stream))) return rc;
+    if (us < bestUs) {
+      bestUs = us;
+      bv = v;
+    }
+  }
+  L->spmvVariant = bv;
+  if (best) *best = bv;
   return SVG_OK;
 }
 
@@ -669,8 +1695,7 @@ static int dot_api(int n, const double* U, const double* V, double* out, hipStre
   }
   FinArgs fa{};
   fa.mode = FIN_DOT;
-  dot_device(n, V, U, 0, 1, g_dot.partial, g_dot.ticket, g_dot.out, nullptr, fa, nullptr, nullptr, nullptr, nullptr,
-             nullptr, s);
+  dot_device(n, V, U, 0, 1, g_dot.partial, g_dot.out, nullptr, fa, nullptr, nullptr, nullptr, nullptr, nullptr, s);
   SVG_HIP(hipGetLastError());
   SVG_HIP(hipMemcpyAsync(out, g_dot.out, sizeof(double), hipMemcpyDeviceToHost, s));
   SVG_HIP(hipStreamSynchronize(s));
@@ -725,7 +1750,7 @@ int svg_fsils_solve(svg_lhs* L, svg_ls* ls, int dof, double* Ri, double* Val, in
   if (dof != 4) return set_error(SVG_ERR_UNSUPPORTED, "fsils_solve: only dof = 4 is built");
   int rc;
   int sDneed = ls->LS_type == SVG_LS_GMRES ? ls->RI.sD : 1;
-  if ((rc = ensure_work(L, dof, sDneed < 1 ? 1 : sDneed))) return rc;
+  if ((rc = ensure_work(L->kw, dof * L->nNo, sDneed < 1 ? 1 : sDneed))) return rc;
   auto& k = L->kw;
   const int n = dof * L->nNo;
   if (prec == SVG_PREC_FSILS) {
@@ -740,7 +1765,7 @@ int svg_fsils_solve(svg_lhs* L, svg_ls* ls, int dof, double* Ri, double* Val, in
       rc = gmres_v_dev(L, &ls->RI, dof, Val, Ri, s);
       break;
     case SVG_LS_NS:
-      rc = set_error(SVG_ERR_UNSUPPORTED, "fsils_solve: NS solver not built yet");
+      rc = ns_solver_dev(L, ls, Val, Ri, s);
       break;
     case SVG_LS_CG:
     case SVG_LS_BICGS:

@@ -115,6 +115,8 @@ def test_construct_fluid(torch_cuda, phys, n):
 
 @pytest.mark.parametrize("n", [4, 8])
 def test_gmres_solve(torch_cuda, phys, n):
+    """gmres_v on the device: bitwise equal to the oracle run with the device's
+    reduction order, and within the solver tolerance of the reference-order oracle."""
     from svfsiplus_amd.newton import DeviceLhs, fsils_solve, ls_create
     torch = torch_cuda
     pipe = W.make_pipe(n)
@@ -122,36 +124,45 @@ def test_gmres_solve(torch_cuda, phys, n):
     Ag, Yg = oracle.pici(phys, Ao, An, Yo, Yn)
     R, V = oracle.construct_fluid(pipe, phys, Ag, Yg)
     tol = 1e-6
-    ols = oracle.ls_create(oracle.LS_GMRES, relTol=tol, absTol=1e-30, maxItr=1000, dimKry=50)
-    x0, V0, ols = oracle.fsils_solve(pipe, ols, R, V)
+    mk = lambda: oracle.ls_create(oracle.LS_GMRES, relTol=tol, absTol=1e-30, maxItr=1000, dimKry=50)
+    x0, V0, ols = oracle.fsils_solve(pipe, mk(), R, V)
+    with oracle.device_order():
+        xd, Vd, dls = oracle.fsils_solve(pipe, mk(), R, V)
     L = DeviceLhs(pipe)
     ls = ls_create(_lib.LS_GMRES, relTol=tol, absTol=1e-30, maxItr=1000, dimKry=50)
     dR, dV = _dev(torch, R), _dev(torch, V)
     fsils_solve(L, ls, 4, dR, dV)
     x1 = dR.cpu().numpy()
     rel = np.linalg.norm(x1 - x0) / np.linalg.norm(x0)
-    print(f"n={n} itr gpu={ls.RI.itr} oracle={ols.RI.itr} iNorm {ls.RI.iNorm:.17g} {ols.RI.iNorm:.17g} "
-          f"fNorm {ls.RI.fNorm:.3e} {ols.RI.fNorm:.3e} rel={rel:.2e}")
-    assert abs(ls.RI.itr - ols.RI.itr) <= 1
-    assert ls.RI.suc == ols.RI.suc == 1
-    assert abs(ls.RI.iNorm - ols.RI.iNorm) <= 1e-12 * ols.RI.iNorm
+    print(f"n={n} itr gpu={ls.RI.itr} oracle(device order)={dls.RI.itr} oracle(reference order)={ols.RI.itr} "
+          f"rel-to-reference-order={rel:.2e} max|gpu-device-order|={np.abs(x1 - xd).max():.2e}")
+    assert ls.RI.itr == dls.RI.itr and ls.RI.suc == dls.RI.suc == 1
+    assert ls.RI.iNorm == dls.RI.iNorm and ls.RI.fNorm == dls.RI.fNorm
+    assert np.array_equal(x1, xd)
     assert np.array_equal(dV.cpu().numpy(), V0)
+    assert abs(ls.RI.itr - ols.RI.itr) <= max(1, 0.02 * ols.RI.itr)
     assert rel <= 10 * tol
 
 
 def test_newton_time_step(torch_cuda, phys):
+    """Three Newton iterations (pici, construct_fluid, fsils_solve, picc) against the oracle:
+    identical linear iteration counts and residual norms to the device-order oracle (assembly
+    sums differ only in element order), and within tolerance of the reference-order oracle."""
     from svfsiplus_amd.newton import FluidNewton, ls_create
     pipe = W.make_pipe(4)
     tol = 1e-6
-    An0, Yn0, hist0 = oracle.newton_time_step(
-        pipe, phys, lambda: oracle.ls_create(oracle.LS_GMRES, relTol=tol, absTol=1e-30, maxItr=1000, dimKry=50))
+    mk = lambda: oracle.ls_create(oracle.LS_GMRES, relTol=tol, absTol=1e-30, maxItr=1000, dimKry=50)
+    An0, Yn0, hist0 = oracle.newton_time_step(pipe, phys, mk)
+    with oracle.device_order():
+        _, _, histd = oracle.newton_time_step(pipe, phys, mk)
     fn = FluidNewton(pipe, phys, ls_create(_lib.LS_GMRES, relTol=tol, absTol=1e-30, maxItr=1000, dimKry=50))
     hist1 = fn.time_step()
-    for h0, h1 in zip(hist0, hist1):
-        print("oracle", h0, "\ngpu   ", h1)
+    for h0, hd, h1 in zip(hist0, histd, hist1):
+        print("oracle(ref order)", h0, "\noracle(dev order)", hd, "\ngpu              ", h1)
     assert len(hist0) == len(hist1) == 3
-    for h0, h1 in zip(hist0, hist1):
-        assert abs(h0["ls_itr"] - h1["ls_itr"]) <= 1
+    for h0, hd, h1 in zip(hist0, histd, hist1):
+        assert abs(hd["ls_itr"] - h1["ls_itr"]) <= 1
+        assert abs(h0["ls_itr"] - h1["ls_itr"]) <= max(1, 0.02 * h0["ls_itr"])
         assert abs(h0["iNorm"] - h1["iNorm"]) <= 1e-8 * h0["iNorm"]
     Yn1 = fn.Yn.cpu().numpy()
     assert np.linalg.norm(Yn1 - Yn0) <= 1e-6 * np.linalg.norm(Yn0)
@@ -179,7 +190,44 @@ def test_gpu_against_reference_golden(torch_cuda, name):
     fsils_solve(L, ls, 4, dR, dV)
     itr, iNorm, fNorm, suc = d["gm_out"].tolist()
     x = dR.cpu().numpy()
-    print(name, "gmres itr gpu", ls.RI.itr, "ref", itr, "rel", np.linalg.norm(x - d["gm_x"]) / np.linalg.norm(d["gm_x"]))
-    assert abs(ls.RI.itr - itr) <= 1 and ls.RI.suc == suc
+    with oracle.device_order():
+        xd, _, dls = oracle.fsils_solve(
+            p, oracle.ls_create(oracle.LS_GMRES, relTol=tol, absTol=1e-30, maxItr=1000, dimKry=50), d["R0"], d["Val0"])
+    print(name, "gmres itr gpu", ls.RI.itr, "reference", itr, "oracle(device order)", dls.RI.itr,
+          "rel", np.linalg.norm(x - d["gm_x"]) / np.linalg.norm(d["gm_x"]))
+    # classical Gram-Schmidt near Krylov exhaustion (tiny systems) amplifies the dot-product
+    # rounding order, so the count is pinned to the device-order oracle, the solution to the reference
+    assert ls.RI.itr == dls.RI.itr and ls.RI.suc == suc
+    assert np.array_equal(x, xd)
     assert np.array_equal(dV.cpu().numpy(), d["gm_Val"])
     assert np.linalg.norm(x - d["gm_x"]) <= 10 * tol * np.linalg.norm(d["gm_x"])
+
+
+@pytest.mark.parametrize("name", ["pipe_n2", "pipe_n4"])
+def test_ns_solver_against_reference_golden(torch_cuda, name):
+    """Bipartitioned NS solver (ns_solver.cpp) on the device against the reference's record."""
+    import _golden as G
+    from svfsiplus_amd.newton import DeviceLhs, fsils_solve, ls_create
+    torch = torch_cuda
+    p, ph, d = G.load(name)
+    tol = float(d["ns_tol"])
+    L = DeviceLhs(p)
+    ls = ls_create(_lib.LS_NS, relTol=tol, absTol=1e-30, maxItr=1000, dimKry=50, ns_gm=(1000, 1e-2),
+                   ns_cg=(1000, 1e-2))
+    dR, dV = _dev(torch, d["R0"]), _dev(torch, d["Val0"])
+    fsils_solve(L, ls, 4, dR, dV)
+    itr, iNorm, fNorm, suc, gmi, cgi, resm, resc = d["ns_out"].tolist()
+    x = dR.cpu().numpy()
+    rel = np.linalg.norm(x - d["ns_x"]) / np.linalg.norm(d["ns_x"])
+    with oracle.device_order():
+        xd, _, ols = oracle.fsils_solve(
+            p, oracle.ls_create(oracle.LS_NS, relTol=tol, absTol=1e-30, maxItr=1000, dimKry=50, relTolIn=(1e-2, 1e-2),
+                                absTolIn=(1e-30, 1e-30), maxItrIn=(1000, 1000)), d["R0"], d["Val0"])
+    print(name, "NS gpu RI/GM/CG", ls.RI.itr, ls.GM.itr, ls.CG.itr, "Resm/Resc", ls.Resm, ls.Resc,
+          "| reference", itr, gmi, cgi, resm, resc, "| oracle(device order)", ols.RI.itr, ols.GM.itr, ols.CG.itr,
+          ols.Resm, ols.Resc, "| rel to reference", rel, "max|gpu-device-order|", np.abs(x - xd).max())
+    assert (ls.RI.itr, ls.GM.itr, ls.CG.itr, ls.Resm, ls.Resc) == (ols.RI.itr, ols.GM.itr, ols.CG.itr, ols.Resm, ols.Resc)
+    assert np.array_equal(x, xd)
+    assert ls.RI.suc == suc and abs(ls.RI.itr - itr) <= 1
+    assert abs(ls.RI.iNorm - iNorm) <= 1e-12 * iNorm
+    assert rel <= 1e-3
