@@ -150,6 +150,22 @@ int svg_ls_create(svg_ls* ls, int LS_type);
 int svg_fsils_solve(svg_lhs* lhs, svg_ls* ls, int dof, double* d_Ri, double* d_Val, int prec, int nIncL,
                     const int* incL, const double* res, void* stream);
 
+/* Solve options beyond the reference (no counterpart in fsils_solve):
+ *   precision SVG_PRECISION_MIXED (LS type GMRES): Arnoldi cycles use an FP32
+ *   copy of the scaled Val and FP32 vector copies for the block SpMV; the
+ *   Gram-Schmidt dots, Givens rotations, least squares and the solution stay
+ *   FP64, and every restart recomputes the FP64 true residual R - A X, which
+ *   both refines the solution and decides convergence (|R - A X| < eps). */
+#define SVG_PRECISION_FP64 0
+#define SVG_PRECISION_MIXED 1
+typedef struct {
+  int precision;
+  int reserved[7];
+} svg_solve_opts;
+
+int svg_fsils_solve_ex(svg_lhs* lhs, svg_ls* ls, int dof, double* d_Ri, double* d_Val, int prec, int nIncL,
+                       const int* incL, const double* res, const svg_solve_opts* opts, void* stream);
+
 /* ------------------------------------------------------------------------ */
 /* Kernels of the solve, exposed for unit parity tests.                      */
 /* fsils_spar_mul_vv (spar_mul.cpp:191-258), dof in {1..4}.                  */
@@ -186,6 +202,9 @@ int svg_fill(double* d, size_t n, double v, void* stream);
 int svg_prof_enable(int on);
 int svg_prof_reset(void);
 int svg_prof_get(const char* name, int* count, double* total_ms);
+/* Summed algorithmic bytes of the named kernel's timed launches (where the
+ * library states them: the block SpMV kernels). */
+int svg_prof_get_bytes(const char* name, double* bytes);
 
 #ifdef __cplusplus
 }

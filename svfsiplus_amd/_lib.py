@@ -50,6 +50,13 @@ class SubLs(C.Structure):
                 ("iNorm", C.c_double), ("fNorm", C.c_double), ("dB", C.c_double), ("callD", C.c_double)]
 
 
+class SolveOpts(C.Structure):
+    _fields_ = [("precision", C.c_int), ("reserved", C.c_int * 7)]
+
+
+PRECISION_FP64, PRECISION_MIXED = 0, 1
+
+
 class Ls(C.Structure):
     _fields_ = [("LS_type", C.c_int), ("Resm", C.c_int), ("Resc", C.c_int), ("GM", SubLs), ("CG", SubLs),
                 ("RI", SubLs)]
@@ -77,6 +84,7 @@ _SIGS = {
     "svg_construct_fluid": (_I, [_P, C.POINTER(FluidParams), _P, _P, _P, _P, _P, _P]),
     "svg_ls_create": (_I, [C.POINTER(Ls), _I]),
     "svg_fsils_solve": (_I, [_P, C.POINTER(Ls), _I, _P, _P, _I, _I, _P, _P, _P]),
+    "svg_fsils_solve_ex": (_I, [_P, C.POINTER(Ls), _I, _P, _P, _I, _I, _P, _P, C.POINTER(SolveOpts), _P]),
     "svg_spar_mul_vv": (_I, [_P, _I, _P, _P, _P, _P]),
     "svg_precond_diag": (_I, [_P, _I, _P, _P, _P, _P]),
     "svg_spmv_num_variants": (_I, [_IP]),
@@ -90,6 +98,7 @@ _SIGS = {
     "svg_prof_enable": (_I, [_I]),
     "svg_prof_reset": (_I, []),
     "svg_prof_get": (_I, [C.c_char_p, _IP, _DP]),
+    "svg_prof_get_bytes": (_I, [C.c_char_p, _DP]),
 }
 
 EXPORTED = tuple(_SIGS)

@@ -20,14 +20,24 @@ int hip_check(hipError_t e, const char* what);
     if (_e != hipSuccess) return ::svg::hip_check(_e, #call); \
   } while (0)
 
-// Event timing of named kernels on the launching stream.
-int prof_begin(const char* name, hipStream_t s);
+// Event timing of named kernels on the launching stream, with the
+// algorithmic bytes of each launch (for achieved-bandwidth reporting).
+int prof_begin(const char* name, hipStream_t s, double bytes = 0.0);
 void prof_end(int slot, hipStream_t s);
 
 template <typename K, typename... Args>
 inline void launch(const char* name, K kernel, dim3 grid, dim3 block, size_t shmem, hipStream_t s, Args... args)
 {
   int slot = prof_begin(name, s);
+  hipLaunchKernelGGL(kernel, grid, block, shmem, s, args...);
+  prof_end(slot, s);
+}
+
+template <typename K, typename... Args>
+inline void launch_b(const char* name, double bytes, K kernel, dim3 grid, dim3 block, size_t shmem, hipStream_t s,
+                     Args... args)
+{
+  int slot = prof_begin(name, s, bytes);
   hipLaunchKernelGGL(kernel, grid, block, shmem, s, args...);
   prof_end(slot, s);
 }
@@ -60,7 +70,7 @@ struct GmState {
   int sD;
   int divzero;  // err(0) == 0 at a restart (Array division by zero)
   int early;    // inner GMRES: |R| <= absTol on entry, X stays 0 (gmres.cpp:146-150)
-  int pad;
+  int tconv;    // mixed precision: the FP64 true residual at a restart is below eps
   double eps;
   double scale;  // err(0) after a restart, 1 / h(i+1,i) after an Arnoldi step
   double iNorm, fNorm;
@@ -100,6 +110,7 @@ struct KrylovWork {
   int* h_done = nullptr;      // mapped host flag written by the device tail
   int* d_done = nullptr;      // device view of h_done
   hipEvent_t ev[8] = {};      // per-step completion events (look-ahead ring)
+  float* u32 = nullptr;       // mixed precision: FP32 copy of the current Arnoldi vector
   double* scal = nullptr;     // scratch scalars for standalone dots
   unsigned* ticket = nullptr; // last-block ticket of the fused reductions
 };
@@ -144,6 +155,7 @@ struct svg_lhs {
   std::vector<int> maskKey;  // incFlag pattern the mask was built for
   bool maskBuilt = false;
   int spmvVariant = -1;  // 4x4 SpMV launch variant picked by the autotuner (-1: not tuned)
+  float* d_val32 = nullptr;  // mixed precision: FP32 copy of the scaled Val
   svg::KrylovWork kw;    // gmres_v on the 4x4 system
   svg::KrylovWork kw3;   // inner GMRES of the NS solver on the 3x3 momentum block
   svg::NsWork ns;

@@ -165,8 +165,9 @@ static void spmv_launch(const svg_lhs* L, const double* K, const double* U, doub
   unsigned nb = blocks_for(nt, 256);
   const int use_xcd = (xcd && nb >= 64) ? 1 : 0;
   if (use_xcd) nb = (nb + 7) & ~7u;
-  launch(name, k_spmv<BR, BC, MODE, UNR, NT>, dim3(nb), dim3(256), 0, s, L->nNo, (const int*)L->d_rowPtr,
-         (const int*)L->d_colPtr, K, U, KU, B, skip, use_xcd);
+  const double bytes = (8.0 * BR * BC + 4.0) * L->nnz + (4.0 + 8.0 * BC + 8.0 * BR * (MODE ? 2 : 1)) * L->nNo;
+  launch_b(name, bytes, k_spmv<BR, BC, MODE, UNR, NT>, dim3(nb), dim3(256), 0, s, L->nNo, (const int*)L->d_rowPtr,
+           (const int*)L->d_colPtr, K, U, KU, B, skip, use_xcd);
 }
 
 template <int BR, int BC, int MODE>
@@ -286,7 +287,9 @@ static unsigned spmv_dots_launch(const svg_lhs* L, const double* K, const double
   unsigned nb = blocks_for((size_t)4 * L->nNo, 1024);
   const int use_xcd = (xcd && nb >= 64) ? 1 : 0;
   if (use_xcd) nb = (nb + 7) & ~7u;
-  launch("spmv_bcsr", k_spmv_dots<UNR, NT>, dim3(nb), dim3(1024), (size_t)16 * m * sizeof(double), s, L->nNo,
+  // Val + colPtr per block, rowPtr + U (read once) + W per row, m-1 basis vectors
+  const double bytes = 132.0 * L->nnz + 68.0 * L->nNo + 32.0 * L->nNo * (m - 1);
+  launch_b("spmv_bcsr", bytes, k_spmv_dots<UNR, NT>, dim3(nb), dim3(1024), (size_t)16 * m * sizeof(double), s, L->nNo,
          (const int*)L->d_rowPtr, (const int*)L->d_colPtr, K, U, W, base, ld, m, partial, skip, use_xcd);
   return nb;
 }
@@ -305,6 +308,85 @@ static unsigned spmv_dots(const svg_lhs* L, const double* K, const double* U, do
   }
 }
 
+// ------------------------------------------------------ mixed precision --
+// Arnoldi step of the mixed-precision solve: W = K32 U32 with FP32 matrix and
+// vector copies and FP32 accumulation in the reference's block order, W stored
+// in FP64, fused with the FP64 Gram-Schmidt dots as k_spmv_dots.
+template <int UNR>
+__global__ __launch_bounds__(1024) void k_spmv32_dots(int nNo, const int* __restrict__ rowPtr,
+                                                      const int* __restrict__ colPtr, const float* __restrict__ K,
+                                                      const float* __restrict__ U, double* __restrict__ W,
+                                                      const double* __restrict__ base, size_t ld, int m,
+                                                      double* __restrict__ partial, const int* __restrict__ skip,
+                                                      int xcd)
+{
+#pragma clang fp contract(off)
+  extern __shared__ double lds[];  // [16 waves][m]
+  if (skip && *skip) return;
+  int bx = blockIdx.x;
+  if (xcd) {
+    bx = xcd_block(blockIdx.x, gridDim.x);
+    if (bx < 0) return;
+  }
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const size_t k = (size_t)bx * 1024 + t;
+  const int row = (int)(k >> 2), i = (int)(k & 3);
+  double w = 0.0;
+  if (row < nNo) {
+    const int s = rowPtr[row], e = rowPtr[row + 1];
+    float acc = 0.0f;
+    int j = s;
+    for (; j + UNR <= e; j += UNR) {
+      int c[UNR];
+      float4 a[UNR], u[UNR];
+#pragma unroll
+      for (int q = 0; q < UNR; q++) c[q] = colPtr[j + q];
+#pragma unroll
+      for (int q = 0; q < UNR; q++) {
+        a[q] = reinterpret_cast<const float4*>(K + 16 * (size_t)(j + q) + 4 * i)[0];
+        u[q] = reinterpret_cast<const float4*>(U + 4 * (size_t)c[q])[0];
+      }
+#pragma unroll
+      for (int q = 0; q < UNR; q++) {
+        acc = __builtin_fmaf(a[q].x, u[q].x, acc);
+        acc = __builtin_fmaf(a[q].y, u[q].y, acc);
+        acc = __builtin_fmaf(a[q].z, u[q].z, acc);
+        acc = __builtin_fmaf(a[q].w, u[q].w, acc);
+      }
+    }
+    for (; j < e; j++) {
+      const float4 a = reinterpret_cast<const float4*>(K + 16 * (size_t)j + 4 * i)[0];
+      const float4 u = reinterpret_cast<const float4*>(U + 4 * (size_t)colPtr[j])[0];
+      acc = __builtin_fmaf(a.x, u.x, acc);
+      acc = __builtin_fmaf(a.y, u.y, acc);
+      acc = __builtin_fmaf(a.z, u.z, acc);
+      acc = __builtin_fmaf(a.w, u.w, acc);
+    }
+    w = (double)acc;
+    W[k] = w;
+  }
+  for (int jj = 0; jj < m; jj++) {
+    double v = (jj == m - 1) ? w : (row < nNo ? base[(size_t)jj * ld + k] : 0.0);
+    double prod = 0.0 + v * w;
+    prod = wave_sum(prod);
+    if (lane == 0) lds[wv * m + jj] = prod;
+  }
+  __syncthreads();
+  const int nblk = gridDim.x;
+  for (int jj = t; jj < m; jj += 1024) {
+    double v = lds[jj];
+    for (int q = 1; q < 16; q++) v = v + lds[q * m + jj];
+    partial[(size_t)jj * nblk + bx] = v;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_d2f(size_t n, const double* __restrict__ a, float* __restrict__ b)
+{
+  size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; k < n; k += stride) b[k] = (float)a[k];
+}
+
 // ------------------------------------------------------------ reductions --
 enum FinMode { FIN_DOT = 0, FIN_NORM_R = 1, FIN_RESTART = 2, FIN_ARNOLDI = 3 };
 
@@ -314,6 +396,7 @@ struct FinArgs {
   int sD;
   int inc;    // FIN_RESTART: itr increment (the first cycle of the inner GMRES adds none)
   int first;  // FIN_RESTART: first cycle of the inner GMRES sets iNorm and eps from err(0)
+  int tcheck; // FIN_RESTART: stop when the (FP64, true) restart residual is below eps
   double absTol, relTol;
 };
 
@@ -416,6 +499,7 @@ __global__ __launch_bounds__(FIN_BS) void k_fin(FinArgs A, const double* __restr
       st->done = 0;
       st->divzero = 0;
       st->early = 0;
+      st->tconv = 0;
       st->last_i = 0;
     }
     return;
@@ -437,6 +521,14 @@ __global__ __launch_bounds__(FIN_BS) void k_fin(FinArgs A, const double* __restr
         st->iNorm = e0;
         st->fNorm = e0;
         st->eps = fmax(A.absTol, A.relTol * e0);
+      }
+      st->tconv = 0;
+      if (A.tcheck && e0 < st->eps) {
+        st->tconv = 1;
+        st->suc = 1;
+        st->done = 1;
+        st->fNorm = e0;
+        return;
       }
       if (e0 == 0.0) {
         st->divzero = 1;
@@ -488,7 +580,8 @@ __global__ __launch_bounds__(FIN_BS) void k_fin(FinArgs A, const double* __restr
 
 // u_{i+1} = (u_{i+1} - sum_j coef_j u_j) * scale, sequentially in j (omp_sum_v, omp_mul_v)
 __global__ __launch_bounds__(256) void k_gs_update(int n, double* __restrict__ u, size_t ld, int i,
-                                                   const double* __restrict__ coef, const GmState* __restrict__ st)
+                                                   const double* __restrict__ coef, const GmState* __restrict__ st,
+                                                   float* __restrict__ u32)
 {
 #pragma clang fp contract(off)
   if (st->done) return;
@@ -497,22 +590,27 @@ __global__ __launch_bounds__(256) void k_gs_update(int n, double* __restrict__ u
   double* w = u + (size_t)(i + 1) * ld;
   double v = w[k];
   for (int j = 0; j <= i; j++) v = v + (-coef[j]) * u[(size_t)j * ld + k];
-  w[k] = st->scale * v;
+  v = st->scale * v;
+  w[k] = v;
+  if (u32) u32[k] = (float)v;
 }
 
 // u0 = u0 / err0 (Array division, Array.h:793-805)
-__global__ __launch_bounds__(256) void k_div(int n, double* __restrict__ u, const GmState* __restrict__ st)
+__global__ __launch_bounds__(256) void k_div(int n, double* __restrict__ u, const GmState* __restrict__ st,
+                                             float* __restrict__ u32)
 {
   if (st->done) return;
   const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= (size_t)n) return;
-  u[k] = u[k] / st->scale;
+  const double v = u[k] / st->scale;
+  u[k] = v;
+  if (u32) u32[k] = (float)v;
 }
 
 __global__ void k_reset_state(GmState* st)
 {
   if (threadIdx.x == 0) {
-    st->done = st->last_i = st->itr = st->suc = st->divzero = st->early = 0;
+    st->done = st->last_i = st->itr = st->suc = st->divzero = st->early = st->tconv = 0;
   }
 }
 
@@ -524,7 +622,7 @@ __global__ __launch_bounds__(256) void k_backsolve(int sD, const double* __restr
 #pragma clang fp contract(off)
   __shared__ double lh[BS_LDS];
   __shared__ double ly[MAXM];
-  if (st->early || st->divzero) return;
+  if (st->early || st->divzero || st->tconv) return;
   int last_i = st->last_i;
   if (last_i >= sD) last_i = sD - 1;
   const int ldh = sD + 1;
@@ -567,7 +665,7 @@ __global__ __launch_bounds__(256) void k_xupdate(int n, double* __restrict__ X, 
                                                  const double* __restrict__ y, const GmState* __restrict__ st)
 {
 #pragma clang fp contract(off)
-  if (st->early || st->divzero) return;
+  if (st->early || st->divzero || st->tconv) return;
   const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= (size_t)n) return;
   int last_i = st->last_i;
@@ -1014,8 +1112,10 @@ static void op_apply(const Op& A, const double* x, double* y, const double* b, c
 // at most LOOKAHEAD steps enqueued beyond the host's last convergence check,
 // then the back substitution and X += sum y_j u_j. Returns after the cycle's
 // state has been copied to the host.
-static int run_cycle(const Op& A, KrylovWork& k, int sD, FinArgs fa, double* X, hipStream_t s)
+static int run_cycle(const Op& A, KrylovWork& k, int sD, FinArgs fa, double* X, hipStream_t s,
+                     const float* K32 = nullptr)
 {
+  const bool mixed = K32 != nullptr;
   const int n = k.n;
   const unsigned nbv = blocks_for(n, 256);
   *k.h_done = 0;
@@ -1026,7 +1126,17 @@ static int run_cycle(const Op& A, KrylovWork& k, int sD, FinArgs fa, double* X, 
     }
     fa.mode = FIN_ARNOLDI;
     fa.i = i;
-    if (A.dof == 4) {
+    if (mixed) {
+      unsigned nb = blocks_for((size_t)4 * A.L->nNo, 1024);
+      const int use_xcd = nb >= 64 ? 1 : 0;
+      if (use_xcd) nb = (nb + 7) & ~7u;
+      const double bytes = 68.0 * A.L->nnz + 52.0 * A.L->nNo + 32.0 * A.L->nNo * (i + 1);
+      launch_b("spmv32", bytes, k_spmv32_dots<2>, dim3(nb), dim3(1024), (size_t)16 * (i + 2) * sizeof(double), s, A.L->nNo,
+             (const int*)A.L->d_rowPtr, (const int*)A.L->d_colPtr, K32, (const float*)k.u32, k.u + k.ld * (i + 1),
+             (const double*)k.u, k.ld, i + 2, k.partial, (const int*)&k.st->done, use_xcd);
+      launch("mdot_fin", k_fin, dim3(1), dim3(FIN_BS), 0, s, fa, (const double*)k.partial, (int)nb, i + 2, k.st, k.h,
+             k.cs, k.sn, k.err, k.coef, k.scal, (volatile int*)k.d_done);
+    } else if (A.dof == 4) {
       const unsigned nb = spmv_dots(A.L, A.K, k.u + k.ld * i, k.u + k.ld * (i + 1), k.u, k.ld, i + 2, k.partial,
                                     &k.st->done, s);
       launch("mdot_fin", k_fin, dim3(1), dim3(FIN_BS), 0, s, fa, (const double*)k.partial, (int)nb, i + 2, k.st, k.h,
@@ -1038,7 +1148,7 @@ static int run_cycle(const Op& A, KrylovWork& k, int sD, FinArgs fa, double* X, 
       if (rc) return rc;
     }
     launch("gs_update", k_gs_update, dim3(nbv), dim3(256), 0, s, n, k.u, k.ld, i, (const double*)k.coef,
-           (const GmState*)k.st);
+           (const GmState*)k.st, mixed ? k.u32 : (float*)nullptr);
     SVG_HIP(hipEventRecord(k.ev[i % 8], s));
   }
   launch("backsolve", k_backsolve, dim3(1), dim3(256), 0, s, sD, (const double*)k.h, (const double*)k.err, k.y, k.st);
@@ -1092,12 +1202,76 @@ static int gmres_v_dev(svg_lhs* L, svg_subls* ls, int dof, const double* Val, do
     fa.first = 0;
     if ((rc = dot_device(n, k.u, k.u, 0, 1, k.partial, k.scal, k.st, fa, k.h, k.cs, k.sn, k.err, k.coef, s)))
       return rc;
-    launch("div", k_div, dim3(nbv), dim3(256), 0, s, n, k.u, (const GmState*)k.st);
+    launch("div", k_div, dim3(nbv), dim3(256), 0, s, n, k.u, (const GmState*)k.st, (float*)nullptr);
     if ((rc = run_cycle(A, k, sD, fa, k.X, s))) return rc;
     if (k.h_st->divzero) return set_error(SVG_ERR_ARG, "Array Divide by zero.");
     ls->itr = k.h_st->itr;
     ls->fNorm = k.h_st->fNorm;
     if (k.h_st->suc) {
+      ls->suc = 1;
+      break;
+    }
+  }
+  SVG_HIP(hipMemcpyAsync(R, k.X, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+  auto t1 = std::chrono::steady_clock::now();
+  ls->callD = std::chrono::duration<double>(t1 - t0).count();
+  ls->dB = 10.0 * log(ls->fNorm / dB);
+  return SVG_OK;
+}
+
+// Mixed-precision gmres_v: Arnoldi cycles with the FP32 SpMV (FP32 copies of
+// the scaled Val and of the current Arnoldi vector), FP64 Gram-Schmidt, Givens,
+// least squares and solution update; every restart recomputes the FP64 true
+// residual u0 = R - A X (iterative refinement) and the solve stops only when
+// that true residual is below eps = max(absTol, relTol |R|).
+static int gmres_v_mixed(svg_lhs* L, svg_subls* ls, const double* Val, double* R, hipStream_t s)
+{
+  if (ls->sD < 1 || ls->sD > 250) return set_error(SVG_ERR_ARG, "gmres_v: Krylov dimension must be in [1, 250]");
+  auto& k = L->kw;
+  int rc = ensure_work(k, 4 * L->nNo, ls->sD);
+  if (rc) return rc;
+  const int n = k.n, sD = ls->sD;
+  if (!k.u32 && (rc = dev_alloc(&k.u32, (size_t)n))) return rc;
+  if (!L->d_val32 && (rc = dev_alloc(&L->d_val32, (size_t)16 * L->nnz))) return rc;
+  const Op A{L, Val, 4};
+  if (L->spmvVariant < 0 && (rc = svg_spmv_tune(L, Val, nullptr, s))) return rc;
+  auto t0 = std::chrono::steady_clock::now();
+  launch("d2f", k_d2f, dim3(8192), dim3(256), 0, s, (size_t)16 * L->nnz, Val, L->d_val32);
+  FinArgs fa{};
+  fa.sD = sD;
+  fa.absTol = ls->absTol;
+  fa.relTol = ls->relTol;
+  fa.mode = FIN_NORM_R;
+  if ((rc = dot_device(n, R, R, 0, 1, k.partial, k.scal, k.st, fa, k.h, k.cs, k.sn, k.err, k.coef, s))) return rc;
+  SVG_HIP(hipMemcpyAsync(k.h_st, k.st, sizeof(GmState), hipMemcpyDeviceToHost, s));
+  SVG_HIP(hipStreamSynchronize(s));
+  ls->suc = 0;
+  ls->iNorm = k.h_st->iNorm;
+  ls->fNorm = k.h_st->fNorm;
+  ls->itr = 0;
+  if (ls->iNorm <= ls->absTol) {
+    ls->callD = DBL_EPSILON;
+    ls->dB = 0.0;
+    return SVG_OK;
+  }
+  svg_fill(k.X, (size_t)n, 0.0, s);
+  const unsigned nbv = blocks_for(n, 256);
+  double dB = ls->fNorm;
+  for (int l = 0; l < ls->mItr; l++) {
+    dB = ls->fNorm;
+    op_apply(A, k.X, k.u, R, nullptr, s);  // FP64 true residual
+    fa.mode = FIN_RESTART;
+    fa.inc = 1;
+    fa.first = 0;
+    fa.tcheck = 1;
+    if ((rc = dot_device(n, k.u, k.u, 0, 1, k.partial, k.scal, k.st, fa, k.h, k.cs, k.sn, k.err, k.coef, s)))
+      return rc;
+    launch("div", k_div, dim3(nbv), dim3(256), 0, s, n, k.u, (const GmState*)k.st, k.u32);
+    if ((rc = run_cycle(A, k, sD, fa, k.X, s, L->d_val32))) return rc;
+    if (k.h_st->divzero) return set_error(SVG_ERR_ARG, "Array Divide by zero.");
+    ls->itr = k.h_st->itr;
+    ls->fNorm = k.h_st->fNorm;
+    if (k.h_st->tconv) {
       ls->suc = 1;
       break;
     }
@@ -1138,7 +1312,7 @@ static int gmres_dev(const Op& A, KrylovWork& k, svg_subls* ls, const double* R,
     fa.first = l == 0 ? 1 : 0;
     if ((rc = dot_device(n, k.u, k.u, 0, 1, k.partial, k.scal, k.st, fa, k.h, k.cs, k.sn, k.err, k.coef, s)))
       return rc;
-    launch("div", k_div, dim3(nbv), dim3(256), 0, s, n, k.u, (const GmState*)k.st);
+    launch("div", k_div, dim3(nbv), dim3(256), 0, s, n, k.u, (const GmState*)k.st, (float*)nullptr);
     double dBprev = ls->fNorm;
     if ((rc = run_cycle(A, k, sD, fa, X, s))) return rc;
     if (l == 0 && k.h_st->early) {
@@ -1721,6 +1895,14 @@ int svg_norm_v(int dof, int nNo, const double* U, double* out, void* stream)
 int svg_fsils_solve(svg_lhs* L, svg_ls* ls, int dof, double* Ri, double* Val, int prec, int nIncL, const int* incL,
                     const double* res, void* stream)
 {
+  return svg_fsils_solve_ex(L, ls, dof, Ri, Val, prec, nIncL, incL, res, nullptr, stream);
+}
+
+int svg_fsils_solve_ex(svg_lhs* L, svg_ls* ls, int dof, double* Ri, double* Val, int prec, int nIncL,
+                       const int* incL, const double* res, const svg_solve_opts* opts, void* stream)
+{
+  if (opts && opts->precision == SVG_PRECISION_MIXED && ls && ls->LS_type != SVG_LS_GMRES)
+    return set_error(SVG_ERR_UNSUPPORTED, "fsils_solve: mixed precision is built for LS type GMRES");
   if (!L || !ls || !Ri || !Val) return set_error(SVG_ERR_ARG, "svg_fsils_solve: null argument");
   hipStream_t s = (hipStream_t)stream;
   // Face flags, solve.cpp:72-112.
@@ -1762,7 +1944,8 @@ int svg_fsils_solve(svg_lhs* L, svg_ls* ls, int dof, double* Ri, double* Val, in
   }
   switch (ls->LS_type) {
     case SVG_LS_GMRES:
-      rc = gmres_v_dev(L, &ls->RI, dof, Val, Ri, s);
+      rc = (opts && opts->precision == SVG_PRECISION_MIXED) ? gmres_v_mixed(L, &ls->RI, Val, Ri, s)
+                                                             : gmres_v_dev(L, &ls->RI, dof, Val, Ri, s);
       break;
     case SVG_LS_NS:
       rc = ns_solver_dev(L, ls, Val, Ri, s);

@@ -27,10 +27,12 @@ int hip_check(hipError_t e, const char* what)
 struct ProfAcc {
   int count = 0;
   double ms = 0.0;
+  double bytes = 0.0;
 };
 struct ProfPending {
   std::string name;
   hipEvent_t a, b;
+  double bytes;
 };
 static std::mutex g_prof_mu;
 static bool g_prof_on = false;
@@ -50,12 +52,13 @@ static hipEvent_t take_event()
   return e;
 }
 
-int prof_begin(const char* name, hipStream_t s)
+int prof_begin(const char* name, hipStream_t s, double bytes)
 {
   if (!g_prof_on) return -1;
   std::lock_guard<std::mutex> lk(g_prof_mu);
   ProfPending p;
   p.name = name;
+  p.bytes = bytes;
   p.a = take_event();
   p.b = take_event();
   if (!p.a || !p.b) return -1;
@@ -80,6 +83,7 @@ static void prof_flush()
       auto& acc = g_prof[p.name];
       acc.count++;
       acc.ms += ms;
+      acc.bytes += p.bytes;
     }
     g_event_pool.push_back(p.a);
     g_event_pool.push_back(p.b);
@@ -146,6 +150,15 @@ int svg_prof_get(const char* name, int* count, double* total_ms)
   auto it = g_prof.find(name ? name : "");
   *count = it == g_prof.end() ? 0 : it->second.count;
   *total_ms = it == g_prof.end() ? 0.0 : it->second.ms;
+  return SVG_OK;
+}
+
+int svg_prof_get_bytes(const char* name, double* bytes)
+{
+  prof_flush();
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  auto it = g_prof.find(name ? name : "");
+  *bytes = it == g_prof.end() ? 0.0 : it->second.bytes;
   return SVG_OK;
 }
 
@@ -278,6 +291,35 @@ int svg_lhs_free(svg_lhs* L)
   dev_free(k.st);
   dev_free(k.scal);
   dev_free(k.ticket);
+  dev_free(k.u32);
+  dev_free(L->d_val32);
+  for (auto* kw : {&L->kw3}) {
+    dev_free(kw->u);
+    dev_free(kw->X);
+    dev_free(kw->W);
+    dev_free(kw->h);
+    dev_free(kw->cs);
+    dev_free(kw->sn);
+    dev_free(kw->err);
+    dev_free(kw->coef);
+    dev_free(kw->y);
+    dev_free(kw->partial);
+    dev_free(kw->st);
+    dev_free(kw->scal);
+    dev_free(kw->ticket);
+    if (kw->h_st) (void)hipHostFree(kw->h_st);
+    if (kw->h_done) (void)hipHostFree(kw->h_done);
+    for (auto& e : kw->ev)
+      if (e) (void)hipEventDestroy(e);
+  }
+  auto& w = L->ns;
+  for (double* p : {w.mK, w.mG, w.mD, w.mL, w.Gt, w.Rm, w.Rmi, w.Rc, w.Rci, w.U, w.MU, w.P, w.MP, w.cgX, w.cgP,
+                    w.cgSP, w.cgGP, w.coef, w.dots, w.partial})
+    if (p) (void)hipFree(p);
+  if (w.cg) (void)hipFree(w.cg);
+  if (w.ticket) (void)hipFree(w.ticket);
+  if (w.h_cg) (void)hipHostFree(w.h_cg);
+  if (w.h_dots) (void)hipHostFree(w.h_dots);
   if (k.h_st) (void)hipHostFree(k.h_st);
   if (k.h_done) (void)hipHostFree(k.h_done);
   for (auto& e : k.ev)

@@ -118,13 +118,18 @@ def construct_fluid(msh: DeviceMesh, phys: Physics, Ag, Yg, R, Val, Bf=None):
                                                None if Bf is None else _ptr(Bf), _ptr(R), _ptr(Val), _stream()))
 
 
-def fsils_solve(lhs: DeviceLhs, ls: _lib.Ls, dof: int, Ri, Val, prec: int = _lib.PREC_FSILS, incL=None, res=None):
-    """Reference signature: fsils_solve(lhs, ls, dof, Ri, Val, prec, incL, res) (solve.cpp:50)."""
+def fsils_solve(lhs: DeviceLhs, ls: _lib.Ls, dof: int, Ri, Val, prec: int = _lib.PREC_FSILS, incL=None, res=None,
+                precision: int = _lib.PRECISION_FP64):
+    """Reference signature: fsils_solve(lhs, ls, dof, Ri, Val, prec, incL, res) (solve.cpp:50);
+    precision = PRECISION_MIXED selects the FP32-SpMV / FP64-refined GMRES."""
     nf = lhs.nFaces
     inc = np.ones(max(1, nf), dtype=np.int32) if incL is None else np.ascontiguousarray(incL, dtype=np.int32)
     rs = None if res is None else np.ascontiguousarray(res, dtype=np.float64)
-    _lib.check(_lib.load().svg_fsils_solve(lhs.h, C.byref(ls), dof, _ptr(Ri), _ptr(Val), prec, nf, inc.ctypes.data,
-                                           None if rs is None else rs.ctypes.data, _stream()))
+    opts = _lib.SolveOpts()
+    opts.precision = precision
+    _lib.check(_lib.load().svg_fsils_solve_ex(lhs.h, C.byref(ls), dof, _ptr(Ri), _ptr(Val), prec, nf,
+                                              inc.ctypes.data, None if rs is None else rs.ctypes.data, C.byref(opts),
+                                              _stream()))
     return ls
 
 
@@ -150,12 +155,14 @@ class EqState:
 class FluidNewton:
     """One fluid equation on one pipe, all state resident in HBM."""
 
-    def __init__(self, pipe: Pipe, phys: Physics, ls: _lib.Ls, eq: EqState | None = None, device="cuda"):
+    def __init__(self, pipe: Pipe, phys: Physics, ls: _lib.Ls, eq: EqState | None = None, device="cuda",
+                 precision: int = _lib.PRECISION_FP64):
         import torch
         if not torch.cuda.is_available():
             raise RuntimeError("FluidNewton needs a GPU (the HIP library has no CPU path)")
         self.torch = torch
         self.pipe, self.phys, self.ls = pipe, phys, ls
+        self.precision = precision
         self.eq = eq or EqState()
         self.lhs = DeviceLhs(pipe)
         self.msh = DeviceMesh(self.lhs, pipe)
@@ -170,13 +177,14 @@ class FluidNewton:
         self.Ag = torch.empty((n, 4), dtype=f64, device=dev)
         self.Yg = torch.empty((n, 4), dtype=f64, device=dev)
         self.R = torch.empty((n, 4), dtype=f64, device=dev)
+        self._init = [t.clone() for t in (self.Ao, self.Yo, self.An, self.Yn)]
         self.Val = torch.empty((pipe.nnz, 16), dtype=f64, device=dev)
         self.history = []
 
     def reset(self):
-        Ao, Yo, An, Yn = dirichlet_state(self.pipe, self.phys)
-        for t, a in ((self.Ao, Ao), (self.Yo, Yo), (self.An, An), (self.Yn, Yn)):
-            t.copy_(self.torch.from_numpy(a))
+        """Back to the initial state of the time step (device-to-device copies)."""
+        for t, a in zip((self.Ao, self.Yo, self.An, self.Yn), self._init):
+            t.copy_(a)
         self.eq = EqState(self.eq.minItr, self.eq.maxItr, self.eq.tol)
         self.history = []
 
@@ -189,7 +197,7 @@ class FluidNewton:
         _lib.check(lib.svg_fill(_ptr(self.R), self.R.numel(), 0.0, s))
         _lib.check(lib.svg_fill(_ptr(self.Val), self.Val.numel(), 0.0, s))
         construct_fluid(self.msh, ph, self.Ag, self.Yg, self.R, self.Val)
-        fsils_solve(self.lhs, ls, 4, self.R, self.Val)
+        fsils_solve(self.lhs, ls, 4, self.R, self.Val, precision=self.precision)
         _lib.check(lib.svg_picc_update(self.pipe.nNo, ph.gam, ph.dt, _ptr(self.R), _ptr(self.An), _ptr(self.Yn), s))
         # Newton convergence, pic.cpp:236-281
         eps = np.finfo(np.float64).eps

@@ -231,3 +231,34 @@ def test_ns_solver_against_reference_golden(torch_cuda, name):
     assert ls.RI.suc == suc and abs(ls.RI.itr - itr) <= 1
     assert abs(ls.RI.iNorm - iNorm) <= 1e-12 * iNorm
     assert rel <= 1e-3
+
+
+@pytest.mark.parametrize("n", [4, 8])
+def test_mixed_precision_solve(torch_cuda, phys, n):
+    """FP32-SpMV / FP64-refined GMRES: the FP64 true residual of the scaled system is below
+    the tolerance, and the solution is within the tolerance of the FP64 oracle's."""
+    from svfsiplus_amd.newton import DeviceLhs, fsils_solve, ls_create
+    torch = torch_cuda
+    pipe = W.make_pipe(n)
+    Ao, Yo, An, Yn = W.dirichlet_state(pipe, phys)
+    Ag, Yg = oracle.pici(phys, Ao, An, Yo, Yn)
+    R, V = oracle.construct_fluid(pipe, phys, Ag, Yg)
+    tol = 1e-6
+    x0, Vs, ols = oracle.fsils_solve(pipe, oracle.ls_create(oracle.LS_GMRES, relTol=tol, absTol=1e-30, maxItr=1000,
+                                                            dimKry=50), R, V)
+    _, Rs, Wd = oracle.precond_diag(pipe, V, R)
+    L = DeviceLhs(pipe)
+    ls = ls_create(_lib.LS_GMRES, relTol=tol, absTol=1e-30, maxItr=1000, dimKry=50)
+    dR, dV = _dev(torch, R), _dev(torch, V)
+    fsils_solve(L, ls, 4, dR, dV, precision=_lib.PRECISION_MIXED)
+    x1 = dR.cpu().numpy()
+    y = x1 / Wd                                   # unscaled back to the scaled system's unknown
+    mask = Wd != 0
+    y[~mask] = 0.0
+    res = Rs - oracle.spar_mul_vv(pipe, 4, Vs, y)
+    rel_res = np.linalg.norm(res) / np.linalg.norm(Rs)
+    err = np.linalg.norm(x1 - x0) / np.linalg.norm(x0)
+    print(f"n={n} mixed: itr {ls.RI.itr} (fp64 oracle {ols.RI.itr}) true rel residual {rel_res:.2e} "
+          f"|x-x_fp64|/|x_fp64| {err:.2e}")
+    assert ls.RI.suc == 1 and rel_res <= tol
+    assert err <= 20 * tol
