@@ -150,21 +150,67 @@ int svg_ls_create(svg_ls* ls, int LS_type);
 int svg_fsils_solve(svg_lhs* lhs, svg_ls* ls, int dof, double* d_Ri, double* d_Val, int prec, int nIncL,
                     const int* incL, const double* res, void* stream);
 
+/* ------------------------------------------------------------------------ */
+/* Learned preconditioner (no counterpart in the reference; it stands beside */
+/* precond_diag, precond.cpp:122-257, and is used only through the options   */
+/* of svg_fsils_solve_ex). A 3-layer message-passing network over the mesh   */
+/* nodes, hidden width 64 (16 channels x 4 dofs), BF16 weights, activations   */
+/* and matrix copy with FP32 accumulation on MFMA:                            */
+/*   x = v sqrt(nNo); h0 = W0 [x, D_i (diagonal block, 16), 1, 0...]          */
+/*   3x: m_i = sum_j A_ij (x) h_j;  h_i += relu(W_l [h_i; m_i] + b_l)          */
+/*   z = v + (Wd h) / sqrt(nNo)          (Wd = 0: the diagonal preconditioner) */
+/* on the diagonally scaled system. Parameters are one FP32 vector:          */
+/*   W0 [64][32] | 3 x (W_l [64][128], b_l [64]) | Wd [4][64].                */
+typedef struct svg_lpc svg_lpc;
+
+int svg_lpc_num_params(int* count);
+int svg_lpc_create(svg_lpc** net); /* all parameters zero */
+int svg_lpc_free(svg_lpc* net);
+int svg_lpc_set_params(svg_lpc* net, const float* params); /* host FP32; resets the Adam state */
+int svg_lpc_get_params(const svg_lpc* net, float* params);
+/* BF16 copies of the blocks and diagonal blocks of the scaled d_Val (after
+ * svg_precond_diag) for the network on this lhs. */
+int svg_lpc_prepare(svg_lhs* lhs, const double* d_Val, void* stream);
+/* d_Z = M(d_V) for nvec stacked vectors (nvec, nNo, 4) (blocking). */
+int svg_lpc_apply(svg_lpc* net, const svg_lhs* lhs, int nvec, const double* d_V, double* d_Z, void* stream);
+/* Training loss L = (1/nvec) sum_b |R_b - A M(R_b)|^2 over nvec stacked
+ * residuals d_R on the scaled FP64 matrix d_Val, and its gradient (host FP32,
+ * svg_lpc_num_params entries; may be NULL). */
+int svg_lpc_loss_grad(svg_lpc* net, svg_lhs* lhs, const double* d_Val, int nvec, const double* d_R, double* loss,
+                      float* grad, void* stream);
+/* One Adam step (beta 0.9/0.999) with learning rate lr on that loss. */
+int svg_lpc_train_step(svg_lpc* net, svg_lhs* lhs, const double* d_Val, int nvec, const double* d_R, float lr,
+                       double* loss, void* stream);
+
 /* Solve options beyond the reference (no counterpart in fsils_solve):
  *   precision SVG_PRECISION_MIXED (LS type GMRES): Arnoldi cycles use an FP32
- *   copy of the scaled Val and FP32 vector copies for the block SpMV; the
+ *   copy of the scaled Val and FP32 copies of the SpMV input; the
  *   Gram-Schmidt dots, Givens rotations, least squares and the solution stay
  *   FP64, and every restart recomputes the FP64 true residual R - A X, which
- *   both refines the solution and decides convergence (|R - A X| < eps). */
+ *   both refines the solution and decides convergence (|R - A X| < eps).
+ *   method SVG_METHOD_LEARNED (LS type GMRES): flexible GMRES with
+ *   z_j = M(u_j) of the network `lpc` on the diagonally scaled system (prec
+ *   SVG_PREC_FSILS scales it; SVG_PREC_NONE means the caller already did),
+ *   restarts on the FP64 true residual; after fallback_window (0: 20)
+ *   consecutive non-decreasing iterations (none lowering the smallest
+ *   Givens residual estimate so far by 0.1 % or more) the solve continues
+ *   from the current X with the plain diagonal-preconditioned GMRES, and
+ *   fallback_itr reports the iteration count at that point (-1: none). */
 #define SVG_PRECISION_FP64 0
 #define SVG_PRECISION_MIXED 1
+#define SVG_METHOD_PLAIN 0
+#define SVG_METHOD_LEARNED 1
 typedef struct {
   int precision;
-  int reserved[7];
+  int method;
+  int fallback_window;
+  int fallback_itr; /* out */
+  int reserved[4];
+  svg_lpc* lpc;
 } svg_solve_opts;
 
 int svg_fsils_solve_ex(svg_lhs* lhs, svg_ls* ls, int dof, double* d_Ri, double* d_Val, int prec, int nIncL,
-                       const int* incL, const double* res, const svg_solve_opts* opts, void* stream);
+                       const int* incL, const double* res, svg_solve_opts* opts, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Kernels of the solve, exposed for unit parity tests.                      */

@@ -113,8 +113,9 @@ void orc_spar_mul_vv(const orc_lhs* lhs, int dof, const double* K, const double*
 /* Reduction order of dot products and norms.
  *   0: the reference's serial left-to-right sums (dot.cpp, norm.cpp).
  *   1: the HIP library's fixed two-pass order: 1024-element spans per
- *      workgroup, one product per lane, butterfly sums inside each 64-lane
- *      wavefront, the 16 wavefront sums added in order; then the span partials
+ *      workgroup of 256 lanes, each lane summing its 4 products at stride 256
+ *      in order, butterfly sums inside each 64-lane wavefront, the 4
+ *      wavefront sums added in order; then the span partials
  *      summed per lane (span index = lane mod 64, ascending) and a final
  *      butterfly. Used to check the device path bit for bit; the reference
  *      order remains the default. */
@@ -138,14 +139,18 @@ static double device_dot(const double* a, const double* b, long n)
   if (nblk < 1) nblk = 1;
   double* parts = (double*)malloc(sizeof(double) * nblk);
   for (long bk = 0; bk < nblk; bk++) {
-    double acc[1024];
-    for (int t = 0; t < 1024; t++) {
-      long k = bk * 1024 + t;
-      double v = k < n ? a[k] : 0.0, w = k < n ? b[k] : 0.0;
-      acc[t] = 0.0 + v * w;
+    double acc[256];
+    for (int t = 0; t < 256; t++) {
+      double s = 0.0;
+      for (int r = 0; r < 4; r++) {
+        long k = bk * 1024 + r * 256 + t;
+        double v = k < n ? a[k] : 0.0, w = k < n ? b[k] : 0.0;
+        s = s + v * w;
+      }
+      acc[t] = s;
     }
     double s = butterfly64(acc);
-    for (int q = 1; q < 16; q++) s = s + butterfly64(acc + 64 * q);
+    for (int q = 1; q < 4; q++) s = s + butterfly64(acc + 64 * q);
     parts[bk] = s;
   }
   double lanes[64];

@@ -51,10 +51,12 @@ class SubLs(C.Structure):
 
 
 class SolveOpts(C.Structure):
-    _fields_ = [("precision", C.c_int), ("reserved", C.c_int * 7)]
+    _fields_ = [("precision", C.c_int), ("method", C.c_int), ("fallback_window", C.c_int),
+                ("fallback_itr", C.c_int), ("reserved", C.c_int * 4), ("lpc", C.c_void_p)]
 
 
 PRECISION_FP64, PRECISION_MIXED = 0, 1
+METHOD_PLAIN, METHOD_LEARNED = 0, 1
 
 
 class Ls(C.Structure):
@@ -99,6 +101,15 @@ _SIGS = {
     "svg_prof_reset": (_I, []),
     "svg_prof_get": (_I, [C.c_char_p, _IP, _DP]),
     "svg_prof_get_bytes": (_I, [C.c_char_p, _DP]),
+    "svg_lpc_num_params": (_I, [_IP]),
+    "svg_lpc_create": (_I, [C.POINTER(_P)]),
+    "svg_lpc_free": (_I, [_P]),
+    "svg_lpc_set_params": (_I, [_P, _P]),
+    "svg_lpc_get_params": (_I, [_P, _P]),
+    "svg_lpc_prepare": (_I, [_P, _P, _P]),
+    "svg_lpc_apply": (_I, [_P, _P, _I, _P, _P, _P]),
+    "svg_lpc_loss_grad": (_I, [_P, _P, _P, _I, _P, _DP, _P, _P]),
+    "svg_lpc_train_step": (_I, [_P, _P, _P, _I, _P, C.c_float, _DP, _P]),
 }
 
 EXPORTED = tuple(_SIGS)

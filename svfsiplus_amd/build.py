@@ -20,8 +20,12 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libsvgpu.so"
 
-HIP_SOURCES = ["svg_runtime.hip", "svg_fsils.hip", "svg_assembly.hip"]
+HIP_SOURCES = ["svg_runtime.hip", "svg_fsils.hip", "svg_assembly.hip", "svg_lpc.hip"]
 CPP_SOURCES = ["host_setup.cpp"]
+HOST_DIR = PKG / "host"
+HOST_SOURCES = ["svg_fsils.cpp"]            # C++ host API (into the library)
+HOST_PROGRAMS = {"svg_newton_step": "newton_step.cpp"}  # C++ drivers (next to the library)
+ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
 
 ARCH = os.environ.get("SVG_OFFLOAD_ARCH", "gfx950")
 
@@ -62,10 +66,27 @@ def build_library(verbose: bool = False) -> Path:
             print(" ".join(cmd))
         _run(cmd)
         objs.append(str(obj))
+    host_flags = ["-D__HIP_PLATFORM_AMD__", f"-I{ROCM / 'include'}", f"-I{HOST_DIR}"]
+    for src in HOST_SOURCES:
+        obj = objdir / ("host_" + src + ".o")
+        cmd = ["g++", *common, *host_flags, "-c", str(HOST_DIR / src), "-o", str(obj)]
+        if verbose:
+            print(" ".join(cmd))
+        _run(cmd)
+        objs.append(str(obj))
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *objs]
     _run(cmd)
     os.replace(tmp, LIB)
+    for exe, src in HOST_PROGRAMS.items():
+        out = PKG / exe
+        cmd = ["g++", *[c for c in common if c != "-fPIC"], *host_flags, str(HOST_DIR / src), "-o", str(out) + ".tmp",
+               f"-L{PKG}", "-lsvgpu", f"-L{ROCM / 'lib'}", "-lamdhip64", "-Wl,-rpath,$ORIGIN",
+               f"-Wl,-rpath,{ROCM / 'lib'}"]
+        if verbose:
+            print(" ".join(cmd))
+        _run(cmd)
+        os.replace(str(out) + ".tmp", out)
     return LIB
 
 

@@ -44,6 +44,14 @@ inline void launch_b(const char* name, double bytes, K kernel, dim3 grid, dim3 b
 
 inline unsigned blocks_for(size_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
 
+__device__ __forceinline__ double wave_sum(double v)
+{
+  // Fixed butterfly over the 64 lanes of a wavefront.
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
 template <typename T>
 int dev_alloc(T** p, size_t count)
 {
@@ -71,6 +79,9 @@ struct GmState {
   int divzero;  // err(0) == 0 at a restart (Array division by zero)
   int early;    // inner GMRES: |R| <= absTol on entry, X stays 0 (gmres.cpp:146-150)
   int tconv;    // mixed precision: the FP64 true residual at a restart is below eps
+  int fallback; // learned preconditioner: no new residual minimum for the fallback window
+  int nondec;   // iterations since the last new minimum of |err|
+  double best;  // smallest |err(i+1)| of the solve
   double eps;
   double scale;  // err(0) after a restart, 1 / h(i+1,i) after an Arnoldi step
   double iNorm, fNorm;
@@ -110,9 +121,9 @@ struct KrylovWork {
   int* h_done = nullptr;      // mapped host flag written by the device tail
   int* d_done = nullptr;      // device view of h_done
   hipEvent_t ev[8] = {};      // per-step completion events (look-ahead ring)
-  float* u32 = nullptr;       // mixed precision: FP32 copy of the current Arnoldi vector
+  float* u32 = nullptr;       // mixed precision: FP32 copy of the current SpMV input
+  double* z = nullptr;        // flexible GMRES: preconditioned basis z_j = M(u_j), sD vectors
   double* scal = nullptr;     // scratch scalars for standalone dots
-  unsigned* ticket = nullptr; // last-block ticket of the fused reductions
 };
 
 // Workspace of the bipartitioned NS solver (ns_solver.cpp:167-499).
@@ -130,8 +141,15 @@ struct NsWork {
   double* h_dots = nullptr;
   double* partial = nullptr;
   int partialCap = 0;
-  unsigned* ticket = nullptr;
 };
+
+// Shared between the translation units.
+void krylov_free(KrylovWork& k);   // every buffer, pinned mirror and event of a Krylov workspace
+int lhs_ensure_trans(svg_lhs* L);  // block-transpose map (pattern must be symmetric)
+int spmv_residual(const svg_lhs* L, const double* K, const double* U, const double* B, double* out, hipStream_t s);
+int lpc_prepare_dev(svg_lhs* L, const double* Val, hipStream_t s);
+int lpc_apply_dev(svg_lpc* N, const svg_lhs* L, const double* V, double* Z, float* Z32, const int* skip,
+                  hipStream_t s);
 
 }  // namespace svg
 
@@ -156,6 +174,8 @@ struct svg_lhs {
   bool maskBuilt = false;
   int spmvVariant = -1;  // 4x4 SpMV launch variant picked by the autotuner (-1: not tuned)
   float* d_val32 = nullptr;  // mixed precision: FP32 copy of the scaled Val
+  unsigned short* d_val16 = nullptr;   // learned preconditioner: BF16 copy of the scaled Val
+  unsigned short* d_diag16 = nullptr;  // learned preconditioner: BF16 diagonal blocks (16, nNo)
   svg::KrylovWork kw;    // gmres_v on the 4x4 system
   svg::KrylovWork kw3;   // inner GMRES of the NS solver on the 3x3 momentum block
   svg::NsWork ns;

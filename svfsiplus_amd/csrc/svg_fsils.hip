@@ -32,19 +32,11 @@
 
 namespace svg {
 
-constexpr int RED_BS = 1024;  // threads per reduction workgroup (16 wavefronts), one element each
-constexpr int RED_EPT = 1;    // elements per thread
+constexpr int RED_BS = 256;   // threads per reduction workgroup (4 wavefronts)
+constexpr int RED_EPT = 4;    // elements per thread, at stride RED_BS inside the workgroup's span
 constexpr int RED_NW = RED_BS / 64;
 constexpr int RED_SPAN = RED_BS * RED_EPT;
 constexpr int LOOKAHEAD = 3;  // Arnoldi steps enqueued ahead of the host's convergence check
-
-__device__ __forceinline__ double wave_sum(double v)
-{
-  // Fixed butterfly over the 64 lanes of a wavefront.
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
 
 // ------------------------------------------------------------------ SpMV --
 // Blocks are dispatched round-robin over the 8 XCDs; remap so that each XCD
@@ -197,187 +189,50 @@ static void spmv(const svg_lhs* L, const double* K, const double* U, double* KU,
   }
 }
 
-// Arnoldi step on the 4x4 system: W = K U (as k_spmv, same order) fused with
-// the classical Gram-Schmidt dots <base_j, W>, j < m (base_{m-1} is W itself):
-// workgroup b owns elements [1024 b, 1024 b + 1024) = rows [256 b, 256 b + 256),
-// one element per lane, so the partial sums follow the same fixed order as
-// k_mdot and the finaliser completes them.
-template <int UNR, bool NT>
-__global__ __launch_bounds__(1024) void k_spmv_dots(int nNo, const int* __restrict__ rowPtr,
-                                                    const int* __restrict__ colPtr, const double* __restrict__ K,
-                                                    const double* __restrict__ U, double* __restrict__ W,
-                                                    const double* __restrict__ base, size_t ld, int m,
-                                                    double* __restrict__ partial, const int* __restrict__ skip, int xcd)
-{
-#pragma clang fp contract(off)
-  extern __shared__ double lds[];  // [16 waves][m]
-  if (skip && *skip) return;
-  int bx = blockIdx.x;
-  if (xcd) {
-    bx = xcd_block(blockIdx.x, gridDim.x);
-    if (bx < 0) return;
-  }
-  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-  const size_t k = (size_t)bx * 1024 + t;
-  const int row = (int)(k >> 2), i = (int)(k & 3);
-  double w = 0.0;
-  if (row < nNo) {
-    const int s = rowPtr[row], e = rowPtr[row + 1];
-    double acc = 0.0;
-    int j = s;
-    for (; j + UNR <= e; j += UNR) {
-      int c[UNR];
-      double2 a0[UNR], a1[UNR], u0[UNR], u1[UNR];
-#pragma unroll
-      for (int q = 0; q < UNR; q++) c[q] = NT ? __builtin_nontemporal_load(colPtr + j + q) : colPtr[j + q];
-#pragma unroll
-      for (int q = 0; q < UNR; q++) {
-        const double* kp = K + 16 * (size_t)(j + q) + 4 * i;
-        if (NT) {
-          a0[q].x = __builtin_nontemporal_load(kp + 0);
-          a0[q].y = __builtin_nontemporal_load(kp + 1);
-          a1[q].x = __builtin_nontemporal_load(kp + 2);
-          a1[q].y = __builtin_nontemporal_load(kp + 3);
-        } else {
-          a0[q] = reinterpret_cast<const double2*>(kp)[0];
-          a1[q] = reinterpret_cast<const double2*>(kp)[1];
-        }
-        u0[q] = reinterpret_cast<const double2*>(U + 4 * (size_t)c[q])[0];
-        u1[q] = reinterpret_cast<const double2*>(U + 4 * (size_t)c[q])[1];
-      }
-#pragma unroll
-      for (int q = 0; q < UNR; q++) {
-        double tt = acc + a0[q].x * u0[q].x;
-        tt = tt + a0[q].y * u0[q].y;
-        tt = tt + a1[q].x * u1[q].x;
-        acc = tt + a1[q].y * u1[q].y;
-      }
-    }
-    for (; j < e; j++) {
-      const int col = colPtr[j];
-      const double* kp = K + 16 * (size_t)j + 4 * i;
-      const double* u = U + 4 * (size_t)col;
-      double tt = acc + kp[0] * u[0];
-      tt = tt + kp[1] * u[1];
-      tt = tt + kp[2] * u[2];
-      acc = tt + kp[3] * u[3];
-    }
-    w = acc;
-    W[k] = w;
-  }
-  for (int jj = 0; jj < m; jj++) {
-    double v = (jj == m - 1) ? w : (row < nNo ? base[(size_t)jj * ld + k] : 0.0);
-    double prod = 0.0 + v * w;
-    prod = wave_sum(prod);
-    if (lane == 0) lds[wv * m + jj] = prod;
-  }
-  __syncthreads();
-  const int nblk = gridDim.x;
-  for (int jj = t; jj < m; jj += 1024) {
-    double v = lds[jj];
-    for (int q = 1; q < 16; q++) v = v + lds[q * m + jj];
-    partial[(size_t)jj * nblk + bx] = v;
-  }
-}
-
-template <int UNR, bool NT>
-static unsigned spmv_dots_launch(const svg_lhs* L, const double* K, const double* U, double* W, const double* base,
-                                 size_t ld, int m, double* partial, const int* skip, hipStream_t s, bool xcd)
-{
-  unsigned nb = blocks_for((size_t)4 * L->nNo, 1024);
-  const int use_xcd = (xcd && nb >= 64) ? 1 : 0;
-  if (use_xcd) nb = (nb + 7) & ~7u;
-  // Val + colPtr per block, rowPtr + U (read once) + W per row, m-1 basis vectors
-  const double bytes = 132.0 * L->nnz + 68.0 * L->nNo + 32.0 * L->nNo * (m - 1);
-  launch_b("spmv_bcsr", bytes, k_spmv_dots<UNR, NT>, dim3(nb), dim3(1024), (size_t)16 * m * sizeof(double), s, L->nNo,
-         (const int*)L->d_rowPtr, (const int*)L->d_colPtr, K, U, W, base, ld, m, partial, skip, use_xcd);
-  return nb;
-}
-
-static unsigned spmv_dots(const svg_lhs* L, const double* K, const double* U, double* W, const double* base,
-                          size_t ld, int m, double* partial, const int* skip, hipStream_t s)
-{
-  const SpmvVariant& sv = kSpmvVariants[L->spmvVariant >= 0 ? L->spmvVariant : kDefaultSpmvVariant];
-  switch (sv.unroll * 2 + (sv.nt ? 1 : 0)) {
-    case 2: return spmv_dots_launch<1, false>(L, K, U, W, base, ld, m, partial, skip, s, sv.xcd);
-    case 4: return spmv_dots_launch<2, false>(L, K, U, W, base, ld, m, partial, skip, s, sv.xcd);
-    case 5: return spmv_dots_launch<2, true>(L, K, U, W, base, ld, m, partial, skip, s, sv.xcd);
-    case 8: return spmv_dots_launch<4, false>(L, K, U, W, base, ld, m, partial, skip, s, sv.xcd);
-    case 9: return spmv_dots_launch<4, true>(L, K, U, W, base, ld, m, partial, skip, s, sv.xcd);
-    default: return spmv_dots_launch<8, false>(L, K, U, W, base, ld, m, partial, skip, s, sv.xcd);
-  }
-}
-
 // ------------------------------------------------------ mixed precision --
-// Arnoldi step of the mixed-precision solve: W = K32 U32 with FP32 matrix and
-// vector copies and FP32 accumulation in the reference's block order, W stored
-// in FP64, fused with the FP64 Gram-Schmidt dots as k_spmv_dots.
+// SpMV of the mixed-precision solve: W = K32 U32 with FP32 copies of the
+// scaled matrix and of the current Arnoldi vector, FP32 accumulation in the
+// reference's block order, one lane per (row, block-row i), W stored in FP64.
 template <int UNR>
-__global__ __launch_bounds__(1024) void k_spmv32_dots(int nNo, const int* __restrict__ rowPtr,
-                                                      const int* __restrict__ colPtr, const float* __restrict__ K,
-                                                      const float* __restrict__ U, double* __restrict__ W,
-                                                      const double* __restrict__ base, size_t ld, int m,
-                                                      double* __restrict__ partial, const int* __restrict__ skip,
-                                                      int xcd)
+__global__ __launch_bounds__(256) void k_spmv32(int nNo, const int* __restrict__ rowPtr,
+                                                const int* __restrict__ colPtr, const float* __restrict__ K,
+                                                const float* __restrict__ U, double* __restrict__ W,
+                                                const int* __restrict__ skip)
 {
-#pragma clang fp contract(off)
-  extern __shared__ double lds[];  // [16 waves][m]
   if (skip && *skip) return;
-  int bx = blockIdx.x;
-  if (xcd) {
-    bx = xcd_block(blockIdx.x, gridDim.x);
-    if (bx < 0) return;
-  }
-  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-  const size_t k = (size_t)bx * 1024 + t;
+  const size_t k = (size_t)blockIdx.x * 256 + threadIdx.x;
   const int row = (int)(k >> 2), i = (int)(k & 3);
-  double w = 0.0;
-  if (row < nNo) {
-    const int s = rowPtr[row], e = rowPtr[row + 1];
-    float acc = 0.0f;
-    int j = s;
-    for (; j + UNR <= e; j += UNR) {
-      int c[UNR];
-      float4 a[UNR], u[UNR];
+  if (row >= nNo) return;
+  const int s = rowPtr[row], e = rowPtr[row + 1];
+  float acc = 0.0f;
+  int j = s;
+  for (; j + UNR <= e; j += UNR) {
+    int c[UNR];
+    float4 a[UNR], u[UNR];
 #pragma unroll
-      for (int q = 0; q < UNR; q++) c[q] = colPtr[j + q];
+    for (int q = 0; q < UNR; q++) c[q] = colPtr[j + q];
 #pragma unroll
-      for (int q = 0; q < UNR; q++) {
-        a[q] = reinterpret_cast<const float4*>(K + 16 * (size_t)(j + q) + 4 * i)[0];
-        u[q] = reinterpret_cast<const float4*>(U + 4 * (size_t)c[q])[0];
-      }
-#pragma unroll
-      for (int q = 0; q < UNR; q++) {
-        acc = __builtin_fmaf(a[q].x, u[q].x, acc);
-        acc = __builtin_fmaf(a[q].y, u[q].y, acc);
-        acc = __builtin_fmaf(a[q].z, u[q].z, acc);
-        acc = __builtin_fmaf(a[q].w, u[q].w, acc);
-      }
+    for (int q = 0; q < UNR; q++) {
+      a[q] = reinterpret_cast<const float4*>(K + 16 * (size_t)(j + q) + 4 * i)[0];
+      u[q] = reinterpret_cast<const float4*>(U + 4 * (size_t)c[q])[0];
     }
-    for (; j < e; j++) {
-      const float4 a = reinterpret_cast<const float4*>(K + 16 * (size_t)j + 4 * i)[0];
-      const float4 u = reinterpret_cast<const float4*>(U + 4 * (size_t)colPtr[j])[0];
-      acc = __builtin_fmaf(a.x, u.x, acc);
-      acc = __builtin_fmaf(a.y, u.y, acc);
-      acc = __builtin_fmaf(a.z, u.z, acc);
-      acc = __builtin_fmaf(a.w, u.w, acc);
+#pragma unroll
+    for (int q = 0; q < UNR; q++) {
+      acc = __builtin_fmaf(a[q].x, u[q].x, acc);
+      acc = __builtin_fmaf(a[q].y, u[q].y, acc);
+      acc = __builtin_fmaf(a[q].z, u[q].z, acc);
+      acc = __builtin_fmaf(a[q].w, u[q].w, acc);
     }
-    w = (double)acc;
-    W[k] = w;
   }
-  for (int jj = 0; jj < m; jj++) {
-    double v = (jj == m - 1) ? w : (row < nNo ? base[(size_t)jj * ld + k] : 0.0);
-    double prod = 0.0 + v * w;
-    prod = wave_sum(prod);
-    if (lane == 0) lds[wv * m + jj] = prod;
+  for (; j < e; j++) {
+    const float4 a = reinterpret_cast<const float4*>(K + 16 * (size_t)j + 4 * i)[0];
+    const float4 u = reinterpret_cast<const float4*>(U + 4 * (size_t)colPtr[j])[0];
+    acc = __builtin_fmaf(a.x, u.x, acc);
+    acc = __builtin_fmaf(a.y, u.y, acc);
+    acc = __builtin_fmaf(a.z, u.z, acc);
+    acc = __builtin_fmaf(a.w, u.w, acc);
   }
-  __syncthreads();
-  const int nblk = gridDim.x;
-  for (int jj = t; jj < m; jj += 1024) {
-    double v = lds[jj];
-    for (int q = 1; q < 16; q++) v = v + lds[q * m + jj];
-    partial[(size_t)jj * nblk + bx] = v;
-  }
+  W[k] = (double)acc;
 }
 
 __global__ __launch_bounds__(256) void k_d2f(size_t n, const double* __restrict__ a, float* __restrict__ b)
@@ -397,10 +252,12 @@ struct FinArgs {
   int inc;    // FIN_RESTART: itr increment (the first cycle of the inner GMRES adds none)
   int first;  // FIN_RESTART: first cycle of the inner GMRES sets iNorm and eps from err(0)
   int tcheck; // FIN_RESTART: stop when the (FP64, true) restart residual is below eps
+  int fbwin;  // FIN_ARNOLDI: fallback after this many non-decreasing iterations of |err| (0: off)
   double absTol, relTol;
 };
 
 constexpr int MAXM = 512;  // sums per reduction (Krylov dimension + 2)
+constexpr double FB_DROP = 1e-3;  // learned preconditioner: smallest relative decrease that counts
 
 // Dots of w against m vectors base + j*ld (j < m), pass 1: each workgroup
 // sums its 1024-element span (256 lanes x 4 elements at stride 256, in order),
@@ -411,7 +268,7 @@ __global__ __launch_bounds__(RED_BS) void k_mdot(int n, const double* __restrict
                                                  const int* __restrict__ skip)
 {
 #pragma clang fp contract(off)
-  extern __shared__ double lds[];  // [16 waves][m]
+  extern __shared__ double lds[];  // [RED_NW waves][m]
   if (skip && *skip) return;
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   const int nblk = gridDim.x;
@@ -501,6 +358,9 @@ __global__ __launch_bounds__(FIN_BS) void k_fin(FinArgs A, const double* __restr
       st->early = 0;
       st->tconv = 0;
       st->last_i = 0;
+      st->fallback = 0;
+      st->nondec = 0;
+      st->best = DBL_MAX;
     }
     return;
   }
@@ -569,6 +429,20 @@ __global__ __launch_bounds__(FIN_BS) void k_fin(FinArgs A, const double* __restr
       if (host_done) {
         *host_done = 1;
         __threadfence_system();
+      }
+    } else if (A.fbwin > 0) {
+      // an iteration counts as decreasing when it lowers the smallest residual
+      // estimate so far by at least FB_DROP (relative)
+      if (fabs(e1) < st->best * (1.0 - FB_DROP)) {
+        st->best = fabs(e1);
+        st->nondec = 0;
+      } else if (++st->nondec >= A.fbwin) {
+        st->fallback = 1;
+        st->done = 1;
+        if (host_done) {
+          *host_done = 1;
+          __threadfence_system();
+        }
       }
     }
   }
@@ -1016,7 +890,7 @@ __global__ __launch_bounds__(256) void k_sub(int n, const double* __restrict__ a
 }
 
 // --------------------------------------------------------- workspace ----
-static void free_work(KrylovWork& k)
+void krylov_free(KrylovWork& k)
 {
   dev_free(k.u);
   dev_free(k.X);
@@ -1030,15 +904,24 @@ static void free_work(KrylovWork& k)
   dev_free(k.partial);
   dev_free(k.st);
   dev_free(k.scal);
-  dev_free(k.ticket);
+  dev_free(k.u32);
+  dev_free(k.z);
   if (k.h_st) (void)hipHostFree(k.h_st);
   k.h_st = nullptr;
+  if (k.h_done) {
+    (void)hipHostFree(k.h_done);
+    for (auto& e : k.ev) (void)hipEventDestroy(e);
+  }
+  k.h_done = nullptr;
+  k.d_done = nullptr;
+  k.n = 0;
+  k.sD = 0;
 }
 
 static int ensure_work(KrylovWork& k, int n, int sD)
 {
   if (k.u && k.n == n && k.sD >= sD) return SVG_OK;
-  free_work(k);
+  krylov_free(k);
   k.n = n;
   k.sD = sD;
   k.ld = ((size_t)n + 63) & ~(size_t)63;
@@ -1048,7 +931,7 @@ static int ensure_work(KrylovWork& k, int n, int sD)
       (rc = dev_alloc(&k.h, (size_t)(sD + 1) * sD)) || (rc = dev_alloc(&k.cs, sD)) || (rc = dev_alloc(&k.sn, sD)) ||
       (rc = dev_alloc(&k.err, sD + 1)) || (rc = dev_alloc(&k.coef, sD + 2)) || (rc = dev_alloc(&k.y, sD)) ||
       (rc = dev_alloc(&k.partial, (size_t)k.nblk * (sD + 2))) || (rc = dev_alloc(&k.st, 1)) ||
-      (rc = dev_alloc(&k.scal, 8)) || (rc = dev_alloc(&k.ticket, 1)))
+      (rc = dev_alloc(&k.scal, 8)))
     return rc;
   SVG_HIP(hipHostMalloc((void**)&k.h_st, sizeof(GmState), hipHostMallocDefault));
   if (!k.h_done) {
@@ -1058,7 +941,6 @@ static int ensure_work(KrylovWork& k, int n, int sD)
   }
   SVG_HIP(hipMemset(k.h, 0, sizeof(double) * (sD + 1) * sD));
   SVG_HIP(hipMemset(k.st, 0, sizeof(GmState)));
-  SVG_HIP(hipMemset(k.ticket, 0, sizeof(unsigned)));
   return SVG_OK;
 }
 
@@ -1067,7 +949,6 @@ struct DotScratch {
   double* partial = nullptr;
   int cap = 0;
   double* out = nullptr;
-  unsigned* ticket = nullptr;
 };
 static DotScratch g_dot;
 
@@ -1108,14 +989,22 @@ static void op_apply(const Op& A, const double* x, double* y, const double* b, c
   }
 }
 
+int spmv_residual(const svg_lhs* L, const double* K, const double* U, const double* B, double* out, hipStream_t s)
+{
+  spmv<4, 4, 1>(L, K, U, out, B, nullptr, s, "spmv_res");
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? SVG_OK : hip_check(e, "spmv_residual");
+}
+
 // One restart cycle (gmres.cpp:529-612 / :533-592): up to sD Arnoldi steps with
 // at most LOOKAHEAD steps enqueued beyond the host's last convergence check,
 // then the back substitution and X += sum y_j u_j. Returns after the cycle's
 // state has been copied to the host.
 static int run_cycle(const Op& A, KrylovWork& k, int sD, FinArgs fa, double* X, hipStream_t s,
-                     const float* K32 = nullptr)
+                     const float* K32 = nullptr, svg_lpc* net = nullptr)
 {
   const bool mixed = K32 != nullptr;
+  const bool flex = net != nullptr;
   const int n = k.n;
   const unsigned nbv = blocks_for(n, 256);
   *k.h_done = 0;
@@ -1126,34 +1015,33 @@ static int run_cycle(const Op& A, KrylovWork& k, int sD, FinArgs fa, double* X, 
     }
     fa.mode = FIN_ARNOLDI;
     fa.i = i;
+    // flexible GMRES: the SpMV acts on z_i = M(u_i) (and its FP32 copy in u32)
+    const double* sin = k.u + k.ld * i;
+    if (flex) {
+      int rc = lpc_apply_dev(net, A.L, k.u + k.ld * i, k.z + k.ld * i, mixed ? k.u32 : nullptr, &k.st->done, s);
+      if (rc) return rc;
+      sin = k.z + k.ld * i;
+    }
     if (mixed) {
-      unsigned nb = blocks_for((size_t)4 * A.L->nNo, 1024);
-      const int use_xcd = nb >= 64 ? 1 : 0;
-      if (use_xcd) nb = (nb + 7) & ~7u;
-      const double bytes = 68.0 * A.L->nnz + 52.0 * A.L->nNo + 32.0 * A.L->nNo * (i + 1);
-      launch_b("spmv32", bytes, k_spmv32_dots<2>, dim3(nb), dim3(1024), (size_t)16 * (i + 2) * sizeof(double), s, A.L->nNo,
-             (const int*)A.L->d_rowPtr, (const int*)A.L->d_colPtr, K32, (const float*)k.u32, k.u + k.ld * (i + 1),
-             (const double*)k.u, k.ld, i + 2, k.partial, (const int*)&k.st->done, use_xcd);
-      launch("mdot_fin", k_fin, dim3(1), dim3(FIN_BS), 0, s, fa, (const double*)k.partial, (int)nb, i + 2, k.st, k.h,
-             k.cs, k.sn, k.err, k.coef, k.scal, (volatile int*)k.d_done);
-    } else if (A.dof == 4) {
-      const unsigned nb = spmv_dots(A.L, A.K, k.u + k.ld * i, k.u + k.ld * (i + 1), k.u, k.ld, i + 2, k.partial,
-                                    &k.st->done, s);
-      launch("mdot_fin", k_fin, dim3(1), dim3(FIN_BS), 0, s, fa, (const double*)k.partial, (int)nb, i + 2, k.st, k.h,
-             k.cs, k.sn, k.err, k.coef, k.scal, (volatile int*)k.d_done);
+      const double bytes = 68.0 * A.L->nnz + 36.0 * A.L->nNo;
+      launch_b("spmv32", bytes, k_spmv32<2>, dim3(blocks_for((size_t)4 * A.L->nNo, 256)), dim3(256), 0, s,
+               A.L->nNo, (const int*)A.L->d_rowPtr, (const int*)A.L->d_colPtr, K32, (const float*)k.u32,
+               k.u + k.ld * (i + 1), (const int*)&k.st->done);
     } else {
-      op_apply(A, k.u + k.ld * i, k.u + k.ld * (i + 1), nullptr, &k.st->done, s);
+      op_apply(A, sin, k.u + k.ld * (i + 1), nullptr, &k.st->done, s);
+    }
+    {
       int rc = dot_device(n, k.u + k.ld * (i + 1), k.u, k.ld, i + 2, k.partial, k.scal, k.st, fa, k.h, k.cs,
                           k.sn, k.err, k.coef, s, k.d_done);
       if (rc) return rc;
     }
     launch("gs_update", k_gs_update, dim3(nbv), dim3(256), 0, s, n, k.u, k.ld, i, (const double*)k.coef,
-           (const GmState*)k.st, mixed ? k.u32 : (float*)nullptr);
+           (const GmState*)k.st, (mixed && !flex) ? k.u32 : (float*)nullptr);
     SVG_HIP(hipEventRecord(k.ev[i % 8], s));
   }
   launch("backsolve", k_backsolve, dim3(1), dim3(256), 0, s, sD, (const double*)k.h, (const double*)k.err, k.y, k.st);
-  launch("xupdate", k_xupdate, dim3(nbv), dim3(256), 0, s, n, X, (const double*)k.u, k.ld, (const double*)k.y,
-         (const GmState*)k.st);
+  launch("xupdate", k_xupdate, dim3(nbv), dim3(256), 0, s, n, X, (const double*)(flex ? k.z : k.u), k.ld,
+         (const double*)k.y, (const GmState*)k.st);
   SVG_HIP(hipGetLastError());
   SVG_HIP(hipMemcpyAsync(k.h_st, k.st, sizeof(GmState), hipMemcpyDeviceToHost, s));
   SVG_HIP(hipStreamSynchronize(s));
@@ -1274,6 +1162,87 @@ static int gmres_v_mixed(svg_lhs* L, svg_subls* ls, const double* Val, double* R
     if (k.h_st->tconv) {
       ls->suc = 1;
       break;
+    }
+  }
+  SVG_HIP(hipMemcpyAsync(R, k.X, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+  auto t1 = std::chrono::steady_clock::now();
+  ls->callD = std::chrono::duration<double>(t1 - t0).count();
+  ls->dB = 10.0 * log(ls->fNorm / dB);
+  return SVG_OK;
+}
+
+// Flexible GMRES with the learned preconditioner on the diagonally scaled
+// system: z_i = M(u_i), w = A z_i, the reference's Gram-Schmidt and Givens
+// steps unchanged, X += sum y_i z_i; every restart recomputes the FP64 true
+// residual R - A X and convergence is decided on it. With `mixed` the SpMV
+// uses the FP32 matrix copy and the decoder's FP32 copy of z_i. When |err|
+// has reached no new minimum for `window` iterations the cycle stops and the
+// solve goes on from the current X with the plain (diagonal-only) GMRES
+// restarts of the same precision.
+static int gmres_v_learned(svg_lhs* L, svg_subls* ls, const double* Val, double* R, bool mixed, svg_lpc* net,
+                           int window, int* fallback_itr, hipStream_t s)
+{
+  if (ls->sD < 1 || ls->sD > 250) return set_error(SVG_ERR_ARG, "gmres_v: Krylov dimension must be in [1, 250]");
+  auto& k = L->kw;
+  int rc = ensure_work(k, 4 * L->nNo, ls->sD);
+  if (rc) return rc;
+  const int n = k.n, sD = ls->sD;
+  if (!k.z && (rc = dev_alloc(&k.z, k.ld * sD))) return rc;
+  if (mixed) {
+    if (!k.u32 && (rc = dev_alloc(&k.u32, (size_t)n))) return rc;
+    if (!L->d_val32 && (rc = dev_alloc(&L->d_val32, (size_t)16 * L->nnz))) return rc;
+  }
+  const Op A{L, Val, 4};
+  if (L->spmvVariant < 0 && (rc = svg_spmv_tune(L, Val, nullptr, s))) return rc;
+  auto t0 = std::chrono::steady_clock::now();
+  if ((rc = lpc_prepare_dev(L, Val, s))) return rc;
+  if (mixed) launch("d2f", k_d2f, dim3(8192), dim3(256), 0, s, (size_t)16 * L->nnz, Val, L->d_val32);
+  FinArgs fa{};
+  fa.sD = sD;
+  fa.absTol = ls->absTol;
+  fa.relTol = ls->relTol;
+  fa.mode = FIN_NORM_R;
+  if ((rc = dot_device(n, R, R, 0, 1, k.partial, k.scal, k.st, fa, k.h, k.cs, k.sn, k.err, k.coef, s))) return rc;
+  SVG_HIP(hipMemcpyAsync(k.h_st, k.st, sizeof(GmState), hipMemcpyDeviceToHost, s));
+  SVG_HIP(hipStreamSynchronize(s));
+  ls->suc = 0;
+  ls->iNorm = k.h_st->iNorm;
+  ls->fNorm = k.h_st->fNorm;
+  ls->itr = 0;
+  *fallback_itr = -1;
+  if (ls->iNorm <= ls->absTol) {
+    ls->callD = DBL_EPSILON;
+    ls->dB = 0.0;
+    return SVG_OK;
+  }
+  svg_fill(k.X, (size_t)n, 0.0, s);
+  const unsigned nbv = blocks_for(n, 256);
+  double dB = ls->fNorm;
+  svg_lpc* pc = net;
+  fa.fbwin = window;
+  for (int l = 0; l < ls->mItr; l++) {
+    dB = ls->fNorm;
+    op_apply(A, k.X, k.u, R, nullptr, s);  // FP64 true residual
+    fa.mode = FIN_RESTART;
+    fa.inc = 1;
+    fa.first = 0;
+    fa.tcheck = 1;
+    if ((rc = dot_device(n, k.u, k.u, 0, 1, k.partial, k.scal, k.st, fa, k.h, k.cs, k.sn, k.err, k.coef, s)))
+      return rc;
+    launch("div", k_div, dim3(nbv), dim3(256), 0, s, n, k.u, (const GmState*)k.st,
+           (mixed && !pc) ? k.u32 : (float*)nullptr);
+    if ((rc = run_cycle(A, k, sD, fa, k.X, s, mixed ? L->d_val32 : nullptr, pc))) return rc;
+    if (k.h_st->divzero) return set_error(SVG_ERR_ARG, "Array Divide by zero.");
+    ls->itr = k.h_st->itr;
+    ls->fNorm = k.h_st->fNorm;
+    if (k.h_st->tconv) {
+      ls->suc = 1;
+      break;
+    }
+    if (pc && k.h_st->fallback) {
+      *fallback_itr = ls->itr;
+      pc = nullptr;
+      fa.fbwin = 0;
     }
   }
   SVG_HIP(hipMemcpyAsync(R, k.X, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
@@ -1431,21 +1400,8 @@ static int ns_setup(svg_lhs* L)
   auto& w = L->ns;
   const int nNo = L->nNo, nnz = L->nnz;
   if (w.mK && w.nNo == nNo) return SVG_OK;
-  if (!L->d_trans) {
-    std::vector<int> tr(nnz);
-    for (int i = 0; i < nNo; i++)
-      for (int j = L->rowPtr[i]; j < L->rowPtr[i + 1]; j++) {
-        const int k = L->colPtr[j];
-        const int* b = L->colPtr.data() + L->rowPtr[k];
-        const int* e = L->colPtr.data() + L->rowPtr[k + 1];
-        const int* it = std::lower_bound(b, e, i);
-        if (it == e || *it != i) return set_error(SVG_ERR_ARG, "ns_solver: the block pattern is not symmetric");
-        tr[j] = (int)(it - L->colPtr.data());
-      }
-    int rc = dev_alloc(&L->d_trans, (size_t)nnz);
-    if (rc) return rc;
-    SVG_HIP(hipMemcpy(L->d_trans, tr.data(), sizeof(int) * nnz, hipMemcpyHostToDevice));
-  }
+  int rc0 = lhs_ensure_trans(L);
+  if (rc0) return rc0;
   w.nNo = nNo;
   w.ld3 = ((size_t)3 * nNo + 63) & ~(size_t)63;
   w.ld1 = ((size_t)nNo + 63) & ~(size_t)63;
@@ -1455,11 +1411,10 @@ static int ns_setup(svg_lhs* L)
       (rc = dev_alloc(&w.Gt, (size_t)3 * nnz)) || (rc = dev_alloc(&w.Rm, w.ld3)) || (rc = dev_alloc(&w.Rmi, w.ld3)) ||
       (rc = dev_alloc(&w.Rc, w.ld1)) || (rc = dev_alloc(&w.Rci, w.ld1)) || (rc = dev_alloc(&w.cgX, w.ld1)) ||
       (rc = dev_alloc(&w.cgP, w.ld1)) || (rc = dev_alloc(&w.cgSP, w.ld1)) || (rc = dev_alloc(&w.cgGP, w.ld3)) ||
-      (rc = dev_alloc(&w.cg, 1)) || (rc = dev_alloc(&w.dots, MAXPAIRS)) || (rc = dev_alloc(&w.ticket, 1)))
+      (rc = dev_alloc(&w.cg, 1)) || (rc = dev_alloc(&w.dots, MAXPAIRS)))
     return rc;
   w.partialCap = (int)blocks_for(3 * nNo, RED_SPAN) * MAXPAIRS + (int)blocks_for(nNo, RED_BS);
   if ((rc = dev_alloc(&w.partial, (size_t)w.partialCap))) return rc;
-  SVG_HIP(hipMemset(w.ticket, 0, sizeof(unsigned)));
   SVG_HIP(hipMemset(w.cg, 0, sizeof(CgState)));
   SVG_HIP(hipHostMalloc((void**)&w.h_cg, sizeof(CgState), hipHostMallocDefault));
   SVG_HIP(hipHostMalloc((void**)&w.h_dots, sizeof(double) * MAXPAIRS, hipHostMallocDefault));
@@ -1863,9 +1818,7 @@ static int dot_api(int n, const double* U, const double* V, double* out, hipStre
   }
   if (!g_dot.out) {
     int rc = dev_alloc(&g_dot.out, 1);
-    if (!rc) rc = dev_alloc(&g_dot.ticket, 1);
     if (rc) return rc;
-    SVG_HIP(hipMemset(g_dot.ticket, 0, sizeof(unsigned)));
   }
   FinArgs fa{};
   fa.mode = FIN_DOT;
@@ -1899,10 +1852,16 @@ int svg_fsils_solve(svg_lhs* L, svg_ls* ls, int dof, double* Ri, double* Val, in
 }
 
 int svg_fsils_solve_ex(svg_lhs* L, svg_ls* ls, int dof, double* Ri, double* Val, int prec, int nIncL,
-                       const int* incL, const double* res, const svg_solve_opts* opts, void* stream)
+                       const int* incL, const double* res, svg_solve_opts* opts, void* stream)
 {
   if (opts && opts->precision == SVG_PRECISION_MIXED && ls && ls->LS_type != SVG_LS_GMRES)
     return set_error(SVG_ERR_UNSUPPORTED, "fsils_solve: mixed precision is built for LS type GMRES");
+  const bool learned = opts && opts->method == SVG_METHOD_LEARNED;
+  if (opts && opts->method != SVG_METHOD_PLAIN && !learned)
+    return set_error(SVG_ERR_ARG, "fsils_solve: unknown method");
+  if (learned && (!opts->lpc || (ls && ls->LS_type != SVG_LS_GMRES)))
+    return set_error(SVG_ERR_UNSUPPORTED, "fsils_solve: the learned preconditioner needs a network and LS type GMRES");
+  if (opts) opts->fallback_itr = -1;
   if (!L || !ls || !Ri || !Val) return set_error(SVG_ERR_ARG, "svg_fsils_solve: null argument");
   hipStream_t s = (hipStream_t)stream;
   // Face flags, solve.cpp:72-112.
@@ -1944,8 +1903,12 @@ int svg_fsils_solve_ex(svg_lhs* L, svg_ls* ls, int dof, double* Ri, double* Val,
   }
   switch (ls->LS_type) {
     case SVG_LS_GMRES:
-      rc = (opts && opts->precision == SVG_PRECISION_MIXED) ? gmres_v_mixed(L, &ls->RI, Val, Ri, s)
-                                                             : gmres_v_dev(L, &ls->RI, dof, Val, Ri, s);
+      if (learned)
+        rc = gmres_v_learned(L, &ls->RI, Val, Ri, opts->precision == SVG_PRECISION_MIXED, opts->lpc,
+                             opts->fallback_window > 0 ? opts->fallback_window : 20, &opts->fallback_itr, s);
+      else
+        rc = (opts && opts->precision == SVG_PRECISION_MIXED) ? gmres_v_mixed(L, &ls->RI, Val, Ri, s)
+                                                               : gmres_v_dev(L, &ls->RI, dof, Val, Ri, s);
       break;
     case SVG_LS_NS:
       rc = ns_solver_dev(L, ls, Val, Ri, s);

@@ -277,53 +277,18 @@ int svg_lhs_free(svg_lhs* L)
   dev_free(L->d_diagPtr);
   dev_free(L->d_trans);
   dev_free(L->d_mask);
-  auto& k = L->kw;
-  dev_free(k.u);
-  dev_free(k.X);
-  dev_free(k.W);
-  dev_free(k.h);
-  dev_free(k.cs);
-  dev_free(k.sn);
-  dev_free(k.err);
-  dev_free(k.coef);
-  dev_free(k.y);
-  dev_free(k.partial);
-  dev_free(k.st);
-  dev_free(k.scal);
-  dev_free(k.ticket);
-  dev_free(k.u32);
+  svg::krylov_free(L->kw);
+  svg::krylov_free(L->kw3);
   dev_free(L->d_val32);
-  for (auto* kw : {&L->kw3}) {
-    dev_free(kw->u);
-    dev_free(kw->X);
-    dev_free(kw->W);
-    dev_free(kw->h);
-    dev_free(kw->cs);
-    dev_free(kw->sn);
-    dev_free(kw->err);
-    dev_free(kw->coef);
-    dev_free(kw->y);
-    dev_free(kw->partial);
-    dev_free(kw->st);
-    dev_free(kw->scal);
-    dev_free(kw->ticket);
-    if (kw->h_st) (void)hipHostFree(kw->h_st);
-    if (kw->h_done) (void)hipHostFree(kw->h_done);
-    for (auto& e : kw->ev)
-      if (e) (void)hipEventDestroy(e);
-  }
+  dev_free(L->d_val16);
+  dev_free(L->d_diag16);
   auto& w = L->ns;
   for (double* p : {w.mK, w.mG, w.mD, w.mL, w.Gt, w.Rm, w.Rmi, w.Rc, w.Rci, w.U, w.MU, w.P, w.MP, w.cgX, w.cgP,
                     w.cgSP, w.cgGP, w.coef, w.dots, w.partial})
     if (p) (void)hipFree(p);
   if (w.cg) (void)hipFree(w.cg);
-  if (w.ticket) (void)hipFree(w.ticket);
   if (w.h_cg) (void)hipHostFree(w.h_cg);
   if (w.h_dots) (void)hipHostFree(w.h_dots);
-  if (k.h_st) (void)hipHostFree(k.h_st);
-  if (k.h_done) (void)hipHostFree(k.h_done);
-  for (auto& e : k.ev)
-    if (e) (void)hipEventDestroy(e);
   delete L;
   return SVG_OK;
 }

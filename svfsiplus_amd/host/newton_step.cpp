@@ -1,0 +1,131 @@
+// One time step of the fluid equation on a seeded pipe, driven from C++ over
+// the host API (svg_fsils.hpp): the reference's Newton loop for one fluid
+// equation (Code/Source/solver/main.cpp:371-617) —
+//   pici -> ls_alloc (R = 0, Val = 0) -> construct_fluid -> fsils_solve -> picc
+// with picc's convergence test (pic.cpp:236-281) on the host.
+//
+//   svg_newton_step <n> [relTol] [krylov] [maxItr]
+//
+// prints one JSON record per Newton iteration (itr, ls_itr, iNorm, fNorm, suc).
+// The workload: pipe radius 0.3, length 3, n x n cross-section, 5n axial,
+// jitter 0.1 h, seed 20261202; rho 1.06, mu 0.04, parabolic inflow peak 30,
+// no-slip walls, zero-traction outlet, dt 1e-3, rho_inf 0.5; GMRES with the
+// diagonal preconditioner, absolute tolerance 1e-30.
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "svg_fsils.hpp"
+
+using namespace svg_host;
+
+namespace {
+
+bool is_zero(double v)  // utils::is_zero, utils.cpp:170-189, with value2 = 0
+{
+  const double a = std::fabs(v), eps = DBL_EPSILON;
+  return a / std::fmax(a, eps) < 10.0 * eps;
+}
+
+}  // namespace
+
+int main(int argc, char** argv)
+{
+  if (argc < 2) {
+    std::fprintf(stderr, "usage: %s <n> [relTol] [krylov] [maxItr]\n", argv[0]);
+    return 2;
+  }
+  const int n = std::atoi(argv[1]);
+  const double relTol = argc > 2 ? std::atof(argv[2]) : 1e-6;
+  const int sD = argc > 3 ? std::atoi(argv[3]) : 50;
+  const int mItr = argc > 4 ? std::atoi(argv[4]) : 1000;
+  try {
+    // mesh and pattern (setup)
+    svg_pipe_spec sp{n, 5 * n, 0.3, 3.0, 0.1, 20261202ULL, 0.0, 0.0, 0.0};
+    int nNo = 0, nEl = 0, nnz = 0;
+    check(svg_pipe_counts(&sp, &nNo, &nEl));
+    std::vector<double> x((size_t)3 * nNo);
+    std::vector<int> IEN((size_t)4 * nEl), flags(nNo);
+    check(svg_pipe_generate(&sp, x.data(), IEN.data(), flags.data()));
+    check(svg_csr_pattern_count(nNo, nEl, 4, IEN.data(), &nnz));
+    std::vector<int> rowPtr(nNo + 1), colPtr(nnz);
+    check(svg_csr_pattern(nNo, nEl, 4, IEN.data(), rowPtr.data(), colPtr.data()));
+    std::vector<int> inlet, wall;
+    for (int a = 0; a < nNo; a++) {
+      if (flags[a] & SVG_NODE_INLET) inlet.push_back(a);
+      if (flags[a] & SVG_NODE_WALL) wall.push_back(a);
+    }
+    FSILS_lhsType lhs;
+    fsils_lhs_create(lhs, nNo, nnz, rowPtr, colPtr, 2);
+    fsils_bc_create(lhs, 0, (int)inlet.size(), 3, BcType::BC_TYPE_Dir, inlet);
+    fsils_bc_create(lhs, 1, (int)wall.size(), 3, BcType::BC_TYPE_Dir, wall);
+    FluidMesh msh;
+    fluid_mesh_create(msh, lhs, nEl, IEN, x);
+
+    // physics and generalized-alpha (initialize.cpp:430-463)
+    const double ro = 0.5, peak = 30.0, R0 = 0.3;
+    svg_fluid_params p{};
+    p.rho = 1.06;
+    p.mu = 0.04;
+    p.am = 0.5 * (3.0 - ro) / (1.0 + ro);
+    p.af = 1.0 / (1.0 + ro);
+    p.gam = 0.5 + p.am - p.af;
+    p.dt = 1e-3;
+    p.qmTET4 = (5.0 + 3.0 * std::sqrt(5.0)) / 20.0;
+
+    // initial state: Ao = Yo = 0, predictor An = Ao (gam-1)/gam, Yn = Yo, then
+    // the Dirichlet values face by face (inlet parabola, wall zero)
+    std::vector<double> Ao((size_t)4 * nNo, 0.0), Yo(Ao), An(Ao), Yn(Ao);
+    for (int a : inlet) {
+      const double r2 = (x[3 * a] * x[3 * a] + x[3 * a + 1] * x[3 * a + 1]) / (R0 * R0);
+      for (int i = 0; i < 3; i++) An[4 * a + i] = Yn[4 * a + i] = 0.0;
+      Yn[4 * a + 2] = peak * (1.0 - r2);
+    }
+    for (int a : wall)
+      for (int i = 0; i < 3; i++) An[4 * a + i] = Yn[4 * a + i] = 0.0;
+    DeviceArray<double> dAo, dYo, dAn, dYn, dAg((size_t)4 * nNo), dYg((size_t)4 * nNo), dR((size_t)4 * nNo),
+        dVal((size_t)16 * nnz);
+    dAo.upload(Ao);
+    dYo.upload(Yo);
+    dAn.upload(An);
+    dYn.upload(Yn);
+
+    FSILS_lsType ls;
+    fsils_ls_create(ls, LinearSolverType::LS_TYPE_GMRES);
+    ls.RI.relTol = relTol;
+    ls.RI.absTol = 1e-30;
+    ls.RI.mItr = mItr;
+    ls.RI.sD = sD;
+
+    // Newton loop with picc's convergence test (minItr = maxItr = 3, tol 0.5)
+    const int minIt = 3, maxIt = 3;
+    const double tol = 0.5;
+    double eqINorm = 0.0, pNorm = DBL_MAX;
+    const std::vector<int> incL(2, 1);
+    const std::vector<double> res(2, 0.0);
+    std::printf("{\"nNo\": %d, \"nEl\": %d, \"nnz\": %d, \"colours\": %d}\n", nNo, nEl, nnz, msh.nColors);
+    for (int itr = 1;; itr++) {
+      pici(nNo, p.am, p.af, dAo.data(), dAn.data(), dYo.data(), dYn.data(), dAg.data(), dYg.data());
+      check(svg_fill(dR.data(), dR.size(), 0.0, nullptr));
+      check(svg_fill(dVal.data(), dVal.size(), 0.0, nullptr));
+      construct_fluid(msh, p, dAg.data(), dYg.data(), nullptr, dR.data(), dVal.data());
+      fsils_solve(lhs, ls, 4, dR.data(), dVal.data(), PreconditionerType::PREC_FSILS, incL, res);
+      picc_update(nNo, p.gam, p.dt, dR.data(), dAn.data(), dYn.data());
+      if (is_zero(ls.RI.iNorm)) ls.RI.iNorm = DBL_EPSILON;
+      if (is_zero(eqINorm)) eqINorm = ls.RI.iNorm;
+      if (itr == 1) pNorm = ls.RI.iNorm / eqINorm;
+      const double r1 = ls.RI.iNorm / eqINorm;
+      const bool ok = itr >= maxIt || ((r1 <= tol || r1 <= tol * pNorm) && itr >= minIt);
+      std::printf("{\"itr\": %d, \"ls_itr\": %d, \"iNorm\": %.17g, \"fNorm\": %.17g, \"suc\": %d, \"r1\": %.17g}\n", itr,
+                  ls.RI.itr, ls.RI.iNorm, ls.RI.fNorm, ls.RI.suc, r1);
+      std::fflush(stdout);
+      if (ok) break;
+    }
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "%s\n", e.what());
+    return 1;
+  }
+  return 0;
+}

@@ -1,0 +1,87 @@
+// C++ host API of the fluid Newton step over the C ABI (see svg_fsils.hpp).
+#include "svg_fsils.hpp"
+
+namespace svg_host {
+
+void check(int rc)
+{
+  if (rc != SVG_OK) throw std::runtime_error(std::string("[svFSIplus-gpu] ") + svg_last_error());
+}
+
+FSILS_lhsType::~FSILS_lhsType()
+{
+  if (h) svg_lhs_free(h);
+}
+
+void fsils_lhs_create(FSILS_lhsType& lhs, int nNo, int nnz, const std::vector<int>& rowPtr,
+                      const std::vector<int>& colPtr, int nFaces)
+{
+  if ((int)rowPtr.size() != nNo + 1 || (int)colPtr.size() != nnz)
+    throw std::runtime_error("fsils_lhs_create: rowPtr/colPtr sizes do not match nNo/nnz");
+  if (lhs.h) fsils_lhs_free(lhs);
+  check(svg_lhs_create(nNo, nnz, rowPtr.data(), colPtr.data(), nFaces, &lhs.h));
+  lhs.nNo = nNo;
+  lhs.nnz = nnz;
+  lhs.nFaces = nFaces;
+}
+
+void fsils_bc_create(FSILS_lhsType& lhs, int faIn, int nNo, int dof, BcType BC_type, const std::vector<int>& gNodes,
+                     const std::vector<double>& Val)
+{
+  if ((int)gNodes.size() != nNo) throw std::runtime_error("fsils_bc_create: gNodes size differs from nNo");
+  std::vector<double> v = Val;
+  if (v.empty()) v.assign((size_t)dof * nNo, 0.0);
+  if (v.size() != (size_t)dof * nNo) throw std::runtime_error("fsils_bc_create: Val must be (dof, nNo)");
+  check(svg_lhs_bc_create(lhs.h, faIn, nNo, dof, (int)BC_type, gNodes.data(), v.data()));
+}
+
+void fsils_ls_create(FSILS_lsType& ls, LinearSolverType LS_type) { check(svg_ls_create(&ls, (int)LS_type)); }
+
+void fsils_solve(FSILS_lhsType& lhs, FSILS_lsType& ls, int dof, double* Ri, double* Val, PreconditionerType prec,
+                 const std::vector<int>& incL, const std::vector<double>& res, hipStream_t stream)
+{
+  check(svg_fsils_solve(lhs.h, &ls, dof, Ri, Val, (int)prec, (int)incL.size(), incL.empty() ? nullptr : incL.data(),
+                        res.empty() ? nullptr : res.data(), stream));
+}
+
+void fsils_lhs_free(FSILS_lhsType& lhs)
+{
+  if (lhs.h) svg_lhs_free(lhs.h);
+  lhs.h = nullptr;
+  lhs.nNo = lhs.nnz = lhs.nFaces = 0;
+}
+
+FluidMesh::~FluidMesh()
+{
+  if (h) svg_msh_free(h);
+}
+
+void fluid_mesh_create(FluidMesh& msh, const FSILS_lhsType& lhs, int nEl, const std::vector<int>& IEN,
+                       const std::vector<double>& x)
+{
+  if (IEN.size() != (size_t)4 * nEl || x.size() != (size_t)3 * lhs.nNo)
+    throw std::runtime_error("fluid_mesh_create: IEN must be (4,nEl) and x (3,nNo)");
+  if (msh.h) svg_msh_free(msh.h);
+  check(svg_msh_create(lhs.h, nEl, IEN.data(), lhs.nNo, x.data(), &msh.h));
+  msh.nEl = nEl;
+  check(svg_msh_num_colors(msh.h, &msh.nColors));
+}
+
+void construct_fluid(FluidMesh& msh, const svg_fluid_params& p, const double* Ag, const double* Yg, const double* Bf,
+                     double* R, double* Val, hipStream_t stream)
+{
+  check(svg_construct_fluid(msh.h, &p, Ag, Yg, Bf, R, Val, stream));
+}
+
+void pici(int nNo, double am, double af, const double* Ao, const double* An, const double* Yo, const double* Yn,
+          double* Ag, double* Yg, hipStream_t stream)
+{
+  check(svg_pici(nNo, am, af, Ao, An, Yo, Yn, Ag, Yg, stream));
+}
+
+void picc_update(int nNo, double gam, double dt, const double* R, double* An, double* Yn, hipStream_t stream)
+{
+  check(svg_picc_update(nNo, gam, dt, R, An, Yn, stream));
+}
+
+}  // namespace svg_host

@@ -119,17 +119,25 @@ def construct_fluid(msh: DeviceMesh, phys: Physics, Ag, Yg, R, Val, Bf=None):
 
 
 def fsils_solve(lhs: DeviceLhs, ls: _lib.Ls, dof: int, Ri, Val, prec: int = _lib.PREC_FSILS, incL=None, res=None,
-                precision: int = _lib.PRECISION_FP64):
-    """Reference signature: fsils_solve(lhs, ls, dof, Ri, Val, prec, incL, res) (solve.cpp:50);
-    precision = PRECISION_MIXED selects the FP32-SpMV / FP64-refined GMRES."""
+                precision: int = _lib.PRECISION_FP64, lpc=None, fallback_window: int = 0, info: dict | None = None):
+    """Reference signature: fsils_solve(lhs, ls, dof, Ri, Val, prec, incL, res) (solve.cpp:50).
+    precision = PRECISION_MIXED selects the FP32-SpMV / FP64-refined GMRES; a
+    network `lpc` (LearnedPC) selects the learned-preconditioned flexible GMRES
+    (info["fallback_itr"] = iteration count at the fallback, -1 if none)."""
     nf = lhs.nFaces
     inc = np.ones(max(1, nf), dtype=np.int32) if incL is None else np.ascontiguousarray(incL, dtype=np.int32)
     rs = None if res is None else np.ascontiguousarray(res, dtype=np.float64)
     opts = _lib.SolveOpts()
     opts.precision = precision
+    if lpc is not None:
+        opts.method = _lib.METHOD_LEARNED
+        opts.lpc = lpc.h.value if hasattr(lpc.h, "value") else lpc.h
+        opts.fallback_window = fallback_window
     _lib.check(_lib.load().svg_fsils_solve_ex(lhs.h, C.byref(ls), dof, _ptr(Ri), _ptr(Val), prec, nf,
                                               inc.ctypes.data, None if rs is None else rs.ctypes.data, C.byref(opts),
                                               _stream()))
+    if info is not None:
+        info["fallback_itr"] = opts.fallback_itr
     return ls
 
 
@@ -156,13 +164,14 @@ class FluidNewton:
     """One fluid equation on one pipe, all state resident in HBM."""
 
     def __init__(self, pipe: Pipe, phys: Physics, ls: _lib.Ls, eq: EqState | None = None, device="cuda",
-                 precision: int = _lib.PRECISION_FP64):
+                 precision: int = _lib.PRECISION_FP64, lpc=None):
         import torch
         if not torch.cuda.is_available():
             raise RuntimeError("FluidNewton needs a GPU (the HIP library has no CPU path)")
         self.torch = torch
         self.pipe, self.phys, self.ls = pipe, phys, ls
         self.precision = precision
+        self.lpc = lpc
         self.eq = eq or EqState()
         self.lhs = DeviceLhs(pipe)
         self.msh = DeviceMesh(self.lhs, pipe)
@@ -197,7 +206,8 @@ class FluidNewton:
         _lib.check(lib.svg_fill(_ptr(self.R), self.R.numel(), 0.0, s))
         _lib.check(lib.svg_fill(_ptr(self.Val), self.Val.numel(), 0.0, s))
         construct_fluid(self.msh, ph, self.Ag, self.Yg, self.R, self.Val)
-        fsils_solve(self.lhs, ls, 4, self.R, self.Val, precision=self.precision)
+        info = {}
+        fsils_solve(self.lhs, ls, 4, self.R, self.Val, precision=self.precision, lpc=self.lpc, info=info)
         _lib.check(lib.svg_picc_update(self.pipe.nNo, ph.gam, ph.dt, _ptr(self.R), _ptr(self.An), _ptr(self.Yn), s))
         # Newton convergence, pic.cpp:236-281
         eps = np.finfo(np.float64).eps
@@ -215,7 +225,7 @@ class FluidNewton:
         if l1 or ((l2 or l3) and l4):
             eq.ok = True
         self.history.append(dict(itr=eq.itr, ls_itr=ls.RI.itr, iNorm=ls.RI.iNorm, fNorm=ls.RI.fNorm, suc=ls.RI.suc,
-                                 r1=r1))
+                                 r1=r1, fallback_itr=info.get("fallback_itr", -1)))
         return eq.ok
 
     def time_step(self):

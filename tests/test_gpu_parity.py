@@ -262,3 +262,25 @@ def test_mixed_precision_solve(torch_cuda, phys, n):
           f"|x-x_fp64|/|x_fp64| {err:.2e}")
     assert ls.RI.suc == 1 and rel_res <= tol
     assert err <= 20 * tol
+
+
+def test_cpp_driver_matches_python_driver(torch_cuda):
+    """The C++ host driver (svg_newton_step, the reference's Newton loop over
+    the C++ host API) and the Python driver give the same Newton records."""
+    import json
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(_lib.LIB_PATH), "svg_newton_step")
+    r = subprocess.run([exe, "4", "1e-6", "50", "1000"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    recs = [json.loads(line) for line in r.stdout.splitlines()[1:]]
+    from svfsiplus_amd.newton import FluidNewton, ls_create
+    pipe = W.make_pipe(4)
+    ls = ls_create(_lib.LS_GMRES, relTol=1e-6, absTol=1e-30, maxItr=1000, dimKry=50)
+    hist = FluidNewton(pipe, W.Physics(), ls).time_step()
+    print("C++ driver", [(x["itr"], x["ls_itr"], x["iNorm"]) for x in recs])
+    print("Python driver", [(x["itr"], x["ls_itr"], x["iNorm"]) for x in hist])
+    assert len(recs) == len(hist)
+    for a, b in zip(recs, hist):
+        assert a["ls_itr"] == b["ls_itr"]
+        assert a["iNorm"] == b["iNorm"] and a["fNorm"] == b["fNorm"]

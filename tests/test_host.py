@@ -94,3 +94,13 @@ def test_stenosis_narrows_the_pipe():
     mid = np.abs(z - 1.25) < 1e-9
     assert np.allclose(r[mid], 0.15, rtol=1e-12)
     assert np.allclose(r[z < 0.99], 0.3, rtol=1e-12)
+
+
+def test_cpp_driver_built_and_usage():
+    """The C++ Newton-step driver links against the library and checks its
+    arguments without touching the GPU."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(_lib.LIB_PATH), "svg_newton_step")
+    assert os.path.exists(exe), "run python -m svfsiplus_amd.build"
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage" in r.stderr
