@@ -4,18 +4,28 @@
 Workload (default): the seeded 1.2M-tet VMS pipe (n = 34: 1 179 120 tets,
 209 475 nodes, 4 dof/node, 4x4 FP64 blocks), rho = 1.06, mu = 0.04, parabolic
 inflow with peak 30, no-slip walls, zero-traction outlet, dt = 1e-3, one time
-step of 3 Newton iterations; FSILS GMRES (Krylov dimension 50) with the
-diagonal preconditioner to a 1e-6 relative FP64 residual (absolute tolerance
-1e-30 so the relative test binds).
+step of 3 Newton iterations; GMRES (Krylov dimension 50) to a 1e-6 relative
+FP64 residual (absolute tolerance 1e-30 so the relative test binds).
 
-One bench "step" = one such time step from the same initial state (reset by
-device-to-device copies, inside the timed region); `value` is seconds per
-Newton step = elapsed / (3 * steps). Inputs are resident in HBM before timing.
+Methods: "plain" = the reference's FSILS GMRES with the diagonal
+preconditioner in FP64; "learned" = flexible GMRES with the learned
+preconditioner and the FP32 block SpMV (FP64 Gram-Schmidt, Givens and
+true-residual restarts, fallback to the diagonal preconditioner); "mixed" =
+plain GMRES with the FP32 SpMV. The default, "compare", times plain and
+learned back to back: `value` is the learned method's seconds per Newton step,
+`plain` and `speedup_vs_plain` stand beside it. The network is loaded from
+svfsiplus_amd/data/lpc_weights.npz (trained on the GPU by
+`python -m svfsiplus_amd.lpc`), or trained here within a bounded budget
+(outside the timed region) when that file is absent.
 
-Prints one JSON line with the metric, the roofline of the dominant kernel
-(HIP events on the launching stream over the timed region) and the CPU
-baseline (the single-core oracle restatement of the reference on a bounded
-sample of the same workload).
+One bench "step" = one time step from the same initial state (reset by
+device-to-device copies, inside the timed region); seconds per Newton step =
+elapsed / (Newton iterations). Inputs are resident in HBM before timing.
+
+Prints one JSON line with the metric, the roofline of the dominant kernel of
+the headline run (HIP events on the launching stream over the timed region)
+and the CPU baseline (the single-core oracle restatement of the reference on
+a bounded sample of the same workload).
 """
 from __future__ import annotations
 
@@ -31,7 +41,11 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 KERNELS = ["spmv_bcsr", "spmv32", "d2f", "mdot", "mdot_fin", "gs_update", "div", "backsolve", "xupdate", "pc_w", "pc_val",
-           "fluid_tet4", "pici", "picc", "fill", "wc_scale", "set_int", "spmv_tune"]
+           "fluid_tet4", "pici", "picc", "fill", "wc_scale", "set_int", "spmv_tune", "lpc_prep", "lpc_encode",
+           "lpc_msg", "lpc_dense"]
+# kernels whose launches carry algorithmic byte counts (roofline candidates)
+HBM_KERNELS = ["spmv_bcsr", "spmv32", "lpc_msg", "lpc_dense", "lpc_encode"]
+DEFAULT_WEIGHTS = os.path.join(ROOT, "svfsiplus_amd", "data", "lpc_weights.npz")
 
 
 def kernel_table(lib):
@@ -134,22 +148,94 @@ def kernel_only(a, fn, pipe, lib):
                       "copy_GBs": round(2 * src.numel() * 8 / copy_avg / 1e9, 1)}))
 
 
+def timed_run(fn, lib, steps, warmup, prof, label):
+    """Warmup, then `steps` time steps timed with a device synchronisation on
+    both sides; per-kernel event table and algorithmic bytes of the run."""
+    import ctypes as C
+    import torch
+    for w in range(warmup):
+        fn.reset()
+        h = fn.time_step()
+        print(f"# {label} warmup {w}: " + " ".join(f"[itr {x['itr']} ls {x['ls_itr']} iNorm {x['iNorm']:.3e}]"
+                                                 for x in h), file=sys.stderr, flush=True)
+    lib.svg_prof_reset()
+    lib.svg_prof_enable(1 if prof else 0)
+    hists = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn.reset()
+        hists.append([dict(x) for x in fn.time_step()])
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    lib.svg_prof_enable(0)
+    ktab = kernel_table(lib)
+    for k in HBM_KERNELS:
+        if k in ktab:
+            tb = C.c_double()
+            lib.svg_prof_get_bytes(k.encode(), C.byref(tb))
+            ktab[k]["bytes"] = tb.value
+    for k, h in enumerate(hists):
+        print(f"# {label} step {k}: " + " ".join(
+            f"[itr {x['itr']} ls {x['ls_itr']} iNorm {x['iNorm']:.3e} fNorm {x['fNorm']:.3e}"
+            + (f" fallback@{x['fallback_itr']}" if x.get('fallback_itr', -1) >= 0 else "") + "]" for x in h),
+            file=sys.stderr, flush=True)
+    n_newton = sum(len(h) for h in hists)
+    return {"elapsed": t1 - t0, "s_per_newton": (t1 - t0) / n_newton, "n_newton": n_newton, "hists": hists,
+            "kernels": ktab}
+
+
+def roofline(ktab):
+    """Roofline of the kernel with the largest total time among those with
+    algorithmic byte counts: achieved = bytes per launch / average duration."""
+    cands = [(v["total_ms"], k) for k, v in ktab.items() if k in HBM_KERNELS and v.get("bytes")]
+    if not cands:
+        return None
+    _, k = max(cands)
+    v = ktab[k]
+    avg_s = v["total_ms"] / v["count"] * 1e-3
+    bpl = v["bytes"] / v["count"]
+    ach = bpl / avg_s / 1e9
+    return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": k, "bytes_per_launch": round(bpl),
+            "avg_us": round(avg_s * 1e6, 2)}
+
+
+def get_network(a, log):
+    """The learned preconditioner: committed weights, or a bounded training run."""
+    from svfsiplus_amd import lpc as L
+    if a.lpc_weights and os.path.exists(a.lpc_weights):
+        return L.LearnedPC.load(a.lpc_weights), {"source": os.path.relpath(a.lpc_weights, ROOT)}
+    t0 = time.perf_counter()
+    data = L.build_training_set(a.train_draws, seed=11, k_max=8, log=log)
+    net = L.LearnedPC(seed=0)
+    hist = L.train(net, data, steps=a.train_steps, lr=2e-3, batch=8, log=log, log_every=500)
+    info = {"source": "trained in this run", "draws": a.train_draws,
+            "triples": int(sum(r.shape[0] for _, r, _ in data)), "steps": a.train_steps,
+            "final_loss": round(float(sum(hist[-100:]) / max(len(hist[-100:]), 1)), 5),
+            "train_s": round(time.perf_counter() - t0, 1)}
+    del data
+    return net, info
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=34, help="pipe cross-section cells (34 -> 1.2M tets)")
     ap.add_argument("--krylov", type=int, default=50)
     ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--method", choices=["compare", "plain", "mixed", "learned"], default="compare")
+    ap.add_argument("--lpc-weights", default=DEFAULT_WEIGHTS)
+    ap.add_argument("--train-draws", type=int, default=40)
+    ap.add_argument("--train-steps", type=int, default=1500)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-prof", action="store_true", help="time without per-kernel events")
     ap.add_argument("--kernel-only", choices=["spmv"], default=None,
                     help="launch only the block SpMV on the assembled system (for counter runs), "
                          "plus a device copy of the same byte count as the calibration kernel")
     ap.add_argument("--launches", type=int, default=20)
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
-                    help="fp64: the plain FSILS GMRES; mixed: FP32 block SpMV inside FP64-refined GMRES")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only (the multi-GPU sharding is a design, not built); "
@@ -160,6 +246,9 @@ def main():
     from svfsiplus_amd import _lib, workload as W
     from svfsiplus_amd.newton import FluidNewton, ls_create
 
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
     lib = _lib.load()
@@ -167,91 +256,69 @@ def main():
     pipe = W.make_pipe(a.n)
     phys = W.Physics()
     ls = ls_create(_lib.LS_GMRES, relTol=a.tol, absTol=1e-30, maxItr=1000, dimKry=a.krylov)
-    precision = _lib.PRECISION_MIXED if a.precision == "mixed" else _lib.PRECISION_FP64
-    fn = FluidNewton(pipe, phys, ls, precision=precision)
+    fn = FluidNewton(pipe, phys, ls)
     t_setup = time.perf_counter() - t_setup
-    print(f"# setup {t_setup:.1f}s: nNo={pipe.nNo} nEl={pipe.nEl} nnzb={pipe.nnz} colours={fn.msh.nColors}",
-          file=sys.stderr, flush=True)
+    log(f"# setup {t_setup:.1f}s: nNo={pipe.nNo} nEl={pipe.nEl} nnzb={pipe.nnz} colours={fn.msh.nColors}")
 
     if a.kernel_only:
         kernel_only(a, fn, pipe, lib)
         return
 
-    for w in range(a.warmup):
-        fn.reset()
-        h = fn.time_step()
-        print(f"# warmup {w}: " + " ".join(f"[itr {x['itr']} ls {x['ls_itr']} iNorm {x['iNorm']:.3e}]" for x in h),
-              file=sys.stderr, flush=True)
-
-    lib.svg_prof_reset()
-    lib.svg_prof_enable(0 if a.no_prof else 1)
-    hists = []
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(a.steps):
-        fn.reset()
-        hists.append([dict(x) for x in fn.time_step()])
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    lib.svg_prof_enable(0)
-    ktab = kernel_table(lib)
-    for k, h in enumerate(hists):
-        print(f"# step {k}: " + " ".join(f"[itr {x['itr']} ls {x['ls_itr']} iNorm {x['iNorm']:.3e} "
-                                          f"fNorm {x['fNorm']:.3e}]" for x in h), file=sys.stderr, flush=True)
-    n_newton = sum(len(h) for h in hists)
-    elapsed = t1 - t0
-    s_per_newton = elapsed / n_newton
-
-    # Roofline of the dominant kernel: the 4x4 block SpMV.
-    # Inside GMRES the SpMV is fused with the Gram-Schmidt dots of the Arnoldi
-    # step, so its algorithmic bytes grow with the step index; the library sums
-    # them per launch (132 B per block + 68 B per row + 32 B per row per basis
-    # vector read).
-    kname = "spmv32" if precision == _lib.PRECISION_MIXED else "spmv_bcsr"
-    sp = ktab.get(kname, {"count": 0, "total_ms": 0.0})
-    roof = None
-    if sp["count"]:
-        import ctypes as C
-        tb = C.c_double()
-        lib.svg_prof_get_bytes(kname.encode(), C.byref(tb))
-        avg_s = sp["total_ms"] / sp["count"] * 1e-3
-        bytes_per_launch = tb.value / sp["count"]
-        ach = bytes_per_launch / avg_s / 1e9
-        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kname,
-                "bytes_per_launch": round(bytes_per_launch), "avg_us": round(avg_s * 1e6, 2)}
-    total_kernel_ms = sum(v["total_ms"] for v in ktab.values())
+    runs = {}
+    net, net_info = None, None
+    if a.method in ("compare", "plain"):
+        fn.precision, fn.lpc = _lib.PRECISION_FP64, None
+        runs["plain"] = timed_run(fn, lib, a.steps, a.warmup, not a.no_prof, "plain")
+    if a.method == "mixed":
+        fn.precision, fn.lpc = _lib.PRECISION_MIXED, None
+        runs["mixed"] = timed_run(fn, lib, a.steps, a.warmup, not a.no_prof, "mixed")
+    if a.method in ("compare", "learned"):
+        net, net_info = get_network(a, log)
+        fn.precision, fn.lpc = _lib.PRECISION_MIXED, net
+        runs["learned"] = timed_run(fn, lib, a.steps, a.warmup, not a.no_prof, "learned")
+    head = "learned" if "learned" in runs else next(iter(runs))
+    R = runs[head]
 
     cpu = None
     if not a.no_cpu:
-        print("# cpu baseline ...", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(pipe, phys, [x for h in hists for x in h])
+        log("# cpu baseline ...")
+        base = runs.get("plain", R)
+        cpu = cpu_baseline(pipe, phys, [x for h in base["hists"] for x in h])
 
+    solver = {"plain": "FSILS GMRES (gmres_v), diagonal preconditioner, FP64",
+              "mixed": "GMRES with FP32 block SpMV, FP64 Gram-Schmidt/Givens, FP64 true-residual restarts, "
+                       "diagonal preconditioner",
+              "learned": "flexible GMRES, learned GNN preconditioner (3 layers, width 64, BF16/MFMA) after the "
+                         "diagonal scaling, FP32 block SpMV, FP64 Gram-Schmidt/Givens and true-residual restarts, "
+                         "fallback to the diagonal preconditioner after 20 non-decreasing iterations"}
     out = {
         "metric": "Newton-step s at FP64 rel-res 1e-6, 1.2M-tet VMS pipe: learned-PC vs plain",
-        "value": round(s_per_newton, 6),
+        "value": round(R["s_per_newton"], 6),
         "unit": "s/Newton-step",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": round(1e3 * elapsed / a.steps, 3),
+        "ms_per_step": round(1e3 * R["elapsed"] / a.steps, 3),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if head == "plain" else "f64+f32+bf16",
         "data": "synthetic (seeded pipe generator)",
         "config": {"workload": f"VMS pipe n={a.n}: {pipe.nEl} tets, {pipe.nNo} nodes, nnzb {pipe.nnz}; "
-                               f"1 time step x 3 Newton iterations; GMRES({a.krylov}) diag PC rel-tol {a.tol:g}",
-                   "solver": ("FSILS GMRES (gmres_v), diagonal preconditioner, FP64" if precision == _lib.PRECISION_FP64
-                              else "GMRES with FP32 block SpMV, FP64 Gram-Schmidt/Givens, FP64 true-residual "
-                                   "restarts, diagonal preconditioner"),
-                   "method": "plain" if precision == _lib.PRECISION_FP64 else "mixed",
-                   "newton_records": hists[-1]},
-        "roofline": roof,
+                               f"1 time step x 3 Newton iterations; GMRES({a.krylov}) rel-tol {a.tol:g} (FP64 "
+                               f"residual of the scaled system)",
+                   "method": head, "solver": solver[head], "newton_records": R["hists"][-1]},
+        "roofline": roofline(R["kernels"]),
         "cpu_baseline": cpu,
-        "kernels": ktab,
-        "kernel_ms_per_newton": round(total_kernel_ms / n_newton, 3),
+        "kernels": R["kernels"],
     }
+    if "plain" in runs and head != "plain":
+        P = runs["plain"]
+        out["plain"] = {"value": round(P["s_per_newton"], 6), "newton_records": P["hists"][-1],
+                        "roofline": roofline(P["kernels"])}
+        out["speedup_vs_plain"] = round(P["s_per_newton"] / R["s_per_newton"], 3)
+    if net_info:
+        out["learned_pc"] = net_info
     print(json.dumps(out))
 
 

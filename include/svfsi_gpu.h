@@ -82,6 +82,25 @@ typedef struct {
 int svg_pipe_counts(const svg_pipe_spec* sp, int* nNo, int* nEl);
 int svg_pipe_generate(const svg_pipe_spec* sp, double* x, int* IEN, int* flags);
 
+/* Bifurcating pipe (the batched workload): a parent pipe (n x n cross-section
+ * on the disc of radius `radius`, nzp layers over length_parent along z) that
+ * splits into two daughters of nzd layers over length_daughter. Daughter 0
+ * takes the parent's end half x <= 0 and turns by angle0 (radians) towards -x,
+ * daughter 1 the half x >= 0 and turns by angle1 towards +x, over the first
+ * max(3, nzd/3) layers; each keeps its D-shaped half section, scaled to its
+ * Murray radius (r0^3 = split r^3, r1^3 = (1 - split) r^3). n is a multiple of
+ * 4; interior nodes are jittered as in the pipe. Node flags: inlet (z = 0),
+ * wall (outer surfaces and the carina line), outlet (both daughter ends). */
+typedef struct {
+  int n, nzp, nzd;
+  double radius, length_parent, length_daughter;
+  double angle0, angle1, split, jitter;
+  unsigned long long seed;
+} svg_bif_spec;
+
+int svg_bif_counts(const svg_bif_spec* sp, int* nNo, int* nEl);
+int svg_bif_generate(const svg_bif_spec* sp, double* x, int* IEN, int* flags);
+
 /* Block-CSR pattern of a mesh (what lhsa builds, lhsa.cpp:153-380). */
 int svg_csr_pattern_count(int nNo, int nEl, int eNoN, const int* IEN, int* nnz);
 int svg_csr_pattern(int nNo, int nEl, int eNoN, const int* IEN, int* rowPtr, int* colPtr);
@@ -124,6 +143,22 @@ typedef struct {
  * caller zeroes them first, as ls_alloc does). d_Bf may be NULL (zero). */
 int svg_construct_fluid(svg_msh* msh, const svg_fluid_params* p, const double* d_Ag, const double* d_Yg,
                         const double* d_Bf, double* d_R, double* d_Val, void* stream);
+
+/* Neumann faces of the fluid mesh (faceType, ComMod.h: IEN, gE) as TRI3
+ * elements: faceIEN(3,nFe) node ids, gE(nFe) the parent element (index into
+ * the IEN given to svg_msh_create), which orients the outward normal as
+ * gnnb does (nn.cpp:552-760). */
+typedef struct svg_face svg_face;
+int svg_face_create(const svg_msh* msh, int nFe, const int* faceIEN, const int* gE, svg_face** face);
+int svg_face_free(svg_face* face);
+/* b_assem_neu_bc (eq_assem.cpp:58-172) for the fluid equation: b_fluid
+ * (fluid.cpp:46-133) on every face element — R -= the Neumann value d_hg (per
+ * node; NULL: zero traction) and the backflow stabilisation with coefficient
+ * beta (Backflow_stabilization_coefficient, default 0.2); Val gets the
+ * backflow tangent. qmTRI3: face quadrature modifier (default 2/3).
+ * Asynchronous on the stream. */
+int svg_b_assem_neu_bc(svg_face* face, const svg_fluid_params* p, double beta, double qmTRI3, const double* d_hg,
+                       const double* d_Yg, double* d_R, double* d_Val, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* FSILS solve (solve.cpp:50-193). Mirrors FSILS_subLsType / FSILS_lsType    */

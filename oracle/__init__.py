@@ -134,6 +134,26 @@ def construct_fluid(pipe, phys, Ag, Yg, Bf=None):
     return R, Val
 
 
+def b_assem_neu_fluid(pipe, phys, Yg, faceIEN, inner, R, Val, hg=None):
+    """Oracle b_assem_neu_bc on TRI3 faces (backflow stabilisation, Neumann value
+    hg per node or zero): accumulates into R (nNo,4) and Val (nnz,16) in place."""
+    lib = load()
+    x = np.ascontiguousarray(pipe.x)
+    Yg = np.ascontiguousarray(Yg, dtype=np.float64)
+    fi = np.ascontiguousarray(faceIEN, dtype=np.int32)
+    inn = np.ascontiguousarray(inner, dtype=np.int32)
+    h = None if hg is None else np.ascontiguousarray(hg, dtype=np.float64)
+    assert R.flags["C_CONTIGUOUS"] and Val.flags["C_CONTIGUOUS"]
+    prm = fluid_params(phys)
+    rc = lib.orc_b_assem_neu_fluid(C.c_int(fi.shape[0]), C.c_void_p(_p(fi)), C.c_void_p(_p(inn)), C.c_void_p(_p(x)),
+                                   C.c_void_p(None if h is None else _p(h)), C.c_void_p(_p(Yg)), C.byref(prm),
+                                   C.c_double(phys.beta), C.c_double(phys.qmTRI3), C.c_void_p(_p(pipe.rowPtr)),
+                                   C.c_void_p(_p(pipe.colPtr)), C.c_void_p(_p(R)), C.c_void_p(_p(Val)))
+    if rc:
+        raise RuntimeError(lib.orc_last_error().decode())
+    return R, Val
+
+
 def ref_construct_fluid(pipe, phys, Ag, Yg, Bf=None):
     """The reference's own construct_fluid (oracle/_ref)."""
     lib = load_ref()
@@ -273,9 +293,10 @@ def ge(A, B):
     return bool(ok), Bv
 
 
-def newton_time_step(pipe, phys, make_ls, minItr=3, maxItr=3, tol=0.5, state=None):
+def newton_time_step(pipe, phys, make_ls, minItr=3, maxItr=3, tol=0.5, state=None, outlet=None):
     """One time step of the oracle Newton loop (main.cpp:371-617 for a single
-    fluid equation): pici -> construct_fluid -> fsils_solve -> picc, with the
+    fluid equation): pici -> construct_fluid (+ the outlet's b_assem_neu_bc when
+    `outlet` = (faceIEN, gE, inner) is given) -> fsils_solve -> picc, with the
     convergence test of pic.cpp:236-281. make_ls() returns a fresh Ls.
     Returns (An, Yn, history)."""
     import sys as _sys
@@ -295,6 +316,8 @@ def newton_time_step(pipe, phys, make_ls, minItr=3, maxItr=3, tol=0.5, state=Non
         itr += 1
         Ag, Yg = pici(phys, Ao, An, Yo, Yn)
         R, V = construct_fluid(pipe, phys, Ag, Yg)
+        if outlet is not None:
+            b_assem_neu_fluid(pipe, phys, Yg, outlet[0], outlet[2], R, V)
         ls = make_ls()
         dx, _, ls = fsils_solve(pipe, ls, R, V)
         picc_update(phys, dx, An, Yn)

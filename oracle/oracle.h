@@ -97,6 +97,14 @@ int orc_construct_fluid(int nEl, const int* IEN, const double* x, const double* 
                         const double* Ag, const double* Yg, const orc_fluid_params* p,
                         const int* rowPtr, const int* colPtr, double* R, double* Val);
 
+/* Neumann-face assembly with backflow stabilisation on TRI3 faces
+ * (b_assem_neu_bc eq_assem.cpp:58-172 + gnnb nn.cpp:552-760 + b_fluid
+ * fluid.cpp:46-133): faceIEN(3,nFe), inner(nFe) = parent tet node off the
+ * face, hg(nNo) or NULL for zero traction, beta = backflow_stab. */
+int orc_b_assem_neu_fluid(int nFe, const int* faceIEN, const int* inner, const double* x, const double* hg,
+                          const double* Yg, const orc_fluid_params* p, double beta, double qmTRI3,
+                          const int* rowPtr, const int* colPtr, double* R, double* Val);
+
 /* Element-local residual/tangent of one element (for unit tests).
  * lR(4,4): lR[i + 4a]; lK(16,4,4): lK[k + 16*(a + 4b)]. */
 int orc_fluid_element(const double xl[3][4], const double al[4][4], const double yl[4][4],

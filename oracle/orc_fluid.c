@@ -530,3 +530,99 @@ int orc_construct_fluid(int nEl, const int* IEN, const double* x, const double* 
   }
   return 0;
 }
+
+/* Neumann-face assembly of the fluid equation on TRI3 faces of a TET4 mesh:
+ *   b_assem_neu_bc   Code/Source/solver/eq_assem.cpp:58-172 (one face, fluid physics)
+ *   gnnb             Code/Source/solver/nn.cpp:552-760 (non-shell branch: area-weighted
+ *                    normal (x0 - x2) x (x1 - x2), flipped to point from the interior node
+ *                    of the parent element towards face node 0)
+ *   TRI3 face rule   nn_elem_gip.h:745-762 (w = 1/6, s = qmTRI3, t = (1 - s)/2),
+ *                    shape functions nn_elem_gnn.h:1561-1572
+ *   b_fluid          Code/Source/solver/fluid.cpp:46-133 (fixed mesh, nsd = 3)
+ *   do_assem         Code/Source/solver/lhsa.cpp:97-141
+ * faceIEN(3,nFe); inner(nFe) = the parent tet's node off the face; hg(nNo) the
+ * Neumann value per node (NULL: zero traction). */
+int orc_b_assem_neu_fluid(int nFe, const int* faceIEN, const int* inner, const double* x, const double* hg,
+                          const double* Yg, const orc_fluid_params* p, double beta, double qmTRI3,
+                          const int* rowPtr, const int* colPtr, double* R, double* Val)
+{
+  const double s = qmTRI3, t = -0.5 * s + 0.5;
+  const double xi[3][2] = {{t, t}, {s, t}, {t, s}};
+  for (int e = 0; e < nFe; e++) {
+    int ptr[3];
+    double xl[3][3], yl[3][3], hl[3];
+    for (int a = 0; a < 3; a++) {
+      int Ac = faceIEN[a + 3 * e];
+      ptr[a] = Ac;
+      hl[a] = hg ? hg[Ac] : 0.0;
+      for (int i = 0; i < 3; i++) {
+        xl[i][a] = x[i + 3 * Ac];
+        yl[i][a] = Yg[i + 4 * Ac];
+      }
+    }
+    double lR[3][3] = {{0}}, lKd[3][3] = {{0}};  /* lK(0,a,b) = lK(5,a,b) = lK(10,a,b) */
+    for (int g = 0; g < 3; g++) {
+      double N[3] = {xi[g][0], xi[g][1], 1.0 - xi[g][0] - xi[g][1]};
+      /* gnnb: xXi(j,i) = sum_a Nx(i,a) x(j,a) with Nx = [1 0 -1; 0 1 -1] */
+      double X0[3], X1[3], n[3];
+      for (int j = 0; j < 3; j++) {
+        X0[j] = 0.0 + 1.0 * xl[j][0] + 0.0 * xl[j][1] + (-1.0) * xl[j][2];
+        X1[j] = 0.0 + 0.0 * xl[j][0] + 1.0 * xl[j][1] + (-1.0) * xl[j][2];
+      }
+      n[0] = X0[1] * X1[2] - X0[2] * X1[1];
+      n[1] = X0[2] * X1[0] - X0[0] * X1[2];
+      n[2] = X0[0] * X1[1] - X0[1] * X1[0];
+      int in = inner[e];
+      double v[3], nv = 0.0;
+      for (int i = 0; i < 3; i++) v[i] = xl[i][0] - x[i + 3 * in];
+      for (int i = 0; i < 3; i++) nv += n[i] * v[i];
+      if (nv < 0.0)
+        for (int i = 0; i < 3; i++) n[i] = -n[i];
+      double nn = 0.0;
+      for (int i = 0; i < 3; i++) nn += n[i] * n[i];
+      double Jac = sqrt(nn);
+      for (int i = 0; i < 3; i++) n[i] = n[i] / Jac;
+      double w = (1.0 / 6.0) * Jac;
+      double h = 0.0, y[3] = {0.0, 0.0, 0.0};
+      for (int a = 0; a < 3; a++) {
+        h = h + N[a] * hl[a];
+        for (int i = 0; i < 3; i++) y[i] = y[i] + N[a] * yl[i][a];
+      }
+      /* b_fluid */
+      double wl = w * p->af * p->gam * p->dt;
+      double udn = 0.0;
+      for (int i = 0; i < 3; i++) udn = udn + y[i] * n[i];
+      udn = 0.50 * beta * p->rho * (udn - fabs(udn));
+      double hc[3];
+      for (int i = 0; i < 3; i++) hc[i] = h * n[i] + udn * y[i];
+      for (int a = 0; a < 3; a++) {
+        for (int i = 0; i < 3; i++) lR[i][a] = lR[i][a] - w * N[a] * hc[i];
+        for (int b = 0; b < 3; b++) {
+          double T1 = wl * N[a] * N[b] * udn;
+          lKd[a][b] = lKd[a][b] - T1;
+        }
+      }
+    }
+    /* do_assem */
+    for (int a = 0; a < 3; a++) {
+      int rowN = ptr[a];
+      for (int i = 0; i < 3; i++) R[i + 4 * rowN] = R[i + 4 * rowN] + lR[i][a];
+      R[3 + 4 * rowN] = R[3 + 4 * rowN] + 0.0;
+      for (int b = 0; b < 3; b++) {
+        int colN = ptr[b];
+        int left = rowPtr[rowN];
+        int right = rowPtr[rowN + 1];
+        int q = (right + left) / 2;
+        while (colN != colPtr[q]) {
+          if (colN > colPtr[q]) left = q; else right = q;
+          q = (right + left) / 2;
+        }
+        for (int k = 0; k < 16; k++) {
+          double lk = (k == 0 || k == 5 || k == 10) ? lKd[a][b] : 0.0;
+          Val[k + 16 * q] = Val[k + 16 * q] + lk;
+        }
+      }
+    }
+  }
+  return 0;
+}

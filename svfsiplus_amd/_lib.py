@@ -38,6 +38,12 @@ class PipeSpec(C.Structure):
                 ("sten_z0", C.c_double), ("sten_len", C.c_double)]
 
 
+class BifSpec(C.Structure):
+    _fields_ = [("n", C.c_int), ("nzp", C.c_int), ("nzd", C.c_int), ("radius", C.c_double),
+                ("length_parent", C.c_double), ("length_daughter", C.c_double), ("angle0", C.c_double),
+                ("angle1", C.c_double), ("split", C.c_double), ("jitter", C.c_double), ("seed", C.c_ulonglong)]
+
+
 class FluidParams(C.Structure):
     _fields_ = [("rho", C.c_double), ("mu", C.c_double), ("f", C.c_double * 3), ("K_darcy", C.c_double),
                 ("af", C.c_double), ("am", C.c_double), ("gam", C.c_double), ("dt", C.c_double),
@@ -74,6 +80,8 @@ _SIGS = {
     "svg_last_error": (C.c_char_p, []),
     "svg_pipe_counts": (_I, [C.POINTER(PipeSpec), _IP, _IP]),
     "svg_pipe_generate": (_I, [C.POINTER(PipeSpec), _P, _P, _P]),
+    "svg_bif_counts": (_I, [C.POINTER(BifSpec), _IP, _IP]),
+    "svg_bif_generate": (_I, [C.POINTER(BifSpec), _P, _P, _P]),
     "svg_csr_pattern_count": (_I, [_I, _I, _I, _P, _IP]),
     "svg_csr_pattern": (_I, [_I, _I, _I, _P, _P, _P]),
     "svg_color_elements": (_I, [_I, _I, _I, _P, _P, _IP]),
@@ -84,6 +92,9 @@ _SIGS = {
     "svg_msh_free": (_I, [_P]),
     "svg_msh_num_colors": (_I, [_P, _IP]),
     "svg_construct_fluid": (_I, [_P, C.POINTER(FluidParams), _P, _P, _P, _P, _P, _P]),
+    "svg_face_create": (_I, [_P, _I, _P, _P, C.POINTER(_P)]),
+    "svg_face_free": (_I, [_P]),
+    "svg_b_assem_neu_bc": (_I, [_P, C.POINTER(FluidParams), _D, _D, _P, _P, _P, _P, _P]),
     "svg_ls_create": (_I, [C.POINTER(Ls), _I]),
     "svg_fsils_solve": (_I, [_P, C.POINTER(Ls), _I, _P, _P, _I, _I, _P, _P, _P]),
     "svg_fsils_solve_ex": (_I, [_P, C.POINTER(Ls), _I, _P, _P, _I, _I, _P, _P, C.POINTER(SolveOpts), _P]),
@@ -124,6 +135,12 @@ def load() -> C.CDLL:
         return _lib
     if not LIB_PATH.exists():
         raise RuntimeError(f"{LIB_PATH} is not built; run `python -m svfsiplus_amd.build`")
+    # Device memory and streams come from torch: load its HIP runtime first so
+    # that the library binds to the same runtime instance in this process.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(str(LIB_PATH))
     for name, (res, args) in _SIGS.items():
         fn = getattr(lib, name)

@@ -117,6 +117,153 @@ int pipe_generate(const svg_pipe_spec& sp, double* x, int* IEN, int* flags)
   return SVG_OK;
 }
 
+// ------------------------------------------------------------ bifurcation --
+// A parent pipe (n x n cross-section on the disc, nzp layers) splitting into
+// two daughters. Daughter d takes half of the parent's end cross-section
+// (columns i <= n/2 for d = 0, i >= n/2 for d = 1), keeps that D-shaped
+// section scaled towards its Murray radius r_d (r_0^3 = split r^3,
+// r_1^3 = (1 - split) r^3), and turns by its angle about the y axis over the
+// first Kt layers (d = 0 towards -x, d = 1 towards +x). The daughters' first
+// layer is the parent's last; their hexes use the parent's per-quadrant Kuhn
+// split by parent column, so the junction is conforming. Flags: inlet (parent
+// z = 0), wall (outer surfaces and the carina line), outlet (daughter ends).
+static void bif_layout(const svg_bif_spec& b, long long* nNo, long long* nEl)
+{
+  const long long n = b.n, half = n / 2;
+  *nNo = (n + 1) * (n + 1) * (b.nzp + 1) + 2LL * (half + 1) * (n + 1) * b.nzd;
+  *nEl = 6LL * n * n * b.nzp + 2LL * 6LL * half * n * b.nzd;
+}
+
+int bif_counts(const svg_bif_spec& b, int* nNo, int* nEl)
+{
+  if (b.n < 4 || (b.n % 4) != 0 || b.nzp < 1 || b.nzd < 2)
+    return set_error(SVG_ERR_ARG, "bifurcation: n must be a multiple of 4 >= 4, nzp >= 1, nzd >= 2");
+  if (!(b.split > 0.0 && b.split < 1.0)) return set_error(SVG_ERR_ARG, "bifurcation: split must be in (0, 1)");
+  long long nn, ne;
+  bif_layout(b, &nn, &ne);
+  if (nn > 0x7fffffffLL || ne > 0x7fffffffLL) return set_error(SVG_ERR_ARG, "bifurcation: too large");
+  *nNo = (int)nn;
+  *nEl = (int)ne;
+  return SVG_OK;
+}
+
+int bif_generate(const svg_bif_spec& b, double* x, int* IEN, int* flags)
+{
+  int nNo, nEl;
+  int rc = bif_counts(b, &nNo, &nEl);
+  if (rc) return rc;
+  const int n = b.n, half = n / 2, nzp = b.nzp, nzd = b.nzd;
+  const double pi = 3.14159265358979323846;
+  const double hs = 2.0 / n, hzp = b.length_parent / nzp, hzd = b.length_daughter / nzd;
+  const int Kt = std::max(3, nzd / 3);
+  SplitMix64 rng(b.seed);
+  const int np = (n + 1) * (n + 1) * (nzp + 1), nd = (half + 1) * (n + 1) * nzd;
+  auto pid = [&](int i, int j, int k) { return i + (n + 1) * (j + (n + 1) * k); };
+  // daughter d, local column ii in [0, half], layer k in [0, nzd] (k = 0: the parent's last layer)
+  auto did = [&](int d, int ii, int j, int k) {
+    if (k == 0) return pid(ii + d * half, j, nzp);
+    return np + d * nd + ii + (half + 1) * (j + (n + 1) * (k - 1));
+  };
+  auto disc = [&](int i, int j, double& X, double& Y) {
+    const double s = -1.0 + hs * i, t = -1.0 + hs * j;
+    X = b.radius * s * std::sqrt(1.0 - 0.5 * t * t);
+    Y = b.radius * t * std::sqrt(1.0 - 0.5 * s * s);
+  };
+  for (int k = 0; k <= nzp; k++)
+    for (int j = 0; j <= n; j++)
+      for (int i = 0; i <= n; i++) {
+        const bool wall = (i == 0 || i == n || j == 0 || j == n) || (k == nzp && i == half);
+        const bool inlet = (k == 0);
+        double X, Y, z = hzp * k;
+        disc(i, j, X, Y);
+        if (!wall && !inlet) {
+          X += b.jitter * hs * b.radius * rng.sym();
+          Y += b.jitter * hs * b.radius * rng.sym();
+          if (k < nzp) z += b.jitter * hzp * rng.sym();
+        }
+        const int a = pid(i, j, k);
+        x[3 * a + 0] = X;
+        x[3 * a + 1] = Y;
+        x[3 * a + 2] = z;
+        if (flags) flags[a] = (inlet ? SVG_NODE_INLET : 0) | (wall ? SVG_NODE_WALL : 0);
+      }
+  for (int d = 0; d < 2; d++) {
+    const double sg = d == 0 ? -1.0 : 1.0;
+    const double rd = std::cbrt(d == 0 ? b.split : 1.0 - b.split) * b.radius;
+    const double rho_d = rd / b.radius;
+    const double th_d = d == 0 ? b.angle0 : b.angle1;
+    const double xc = sg * 4.0 * b.radius / (3.0 * pi);  // centroid of the half disc
+    double o[3] = {xc, 0.0, b.length_parent};
+    for (int k = 1; k <= nzd; k++) {
+      const double thm = th_d * std::min(1.0, (k - 0.5) / Kt), th = th_d * std::min(1.0, (double)k / Kt);
+      o[0] += hzd * sg * std::sin(thm);
+      o[2] += hzd * std::cos(thm);
+      const double rho = 1.0 + (rho_d - 1.0) * std::min(1.0, (double)k / Kt);
+      const double u[3] = {std::cos(th), 0.0, -sg * std::sin(th)};
+      const double dir[3] = {sg * std::sin(th), 0.0, std::cos(th)};
+      for (int j = 0; j <= n; j++)
+        for (int ii = 0; ii <= half; ii++) {
+          const bool wall = (ii == 0 || ii == half || j == 0 || j == n);
+          const bool outlet = (k == nzd);
+          double X, Y;
+          disc(ii + d * half, j, X, Y);
+          double qx = rho * (X - xc), qy = rho * Y, qz = 0.0;
+          if (!wall && !outlet) {
+            const double hl = b.jitter * hs * rd;
+            qx += hl * rng.sym();
+            qy += hl * rng.sym();
+            qz += b.jitter * hzd * rng.sym();
+          }
+          const int a = did(d, ii, j, k);
+          for (int c = 0; c < 3; c++) x[3 * a + c] = o[c] + qx * u[c] + qz * dir[c] + (c == 1 ? qy : 0.0);
+          if (flags) flags[a] = (wall ? SVG_NODE_WALL : 0) | (outlet ? SVG_NODE_OUTLET : 0);
+        }
+    }
+  }
+  // Six Kuhn tets per hex, mirrored per parent quadrant (see pipe_generate).
+  static const int perm[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+  int e = 0;
+  auto emit = [&](auto&& corner) {
+    for (int p = 0; p < 6; p++) {
+      int bb[3] = {0, 0, 0};
+      int v[4];
+      v[0] = corner(0, 0, 0);
+      for (int q = 0; q < 3; q++) {
+        bb[perm[p][q]] = 1;
+        v[q + 1] = corner(bb[0], bb[1], bb[2]);
+      }
+      const double* x0 = x + 3 * v[0];
+      double dd[3][3];
+      for (int q = 0; q < 3; q++)
+        for (int c = 0; c < 3; c++) dd[q][c] = x[3 * v[q + 1] + c] - x0[c];
+      const double det = dd[0][0] * (dd[1][1] * dd[2][2] - dd[1][2] * dd[2][1]) -
+                         dd[0][1] * (dd[1][0] * dd[2][2] - dd[1][2] * dd[2][0]) +
+                         dd[0][2] * (dd[1][0] * dd[2][1] - dd[1][1] * dd[2][0]);
+      if (det < 0) std::swap(v[2], v[3]);
+      for (int a = 0; a < 4; a++) IEN[4 * e + a] = v[a];
+      e++;
+    }
+  };
+  for (int k = 0; k < nzp; k++)
+    for (int j = 0; j < n; j++)
+      for (int i = 0; i < n; i++) {
+        const bool fx = (i < half), fy = (j < half);
+        emit([&](int bx, int by, int bz) {
+          return pid(i + (fx ? 1 - bx : bx), j + (fy ? 1 - by : by), k + bz);
+        });
+      }
+  for (int d = 0; d < 2; d++)
+    for (int k = 0; k < nzd; k++)
+      for (int j = 0; j < n; j++)
+        for (int ii = 0; ii < half; ii++) {
+          const bool fx = (ii + d * half < half), fy = (j < half);
+          emit([&](int bx, int by, int bz) {
+            return did(d, ii + (fx ? 1 - bx : bx), j + (fy ? 1 - by : by), k + bz);
+          });
+        }
+  return SVG_OK;
+}
+
 // Node -> element incidence in CSR form.
 static void node_elements(int nNo, int nEl, int eNoN, const int* IEN, std::vector<int>& ptr, std::vector<int>& lst)
 {
@@ -212,6 +359,18 @@ int svg_pipe_generate(const svg_pipe_spec* sp, double* x, int* IEN, int* flags)
 {
   if (!sp || !x || !IEN) return svg::set_error(SVG_ERR_ARG, "svg_pipe_generate: null argument");
   return svg::pipe_generate(*sp, x, IEN, flags);
+}
+
+int svg_bif_counts(const svg_bif_spec* sp, int* nNo, int* nEl)
+{
+  if (!sp || !nNo || !nEl) return svg::set_error(SVG_ERR_ARG, "svg_bif_counts: null argument");
+  return svg::bif_counts(*sp, nNo, nEl);
+}
+
+int svg_bif_generate(const svg_bif_spec* sp, double* x, int* IEN, int* flags)
+{
+  if (!sp || !x || !IEN) return svg::set_error(SVG_ERR_ARG, "svg_bif_generate: null argument");
+  return svg::bif_generate(*sp, x, IEN, flags);
 }
 
 int svg_csr_pattern_count(int nNo, int nEl, int eNoN, const int* IEN, int* nnz)

@@ -324,6 +324,85 @@ __global__ __launch_bounds__(128) void k_fluid_tet4(int q0, int qn, const int4* 
 
 __global__ void k_set_int(int* p, int v) { *p = v; }
 
+// Neumann-face assembly of the fluid equation on TRI3 faces, one face element
+// per lane, launched per face colour: b_assem_neu_bc (eq_assem.cpp:58-172)
+// with gnnb's outward area normal (nn.cpp:552-760) and b_fluid
+// (fluid.cpp:46-133, fixed mesh): R(i,a) -= w N_a (h n_i + udn u_i),
+// Val(4i+i, a b) -= wl N_a N_b udn, udn = beta rho (u.n - |u.n|)/2.
+struct FaceConst {
+  double rho, beta, af, gam, dt;
+  double N[3][3];         // N[g][a] at the three TRI3 Gauss points
+};
+
+__global__ __launch_bounds__(128) void k_b_fluid_tri3(int q0, int qn, const int4* __restrict__ ien,
+                                                      const int* __restrict__ slot, int nFePad,
+                                                      const double* __restrict__ x, const double* __restrict__ hg,
+                                                      const double* __restrict__ Yg, FaceConst C,
+                                                      double* __restrict__ R, double* __restrict__ Val)
+{
+#pragma clang fp contract(off)
+  const int q = q0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= q0 + qn) return;
+  const int4 nd = ien[q];
+  const int id[3] = {nd.x, nd.y, nd.z};
+  double xl[3][3], yl[3][3], hl[3];
+  for (int a = 0; a < 3; a++) {
+    for (int i = 0; i < 3; i++) {
+      xl[i][a] = x[3 * (size_t)id[a] + i];
+      yl[i][a] = Yg[4 * (size_t)id[a] + i];
+    }
+    hl[a] = hg ? hg[id[a]] : 0.0;
+  }
+  // gnnb: (x0 - x2) x (x1 - x2), oriented away from the interior node
+  double X0[3], X1[3], n[3];
+  for (int j = 0; j < 3; j++) {
+    X0[j] = 0.0 + 1.0 * xl[j][0] + 0.0 * xl[j][1] + (-1.0) * xl[j][2];
+    X1[j] = 0.0 + 0.0 * xl[j][0] + 1.0 * xl[j][1] + (-1.0) * xl[j][2];
+  }
+  n[0] = X0[1] * X1[2] - X0[2] * X1[1];
+  n[1] = X0[2] * X1[0] - X0[0] * X1[2];
+  n[2] = X0[0] * X1[1] - X0[1] * X1[0];
+  double nv = 0.0;
+  for (int i = 0; i < 3; i++) nv += n[i] * (xl[i][0] - x[3 * (size_t)nd.w + i]);
+  if (nv < 0.0)
+    for (int i = 0; i < 3; i++) n[i] = -n[i];
+  double nn = 0.0;
+  for (int i = 0; i < 3; i++) nn += n[i] * n[i];
+  const double Jac = sqrt(nn);
+  for (int i = 0; i < 3; i++) n[i] = n[i] / Jac;
+  const double w = (1.0 / 6.0) * Jac;
+  const double wl = w * C.af * C.gam * C.dt;
+  double lR[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+  double lK[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+  for (int g = 0; g < 3; g++) {
+    const double* N = C.N[g];
+    double h = 0.0, y[3] = {0.0, 0.0, 0.0};
+    for (int a = 0; a < 3; a++) {
+      h = h + N[a] * hl[a];
+      for (int i = 0; i < 3; i++) y[i] = y[i] + N[a] * yl[i][a];
+    }
+    double udn = 0.0;
+    for (int i = 0; i < 3; i++) udn = udn + y[i] * n[i];
+    udn = 0.50 * C.beta * C.rho * (udn - fabs(udn));
+    double hc[3];
+    for (int i = 0; i < 3; i++) hc[i] = h * n[i] + udn * y[i];
+    for (int a = 0; a < 3; a++) {
+      for (int i = 0; i < 3; i++) lR[i][a] = lR[i][a] - w * N[a] * hc[i];
+      for (int b = 0; b < 3; b++) lK[a][b] = lK[a][b] - wl * N[a] * N[b] * udn;
+    }
+  }
+  for (int a = 0; a < 3; a++) {
+    double* r = R + 4 * (size_t)id[a];
+    for (int i = 0; i < 3; i++) r[i] = r[i] + lR[i][a];
+    for (int b = 0; b < 3; b++) {
+      double* v = Val + 16 * (size_t)slot[(size_t)(3 * a + b) * nFePad + q];
+      v[0] = v[0] + lK[a][b];
+      v[5] = v[5] + lK[a][b];
+      v[10] = v[10] + lK[a][b];
+    }
+  }
+}
+
 }  // namespace svg
 
 using namespace svg;
@@ -372,3 +451,106 @@ extern "C" int svg_construct_fluid(svg_msh* M, const svg_fluid_params* p, const 
   }
   return SVG_OK;
 }
+
+extern "C" int svg_face_create(const svg_msh* M, int nFe, const int* faceIEN, const int* gE, svg_face** out)
+{
+  if (!M || !faceIEN || !gE || !out || nFe <= 0) return set_error(SVG_ERR_ARG, "svg_face_create: bad argument");
+  const svg_lhs* L = M->lhs;
+  std::vector<int> ien4((size_t)4 * nFe);
+  for (int e = 0; e < nFe; e++) {
+    const int E = gE[e];
+    if (E < 0 || E >= M->nEl) return set_error(SVG_ERR_ARG, "svg_face_create: parent element out of range");
+    // the parent's node off the face (gnnb, nn.cpp:586-624)
+    int inner = -1, found = 0;
+    for (int b = 0; b < 4; b++) {
+      const int Bc = M->ien[4 * (size_t)E + b];
+      bool on = false;
+      for (int a = 0; a < 3; a++) on = on || (faceIEN[3 * (size_t)e + a] == Bc);
+      if (on)
+        found++;
+      else
+        inner = Bc;
+    }
+    if (found != 3 || inner < 0)
+      return set_error(SVG_ERR_ARG, "[svFSIplus::gnnb] a face node could not be matched to a node of its element");
+    for (int a = 0; a < 3; a++) ien4[4 * (size_t)e + a] = faceIEN[3 * (size_t)e + a];
+    ien4[4 * (size_t)e + 3] = inner;
+  }
+  std::vector<int> color, slots;
+  int nColors = 0;
+  int rc = color_elements(M->nNo, nFe, 3, faceIEN, color, &nColors);
+  if (rc) return rc;
+  if ((rc = slot_map(nFe, 3, faceIEN, L->rowPtr.data(), L->colPtr.data(), slots))) return rc;
+  std::vector<int> off(nColors + 1, 0), order(nFe);
+  for (int e = 0; e < nFe; e++) off[color[e] + 1]++;
+  for (int c = 0; c < nColors; c++) off[c + 1] += off[c];
+  {
+    std::vector<int> fill(off.begin(), off.end() - 1);
+    for (int e = 0; e < nFe; e++) order[fill[color[e]]++] = e;
+  }
+  auto* Fa = new svg_face();
+  Fa->msh = M;
+  Fa->nFe = nFe;
+  Fa->nColors = nColors;
+  Fa->colorOff = off;
+  Fa->nFePad = (nFe + 63) & ~63;
+  std::vector<int> ie((size_t)4 * nFe), sl((size_t)9 * Fa->nFePad, 0);
+  for (int q = 0; q < nFe; q++) {
+    const int e = order[q];
+    for (int a = 0; a < 4; a++) ie[4 * (size_t)q + a] = ien4[4 * (size_t)e + a];
+    for (int p = 0; p < 9; p++) sl[(size_t)p * Fa->nFePad + q] = slots[9 * (size_t)e + p];
+  }
+  if ((rc = dev_alloc(&Fa->d_ien, ie.size())) || (rc = dev_alloc(&Fa->d_slot, sl.size()))) {
+    svg_face_free(Fa);
+    return rc;
+  }
+  hipError_t e = hipMemcpy(Fa->d_ien, ie.data(), sizeof(int) * ie.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(Fa->d_slot, sl.data(), sizeof(int) * sl.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    svg_face_free(Fa);
+    return hip_check(e, "svg_face_create: upload");
+  }
+  *out = Fa;
+  return SVG_OK;
+}
+
+extern "C" int svg_face_free(svg_face* Fa)
+{
+  if (!Fa) return SVG_OK;
+  dev_free(Fa->d_ien);
+  dev_free(Fa->d_slot);
+  delete Fa;
+  return SVG_OK;
+}
+
+extern "C" int svg_b_assem_neu_bc(svg_face* Fa, const svg_fluid_params* p, double beta, double qmTRI3,
+                                  const double* hg, const double* Yg, double* R, double* Val, void* stream)
+{
+  if (!Fa || !p || !Yg || !R || !Val) return set_error(SVG_ERR_ARG, "svg_b_assem_neu_bc: null argument");
+  hipStream_t s = (hipStream_t)stream;
+  FaceConst C;
+  C.rho = p->rho;
+  C.beta = beta;
+  C.af = p->af;
+  C.gam = p->gam;
+  C.dt = p->dt;
+  // TRI3 face rule (nn_elem_gip.h:745-762) and shape functions (nn_elem_gnn.h:1561-1572)
+  const double sq = qmTRI3, t = -0.5 * sq + 0.5;
+  const double xi[3][2] = {{t, t}, {sq, t}, {t, sq}};
+  for (int g = 0; g < 3; g++) {
+    C.N[g][0] = xi[g][0];
+    C.N[g][1] = xi[g][1];
+    C.N[g][2] = 1.0 - xi[g][0] - xi[g][1];
+  }
+  const svg_msh* M = Fa->msh;
+  for (int c = 0; c < Fa->nColors; c++) {
+    const int q0 = Fa->colorOff[c], qn = Fa->colorOff[c + 1] - q0;
+    if (qn <= 0) continue;
+    launch("b_fluid", k_b_fluid_tri3, dim3(blocks_for(qn, 128)), dim3(128), 0, s, q0, qn,
+           reinterpret_cast<const int4*>(Fa->d_ien), (const int*)Fa->d_slot, Fa->nFePad, (const double*)M->d_x, hg, Yg,
+           C, R, Val);
+  }
+  SVG_HIP(hipGetLastError());
+  return SVG_OK;
+}
+

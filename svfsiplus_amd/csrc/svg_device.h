@@ -191,4 +191,15 @@ struct svg_msh {
   int* d_err = nullptr;        // zero-Jacobian flag (element index + 1)
   int* h_err = nullptr;        // pinned mirror
   int nElPad = 0;
+  std::vector<int> ien;        // (4, nEl) in the caller's element order (face -> parent lookups)
+};
+
+// A TRI3 Neumann face of a fluid mesh (faceType: IEN, gE), colour-sorted.
+struct svg_face {
+  const svg_msh* msh = nullptr;
+  int nFe = 0, nColors = 0;
+  std::vector<int> colorOff;
+  int* d_ien = nullptr;    // (4, nFe): the 3 face nodes and the parent element's interior node
+  int* d_slot = nullptr;   // 9 slots per face element, SoA: slot[p*nFePad + e]
+  int nFePad = 0;
 };

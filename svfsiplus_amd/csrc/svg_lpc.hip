@@ -31,6 +31,7 @@ using svg::blocks_for;
 using svg::dev_alloc;
 using svg::dev_free;
 using svg::launch;
+using svg::launch_b;
 using svg::set_error;
 
 namespace svg {
@@ -168,73 +169,99 @@ __global__ __launch_bounds__(BS) void k_encode(int n, int rows, const double* __
     }
 }
 
-// One message-passing layer on a tile of 64 node rows: messages gathered
-// through the BF16 block matrix (lane = 16 * row-in-group + channel, four rows
-// per pass), [h | m] staged in LDS, the 64 x 128 layer on MFMA, residual ReLU
-// update. Training stores m and p; the last layer also decodes z.
-__global__ __launch_bounds__(BS) void k_layer(int n, int rows, const int* __restrict__ rowPtr,
-                                              const int* __restrict__ colPtr, const bf16* __restrict__ A16,
-                                              const bf16* __restrict__ Hin, bf16* __restrict__ Hout,
-                                              const bf16* __restrict__ W, const float* __restrict__ bias,
-                                              bf16* __restrict__ Mst, bf16* __restrict__ Pst,
+// Messages of one layer: M[r] = sum_{j in row i} A_ij (x) H[j] through the
+// BF16 block matrix, one lane per (row, channel) computing the channel's 4
+// dofs, neighbours in row order (FP32 accumulation), four neighbours' loads
+// issued together. No LDS, so the gather runs at full occupancy.
+template <int UNR>
+__global__ __launch_bounds__(256) void k_msg(int n, int rows, const int* __restrict__ rowPtr,
+                                             const int* __restrict__ colPtr, const bf16* __restrict__ A16,
+                                             const bf16* __restrict__ H, bf16* __restrict__ M,
+                                             const int* __restrict__ skip)
+{
+  if (skip && *skip) return;
+  const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+  const int r = (int)(idx >> 4), c = (int)(idx & 15);
+  if (r >= rows) return;
+  const int b = r / n, i = r - b * n;
+  const bf16* Hb = H + (size_t)b * n * F + 4 * c;
+  const int s = rowPtr[i], e = rowPtr[i + 1];
+  float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f;
+  auto acc = [&](const uint4& a0, const uint4& a1, const uint2& hv) {
+    const float h0 = lo(hv.x), h1 = hi(hv.x), h2 = lo(hv.y), h3 = hi(hv.y);
+    m0 = fmaf(lo(a0.x), h0, m0);
+    m0 = fmaf(hi(a0.x), h1, m0);
+    m0 = fmaf(lo(a0.y), h2, m0);
+    m0 = fmaf(hi(a0.y), h3, m0);
+    m1 = fmaf(lo(a0.z), h0, m1);
+    m1 = fmaf(hi(a0.z), h1, m1);
+    m1 = fmaf(lo(a0.w), h2, m1);
+    m1 = fmaf(hi(a0.w), h3, m1);
+    m2 = fmaf(lo(a1.x), h0, m2);
+    m2 = fmaf(hi(a1.x), h1, m2);
+    m2 = fmaf(lo(a1.y), h2, m2);
+    m2 = fmaf(hi(a1.y), h3, m2);
+    m3 = fmaf(lo(a1.z), h0, m3);
+    m3 = fmaf(hi(a1.z), h1, m3);
+    m3 = fmaf(lo(a1.w), h2, m3);
+    m3 = fmaf(hi(a1.w), h3, m3);
+  };
+  int k = s;
+  for (; k + UNR <= e; k += UNR) {
+    int j[UNR];
+    uint4 a0[UNR], a1[UNR];
+    uint2 hv[UNR];
+#pragma unroll
+    for (int q = 0; q < UNR; q++) j[q] = colPtr[k + q];
+#pragma unroll
+    for (int q = 0; q < UNR; q++) {
+      a0[q] = reinterpret_cast<const uint4*>(A16 + 16 * (size_t)(k + q))[0];
+      a1[q] = reinterpret_cast<const uint4*>(A16 + 16 * (size_t)(k + q))[1];
+      hv[q] = *reinterpret_cast<const uint2*>(Hb + (size_t)j[q] * F);
+    }
+#pragma unroll
+    for (int q = 0; q < UNR; q++) acc(a0[q], a1[q], hv[q]);
+  }
+  for (; k < e; k++) {
+    const uint4 a0 = reinterpret_cast<const uint4*>(A16 + 16 * (size_t)k)[0];
+    const uint4 a1 = reinterpret_cast<const uint4*>(A16 + 16 * (size_t)k)[1];
+    const uint2 hv = *reinterpret_cast<const uint2*>(Hb + (size_t)colPtr[k] * F);
+    acc(a0, a1, hv);
+  }
+  uint2 pk;
+  pk.x = (unsigned)f2bf(m0) | ((unsigned)f2bf(m1) << 16);
+  pk.y = (unsigned)f2bf(m2) | ((unsigned)f2bf(m3) << 16);
+  *reinterpret_cast<uint2*>(M + (size_t)r * F + 4 * c) = pk;
+}
+
+// Dense part of one layer on a tile of 64 node rows: [h | m] and W staged in
+// LDS, the 64 x 128 x 64 product on MFMA, residual ReLU update; training
+// stores p; the last layer also decodes z = v + (Wd h) / sqrt(nNo).
+__global__ __launch_bounds__(BS) void k_dense(int rows, const bf16* __restrict__ Hin, const bf16* __restrict__ M,
+                                              bf16* __restrict__ Hout, const bf16* __restrict__ W,
+                                              const float* __restrict__ bias, bf16* __restrict__ Pst,
                                               const bf16* __restrict__ Wd, const double* __restrict__ V,
                                               double* __restrict__ Z, float* __restrict__ Z32, double inv_sc,
                                               const int* __restrict__ skip)
 {
-  __shared__ bf16 X[TILE * XS];
-  __shared__ bf16 Ws[F * XS];
+  __shared__ __align__(16) bf16 X[TILE * XS];
+  __shared__ __align__(16) bf16 Ws[F * XS];
   if (skip && *skip) return;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int r0 = blockIdx.x * TILE;
-  for (int q = t; q < F * 2 * F; q += BS) Ws[(q >> 7) * XS + (q & 127)] = W[q];
-  for (int q = t; q < TILE * F; q += BS) {
-    const int row = q >> 6, f = q & 63, r = r0 + row;
-    X[row * XS + f] = r < rows ? Hin[(size_t)r * F + f] : (bf16)0;
+  for (int q = t; q < F * 2 * F / 8; q += BS) {  // 8 bf16 per load
+    const int o = (q >> 4), k8 = (q & 15) * 8;
+    *reinterpret_cast<uint4*>(Ws + o * XS + k8) = reinterpret_cast<const uint4*>(W)[q];
   }
-  const int c = lane & 15, sub = lane >> 4;
-  for (int pass = 0; pass < 4; pass++) {
-    const int row = wave * 16 + pass * 4 + sub, r = r0 + row;
-    float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f;
+  for (int q = t; q < TILE * F / 8; q += BS) {
+    const int row = q >> 3, f8 = (q & 7) * 8, r = r0 + row;
+    uint4 hv = {0u, 0u, 0u, 0u}, mv = {0u, 0u, 0u, 0u};
     if (r < rows) {
-      const int b = r / n, i = r - b * n;
-      const bf16* Hb = Hin + (size_t)b * n * F + 4 * c;
-      const int s = rowPtr[i], e = rowPtr[i + 1];
-      for (int k = s; k < e; k++) {
-        const int j = colPtr[k];
-        const uint4 a0 = reinterpret_cast<const uint4*>(A16 + 16 * (size_t)k)[0];
-        const uint4 a1 = reinterpret_cast<const uint4*>(A16 + 16 * (size_t)k)[1];
-        const uint2 hv = *reinterpret_cast<const uint2*>(Hb + (size_t)j * F);
-        const float h0 = lo(hv.x), h1 = hi(hv.x), h2 = lo(hv.y), h3 = hi(hv.y);
-        m0 = fmaf(lo(a0.x), h0, m0);
-        m0 = fmaf(hi(a0.x), h1, m0);
-        m0 = fmaf(lo(a0.y), h2, m0);
-        m0 = fmaf(hi(a0.y), h3, m0);
-        m1 = fmaf(lo(a0.z), h0, m1);
-        m1 = fmaf(hi(a0.z), h1, m1);
-        m1 = fmaf(lo(a0.w), h2, m1);
-        m1 = fmaf(hi(a0.w), h3, m1);
-        m2 = fmaf(lo(a1.x), h0, m2);
-        m2 = fmaf(hi(a1.x), h1, m2);
-        m2 = fmaf(lo(a1.y), h2, m2);
-        m2 = fmaf(hi(a1.y), h3, m2);
-        m3 = fmaf(lo(a1.z), h0, m3);
-        m3 = fmaf(hi(a1.z), h1, m3);
-        m3 = fmaf(lo(a1.w), h2, m3);
-        m3 = fmaf(hi(a1.w), h3, m3);
-      }
+      hv = *reinterpret_cast<const uint4*>(Hin + (size_t)r * F + f8);
+      mv = *reinterpret_cast<const uint4*>(M + (size_t)r * F + f8);
     }
-    const bf16 q0 = f2bf(m0), q1 = f2bf(m1), q2 = f2bf(m2), q3 = f2bf(m3);
-    bf16* xm = X + row * XS + F + 4 * c;
-    xm[0] = q0;
-    xm[1] = q1;
-    xm[2] = q2;
-    xm[3] = q3;
-    if (Mst && r < rows) {
-      uint2 pk;
-      pk.x = (unsigned)q0 | ((unsigned)q1 << 16);
-      pk.y = (unsigned)q2 | ((unsigned)q3 << 16);
-      *reinterpret_cast<uint2*>(Mst + (size_t)r * F + 4 * c) = pk;
-    }
+    *reinterpret_cast<uint4*>(X + row * XS + f8) = hv;
+    *reinterpret_cast<uint4*>(X + row * XS + F + f8) = mv;
   }
   __syncthreads();
   v4f acc[4];
@@ -565,9 +592,9 @@ static int lpc_ensure(svg_lpc* N, int rows, bool train)
   int rc;
   for (auto& h : N->H)
     if ((rc = dev_alloc(&h, (size_t)cap * F))) return rc;
+  for (int l = 0; l < (train ? NL : 1); l++)
+    if ((rc = dev_alloc(&N->M[l], (size_t)cap * F))) return rc;
   if (train) {
-    for (auto& m : N->M)
-      if ((rc = dev_alloc(&m, (size_t)cap * F))) return rc;
     for (auto& p : N->P)
       if ((rc = dev_alloc(&p, (size_t)cap * F))) return rc;
     if ((rc = dev_alloc(&N->G, (size_t)cap * F)) || (rc = dev_alloc(&N->G2, (size_t)cap * F)) ||
@@ -621,17 +648,24 @@ static int lpc_forward(svg_lpc* N, const svg_lhs* L, int nvec, const double* V, 
   const double sc = std::sqrt((double)n);
   const unsigned nt = blocks_for(rows, TILE);
   const bf16* pb = N->pb;
-  launch("lpc_encode", k_encode, dim3(nt), dim3(BS), 0, s, n, rows, V, sc, (const bf16*)L->d_diag16, pb + OFF_W0,
-         N->H[0], skip);
+  // algorithmic bytes: V and the diagonal blocks read, h written
+  launch_b("lpc_encode", (32.0 + 32.0 + 2.0 * F) * rows, k_encode, dim3(nt), dim3(BS), 0, s, n, rows, V, sc,
+           (const bf16*)L->d_diag16, pb + OFF_W0, N->H[0], skip);
+  const unsigned nm = blocks_for((size_t)rows * 16, 256);
   for (int l = 0; l < NL; l++) {
     const bool last = l == NL - 1;
     const bf16* Hin = train ? N->H[l] : N->H[l & 1];
     bf16* Hout = train ? N->H[l + 1] : N->H[(l + 1) & 1];
-    launch("lpc_layer", k_layer, dim3(nt), dim3(BS), 0, s, n, rows, (const int*)L->d_rowPtr,
-           (const int*)L->d_colPtr, (const bf16*)L->d_val16, Hin, Hout, pb + OFF_L + l * LSZ,
-           (const float*)(N->p + OFF_L + l * LSZ + F * 2 * F), train ? N->M[l] : (bf16*)nullptr,
-           train ? N->P[l] : (bf16*)nullptr, last ? pb + OFF_WD : (const bf16*)nullptr, V, last ? Z : (double*)nullptr,
-           last ? Z32 : (float*)nullptr, 1.0 / sc, skip);
+    bf16* Ml = train ? N->M[l] : N->M[0];
+    // BF16 blocks + colPtr per block (per sample), rowPtr + h + m out per row
+    launch_b("lpc_msg", (32.0 + 4.0) * L->nnz * nvec + (4.0 + 2.0 * F) * rows, k_msg<4>, dim3(nm), dim3(256), 0, s, n,
+             rows, (const int*)L->d_rowPtr, (const int*)L->d_colPtr, (const bf16*)L->d_val16, Hin, Ml, skip);
+    // h, m in, h out (+ p for training; + V in, z out for the last layer)
+    const double db = 6.0 * F * rows + (train ? 2.0 * F * rows : 0.0) + (last ? (64.0 + (Z32 ? 16.0 : 0.0)) * rows : 0.0);
+    launch_b("lpc_dense", db, k_dense, dim3(nt), dim3(BS), 0, s, rows, Hin, (const bf16*)Ml, Hout,
+             pb + OFF_L + l * LSZ, (const float*)(N->p + OFF_L + l * LSZ + F * 2 * F),
+             train ? N->P[l] : (bf16*)nullptr, last ? pb + OFF_WD : (const bf16*)nullptr, V,
+             last ? Z : (double*)nullptr, last ? Z32 : (float*)nullptr, 1.0 / sc, skip);
   }
   SVG_HIP(hipGetLastError());
   return SVG_OK;

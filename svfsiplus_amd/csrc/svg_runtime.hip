@@ -319,6 +319,7 @@ int svg_msh_create(const svg_lhs* L, int nEl, const int* IEN, int nNo, const dou
   M->nColors = nColors;
   M->colorOff = off;
   M->nElPad = (nEl + 63) & ~63;
+  M->ien.assign(IEN, IEN + (size_t)4 * nEl);
   std::vector<int> ien((size_t)4 * nEl), sl((size_t)16 * M->nElPad, 0);
   for (int q = 0; q < nEl; q++) {
     int e = order[q];

@@ -1,7 +1,8 @@
 // One time step of the fluid equation on a seeded pipe, driven from C++ over
 // the host API (svg_fsils.hpp): the reference's Newton loop for one fluid
 // equation (Code/Source/solver/main.cpp:371-617) —
-//   pici -> ls_alloc (R = 0, Val = 0) -> construct_fluid -> fsils_solve -> picc
+//   pici -> ls_alloc (R = 0, Val = 0) -> construct_fluid -> b_assem_neu_bc on
+//   the outlet (zero traction + backflow stabilisation) -> fsils_solve -> picc
 // with picc's convergence test (pic.cpp:236-281) on the host.
 //
 //   svg_newton_step <n> [relTol] [krylov] [maxItr]
@@ -63,6 +64,24 @@ int main(int argc, char** argv)
     fsils_bc_create(lhs, 1, (int)wall.size(), 3, BcType::BC_TYPE_Dir, wall);
     FluidMesh msh;
     fluid_mesh_create(msh, lhs, nEl, IEN, x);
+    // outlet: every tet face whose three nodes carry the outlet flag
+    std::vector<int> fIEN, gE;
+    for (int e = 0; e < nEl; e++)
+      for (int k = 0; k < 4; k++) {
+        int tri[3], m = 0;
+        bool out = true;
+        for (int a = 0; a < 4; a++)
+          if (a != k) {
+            tri[m++] = IEN[4 * e + a];
+            out = out && (flags[IEN[4 * e + a]] & SVG_NODE_OUTLET);
+          }
+        if (out) {
+          fIEN.insert(fIEN.end(), tri, tri + 3);
+          gE.push_back(e);
+        }
+      }
+    FluidFace outlet;
+    fluid_face_create(outlet, msh, fIEN, gE);
 
     // physics and generalized-alpha (initialize.cpp:430-463)
     const double ro = 0.5, peak = 30.0, R0 = 0.3;
@@ -111,6 +130,7 @@ int main(int argc, char** argv)
       check(svg_fill(dR.data(), dR.size(), 0.0, nullptr));
       check(svg_fill(dVal.data(), dVal.size(), 0.0, nullptr));
       construct_fluid(msh, p, dAg.data(), dYg.data(), nullptr, dR.data(), dVal.data());
+      b_assem_neu_bc(outlet, p, 0.2, 2.0 / 3.0, nullptr, dYg.data(), dR.data(), dVal.data());
       fsils_solve(lhs, ls, 4, dR.data(), dVal.data(), PreconditionerType::PREC_FSILS, incL, res);
       picc_update(nNo, p.gam, p.dt, dR.data(), dAn.data(), dYn.data());
       if (is_zero(ls.RI.iNorm)) ls.RI.iNorm = DBL_EPSILON;

@@ -67,6 +67,27 @@ void fluid_mesh_create(FluidMesh& msh, const FSILS_lhsType& lhs, int nEl, const 
   check(svg_msh_num_colors(msh.h, &msh.nColors));
 }
 
+FluidFace::~FluidFace()
+{
+  if (h) svg_face_free(h);
+}
+
+void fluid_face_create(FluidFace& fa, const FluidMesh& msh, const std::vector<int>& faceIEN,
+                       const std::vector<int>& gE)
+{
+  if (faceIEN.size() != 3 * gE.size()) throw std::runtime_error("fluid_face_create: faceIEN must be (3,nFe)");
+  if (fa.h) svg_face_free(fa.h);
+  fa.h = nullptr;
+  check(svg_face_create(msh.h, (int)gE.size(), faceIEN.data(), gE.data(), &fa.h));
+  fa.nFe = (int)gE.size();
+}
+
+void b_assem_neu_bc(FluidFace& fa, const svg_fluid_params& p, double beta, double qmTRI3, const double* hg,
+                    const double* Yg, double* R, double* Val, hipStream_t stream)
+{
+  check(svg_b_assem_neu_bc(fa.h, &p, beta, qmTRI3, hg, Yg, R, Val, stream));
+}
+
 void construct_fluid(FluidMesh& msh, const svg_fluid_params& p, const double* Ag, const double* Yg, const double* Bf,
                      double* R, double* Val, hipStream_t stream)
 {

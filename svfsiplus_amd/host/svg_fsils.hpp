@@ -125,6 +125,23 @@ struct FluidMesh {
 void fluid_mesh_create(FluidMesh& msh, const FSILS_lhsType& lhs, int nEl, const std::vector<int>& IEN,
                        const std::vector<double>& x);
 
+// A TRI3 Neumann face of the fluid mesh (faceType IEN(3,nFe), gE(nFe)).
+struct FluidFace {
+  svg_face* h = nullptr;
+  int nFe = 0;
+  FluidFace() = default;
+  FluidFace(const FluidFace&) = delete;
+  FluidFace& operator=(const FluidFace&) = delete;
+  ~FluidFace();
+};
+void fluid_face_create(FluidFace& fa, const FluidMesh& msh, const std::vector<int>& faceIEN,
+                       const std::vector<int>& gE);
+// b_assem_neu_bc (eq_assem.cpp:58-172) with b_fluid (fluid.cpp:46-133): the
+// Neumann value hg (device, per node; null: zero traction) and the backflow
+// stabilisation beta.
+void b_assem_neu_bc(FluidFace& fa, const svg_fluid_params& p, double beta, double qmTRI3, const double* hg,
+                    const double* Yg, double* R, double* Val, hipStream_t stream = nullptr);
+
 // construct_fluid (fluid.cpp:464-708): R(4,nNo) += residuals, Val(16,nnz) +=
 // tangents; every array in device memory; Bf may be null.
 void construct_fluid(FluidMesh& msh, const svg_fluid_params& p, const double* Ag, const double* Yg, const double* Bf,

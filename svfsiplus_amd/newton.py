@@ -6,6 +6,7 @@ Mirrors the reference's Newton loop for one fluid equation
     pici   (pic.cpp:486-574)          Ag, Yg from Ao/An, Yo/Yn
     ls_alloc                          R = 0, Val = 0
     construct_fluid (fluid.cpp:464)   element assembly          -> svg_construct_fluid
+    set_bc_neu -> b_assem_neu_bc      outlet backflow term      -> svg_b_assem_neu_bc
     ls_solve -> fsils_solve (solve.cpp:50)                       -> svg_fsils_solve
     picc   (pic.cpp:74-330)           An -= R, Yn -= gam dt R, Newton convergence test
 
@@ -22,7 +23,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib
-from .workload import Physics, Pipe, dirichlet_state
+from .workload import Physics, Pipe, dirichlet_state, outlet_faces
 
 
 def _ptr(t) -> int:
@@ -86,6 +87,38 @@ class DeviceMesh:
             self.free()
         except Exception:
             pass
+
+
+class DeviceFace:
+    """A TRI3 Neumann face of the fluid mesh on the device (faceType IEN, gE)."""
+
+    def __init__(self, msh: DeviceMesh, faceIEN, gE):
+        lib = _lib.load()
+        h = C.c_void_p()
+        fi = np.ascontiguousarray(faceIEN, dtype=np.int32)
+        ge = np.ascontiguousarray(gE, dtype=np.int32)
+        _lib.check(lib.svg_face_create(msh.h, fi.shape[0], fi.ctypes.data, ge.ctypes.data, C.byref(h)))
+        self.h = h
+        self.nFe = fi.shape[0]
+
+    def free(self):
+        if self.h:
+            _lib.load().svg_face_free(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def b_assem_neu_bc(face: DeviceFace, phys: Physics, Yg, R, Val, hg=None):
+    """Neumann-face assembly with backflow stabilisation (eq_assem.cpp:58-172)."""
+    prm = phys.fluid_params()
+    _lib.check(_lib.load().svg_b_assem_neu_bc(face.h, C.byref(prm), phys.beta, phys.qmTRI3,
+                                              None if hg is None else _ptr(hg), _ptr(Yg), _ptr(R), _ptr(Val),
+                                              _stream()))
 
 
 def ls_create(ls_type: int, relTol=None, absTol=None, maxItr=None, dimKry=None, ns_gm=None, ns_cg=None) -> _lib.Ls:
@@ -164,7 +197,7 @@ class FluidNewton:
     """One fluid equation on one pipe, all state resident in HBM."""
 
     def __init__(self, pipe: Pipe, phys: Physics, ls: _lib.Ls, eq: EqState | None = None, device="cuda",
-                 precision: int = _lib.PRECISION_FP64, lpc=None):
+                 precision: int = _lib.PRECISION_FP64, lpc=None, outlet: bool = True):
         import torch
         if not torch.cuda.is_available():
             raise RuntimeError("FluidNewton needs a GPU (the HIP library has no CPU path)")
@@ -175,6 +208,12 @@ class FluidNewton:
         self.eq = eq or EqState()
         self.lhs = DeviceLhs(pipe)
         self.msh = DeviceMesh(self.lhs, pipe)
+        # the zero-traction outlet is a Neumann face: backflow stabilisation (b_assem_neu_bc)
+        self.outlet = None
+        if outlet:
+            fi, ge, _ = outlet_faces(pipe)
+            if fi.shape[0]:
+                self.outlet = DeviceFace(self.msh, fi, ge)
         dev = torch.device(device)
         f64 = torch.float64
         n = pipe.nNo
@@ -206,6 +245,8 @@ class FluidNewton:
         _lib.check(lib.svg_fill(_ptr(self.R), self.R.numel(), 0.0, s))
         _lib.check(lib.svg_fill(_ptr(self.Val), self.Val.numel(), 0.0, s))
         construct_fluid(self.msh, ph, self.Ag, self.Yg, self.R, self.Val)
+        if self.outlet is not None:
+            b_assem_neu_bc(self.outlet, ph, self.Yg, self.R, self.Val)
         info = {}
         fsils_solve(self.lhs, ls, 4, self.R, self.Val, precision=self.precision, lpc=self.lpc, info=info)
         _lib.check(lib.svg_picc_update(self.pipe.nNo, ph.gam, ph.dt, _ptr(self.R), _ptr(self.An), _ptr(self.Yn), s))

@@ -32,6 +32,8 @@ class Physics:
     f: tuple = (0.0, 0.0, 0.0)
     K_darcy: float = 0.0
     qmTET4: float = (5.0 + 3.0 * math.sqrt(5.0)) / 20.0
+    beta: float = 0.2           # Backflow_stabilization_coefficient (Parameters.cpp:1390 default)
+    qmTRI3: float = 2.0 / 3.0   # TRI3 face quadrature modifier (ComMod.h:611)
 
     # generalized-alpha for a first-order equation (initialize.cpp:430-463)
     @property
@@ -120,6 +122,59 @@ def make_pipe(n: int, nz: int | None = None, radius: float = 0.3, length: float 
     return p
 
 
+def _finish_mesh(x, IEN, flags, n, nz, radius, length) -> Pipe:
+    lib = _lib.load()
+    nNo, nEl = x.shape[0], IEN.shape[0]
+    nnz = C.c_int()
+    _lib.check(lib.svg_csr_pattern_count(nNo, nEl, 4, IEN.ctypes.data, C.byref(nnz)))
+    rowPtr = np.empty(nNo + 1, dtype=np.int32)
+    colPtr = np.empty(nnz.value, dtype=np.int32)
+    _lib.check(lib.svg_csr_pattern(nNo, nEl, 4, IEN.ctypes.data, rowPtr.ctypes.data, colPtr.ctypes.data))
+    inlet = np.nonzero(flags & _lib.NODE_INLET)[0].astype(np.int32)
+    wall = np.nonzero(flags & _lib.NODE_WALL)[0].astype(np.int32)
+    p = Pipe(n, nz, x, IEN, flags, rowPtr, colPtr, radius, length)
+    p.faces = [("inlet", inlet), ("wall", wall)]
+    return p
+
+
+def make_bifurcation(n: int, nzp: int, nzd: int, radius: float = 0.3, length_parent: float = 1.5,
+                     length_daughter: float = 1.5, angle0: float = 0.6, angle1: float = 0.6, split: float = 0.5,
+                     jitter: float = 0.1, seed: int = SEED_MESH) -> Pipe:
+    """A bifurcating pipe (svg_bif_generate): parent + two Murray-law daughters."""
+    lib = _lib.load()
+    sp = _lib.BifSpec(n, nzp, nzd, radius, length_parent, length_daughter, angle0, angle1, split, jitter, seed)
+    nNo, nEl = C.c_int(), C.c_int()
+    _lib.check(lib.svg_bif_counts(C.byref(sp), C.byref(nNo), C.byref(nEl)))
+    x = np.empty((nNo.value, 3), dtype=np.float64)
+    IEN = np.empty((nEl.value, 4), dtype=np.int32)
+    flags = np.empty(nNo.value, dtype=np.int32)
+    _lib.check(lib.svg_bif_generate(C.byref(sp), x.ctypes.data, IEN.ctypes.data, flags.ctypes.data))
+    return _finish_mesh(x, IEN, flags, n, nzp + nzd, radius, length_parent)
+
+
+def bifurcation_batch(count: int = 256, seed: int = 20261202, tets_min: float = 5e3, tets_max: float = 8e4,
+                      radius: float = 0.3):
+    """Seeded specs of the batched workload: tet counts log-uniform in
+    [tets_min, tets_max] (cross-section n = 4, 8 or 12 by size, layers from the
+    count, segment lengths keeping the axial spacing at 1.5 cross-section cells),
+    daughter angles uniform in [25, 45] degrees, Murray split uniform in
+    [0.35, 0.65]. Returns keyword dicts for make_bifurcation."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        T = math.exp(rng.uniform(math.log(tets_min), math.log(tets_max)))
+        n = 4 if T < 1.2e4 else (8 if T < 5e4 else 12)
+        layers = max(6, int(round(T / (6.0 * n * n))))   # parent + daughter layers, n^2 hexes each
+        nzp = max(2, layers // 2)
+        nzd = max(3, layers - nzp)
+        hz = 1.5 * 2.0 * radius / n
+        a0, a1 = np.radians(rng.uniform(25.0, 45.0, 2))
+        out.append(dict(n=n, nzp=nzp, nzd=nzd, radius=radius, length_parent=nzp * hz, length_daughter=nzd * hz,
+                        angle0=float(a0), angle1=float(a1), split=float(rng.uniform(0.35, 0.65)),
+                        seed=int(rng.integers(1, 2**31))))
+    return out
+
+
 def dirichlet_state(p: Pipe, phys: Physics):
     """Initial (Ao, Yo, An, Yn) of the first time step: Ao = Yo = 0, then the
     predictor (picp, pic.cpp:591-715: An = Ao (gam-1)/gam, Yn = Yo) and the
@@ -150,3 +205,22 @@ def face_arrays(p: Pipe):
     glob = np.concatenate([f[1] for f in p.faces]).astype(np.int32)
     val = np.zeros((glob.shape[0], 3))
     return nn, glob, val
+
+
+def outlet_faces(p: Pipe):
+    """TRI3 faces of the outlet (every tet face whose three nodes carry the
+    outlet flag): faceIEN (nFe, 3) in the tet's node order, the parent tet
+    gE (nFe,) and its fourth node (nFe,), which orients the outward normal as
+    gnnb does."""
+    out = (p.flags & _lib.NODE_OUTLET) != 0
+    faces, parent, inner = [], [], []
+    for k in range(4):
+        keep = [a for a in range(4) if a != k]
+        tri = p.IEN[:, keep]
+        hit = np.all(out[tri], axis=1)
+        faces.append(tri[hit])
+        parent.append(np.nonzero(hit)[0])
+        inner.append(p.IEN[hit, k])
+    return (np.ascontiguousarray(np.concatenate(faces), dtype=np.int32),
+            np.ascontiguousarray(np.concatenate(parent), dtype=np.int32),
+            np.ascontiguousarray(np.concatenate(inner), dtype=np.int32))
