@@ -228,7 +228,7 @@ int svg_lpc_train_step(svg_lpc* net, svg_lhs* lhs, const double* d_Val, int nvec
  *   SVG_PREC_FSILS scales it; SVG_PREC_NONE means the caller already did),
  *   restarts on the FP64 true residual; after fallback_window (0: 20)
  *   consecutive non-decreasing iterations (none lowering the smallest
- *   Givens residual estimate so far by 0.1 % or more) the solve continues
+ *   Givens residual estimate so far by 0.01 % or more) the solve continues
  *   from the current X with the plain diagonal-preconditioned GMRES, and
  *   fallback_itr reports the iteration count at that point (-1: none). */
 #define SVG_PRECISION_FP64 0
@@ -246,6 +246,27 @@ typedef struct {
 
 int svg_fsils_solve_ex(svg_lhs* lhs, svg_ls* ls, int dof, double* d_Ri, double* d_Val, int prec, int nIncL,
                        const int* incL, const double* res, svg_solve_opts* opts, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Batched GMRES (no counterpart in the reference): many independent systems  */
+/* assembled into one block-diagonal lhs (system q owns nodes                 */
+/* [nodeOff[q], nodeOff[q+1])) are solved together by gmres_v's algorithm    */
+/* (gmres.cpp:450-629) with per-system Krylov spaces, dot products, Givens    */
+/* rotations and convergence; each system's arithmetic is that of a solve of  */
+/* the system alone. One restart cycle of all systems is captured into a      */
+/* hipGraph and replayed until every system has converged or mItr cycles ran. */
+/* d_R, d_Val: the union's (already diagonally scaled) right-hand side and    */
+/* matrix; d_R receives the per-system solutions; out[nsys] the records.      */
+/* precision SVG_PRECISION_MIXED: FP32 SpMV, FP64 true-residual restarts.     */
+typedef struct svg_batch svg_batch;
+int svg_batch_create(svg_lhs* lhs, int nsys, const int* nodeOff, int sD, svg_batch** batch);
+int svg_batch_free(svg_batch* batch);
+int svg_batch_gmres(svg_batch* batch, const svg_subls* proto, double* d_R, const double* d_Val, int precision,
+                    svg_subls* out, void* stream);
+/* fsils_solve (solve.cpp:50-193) for the batch: the diagonal preconditioner
+ * over the union (every face included), svg_batch_gmres, R = W R. */
+int svg_batch_fsils_solve(svg_batch* batch, const svg_subls* proto, double* d_R, double* d_Val, int precision,
+                          svg_subls* out, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Kernels of the solve, exposed for unit parity tests.                      */

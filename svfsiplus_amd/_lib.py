@@ -112,6 +112,10 @@ _SIGS = {
     "svg_prof_reset": (_I, []),
     "svg_prof_get": (_I, [C.c_char_p, _IP, _DP]),
     "svg_prof_get_bytes": (_I, [C.c_char_p, _DP]),
+    "svg_batch_create": (_I, [_P, _I, _P, _I, C.POINTER(_P)]),
+    "svg_batch_free": (_I, [_P]),
+    "svg_batch_gmres": (_I, [_P, C.POINTER(SubLs), _P, _P, _I, C.POINTER(SubLs), _P]),
+    "svg_batch_fsils_solve": (_I, [_P, C.POINTER(SubLs), _P, _P, _I, C.POINTER(SubLs), _P]),
     "svg_lpc_num_params": (_I, [_IP]),
     "svg_lpc_create": (_I, [C.POINTER(_P)]),
     "svg_lpc_free": (_I, [_P]),

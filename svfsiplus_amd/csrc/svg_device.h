@@ -23,6 +23,7 @@ int hip_check(hipError_t e, const char* what);
 // Event timing of named kernels on the launching stream, with the
 // algorithmic bytes of each launch (for achieved-bandwidth reporting).
 int prof_begin(const char* name, hipStream_t s, double bytes = 0.0);
+bool prof_set(bool on);  // returns the previous setting (event timing off under graph capture)
 void prof_end(int slot, hipStream_t s);
 
 template <typename K, typename... Args>
@@ -81,6 +82,8 @@ struct GmState {
   int tconv;    // mixed precision: the FP64 true residual at a restart is below eps
   int fallback; // learned preconditioner: no new residual minimum for the fallback window
   int nondec;   // iterations since the last new minimum of |err|
+  int act;      // batched solve: the system still iterates (not converged before this cycle)
+  int pad;
   double best;  // smallest |err(i+1)| of the solve
   double eps;
   double scale;  // err(0) after a restart, 1 / h(i+1,i) after an Arnoldi step
@@ -179,6 +182,27 @@ struct svg_lhs {
   svg::KrylovWork kw;    // gmres_v on the 4x4 system
   svg::KrylovWork kw3;   // inner GMRES of the NS solver on the 3x3 momentum block
   svg::NsWork ns;
+};
+
+// Batched GMRES over a block-diagonal union of independent systems (one lhs
+// over all of them): per-system Krylov spaces, reductions and convergence.
+struct svg_batch {
+  svg_lhs* L = nullptr;
+  int nsys = 0, sD = 0, n = 0, nsp = 0;   // n = 4 * union nodes; nsp = reduction spans
+  size_t ld = 0;
+  std::vector<int> nodeOff;               // nsys + 1
+  int *d_sysOfNode = nullptr, *d_spanStart = nullptr, *d_spanEnd = nullptr, *d_spanSys = nullptr;
+  int* d_sysSpanOff = nullptr;            // nsys + 1
+  double *u = nullptr, *X = nullptr, *h = nullptr, *cs = nullptr, *sn = nullptr, *err = nullptr, *coef = nullptr,
+         *y = nullptr, *partial = nullptr;
+  svg::GmState* st = nullptr;
+  svg::GmState* h_st = nullptr;           // pinned mirror (nsys)
+  float* u32 = nullptr;
+  hipStream_t cs_stream = nullptr;        // capture/replay stream
+  hipGraphExec_t graph[2] = {nullptr, nullptr};  // restart cycle: FP64 / mixed
+  const double* gR[2] = {nullptr, nullptr};      // the R and Val each graph was captured with
+  const double* gVal[2] = {nullptr, nullptr};
+  hipEvent_t ev = nullptr;
 };
 
 struct svg_msh {

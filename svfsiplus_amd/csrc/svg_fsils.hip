@@ -257,7 +257,7 @@ struct FinArgs {
 };
 
 constexpr int MAXM = 512;  // sums per reduction (Krylov dimension + 2)
-constexpr double FB_DROP = 1e-3;  // learned preconditioner: smallest relative decrease that counts
+constexpr double FB_DROP = 1e-4;  // learned preconditioner: smallest relative decrease that counts
 
 // Dots of w against m vectors base + j*ld (j < m), pass 1: each workgroup
 // sums its 1024-element span (256 lanes x 4 elements at stride 256, in order),
@@ -309,17 +309,29 @@ constexpr int FIN_BS = 1024;  // finaliser: one workgroup of 16 wavefronts
 //   FIN_RESTART  err(0) = sqrt(sum), itr += inc; the inner GMRES's first
 //                cycle also sets iNorm and eps from err(0) (gmres.cpp:125-148)
 //   FIN_ARNOLDI  Hessenberg column i, Pythagorean h(i+1,i), Givens, err, stop (gmres.cpp:550-592)
-__global__ __launch_bounds__(FIN_BS) void k_fin(FinArgs A, const double* __restrict__ partial, int nblk, int m,
-                                                GmState* st, double* __restrict__ h, double* __restrict__ cs,
-                                                double* __restrict__ sn, double* __restrict__ err,
-                                                double* __restrict__ coef, double* __restrict__ out,
-                                                volatile int* host_done)
+struct FinLds {
+  double sums[MAXM];
+  double lh[MAXM], lcs[MAXM], lsn[MAXM];
+  double pre_err, pre_eps;
+  int pre_itr;
+};
+
+// The finaliser's work for one system: column j's partials at
+// partial[j * pstride + b], b < nblk.
+__device__ __forceinline__ void fin_body(const FinArgs& A, const double* __restrict__ partial, size_t pstride,
+                                         int nblk, int m, GmState* st, double* __restrict__ h,
+                                         double* __restrict__ cs, double* __restrict__ sn, double* __restrict__ err,
+                                         double* __restrict__ coef, double* __restrict__ out,
+                                         volatile int* host_done, FinLds& S)
 {
 #pragma clang fp contract(off)
-  __shared__ double sums[MAXM];
-  __shared__ double lh[MAXM], lcs[MAXM], lsn[MAXM];
-  __shared__ double pre_err, pre_eps;
-  __shared__ int pre_itr;
+  double* sums = S.sums;
+  double* lh = S.lh;
+  double* lcs = S.lcs;
+  double* lsn = S.lsn;
+  double& pre_err = S.pre_err;
+  double& pre_eps = S.pre_eps;
+  int& pre_itr = S.pre_itr;
   if (A.mode == FIN_ARNOLDI && st->done) return;
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   if (A.mode == FIN_ARNOLDI) {
@@ -328,7 +340,7 @@ __global__ __launch_bounds__(FIN_BS) void k_fin(FinArgs A, const double* __restr
     if (t == 192) pre_itr = st->itr;
   }
   for (int j = wv; j < m; j += FIN_BS / 64) {
-    const double* pj = partial + (size_t)j * nblk;
+    const double* pj = partial + (size_t)j * pstride;
     double acc = 0.0;
     for (int b = lane; b < nblk; b += 64) acc = acc + pj[b];
     acc = wave_sum(acc);
@@ -361,6 +373,7 @@ __global__ __launch_bounds__(FIN_BS) void k_fin(FinArgs A, const double* __restr
       st->fallback = 0;
       st->nondec = 0;
       st->best = DBL_MAX;
+      st->act = nrm > A.absTol ? 1 : 0;
     }
     return;
   }
@@ -452,6 +465,16 @@ __global__ __launch_bounds__(FIN_BS) void k_fin(FinArgs A, const double* __restr
   for (int j = t; j <= i; j += FIN_BS) coef[j] = sums[j];
 }
 
+__global__ __launch_bounds__(FIN_BS) void k_fin(FinArgs A, const double* __restrict__ partial, int nblk, int m,
+                                                GmState* st, double* __restrict__ h, double* __restrict__ cs,
+                                                double* __restrict__ sn, double* __restrict__ err,
+                                                double* __restrict__ coef, double* __restrict__ out,
+                                                volatile int* host_done)
+{
+  __shared__ FinLds S;
+  fin_body(A, partial, (size_t)nblk, nblk, m, st, h, cs, sn, err, coef, out, host_done, S);
+}
+
 // u_{i+1} = (u_{i+1} - sum_j coef_j u_j) * scale, sequentially in j (omp_sum_v, omp_mul_v)
 __global__ __launch_bounds__(256) void k_gs_update(int n, double* __restrict__ u, size_t ld, int i,
                                                    const double* __restrict__ coef, const GmState* __restrict__ st,
@@ -490,12 +513,10 @@ __global__ void k_reset_state(GmState* st)
 
 // Back substitution, gmres.cpp:595-608; the triangle is staged in LDS when it fits.
 constexpr int BS_LDS = 6144;  // doubles of the staged Hessenberg (48 KB)
-__global__ __launch_bounds__(256) void k_backsolve(int sD, const double* __restrict__ h, const double* __restrict__ err,
-                                                   double* __restrict__ y, GmState* st)
+__device__ __forceinline__ void backsolve_body(int sD, const double* __restrict__ h, const double* __restrict__ err,
+                                               double* __restrict__ y, GmState* st, double* lh, double* ly)
 {
 #pragma clang fp contract(off)
-  __shared__ double lh[BS_LDS];
-  __shared__ double ly[MAXM];
   if (st->early || st->divzero || st->tconv) return;
   int last_i = st->last_i;
   if (last_i >= sD) last_i = sD - 1;
@@ -532,6 +553,14 @@ __global__ __launch_bounds__(256) void k_backsolve(int sD, const double* __restr
   }
   __syncthreads();
   for (int q = threadIdx.x; q < nc; q += blockDim.x) y[q] = ly[q];
+}
+
+__global__ __launch_bounds__(256) void k_backsolve(int sD, const double* __restrict__ h, const double* __restrict__ err,
+                                                   double* __restrict__ y, GmState* st)
+{
+  __shared__ double lh[BS_LDS];
+  __shared__ double ly[MAXM];
+  backsolve_body(sD, h, err, y, st, lh, ly);
 }
 
 // X += sum_j y_j u_j, sequentially in j (gmres.cpp:610-612)
@@ -1686,6 +1715,215 @@ static int precond_diag_dev(svg_lhs* L, int dof, double* Val, double* R, double*
   return SVG_OK;
 }
 
+
+// --------------------------------------------------------- batched GMRES --
+// Independent systems s = 0..nsys-1 on one block-diagonal union (rows
+// [4 nodeOff[s], 4 nodeOff[s+1])): the union SpMV serves all of them, every
+// reduction is per system over 1024-element spans that start at the system's
+// first row (the partition a solve of the system alone uses, so each system's
+// arithmetic is the single-system gmres_v's, bit for bit), and every system
+// has its own Hessenberg, Givens, state and convergence. One restart cycle of
+// all systems is captured once into a hipGraph and replayed.
+__global__ __launch_bounds__(RED_BS) void k_bmdot(const int* __restrict__ spanStart, const int* __restrict__ spanEnd,
+                                                  const int* __restrict__ spanSys, int nsp,
+                                                  const double* __restrict__ w, const double* __restrict__ base,
+                                                  size_t ld, int m, double* __restrict__ partial,
+                                                  const GmState* __restrict__ st, int check)
+{
+#pragma clang fp contract(off)
+  extern __shared__ double lds[];  // [RED_NW waves][m]
+  const int b = blockIdx.x;
+  if (check) {  // 1: skip systems that stopped iterating; 2: also those done in this cycle
+    const GmState& S = st[spanSys[b]];
+    if (!S.act || (check == 2 && S.done)) return;
+  }
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const size_t k0 = (size_t)spanStart[b] + t, kend = (size_t)spanEnd[b];
+  double wr[RED_EPT];
+#pragma unroll
+  for (int r = 0; r < RED_EPT; r++) {
+    size_t k = k0 + (size_t)r * RED_BS;
+    wr[r] = k < kend ? w[k] : 0.0;
+  }
+  for (int j = 0; j < m; j++) {
+    const double* uj = base + (size_t)j * ld;
+    double acc = 0.0;
+#pragma unroll
+    for (int r = 0; r < RED_EPT; r++) {
+      size_t k = k0 + (size_t)r * RED_BS;
+      double v = k < kend ? uj[k] : 0.0;
+      acc = acc + v * wr[r];
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) lds[wv * m + j] = acc;
+  }
+  __syncthreads();
+  for (int j = t; j < m; j += RED_BS) {
+    double v = lds[j];
+    for (int q = 1; q < RED_NW; q++) v = v + lds[q * m + j];
+    partial[(size_t)j * nsp + b] = v;
+  }
+}
+
+// One workgroup per system: the finaliser on the system's span partials.
+__global__ __launch_bounds__(FIN_BS) void k_bfin(FinArgs A, const int* __restrict__ sysSpanOff, int nsp,
+                                                 const double* __restrict__ partial, int m, GmState* st,
+                                                 double* __restrict__ h, double* __restrict__ cs,
+                                                 double* __restrict__ sn, double* __restrict__ err,
+                                                 double* __restrict__ coef)
+{
+  __shared__ FinLds S;
+  const int sy = blockIdx.x, sD = A.sD;
+  GmState* g = st + sy;
+  if (A.mode != FIN_NORM_R && !g->act) return;
+  const int off = sysSpanOff[sy], nb = sysSpanOff[sy + 1] - off;
+  fin_body(A, partial + off, (size_t)nsp, nb, m, g, h + (size_t)sy * (sD + 1) * sD, cs + (size_t)sy * sD,
+           sn + (size_t)sy * sD, err + (size_t)sy * (sD + 1), coef + (size_t)sy * (sD + 2), nullptr, nullptr, S);
+}
+
+__global__ __launch_bounds__(256) void k_bgs_update(int n, double* __restrict__ u, size_t ld, int i, int sD,
+                                                    const double* __restrict__ coef, const GmState* __restrict__ st,
+                                                    const int* __restrict__ sysOfNode, float* __restrict__ u32)
+{
+#pragma clang fp contract(off)
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (size_t)n) return;
+  const int sy = sysOfNode[k >> 2];
+  const GmState& S = st[sy];
+  if (!S.act || S.done) return;
+  const double* c = coef + (size_t)sy * (sD + 2);
+  double* w = u + (size_t)(i + 1) * ld;
+  double v = w[k];
+  for (int j = 0; j <= i; j++) v = v + (-c[j]) * u[(size_t)j * ld + k];
+  v = S.scale * v;
+  w[k] = v;
+  if (u32) u32[k] = (float)v;
+}
+
+__global__ __launch_bounds__(256) void k_bdiv(int n, double* __restrict__ u, const GmState* __restrict__ st,
+                                              const int* __restrict__ sysOfNode, float* __restrict__ u32)
+{
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (size_t)n) return;
+  const GmState& S = st[sysOfNode[k >> 2]];
+  if (!S.act || S.done) return;
+  const double v = u[k] / S.scale;
+  u[k] = v;
+  if (u32) u32[k] = (float)v;
+}
+
+__global__ __launch_bounds__(256) void k_bbacksolve(int sD, const double* __restrict__ h,
+                                                    const double* __restrict__ err, double* __restrict__ y,
+                                                    GmState* st)
+{
+  __shared__ double lh[BS_LDS];
+  __shared__ double ly[MAXM];
+  const int sy = blockIdx.x;
+  if (!st[sy].act) return;
+  backsolve_body(sD, h + (size_t)sy * (sD + 1) * sD, err + (size_t)sy * (sD + 1), y + (size_t)sy * sD, st + sy, lh,
+                 ly);
+}
+
+__global__ __launch_bounds__(256) void k_bxupdate(int n, double* __restrict__ X, const double* __restrict__ u,
+                                                  size_t ld, int sD, const double* __restrict__ y,
+                                                  const GmState* __restrict__ st, const int* __restrict__ sysOfNode)
+{
+#pragma clang fp contract(off)
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (size_t)n) return;
+  const int sy = sysOfNode[k >> 2];
+  const GmState& S = st[sy];
+  if (!S.act || S.early || S.divzero || S.tconv) return;
+  const double* ys = y + (size_t)sy * sD;
+  double x = X[k];
+  for (int j = 0; j <= S.last_i; j++) x = x + ys[j] * u[(size_t)j * ld + k];
+  X[k] = x;
+}
+
+// After a cycle: systems that converged stop iterating (plain: the Givens
+// estimate, gmres.cpp:585-589; mixed: the FP64 true residual at the restart).
+__global__ void k_bclose(int nsys, GmState* st, int tcheck)
+{
+  const int sy = blockIdx.x * blockDim.x + threadIdx.x;
+  if (sy >= nsys) return;
+  GmState& S = st[sy];
+  if (S.act && (tcheck ? S.tconv : S.suc)) S.act = 0;
+  if (S.divzero) S.act = 0;
+}
+
+// R = X for the systems that iterated; the others keep R (gmres.cpp:489-493).
+__global__ __launch_bounds__(256) void k_bresult(int n, double* __restrict__ R, const double* __restrict__ X,
+                                                 const GmState* __restrict__ st, const int* __restrict__ sysOfNode,
+                                                 double absTol)
+{
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (size_t)n) return;
+  const GmState& S = st[sysOfNode[k >> 2]];
+  if (S.iNorm > absTol) R[k] = X[k];
+}
+
+
+static void batch_graph_free(svg_batch* B)
+{
+  for (auto& g : B->graph)
+    if (g) (void)hipGraphExecDestroy(g), g = nullptr;
+}
+
+static void batch_mdot(svg_batch* B, const double* w, const double* base, size_t ld, int m, int check, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_bmdot, dim3(B->nsp), dim3(RED_BS), (size_t)RED_NW * m * sizeof(double), s,
+                     (const int*)B->d_spanStart, (const int*)B->d_spanEnd, (const int*)B->d_spanSys, B->nsp, w, base,
+                     ld, m, B->partial, (const GmState*)B->st, check);
+}
+
+static void batch_fin(svg_batch* B, const FinArgs& fa, int m, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_bfin, dim3(B->nsys), dim3(FIN_BS), 0, s, fa, (const int*)B->d_sysSpanOff, B->nsp,
+                     (const double*)B->partial, m, B->st, B->h, B->cs, B->sn, B->err, B->coef);
+}
+
+// One restart cycle of every system still iterating (the body of gmres_v's
+// restart loop, gmres.cpp:529-612), enqueued on s (captured into a graph).
+static void batch_cycle(svg_batch* B, const double* Val, const double* R, bool mixed, const FinArgs& base,
+                        hipStream_t s)
+{
+  svg_lhs* L = B->L;
+  const int n = B->n, sD = B->sD;
+  const unsigned nbv = blocks_for(n, 256);
+  const Op A{L, Val, 4};
+  FinArgs fa = base;
+  op_apply(A, B->X, B->u, R, nullptr, s);  // u0 = R - A X
+  batch_mdot(B, B->u, B->u, 0, 1, 1, s);
+  fa.mode = FIN_RESTART;
+  fa.inc = 1;
+  fa.first = 0;
+  fa.tcheck = mixed ? 1 : 0;
+  batch_fin(B, fa, 1, s);
+  hipLaunchKernelGGL(k_bdiv, dim3(nbv), dim3(256), 0, s, n, B->u, (const GmState*)B->st,
+                     (const int*)B->d_sysOfNode, mixed ? B->u32 : (float*)nullptr);
+  for (int i = 0; i < sD; i++) {
+    double* ui = B->u + B->ld * i;
+    double* un = B->u + B->ld * (i + 1);
+    if (mixed)
+      hipLaunchKernelGGL(k_spmv32<2>, dim3(blocks_for((size_t)4 * L->nNo, 256)), dim3(256), 0, s, L->nNo,
+                         (const int*)L->d_rowPtr, (const int*)L->d_colPtr, (const float*)L->d_val32,
+                         (const float*)B->u32, un, (const int*)nullptr);
+    else
+      op_apply(A, ui, un, nullptr, nullptr, s);
+    batch_mdot(B, un, B->u, B->ld, i + 2, 2, s);
+    fa.mode = FIN_ARNOLDI;
+    fa.i = i;
+    batch_fin(B, fa, i + 2, s);
+    hipLaunchKernelGGL(k_bgs_update, dim3(nbv), dim3(256), 0, s, n, B->u, B->ld, i, sD, (const double*)B->coef,
+                       (const GmState*)B->st, (const int*)B->d_sysOfNode, mixed ? B->u32 : (float*)nullptr);
+  }
+  hipLaunchKernelGGL(k_bbacksolve, dim3(B->nsys), dim3(256), 0, s, sD, (const double*)B->h, (const double*)B->err,
+                     B->y, B->st);
+  hipLaunchKernelGGL(k_bxupdate, dim3(nbv), dim3(256), 0, s, n, B->X, (const double*)B->u, B->ld, sD,
+                     (const double*)B->y, (const GmState*)B->st, (const int*)B->d_sysOfNode);
+  hipLaunchKernelGGL(k_bclose, dim3(blocks_for(B->nsys, 256)), dim3(256), 0, s, B->nsys, B->st, mixed ? 1 : 0);
+}
+
 }  // namespace svg
 
 using namespace svg;
@@ -1922,6 +2160,190 @@ int svg_fsils_solve_ex(svg_lhs* L, svg_ls* ls, int dof, double* Ri, double* Val,
   }
   if (rc) return rc;
   launch("wc_scale", k_mul, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, (const double*)k.W, (const double*)Ri, Ri);
+  SVG_HIP(hipGetLastError());
+  SVG_HIP(hipStreamSynchronize(s));
+  return SVG_OK;
+}
+
+
+int svg_batch_create(svg_lhs* L, int nsys, const int* nodeOff, int sD, svg_batch** out)
+{
+  if (!L || !nodeOff || !out || nsys < 1) return set_error(SVG_ERR_ARG, "svg_batch_create: bad argument");
+  if (sD < 1 || sD > 250) return set_error(SVG_ERR_ARG, "svg_batch_create: Krylov dimension must be in [1, 250]");
+  if (nodeOff[0] != 0 || nodeOff[nsys] != L->nNo)
+    return set_error(SVG_ERR_ARG, "svg_batch_create: nodeOff must run from 0 to the lhs's nNo");
+  for (int q = 0; q < nsys; q++)
+    if (nodeOff[q + 1] <= nodeOff[q]) return set_error(SVG_ERR_ARG, "svg_batch_create: empty or unordered system");
+  auto* B = new svg_batch();
+  B->L = L;
+  B->nsys = nsys;
+  B->sD = sD;
+  B->n = 4 * L->nNo;
+  B->ld = ((size_t)B->n + 63) & ~(size_t)63;
+  B->nodeOff.assign(nodeOff, nodeOff + nsys + 1);
+  std::vector<int> sysOfNode(L->nNo), spanStart, spanEnd, spanSys, sysSpanOff(nsys + 1, 0);
+  for (int q = 0; q < nsys; q++) {
+    for (int a = nodeOff[q]; a < nodeOff[q + 1]; a++) sysOfNode[a] = q;
+    const int r0 = 4 * nodeOff[q], r1 = 4 * nodeOff[q + 1];
+    sysSpanOff[q] = (int)spanStart.size();
+    for (int k = r0; k < r1; k += RED_SPAN) {
+      spanStart.push_back(k);
+      spanEnd.push_back(std::min(r1, k + RED_SPAN));
+      spanSys.push_back(q);
+    }
+  }
+  sysSpanOff[nsys] = (int)spanStart.size();
+  B->nsp = (int)spanStart.size();
+  int rc;
+  if ((rc = dev_alloc(&B->d_sysOfNode, (size_t)L->nNo)) || (rc = dev_alloc(&B->d_spanStart, (size_t)B->nsp)) ||
+      (rc = dev_alloc(&B->d_spanEnd, (size_t)B->nsp)) || (rc = dev_alloc(&B->d_spanSys, (size_t)B->nsp)) ||
+      (rc = dev_alloc(&B->d_sysSpanOff, (size_t)nsys + 1)) || (rc = dev_alloc(&B->u, B->ld * (sD + 1))) ||
+      (rc = dev_alloc(&B->X, (size_t)B->n)) || (rc = dev_alloc(&B->h, (size_t)nsys * (sD + 1) * sD)) ||
+      (rc = dev_alloc(&B->cs, (size_t)nsys * sD)) || (rc = dev_alloc(&B->sn, (size_t)nsys * sD)) ||
+      (rc = dev_alloc(&B->err, (size_t)nsys * (sD + 1))) || (rc = dev_alloc(&B->coef, (size_t)nsys * (sD + 2))) ||
+      (rc = dev_alloc(&B->y, (size_t)nsys * sD)) || (rc = dev_alloc(&B->partial, (size_t)B->nsp * (sD + 2))) ||
+      (rc = dev_alloc(&B->st, (size_t)nsys))) {
+    svg_batch_free(B);
+    return rc;
+  }
+  hipError_t e = hipMemcpy(B->d_sysOfNode, sysOfNode.data(), sizeof(int) * L->nNo, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(B->d_spanStart, spanStart.data(), sizeof(int) * B->nsp, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(B->d_spanEnd, spanEnd.data(), sizeof(int) * B->nsp, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(B->d_spanSys, spanSys.data(), sizeof(int) * B->nsp, hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(B->d_sysSpanOff, sysSpanOff.data(), sizeof(int) * (nsys + 1), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(B->h, 0, sizeof(double) * nsys * (sD + 1) * sD);
+  if (e == hipSuccess) e = hipMemset(B->st, 0, sizeof(GmState) * nsys);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&B->h_st, sizeof(GmState) * nsys, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&B->cs_stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&B->ev, hipEventDisableTiming);
+  if (e != hipSuccess) {
+    svg_batch_free(B);
+    return hip_check(e, "svg_batch_create");
+  }
+  *out = B;
+  return SVG_OK;
+}
+
+int svg_batch_free(svg_batch* B)
+{
+  if (!B) return SVG_OK;
+  batch_graph_free(B);
+  for (int* p : {B->d_sysOfNode, B->d_spanStart, B->d_spanEnd, B->d_spanSys, B->d_sysSpanOff})
+    if (p) (void)hipFree(p);
+  for (double* p : {B->u, B->X, B->h, B->cs, B->sn, B->err, B->coef, B->y, B->partial})
+    if (p) (void)hipFree(p);
+  if (B->st) (void)hipFree(B->st);
+  if (B->u32) (void)hipFree(B->u32);
+  if (B->h_st) (void)hipHostFree(B->h_st);
+  if (B->ev) (void)hipEventDestroy(B->ev);
+  if (B->cs_stream) (void)hipStreamDestroy(B->cs_stream);
+  delete B;
+  return SVG_OK;
+}
+
+int svg_batch_gmres(svg_batch* B, const svg_subls* proto, double* R, const double* Val, int precision,
+                    svg_subls* out, void* stream)
+{
+  if (!B || !proto || !R || !Val || !out) return set_error(SVG_ERR_ARG, "svg_batch_gmres: null argument");
+  if (proto->mItr < 1) return set_error(SVG_ERR_ARG, "svg_batch_gmres: mItr must be >= 1");
+  const bool mixed = precision == SVG_PRECISION_MIXED;
+  svg_lhs* L = B->L;
+  hipStream_t caller = (hipStream_t)stream, s = B->cs_stream;
+  auto t0 = std::chrono::steady_clock::now();
+  int rc;
+  if (L->spmvVariant < 0 && (rc = svg_spmv_tune(L, Val, nullptr, caller))) return rc;
+  if (mixed) {
+    if (!L->d_val32 && (rc = dev_alloc(&L->d_val32, (size_t)16 * L->nnz))) return rc;
+    if (!B->u32 && (rc = dev_alloc(&B->u32, (size_t)B->n))) return rc;
+    launch("d2f", k_d2f, dim3(8192), dim3(256), 0, caller, (size_t)16 * L->nnz, Val, L->d_val32);
+  }
+  SVG_HIP(hipEventRecord(B->ev, caller));
+  SVG_HIP(hipStreamWaitEvent(s, B->ev, 0));
+  FinArgs fa{};
+  fa.sD = B->sD;
+  fa.absTol = proto->absTol;
+  fa.relTol = proto->relTol;
+  fa.mode = FIN_NORM_R;
+  batch_mdot(B, R, R, 0, 1, 0, s);
+  batch_fin(B, fa, 1, s);
+  svg_fill(B->X, (size_t)B->n, 0.0, s);
+  SVG_HIP(hipGetLastError());
+  // the restart cycle as one graph (pointers R and Val are baked in)
+  const int gi = mixed ? 1 : 0;
+  if (B->graph[gi] && (B->gR[gi] != R || B->gVal[gi] != Val)) {
+    (void)hipGraphExecDestroy(B->graph[gi]);
+    B->graph[gi] = nullptr;
+  }
+  if (!B->graph[gi]) {
+    const bool prev = prof_set(false);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      batch_cycle(B, Val, R, mixed, fa, s);
+      e = hipStreamEndCapture(s, &g);
+    }
+    prof_set(prev);
+    if (e == hipSuccess) e = hipGraphInstantiate(&B->graph[gi], g, nullptr, nullptr, 0);
+    if (g) (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return hip_check(e, "svg_batch_gmres: graph capture");
+    B->gR[gi] = R;
+    B->gVal[gi] = Val;
+  }
+  std::vector<double> dB(B->nsys);
+  SVG_HIP(hipMemcpyAsync(B->h_st, B->st, sizeof(GmState) * B->nsys, hipMemcpyDeviceToHost, s));
+  SVG_HIP(hipStreamSynchronize(s));
+  for (int q = 0; q < B->nsys; q++) dB[q] = B->h_st[q].fNorm;
+  int cycles = 0;
+  for (int l = 0; l < proto->mItr; l++) {
+    int active = 0;
+    for (int q = 0; q < B->nsys; q++) active += B->h_st[q].act;
+    if (!active) break;
+    for (int q = 0; q < B->nsys; q++)
+      if (B->h_st[q].act) dB[q] = B->h_st[q].fNorm;
+    SVG_HIP(hipGraphLaunch(B->graph[gi], s));
+    SVG_HIP(hipMemcpyAsync(B->h_st, B->st, sizeof(GmState) * B->nsys, hipMemcpyDeviceToHost, s));
+    SVG_HIP(hipStreamSynchronize(s));
+    cycles++;
+    for (int q = 0; q < B->nsys; q++)
+      if (B->h_st[q].divzero) return set_error(SVG_ERR_ARG, "Array Divide by zero.");
+  }
+  hipLaunchKernelGGL(k_bresult, dim3(blocks_for(B->n, 256)), dim3(256), 0, s, B->n, R, (const double*)B->X,
+                     (const GmState*)B->st, (const int*)B->d_sysOfNode, proto->absTol);
+  SVG_HIP(hipGetLastError());
+  SVG_HIP(hipEventRecord(B->ev, s));
+  SVG_HIP(hipStreamWaitEvent(caller, B->ev, 0));
+  SVG_HIP(hipStreamSynchronize(s));
+  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  for (int q = 0; q < B->nsys; q++) {
+    const GmState& g = B->h_st[q];
+    svg_subls& o = out[q];
+    o = *proto;
+    o.itr = g.itr;
+    o.iNorm = g.iNorm;
+    o.fNorm = g.fNorm;
+    o.suc = mixed ? g.tconv : g.suc;
+    o.callD = secs;
+    o.dB = (g.iNorm > proto->absTol && dB[q] > 0.0) ? 10.0 * log(g.fNorm / dB[q]) : 0.0;
+  }
+  (void)cycles;
+  return SVG_OK;
+}
+
+
+int svg_batch_fsils_solve(svg_batch* B, const svg_subls* proto, double* R, double* Val, int precision,
+                          svg_subls* out, void* stream)
+{
+  if (!B || !R || !Val) return set_error(SVG_ERR_ARG, "svg_batch_fsils_solve: null argument");
+  svg_lhs* L = B->L;
+  hipStream_t s = (hipStream_t)stream;
+  for (auto& f : L->face) f.incFlag = true;
+  int rc;
+  if ((rc = ensure_work(L->kw, 4 * L->nNo, 1))) return rc;
+  if ((rc = precond_diag_dev(L, 4, Val, R, L->kw.W, s))) return rc;
+  if ((rc = svg_batch_gmres(B, proto, R, Val, precision, out, stream))) return rc;
+  launch("wc_scale", k_mul, dim3(blocks_for(B->n, 256)), dim3(256), 0, s, B->n, (const double*)L->kw.W,
+         (const double*)R, R);
   SVG_HIP(hipGetLastError());
   SVG_HIP(hipStreamSynchronize(s));
   return SVG_OK;

@@ -206,13 +206,22 @@ __global__ __launch_bounds__(256) void k_msg(int n, int rows, const int* __restr
     m3 = fmaf(lo(a1.w), h2, m3);
     m3 = fmaf(hi(a1.w), h3, m3);
   };
+  // the row's first 32 column indices, loaded once by its 16 lanes and
+  // handed out by shuffles within the 16-lane group
+  const int cnt = e - s;
+  const int jA = c < cnt ? colPtr[s + c] : 0;
+  const int jB = 16 + c < cnt ? colPtr[s + 16 + c] : 0;
+  auto col = [&](int kk) {  // kk: index within the row, uniform over the 16 lanes
+    const int a = __shfl(jA, kk & 15, 16), b2 = __shfl(jB, kk & 15, 16);
+    return kk < 16 ? a : (kk < 32 ? b2 : colPtr[s + kk]);
+  };
   int k = s;
   for (; k + UNR <= e; k += UNR) {
     int j[UNR];
     uint4 a0[UNR], a1[UNR];
     uint2 hv[UNR];
 #pragma unroll
-    for (int q = 0; q < UNR; q++) j[q] = colPtr[k + q];
+    for (int q = 0; q < UNR; q++) j[q] = col(k + q - s);
 #pragma unroll
     for (int q = 0; q < UNR; q++) {
       a0[q] = reinterpret_cast<const uint4*>(A16 + 16 * (size_t)(k + q))[0];
@@ -223,9 +232,10 @@ __global__ __launch_bounds__(256) void k_msg(int n, int rows, const int* __restr
     for (int q = 0; q < UNR; q++) acc(a0[q], a1[q], hv[q]);
   }
   for (; k < e; k++) {
+    const int jj = col(k - s);
     const uint4 a0 = reinterpret_cast<const uint4*>(A16 + 16 * (size_t)k)[0];
     const uint4 a1 = reinterpret_cast<const uint4*>(A16 + 16 * (size_t)k)[1];
-    const uint2 hv = *reinterpret_cast<const uint2*>(Hb + (size_t)colPtr[k] * F);
+    const uint2 hv = *reinterpret_cast<const uint2*>(Hb + (size_t)jj * F);
     acc(a0, a1, hv);
   }
   uint2 pk;

@@ -36,6 +36,13 @@ struct ProfPending {
 };
 static std::mutex g_prof_mu;
 static bool g_prof_on = false;
+
+bool prof_set(bool on)
+{
+  const bool prev = g_prof_on;
+  g_prof_on = on;
+  return prev;
+}
 static std::map<std::string, ProfAcc> g_prof;
 static std::vector<ProfPending> g_pending;
 static std::vector<hipEvent_t> g_event_pool;

@@ -1,0 +1,79 @@
+"""Batched solve of many independent meshes (the library's svg_batch_*):
+each mesh's result equals a solve of that mesh alone."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from svfsiplus_amd import _lib, batch as B, workload as W
+from svfsiplus_amd.newton import DeviceLhs, FluidNewton, fsils_solve, ls_create
+
+pytestmark = pytest.mark.gpu
+
+
+def _meshes(count=3, tets_max=1.2e4, seed=5):
+    return B.make_batch(W.bifurcation_batch(count, seed=seed, tets_max=tets_max))
+
+
+def _assembled_union(torch, meshes, phys):
+    bn = B.BatchNewton(meshes, phys)
+    fn = bn.fn
+    lib = _lib.load()
+    from svfsiplus_amd.newton import _stream, b_assem_neu_bc, construct_fluid
+    _lib.check(lib.svg_pici(bn.pipe.nNo, phys.am, phys.af, fn.Ao.data_ptr(), fn.An.data_ptr(), fn.Yo.data_ptr(),
+                            fn.Yn.data_ptr(), fn.Ag.data_ptr(), fn.Yg.data_ptr(), _stream()))
+    fn.R.zero_()
+    fn.Val.zero_()
+    construct_fluid(fn.msh, phys, fn.Ag, fn.Yg, fn.R, fn.Val)
+    b_assem_neu_bc(fn.outlet, phys, fn.Yg, fn.R, fn.Val)
+    torch.cuda.synchronize()
+    return bn
+
+
+@pytest.mark.parametrize("precision", [_lib.PRECISION_FP64, _lib.PRECISION_MIXED])
+def test_batch_solve_equals_individual_solves(torch_cuda, precision):
+    torch = torch_cuda
+    meshes = _meshes()
+    phys = W.Physics()
+    bn = _assembled_union(torch, meshes, phys)
+    R0, V0 = bn.fn.R.clone(), bn.fn.Val.clone()
+    # per-mesh systems: the union's rows and blocks of each mesh
+    boff = np.concatenate([[0], np.cumsum([m.nnz for m in meshes])])
+    singles = []
+    for q, m in enumerate(meshes):
+        a, b = int(bn.noff[q]), int(bn.noff[q + 1])
+        Rq = R0[a:b].clone()
+        Vq = V0[boff[q]:boff[q + 1]].clone()
+        ls = ls_create(_lib.LS_GMRES, relTol=1e-6, absTol=1e-30, maxItr=1000, dimKry=50)
+        fsils_solve(DeviceLhs(m), ls, 4, Rq, Vq, precision=precision)
+        singles.append((Rq, ls.RI.itr, ls.RI.iNorm))
+    Rb, Vb = R0.clone(), V0.clone()
+    _lib.check(_lib.load().svg_batch_fsils_solve(bn.h, C.byref(bn.proto), Rb.data_ptr(), Vb.data_ptr(), precision,
+                                                 bn.out, None))
+    torch.cuda.synchronize()
+    for q, (Rq, itr, iN) in enumerate(singles):
+        a, b = int(bn.noff[q]), int(bn.noff[q + 1])
+        diff = float((Rb[a:b] - Rq).abs().max())
+        print(f"mesh {q}: nNo {b - a} itr batch {bn.out[q].itr} single {itr} iNorm {bn.out[q].iNorm:.6e}/{iN:.6e} "
+              f"max|dx| {diff:.2e}")
+        assert bn.out[q].itr == itr
+        assert bn.out[q].iNorm == iN
+        assert diff == 0.0
+
+
+def test_batch_newton_time_step(torch_cuda):
+    """Three Newton iterations of every mesh at once against the same iterations
+    run on each mesh alone (assembly sums differ only in element order)."""
+    meshes = _meshes(4, seed=9)
+    phys = W.Physics()
+    bn = B.BatchNewton(meshes, phys)
+    recs = bn.time_step(3)
+    for q, m in enumerate(meshes):
+        fn = FluidNewton(m, phys, ls_create(_lib.LS_GMRES, relTol=1e-6, absTol=1e-30, maxItr=1000, dimKry=50))
+        h = fn.time_step()
+        for it in range(3):
+            rb, rs = recs[it][q], h[it]
+            print(f"mesh {q} newton {it + 1}: batch itr {rb['itr']} iNorm {rb['iNorm']:.6e} | "
+                  f"single itr {rs['ls_itr']} iNorm {rs['iNorm']:.6e}")
+            assert abs(rb["itr"] - rs["ls_itr"]) <= max(2, 0.02 * rs["ls_itr"])
+            assert abs(rb["iNorm"] - rs["iNorm"]) <= 1e-8 * rs["iNorm"]

@@ -104,3 +104,38 @@ def test_cpp_driver_built_and_usage():
     assert os.path.exists(exe), "run python -m svfsiplus_amd.build"
     r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
     assert r.returncode == 2 and "usage" in r.stderr
+
+
+def test_bifurcation_mesh_valid_and_union():
+    """Bifurcation generator: positive tets, a manifold conforming mesh whose
+    boundary faces are exactly inlet, wall and the two outlets; the batch union
+    keeps each mesh's pattern block-diagonal."""
+    from collections import Counter
+    from svfsiplus_amd import batch as B
+    for sp in W.bifurcation_batch(3, seed=3, tets_max=2e4):
+        p = W.make_bifurcation(**sp)
+        x, T = p.x, p.IEN
+        det = np.einsum("ij,ij->i", x[T[:, 1]] - x[T[:, 0]],
+                        np.cross(x[T[:, 2]] - x[T[:, 0]], x[T[:, 3]] - x[T[:, 0]]))
+        assert det.min() > 0
+        faces = Counter()
+        for k in range(4):
+            for f in map(tuple, np.sort(np.delete(T, k, axis=1), axis=1)):
+                faces[f] += 1
+        assert max(faces.values()) == 2
+        kinds = Counter()
+        for f, c in faces.items():
+            if c == 1:
+                fl = p.flags[list(f)]
+                kinds["inlet" if np.all(fl & 1) else "outlet" if np.all(fl & 4) else
+                      "wall" if np.all(fl & 2) else "other"] += 1
+        assert kinds["other"] == 0 and kinds["inlet"] == 2 * sp["n"] ** 2 and kinds["outlet"] == 2 * sp["n"] ** 2
+    ms = B.make_batch(W.bifurcation_batch(3, seed=4, tets_max=1e4))
+    u, off = B.union(ms)
+    nnz = C.c_int()
+    _lib.check(_lib.load().svg_csr_pattern_count(u.nNo, u.nEl, 4, u.IEN.ctypes.data, C.byref(nnz)))
+    rp = np.empty(u.nNo + 1, np.int32)
+    cp = np.empty(nnz.value, np.int32)
+    _lib.check(_lib.load().svg_csr_pattern(u.nNo, u.nEl, 4, u.IEN.ctypes.data, rp.ctypes.data, cp.ctypes.data))
+    assert np.array_equal(rp, u.rowPtr) and np.array_equal(cp, u.colPtr)
+    assert off[-1] == u.nNo
