@@ -151,28 +151,34 @@ __global__ __launch_bounds__(BS) void k_encode(int n, int rows, const double* __
   __shared__ bf16 Ws[F * US];
   if (skip && *skip) return;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int r0 = blockIdx.x * TILE;
-  for (int q = t; q < F * NIN; q += BS) Ws[(q / NIN) * US + q % NIN] = W0[q];
-  load_u(U, t, r0, rows, n, V, sc, Dg);
-  __syncthreads();
-  v4f acc[4];
+  const int ntiles = (rows + TILE - 1) / TILE;
+  for (int q = t; q < F * NIN; q += BS) Ws[(q / NIN) * US + q % NIN] = W0[q];  // once per workgroup
+  for (int tl = blockIdx.x; tl < ntiles; tl += gridDim.x) {
+    const int r0 = tl * TILE;
+    load_u(U, t, r0, rows, n, V, sc, Dg);
+    __syncthreads();
+    v4f acc[4];
 #pragma unroll
-  for (int qi = 0; qi < 4; qi++) acc[qi] = v4f{0.f, 0.f, 0.f, 0.f};
-  mma<TILE, F, NIN>(U, US, 1, Ws, 1, US, acc, wave, lane);  // B(k,n) = W0[n][k]
+    for (int qi = 0; qi < 4; qi++) acc[qi] = v4f{0.f, 0.f, 0.f, 0.f};
+    mma<TILE, F, NIN>(U, US, 1, Ws, 1, US, acc, wave, lane);  // B(k,n) = W0[n][k]
 #pragma unroll
-  for (int qi = 0; qi < 4; qi++)
+    for (int qi = 0; qi < 4; qi++)
 #pragma unroll
-    for (int v = 0; v < 4; v++) {
-      int row, col;
-      acc_rc<F>(wave, lane, qi, v, row, col);
-      if (r0 + row < rows) H[(size_t)(r0 + row) * F + col] = f2bf(acc[qi][v]);
-    }
+      for (int v = 0; v < 4; v++) {
+        int row, col;
+        acc_rc<F>(wave, lane, qi, v, row, col);
+        if (r0 + row < rows) H[(size_t)(r0 + row) * F + col] = f2bf(acc[qi][v]);
+      }
+    __syncthreads();
+  }
 }
 
 // Messages of one layer: M[r] = sum_{j in row i} A_ij (x) H[j] through the
-// BF16 block matrix, one lane per (row, channel) computing the channel's 4
-// dofs, neighbours in row order (FP32 accumulation), four neighbours' loads
-// issued together. No LDS, so the gather runs at full occupancy.
+// BF16 block matrix. One lane per (row, group of 4 channels) computing 16
+// outputs, so a wavefront keeps 16 rows in flight; neighbours in row order
+// (FP32 accumulation, per output the same sequence of FMAs as a per-channel
+// lane), the next batch's column indices loaded while the current batch is
+// consumed. No LDS, so the gather runs at full occupancy.
 template <int UNR>
 __global__ __launch_bounds__(256) void k_msg(int n, int rows, const int* __restrict__ rowPtr,
                                              const int* __restrict__ colPtr, const bf16* __restrict__ A16,
@@ -181,67 +187,76 @@ __global__ __launch_bounds__(256) void k_msg(int n, int rows, const int* __restr
 {
   if (skip && *skip) return;
   const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
-  const int r = (int)(idx >> 4), c = (int)(idx & 15);
+  const int r = (int)(idx >> 2), g = (int)(idx & 3);
   if (r >= rows) return;
   const int b = r / n, i = r - b * n;
-  const bf16* Hb = H + (size_t)b * n * F + 4 * c;
+  const bf16* Hb = H + (size_t)b * n * F + 16 * g;
   const int s = rowPtr[i], e = rowPtr[i + 1];
-  float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f;
-  auto acc = [&](const uint4& a0, const uint4& a1, const uint2& hv) {
-    const float h0 = lo(hv.x), h1 = hi(hv.x), h2 = lo(hv.y), h3 = hi(hv.y);
-    m0 = fmaf(lo(a0.x), h0, m0);
-    m0 = fmaf(hi(a0.x), h1, m0);
-    m0 = fmaf(lo(a0.y), h2, m0);
-    m0 = fmaf(hi(a0.y), h3, m0);
-    m1 = fmaf(lo(a0.z), h0, m1);
-    m1 = fmaf(hi(a0.z), h1, m1);
-    m1 = fmaf(lo(a0.w), h2, m1);
-    m1 = fmaf(hi(a0.w), h3, m1);
-    m2 = fmaf(lo(a1.x), h0, m2);
-    m2 = fmaf(hi(a1.x), h1, m2);
-    m2 = fmaf(lo(a1.y), h2, m2);
-    m2 = fmaf(hi(a1.y), h3, m2);
-    m3 = fmaf(lo(a1.z), h0, m3);
-    m3 = fmaf(hi(a1.z), h1, m3);
-    m3 = fmaf(lo(a1.w), h2, m3);
-    m3 = fmaf(hi(a1.w), h3, m3);
-  };
-  // the row's first 32 column indices, loaded once by its 16 lanes and
-  // handed out by shuffles within the 16-lane group
-  const int cnt = e - s;
-  const int jA = c < cnt ? colPtr[s + c] : 0;
-  const int jB = 16 + c < cnt ? colPtr[s + 16 + c] : 0;
-  auto col = [&](int kk) {  // kk: index within the row, uniform over the 16 lanes
-    const int a = __shfl(jA, kk & 15, 16), b2 = __shfl(jB, kk & 15, 16);
-    return kk < 16 ? a : (kk < 32 ? b2 : colPtr[s + kk]);
+  float m[4][4];
+#pragma unroll
+  for (int c = 0; c < 4; c++)
+#pragma unroll
+    for (int d = 0; d < 4; d++) m[c][d] = 0.f;
+  auto acc = [&](const uint4& a0, const uint4& a1, const uint4& h0, const uint4& h1) {
+    const float A[4][4] = {{lo(a0.x), hi(a0.x), lo(a0.y), hi(a0.y)},
+                           {lo(a0.z), hi(a0.z), lo(a0.w), hi(a0.w)},
+                           {lo(a1.x), hi(a1.x), lo(a1.y), hi(a1.y)},
+                           {lo(a1.z), hi(a1.z), lo(a1.w), hi(a1.w)}};
+    const float hv[4][4] = {{lo(h0.x), hi(h0.x), lo(h0.y), hi(h0.y)},
+                            {lo(h0.z), hi(h0.z), lo(h0.w), hi(h0.w)},
+                            {lo(h1.x), hi(h1.x), lo(h1.y), hi(h1.y)},
+                            {lo(h1.z), hi(h1.z), lo(h1.w), hi(h1.w)}};
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+#pragma unroll
+      for (int d = 0; d < 4; d++) {
+        float t = m[c][d];
+#pragma unroll
+        for (int q = 0; q < 4; q++) t = fmaf(A[d][q], hv[c][q], t);
+        m[c][d] = t;
+      }
   };
   int k = s;
+  int jn[UNR];
+#pragma unroll
+  for (int q = 0; q < UNR; q++) jn[q] = k + q < e ? colPtr[k + q] : 0;
   for (; k + UNR <= e; k += UNR) {
     int j[UNR];
-    uint4 a0[UNR], a1[UNR];
-    uint2 hv[UNR];
 #pragma unroll
-    for (int q = 0; q < UNR; q++) j[q] = col(k + q - s);
+    for (int q = 0; q < UNR; q++) j[q] = jn[q];
+    uint4 a0[UNR], a1[UNR], h0[UNR], h1[UNR];
 #pragma unroll
     for (int q = 0; q < UNR; q++) {
       a0[q] = reinterpret_cast<const uint4*>(A16 + 16 * (size_t)(k + q))[0];
       a1[q] = reinterpret_cast<const uint4*>(A16 + 16 * (size_t)(k + q))[1];
-      hv[q] = *reinterpret_cast<const uint2*>(Hb + (size_t)j[q] * F);
+      h0[q] = reinterpret_cast<const uint4*>(Hb + (size_t)j[q] * F)[0];
+      h1[q] = reinterpret_cast<const uint4*>(Hb + (size_t)j[q] * F)[1];
     }
 #pragma unroll
-    for (int q = 0; q < UNR; q++) acc(a0[q], a1[q], hv[q]);
+    for (int q = 0; q < UNR; q++) jn[q] = k + UNR + q < e ? colPtr[k + UNR + q] : 0;
+#pragma unroll
+    for (int q = 0; q < UNR; q++) acc(a0[q], a1[q], h0[q], h1[q]);
   }
-  for (; k < e; k++) {
-    const int jj = col(k - s);
+  for (int q = 0; k < e; k++, q++) {
+    const int jj = jn[q];
     const uint4 a0 = reinterpret_cast<const uint4*>(A16 + 16 * (size_t)k)[0];
     const uint4 a1 = reinterpret_cast<const uint4*>(A16 + 16 * (size_t)k)[1];
-    const uint2 hv = *reinterpret_cast<const uint2*>(Hb + (size_t)jj * F);
-    acc(a0, a1, hv);
+    const uint4 h0 = reinterpret_cast<const uint4*>(Hb + (size_t)jj * F)[0];
+    const uint4 h1 = reinterpret_cast<const uint4*>(Hb + (size_t)jj * F)[1];
+    acc(a0, a1, h0, h1);
   }
-  uint2 pk;
-  pk.x = (unsigned)f2bf(m0) | ((unsigned)f2bf(m1) << 16);
-  pk.y = (unsigned)f2bf(m2) | ((unsigned)f2bf(m3) << 16);
-  *reinterpret_cast<uint2*>(M + (size_t)r * F + 4 * c) = pk;
+  uint4 o0, o1;
+  o0.x = (unsigned)f2bf(m[0][0]) | ((unsigned)f2bf(m[0][1]) << 16);
+  o0.y = (unsigned)f2bf(m[0][2]) | ((unsigned)f2bf(m[0][3]) << 16);
+  o0.z = (unsigned)f2bf(m[1][0]) | ((unsigned)f2bf(m[1][1]) << 16);
+  o0.w = (unsigned)f2bf(m[1][2]) | ((unsigned)f2bf(m[1][3]) << 16);
+  o1.x = (unsigned)f2bf(m[2][0]) | ((unsigned)f2bf(m[2][1]) << 16);
+  o1.y = (unsigned)f2bf(m[2][2]) | ((unsigned)f2bf(m[2][3]) << 16);
+  o1.z = (unsigned)f2bf(m[3][0]) | ((unsigned)f2bf(m[3][1]) << 16);
+  o1.w = (unsigned)f2bf(m[3][2]) | ((unsigned)f2bf(m[3][3]) << 16);
+  uint4* out = reinterpret_cast<uint4*>(M + (size_t)r * F + 16 * g);
+  out[0] = o0;
+  out[1] = o1;
 }
 
 // Dense part of one layer on a tile of 64 node rows: [h | m] and W staged in
@@ -258,56 +273,63 @@ __global__ __launch_bounds__(BS) void k_dense(int rows, const bf16* __restrict__
   __shared__ __align__(16) bf16 Ws[F * XS];
   if (skip && *skip) return;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int r0 = blockIdx.x * TILE;
+  const int ntiles = (rows + TILE - 1) / TILE;
+  // the layer's weights once per workgroup; the workgroup then walks tiles
   for (int q = t; q < F * 2 * F / 8; q += BS) {  // 8 bf16 per load
     const int o = (q >> 4), k8 = (q & 15) * 8;
     *reinterpret_cast<uint4*>(Ws + o * XS + k8) = reinterpret_cast<const uint4*>(W)[q];
   }
-  for (int q = t; q < TILE * F / 8; q += BS) {
-    const int row = q >> 3, f8 = (q & 7) * 8, r = r0 + row;
-    uint4 hv = {0u, 0u, 0u, 0u}, mv = {0u, 0u, 0u, 0u};
-    if (r < rows) {
-      hv = *reinterpret_cast<const uint4*>(Hin + (size_t)r * F + f8);
-      mv = *reinterpret_cast<const uint4*>(M + (size_t)r * F + f8);
-    }
-    *reinterpret_cast<uint4*>(X + row * XS + f8) = hv;
-    *reinterpret_cast<uint4*>(X + row * XS + F + f8) = mv;
-  }
-  __syncthreads();
-  v4f acc[4];
+  float bv[4];
 #pragma unroll
   for (int qi = 0; qi < 4; qi++) {
     int row, col;
     acc_rc<F>(wave, lane, qi, 0, row, col);
-    const float bv = bias[col];
-    acc[qi] = v4f{bv, bv, bv, bv};
+    bv[qi] = bias[col];
   }
-  mma<TILE, F, 2 * F>(X, XS, 1, Ws, 1, XS, acc, wave, lane);  // B(k,n) = W[n][k]
-  __syncthreads();
-#pragma unroll
-  for (int qi = 0; qi < 4; qi++)
-#pragma unroll
-    for (int v = 0; v < 4; v++) {
-      int row, col;
-      acc_rc<F>(wave, lane, qi, v, row, col);
-      const float p = acc[qi][v];
-      const bf16 ho = f2bf(bf2f(X[row * XS + col]) + fmaxf(p, 0.f));
-      X[row * XS + col] = ho;
-      const int r = r0 + row;
+  for (int tl = blockIdx.x; tl < ntiles; tl += gridDim.x) {
+    const int r0 = tl * TILE;
+    for (int q = t; q < TILE * F / 8; q += BS) {
+      const int row = q >> 3, f8 = (q & 7) * 8, r = r0 + row;
+      uint4 hv = {0u, 0u, 0u, 0u}, mv = {0u, 0u, 0u, 0u};
       if (r < rows) {
-        Hout[(size_t)r * F + col] = ho;
-        if (Pst) Pst[(size_t)r * F + col] = f2bf(p);
+        hv = *reinterpret_cast<const uint4*>(Hin + (size_t)r * F + f8);
+        mv = *reinterpret_cast<const uint4*>(M + (size_t)r * F + f8);
       }
+      *reinterpret_cast<uint4*>(X + row * XS + f8) = hv;
+      *reinterpret_cast<uint4*>(X + row * XS + F + f8) = mv;
     }
-  if (Wd) {
     __syncthreads();
-    const int node = t >> 2, d = t & 3, r = r0 + node;
-    if (r < rows) {
-      float y = 0.f;
-      for (int f = 0; f < F; f++) y = fmaf(bf2f(Wd[d * F + f]), bf2f(X[node * XS + f]), y);
-      const double z = V[(size_t)r * 4 + d] + (double)y * inv_sc;
-      Z[(size_t)r * 4 + d] = z;
-      if (Z32) Z32[(size_t)r * 4 + d] = (float)z;
+    v4f acc[4];
+#pragma unroll
+    for (int qi = 0; qi < 4; qi++) acc[qi] = v4f{bv[qi], bv[qi], bv[qi], bv[qi]};
+    mma<TILE, F, 2 * F>(X, XS, 1, Ws, 1, XS, acc, wave, lane);  // B(k,n) = W[n][k]
+    __syncthreads();
+#pragma unroll
+    for (int qi = 0; qi < 4; qi++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        int row, col;
+        acc_rc<F>(wave, lane, qi, v, row, col);
+        const float p = acc[qi][v];
+        const bf16 ho = f2bf(bf2f(X[row * XS + col]) + fmaxf(p, 0.f));
+        X[row * XS + col] = ho;
+        const int r = r0 + row;
+        if (r < rows) {
+          Hout[(size_t)r * F + col] = ho;
+          if (Pst) Pst[(size_t)r * F + col] = f2bf(p);
+        }
+      }
+    __syncthreads();
+    if (Wd) {
+      const int node = t >> 2, d = t & 3, r = r0 + node;
+      if (r < rows) {
+        float y = 0.f;
+        for (int f = 0; f < F; f++) y = fmaf(bf2f(Wd[d * F + f]), bf2f(X[node * XS + f]), y);
+        const double z = V[(size_t)r * 4 + d] + (double)y * inv_sc;
+        Z[(size_t)r * 4 + d] = z;
+        if (Z32) Z32[(size_t)r * 4 + d] = (float)z;
+      }
+      __syncthreads();
     }
   }
 }
@@ -659,20 +681,21 @@ static int lpc_forward(svg_lpc* N, const svg_lhs* L, int nvec, const double* V, 
   const unsigned nt = blocks_for(rows, TILE);
   const bf16* pb = N->pb;
   // algorithmic bytes: V and the diagonal blocks read, h written
-  launch_b("lpc_encode", (32.0 + 32.0 + 2.0 * F) * rows, k_encode, dim3(nt), dim3(BS), 0, s, n, rows, V, sc,
+  launch_b("lpc_encode", (32.0 + 32.0 + 2.0 * F) * rows, k_encode, dim3(std::min(nt, 2048u)), dim3(BS), 0, s, n,
+           rows, V, sc,
            (const bf16*)L->d_diag16, pb + OFF_W0, N->H[0], skip);
-  const unsigned nm = blocks_for((size_t)rows * 16, 256);
+  const unsigned nm = blocks_for((size_t)rows * 4, 256);
   for (int l = 0; l < NL; l++) {
     const bool last = l == NL - 1;
     const bf16* Hin = train ? N->H[l] : N->H[l & 1];
     bf16* Hout = train ? N->H[l + 1] : N->H[(l + 1) & 1];
     bf16* Ml = train ? N->M[l] : N->M[0];
     // BF16 blocks + colPtr per block (per sample), rowPtr + h + m out per row
-    launch_b("lpc_msg", (32.0 + 4.0) * L->nnz * nvec + (4.0 + 2.0 * F) * rows, k_msg<4>, dim3(nm), dim3(256), 0, s, n,
+    launch_b("lpc_msg", (32.0 + 4.0) * L->nnz * nvec + (4.0 + 2.0 * F) * rows, k_msg<2>, dim3(nm), dim3(256), 0, s, n,
              rows, (const int*)L->d_rowPtr, (const int*)L->d_colPtr, (const bf16*)L->d_val16, Hin, Ml, skip);
     // h, m in, h out (+ p for training; + V in, z out for the last layer)
     const double db = 6.0 * F * rows + (train ? 2.0 * F * rows : 0.0) + (last ? (64.0 + (Z32 ? 16.0 : 0.0)) * rows : 0.0);
-    launch_b("lpc_dense", db, k_dense, dim3(nt), dim3(BS), 0, s, rows, Hin, (const bf16*)Ml, Hout,
+    launch_b("lpc_dense", db, k_dense, dim3(std::min(nt, 2048u)), dim3(BS), 0, s, rows, Hin, (const bf16*)Ml, Hout,
              pb + OFF_L + l * LSZ, (const float*)(N->p + OFF_L + l * LSZ + F * 2 * F),
              train ? N->P[l] : (bf16*)nullptr, last ? pb + OFF_WD : (const bf16*)nullptr, V,
              last ? Z : (double*)nullptr, last ? Z32 : (float*)nullptr, 1.0 / sc, skip);
