@@ -240,7 +240,9 @@ typedef struct {
   int method;
   int fallback_window;
   int fallback_itr; /* out */
-  int reserved[4];
+  int net_period;   /* learned: the network preconditions Arnoldi steps i with i % net_period == 0, the
+                       others use the diagonal scaling alone (0 or 1: every step) */
+  int reserved[3];
   svg_lpc* lpc;
 } svg_solve_opts;
 

@@ -58,7 +58,7 @@ class SubLs(C.Structure):
 
 class SolveOpts(C.Structure):
     _fields_ = [("precision", C.c_int), ("method", C.c_int), ("fallback_window", C.c_int),
-                ("fallback_itr", C.c_int), ("reserved", C.c_int * 4), ("lpc", C.c_void_p)]
+                ("fallback_itr", C.c_int), ("net_period", C.c_int), ("reserved", C.c_int * 3), ("lpc", C.c_void_p)]
 
 
 PRECISION_FP64, PRECISION_MIXED = 0, 1

@@ -492,6 +492,19 @@ __global__ __launch_bounds__(256) void k_gs_update(int n, double* __restrict__ u
   if (u32) u32[k] = (float)v;
 }
 
+// z_i = u_i (and its FP32 copy): an Arnoldi step of the flexible solve that
+// the network does not precondition.
+__global__ __launch_bounds__(256) void k_zcopy(int n, const double* __restrict__ u, double* __restrict__ z,
+                                               float* __restrict__ z32, const GmState* __restrict__ st)
+{
+  if (st->done) return;
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (size_t)n) return;
+  const double v = u[k];
+  z[k] = v;
+  if (z32) z32[k] = (float)v;
+}
+
 // u0 = u0 / err0 (Array division, Array.h:793-805)
 __global__ __launch_bounds__(256) void k_div(int n, double* __restrict__ u, const GmState* __restrict__ st,
                                              float* __restrict__ u32)
@@ -1030,7 +1043,7 @@ int spmv_residual(const svg_lhs* L, const double* K, const double* U, const doub
 // then the back substitution and X += sum y_j u_j. Returns after the cycle's
 // state has been copied to the host.
 static int run_cycle(const Op& A, KrylovWork& k, int sD, FinArgs fa, double* X, hipStream_t s,
-                     const float* K32 = nullptr, svg_lpc* net = nullptr)
+                     const float* K32 = nullptr, svg_lpc* net = nullptr, int net_period = 1)
 {
   const bool mixed = K32 != nullptr;
   const bool flex = net != nullptr;
@@ -1047,8 +1060,13 @@ static int run_cycle(const Op& A, KrylovWork& k, int sD, FinArgs fa, double* X, 
     // flexible GMRES: the SpMV acts on z_i = M(u_i) (and its FP32 copy in u32)
     const double* sin = k.u + k.ld * i;
     if (flex) {
-      int rc = lpc_apply_dev(net, A.L, k.u + k.ld * i, k.z + k.ld * i, mixed ? k.u32 : nullptr, &k.st->done, s);
-      if (rc) return rc;
+      if (net_period <= 1 || i % net_period == 0) {
+        int rc = lpc_apply_dev(net, A.L, k.u + k.ld * i, k.z + k.ld * i, mixed ? k.u32 : nullptr, &k.st->done, s);
+        if (rc) return rc;
+      } else {
+        launch("zcopy", k_zcopy, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, (const double*)(k.u + k.ld * i),
+               k.z + k.ld * i, mixed ? k.u32 : (float*)nullptr, (const GmState*)k.st);
+      }
       sin = k.z + k.ld * i;
     }
     if (mixed) {
@@ -1209,7 +1227,7 @@ static int gmres_v_mixed(svg_lhs* L, svg_subls* ls, const double* Val, double* R
 // solve goes on from the current X with the plain (diagonal-only) GMRES
 // restarts of the same precision.
 static int gmres_v_learned(svg_lhs* L, svg_subls* ls, const double* Val, double* R, bool mixed, svg_lpc* net,
-                           int window, int* fallback_itr, hipStream_t s)
+                           int window, int net_period, int* fallback_itr, hipStream_t s)
 {
   if (ls->sD < 1 || ls->sD > 250) return set_error(SVG_ERR_ARG, "gmres_v: Krylov dimension must be in [1, 250]");
   auto& k = L->kw;
@@ -1260,7 +1278,7 @@ static int gmres_v_learned(svg_lhs* L, svg_subls* ls, const double* Val, double*
       return rc;
     launch("div", k_div, dim3(nbv), dim3(256), 0, s, n, k.u, (const GmState*)k.st,
            (mixed && !pc) ? k.u32 : (float*)nullptr);
-    if ((rc = run_cycle(A, k, sD, fa, k.X, s, mixed ? L->d_val32 : nullptr, pc))) return rc;
+    if ((rc = run_cycle(A, k, sD, fa, k.X, s, mixed ? L->d_val32 : nullptr, pc, net_period))) return rc;
     if (k.h_st->divzero) return set_error(SVG_ERR_ARG, "Array Divide by zero.");
     ls->itr = k.h_st->itr;
     ls->fNorm = k.h_st->fNorm;
@@ -2143,7 +2161,8 @@ int svg_fsils_solve_ex(svg_lhs* L, svg_ls* ls, int dof, double* Ri, double* Val,
     case SVG_LS_GMRES:
       if (learned)
         rc = gmres_v_learned(L, &ls->RI, Val, Ri, opts->precision == SVG_PRECISION_MIXED, opts->lpc,
-                             opts->fallback_window > 0 ? opts->fallback_window : 20, &opts->fallback_itr, s);
+                             opts->fallback_window > 0 ? opts->fallback_window : 20, opts->net_period,
+                             &opts->fallback_itr, s);
       else
         rc = (opts && opts->precision == SVG_PRECISION_MIXED) ? gmres_v_mixed(L, &ls->RI, Val, Ri, s)
                                                                : gmres_v_dev(L, &ls->RI, dof, Val, Ri, s);

@@ -152,7 +152,8 @@ def construct_fluid(msh: DeviceMesh, phys: Physics, Ag, Yg, R, Val, Bf=None):
 
 
 def fsils_solve(lhs: DeviceLhs, ls: _lib.Ls, dof: int, Ri, Val, prec: int = _lib.PREC_FSILS, incL=None, res=None,
-                precision: int = _lib.PRECISION_FP64, lpc=None, fallback_window: int = 0, info: dict | None = None):
+                precision: int = _lib.PRECISION_FP64, lpc=None, fallback_window: int = 0, net_period: int = 1,
+                info: dict | None = None):
     """Reference signature: fsils_solve(lhs, ls, dof, Ri, Val, prec, incL, res) (solve.cpp:50).
     precision = PRECISION_MIXED selects the FP32-SpMV / FP64-refined GMRES; a
     network `lpc` (LearnedPC) selects the learned-preconditioned flexible GMRES
@@ -166,6 +167,7 @@ def fsils_solve(lhs: DeviceLhs, ls: _lib.Ls, dof: int, Ri, Val, prec: int = _lib
         opts.method = _lib.METHOD_LEARNED
         opts.lpc = lpc.h.value if hasattr(lpc.h, "value") else lpc.h
         opts.fallback_window = fallback_window
+        opts.net_period = net_period
     _lib.check(_lib.load().svg_fsils_solve_ex(lhs.h, C.byref(ls), dof, _ptr(Ri), _ptr(Val), prec, nf,
                                               inc.ctypes.data, None if rs is None else rs.ctypes.data, C.byref(opts),
                                               _stream()))
@@ -197,7 +199,7 @@ class FluidNewton:
     """One fluid equation on one pipe, all state resident in HBM."""
 
     def __init__(self, pipe: Pipe, phys: Physics, ls: _lib.Ls, eq: EqState | None = None, device="cuda",
-                 precision: int = _lib.PRECISION_FP64, lpc=None, outlet: bool = True):
+                 precision: int = _lib.PRECISION_FP64, lpc=None, outlet: bool = True, net_period: int = 1):
         import torch
         if not torch.cuda.is_available():
             raise RuntimeError("FluidNewton needs a GPU (the HIP library has no CPU path)")
@@ -205,6 +207,7 @@ class FluidNewton:
         self.pipe, self.phys, self.ls = pipe, phys, ls
         self.precision = precision
         self.lpc = lpc
+        self.net_period = net_period
         self.eq = eq or EqState()
         self.lhs = DeviceLhs(pipe)
         self.msh = DeviceMesh(self.lhs, pipe)
@@ -248,7 +251,8 @@ class FluidNewton:
         if self.outlet is not None:
             b_assem_neu_bc(self.outlet, ph, self.Yg, self.R, self.Val)
         info = {}
-        fsils_solve(self.lhs, ls, 4, self.R, self.Val, precision=self.precision, lpc=self.lpc, info=info)
+        fsils_solve(self.lhs, ls, 4, self.R, self.Val, precision=self.precision, lpc=self.lpc,
+                    net_period=self.net_period, info=info)
         _lib.check(lib.svg_picc_update(self.pipe.nNo, ph.gam, ph.dt, _ptr(self.R), _ptr(self.An), _ptr(self.Yn), s))
         # Newton convergence, pic.cpp:236-281
         eps = np.finfo(np.float64).eps

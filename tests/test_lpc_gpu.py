@@ -97,13 +97,13 @@ def test_lpc_training_reduces_loss(torch_cuda):
     assert after < 0.8 * before
 
 
-def _solve(sys, lpc=None, precision=_lib.PRECISION_FP64, window=0, tol=1e-6):
+def _solve(sys, lpc=None, precision=_lib.PRECISION_FP64, window=0, tol=1e-6, period=1):
     from svfsiplus_amd.newton import fsils_solve, ls_create
     x = sys.R.clone()
     ls = ls_create(_lib.LS_GMRES, relTol=tol, absTol=1e-30, maxItr=1000, dimKry=50)
     info = {}
     fsils_solve(sys.lhs, ls, 4, x, sys.Val, prec=_lib.PREC_NONE, precision=precision, lpc=lpc,
-                fallback_window=window, info=info)
+                fallback_window=window, net_period=period, info=info)
     import torch
     rel = float(torch.linalg.vector_norm(sys.residual(x)) / torch.linalg.vector_norm(sys.R))
     return x, ls, rel, info.get("fallback_itr", -1)
@@ -111,6 +111,7 @@ def _solve(sys, lpc=None, precision=_lib.PRECISION_FP64, window=0, tol=1e-6):
 
 @pytest.mark.parametrize("precision", [_lib.PRECISION_FP64, _lib.PRECISION_MIXED])
 def test_fgmres_identity_network_equals_gmres(torch_cuda, precision):
+    """With Wd = 0 the flexible solve is the plain GMRES (z_i = u_i)."""
     torch = torch_cuda
     sys = _sys(torch, n=6)
     p = L.init_params(0)  # Wd = 0: M = identity on the scaled system
@@ -147,8 +148,9 @@ def test_trained_network_gain(torch_cuda):
     L.train(net, data, steps=600, lr=2e-3, batch=6)
     held = _sys(torch, n=6, seed=99, mu=0.05, peak=25.0, sten=0.3)
     _, ls0, rel0, _ = _solve(held)
-    _, ls1, rel1, fb = _solve(held, lpc=net, window=20)
-    print(f"trained network on held-out pipe: plain itr {ls0.RI.itr} | learned itr {ls1.RI.itr} "
-          f"(fallback {fb}) rel {rel0:.2e} / {rel1:.2e}")
-    assert ls1.RI.suc == 1 and rel1 < 1e-6
-    assert ls1.RI.itr < ls0.RI.itr
+    for period in (1, 5):
+        _, ls1, rel1, fb = _solve(held, lpc=net, window=20, period=period)
+        print(f"trained network on held-out pipe: plain itr {ls0.RI.itr} | learned (network every {period}. step) "
+              f"itr {ls1.RI.itr} (fallback {fb}) rel {rel0:.2e} / {rel1:.2e}")
+        assert ls1.RI.suc == 1 and rel1 < 1e-6
+        assert ls1.RI.itr < ls0.RI.itr
