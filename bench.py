@@ -146,8 +146,21 @@ def kernel_only(a, fn, pipe, lib):
     _lib.check(lib.svg_spmv_tune(fn.lhs.h, fn.Val.data_ptr(), C.byref(best), _stream()))
     lib.svg_prof_reset()
     lib.svg_prof_enable(1)
-    for _ in range(a.launches):
-        _lib.check(lib.svg_spar_mul_vv(fn.lhs.h, 4, fn.Val.data_ptr(), U.data_ptr(), KU.data_ptr(), _stream()))
+    kname = "spmv_bcsr"
+    if a.kernel_only == "spmv32":
+        # the mixed-precision solve's FP32 SpMV on FP32 copies of the same matrix and vector
+        kname = "spmv32"
+        K32 = fn.Val.to(torch.float32)
+        U32 = U.to(torch.float32)
+        nbytes = pipe.nnz * (16 * 4 + 4) + pipe.nNo * (4 + 4 * 4 + 4 * 8)
+        src = torch.empty(nbytes // 2 // 8, dtype=torch.float64, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(a.launches):
+            _lib.check(lib.svg_spmv32(fn.lhs.h, K32.data_ptr(), U32.data_ptr(), KU.data_ptr(), _stream()))
+    else:
+        for _ in range(a.launches):
+            _lib.check(lib.svg_spar_mul_vv(fn.lhs.h, 4, fn.Val.data_ptr(), U.data_ptr(), KU.data_ptr(), _stream()))
     torch.cuda.synchronize()
     lib.svg_prof_enable(0)
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -157,10 +170,10 @@ def kernel_only(a, fn, pipe, lib):
     ev1.record()
     torch.cuda.synchronize()
     cnt, ms = C.c_int(), C.c_double()
-    lib.svg_prof_get(b"spmv_bcsr", C.byref(cnt), C.byref(ms))
+    lib.svg_prof_get(kname.encode(), C.byref(cnt), C.byref(ms))
     avg = ms.value / cnt.value * 1e-3
     copy_avg = ev0.elapsed_time(ev1) / a.launches * 1e-3
-    print(json.dumps({"kernel": "spmv_bcsr", "launches": cnt.value, "bytes_per_launch": nbytes,
+    print(json.dumps({"kernel": kname, "launches": cnt.value, "bytes_per_launch": nbytes,
                       "avg_us": round(avg * 1e6, 2), "achieved_GBs": round(nbytes / avg / 1e9, 1),
                       "variant_avg_us": variants, "tuned_variant": best.value,
                       "copy_bytes_per_launch": 2 * src.numel() * 8, "copy_avg_us": round(copy_avg * 1e6, 2),
@@ -297,7 +310,7 @@ def main():
     ap.add_argument("--batch", type=int, default=256, help="meshes of the batched workload (0: skip it)")
     ap.add_argument("--batch-precision", choices=["fp64", "mixed"], default="mixed")
     ap.add_argument("--no-prof", action="store_true", help="time without per-kernel events")
-    ap.add_argument("--kernel-only", choices=["spmv"], default=None,
+    ap.add_argument("--kernel-only", choices=["spmv", "spmv32"], default=None,
                     help="launch only the block SpMV on the assembled system (for counter runs), "
                          "plus a device copy of the same byte count as the calibration kernel")
     ap.add_argument("--launches", type=int, default=20)

@@ -283,6 +283,11 @@ int svg_spmv_num_variants(int* count);
 int svg_spmv_time_variant(const svg_lhs* lhs, int variant, const double* d_K, const double* d_U, double* d_KU,
                           int launches, double* avg_us, void* stream);
 int svg_spmv_tune(svg_lhs* lhs, const double* d_K, int* best_variant, void* stream);
+/* The FP32 block SpMV of the mixed-precision solve (no counterpart in the
+ * reference): KU = K32 U32 with FP32 blocks and vector, FP32 fused
+ * multiply-adds in the block order of spar_mul.cpp:228-237, KU in FP64.
+ * Asynchronous on the stream. */
+int svg_spmv32(const svg_lhs* lhs, const float* d_K32, const float* d_U32, double* d_KU, void* stream);
 /* precond_diag (precond.cpp:122-257) with every registered face included. */
 int svg_precond_diag(svg_lhs* lhs, int dof, double* d_Val, double* d_R, double* d_W, void* stream);
 /* fsils_dot_v / fsi_ls_normv (dot.cpp:69-117, norm.cpp:67-112); blocking. */

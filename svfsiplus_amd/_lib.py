@@ -100,6 +100,7 @@ _SIGS = {
     "svg_fsils_solve_ex": (_I, [_P, C.POINTER(Ls), _I, _P, _P, _I, _I, _P, _P, C.POINTER(SolveOpts), _P]),
     "svg_spar_mul_vv": (_I, [_P, _I, _P, _P, _P, _P]),
     "svg_precond_diag": (_I, [_P, _I, _P, _P, _P, _P]),
+    "svg_spmv32": (_I, [_P, _P, _P, _P, _P]),
     "svg_spmv_num_variants": (_I, [_IP]),
     "svg_spmv_time_variant": (_I, [_P, _I, _P, _P, _P, _I, _DP, _P]),
     "svg_spmv_tune": (_I, [_P, _P, _IP, _P]),

@@ -194,15 +194,10 @@ static void spmv(const svg_lhs* L, const double* K, const double* U, double* KU,
 // scaled matrix and of the current Arnoldi vector, FP32 accumulation in the
 // reference's block order, one lane per (row, block-row i), W stored in FP64.
 template <int UNR>
-__global__ __launch_bounds__(256) void k_spmv32(int nNo, const int* __restrict__ rowPtr,
-                                                const int* __restrict__ colPtr, const float* __restrict__ K,
-                                                const float* __restrict__ U, double* __restrict__ W,
-                                                const int* __restrict__ skip)
+__device__ __forceinline__ void spmv32_row(int row, int i, const int* __restrict__ rowPtr,
+                                           const int* __restrict__ colPtr, const float* __restrict__ K,
+                                           const float* __restrict__ U, double* __restrict__ W, size_t k)
 {
-  if (skip && *skip) return;
-  const size_t k = (size_t)blockIdx.x * 256 + threadIdx.x;
-  const int row = (int)(k >> 2), i = (int)(k & 3);
-  if (row >= nNo) return;
   const int s = rowPtr[row], e = rowPtr[row + 1];
   float acc = 0.0f;
   int j = s;
@@ -233,6 +228,22 @@ __global__ __launch_bounds__(256) void k_spmv32(int nNo, const int* __restrict__
     acc = __builtin_fmaf(a.w, u.w, acc);
   }
   W[k] = (double)acc;
+}
+
+// SpMV of the mixed-precision solve: W = K32 U32 with FP32 copies of the
+// scaled matrix and of the current Arnoldi vector, FP32 accumulation in the
+// reference's block order, one lane per (row, block-row i), W stored in FP64.
+template <int UNR>
+__global__ __launch_bounds__(256) void k_spmv32(int nNo, const int* __restrict__ rowPtr,
+                                                const int* __restrict__ colPtr, const float* __restrict__ K,
+                                                const float* __restrict__ U, double* __restrict__ W,
+                                                const int* __restrict__ skip)
+{
+  if (skip && *skip) return;
+  const size_t k = (size_t)blockIdx.x * 256 + threadIdx.x;
+  const int row = (int)(k >> 2), i = (int)(k & 3);
+  if (row >= nNo) return;
+  spmv32_row<UNR>(row, i, rowPtr, colPtr, K, U, W, k);
 }
 
 __global__ __launch_bounds__(256) void k_d2f(size_t n, const double* __restrict__ a, float* __restrict__ b)
@@ -1783,6 +1794,22 @@ __global__ __launch_bounds__(RED_BS) void k_bmdot(const int* __restrict__ spanSt
   }
 }
 
+// The FP32 SpMV of the batched solve: rows of systems that converged or
+// stopped skip (their basis vectors are no longer read).
+template <int UNR>
+__global__ __launch_bounds__(256) void k_bspmv32(int nNo, const int* __restrict__ rowPtr,
+                                                 const int* __restrict__ colPtr, const float* __restrict__ K,
+                                                 const float* __restrict__ U, double* __restrict__ W,
+                                                 const int* __restrict__ sysOfNode, const GmState* __restrict__ st)
+{
+  const size_t k = (size_t)blockIdx.x * 256 + threadIdx.x;
+  const int row = (int)(k >> 2), i = (int)(k & 3);
+  if (row >= nNo) return;
+  const GmState& S = st[sysOfNode[row]];
+  if (!S.act || S.done) return;
+  spmv32_row<UNR>(row, i, rowPtr, colPtr, K, U, W, k);
+}
+
 // One workgroup per system: the finaliser on the system's span partials.
 __global__ __launch_bounds__(FIN_BS) void k_bfin(FinArgs A, const int* __restrict__ sysSpanOff, int nsp,
                                                  const double* __restrict__ partial, int m, GmState* st,
@@ -1923,9 +1950,9 @@ static void batch_cycle(svg_batch* B, const double* Val, const double* R, bool m
     double* ui = B->u + B->ld * i;
     double* un = B->u + B->ld * (i + 1);
     if (mixed)
-      hipLaunchKernelGGL(k_spmv32<2>, dim3(blocks_for((size_t)4 * L->nNo, 256)), dim3(256), 0, s, L->nNo,
+      hipLaunchKernelGGL(k_bspmv32<2>, dim3(blocks_for((size_t)4 * L->nNo, 256)), dim3(256), 0, s, L->nNo,
                          (const int*)L->d_rowPtr, (const int*)L->d_colPtr, (const float*)L->d_val32,
-                         (const float*)B->u32, un, (const int*)nullptr);
+                         (const float*)B->u32, un, (const int*)B->d_sysOfNode, (const GmState*)B->st);
     else
       op_apply(A, ui, un, nullptr, nullptr, s);
     batch_mdot(B, un, B->u, B->ld, i + 2, 2, s);
@@ -2365,6 +2392,18 @@ int svg_batch_fsils_solve(svg_batch* B, const svg_subls* proto, double* R, doubl
          (const double*)R, R);
   SVG_HIP(hipGetLastError());
   SVG_HIP(hipStreamSynchronize(s));
+  return SVG_OK;
+}
+
+
+int svg_spmv32(const svg_lhs* L, const float* K32, const float* U32, double* KU, void* stream)
+{
+  if (!L || !K32 || !U32 || !KU) return set_error(SVG_ERR_ARG, "svg_spmv32: null argument");
+  hipStream_t s = (hipStream_t)stream;
+  const double bytes = 68.0 * L->nnz + 36.0 * L->nNo;
+  launch_b("spmv32", bytes, k_spmv32<2>, dim3(blocks_for((size_t)4 * L->nNo, 256)), dim3(256), 0, s, L->nNo,
+           (const int*)L->d_rowPtr, (const int*)L->d_colPtr, K32, U32, KU, (const int*)nullptr);
+  SVG_HIP(hipGetLastError());
   return SVG_OK;
 }
 

@@ -317,3 +317,22 @@ def test_b_assem_neu_bc_matches_oracle(torch_cuda, pipe4, phys):
     assert np.linalg.norm(R1 - R0) <= 1e-12 * np.linalg.norm(R0)
     assert np.linalg.norm(V1 - V0) <= 1e-12 * np.linalg.norm(V0)
 
+
+
+def test_spmv32_matches_fp64_product(torch_cuda, pipe4):
+    """The FP32 block SpMV of the mixed-precision solve against the FP64 product
+    of the same FP32 inputs (FP32 accumulation: relative error ~1e-7)."""
+    from svfsiplus_amd.newton import DeviceLhs
+    torch = torch_cuda
+    rng = np.random.default_rng(3)
+    K = rng.standard_normal((pipe4.nnz, 16)).astype(np.float32)
+    U = rng.standard_normal((pipe4.nNo, 4)).astype(np.float32)
+    L = DeviceLhs(pipe4)
+    dK, dU = torch.tensor(K, device="cuda"), torch.tensor(U, device="cuda")
+    dKU = torch.zeros((pipe4.nNo, 4), dtype=torch.float64, device="cuda")
+    _lib.check(_lib.load().svg_spmv32(L.h, dK.data_ptr(), dU.data_ptr(), dKU.data_ptr(), None))
+    torch.cuda.synchronize()
+    ref = oracle.spar_mul_vv(pipe4, 4, K.astype(np.float64), U.astype(np.float64))
+    rel = np.linalg.norm(dKU.cpu().numpy() - ref) / np.linalg.norm(ref)
+    print(f"spmv32 rel err vs FP64 product {rel:.2e}")
+    assert rel < 1e-6
