@@ -48,6 +48,10 @@ KERNELS = ["spmv_bcsr", "spmv32", "d2f", "mdot", "mdot_fin", "gs_update", "div",
 # kernels whose launches carry algorithmic byte counts (roofline candidates)
 HBM_KERNELS = ["spmv_bcsr", "spmv32", "lpc_msg", "lpc_dense", "lpc_encode"]
 DEFAULT_WEIGHTS = os.path.join(ROOT, "svfsiplus_amd", "data", "lpc_weights.npz")
+# HBM bytes per launch on the 1.2M-tet pipe (n = 34), measured with rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE on `bench.py --kernel-only spmv|spmv32`, each scaled by the same counter's reading on a
+# device copy of known size (FETCH_SIZE reports half of a copy's reads, WRITE_SIZE all of its writes).
+MEASURED_TRAFFIC_N34 = {"spmv_bcsr": 448.1e6, "spmv32": 287.0e6}
 
 
 def kernel_table(lib):
@@ -214,12 +218,13 @@ def timed_run(fn, lib, steps, warmup, prof, label):
             file=sys.stderr, flush=True)
     n_newton = sum(len(h) for h in hists)
     return {"elapsed": t1 - t0, "s_per_newton": (t1 - t0) / n_newton, "n_newton": n_newton, "hists": hists,
-            "kernels": ktab}
+            "kernels": ktab, "Yn": fn.Yn.clone(), "An": fn.An.clone()}
 
 
-def roofline(ktab):
+def roofline(ktab, measured=None):
     """Roofline of the kernel with the largest total time among those with
-    algorithmic byte counts: achieved = bytes per launch / average duration."""
+    algorithmic byte counts: achieved = bytes per launch / average duration;
+    traffic = the counter-measured HBM bytes per launch where known."""
     cands = [(v["total_ms"], k) for k, v in ktab.items() if k in HBM_KERNELS and v.get("bytes")]
     if not cands:
         return None
@@ -229,7 +234,8 @@ def roofline(ktab):
     bpl = v["bytes"] / v["count"]
     ach = bpl / avg_s / 1e9
     return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": k, "bytes_per_launch": round(bpl),
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": (measured or {}).get(k), "kernel": k,
+            "bytes_per_launch": round(bpl),
             "avg_us": round(avg_s * 1e6, 2)}
 
 
@@ -296,7 +302,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n", type=int, default=34, help="pipe cross-section cells (34 -> 1.2M tets)")
+    ap.add_argument("--n", type=int, default=34, help="pipe cross-section cells (34 -> 1.2M tets, 51 -> 4.0M)")
+    ap.add_argument("--stenosis", type=float, default=0.0,
+                    help="sinusoidal stenosis depth (fraction of the radius) over z in [1.25, 1.75]")
     ap.add_argument("--krylov", type=int, default=50)
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--method", choices=["compare", "plain", "mixed", "learned"], default="compare")
@@ -331,7 +339,8 @@ def main():
     torch.cuda.set_device(0)
     lib = _lib.load()
     t_setup = time.perf_counter()
-    pipe = W.make_pipe(a.n)
+    pipe = (W.make_pipe(a.n, sten_depth=a.stenosis, sten_z0=1.25, sten_len=0.5) if a.stenosis > 0.0
+            else W.make_pipe(a.n))
     phys = W.Physics()
     ls = ls_create(_lib.LS_GMRES, relTol=a.tol, absTol=1e-30, maxItr=1000, dimKry=a.krylov)
     fn = FluidNewton(pipe, phys, ls)
@@ -357,6 +366,7 @@ def main():
     head = "learned" if "learned" in runs else next(iter(runs))
     R = runs[head]
 
+    traffic = MEASURED_TRAFFIC_N34 if (a.n == 34 and a.stenosis == 0.0) else None
     cpu = None
     if not a.no_cpu:
         log("# cpu baseline ...")
@@ -383,19 +393,29 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if head == "plain" else "f64+f32+bf16",
         "data": "synthetic (seeded pipe generator)",
-        "config": {"workload": f"VMS pipe n={a.n}: {pipe.nEl} tets, {pipe.nNo} nodes, nnzb {pipe.nnz}; "
+        "config": {"workload": f"VMS pipe n={a.n}{f' (stenosis {a.stenosis:g})' if a.stenosis > 0 else ''}: "
+                               f"{pipe.nEl} tets, {pipe.nNo} nodes, nnzb {pipe.nnz}; "
                                f"1 time step x 3 Newton iterations; GMRES({a.krylov}) rel-tol {a.tol:g} (FP64 "
                                f"residual of the scaled system)",
                    "method": head, "solver": solver[head], "newton_records": R["hists"][-1]},
-        "roofline": roofline(R["kernels"]),
+        "roofline": roofline(R["kernels"], traffic),
         "cpu_baseline": cpu,
         "kernels": R["kernels"],
     }
     if "plain" in runs and head != "plain":
         P = runs["plain"]
         out["plain"] = {"value": round(P["s_per_newton"], 6), "newton_records": P["hists"][-1],
-                        "roofline": roofline(P["kernels"])}
+                        "roofline": roofline(P["kernels"], traffic)}
         out["speedup_vs_plain"] = round(P["s_per_newton"] / R["s_per_newton"], 3)
+        # the time step's result against the FP64 plain path's (velocity, pressure, acceleration)
+        import torch
+        def rel(a, b):
+            return float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
+        out["error_vs_fp64"] = {"velocity": rel(R["Yn"][:, :3], P["Yn"][:, :3]),
+                                "pressure": rel(R["Yn"][:, 3], P["Yn"][:, 3]),
+                                "acceleration": rel(R["An"][:, :3], P["An"][:, :3]),
+                                "final_fNorm": R["hists"][-1][-1]["fNorm"],
+                                "final_fNorm_fp64": P["hists"][-1][-1]["fNorm"]}
     if net_info:
         out["learned_pc"] = net_info
     if a.batch > 0 and a.method == "compare":
