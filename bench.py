@@ -302,7 +302,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n", type=int, default=34, help="pipe cross-section cells (34 -> 1.2M tets, 51 -> 4.0M)")
+    ap.add_argument("--n", type=int, default=34, help="pipe cross-section cells (even; 34 -> 1.2M tets, 52 -> 4.2M)")
     ap.add_argument("--stenosis", type=float, default=0.0,
                     help="sinusoidal stenosis depth (fraction of the radius) over z in [1.25, 1.75]")
     ap.add_argument("--krylov", type=int, default=50)
