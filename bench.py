@@ -408,9 +408,8 @@ def main():
                         "roofline": roofline(P["kernels"], traffic)}
         out["speedup_vs_plain"] = round(P["s_per_newton"] / R["s_per_newton"], 3)
         # the time step's result against the FP64 plain path's (velocity, pressure, acceleration)
-        import torch
-        def rel(a, b):
-            return float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
+        def rel(x, ref):
+            return float(torch.linalg.vector_norm(x - ref) / torch.linalg.vector_norm(ref))
         out["error_vs_fp64"] = {"velocity": rel(R["Yn"][:, :3], P["Yn"][:, :3]),
                                 "pressure": rel(R["Yn"][:, 3], P["Yn"][:, 3]),
                                 "acceleration": rel(R["An"][:, :3], P["An"][:, :3]),

@@ -139,3 +139,12 @@ def test_bifurcation_mesh_valid_and_union():
     _lib.check(_lib.load().svg_csr_pattern(u.nNo, u.nEl, 4, u.IEN.ctypes.data, rp.ctypes.data, cp.ctypes.data))
     assert np.array_equal(rp, u.rowPtr) and np.array_equal(cp, u.colPtr)
     assert off[-1] == u.nNo
+
+
+def test_bench_refuses_more_than_one_gpu():
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr
