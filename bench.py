@@ -268,7 +268,11 @@ def batch_run(a, log):
     meshes = B.make_batch(specs)
     phys = W.Physics()
     prec = _lib.PRECISION_MIXED if a.batch_precision == "mixed" else _lib.PRECISION_FP64
-    bn = B.BatchNewton(meshes, phys, sD=a.krylov, rel_tol=a.tol, precision=prec)
+    net = None
+    if a.batch_method == "learned":
+        from svfsiplus_amd import lpc as L
+        net = L.LearnedPC.load(a.lpc_weights)
+    bn = B.BatchNewton(meshes, phys, sD=a.krylov, rel_tol=a.tol, precision=prec, lpc=net, net_period=a.net_period)
     log(f"# batch setup {time.perf_counter() - t0:.1f}s: {len(meshes)} meshes, {bn.pipe.nEl} tets, "
         f"{bn.pipe.nNo} nodes, nnzb {bn.pipe.nnz}")
     for w in range(a.warmup):
@@ -285,14 +289,19 @@ def batch_run(a, log):
     newton = len(meshes) * 3 * a.steps
     itr = [[r["itr"] for r in it] for it in recs[-1]]
     ok = all(r["suc"] for it in recs[-1] for r in it)
+    fallbacks = [sum(1 for r in it if r.get("fallback_itr", -1) >= 0) for it in recs[-1]]
     log(f"# batch: {newton} Newton steps in {el:.2f}s; max linear iterations per Newton iteration "
         f"{[max(x) for x in itr]}, all converged {ok}")
     out = {"meshes": len(meshes), "tets": int(bn.pipe.nEl), "nodes": int(bn.pipe.nNo), "nnzb": int(bn.pipe.nnz),
            "newton_steps_per_s": round(newton / el, 2), "s_per_time_step": round(el / a.steps, 4),
-           "precision": a.batch_precision, "all_converged": ok,
+           "precision": a.batch_precision, "method": a.batch_method, "all_converged": ok,
            "linear_iterations": {"max": [max(x) for x in itr], "mean": [round(float(np.mean(x)), 1) for x in itr]},
-           "solver": "batched GMRES(%d), per-mesh Krylov spaces and convergence, restart cycle captured in one "
-                     "hipGraph, block SpMV variant from the autotuner" % a.krylov}
+           "solver": "batched %sGMRES(%d), per-mesh Krylov spaces and convergence; each Newton iteration is three "
+                     "replayed hipGraphs (pici + assembly + preconditioner, the restart cycle, the corrector); "
+                     "block SpMV variant from the autotuner" % ("flexible " if net else "", a.krylov)}
+    if net is not None:
+        out["learned_pc"] = {"weights": os.path.relpath(a.lpc_weights, ROOT), "net_period": a.net_period,
+                             "meshes_fallen_back": fallbacks}
     bn.free()
     return out
 
@@ -317,11 +326,16 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--batch", type=int, default=256, help="meshes of the batched workload (0: skip it)")
     ap.add_argument("--batch-precision", choices=["fp64", "mixed"], default="mixed")
+    ap.add_argument("--batch-method", choices=["plain", "learned"], default="plain",
+                    help="batched solve: diagonal preconditioner, or the learned one (flexible GMRES)")
     ap.add_argument("--no-prof", action="store_true", help="time without per-kernel events")
     ap.add_argument("--kernel-only", choices=["spmv", "spmv32"], default=None,
                     help="launch only the block SpMV on the assembled system (for counter runs), "
                          "plus a device copy of the same byte count as the calibration kernel")
     ap.add_argument("--launches", type=int, default=20)
+    ap.add_argument("--batch-only", action="store_true",
+                    help="run only the batched workload and print its record (a measurement aid; the "
+                         "headline line comes from the default run)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only (the multi-GPU sharding is a design, not built); "
@@ -338,6 +352,9 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
     lib = _lib.load()
+    if a.batch_only:
+        print(json.dumps({"batch": batch_run(a, log)}))
+        return
     t_setup = time.perf_counter()
     pipe = (W.make_pipe(a.n, sten_depth=a.stenosis, sten_z0=1.25, sten_len=0.5) if a.stenosis > 0.0
             else W.make_pipe(a.n))

@@ -269,6 +269,32 @@ int svg_batch_gmres(svg_batch* batch, const svg_subls* proto, double* d_R, const
  * over the union (every face included), svg_batch_gmres, R = W R. */
 int svg_batch_fsils_solve(svg_batch* batch, const svg_subls* proto, double* d_R, double* d_Val, int precision,
                           svg_subls* out, void* stream);
+/* One Newton iteration of every mesh of the batch, device-resident from the
+ * initiator to the corrector (the loop body of main.cpp:371-617 for the fluid
+ * equation): pici (pic.cpp:486-574), R = Val = 0 (ls_alloc), construct_fluid,
+ * b_assem_neu_bc on face (null: none; Neumann value zero, backflow beta),
+ * svg_batch_fsils_solve and the picc update (pic.cpp:155-167). msh and face
+ * are over the batch's lhs. The work before the restart loop (assembly
+ * included) is one captured hipGraph, the restart cycle a second and the
+ * solution/scaling/corrector a third; all are captured at the first call and
+ * replayed while the pointers and parameters stay the same. construct_fluid's
+ * zero-Jacobian error is reported after the iteration (SVG_ERR_JACOBIAN). */
+int svg_batch_newton_iteration(svg_batch* batch, svg_msh* msh, svg_face* face, const svg_fluid_params* p,
+                               double beta, double qmTRI3, const double* d_Ao, double* d_An, const double* d_Yo,
+                               double* d_Yn, double* d_Ag, double* d_Yg, double* d_R, double* d_Val,
+                               const svg_subls* proto, const svg_solve_opts* opts, svg_subls* out, void* stream);
+/* The batched solves with the options of svg_fsils_solve_ex (opts null: FP64
+ * plain): precision, and method SVG_METHOD_LEARNED with opts->lpc — flexible
+ * GMRES with the learned preconditioner for every system (each normalised by
+ * its own size), with the fallback to the diagonal preconditioner decided per
+ * system; each system's arithmetic is that of svg_fsils_solve_ex on the system
+ * alone. opts->fallback_itr is not written: svg_batch_fallback_itr gives the
+ * per-system iteration count at the fallback of the last solve (-1: none). */
+int svg_batch_gmres_ex(svg_batch* batch, const svg_subls* proto, double* d_R, const double* d_Val,
+                       const svg_solve_opts* opts, svg_subls* out, void* stream);
+int svg_batch_fsils_solve_ex(svg_batch* batch, const svg_subls* proto, double* d_R, double* d_Val,
+                             const svg_solve_opts* opts, svg_subls* out, void* stream);
+int svg_batch_fallback_itr(const svg_batch* batch, int* itr);
 
 /* ------------------------------------------------------------------------ */
 /* Kernels of the solve, exposed for unit parity tests.                      */
@@ -301,6 +327,19 @@ int svg_pici(int nNo, double am, double af, const double* d_Ao, const double* d_
              const double* d_Yn, double* d_Ag, double* d_Yg, void* stream);
 /* picc update (pic.cpp:155-167): An -= R, Yn -= gam dt R. */
 int svg_picc_update(int nNo, double gam, double dt, const double* d_R, double* d_An, double* d_Yn, void* stream);
+/* picp (pic.cpp:591-715) for a fluid equation (no displacement unknowns):
+ * An = Ao (gam-1)/gam, Yn = Yo over (4, nNo). */
+int svg_picp(int nNo, double gam, const double* d_Ao, const double* d_Yo, double* d_An, double* d_Yn, void* stream);
+/* set_bc_dir (set_bc.cpp:794-1063) for one strong Dirichlet face with a
+ * steady or Fourier-interpolated value (set_bc_dir_l's non-generic branch,
+ * no impD): for face node a (global node d_gN[a]) and k < lDof,
+ *   A(row, gN[a]) = dirA * gxn(k, a),  Y(row, gN[a]) = dirY * gxn(k, a),
+ * where gxn(lDof, nNo) holds gx(a) nV(k, a) when lDof = nsd and gx(a)
+ * otherwise, A and Y are (tDof, tnNo), and row = s + k (eDir = 0) or s plus
+ * the k-th direction set in the mask eDir (bc.eDrn). The host evaluates dirA
+ * and dirY (bc.g, or ifft of bc.gt at the current time). */
+int svg_set_bc_dir(int nNo, const int* d_gN, int tDof, int s, int lDof, unsigned eDir, const double* d_gxn,
+                   double dirA, double dirY, double* d_A, double* d_Y, void* stream);
 /* d[0..n) = v (a fill kernel, safe under graph capture). */
 int svg_fill(double* d, size_t n, double v, void* stream);
 

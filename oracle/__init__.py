@@ -67,7 +67,7 @@ def load():
         lib.orc_dot_s.restype = C.c_double
         lib.orc_norm_v.restype = C.c_double
         lib.orc_norm_s.restype = C.c_double
-        for fn in ("orc_pici", "orc_picc_update"):
+        for fn in ("orc_pici", "orc_picc_update", "orc_picp", "orc_set_bc_dir"):
             getattr(lib, fn).restype = None
         _lib = lib
     return _lib
@@ -284,6 +284,29 @@ def picc_update(phys, R, An, Yn):
                         C.c_void_p(_p(An)), C.c_void_p(_p(Yn)))
 
 
+def picp(phys, Ao, Yo):
+    """The predictor of a fluid equation (pic.cpp:676-693): (An, Yn)."""
+    lib = load()
+    Ao = np.ascontiguousarray(Ao, dtype=np.float64)
+    Yo = np.ascontiguousarray(Yo, dtype=np.float64)
+    An, Yn = np.empty_like(Ao), np.empty_like(Yo)
+    lib.orc_picp(C.c_int(Ao.shape[0]), C.c_double(phys.gam), C.c_void_p(_p(Ao)), C.c_void_p(_p(Yo)),
+                 C.c_void_p(_p(An)), C.c_void_p(_p(Yn)))
+    return An, Yn
+
+
+def set_bc_dir(gN, gxn, A, Y, s=0, eDir=0, dirA=0.0, dirY=1.0):
+    """Strong Dirichlet values of one face into A, Y (tDof, tnNo) in place
+    (set_bc.cpp:794-1063): gxn(nNo, lDof) = gx nV (lDof = nsd) or gx."""
+    lib = load()
+    gN = np.ascontiguousarray(gN, dtype=np.int32)
+    gxn = np.ascontiguousarray(gxn, dtype=np.float64)
+    assert A.flags.c_contiguous and Y.flags.c_contiguous and A.dtype == np.float64 and Y.dtype == np.float64
+    lib.orc_set_bc_dir(C.c_int(gN.shape[0]), C.c_void_p(_p(gN)), C.c_int(A.shape[1]), C.c_int(s),
+                       C.c_int(gxn.shape[1]), C.c_uint(eDir), C.c_void_p(_p(gxn)), C.c_double(dirA),
+                       C.c_double(dirY), C.c_void_p(_p(A)), C.c_void_p(_p(Y)))
+
+
 def ge(A, B):
     lib = load()
     A = np.asfortranarray(A, dtype=np.float64)
@@ -293,12 +316,14 @@ def ge(A, B):
     return bool(ok), Bv
 
 
-def newton_time_step(pipe, phys, make_ls, minItr=3, maxItr=3, tol=0.5, state=None, outlet=None):
+def newton_time_step(pipe, phys, make_ls, minItr=3, maxItr=3, tol=0.5, state=None, outlet=None, eq_iNorm=0.0):
     """One time step of the oracle Newton loop (main.cpp:371-617 for a single
     fluid equation): pici -> construct_fluid (+ the outlet's b_assem_neu_bc when
     `outlet` = (faceIEN, gE, inner) is given) -> fsils_solve -> picc, with the
     convergence test of pic.cpp:236-281. make_ls() returns a fresh Ls.
-    Returns (An, Yn, history)."""
+    eq_iNorm carries eq.iNorm from earlier time steps (it is set once, at the
+    first Newton iteration of the run, pic.cpp:243-245). Returns (An, Yn,
+    history); each record holds eq_iNorm after that iteration."""
     import sys as _sys
     from svfsiplus_amd.workload import dirichlet_state
     Ao, Yo, An, Yn = dirichlet_state(pipe, phys) if state is None else state
@@ -310,7 +335,7 @@ def newton_time_step(pipe, phys, make_ls, minItr=3, maxItr=3, tol=0.5, state=Non
         a = abs(v)
         return a / max(a, eps) < 10.0 * eps
 
-    itr, eq_iNorm, pNorm = 0, 0.0, np.finfo(np.float64).max
+    itr, pNorm = 0, np.finfo(np.float64).max
     hist = []
     while True:
         itr += 1
@@ -329,7 +354,28 @@ def newton_time_step(pipe, phys, make_ls, minItr=3, maxItr=3, tol=0.5, state=Non
         if itr == 1:
             pNorm = iN / eq_iNorm
         r1 = iN / eq_iNorm
-        hist.append(dict(itr=itr, ls_itr=ls.RI.itr, iNorm=ls.RI.iNorm, fNorm=ls.RI.fNorm, suc=ls.RI.suc, r1=r1))
+        hist.append(dict(itr=itr, ls_itr=ls.RI.itr, iNorm=ls.RI.iNorm, fNorm=ls.RI.fNorm, suc=ls.RI.suc, r1=r1,
+                         eq_iNorm=eq_iNorm))
         if itr >= maxItr or ((r1 <= tol or r1 <= tol * pNorm) and itr >= minItr):
             break
     return An, Yn, hist
+
+
+def time_steps(pipe, phys, make_ls, nsteps, outlet=None, **newton):
+    """nsteps time steps of the oracle (main.cpp:300-760 for one fluid
+    equation): picp, set_bc_dir on every Dirichlet face, the Newton loop,
+    then Ao = An, Yo = Yn. Starts from Ao = Yo = 0. Returns (An, Yn, [history per step])."""
+    from svfsiplus_amd.workload import dirichlet_faces
+    Ao = np.zeros((pipe.nNo, 4))
+    Yo = np.zeros((pipe.nNo, 4))
+    hists, eq_iNorm = [], 0.0
+    for _ in range(nsteps):
+        An, Yn = picp(phys, Ao, Yo)
+        for gN, gxn in dirichlet_faces(pipe, phys):
+            set_bc_dir(gN, gxn, An, Yn)
+        An, Yn, h = newton_time_step(pipe, phys, make_ls, state=(Ao, Yo, An, Yn), outlet=outlet, eq_iNorm=eq_iNorm,
+                                     **newton)
+        eq_iNorm = h[-1]["eq_iNorm"]
+        hists.append(h)
+        Ao, Yo = An, Yn
+    return An, Yn, hists

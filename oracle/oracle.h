@@ -32,7 +32,10 @@
  *   orc_gmres_v         : fixtures (solution + iteration count) + dense LU.
  *   orc_ns_solver       : fixtures (solution, RI.itr, Resm/Resc) + dense LU.
  *   orc_ge              : numpy.linalg.solve.
- *   orc_pici/picc       : closed-form generalized-alpha algebra.
+ *   orc_pici/picc/picp  : closed-form generalized-alpha algebra.
+ *   orc_set_bc_dir      : the first time step's state (Ao = Yo = 0) equals
+ *                         the closed-form Dirichlet state of the workload;
+ *                         direction masks against hand-placed values.
  */
 #ifndef SVFSI_ORACLE_H
 #define SVFSI_ORACLE_H
@@ -142,6 +145,9 @@ int orc_fsils_solve(orc_lhs* lhs, orc_ls* ls, int dof, double* Ri, double* Val, 
 void orc_pici(int nNo, double am, double af, const double* Ao, const double* An,
               const double* Yo, const double* Yn, double* Ag, double* Yg);
 void orc_picc_update(int nNo, double gam, double dt, const double* R, double* An, double* Yn);
+void orc_picp(int nNo, double gam, const double* Ao, const double* Yo, double* An, double* Yn);
+void orc_set_bc_dir(int nNo, const int* gN, int tDof, int s, int lDof, unsigned eDir, const double* gxn,
+                    double dirA, double dirY, double* A, double* Y);
 
 #ifdef __cplusplus
 }

@@ -6,6 +6,12 @@
  *   picc   Code/Source/solver/pic.cpp:155-167   An -= R, Yn -= gam dt R
  *          (the Dn update of the same loop is not part of a fluid-only run:
  *           dFlag is false for fluid, initialize.cpp:420-421)
+ *   picp   Code/Source/solver/pic.cpp:676-693   An = Ao (gam-1)/gam, Yn = Yo (rows s..e of the
+ *          equation; Dn = Do is not carried)
+ *   set_bc_dir  Code/Source/solver/set_bc.cpp:794-1063 for one strong Dirichlet face:
+ *          set_bc_dir_l's non-generic branch (tmpA = dirA gx nV or dirA gx, tmpY the
+ *          same with dirY; here gxn = gx nV or gx is given) and the scatter without
+ *          impD, over all lDof rows (eDir all false) or the eDrn directions
  */
 #include "oracle.h"
 
@@ -28,4 +34,37 @@ void orc_picc_update(int nNo, double gam, double dt, const double* R, double* An
       An[i + 4 * a] = An[i + 4 * a] - R[i + 4 * a];
       Yn[i + 4 * a] = Yn[i + 4 * a] - R[i + 4 * a] * coef0;
     }
+}
+
+void orc_picp(int nNo, double gam, const double* Ao, const double* Yo, double* An, double* Yn)
+{
+  double coef = (gam - 1.0) / gam;
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < nNo; j++) {
+      An[i + 4 * j] = Ao[i + 4 * j] * coef;
+      Yn[i + 4 * j] = Yo[i + 4 * j];
+    }
+}
+
+void orc_set_bc_dir(int nNo, const int* gN, int tDof, int s, int lDof, unsigned eDir, const double* gxn,
+                    double dirA, double dirY, double* A, double* Y)
+{
+  /* set_bc_dir_l: tmpA(i,a) = dirA * gxn(i,a), tmpY(i,a) = dirY * gxn(i,a) */
+  for (int a = 0; a < nNo; a++) {
+    int Ac = gN[a];
+    if (eDir) {
+      int l = 0;
+      for (int i = 0; i < 32; i++)
+        if ((eDir >> i) & 1u) {
+          A[(s + i) + tDof * Ac] = dirA * gxn[l + lDof * a];
+          Y[(s + i) + tDof * Ac] = dirY * gxn[l + lDof * a];
+          l = l + 1;
+        }
+    } else {
+      for (int i = 0; i < lDof; i++) {
+        A[(i + s) + tDof * Ac] = dirA * gxn[i + lDof * a];
+        Y[(i + s) + tDof * Ac] = dirY * gxn[i + lDof * a];
+      }
+    }
+  }
 }

@@ -109,6 +109,8 @@ _SIGS = {
     "svg_pici": (_I, [_I, _D, _D, _P, _P, _P, _P, _P, _P, _P]),
     "svg_picc_update": (_I, [_I, _D, _D, _P, _P, _P, _P]),
     "svg_fill": (_I, [_P, C.c_size_t, _D, _P]),
+    "svg_picp": (_I, [_I, _D, _P, _P, _P, _P, _P]),
+    "svg_set_bc_dir": (_I, [_I, _P, _I, _I, _I, C.c_uint, _P, _D, _D, _P, _P, _P]),
     "svg_prof_enable": (_I, [_I]),
     "svg_prof_reset": (_I, []),
     "svg_prof_get": (_I, [C.c_char_p, _IP, _DP]),
@@ -117,6 +119,11 @@ _SIGS = {
     "svg_batch_free": (_I, [_P]),
     "svg_batch_gmres": (_I, [_P, C.POINTER(SubLs), _P, _P, _I, C.POINTER(SubLs), _P]),
     "svg_batch_fsils_solve": (_I, [_P, C.POINTER(SubLs), _P, _P, _I, C.POINTER(SubLs), _P]),
+    "svg_batch_newton_iteration": (_I, [_P, _P, _P, C.POINTER(FluidParams), _D, _D, _P, _P, _P, _P, _P, _P, _P, _P,
+                                        C.POINTER(SubLs), C.POINTER(SolveOpts), C.POINTER(SubLs), _P]),
+    "svg_batch_gmres_ex": (_I, [_P, C.POINTER(SubLs), _P, _P, C.POINTER(SolveOpts), C.POINTER(SubLs), _P]),
+    "svg_batch_fsils_solve_ex": (_I, [_P, C.POINTER(SubLs), _P, _P, C.POINTER(SolveOpts), C.POINTER(SubLs), _P]),
+    "svg_batch_fallback_itr": (_I, [_P, _P]),
     "svg_lpc_num_params": (_I, [_IP]),
     "svg_lpc_create": (_I, [C.POINTER(_P)]),
     "svg_lpc_free": (_I, [_P]),

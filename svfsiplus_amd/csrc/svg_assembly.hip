@@ -403,15 +403,9 @@ __global__ __launch_bounds__(128) void k_b_fluid_tri3(int q0, int qn, const int4
   }
 }
 
-}  // namespace svg
-
-using namespace svg;
-
-extern "C" int svg_construct_fluid(svg_msh* M, const svg_fluid_params* p, const double* Ag, const double* Yg,
-                                   const double* Bf, double* R, double* Val, void* stream)
+int construct_fluid_enqueue(svg_msh* M, const svg_fluid_params* p, const double* Ag, const double* Yg,
+                            const double* Bf, double* R, double* Val, hipStream_t s)
 {
-  if (!M || !p || !Ag || !Yg || !R || !Val) return set_error(SVG_ERR_ARG, "svg_construct_fluid: null argument");
-  hipStream_t s = (hipStream_t)stream;
   FluidConst C;
   C.rho = p->rho;
   C.mu = p->mu;
@@ -443,6 +437,11 @@ extern "C" int svg_construct_fluid(svg_msh* M, const svg_fluid_params* p, const 
            Bf, C, R, Val, M->d_err);
   }
   SVG_HIP(hipGetLastError());
+  return SVG_OK;
+}
+
+int construct_fluid_check(svg_msh* M, hipStream_t s)
+{
   SVG_HIP(hipMemcpyAsync(M->h_err, M->d_err, sizeof(int), hipMemcpyDeviceToHost, s));
   SVG_HIP(hipStreamSynchronize(s));
   if (*M->h_err != INT_MAX) {
@@ -450,6 +449,19 @@ extern "C" int svg_construct_fluid(svg_msh* M, const svg_fluid_params* p, const 
                                            std::to_string(*M->h_err) + ") is < 0.");
   }
   return SVG_OK;
+}
+
+}  // namespace svg
+
+using namespace svg;
+
+extern "C" int svg_construct_fluid(svg_msh* M, const svg_fluid_params* p, const double* Ag, const double* Yg,
+                                   const double* Bf, double* R, double* Val, void* stream)
+{
+  if (!M || !p || !Ag || !Yg || !R || !Val) return set_error(SVG_ERR_ARG, "svg_construct_fluid: null argument");
+  hipStream_t s = (hipStream_t)stream;
+  int rc = construct_fluid_enqueue(M, p, Ag, Yg, Bf, R, Val, s);
+  return rc ? rc : construct_fluid_check(M, s);
 }
 
 extern "C" int svg_face_create(const svg_msh* M, int nFe, const int* faceIEN, const int* gE, svg_face** out)

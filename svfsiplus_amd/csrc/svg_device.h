@@ -152,7 +152,15 @@ int lhs_ensure_trans(svg_lhs* L);  // block-transpose map (pattern must be symme
 int spmv_residual(const svg_lhs* L, const double* K, const double* U, const double* B, double* out, hipStream_t s);
 int lpc_prepare_dev(svg_lhs* L, const double* Val, hipStream_t s);
 int lpc_apply_dev(svg_lpc* N, const svg_lhs* L, const double* V, double* Z, float* Z32, const int* skip,
-                  hipStream_t s);
+                  hipStream_t s, const double2* nsc = nullptr);
+// Every buffer lpc_prepare_dev / lpc_apply_dev use on L, allocated ahead (graph capture).
+int lpc_reserve(svg_lpc* N, svg_lhs* L);
+// construct_fluid split in two: the launches alone (no host synchronisation, so
+// they can be captured into a graph; the zero-Jacobian flag stays on the device)
+// and the check of that flag, which synchronises s.
+int construct_fluid_enqueue(svg_msh* M, const svg_fluid_params* p, const double* Ag, const double* Yg,
+                            const double* Bf, double* R, double* Val, hipStream_t s);
+int construct_fluid_check(svg_msh* M, hipStream_t s);
 
 }  // namespace svg
 
@@ -198,10 +206,14 @@ struct svg_batch {
   svg::GmState* st = nullptr;
   svg::GmState* h_st = nullptr;           // pinned mirror (nsys)
   float* u32 = nullptr;
+  double* z = nullptr;                    // learned preconditioner: flexible basis z_j (sD vectors)
+  double2* d_nsc = nullptr;               // per node: (sqrt(nNo_q), 1 / sqrt(nNo_q)) of its system q
+  std::vector<int> fbItr;                 // per system: iteration count at the fallback (-1: none)
   hipStream_t cs_stream = nullptr;        // capture/replay stream
   hipGraphExec_t graph[2] = {nullptr, nullptr};  // restart cycle: FP64 / mixed
-  const double* gR[2] = {nullptr, nullptr};      // the R and Val each graph was captured with
-  const double* gVal[2] = {nullptr, nullptr};
+  std::vector<unsigned char> gKey[2];            // the arguments each cycle graph was captured with
+  hipGraphExec_t pro = nullptr, epi = nullptr;   // Newton iteration: pici..initial norms / result..picc
+  std::vector<unsigned char> newtonKey;          // the arguments pro and epi were captured with
   hipEvent_t ev = nullptr;
 };
 

@@ -454,9 +454,10 @@ __device__ __forceinline__ void fin_body(const FinArgs& A, const double* __restr
         *host_done = 1;
         __threadfence_system();
       }
-    } else if (A.fbwin > 0) {
+    } else if (A.fbwin > 0 && !st->fallback) {
       // an iteration counts as decreasing when it lowers the smallest residual
-      // estimate so far by at least FB_DROP (relative)
+      // estimate so far by at least FB_DROP (relative); a system that fell back
+      // keeps iterating without the check
       if (fabs(e1) < st->best * (1.0 - FB_DROP)) {
         st->best = fabs(e1);
         st->nondec = 0;
@@ -1912,6 +1913,9 @@ static void batch_graph_free(svg_batch* B)
 {
   for (auto& g : B->graph)
     if (g) (void)hipGraphExecDestroy(g), g = nullptr;
+  for (hipGraphExec_t* g : {&B->pro, &B->epi})
+    if (*g) (void)hipGraphExecDestroy(*g), *g = nullptr;
+  B->newtonKey.clear();
 }
 
 static void batch_mdot(svg_batch* B, const double* w, const double* base, size_t ld, int m, int check, hipStream_t s)
@@ -1927,46 +1931,263 @@ static void batch_fin(svg_batch* B, const FinArgs& fa, int m, hipStream_t s)
                      (const double*)B->partial, m, B->st, B->h, B->cs, B->sn, B->err, B->coef);
 }
 
+// The form of a batched solve: FP32 SpMV (mixed) and the learned
+// preconditioner (net, applied at Arnoldi steps i % period == 0, fallback
+// after `window` non-decreasing iterations, per system).
+struct BatchMode {
+  bool mixed = false;
+  svg_lpc* net = nullptr;
+  int period = 1, window = 20;
+  bool tcheck() const { return mixed || net; }  // convergence on the FP64 true restart residual
+};
+
+// z_j = u_j (and its FP32 copy) for the systems still iterating in the cycle;
+// after a network application (onlyFallback) only for the systems that fell
+// back to the diagonal preconditioner, whose flexible basis is then the plain one.
+__global__ __launch_bounds__(256) void k_bzcopy(int n, const double* __restrict__ u, double* __restrict__ z,
+                                                float* __restrict__ z32, const GmState* __restrict__ st,
+                                                const int* __restrict__ sysOfNode, int onlyFallback)
+{
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (size_t)n) return;
+  const GmState& S = st[sysOfNode[k >> 2]];
+  if (!S.act || S.done || (onlyFallback && !S.fallback)) return;
+  const double v = u[k];
+  z[k] = v;
+  if (z32) z32[k] = (float)v;
+}
+
 // One restart cycle of every system still iterating (the body of gmres_v's
-// restart loop, gmres.cpp:529-612), enqueued on s (captured into a graph).
-static void batch_cycle(svg_batch* B, const double* Val, const double* R, bool mixed, const FinArgs& base,
-                        hipStream_t s)
+// restart loop, gmres.cpp:529-612; flexible with the learned preconditioner as
+// the single-system learned solve), enqueued on s (captured into a graph).
+static int batch_cycle(svg_batch* B, const double* Val, const double* R, const BatchMode& md, const FinArgs& base,
+                       hipStream_t s)
 {
   svg_lhs* L = B->L;
   const int n = B->n, sD = B->sD;
   const unsigned nbv = blocks_for(n, 256);
   const Op A{L, Val, 4};
+  const bool mixed = md.mixed, flex = md.net != nullptr;
   FinArgs fa = base;
   op_apply(A, B->X, B->u, R, nullptr, s);  // u0 = R - A X
   batch_mdot(B, B->u, B->u, 0, 1, 1, s);
   fa.mode = FIN_RESTART;
   fa.inc = 1;
   fa.first = 0;
-  fa.tcheck = mixed ? 1 : 0;
+  fa.tcheck = md.tcheck() ? 1 : 0;
   batch_fin(B, fa, 1, s);
+  float* u32w = (mixed && !flex) ? B->u32 : (float*)nullptr;  // flexible: z's copy feeds the SpMV
   hipLaunchKernelGGL(k_bdiv, dim3(nbv), dim3(256), 0, s, n, B->u, (const GmState*)B->st,
-                     (const int*)B->d_sysOfNode, mixed ? B->u32 : (float*)nullptr);
+                     (const int*)B->d_sysOfNode, u32w);
   for (int i = 0; i < sD; i++) {
     double* ui = B->u + B->ld * i;
     double* un = B->u + B->ld * (i + 1);
+    const double* sin = ui;
+    if (flex) {
+      double* zi = B->z + B->ld * i;
+      float* z32 = mixed ? B->u32 : (float*)nullptr;
+      if (md.period <= 1 || i % md.period == 0) {
+        int rc = lpc_apply_dev(md.net, L, ui, zi, z32, nullptr, s, B->d_nsc);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_bzcopy, dim3(nbv), dim3(256), 0, s, n, (const double*)ui, zi, z32,
+                           (const GmState*)B->st, (const int*)B->d_sysOfNode, 1);
+      } else {
+        hipLaunchKernelGGL(k_bzcopy, dim3(nbv), dim3(256), 0, s, n, (const double*)ui, zi, z32,
+                           (const GmState*)B->st, (const int*)B->d_sysOfNode, 0);
+      }
+      sin = zi;
+    }
     if (mixed)
       hipLaunchKernelGGL(k_bspmv32<2>, dim3(blocks_for((size_t)4 * L->nNo, 256)), dim3(256), 0, s, L->nNo,
                          (const int*)L->d_rowPtr, (const int*)L->d_colPtr, (const float*)L->d_val32,
                          (const float*)B->u32, un, (const int*)B->d_sysOfNode, (const GmState*)B->st);
     else
-      op_apply(A, ui, un, nullptr, nullptr, s);
+      op_apply(A, sin, un, nullptr, nullptr, s);
     batch_mdot(B, un, B->u, B->ld, i + 2, 2, s);
     fa.mode = FIN_ARNOLDI;
     fa.i = i;
     batch_fin(B, fa, i + 2, s);
     hipLaunchKernelGGL(k_bgs_update, dim3(nbv), dim3(256), 0, s, n, B->u, B->ld, i, sD, (const double*)B->coef,
-                       (const GmState*)B->st, (const int*)B->d_sysOfNode, mixed ? B->u32 : (float*)nullptr);
+                       (const GmState*)B->st, (const int*)B->d_sysOfNode, u32w);
   }
   hipLaunchKernelGGL(k_bbacksolve, dim3(B->nsys), dim3(256), 0, s, sD, (const double*)B->h, (const double*)B->err,
                      B->y, B->st);
-  hipLaunchKernelGGL(k_bxupdate, dim3(nbv), dim3(256), 0, s, n, B->X, (const double*)B->u, B->ld, sD,
-                     (const double*)B->y, (const GmState*)B->st, (const int*)B->d_sysOfNode);
-  hipLaunchKernelGGL(k_bclose, dim3(blocks_for(B->nsys, 256)), dim3(256), 0, s, B->nsys, B->st, mixed ? 1 : 0);
+  hipLaunchKernelGGL(k_bxupdate, dim3(nbv), dim3(256), 0, s, n, B->X, (const double*)(flex ? B->z : B->u), B->ld,
+                     sD, (const double*)B->y, (const GmState*)B->st, (const int*)B->d_sysOfNode);
+  hipLaunchKernelGGL(k_bclose, dim3(blocks_for(B->nsys, 256)), dim3(256), 0, s, B->nsys, B->st,
+                     md.tcheck() ? 1 : 0);
+  SVG_HIP(hipGetLastError());
+  return SVG_OK;
+}
+
+// The bytes of the arguments a captured graph bakes in (a graph is replayed
+// only while they are unchanged).
+struct GraphKey {
+  std::vector<unsigned char> b;
+  template <typename T>
+  GraphKey& add(const T& v)
+  {
+    const unsigned char* p = reinterpret_cast<const unsigned char*>(&v);
+    b.insert(b.end(), p, p + sizeof(T));
+    return *this;
+  }
+};
+
+// Captures what enqueue() puts on s into an executable graph; per-launch event
+// timing is off while capturing.
+template <typename F>
+static int capture(hipStream_t s, hipGraphExec_t* out, F enqueue)
+{
+  const bool prev = prof_set(false);
+  hipGraph_t g = nullptr;
+  int rc = SVG_OK;
+  hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    rc = enqueue();
+    hipError_t e2 = hipStreamEndCapture(s, &g);
+    if (e == hipSuccess) e = e2;
+  }
+  prof_set(prev);
+  if (!rc && e == hipSuccess) e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+  if (g) (void)hipGraphDestroy(g);
+  if (rc) return rc;
+  return e == hipSuccess ? SVG_OK : hip_check(e, "graph capture");
+}
+
+static FinArgs batch_fin_args(const svg_subls* proto, int sD, const BatchMode& md)
+{
+  FinArgs fa{};
+  fa.sD = sD;
+  fa.absTol = proto->absTol;
+  fa.relTol = proto->relTol;
+  fa.mode = FIN_NORM_R;
+  fa.fbwin = md.net ? md.window : 0;
+  return fa;
+}
+
+static int batch_mode(const svg_solve_opts* o, BatchMode* md)
+{
+  *md = BatchMode{};
+  if (!o) return SVG_OK;
+  md->mixed = o->precision == SVG_PRECISION_MIXED;
+  if (o->method == SVG_METHOD_LEARNED) {
+    if (!o->lpc) return set_error(SVG_ERR_ARG, "batched solve: the learned method needs a network (opts.lpc)");
+    md->net = o->lpc;
+    md->period = o->net_period > 1 ? o->net_period : 1;
+    md->window = o->fallback_window > 0 ? o->fallback_window : 20;
+  } else if (o->method != SVG_METHOD_PLAIN) {
+    return set_error(SVG_ERR_ARG, "batched solve: unknown method");
+  }
+  return SVG_OK;
+}
+
+// What must exist before the batched solve is enqueued or captured: the SpMV
+// launch variant (tuned once per lhs), the FP32 copies (mixed), the flexible
+// basis, per-node scales and network buffers (learned).
+static int batch_prepare(svg_batch* B, const double* Val, const BatchMode& md, hipStream_t caller)
+{
+  svg_lhs* L = B->L;
+  int rc;
+  if (L->spmvVariant < 0 && (rc = svg_spmv_tune(L, Val, nullptr, caller))) return rc;
+  if (md.mixed) {
+    if (!L->d_val32 && (rc = dev_alloc(&L->d_val32, (size_t)16 * L->nnz))) return rc;
+    if (!B->u32 && (rc = dev_alloc(&B->u32, (size_t)B->n))) return rc;
+  }
+  if (md.net) {
+    if (!B->z && (rc = dev_alloc(&B->z, B->ld * B->sD))) return rc;
+    if (!B->d_nsc) {
+      std::vector<double2> sc(L->nNo);
+      for (int q = 0; q < B->nsys; q++) {
+        const double v = std::sqrt((double)(B->nodeOff[q + 1] - B->nodeOff[q]));
+        for (int a = B->nodeOff[q]; a < B->nodeOff[q + 1]; a++) sc[a] = make_double2(v, 1.0 / v);
+      }
+      if ((rc = dev_alloc(&B->d_nsc, (size_t)L->nNo))) return rc;
+      SVG_HIP(hipMemcpy(B->d_nsc, sc.data(), sizeof(double2) * L->nNo, hipMemcpyHostToDevice));
+    }
+    if ((rc = lpc_reserve(md.net, L))) return rc;
+  }
+  return SVG_OK;
+}
+
+// gmres_v before its restart loop (gmres.cpp:466-527) for every system: the
+// FP32 matrix copy (mixed), the network's BF16 copies (learned), ||R|| per
+// system with eps and the state, X = 0.
+static int batch_begin(svg_batch* B, const double* R, const double* Val, const BatchMode& md, const FinArgs& fa,
+                       hipStream_t s)
+{
+  svg_lhs* L = B->L;
+  int rc;
+  if (md.net && (rc = lpc_prepare_dev(L, Val, s))) return rc;
+  if (md.mixed) launch("d2f", k_d2f, dim3(8192), dim3(256), 0, s, (size_t)16 * L->nnz, Val, L->d_val32);
+  batch_mdot(B, R, R, 0, 1, 0, s);
+  batch_fin(B, fa, 1, s);
+  int rc2 = svg_fill(B->X, (size_t)B->n, 0.0, s);
+  if (rc2) return rc2;
+  SVG_HIP(hipGetLastError());
+  return SVG_OK;
+}
+
+// The restart cycle as one graph (R, Val, the tolerances and the form are
+// baked in); graph[0] serves the FP64 forms, graph[1] the mixed ones.
+static int batch_cycle_graph(svg_batch* B, const double* R, const double* Val, const BatchMode& md, const FinArgs& fa)
+{
+  const int gi = md.mixed ? 1 : 0;
+  GraphKey k;
+  k.add(R).add(Val).add(fa.absTol).add(fa.relTol).add(fa.fbwin).add(B->L->d_val32).add(B->u32);
+  k.add(B->L->spmvVariant).add(md.net).add(md.period).add(B->z).add(B->d_nsc);
+  if (B->graph[gi] && B->gKey[gi] == k.b) return SVG_OK;
+  if (B->graph[gi]) (void)hipGraphExecDestroy(B->graph[gi]), B->graph[gi] = nullptr;
+  hipStream_t s = B->cs_stream;
+  int rc = capture(s, &B->graph[gi], [&]() -> int { return batch_cycle(B, Val, R, md, fa, s); });
+  if (rc) return rc;
+  B->gKey[gi] = k.b;
+  return SVG_OK;
+}
+
+// Replays the restart cycle (enqueued after the begin step on the capture
+// stream) until no system iterates or mItr cycles ran; dB[q] is system q's
+// residual norm at the start of its last cycle; fbItr[q] the iteration count
+// after the cycle in which system q fell back (-1: none).
+static int batch_iterate(svg_batch* B, const BatchMode& md, int mItr, std::vector<double>& dB)
+{
+  hipStream_t s = B->cs_stream;
+  SVG_HIP(hipMemcpyAsync(B->h_st, B->st, sizeof(GmState) * B->nsys, hipMemcpyDeviceToHost, s));
+  SVG_HIP(hipStreamSynchronize(s));
+  dB.assign(B->nsys, 0.0);
+  B->fbItr.assign(B->nsys, -1);
+  for (int q = 0; q < B->nsys; q++) dB[q] = B->h_st[q].fNorm;
+  for (int l = 0; l < mItr; l++) {
+    int active = 0;
+    for (int q = 0; q < B->nsys; q++) active += B->h_st[q].act;
+    if (!active) break;
+    for (int q = 0; q < B->nsys; q++)
+      if (B->h_st[q].act) dB[q] = B->h_st[q].fNorm;
+    SVG_HIP(hipGraphLaunch(B->graph[md.mixed ? 1 : 0], s));
+    SVG_HIP(hipMemcpyAsync(B->h_st, B->st, sizeof(GmState) * B->nsys, hipMemcpyDeviceToHost, s));
+    SVG_HIP(hipStreamSynchronize(s));
+    for (int q = 0; q < B->nsys; q++) {
+      if (B->h_st[q].divzero) return set_error(SVG_ERR_ARG, "Array Divide by zero.");
+      if (md.net && B->h_st[q].fallback && B->fbItr[q] < 0) B->fbItr[q] = B->h_st[q].itr;
+    }
+  }
+  return SVG_OK;
+}
+
+// The per-system records of ls.RI after a batched solve (h_st is current).
+static void batch_records(const svg_batch* B, const svg_subls* proto, const BatchMode& md,
+                          const std::vector<double>& dB, double secs, svg_subls* out)
+{
+  for (int q = 0; q < B->nsys; q++) {
+    const GmState& g = B->h_st[q];
+    svg_subls& o = out[q];
+    o = *proto;
+    o.itr = g.itr;
+    o.iNorm = g.iNorm;
+    o.fNorm = g.fNorm;
+    o.suc = md.tcheck() ? g.tconv : g.suc;
+    o.callD = secs;
+    o.dB = (g.iNorm > proto->absTol && dB[q] > 0.0) ? 10.0 * log(g.fNorm / dB[q]) : 0.0;
+  }
 }
 
 }  // namespace svg
@@ -2281,6 +2502,8 @@ int svg_batch_free(svg_batch* B)
     if (p) (void)hipFree(p);
   if (B->st) (void)hipFree(B->st);
   if (B->u32) (void)hipFree(B->u32);
+  if (B->z) (void)hipFree(B->z);
+  if (B->d_nsc) (void)hipFree(B->d_nsc);
   if (B->h_st) (void)hipHostFree(B->h_st);
   if (B->ev) (void)hipEventDestroy(B->ev);
   if (B->cs_stream) (void)hipStreamDestroy(B->cs_stream);
@@ -2288,72 +2511,23 @@ int svg_batch_free(svg_batch* B)
   return SVG_OK;
 }
 
-int svg_batch_gmres(svg_batch* B, const svg_subls* proto, double* R, const double* Val, int precision,
-                    svg_subls* out, void* stream)
+int svg_batch_gmres_ex(svg_batch* B, const svg_subls* proto, double* R, const double* Val,
+                       const svg_solve_opts* opts, svg_subls* out, void* stream)
 {
   if (!B || !proto || !R || !Val || !out) return set_error(SVG_ERR_ARG, "svg_batch_gmres: null argument");
   if (proto->mItr < 1) return set_error(SVG_ERR_ARG, "svg_batch_gmres: mItr must be >= 1");
-  const bool mixed = precision == SVG_PRECISION_MIXED;
-  svg_lhs* L = B->L;
+  BatchMode md;
+  int rc;
+  if ((rc = batch_mode(opts, &md))) return rc;
   hipStream_t caller = (hipStream_t)stream, s = B->cs_stream;
   auto t0 = std::chrono::steady_clock::now();
-  int rc;
-  if (L->spmvVariant < 0 && (rc = svg_spmv_tune(L, Val, nullptr, caller))) return rc;
-  if (mixed) {
-    if (!L->d_val32 && (rc = dev_alloc(&L->d_val32, (size_t)16 * L->nnz))) return rc;
-    if (!B->u32 && (rc = dev_alloc(&B->u32, (size_t)B->n))) return rc;
-    launch("d2f", k_d2f, dim3(8192), dim3(256), 0, caller, (size_t)16 * L->nnz, Val, L->d_val32);
-  }
+  if ((rc = batch_prepare(B, Val, md, caller))) return rc;
   SVG_HIP(hipEventRecord(B->ev, caller));
   SVG_HIP(hipStreamWaitEvent(s, B->ev, 0));
-  FinArgs fa{};
-  fa.sD = B->sD;
-  fa.absTol = proto->absTol;
-  fa.relTol = proto->relTol;
-  fa.mode = FIN_NORM_R;
-  batch_mdot(B, R, R, 0, 1, 0, s);
-  batch_fin(B, fa, 1, s);
-  svg_fill(B->X, (size_t)B->n, 0.0, s);
-  SVG_HIP(hipGetLastError());
-  // the restart cycle as one graph (pointers R and Val are baked in)
-  const int gi = mixed ? 1 : 0;
-  if (B->graph[gi] && (B->gR[gi] != R || B->gVal[gi] != Val)) {
-    (void)hipGraphExecDestroy(B->graph[gi]);
-    B->graph[gi] = nullptr;
-  }
-  if (!B->graph[gi]) {
-    const bool prev = prof_set(false);
-    hipGraph_t g = nullptr;
-    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
-    if (e == hipSuccess) {
-      batch_cycle(B, Val, R, mixed, fa, s);
-      e = hipStreamEndCapture(s, &g);
-    }
-    prof_set(prev);
-    if (e == hipSuccess) e = hipGraphInstantiate(&B->graph[gi], g, nullptr, nullptr, 0);
-    if (g) (void)hipGraphDestroy(g);
-    if (e != hipSuccess) return hip_check(e, "svg_batch_gmres: graph capture");
-    B->gR[gi] = R;
-    B->gVal[gi] = Val;
-  }
-  std::vector<double> dB(B->nsys);
-  SVG_HIP(hipMemcpyAsync(B->h_st, B->st, sizeof(GmState) * B->nsys, hipMemcpyDeviceToHost, s));
-  SVG_HIP(hipStreamSynchronize(s));
-  for (int q = 0; q < B->nsys; q++) dB[q] = B->h_st[q].fNorm;
-  int cycles = 0;
-  for (int l = 0; l < proto->mItr; l++) {
-    int active = 0;
-    for (int q = 0; q < B->nsys; q++) active += B->h_st[q].act;
-    if (!active) break;
-    for (int q = 0; q < B->nsys; q++)
-      if (B->h_st[q].act) dB[q] = B->h_st[q].fNorm;
-    SVG_HIP(hipGraphLaunch(B->graph[gi], s));
-    SVG_HIP(hipMemcpyAsync(B->h_st, B->st, sizeof(GmState) * B->nsys, hipMemcpyDeviceToHost, s));
-    SVG_HIP(hipStreamSynchronize(s));
-    cycles++;
-    for (int q = 0; q < B->nsys; q++)
-      if (B->h_st[q].divzero) return set_error(SVG_ERR_ARG, "Array Divide by zero.");
-  }
+  const FinArgs fa = batch_fin_args(proto, B->sD, md);
+  if ((rc = batch_begin(B, R, Val, md, fa, s)) || (rc = batch_cycle_graph(B, R, Val, md, fa))) return rc;
+  std::vector<double> dB;
+  if ((rc = batch_iterate(B, md, proto->mItr, dB))) return rc;
   hipLaunchKernelGGL(k_bresult, dim3(blocks_for(B->n, 256)), dim3(256), 0, s, B->n, R, (const double*)B->X,
                      (const GmState*)B->st, (const int*)B->d_sysOfNode, proto->absTol);
   SVG_HIP(hipGetLastError());
@@ -2361,24 +2535,20 @@ int svg_batch_gmres(svg_batch* B, const svg_subls* proto, double* R, const doubl
   SVG_HIP(hipStreamWaitEvent(caller, B->ev, 0));
   SVG_HIP(hipStreamSynchronize(s));
   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  for (int q = 0; q < B->nsys; q++) {
-    const GmState& g = B->h_st[q];
-    svg_subls& o = out[q];
-    o = *proto;
-    o.itr = g.itr;
-    o.iNorm = g.iNorm;
-    o.fNorm = g.fNorm;
-    o.suc = mixed ? g.tconv : g.suc;
-    o.callD = secs;
-    o.dB = (g.iNorm > proto->absTol && dB[q] > 0.0) ? 10.0 * log(g.fNorm / dB[q]) : 0.0;
-  }
-  (void)cycles;
+  batch_records(B, proto, md, dB, secs, out);
   return SVG_OK;
 }
 
+int svg_batch_gmres(svg_batch* B, const svg_subls* proto, double* R, const double* Val, int precision,
+                    svg_subls* out, void* stream)
+{
+  svg_solve_opts o{};
+  o.precision = precision;
+  return svg_batch_gmres_ex(B, proto, R, Val, &o, out, stream);
+}
 
-int svg_batch_fsils_solve(svg_batch* B, const svg_subls* proto, double* R, double* Val, int precision,
-                          svg_subls* out, void* stream)
+int svg_batch_fsils_solve_ex(svg_batch* B, const svg_subls* proto, double* R, double* Val,
+                             const svg_solve_opts* opts, svg_subls* out, void* stream)
 {
   if (!B || !R || !Val) return set_error(SVG_ERR_ARG, "svg_batch_fsils_solve: null argument");
   svg_lhs* L = B->L;
@@ -2387,11 +2557,99 @@ int svg_batch_fsils_solve(svg_batch* B, const svg_subls* proto, double* R, doubl
   int rc;
   if ((rc = ensure_work(L->kw, 4 * L->nNo, 1))) return rc;
   if ((rc = precond_diag_dev(L, 4, Val, R, L->kw.W, s))) return rc;
-  if ((rc = svg_batch_gmres(B, proto, R, Val, precision, out, stream))) return rc;
+  if ((rc = svg_batch_gmres_ex(B, proto, R, Val, opts, out, stream))) return rc;
   launch("wc_scale", k_mul, dim3(blocks_for(B->n, 256)), dim3(256), 0, s, B->n, (const double*)L->kw.W,
          (const double*)R, R);
   SVG_HIP(hipGetLastError());
   SVG_HIP(hipStreamSynchronize(s));
+  return SVG_OK;
+}
+
+int svg_batch_fsils_solve(svg_batch* B, const svg_subls* proto, double* R, double* Val, int precision,
+                          svg_subls* out, void* stream)
+{
+  svg_solve_opts o{};
+  o.precision = precision;
+  return svg_batch_fsils_solve_ex(B, proto, R, Val, &o, out, stream);
+}
+
+int svg_batch_fallback_itr(const svg_batch* B, int* itr)
+{
+  if (!B || !itr) return set_error(SVG_ERR_ARG, "svg_batch_fallback_itr: null argument");
+  for (int q = 0; q < B->nsys; q++) itr[q] = q < (int)B->fbItr.size() ? B->fbItr[q] : -1;
+  return SVG_OK;
+}
+
+int svg_batch_newton_iteration(svg_batch* B, svg_msh* M, svg_face* Fa, const svg_fluid_params* p, double beta,
+                               double qmTRI3, const double* Ao, double* An, const double* Yo, double* Yn, double* Ag,
+                               double* Yg, double* R, double* Val, const svg_subls* proto, const svg_solve_opts* opts,
+                               svg_subls* out, void* stream)
+{
+  if (!B || !M || !p || !Ao || !An || !Yo || !Yn || !Ag || !Yg || !R || !Val || !proto || !out)
+    return set_error(SVG_ERR_ARG, "svg_batch_newton_iteration: null argument");
+  if (M->lhs != B->L || (Fa && Fa->msh != M))
+    return set_error(SVG_ERR_ARG, "svg_batch_newton_iteration: the mesh, face and batch must share one lhs");
+  if (proto->mItr < 1) return set_error(SVG_ERR_ARG, "svg_batch_newton_iteration: mItr must be >= 1");
+  BatchMode md;
+  int rc;
+  if ((rc = batch_mode(opts, &md))) return rc;
+  svg_lhs* L = B->L;
+  hipStream_t caller = (hipStream_t)stream, s = B->cs_stream;
+  auto t0 = std::chrono::steady_clock::now();
+  for (auto& f : L->face) f.incFlag = true;
+  if ((rc = ensure_work(L->kw, 4 * L->nNo, 1)) || (rc = build_mask(L, 4)) ||
+      (rc = batch_prepare(B, Val, md, caller)))
+    return rc;
+  const FinArgs fa = batch_fin_args(proto, B->sD, md);
+  double* W = L->kw.W;
+  GraphKey k;
+  k.add(M).add(Fa).add(*p).add(beta).add(qmTRI3).add(Ao).add(An).add(Yo).add(Yn).add(Ag).add(Yg).add(R).add(Val);
+  k.add(W).add(md.mixed).add(md.net).add(fa.absTol).add(fa.relTol).add(L->d_val32).add(L->d_val16);
+  if (B->pro && B->newtonKey != k.b) {
+    for (hipGraphExec_t* g : {&B->pro, &B->epi})
+      if (*g) (void)hipGraphExecDestroy(*g), *g = nullptr;
+  }
+  if (!B->pro) {
+    // pici, ls_alloc's R = Val = 0, construct_fluid, b_assem_neu_bc,
+    // precond_diag and gmres_v up to its restart loop
+    rc = capture(s, &B->pro, [&]() -> int {
+      int r;
+      if ((r = svg_pici(L->nNo, p->am, p->af, Ao, An, Yo, Yn, Ag, Yg, s)) ||
+          (r = svg_fill(R, (size_t)4 * L->nNo, 0.0, s)) || (r = svg_fill(Val, (size_t)16 * L->nnz, 0.0, s)) ||
+          (r = construct_fluid_enqueue(M, p, Ag, Yg, nullptr, R, Val, s)) ||
+          (Fa && (r = svg_b_assem_neu_bc(Fa, p, beta, qmTRI3, nullptr, Yg, R, Val, s))) ||
+          (r = precond_diag_dev(L, 4, Val, R, W, s)))
+        return r;
+      return batch_begin(B, R, Val, md, fa, s);
+    });
+    // the solutions, R = W R (solve.cpp:184-186) and the picc update
+    if (!rc)
+      rc = capture(s, &B->epi, [&]() -> int {
+        hipLaunchKernelGGL(k_bresult, dim3(blocks_for(B->n, 256)), dim3(256), 0, s, B->n, R, (const double*)B->X,
+                           (const GmState*)B->st, (const int*)B->d_sysOfNode, fa.absTol);
+        launch("wc_scale", k_mul, dim3(blocks_for(B->n, 256)), dim3(256), 0, s, B->n, (const double*)W,
+               (const double*)R, R);
+        SVG_HIP(hipGetLastError());
+        return svg_picc_update(L->nNo, p->gam, p->dt, R, An, Yn, s);
+      });
+    if (rc) {
+      batch_graph_free(B);
+      return rc;
+    }
+    B->newtonKey = k.b;
+  }
+  if ((rc = batch_cycle_graph(B, R, Val, md, fa))) return rc;
+  SVG_HIP(hipEventRecord(B->ev, caller));
+  SVG_HIP(hipStreamWaitEvent(s, B->ev, 0));
+  SVG_HIP(hipGraphLaunch(B->pro, s));
+  std::vector<double> dB;
+  if ((rc = batch_iterate(B, md, proto->mItr, dB))) return rc;
+  SVG_HIP(hipGraphLaunch(B->epi, s));
+  SVG_HIP(hipEventRecord(B->ev, s));
+  SVG_HIP(hipStreamWaitEvent(caller, B->ev, 0));
+  if ((rc = construct_fluid_check(M, s))) return rc;
+  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  batch_records(B, proto, md, dB, secs, out);
   return SVG_OK;
 }
 

@@ -106,7 +106,7 @@ __device__ __forceinline__ void acc_rc(int wave, int lane, int qi, int v, int& r
 
 // Encoder input u of node row r (sample-major rows r = b*n + i) into LDS.
 __device__ __forceinline__ void load_u(bf16* U, int t, int r0, int rows, int n, const double* __restrict__ V,
-                                       double sc, const bf16* __restrict__ Dg)
+                                       double sc, const double2* __restrict__ nsc, const bf16* __restrict__ Dg)
 {
   for (int q = t; q < TILE * NIN; q += BS) {
     const int row = q / NIN, c = q % NIN, r = r0 + row;
@@ -114,7 +114,7 @@ __device__ __forceinline__ void load_u(bf16* U, int t, int r0, int rows, int n, 
     if (r < rows) {
       const int i = r % n;
       if (c < 4)
-        v = f2bf((float)(V[(size_t)r * 4 + c] * sc));
+        v = f2bf((float)(V[(size_t)r * 4 + c] * (nsc ? nsc[i].x : sc)));
       else if (c < 20)
         v = Dg[(size_t)i * 16 + (c - 4)];
       else if (c == 20)
@@ -144,8 +144,9 @@ __global__ __launch_bounds__(256) void k_diag_bf16(int nNo, const int* __restric
 
 // -------------------------------------------------------------- forward ---
 __global__ __launch_bounds__(BS) void k_encode(int n, int rows, const double* __restrict__ V, double sc,
-                                               const bf16* __restrict__ Dg, const bf16* __restrict__ W0,
-                                               bf16* __restrict__ H, const int* __restrict__ skip)
+                                               const double2* __restrict__ nsc, const bf16* __restrict__ Dg,
+                                               const bf16* __restrict__ W0, bf16* __restrict__ H,
+                                               const int* __restrict__ skip)
 {
   __shared__ bf16 U[TILE * US];
   __shared__ bf16 Ws[F * US];
@@ -155,7 +156,7 @@ __global__ __launch_bounds__(BS) void k_encode(int n, int rows, const double* __
   for (int q = t; q < F * NIN; q += BS) Ws[(q / NIN) * US + q % NIN] = W0[q];  // once per workgroup
   for (int tl = blockIdx.x; tl < ntiles; tl += gridDim.x) {
     const int r0 = tl * TILE;
-    load_u(U, t, r0, rows, n, V, sc, Dg);
+    load_u(U, t, r0, rows, n, V, sc, nsc, Dg);
     __syncthreads();
     v4f acc[4];
 #pragma unroll
@@ -267,7 +268,7 @@ __global__ __launch_bounds__(BS) void k_dense(int rows, const bf16* __restrict__
                                               const float* __restrict__ bias, bf16* __restrict__ Pst,
                                               const bf16* __restrict__ Wd, const double* __restrict__ V,
                                               double* __restrict__ Z, float* __restrict__ Z32, double inv_sc,
-                                              const int* __restrict__ skip)
+                                              const double2* __restrict__ nsc, const int* __restrict__ skip)
 {
   __shared__ __align__(16) bf16 X[TILE * XS];
   __shared__ __align__(16) bf16 Ws[F * XS];
@@ -325,7 +326,7 @@ __global__ __launch_bounds__(BS) void k_dense(int rows, const bf16* __restrict__
       if (r < rows) {
         float y = 0.f;
         for (int f = 0; f < F; f++) y = fmaf(bf2f(Wd[d * F + f]), bf2f(X[node * XS + f]), y);
-        const double z = V[(size_t)r * 4 + d] + (double)y * inv_sc;
+        const double z = V[(size_t)r * 4 + d] + (double)y * (nsc ? nsc[r].y : inv_sc);
         Z[(size_t)r * 4 + d] = z;
         if (Z32) Z32[(size_t)r * 4 + d] = (float)z;
       }
@@ -520,7 +521,7 @@ __global__ __launch_bounds__(BS) void k_enc_bwd(int n, int rows, const double* _
   acc[1] = v4f{0.f, 0.f, 0.f, 0.f};
   for (int tl = blockIdx.x; tl < ntiles; tl += gridDim.x) {
     const int r0 = tl * TILE;
-    load_u(U, t, r0, rows, n, V, sc, Dg);
+    load_u(U, t, r0, rows, n, V, sc, nullptr, Dg);
     for (int q = t; q < TILE * F; q += BS) {
       const int row = q >> 6, f = q & 63, r = r0 + row;
       GH[row * GS + f] = r < rows ? f2bf(G0[(size_t)r * F + f]) : (bf16)0;
@@ -659,11 +660,18 @@ static int lhs_trans(svg_lhs* L)
 
 int lhs_ensure_trans(svg_lhs* L) { return lhs_trans(L); }
 
-int lpc_prepare_dev(svg_lhs* L, const double* Val, hipStream_t s)
+static int lhs_bf16_alloc(svg_lhs* L)
 {
   int rc;
   if (!L->d_val16 && (rc = dev_alloc(&L->d_val16, (size_t)16 * L->nnz))) return rc;
   if (!L->d_diag16 && (rc = dev_alloc(&L->d_diag16, (size_t)16 * L->nNo))) return rc;
+  return SVG_OK;
+}
+
+int lpc_prepare_dev(svg_lhs* L, const double* Val, hipStream_t s)
+{
+  int rc;
+  if ((rc = lhs_bf16_alloc(L))) return rc;
   launch("lpc_prep", k_val_bf16, dim3(4096), dim3(256), 0, s, (size_t)16 * L->nnz, Val, (bf16*)L->d_val16);
   launch("lpc_prep", k_diag_bf16, dim3(blocks_for((size_t)16 * L->nNo, 256)), dim3(256), 0, s, L->nNo,
          (const int*)L->d_diagPtr, Val, (bf16*)L->d_diag16);
@@ -672,18 +680,19 @@ int lpc_prepare_dev(svg_lhs* L, const double* Val, hipStream_t s)
 }
 
 // Forward pass on nvec stacked vectors V (nvec, nNo, 4) -> Z; training keeps
-// m and p of every layer.
+// m and p of every layer. nsc (nvec = 1 only): per-node (sc, 1/sc) in place of
+// sqrt(nNo) (independent systems of a block-diagonal union, each scaled by its own size).
 static int lpc_forward(svg_lpc* N, const svg_lhs* L, int nvec, const double* V, double* Z, float* Z32,
-                       bool train, const int* skip, hipStream_t s)
+                       bool train, const int* skip, hipStream_t s, const double2* nsc = nullptr)
 {
   const int n = L->nNo, rows = nvec * n;
+  if (nsc && nvec != 1) return set_error(SVG_ERR_ARG, "lpc_forward: per-node scales need one vector");
   const double sc = std::sqrt((double)n);
   const unsigned nt = blocks_for(rows, TILE);
   const bf16* pb = N->pb;
   // algorithmic bytes: V and the diagonal blocks read, h written
   launch_b("lpc_encode", (32.0 + 32.0 + 2.0 * F) * rows, k_encode, dim3(std::min(nt, 2048u)), dim3(BS), 0, s, n,
-           rows, V, sc,
-           (const bf16*)L->d_diag16, pb + OFF_W0, N->H[0], skip);
+           rows, V, sc, nsc, (const bf16*)L->d_diag16, pb + OFF_W0, N->H[0], skip);
   const unsigned nm = blocks_for((size_t)rows * 4, 256);
   for (int l = 0; l < NL; l++) {
     const bool last = l == NL - 1;
@@ -698,18 +707,24 @@ static int lpc_forward(svg_lpc* N, const svg_lhs* L, int nvec, const double* V, 
     launch_b("lpc_dense", db, k_dense, dim3(std::min(nt, 2048u)), dim3(BS), 0, s, rows, Hin, (const bf16*)Ml, Hout,
              pb + OFF_L + l * LSZ, (const float*)(N->p + OFF_L + l * LSZ + F * 2 * F),
              train ? N->P[l] : (bf16*)nullptr, last ? pb + OFF_WD : (const bf16*)nullptr, V,
-             last ? Z : (double*)nullptr, last ? Z32 : (float*)nullptr, 1.0 / sc, skip);
+             last ? Z : (double*)nullptr, last ? Z32 : (float*)nullptr, 1.0 / sc, nsc, skip);
   }
   SVG_HIP(hipGetLastError());
   return SVG_OK;
 }
 
 int lpc_apply_dev(svg_lpc* N, const svg_lhs* L, const double* V, double* Z, float* Z32, const int* skip,
-                  hipStream_t s)
+                  hipStream_t s, const double2* nsc)
 {
   int rc = lpc_ensure(N, L->nNo, false);
   if (rc) return rc;
-  return lpc_forward(N, L, 1, V, Z, Z32, false, skip, s);
+  return lpc_forward(N, L, 1, V, Z, Z32, false, skip, s, nsc);
+}
+
+int lpc_reserve(svg_lpc* N, svg_lhs* L)
+{
+  int rc = lhs_bf16_alloc(L);
+  return rc ? rc : lpc_ensure(N, L->nNo, false);
 }
 
 // Loss and gradient of L = (1/B) sum_b |R_b - A M(R_b)|^2 into N->g.

@@ -121,6 +121,46 @@ __global__ void k_picc(int n, double coef, const double* __restrict__ R, double*
   Yn[k] = Yn[k] - r * coef;
 }
 
+// picp for the fluid rows (pic.cpp:676-693): An = Ao coef, Yn = Yo.
+__global__ void k_picp(int n, double coef, const double* __restrict__ Ao, const double* __restrict__ Yo,
+                       double* __restrict__ An, double* __restrict__ Yn)
+{
+#pragma clang fp contract(off)
+  size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (size_t)n) return;
+  An[k] = Ao[k] * coef;
+  Yn[k] = Yo[k];
+}
+
+// set_bc_dir_l (set_bc.cpp:1005-1063, non-generic profile) with the scatter of
+// set_bc_dir (set_bc.cpp:884-938, no impD): one lane per (face node a, k < lDof).
+// Row of k: s + k, or, with the direction mask eDir, s + (the k-th set bit).
+__global__ void k_set_bc_dir(int nNo, int lDof, int tDof, int s, unsigned eDir, const int* __restrict__ gN,
+                             const double* __restrict__ gxn, double dirA, double dirY, double* __restrict__ A,
+                             double* __restrict__ Y)
+{
+#pragma clang fp contract(off)
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)nNo * lDof) return;
+  const int a = (int)(t / lDof), k = (int)(t % lDof);
+  int row = s + k;
+  if (eDir) {
+    int c = 0;
+    for (int i = 0; i < 32; i++)
+      if ((eDir >> i) & 1u) {
+        if (c == k) {
+          row = s + i;
+          break;
+        }
+        c++;
+      }
+  }
+  const double g = gxn[(size_t)lDof * a + k];
+  const size_t at = (size_t)tDof * gN[a] + row;
+  A[at] = dirA * g;
+  Y[at] = dirY * g;
+}
+
 __global__ void k_fill(size_t n, double v, double* __restrict__ d)
 {
   size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -187,6 +227,38 @@ int svg_picc_update(int nNo, double gam, double dt, const double* R, double* An,
   if (nNo == 0) return SVG_OK;
   int n = 4 * nNo;
   launch("picc", k_picc, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream, n, gam * dt, R, An, Yn);
+  SVG_HIP(hipGetLastError());
+  return SVG_OK;
+}
+
+int svg_picp(int nNo, double gam, const double* Ao, const double* Yo, double* An, double* Yn, void* stream)
+{
+  if (nNo < 0) return set_error(SVG_ERR_ARG, "svg_picp: nNo < 0");
+  if (nNo == 0) return SVG_OK;
+  if (!Ao || !Yo || !An || !Yn) return set_error(SVG_ERR_ARG, "svg_picp: null pointer");
+  if (gam == 0.0) return set_error(SVG_ERR_ARG, "svg_picp: gam = 0");
+  const int n = 4 * nNo;
+  launch("picp", k_picp, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream, n, (gam - 1.0) / gam, Ao, Yo,
+         An, Yn);
+  SVG_HIP(hipGetLastError());
+  return SVG_OK;
+}
+
+int svg_set_bc_dir(int nNo, const int* gN, int tDof, int s, int lDof, unsigned eDir, const double* gxn, double dirA,
+                   double dirY, double* A, double* Y, void* stream)
+{
+  if (nNo < 0 || lDof < 1 || tDof < 1 || s < 0) return set_error(SVG_ERR_ARG, "svg_set_bc_dir: bad sizes");
+  if (nNo == 0) return SVG_OK;
+  if (!gN || !gxn || !A || !Y) return set_error(SVG_ERR_ARG, "svg_set_bc_dir: null pointer");
+  int top = s + lDof - 1;
+  if (eDir) {
+    if (__builtin_popcount(eDir) != lDof) return set_error(SVG_ERR_ARG, "svg_set_bc_dir: lDof != directions in eDir");
+    top = s + 31 - __builtin_clz(eDir);
+  }
+  if (top >= tDof) return set_error(SVG_ERR_ARG, "svg_set_bc_dir: rows beyond tDof");
+  const size_t n = (size_t)nNo * lDof;
+  launch("set_bc_dir", k_set_bc_dir, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream, nNo, lDof, tDof,
+         s, eDir, gN, gxn, dirA, dirY, A, Y);
   SVG_HIP(hipGetLastError());
   return SVG_OK;
 }

@@ -1,13 +1,15 @@
-// One time step of the fluid equation on a seeded pipe, driven from C++ over
-// the host API (svg_fsils.hpp): the reference's Newton loop for one fluid
-// equation (Code/Source/solver/main.cpp:371-617) —
-//   pici -> ls_alloc (R = 0, Val = 0) -> construct_fluid -> b_assem_neu_bc on
-//   the outlet (zero traction + backflow stabilisation) -> fsils_solve -> picc
-// with picc's convergence test (pic.cpp:236-281) on the host.
+// Time steps of the fluid equation on a seeded pipe, driven from C++ over the
+// host API (svg_fsils.hpp): the reference's time loop for one fluid equation
+// (Code/Source/solver/main.cpp:300-760) —
+//   picp -> set_bc_dir (inlet, wall) -> Newton loop { pici -> ls_alloc (R = 0,
+//   Val = 0) -> construct_fluid -> b_assem_neu_bc on the outlet (zero traction
+//   + backflow stabilisation) -> fsils_solve -> picc } -> Ao = An, Yo = Yn
+// with picc's convergence test (pic.cpp:236-281) on the host; the state stays
+// on the device throughout.
 //
-//   svg_newton_step <n> [relTol] [krylov] [maxItr]
+//   svg_newton_step <n> [relTol] [krylov] [maxItr] [steps]
 //
-// prints one JSON record per Newton iteration (itr, ls_itr, iNorm, fNorm, suc).
+// prints one JSON record per Newton iteration (step, itr, ls_itr, iNorm, fNorm, suc).
 // The workload: pipe radius 0.3, length 3, n x n cross-section, 5n axial,
 // jitter 0.1 h, seed 20261202; rho 1.06, mu 0.04, parabolic inflow peak 30,
 // no-slip walls, zero-traction outlet, dt 1e-3, rho_inf 0.5; GMRES with the
@@ -42,6 +44,11 @@ int main(int argc, char** argv)
   const double relTol = argc > 2 ? std::atof(argv[2]) : 1e-6;
   const int sD = argc > 3 ? std::atoi(argv[3]) : 50;
   const int mItr = argc > 4 ? std::atoi(argv[4]) : 1000;
+  const int nSteps = argc > 5 ? std::atoi(argv[5]) : 1;
+  if (n < 2 || n % 2 || nSteps < 1) {
+    std::fprintf(stderr, "usage: %s <n> [relTol] [krylov] [maxItr] [steps]  (n even >= 2, steps >= 1)\n", argv[0]);
+    return 2;
+  }
   try {
     // mesh and pattern (setup)
     svg_pipe_spec sp{n, 5 * n, 0.3, 3.0, 0.1, 20261202ULL, 0.0, 0.0, 0.0};
@@ -94,22 +101,24 @@ int main(int argc, char** argv)
     p.dt = 1e-3;
     p.qmTET4 = (5.0 + 3.0 * std::sqrt(5.0)) / 20.0;
 
-    // initial state: Ao = Yo = 0, predictor An = Ao (gam-1)/gam, Yn = Yo, then
-    // the Dirichlet values face by face (inlet parabola, wall zero)
-    std::vector<double> Ao((size_t)4 * nNo, 0.0), Yo(Ao), An(Ao), Yn(Ao);
-    for (int a : inlet) {
+    // strong Dirichlet faces (steady: dirA = 0, dirY = 1 with gx nV holding the
+    // values): the inlet parabola u_z = peak (1 - r^2/R^2), then the wall (u = 0)
+    std::vector<double> gIn((size_t)3 * inlet.size(), 0.0), gWall((size_t)3 * wall.size(), 0.0);
+    for (size_t k = 0; k < inlet.size(); k++) {
+      const int a = inlet[k];
       const double r2 = (x[3 * a] * x[3 * a] + x[3 * a + 1] * x[3 * a + 1]) / (R0 * R0);
-      for (int i = 0; i < 3; i++) An[4 * a + i] = Yn[4 * a + i] = 0.0;
-      Yn[4 * a + 2] = peak * (1.0 - r2);
+      gIn[3 * k + 2] = peak * (1.0 - r2);
     }
-    for (int a : wall)
-      for (int i = 0; i < 3; i++) An[4 * a + i] = Yn[4 * a + i] = 0.0;
-    DeviceArray<double> dAo, dYo, dAn, dYn, dAg((size_t)4 * nNo), dYg((size_t)4 * nNo), dR((size_t)4 * nNo),
-        dVal((size_t)16 * nnz);
-    dAo.upload(Ao);
-    dYo.upload(Yo);
-    dAn.upload(An);
-    dYn.upload(Yn);
+    DirichletFace dirIn, dirWall;
+    dirichlet_face_create(dirIn, inlet, 3, gIn);
+    dirichlet_face_create(dirWall, wall, 3, gWall);
+
+    // initial state Ao = Yo = 0
+    const std::vector<double> zero((size_t)4 * nNo, 0.0);
+    DeviceArray<double> dAo, dYo, dAn((size_t)4 * nNo), dYn((size_t)4 * nNo), dAg((size_t)4 * nNo),
+        dYg((size_t)4 * nNo), dR((size_t)4 * nNo), dVal((size_t)16 * nnz);
+    dAo.upload(zero);
+    dYo.upload(zero);
 
     FSILS_lsType ls;
     fsils_ls_create(ls, LinearSolverType::LS_TYPE_GMRES);
@@ -125,23 +134,34 @@ int main(int argc, char** argv)
     const std::vector<int> incL(2, 1);
     const std::vector<double> res(2, 0.0);
     std::printf("{\"nNo\": %d, \"nEl\": %d, \"nnz\": %d, \"colours\": %d}\n", nNo, nEl, nnz, msh.nColors);
-    for (int itr = 1;; itr++) {
-      pici(nNo, p.am, p.af, dAo.data(), dAn.data(), dYo.data(), dYn.data(), dAg.data(), dYg.data());
-      check(svg_fill(dR.data(), dR.size(), 0.0, nullptr));
-      check(svg_fill(dVal.data(), dVal.size(), 0.0, nullptr));
-      construct_fluid(msh, p, dAg.data(), dYg.data(), nullptr, dR.data(), dVal.data());
-      b_assem_neu_bc(outlet, p, 0.2, 2.0 / 3.0, nullptr, dYg.data(), dR.data(), dVal.data());
-      fsils_solve(lhs, ls, 4, dR.data(), dVal.data(), PreconditionerType::PREC_FSILS, incL, res);
-      picc_update(nNo, p.gam, p.dt, dR.data(), dAn.data(), dYn.data());
-      if (is_zero(ls.RI.iNorm)) ls.RI.iNorm = DBL_EPSILON;
-      if (is_zero(eqINorm)) eqINorm = ls.RI.iNorm;
-      if (itr == 1) pNorm = ls.RI.iNorm / eqINorm;
-      const double r1 = ls.RI.iNorm / eqINorm;
-      const bool ok = itr >= maxIt || ((r1 <= tol || r1 <= tol * pNorm) && itr >= minIt);
-      std::printf("{\"itr\": %d, \"ls_itr\": %d, \"iNorm\": %.17g, \"fNorm\": %.17g, \"suc\": %d, \"r1\": %.17g}\n", itr,
-                  ls.RI.itr, ls.RI.iNorm, ls.RI.fNorm, ls.RI.suc, r1);
-      std::fflush(stdout);
-      if (ok) break;
+    for (int step = 1; step <= nSteps; step++) {
+      // predictor and strong Dirichlet values (main.cpp:346-360)
+      picp(nNo, p.gam, dAo.data(), dYo.data(), dAn.data(), dYn.data());
+      set_bc_dir(dirIn, 4, 0, 0.0, 1.0, dAn.data(), dYn.data());
+      set_bc_dir(dirWall, 4, 0, 0.0, 1.0, dAn.data(), dYn.data());
+      for (int itr = 1;; itr++) {
+        pici(nNo, p.am, p.af, dAo.data(), dAn.data(), dYo.data(), dYn.data(), dAg.data(), dYg.data());
+        check(svg_fill(dR.data(), dR.size(), 0.0, nullptr));
+        check(svg_fill(dVal.data(), dVal.size(), 0.0, nullptr));
+        construct_fluid(msh, p, dAg.data(), dYg.data(), nullptr, dR.data(), dVal.data());
+        b_assem_neu_bc(outlet, p, 0.2, 2.0 / 3.0, nullptr, dYg.data(), dR.data(), dVal.data());
+        fsils_solve(lhs, ls, 4, dR.data(), dVal.data(), PreconditionerType::PREC_FSILS, incL, res);
+        picc_update(nNo, p.gam, p.dt, dR.data(), dAn.data(), dYn.data());
+        if (is_zero(ls.RI.iNorm)) ls.RI.iNorm = DBL_EPSILON;
+        if (is_zero(eqINorm)) eqINorm = ls.RI.iNorm;
+        if (itr == 1) pNorm = ls.RI.iNorm / eqINorm;
+        const double r1 = ls.RI.iNorm / eqINorm;
+        const bool ok = itr >= maxIt || ((r1 <= tol || r1 <= tol * pNorm) && itr >= minIt);
+        std::printf(
+            "{\"step\": %d, \"itr\": %d, \"ls_itr\": %d, \"iNorm\": %.17g, \"fNorm\": %.17g, \"suc\": %d, "
+            "\"r1\": %.17g}\n",
+            step, itr, ls.RI.itr, ls.RI.iNorm, ls.RI.fNorm, ls.RI.suc, r1);
+        std::fflush(stdout);
+        if (ok) break;
+      }
+      // Ao = An, Yo = Yn (main.cpp:754-755)
+      dAo.copy_from(dAn);
+      dYo.copy_from(dYn);
     }
   } catch (const std::exception& e) {
     std::fprintf(stderr, "%s\n", e.what());

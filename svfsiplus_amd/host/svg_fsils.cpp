@@ -105,4 +105,27 @@ void picc_update(int nNo, double gam, double dt, const double* R, double* An, do
   check(svg_picc_update(nNo, gam, dt, R, An, Yn, stream));
 }
 
+void picp(int nNo, double gam, const double* Ao, const double* Yo, double* An, double* Yn, hipStream_t stream)
+{
+  check(svg_picp(nNo, gam, Ao, Yo, An, Yn, stream));
+}
+
+void dirichlet_face_create(DirichletFace& f, const std::vector<int>& gN, int lDof, const std::vector<double>& gxn,
+                           unsigned eDir)
+{
+  if (lDof < 1 || gxn.size() != (size_t)lDof * gN.size())
+    throw std::runtime_error("dirichlet_face_create: gxn must be (lDof, nNo)");
+  f.gN.upload(gN);
+  f.gxn.upload(gxn);
+  f.nNo = (int)gN.size();
+  f.lDof = lDof;
+  f.eDir = eDir;
+}
+
+void set_bc_dir(const DirichletFace& f, int tDof, int s, double dirA, double dirY, double* A, double* Y,
+                hipStream_t stream)
+{
+  check(svg_set_bc_dir(f.nNo, f.gN.data(), tDof, s, f.lDof, f.eDir, f.gxn.data(), dirA, dirY, A, Y, stream));
+}
+
 }  // namespace svg_host

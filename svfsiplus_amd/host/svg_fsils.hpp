@@ -6,7 +6,8 @@
 //   fsi_linear_solver::fsils_lhs_create / fsils_bc_create / fsils_ls_create /
 //   fsils_solve / fsils_lhs_free        (Code/Source/liner_solver/fsils_api.hpp:37-61)
 //   fluid::construct_fluid              (Code/Source/solver/fluid.h:51, fluid.cpp:464)
-//   pic::pici / the picc update          (Code/Source/solver/pic.cpp:486-574, 155-167)
+//   pic::pici / the picc update / picp   (Code/Source/solver/pic.cpp:486-574, 155-167, 591-715)
+//   set_bc::set_bc_dir                   (Code/Source/solver/set_bc.cpp:794-1063)
 // Arrays keep the reference layouts; the solver arrays (Ri, Val) and the
 // nodal state live in device memory (DeviceArray), the pattern and face
 // lists are host vectors as in the reference.
@@ -151,5 +152,23 @@ void construct_fluid(FluidMesh& msh, const svg_fluid_params& p, const double* Ag
 void pici(int nNo, double am, double af, const double* Ao, const double* An, const double* Yo, const double* Yn,
           double* Ag, double* Yg, hipStream_t stream = nullptr);
 void picc_update(int nNo, double gam, double dt, const double* R, double* An, double* Yn, hipStream_t stream = nullptr);
+// picp (pic.cpp:591-715) for the fluid equation: An = Ao (gam-1)/gam, Yn = Yo.
+void picp(int nNo, double gam, const double* Ao, const double* Yo, double* An, double* Yn,
+          hipStream_t stream = nullptr);
+
+// One strong Dirichlet face for set_bc_dir (set_bc.cpp:794-1063): its nodes
+// (fa.gN) and gx nV (lDof = nsd) or gx per node, (lDof, nNo), on the device.
+struct DirichletFace {
+  DeviceArray<int> gN;
+  DeviceArray<double> gxn;
+  int nNo = 0, lDof = 0;
+  unsigned eDir = 0;  // bc.eDrn as a direction mask (0: all lDof rows)
+};
+void dirichlet_face_create(DirichletFace& f, const std::vector<int>& gN, int lDof, const std::vector<double>& gxn,
+                           unsigned eDir = 0);
+// A(s.., gN) = dirA gxn, Y(s.., gN) = dirY gxn for A, Y (tDof, tnNo); steady
+// values: dirA = 0, dirY = bc.g.
+void set_bc_dir(const DirichletFace& f, int tDof, int s, double dirA, double dirY, double* A, double* Y,
+                hipStream_t stream = nullptr);
 
 }  // namespace svg_host

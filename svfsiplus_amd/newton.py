@@ -23,7 +23,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib
-from .workload import Physics, Pipe, dirichlet_state, outlet_faces
+from .workload import Physics, Pipe, dirichlet_faces, dirichlet_state, outlet_faces
 
 
 def _ptr(t) -> int:
@@ -151,6 +151,19 @@ def construct_fluid(msh: DeviceMesh, phys: Physics, Ag, Yg, R, Val, Bf=None):
                                                None if Bf is None else _ptr(Bf), _ptr(R), _ptr(Val), _stream()))
 
 
+def solve_opts(precision: int = _lib.PRECISION_FP64, lpc=None, fallback_window: int = 0,
+               net_period: int = 1) -> _lib.SolveOpts:
+    """The svg_solve_opts of a solve: precision, and the learned method when a network is given."""
+    opts = _lib.SolveOpts()
+    opts.precision = precision
+    if lpc is not None:
+        opts.method = _lib.METHOD_LEARNED
+        opts.lpc = lpc.h.value if hasattr(lpc.h, "value") else lpc.h
+        opts.fallback_window = fallback_window
+        opts.net_period = net_period
+    return opts
+
+
 def fsils_solve(lhs: DeviceLhs, ls: _lib.Ls, dof: int, Ri, Val, prec: int = _lib.PREC_FSILS, incL=None, res=None,
                 precision: int = _lib.PRECISION_FP64, lpc=None, fallback_window: int = 0, net_period: int = 1,
                 info: dict | None = None):
@@ -161,13 +174,7 @@ def fsils_solve(lhs: DeviceLhs, ls: _lib.Ls, dof: int, Ri, Val, prec: int = _lib
     nf = lhs.nFaces
     inc = np.ones(max(1, nf), dtype=np.int32) if incL is None else np.ascontiguousarray(incL, dtype=np.int32)
     rs = None if res is None else np.ascontiguousarray(res, dtype=np.float64)
-    opts = _lib.SolveOpts()
-    opts.precision = precision
-    if lpc is not None:
-        opts.method = _lib.METHOD_LEARNED
-        opts.lpc = lpc.h.value if hasattr(lpc.h, "value") else lpc.h
-        opts.fallback_window = fallback_window
-        opts.net_period = net_period
+    opts = solve_opts(precision, lpc, fallback_window, net_period)
     _lib.check(_lib.load().svg_fsils_solve_ex(lhs.h, C.byref(ls), dof, _ptr(Ri), _ptr(Val), prec, nf,
                                               inc.ctypes.data, None if rs is None else rs.ctypes.data, C.byref(opts),
                                               _stream()))
@@ -231,6 +238,10 @@ class FluidNewton:
         self._init = [t.clone() for t in (self.Ao, self.Yo, self.An, self.Yn)]
         self.Val = torch.empty((pipe.nnz, 16), dtype=f64, device=dev)
         self.history = []
+        # strong Dirichlet faces for set_bc_dir at later time steps (node ids, gx nV)
+        self._dir = [(torch.tensor(g, dtype=torch.int32, device=dev), torch.tensor(v, dtype=f64, device=dev))
+                     for g, v in dirichlet_faces(pipe, phys)]
+        self.step = 0
 
     def reset(self):
         """Back to the initial state of the time step (device-to-device copies)."""
@@ -238,6 +249,7 @@ class FluidNewton:
             t.copy_(a)
         self.eq = EqState(self.eq.minItr, self.eq.maxItr, self.eq.tol)
         self.history = []
+        self.step = 0
 
     def newton_iteration(self):
         lib, ph, eq, ls = _lib.load(), self.phys, self.eq, self.ls
@@ -280,3 +292,27 @@ class FluidNewton:
         while not self.newton_iteration():
             pass
         return self.history
+
+    def advance(self):
+        """To the next time step on the device: Ao = An, Yo = Yn (main.cpp:754-755),
+        the predictor (picp, pic.cpp:591-715) and the strong Dirichlet values
+        of every face (set_bc_dir, set_bc.cpp:794-1063; steady: dirA = 0, dirY = 1)."""
+        lib, s = _lib.load(), _stream()
+        self.Ao.copy_(self.An)
+        self.Yo.copy_(self.Yn)
+        _lib.check(lib.svg_picp(self.pipe.nNo, self.phys.gam, _ptr(self.Ao), _ptr(self.Yo), _ptr(self.An),
+                                _ptr(self.Yn), s))
+        for gN, gxn in self._dir:
+            _lib.check(lib.svg_set_bc_dir(gN.shape[0], _ptr(gN), 4, 0, 3, 0, _ptr(gxn), 0.0, 1.0, _ptr(self.An),
+                                          _ptr(self.Yn), s))
+        self.history = []
+        self.step += 1
+
+    def time_steps(self, nsteps: int):
+        """nsteps time steps from the current state; returns the Newton history of each."""
+        out = []
+        for k in range(nsteps):
+            if k:
+                self.advance()
+            out.append(list(self.time_step()))
+        return out

@@ -199,6 +199,18 @@ def dirichlet_state(p: Pipe, phys: Physics):
     return Ao, Yo, An, Yn
 
 
+def dirichlet_faces(p: Pipe, phys: Physics):
+    """The strong Dirichlet faces in FSILS face order, as set_bc_dir takes them:
+    [(gN, gxn(nNo, 3))] with dirA = 0, dirY = 1 (steady values): the inlet's
+    parabola u_z = peak (1 - r^2/R^2), then the wall (u = 0)."""
+    inlet = np.ascontiguousarray(p.faces[0][1], dtype=np.int32)
+    r2 = (p.x[inlet, 0] ** 2 + p.x[inlet, 1] ** 2) / (p.radius ** 2)
+    g_in = np.zeros((inlet.shape[0], 3))
+    g_in[:, 2] = phys.peak * (1.0 - r2)
+    wall = np.ascontiguousarray(p.faces[1][1], dtype=np.int32)
+    return [(inlet, g_in), (wall, np.zeros((wall.shape[0], 3)))]
+
+
 def face_arrays(p: Pipe):
     """Concatenated FSILS face data: (nNo per face, glob, val(3,n) with val = 0)."""
     nn = np.array([len(f[1]) for f in p.faces], dtype=np.int32)

@@ -77,3 +77,82 @@ def test_batch_newton_time_step(torch_cuda):
                   f"single itr {rs['ls_itr']} iNorm {rs['iNorm']:.6e}")
             assert abs(rb["itr"] - rs["ls_itr"]) <= max(2, 0.02 * rs["ls_itr"])
             assert abs(rb["iNorm"] - rs["iNorm"]) <= 1e-8 * rs["iNorm"]
+
+
+@pytest.mark.parametrize("precision", [_lib.PRECISION_FP64, _lib.PRECISION_MIXED])
+def test_batch_newton_graph_equals_eager(torch_cuda, precision):
+    """The graph-captured Newton iteration (svg_batch_newton_iteration) against
+    the same launches enqueued one by one: identical records and state, bit for
+    bit, on every replay (two time steps, eager work and a reset in between)."""
+    torch = torch_cuda
+    meshes = _meshes(3, seed=11)
+    phys = W.Physics()
+    bn = B.BatchNewton(meshes, phys, precision=precision)
+    ref = bn.time_step(3, graph=False)
+    state = [t.clone() for t in (bn.fn.An, bn.fn.Yn, bn.fn.R)]
+    for rep in range(2):
+        bn.reset()
+        torch.cuda.synchronize()
+        got = bn.time_step(3, graph=True)
+        torch.cuda.synchronize()
+        for it in range(3):
+            for q in range(len(meshes)):
+                assert got[it][q] == ref[it][q], (rep, it, q, got[it][q], ref[it][q])
+        for a, b in zip((bn.fn.An, bn.fn.Yn, bn.fn.R), state):
+            assert torch.equal(a, b), rep
+        # eager work between the replays
+        bn.fn.R.mul_(0.5)
+        print(f"replay {rep}: itr {[r['itr'] for r in got[0]]} identical to the eager iteration")
+
+
+@pytest.mark.parametrize("precision", [_lib.PRECISION_FP64, _lib.PRECISION_MIXED])
+def test_batch_learned_equals_individual_learned(torch_cuda, precision):
+    """The learned preconditioner in the batched solve: each mesh's flexible
+    GMRES (per-mesh fallback) is bit for bit the single-system learned solve of
+    that mesh (iterations, norms, solution, fallback point): the trained network
+    on every 5th step, and a harmful one (large random decoder) on every step,
+    which makes the meshes fall back to the diagonal preconditioner."""
+    import os
+    from oracle import lpc as O
+    from svfsiplus_amd.lpc import LearnedPC, init_params
+    from svfsiplus_amd.newton import solve_opts
+    torch = torch_cuda
+    trained = LearnedPC.load(os.path.join(os.path.dirname(_lib.LIB_PATH), "data", "lpc_weights.npz"))
+    p = init_params(5)
+    p[O.OFF_WD:] = (np.random.default_rng(105).standard_normal(4 * O.F) * 50.0).astype(np.float32)
+    harmful = LearnedPC(p)
+    meshes = _meshes(4, seed=13)
+    phys = W.Physics()
+    bn = _assembled_union(torch, meshes, phys)
+    R0, V0 = bn.fn.R.clone(), bn.fn.Val.clone()
+    boff = np.concatenate([[0], np.cumsum([m.nnz for m in meshes])])
+    window = 20
+    for case, net, period in (("trained", trained, 5), ("harmful", harmful, 1)):
+        singles = []
+        for q, m in enumerate(meshes):
+            a, b = int(bn.noff[q]), int(bn.noff[q + 1])
+            Rq = R0[a:b].clone()
+            Vq = V0[boff[q]:boff[q + 1]].clone()
+            ls = ls_create(_lib.LS_GMRES, relTol=1e-6, absTol=1e-30, maxItr=1000, dimKry=50)
+            info = {}
+            fsils_solve(DeviceLhs(m), ls, 4, Rq, Vq, precision=precision, lpc=net, fallback_window=window,
+                        net_period=period, info=info)
+            singles.append((Rq, ls.RI.itr, ls.RI.iNorm, ls.RI.fNorm, ls.RI.suc, info["fallback_itr"]))
+        Rb, Vb = R0.clone(), V0.clone()
+        opts = solve_opts(precision, net, window, period)
+        _lib.check(_lib.load().svg_batch_fsils_solve_ex(bn.h, C.byref(bn.proto), Rb.data_ptr(), Vb.data_ptr(),
+                                                        C.byref(opts), bn.out, None))
+        torch.cuda.synchronize()
+        fb = np.empty(len(meshes), dtype=np.int32)
+        _lib.check(_lib.load().svg_batch_fallback_itr(bn.h, fb.ctypes.data))
+        for q, (Rq, itr, iN, fN, suc, fbi) in enumerate(singles):
+            a, b = int(bn.noff[q]), int(bn.noff[q + 1])
+            diff = float((Rb[a:b] - Rq).abs().max())
+            print(f"{case} mesh {q}: itr batch {bn.out[q].itr} single {itr} fallback {fb[q]}/{fbi} "
+                  f"fNorm {bn.out[q].fNorm:.6e}/{fN:.6e} max|dx| {diff:.2e}")
+            assert (bn.out[q].itr, bn.out[q].iNorm, bn.out[q].fNorm, bn.out[q].suc) == (itr, iN, fN, suc)
+            assert fb[q] == fbi
+            assert diff == 0.0
+            assert bn.out[q].suc == 1
+        if case == "harmful":
+            assert (fb >= window).all(), fb

@@ -169,6 +169,64 @@ def test_newton_time_step(torch_cuda, phys):
     assert np.linalg.norm(Yn1 - Yn0) <= 1e-6 * np.linalg.norm(Yn0)
 
 
+def test_picp_set_bc_dir_bitwise(torch_cuda, phys):
+    """svg_picp and svg_set_bc_dir against the oracle, bit for bit (all rows and a direction mask)."""
+    torch = torch_cuda
+    lib = _lib.load()
+    rng = np.random.default_rng(21)
+    n = 1000
+    Ao, Yo = rng.standard_normal((n, 4)), rng.standard_normal((n, 4))
+    An0, Yn0 = oracle.picp(phys, Ao, Yo)
+    dAo, dYo = _dev(torch, Ao), _dev(torch, Yo)
+    dAn = torch.empty_like(dAo)
+    dYn = torch.empty_like(dYo)
+    _lib.check(lib.svg_picp(n, phys.gam, dAo.data_ptr(), dYo.data_ptr(), dAn.data_ptr(), dYn.data_ptr(), None))
+    torch.cuda.synchronize()
+    assert np.array_equal(dAn.cpu().numpy(), An0) and np.array_equal(dYn.cpu().numpy(), Yn0)
+    gN = rng.choice(n, 300, replace=False).astype(np.int32)
+    for s0, lDof, eDir in ((0, 3, 0), (0, 2, 0b101), (1, 2, 0b011), (0, 4, 0)):
+        gxn = rng.standard_normal((gN.size, lDof))
+        A, Y = An0.copy(), Yn0.copy()
+        oracle.set_bc_dir(gN, gxn, A, Y, s=s0, eDir=eDir, dirA=0.25, dirY=-1.5)
+        dA, dY = _dev(torch, An0), _dev(torch, Yn0)
+        dg = torch.tensor(gN, device="cuda")
+        dx = _dev(torch, gxn)
+        _lib.check(lib.svg_set_bc_dir(gN.size, dg.data_ptr(), 4, s0, lDof, eDir, dx.data_ptr(), 0.25, -1.5,
+                                      dA.data_ptr(), dY.data_ptr(), None))
+        torch.cuda.synchronize()
+        assert np.array_equal(dA.cpu().numpy(), A) and np.array_equal(dY.cpu().numpy(), Y), (s0, lDof, eDir)
+    # rows past tDof are an error, not a write
+    assert lib.svg_set_bc_dir(gN.size, dg.data_ptr(), 4, 2, 3, 0, dx.data_ptr(), 0.0, 1.0, dA.data_ptr(),
+                              dY.data_ptr(), None) != 0
+
+
+def test_two_time_steps(torch_cuda, phys):
+    """Two time steps on the device (advance: Ao = An, Yo = Yn, picp and
+    set_bc_dir between them) against the oracle's time loop."""
+    from svfsiplus_amd.newton import FluidNewton, ls_create
+    pipe = W.make_pipe(4)
+    tol = 1e-6
+    mk = lambda: oracle.ls_create(oracle.LS_GMRES, relTol=tol, absTol=1e-30, maxItr=1000, dimKry=50)
+    outlet = W.outlet_faces(pipe)
+    An0, Yn0, hists0 = oracle.time_steps(pipe, phys, mk, 2, outlet=outlet)
+    fn = FluidNewton(pipe, phys, ls_create(_lib.LS_GMRES, relTol=tol, absTol=1e-30, maxItr=1000, dimKry=50))
+    hists1 = fn.time_steps(2)
+    assert len(hists1) == 2
+    for step, (h0s, h1s) in enumerate(zip(hists0, hists1)):
+        assert len(h0s) == len(h1s) == 3
+        for h0, h1 in zip(h0s, h1s):
+            print(f"step {step + 1}: oracle itr {h0['ls_itr']} iNorm {h0['iNorm']:.9e} r1 {h0['r1']:.6e} | "
+                  f"gpu itr {h1['ls_itr']} iNorm {h1['iNorm']:.9e} r1 {h1['r1']:.6e}")
+            assert abs(h0["ls_itr"] - h1["ls_itr"]) <= max(1, 0.02 * h0["ls_itr"])
+            assert abs(h0["iNorm"] - h1["iNorm"]) <= 1e-8 * h0["iNorm"]
+            assert abs(h0["r1"] - h1["r1"]) <= 1e-8 * h0["r1"]
+    # the second step starts from the first one's solution: different residuals
+    assert hists0[1][0]["iNorm"] != hists0[0][0]["iNorm"]
+    Yn1 = fn.Yn.cpu().numpy()
+    assert np.linalg.norm(Yn1 - Yn0) <= 1e-6 * np.linalg.norm(Yn0)
+    assert np.linalg.norm(fn.An.cpu().numpy() - An0) <= 1e-6 * np.linalg.norm(An0)
+
+
 @pytest.mark.parametrize("name", ["pipe_n2", "pipe_n4"])
 def test_gpu_against_reference_golden(torch_cuda, name):
     """The HIP path against the reference's own outputs (fixtures written from the reference build)."""
@@ -266,19 +324,21 @@ def test_mixed_precision_solve(torch_cuda, phys, n):
 
 
 def test_cpp_driver_matches_python_driver(torch_cuda):
-    """The C++ host driver (svg_newton_step, the reference's Newton loop over
-    the C++ host API) and the Python driver give the same Newton records."""
+    """The C++ host driver (svg_newton_step, the reference's time loop over
+    the C++ host API) and the Python driver give the same Newton records over
+    two time steps."""
     import json
     import os
     import subprocess
     exe = os.path.join(os.path.dirname(_lib.LIB_PATH), "svg_newton_step")
-    r = subprocess.run([exe, "4", "1e-6", "50", "1000"], capture_output=True, text=True, timeout=120)
+    r = subprocess.run([exe, "4", "1e-6", "50", "1000", "2"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     recs = [json.loads(line) for line in r.stdout.splitlines()[1:]]
     from svfsiplus_amd.newton import FluidNewton, ls_create
     pipe = W.make_pipe(4)
     ls = ls_create(_lib.LS_GMRES, relTol=1e-6, absTol=1e-30, maxItr=1000, dimKry=50)
-    hist = FluidNewton(pipe, W.Physics(), ls).time_step()
+    hist = [h for step in FluidNewton(pipe, W.Physics(), ls).time_steps(2) for h in step]
+    assert [x["step"] for x in recs] == [1, 1, 1, 2, 2, 2]
     print("C++ driver", [(x["itr"], x["ls_itr"], x["iNorm"]) for x in recs])
     print("Python driver", [(x["itr"], x["ls_itr"], x["iNorm"]) for x in hist])
     assert len(recs) == len(hist)

@@ -256,3 +256,35 @@ def test_pici_picc_closed_form():
     # gen-alpha coefficients for rho_inf = 0.5 (initialize.cpp:430-463)
     assert abs(ph.am - 0.5 * 2.5 / 1.5) < 1e-15 and abs(ph.af - 1 / 1.5) < 1e-15
     assert abs(ph.gam - (0.5 + ph.am - ph.af)) < 1e-15
+
+
+def test_picp_set_bc_dir_closed_forms():
+    """picp: An = Ao (gam-1)/gam, Yn = Yo. set_bc_dir from Ao = Yo = 0 gives the
+    workload's closed-form first-step state; with a direction mask only the
+    masked rows are written, in mask order, and dirA/dirY scale A/Y."""
+    ph = W.Physics()
+    rng = np.random.default_rng(12)
+    Ao, Yo = rng.standard_normal((7, 4)), rng.standard_normal((7, 4))
+    An, Yn = oracle.picp(ph, Ao, Yo)
+    assert np.array_equal(An, Ao * ((ph.gam - 1.0) / ph.gam)) and np.array_equal(Yn, Yo)
+    # rho_inf = 0.5: am = 5/6, af = 2/3, gam = 2/3, so (gam-1)/gam = -1/2
+    assert abs((ph.gam - 1.0) / ph.gam + 0.5) < 1e-15
+    pipe = W.make_pipe(4)
+    z = np.zeros((pipe.nNo, 4))
+    An, Yn = oracle.picp(ph, z, z)
+    for gN, gxn in W.dirichlet_faces(pipe, ph):
+        oracle.set_bc_dir(gN, gxn, An, Yn)
+    Ao0, Yo0, An0, Yn0 = W.dirichlet_state(pipe, ph)
+    assert np.array_equal(An, An0) and np.array_equal(Yn, Yn0)
+    # the inlet's velocity is the parabola, not zero: the wall (written after) keeps its rim
+    inlet, wall = pipe.faces[0][1], pipe.faces[1][1]
+    core = np.setdiff1d(inlet, wall)
+    assert core.size and np.all(Yn[core, 2] > 0.0)
+    # direction mask 0b101 (x and z) with lDof = 2, s = 0, dirA = 2, dirY = -3
+    A = np.full((3, 4), 9.0)
+    Y = np.full((3, 4), 9.0)
+    gxn = np.array([[1.0, 2.0], [3.0, 4.0]])
+    oracle.set_bc_dir(np.array([2, 0]), gxn, A, Y, s=0, eDir=0b101, dirA=2.0, dirY=-3.0)
+    assert A[2].tolist() == [2.0, 9.0, 4.0, 9.0] and Y[2].tolist() == [-3.0, 9.0, -6.0, 9.0]
+    assert A[0].tolist() == [6.0, 9.0, 8.0, 9.0] and Y[0].tolist() == [-9.0, 9.0, -12.0, 9.0]
+    assert A[1].tolist() == [9.0] * 4 and Y[1].tolist() == [9.0] * 4
