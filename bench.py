@@ -47,6 +47,7 @@ KERNELS = ["spmv_bcsr", "spmv32", "d2f", "mdot", "mdot_fin", "gs_update", "div",
            "lpc_msg", "lpc_dense"]
 # kernels whose launches carry algorithmic byte counts (roofline candidates)
 HBM_KERNELS = ["spmv_bcsr", "spmv32", "lpc_msg", "lpc_dense", "lpc_encode"]
+ROOFLINE_KERNELS = ["spmv_bcsr", "spmv32"]  # timed with events inside the timed region
 DEFAULT_WEIGHTS = os.path.join(ROOT, "svfsiplus_amd", "data", "lpc_weights.npz")
 # HBM bytes per launch on the 1.2M-tet pipe (n = 34), measured with rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE on `bench.py --kernel-only spmv|spmv32`, each scaled by the same counter's reading on a
@@ -184,17 +185,35 @@ def kernel_only(a, fn, pipe, lib):
                       "copy_GBs": round(2 * src.numel() * 8 / copy_avg / 1e9, 1)}))
 
 
-def timed_run(fn, lib, steps, warmup, prof, label):
-    """Warmup, then `steps` time steps timed with a device synchronisation on
-    both sides; per-kernel event table and algorithmic bytes of the run."""
+def _bytes(lib, ktab):
     import ctypes as C
+    for k in HBM_KERNELS:
+        if k in ktab:
+            tb = C.c_double()
+            lib.svg_prof_get_bytes(k.encode(), C.byref(tb))
+            ktab[k]["bytes"] = tb.value
+    return ktab
+
+
+def timed_run(fn, lib, steps, warmup, prof, label):
+    """Warmup (every named kernel timed with events: the per-kernel table),
+    then `steps` time steps timed with a device synchronisation on both sides,
+    in which events bracket only the block SpMV kernels the roofline reports
+    on (events between launches cost time; prof=False: none at all)."""
     import torch
+    warm_tab = {}
     for w in range(warmup):
+        lib.svg_prof_reset()
+        lib.svg_prof_only(None)
+        lib.svg_prof_enable(1 if prof else 0)
         fn.reset()
         h = fn.time_step()
+        lib.svg_prof_enable(0)
+        warm_tab = _bytes(lib, kernel_table(lib))
         print(f"# {label} warmup {w}: " + " ".join(f"[itr {x['itr']} ls {x['ls_itr']} iNorm {x['iNorm']:.3e}]"
                                                  for x in h), file=sys.stderr, flush=True)
     lib.svg_prof_reset()
+    lib.svg_prof_only(",".join(ROOFLINE_KERNELS).encode())
     lib.svg_prof_enable(1 if prof else 0)
     hists = []
     torch.cuda.synchronize()
@@ -205,12 +224,8 @@ def timed_run(fn, lib, steps, warmup, prof, label):
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     lib.svg_prof_enable(0)
-    ktab = kernel_table(lib)
-    for k in HBM_KERNELS:
-        if k in ktab:
-            tb = C.c_double()
-            lib.svg_prof_get_bytes(k.encode(), C.byref(tb))
-            ktab[k]["bytes"] = tb.value
+    lib.svg_prof_only(None)
+    ktab = _bytes(lib, kernel_table(lib))
     for k, h in enumerate(hists):
         print(f"# {label} step {k}: " + " ".join(
             f"[itr {x['itr']} ls {x['ls_itr']} iNorm {x['iNorm']:.3e} fNorm {x['fNorm']:.3e}"
@@ -218,7 +233,7 @@ def timed_run(fn, lib, steps, warmup, prof, label):
             file=sys.stderr, flush=True)
     n_newton = sum(len(h) for h in hists)
     return {"elapsed": t1 - t0, "s_per_newton": (t1 - t0) / n_newton, "n_newton": n_newton, "hists": hists,
-            "kernels": ktab, "Yn": fn.Yn.clone(), "An": fn.An.clone()}
+            "kernels": ktab, "kernels_warmup": warm_tab, "Yn": fn.Yn.clone(), "An": fn.An.clone()}
 
 
 def roofline(ktab, measured=None):
@@ -259,8 +274,9 @@ def get_network(a, log):
 def batch_run(a, log):
     """The batched workload: independent bifurcating-pipe meshes (tet counts
     log-uniform in 5k-80k, daughter angles 25-45 degrees, Murray radii) solved
-    together: one union assembly and the batched mixed-precision GMRES with a
-    hipGraph-captured restart cycle. Aggregate Newton steps per second."""
+    together: one union assembly and the batched mixed-precision GMRES (by
+    default with the learned preconditioner, per-mesh fallback), each Newton
+    iteration as three replayed hipGraphs. Aggregate Newton steps per second."""
     import torch
     from svfsiplus_amd import _lib, batch as B, workload as W
     t0 = time.perf_counter()
@@ -326,7 +342,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--batch", type=int, default=256, help="meshes of the batched workload (0: skip it)")
     ap.add_argument("--batch-precision", choices=["fp64", "mixed"], default="mixed")
-    ap.add_argument("--batch-method", choices=["plain", "learned"], default="plain",
+    ap.add_argument("--batch-method", choices=["plain", "learned"], default="learned",
                     help="batched solve: diagonal preconditioner, or the learned one (flexible GMRES)")
     ap.add_argument("--no-prof", action="store_true", help="time without per-kernel events")
     ap.add_argument("--kernel-only", choices=["spmv", "spmv32"], default=None,
@@ -417,7 +433,9 @@ def main():
                    "method": head, "solver": solver[head], "newton_records": R["hists"][-1]},
         "roofline": roofline(R["kernels"], traffic),
         "cpu_baseline": cpu,
-        "kernels": R["kernels"],
+        "kernels": R["kernels_warmup"] or R["kernels"],
+        "kernels_note": "per-kernel event table of the untimed warmup step; the timed steps bracket only "
+                        + ", ".join(ROOFLINE_KERNELS) + " with events",
     }
     if "plain" in runs and head != "plain":
         P = runs["plain"]

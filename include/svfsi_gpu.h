@@ -348,6 +348,10 @@ int svg_fill(double* d, size_t n, double v, void* stream);
 /* When enabled, every launch of the named kernels is bracketed by events;   */
 /* svg_prof_get returns the launch count and summed duration (ms).           */
 int svg_prof_enable(int on);
+/* Restricts the event timing to the kernels named in the comma-separated
+ * list (NULL or "": every named kernel). Events cost time between launches,
+ * so a timed run brackets only the kernels it reports on. */
+int svg_prof_only(const char* names);
 int svg_prof_reset(void);
 int svg_prof_get(const char* name, int* count, double* total_ms);
 /* Summed algorithmic bytes of the named kernel's timed launches (where the

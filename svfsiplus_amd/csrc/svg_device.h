@@ -151,8 +151,10 @@ void krylov_free(KrylovWork& k);   // every buffer, pinned mirror and event of a
 int lhs_ensure_trans(svg_lhs* L);  // block-transpose map (pattern must be symmetric)
 int spmv_residual(const svg_lhs* L, const double* K, const double* U, const double* B, double* out, hipStream_t s);
 int lpc_prepare_dev(svg_lhs* L, const double* Val, hipStream_t s);
+// nsc: per-node (sc, 1/sc); sysOf/st: skip the rows of batched systems that stopped iterating.
 int lpc_apply_dev(svg_lpc* N, const svg_lhs* L, const double* V, double* Z, float* Z32, const int* skip,
-                  hipStream_t s, const double2* nsc = nullptr);
+                  hipStream_t s, const double2* nsc = nullptr, const int* sysOf = nullptr,
+                  const GmState* st = nullptr);
 // Every buffer lpc_prepare_dev / lpc_apply_dev use on L, allocated ahead (graph capture).
 int lpc_reserve(svg_lpc* N, svg_lhs* L);
 // construct_fluid split in two: the launches alone (no host synchronisation, so

@@ -1987,7 +1987,7 @@ static int batch_cycle(svg_batch* B, const double* Val, const double* R, const B
       double* zi = B->z + B->ld * i;
       float* z32 = mixed ? B->u32 : (float*)nullptr;
       if (md.period <= 1 || i % md.period == 0) {
-        int rc = lpc_apply_dev(md.net, L, ui, zi, z32, nullptr, s, B->d_nsc);
+        int rc = lpc_apply_dev(md.net, L, ui, zi, z32, nullptr, s, B->d_nsc, B->d_sysOfNode, B->st);
         if (rc) return rc;
         hipLaunchKernelGGL(k_bzcopy, dim3(nbv), dim3(256), 0, s, n, (const double*)ui, zi, z32,
                            (const GmState*)B->st, (const int*)B->d_sysOfNode, 1);

@@ -143,10 +143,36 @@ __global__ __launch_bounds__(256) void k_diag_bf16(int nNo, const int* __restric
 }
 
 // -------------------------------------------------------------- forward ---
+// Batched solves: rows of systems that stopped iterating (converged before the
+// cycle, or done in it) are skipped; sysOf null: every row is live.
+struct RowGate {
+  const int* sysOf;
+  const GmState* st;
+  __device__ __forceinline__ bool live(int node) const
+  {
+    const GmState& S = st[sysOf[node]];
+    return S.act && !S.done;
+  }
+};
+
+// Whether a row of the 64-row tile at r0 is live (the same answer in every
+// thread of the workgroup).
+__device__ __forceinline__ bool tile_live(const RowGate& g, int r0, int rows, int n, int t)
+{
+  __shared__ int any;
+  if (t == 0) any = 0;
+  __syncthreads();
+  if (t < TILE && r0 + t < rows && g.live((r0 + t) % n)) any = 1;
+  __syncthreads();
+  const bool lv = any != 0;
+  __syncthreads();
+  return lv;
+}
+
 __global__ __launch_bounds__(BS) void k_encode(int n, int rows, const double* __restrict__ V, double sc,
                                                const double2* __restrict__ nsc, const bf16* __restrict__ Dg,
                                                const bf16* __restrict__ W0, bf16* __restrict__ H,
-                                               const int* __restrict__ skip)
+                                               const int* __restrict__ skip, RowGate gate)
 {
   __shared__ bf16 U[TILE * US];
   __shared__ bf16 Ws[F * US];
@@ -156,6 +182,7 @@ __global__ __launch_bounds__(BS) void k_encode(int n, int rows, const double* __
   for (int q = t; q < F * NIN; q += BS) Ws[(q / NIN) * US + q % NIN] = W0[q];  // once per workgroup
   for (int tl = blockIdx.x; tl < ntiles; tl += gridDim.x) {
     const int r0 = tl * TILE;
+    if (gate.sysOf && !tile_live(gate, r0, rows, n, t)) continue;
     load_u(U, t, r0, rows, n, V, sc, nsc, Dg);
     __syncthreads();
     v4f acc[4];
@@ -184,13 +211,14 @@ template <int UNR>
 __global__ __launch_bounds__(256) void k_msg(int n, int rows, const int* __restrict__ rowPtr,
                                              const int* __restrict__ colPtr, const bf16* __restrict__ A16,
                                              const bf16* __restrict__ H, bf16* __restrict__ M,
-                                             const int* __restrict__ skip)
+                                             const int* __restrict__ skip, RowGate gate)
 {
   if (skip && *skip) return;
   const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
   const int r = (int)(idx >> 2), g = (int)(idx & 3);
   if (r >= rows) return;
   const int b = r / n, i = r - b * n;
+  if (gate.sysOf && !gate.live(i)) return;
   const bf16* Hb = H + (size_t)b * n * F + 16 * g;
   const int s = rowPtr[i], e = rowPtr[i + 1];
   float m[4][4];
@@ -268,7 +296,8 @@ __global__ __launch_bounds__(BS) void k_dense(int rows, const bf16* __restrict__
                                               const float* __restrict__ bias, bf16* __restrict__ Pst,
                                               const bf16* __restrict__ Wd, const double* __restrict__ V,
                                               double* __restrict__ Z, float* __restrict__ Z32, double inv_sc,
-                                              const double2* __restrict__ nsc, const int* __restrict__ skip)
+                                              const double2* __restrict__ nsc, const int* __restrict__ skip,
+                                              RowGate gate)
 {
   __shared__ __align__(16) bf16 X[TILE * XS];
   __shared__ __align__(16) bf16 Ws[F * XS];
@@ -289,6 +318,7 @@ __global__ __launch_bounds__(BS) void k_dense(int rows, const bf16* __restrict__
   }
   for (int tl = blockIdx.x; tl < ntiles; tl += gridDim.x) {
     const int r0 = tl * TILE;
+    if (gate.sysOf && !tile_live(gate, r0, rows, rows, t)) continue;
     for (int q = t; q < TILE * F / 8; q += BS) {
       const int row = q >> 3, f8 = (q & 7) * 8, r = r0 + row;
       uint4 hv = {0u, 0u, 0u, 0u}, mv = {0u, 0u, 0u, 0u};
@@ -683,16 +713,18 @@ int lpc_prepare_dev(svg_lhs* L, const double* Val, hipStream_t s)
 // m and p of every layer. nsc (nvec = 1 only): per-node (sc, 1/sc) in place of
 // sqrt(nNo) (independent systems of a block-diagonal union, each scaled by its own size).
 static int lpc_forward(svg_lpc* N, const svg_lhs* L, int nvec, const double* V, double* Z, float* Z32,
-                       bool train, const int* skip, hipStream_t s, const double2* nsc = nullptr)
+                       bool train, const int* skip, hipStream_t s, const double2* nsc = nullptr,
+                       const int* sysOf = nullptr, const GmState* st = nullptr)
 {
   const int n = L->nNo, rows = nvec * n;
-  if (nsc && nvec != 1) return set_error(SVG_ERR_ARG, "lpc_forward: per-node scales need one vector");
+  if ((nsc || sysOf) && nvec != 1) return set_error(SVG_ERR_ARG, "lpc_forward: per-node data need one vector");
+  const lpc::RowGate gate{sysOf, st};
   const double sc = std::sqrt((double)n);
   const unsigned nt = blocks_for(rows, TILE);
   const bf16* pb = N->pb;
   // algorithmic bytes: V and the diagonal blocks read, h written
   launch_b("lpc_encode", (32.0 + 32.0 + 2.0 * F) * rows, k_encode, dim3(std::min(nt, 2048u)), dim3(BS), 0, s, n,
-           rows, V, sc, nsc, (const bf16*)L->d_diag16, pb + OFF_W0, N->H[0], skip);
+           rows, V, sc, nsc, (const bf16*)L->d_diag16, pb + OFF_W0, N->H[0], skip, gate);
   const unsigned nm = blocks_for((size_t)rows * 4, 256);
   for (int l = 0; l < NL; l++) {
     const bool last = l == NL - 1;
@@ -701,24 +733,24 @@ static int lpc_forward(svg_lpc* N, const svg_lhs* L, int nvec, const double* V, 
     bf16* Ml = train ? N->M[l] : N->M[0];
     // BF16 blocks + colPtr per block (per sample), rowPtr + h + m out per row
     launch_b("lpc_msg", (32.0 + 4.0) * L->nnz * nvec + (4.0 + 2.0 * F) * rows, k_msg<2>, dim3(nm), dim3(256), 0, s, n,
-             rows, (const int*)L->d_rowPtr, (const int*)L->d_colPtr, (const bf16*)L->d_val16, Hin, Ml, skip);
+             rows, (const int*)L->d_rowPtr, (const int*)L->d_colPtr, (const bf16*)L->d_val16, Hin, Ml, skip, gate);
     // h, m in, h out (+ p for training; + V in, z out for the last layer)
     const double db = 6.0 * F * rows + (train ? 2.0 * F * rows : 0.0) + (last ? (64.0 + (Z32 ? 16.0 : 0.0)) * rows : 0.0);
     launch_b("lpc_dense", db, k_dense, dim3(std::min(nt, 2048u)), dim3(BS), 0, s, rows, Hin, (const bf16*)Ml, Hout,
              pb + OFF_L + l * LSZ, (const float*)(N->p + OFF_L + l * LSZ + F * 2 * F),
              train ? N->P[l] : (bf16*)nullptr, last ? pb + OFF_WD : (const bf16*)nullptr, V,
-             last ? Z : (double*)nullptr, last ? Z32 : (float*)nullptr, 1.0 / sc, nsc, skip);
+             last ? Z : (double*)nullptr, last ? Z32 : (float*)nullptr, 1.0 / sc, nsc, skip, gate);
   }
   SVG_HIP(hipGetLastError());
   return SVG_OK;
 }
 
 int lpc_apply_dev(svg_lpc* N, const svg_lhs* L, const double* V, double* Z, float* Z32, const int* skip,
-                  hipStream_t s, const double2* nsc)
+                  hipStream_t s, const double2* nsc, const int* sysOf, const GmState* st)
 {
   int rc = lpc_ensure(N, L->nNo, false);
   if (rc) return rc;
-  return lpc_forward(N, L, 1, V, Z, Z32, false, skip, s, nsc);
+  return lpc_forward(N, L, 1, V, Z, Z32, false, skip, s, nsc, sysOf, st);
 }
 
 int lpc_reserve(svg_lpc* N, svg_lhs* L)

@@ -36,6 +36,7 @@ struct ProfPending {
 };
 static std::mutex g_prof_mu;
 static bool g_prof_on = false;
+static std::vector<std::string> g_prof_only;  // empty: every named launch is timed
 
 bool prof_set(bool on)
 {
@@ -63,6 +64,9 @@ int prof_begin(const char* name, hipStream_t s, double bytes)
 {
   if (!g_prof_on) return -1;
   std::lock_guard<std::mutex> lk(g_prof_mu);
+  if (!g_prof_only.empty() &&
+      std::find(g_prof_only.begin(), g_prof_only.end(), std::string(name)) == g_prof_only.end())
+    return -1;
   ProfPending p;
   p.name = name;
   p.bytes = bytes;
@@ -179,6 +183,24 @@ const char* svg_last_error(void) { return svg::g_err.c_str(); }
 int svg_prof_enable(int on)
 {
   g_prof_on = (on != 0);
+  return SVG_OK;
+}
+
+int svg_prof_only(const char* names)
+{
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  g_prof_only.clear();
+  if (!names) return SVG_OK;
+  std::string cur;
+  for (const char* c = names;; c++) {
+    if (*c == ',' || *c == 0) {
+      if (!cur.empty()) g_prof_only.push_back(cur);
+      cur.clear();
+      if (*c == 0) break;
+    } else {
+      cur += *c;
+    }
+  }
   return SVG_OK;
 }
 
