@@ -180,7 +180,11 @@ int svg_ls_create(svg_ls* ls, int LS_type);
 /* Ri(dof,nNo): residual in, Newton increment out. Val(dof*dof,nnz) is scaled
  * in place by the diagonal preconditioner, as in the reference. dof = 4.
  * incL(nIncL)/res(nIncL) have the reference meaning (res may be NULL when no
- * Neumann face is registered). Synchronous: returns when Ri holds the result
+ * Neumann face is registered). A Neumann face with res != 0 is coupled
+ * (solve.cpp:96-111): with LS type GMRES every block product adds
+ * res valM (valM^T x) (add_bc_mul.cpp:53-121, valM = val W from
+ * precond_diag, precond.cpp:246-256); the NS solver with a coupled face
+ * returns SVG_ERR_UNSUPPORTED. Synchronous: returns when Ri holds the result
  * and ls holds the iteration record. */
 int svg_fsils_solve(svg_lhs* lhs, svg_ls* ls, int dof, double* d_Ri, double* d_Val, int prec, int nIncL,
                     const int* incL, const double* res, void* stream);

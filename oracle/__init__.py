@@ -43,7 +43,17 @@ class Ls(C.Structure):
 
 class Face(C.Structure):
     _fields_ = [("nNo", C.c_int), ("dof", C.c_int), ("bGrp", C.c_int), ("incFlag", C.c_int),
-                ("coupledFlag", C.c_int), ("glob", C.c_void_p), ("val", C.c_void_p)]
+                ("coupledFlag", C.c_int), ("glob", C.c_void_p), ("val", C.c_void_p), ("res", C.c_double),
+                ("valM", C.c_void_p)]
+
+
+def face_spec(f):
+    """A face as (node ids, bGrp, val(nNo, 3)): (name, glob) is a Dirichlet face
+    with val = 0; (name, glob, "neu", val) a Neumann face."""
+    glob = np.ascontiguousarray(f[1], dtype=np.int32)
+    if len(f) > 2 and f[2] == "neu":
+        return glob, 1, np.ascontiguousarray(f[3], dtype=np.float64).reshape(len(glob), 3)
+    return glob, 0, np.zeros((len(glob), 3))
 
 
 class Lhs(C.Structure):
@@ -183,11 +193,13 @@ class OracleLhs:
         self.colPtr = np.ascontiguousarray(pipe.colPtr, dtype=np.int32)
         self.diagPtr = np.ascontiguousarray(pipe.diagPtr())
         faces = pipe.faces if faces is None else faces
-        self._glob = [np.ascontiguousarray(f[1], dtype=np.int32) for f in faces]
-        self._val = [np.zeros((len(f[1]), 3)) for f in faces]
+        specs = [face_spec(f) for f in faces]
+        self._glob = [g for g, _, _ in specs]
+        self._val = [v for _, _, v in specs]
+        self._valM = [np.zeros_like(v) for v in self._val]
         self.faces = (Face * max(1, len(faces)))()
-        for i in range(len(faces)):
-            self.faces[i] = Face(len(self._glob[i]), 3, 0, 1, 0, _p(self._glob[i]), _p(self._val[i]))
+        for i, (g, bg, _) in enumerate(specs):
+            self.faces[i] = Face(len(g), 3, bg, 1, 0, _p(g), _p(self._val[i]), 0.0, _p(self._valM[i]))
         self.lhs = Lhs(pipe.nNo, pipe.nnz, _p(self.rowPtr2), _p(self.colPtr), _p(self.diagPtr), len(faces),
                        C.cast(self.faces, C.POINTER(Face)))
 
@@ -203,15 +215,16 @@ def ls_create(ls_type, relTol=-1.0, absTol=-1.0, maxItr=-1, dimKry=-1, relTolIn=
     return ls
 
 
-def fsils_solve(pipe, ls, R, Val, prec=PREC_FSILS, faces=None):
-    """Oracle fsils_solve on copies; returns (increment (nNo,4), scaled Val, ls)."""
+def fsils_solve(pipe, ls, R, Val, prec=PREC_FSILS, faces=None, res=None):
+    """Oracle fsils_solve on copies; returns (increment (nNo,4), scaled Val, ls).
+    res: per-face resistance (a Neumann face with res != 0 is coupled)."""
     lib = load()
     L = OracleLhs(pipe, faces)
     Ri = np.ascontiguousarray(R, dtype=np.float64).copy()
     V = np.ascontiguousarray(Val, dtype=np.float64).copy()
     nf = L.lhs.nFaces
     incL = np.ones(max(1, nf), dtype=np.int32)
-    res = np.zeros(max(1, nf))
+    res = np.zeros(max(1, nf)) if res is None else np.ascontiguousarray(res, dtype=np.float64)
     rc = lib.orc_fsils_solve(C.byref(L.lhs), C.byref(ls), C.c_int(4), C.c_void_p(_p(Ri)), C.c_void_p(_p(V)),
                              C.c_int(prec), C.c_int(nf), C.c_void_p(_p(incL)), C.c_void_p(_p(res)))
     if rc:
@@ -219,22 +232,27 @@ def fsils_solve(pipe, ls, R, Val, prec=PREC_FSILS, faces=None):
     return Ri, V, ls
 
 
-def ref_fsils_solve(pipe, ls_type, ls_prm, R, Val, prec=PREC_FSILS, faces=None):
+def ref_fsils_solve(pipe, ls_type, ls_prm, R, Val, prec=PREC_FSILS, faces=None, res=None):
     """The reference's own fsils_solve; ls_prm = (RI.relTol, RI.absTol, RI.mItr, RI.sD, GM.relTol, GM.absTol,
-    GM.mItr, GM.sD, CG.relTol, CG.absTol, CG.mItr). Returns (increment, scaled Val, out dict)."""
+    GM.mItr, GM.sD, CG.relTol, CG.absTol, CG.mItr); faces as in face_spec, res per face.
+    Returns (increment, scaled Val, out dict)."""
     lib = load_ref()
     faces = pipe.faces if faces is None else faces
-    nn = np.array([len(f[1]) for f in faces], dtype=np.int32)
-    glob = np.ascontiguousarray(np.concatenate([f[1] for f in faces]), dtype=np.int32)
-    val = np.zeros((glob.shape[0], 3))
+    specs = [face_spec(f) for f in faces]
+    nn = np.array([len(g) for g, _, _ in specs], dtype=np.int32)
+    glob = np.ascontiguousarray(np.concatenate([g for g, _, _ in specs]), dtype=np.int32)
+    val = np.ascontiguousarray(np.concatenate([v for _, _, v in specs]))
+    bgrp = np.array([bg for _, bg, _ in specs], dtype=np.int32)
+    rs = np.zeros(len(faces)) if res is None else np.ascontiguousarray(res, dtype=np.float64)
     Ri = np.ascontiguousarray(R, dtype=np.float64).copy()
     V = np.ascontiguousarray(Val, dtype=np.float64).copy()
     prm = np.array(ls_prm, dtype=np.float64)
     out = np.zeros(9)
-    rc = lib.ref_fsils_solve(C.c_int(pipe.nNo), C.c_int(pipe.nnz), C.c_void_p(_p(pipe.rowPtr)),
-                             C.c_void_p(_p(pipe.colPtr)), C.c_int(len(faces)), C.c_void_p(_p(nn)),
-                             C.c_void_p(_p(glob)), C.c_void_p(_p(val)), C.c_int(ls_type), C.c_void_p(_p(prm)),
-                             C.c_int(4), C.c_void_p(_p(Ri)), C.c_void_p(_p(V)), C.c_int(prec), C.c_void_p(_p(out)))
+    rc = lib.ref_fsils_solve_bc(C.c_int(pipe.nNo), C.c_int(pipe.nnz), C.c_void_p(_p(pipe.rowPtr)),
+                                C.c_void_p(_p(pipe.colPtr)), C.c_int(len(faces)), C.c_void_p(_p(nn)),
+                                C.c_void_p(_p(glob)), C.c_void_p(_p(val)), C.c_void_p(_p(bgrp)), C.c_void_p(_p(rs)),
+                                C.c_int(ls_type), C.c_void_p(_p(prm)), C.c_int(4), C.c_void_p(_p(Ri)),
+                                C.c_void_p(_p(V)), C.c_int(prec), C.c_void_p(_p(out)))
     if rc:
         raise RuntimeError(lib.ref_last_error().decode())
     keys = ["itr", "iNorm", "fNorm", "suc", "dB", "GM_itr", "CG_itr", "Resm", "Resc"]

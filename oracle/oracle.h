@@ -75,6 +75,8 @@ typedef struct {
   int incFlag, coupledFlag;
   const int* glob;     /* nNo node ids */
   const double* val;   /* (dof, nNo) */
+  double res;          /* resistance of a coupled Neumann face (set by fsils_solve) */
+  double* valM;        /* (dof, nNo): val scaled by the preconditioner (caller-owned, zero at creation) */
 } orc_face;
 
 typedef struct {
@@ -116,6 +118,8 @@ int orc_fluid_element(const double xl[3][4], const double al[4][4], const double
 
 /* ---- FSILS pieces ---- */
 void orc_spar_mul_vv(const orc_lhs* lhs, int dof, const double* K, const double* U, double* KU);
+/* add_bc_mul (add_bc_mul.cpp:53-121), op ADD: Y += res valM (valM^T X) for each coupled face. */
+void orc_add_bc_mul(const orc_lhs* lhs, int dof, const double* X, double* Y);
 void orc_spar_mul_sv(const orc_lhs* lhs, int dof, const double* K, const double* U, double* KU);
 void orc_spar_mul_vs(const orc_lhs* lhs, int dof, const double* K, const double* U, double* KU);
 void orc_spar_mul_ss(const orc_lhs* lhs, const double* K, const double* U, double* KU);

@@ -16,10 +16,14 @@
  *   fsils_ls_create                liner_solver/ls.cpp:40-120
  *   fsils_solve                    liner_solver/solve.cpp:50-193
  *
- * Coupled (resistance) faces are not modelled: add_bc_mul and the face.nS
- * terms are identities when no face has coupledFlag set, which is the case
- * for the Dirichlet + zero-traction pipe configurations. A coupled face makes
- * orc_fsils_solve return an error instead of computing something different.
+ *   add_bc_mul (op ADD)            liner_solver/add_bc_mul.cpp:53-121
+ *
+ * Coupled (resistance) Neumann faces: fsils_solve sets coupledFlag and res
+ * (solve.cpp:96-111), precond_diag sets valM = val W (precond.cpp:246-256) and
+ * gmres_v adds res valM (valM^T x) after every block product (gmres.cpp:510,
+ * 539; its BCOP_TYPE_PRE branches never run: `flag` stays false). The NS
+ * solver's coupled-face terms (ns_solver.cpp:334, cgrad.cpp:104-106) are not
+ * restated: orc_fsils_solve returns an error for LS type NS with a coupled face.
  */
 #include "oracle.h"
 
@@ -229,6 +233,49 @@ static void omp_mul(int n, double r, double* U)
   for (int i = 0; i < n; i++) U[i] = r * U[i];
 }
 
+/* S = valM^T X over a coupled face, entries q = a nsd + i. Reference order:
+ * serial in q (add_bc_mul.cpp:100-106). Device order: one 256-lane
+ * workgroup, lane t sums q = t, t+256, ... in order, a butterfly per
+ * 64-lane wavefront, the 4 wavefront sums added in order. */
+static double face_dot(const orc_face* face, int nsd, int dof, const double* X)
+{
+  long nq = (long)face->nNo * nsd;
+  if (g_dot_order == 0) {
+    double S = 0.0;
+    for (int a = 0; a < face->nNo; a++) {
+      int Ac = face->glob[a];
+      for (int i = 0; i < nsd; i++) S = S + face->valM[i + face->dof * a] * X[i + dof * Ac];
+    }
+    return S;
+  }
+  double acc[256];
+  for (int t = 0; t < 256; t++) {
+    double s = 0.0;
+    for (long q = t; q < nq; q += 256) {
+      int a = (int)(q / nsd), i = (int)(q % nsd);
+      s = s + face->valM[i + face->dof * a] * X[i + dof * face->glob[a]];
+    }
+    acc[t] = s;
+  }
+  double S = butterfly64(acc);
+  for (int w = 1; w < 4; w++) S = S + butterfly64(acc + 64 * w);
+  return S;
+}
+
+void orc_add_bc_mul(const orc_lhs* lhs, int dof, const double* X, double* Y)
+{
+  for (int f = 0; f < lhs->nFaces; f++) {
+    const orc_face* face = &lhs->face[f];
+    if (!face->coupledFlag) continue;
+    int nsd = face->dof < dof ? face->dof : dof;
+    double S = face->res * face_dot(face, nsd, dof, X);
+    for (int a = 0; a < face->nNo; a++) {
+      int Ac = face->glob[a];
+      for (int i = 0; i < nsd; i++) Y[i + dof * Ac] = Y[i + dof * Ac] + face->valM[i + face->dof * a] * S;
+    }
+  }
+}
+
 void orc_precond_diag(const orc_lhs* lhs, int dof, double* Val, double* R, double* W)
 {
   int nNo = lhs->nNo;
@@ -264,6 +311,16 @@ void orc_precond_diag(const orc_lhs* lhs, int dof, double* Val, double* R, doubl
       for (int r = 0; r < dof; r++)
         for (int i = 0; i < dof; i++) Val[r * dof + i + d2 * j] = Val[r * dof + i + d2 * j] * W[i + dof * a];
     }
+  /* coupled faces: valM = val W (precond.cpp:246-256) */
+  for (int f = 0; f < lhs->nFaces; f++) {
+    const orc_face* face = &lhs->face[f];
+    if (!face->coupledFlag || !face->valM) continue;
+    int n = face->dof < dof ? face->dof : dof;
+    for (int a = 0; a < face->nNo; a++) {
+      int Ac = face->glob[a];
+      for (int i = 0; i < n; i++) face->valM[i + face->dof * a] = face->val[i + face->dof * a] * W[i + dof * Ac];
+    }
+  }
 }
 
 int orc_ge(int N, int ldA, const double* A, double* B)
@@ -350,6 +407,7 @@ static int arnoldi_cycle(const orc_lhs* lhs, orc_subls* ls, int dof, const doubl
     ls->itr = ls->itr + 1;
     last_i = i;
     orc_spar_mul_vv(lhs, dof, Val, u + n * i, u + n * (i + 1));
+    orc_add_bc_mul(lhs, dof, u + n * i, u + n * (i + 1));
     for (int j = 0; j <= i + 1; j++) H(j, i) = orc_dot_v(dof, nNo, u + n * j, u + n * (i + 1));
     for (int j = 0; j <= i; j++) {
       omp_sum((int)n, -H(j, i), u + n * (i + 1), u + n * j);
@@ -418,6 +476,7 @@ void orc_gmres_v(const orc_lhs* lhs, orc_subls* ls, int dof, const double* Val, 
     ls->dB = ls->fNorm;
     ls->itr = ls->itr + 1;
     orc_spar_mul_vv(lhs, dof, Val, X, u);
+    orc_add_bc_mul(lhs, dof, X, u);
     for (size_t k = 0; k < n; k++) u[k] = R[k] - u[k];
     err[0] = orc_norm_v(dof, nNo, u);
     for (size_t k = 0; k < n; k++) u[k] = u[k] / err[0];
@@ -748,8 +807,12 @@ int orc_fsils_solve(orc_lhs* lhs, orc_ls* ls, int dof, double* Ri, double* Val, 
       face->coupledFlag = 0;
       if (!face->incFlag) continue;
       if (face->bGrp == 1 && res[f] != 0.0) {
-        snprintf(g_err2, sizeof g_err2, "coupled Neumann faces are not supported by the oracle");
-        return 8;
+        face->res = res[f];
+        face->coupledFlag = 1;
+        if (ls->LS_type == ORC_LS_NS) {
+          snprintf(g_err2, sizeof g_err2, "coupled Neumann faces with the NS solver are not restated by the oracle");
+          return 8;
+        }
       }
     }
   }

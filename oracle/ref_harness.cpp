@@ -106,9 +106,13 @@ extern "C" int ref_construct_fluid(int nEl, const int* IEN, int nNo, const doubl
 // ls_prm: RI.relTol, RI.absTol, RI.mItr, RI.sD, GM.relTol, GM.absTol, GM.mItr, GM.sD,
 //         CG.relTol, CG.absTol, CG.mItr
 // out:    RI.itr, RI.iNorm, RI.fNorm, RI.suc, RI.dB, GM.itr, CG.itr, Resm, Resc
-extern "C" int ref_fsils_solve(int nNo, int nnz, const int* rowPtr, const int* colPtr, int nFaces,
-                               const int* faceNNo, const int* faceGlob, const double* faceVal, int LS_type,
-                               const double* ls_prm, int dof, double* Ri, double* Val, int prec, double* out)
+// faceBGrp (null: every face Dirichlet) gives each face's group (0 Dir, 1 Neu)
+// and res (null: zeros) the resistance passed to fsils_solve; faces are
+// registered with dof 3 and val(3, nNo) = faceVal.
+extern "C" int ref_fsils_solve_bc(int nNo, int nnz, const int* rowPtr, const int* colPtr, int nFaces,
+                                  const int* faceNNo, const int* faceGlob, const double* faceVal,
+                                  const int* faceBGrp, const double* faceRes, int LS_type, const double* ls_prm,
+                                  int dof, double* Ri, double* Val, int prec, double* out)
 {
   using namespace fsi_linear_solver;
   try {
@@ -134,7 +138,8 @@ extern "C" int ref_fsils_solve(int nNo, int nnz, const int* rowPtr, const int* c
         g(a) = faceGlob[off + a];
         for (int i = 0; i < 3; i++) v(i, a) = faceVal[3 * (off + a) + i];
       }
-      fsils_bc_create(lhs, f, n, 3, BcType::BC_TYPE_Dir, g, v);
+      const BcType bt = (faceBGrp && faceBGrp[f] == 1) ? BcType::BC_TYPE_Neu : BcType::BC_TYPE_Dir;
+      fsils_bc_create(lhs, f, n, 3, bt, g, v);
       off += n;
     }
     FSILS_lsType ls;
@@ -156,7 +161,10 @@ extern "C" int ref_fsils_solve(int nNo, int nnz, const int* rowPtr, const int* c
     std::memcpy(aVal.data(), Val, sizeof(double) * dof * dof * (size_t)nnz);
     Vector<int> incL(nFaces);
     Vector<double> res(nFaces);
-    for (int f = 0; f < nFaces; f++) incL(f) = 1;
+    for (int f = 0; f < nFaces; f++) {
+      incL(f) = 1;
+      res(f) = faceRes ? faceRes[f] : 0.0;
+    }
     fsils_solve(lhs, ls, dof, aRi, aVal,
                 prec == 701 ? consts::PreconditionerType::PREC_FSILS : consts::PreconditionerType::PREC_NONE,
                 incL, res);
@@ -176,4 +184,12 @@ extern "C" int ref_fsils_solve(int nNo, int nnz, const int* rowPtr, const int* c
     return 1;
   }
   return 0;
+}
+
+extern "C" int ref_fsils_solve(int nNo, int nnz, const int* rowPtr, const int* colPtr, int nFaces,
+                               const int* faceNNo, const int* faceGlob, const double* faceVal, int LS_type,
+                               const double* ls_prm, int dof, double* Ri, double* Val, int prec, double* out)
+{
+  return ref_fsils_solve_bc(nNo, nnz, rowPtr, colPtr, nFaces, faceNNo, faceGlob, faceVal, nullptr, nullptr, LS_type,
+                            ls_prm, dof, Ri, Val, prec, out);
 }

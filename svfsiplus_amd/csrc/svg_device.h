@@ -172,6 +172,10 @@ struct svg_face_rec {
   double res = 0.0;
   std::vector<int> glob;
   std::vector<double> val;
+  // Neumann faces: device copies for add_bc_mul; valM = val W after precond_diag
+  int* d_glob = nullptr;
+  double* d_val = nullptr;
+  double* d_valM = nullptr;
 };
 
 struct svg_lhs {

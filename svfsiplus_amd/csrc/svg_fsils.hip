@@ -1021,6 +1021,79 @@ static int dot_device(int n, const double* w, const double* base, size_t ld, int
   return SVG_OK;
 }
 
+// add_bc_mul (add_bc_mul.cpp:53-121), op ADD, for one coupled face:
+// S = res valM^T X in one workgroup (lane t sums the entries q = t, t+256, ...
+// of q = a nsd + i in order, a butterfly per wavefront, the 4 wavefront sums
+// in order), then Y(i, glob(a)) += valM(i, a) S.
+__global__ __launch_bounds__(256) void k_add_bc_mul(int nNo, int fdof, int nsd, int dof, const int* __restrict__ glob,
+                                                    const double* __restrict__ valM, double res,
+                                                    const double* __restrict__ X, double* __restrict__ Y,
+                                                    const int* __restrict__ skip)
+{
+#pragma clang fp contract(off)
+  __shared__ double ws[4];
+  __shared__ double Ssh;
+  if (skip && *skip) return;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const long nq = (long)nNo * nsd;
+  double acc = 0.0;
+  for (long q = t; q < nq; q += 256) {
+    const int a = (int)(q / nsd), i = (int)(q - (long)a * nsd);
+    acc = acc + valM[i + (size_t)fdof * a] * X[i + (size_t)dof * glob[a]];
+  }
+  acc = wave_sum(acc);
+  if (lane == 0) ws[wv] = acc;
+  __syncthreads();
+  if (t == 0) {
+    double S = ws[0];
+    for (int w = 1; w < 4; w++) S = S + ws[w];
+    Ssh = res * S;
+  }
+  __syncthreads();
+  const double S = Ssh;
+  for (long q = t; q < nq; q += 256) {
+    const int a = (int)(q / nsd), i = (int)(q - (long)a * nsd);
+    double* y = Y + i + (size_t)dof * glob[a];
+    *y = *y + valM[i + (size_t)fdof * a] * S;
+  }
+}
+
+// valM(i, a) = val(i, a) W(i, glob(a)) for i < nsd (precond.cpp:246-256)
+__global__ __launch_bounds__(256) void k_face_valM(int nNo, int fdof, int nsd, int dof, const int* __restrict__ glob,
+                                                   const double* __restrict__ val, const double* __restrict__ W,
+                                                   double* __restrict__ valM)
+{
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (long)nNo * nsd) return;
+  const int a = (int)(q / nsd), i = (int)(q - (long)a * nsd);
+  valM[i + (size_t)fdof * a] = val[i + (size_t)fdof * a] * W[i + (size_t)dof * glob[a]];
+}
+
+// y = b - y
+__global__ __launch_bounds__(256) void k_rsub(int n, const double* __restrict__ b, double* __restrict__ y)
+{
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < (size_t)n) y[k] = b[k] - y[k];
+}
+
+static bool has_coupled(const svg_lhs* L)
+{
+  for (const auto& F : L->face)
+    if (F.coupledFlag) return true;
+  return false;
+}
+
+// Y += res valM (valM^T X) for every coupled face (a no-op without one).
+static void add_bc_mul_dev(const svg_lhs* L, int dof, const double* X, double* Y, const int* skip, hipStream_t s)
+{
+  for (const auto& F : L->face) {
+    if (!F.coupledFlag) continue;
+    const int nsd = F.dof < dof ? F.dof : dof;
+    launch("add_bc_mul", k_add_bc_mul, dim3(1), dim3(256), 0, s, F.nNo, F.dof, nsd, dof, (const int*)F.d_glob,
+           (const double*)F.d_valM, F.res, X, Y, skip);
+  }
+}
+
 // The operator of a Krylov solve: the dof x dof block matrix K over lhs's pattern.
 struct Op {
   const svg_lhs* L;
@@ -1030,6 +1103,13 @@ struct Op {
 
 static void op_apply(const Op& A, const double* x, double* y, const double* b, const int* skip, hipStream_t s)
 {
+  if (A.dof == 4 && has_coupled(A.L)) {
+    // K x + add_bc_mul (gmres.cpp:508-512, 537-539); the residual form as b - (K x + res v v^T x)
+    spmv<4, 4, 0>(A.L, A.K, x, y, nullptr, skip, s);
+    add_bc_mul_dev(A.L, 4, x, y, skip, s);
+    if (b) launch("rsub", k_rsub, dim3(blocks_for((size_t)4 * A.L->nNo, 256)), dim3(256), 0, s, 4 * A.L->nNo, b, y);
+    return;
+  }
   if (A.dof == 4) {
     if (b)
       spmv<4, 4, 1>(A.L, A.K, x, y, b, skip, s);
@@ -1086,6 +1166,7 @@ static int run_cycle(const Op& A, KrylovWork& k, int sD, FinArgs fa, double* X, 
       launch_b("spmv32", bytes, k_spmv32<2>, dim3(blocks_for((size_t)4 * A.L->nNo, 256)), dim3(256), 0, s,
                A.L->nNo, (const int*)A.L->d_rowPtr, (const int*)A.L->d_colPtr, K32, (const float*)k.u32,
                k.u + k.ld * (i + 1), (const int*)&k.st->done);
+      add_bc_mul_dev(A.L, 4, sin, k.u + k.ld * (i + 1), (const int*)&k.st->done, s);  // FP64 rank-one term
     } else {
       op_apply(A, sin, k.u + k.ld * (i + 1), nullptr, &k.st->done, s);
     }
@@ -2388,7 +2469,11 @@ int svg_fsils_solve_ex(svg_lhs* L, svg_ls* ls, int dof, double* Ri, double* Val,
       F.coupledFlag = false;
       if (!F.incFlag) continue;
       if (F.bGrp == SVG_BC_NEU && res[f] != 0.0) {
-        return set_error(SVG_ERR_UNSUPPORTED, "fsils_solve: coupled (resistance) Neumann faces are not built");
+        F.res = res[f];
+        F.coupledFlag = true;
+        if (ls->LS_type != SVG_LS_GMRES)
+          return set_error(SVG_ERR_UNSUPPORTED,
+                           "fsils_solve: coupled (resistance) Neumann faces are built for LS type GMRES");
       }
     }
   }
@@ -2400,6 +2485,12 @@ int svg_fsils_solve_ex(svg_lhs* L, svg_ls* ls, int dof, double* Ri, double* Val,
   const int n = dof * L->nNo;
   if (prec == SVG_PREC_FSILS) {
     if ((rc = precond_diag_dev(L, dof, Val, Ri, k.W, s))) return rc;
+    for (const auto& F : L->face) {
+      if (!F.coupledFlag) continue;
+      const int nsd = F.dof < dof ? F.dof : dof;
+      launch("face_valM", k_face_valM, dim3(blocks_for((size_t)F.nNo * nsd, 256)), dim3(256), 0, s, F.nNo, F.dof,
+             nsd, dof, (const int*)F.d_glob, (const double*)F.d_val, (const double*)k.W, F.d_valM);
+    }
   } else if (prec == SVG_PREC_NONE) {
     svg_fill(k.W, (size_t)n, 1.0, s);
   } else {
@@ -2553,7 +2644,10 @@ int svg_batch_fsils_solve_ex(svg_batch* B, const svg_subls* proto, double* R, do
   if (!B || !R || !Val) return set_error(SVG_ERR_ARG, "svg_batch_fsils_solve: null argument");
   svg_lhs* L = B->L;
   hipStream_t s = (hipStream_t)stream;
-  for (auto& f : L->face) f.incFlag = true;
+  for (auto& f : L->face) {
+    f.incFlag = true;
+    f.coupledFlag = false;  // no resistances in the batched solve
+  }
   int rc;
   if ((rc = ensure_work(L->kw, 4 * L->nNo, 1))) return rc;
   if ((rc = precond_diag_dev(L, 4, Val, R, L->kw.W, s))) return rc;
@@ -2596,7 +2690,10 @@ int svg_batch_newton_iteration(svg_batch* B, svg_msh* M, svg_face* Fa, const svg
   svg_lhs* L = B->L;
   hipStream_t caller = (hipStream_t)stream, s = B->cs_stream;
   auto t0 = std::chrono::steady_clock::now();
-  for (auto& f : L->face) f.incFlag = true;
+  for (auto& f : L->face) {
+    f.incFlag = true;
+    f.coupledFlag = false;  // no resistances in the batched solve
+  }
   if ((rc = ensure_work(L->kw, 4 * L->nNo, 1)) || (rc = build_mask(L, 4)) ||
       (rc = batch_prepare(B, Val, md, caller)))
     return rc;

@@ -367,6 +367,19 @@ int svg_lhs_bc_create(svg_lhs* L, int faIn, int nNo, int dof, int bGrp, const in
   f.val.assign((size_t)dof * nNo, 0.0);
   if (val) std::memcpy(f.val.data(), val, sizeof(double) * dof * nNo);
   L->maskBuilt = false;
+  dev_free(f.d_glob);
+  dev_free(f.d_val);
+  dev_free(f.d_valM);
+  if (bGrp == SVG_BC_NEU && nNo > 0) {
+    // a Neumann face may be coupled (fsils_solve's res): add_bc_mul needs it on the device
+    int rc;
+    if ((rc = dev_alloc(&f.d_glob, (size_t)nNo)) || (rc = dev_alloc(&f.d_val, (size_t)dof * nNo)) ||
+        (rc = dev_alloc(&f.d_valM, (size_t)dof * nNo)))
+      return rc;
+    SVG_HIP(hipMemcpy(f.d_glob, f.glob.data(), sizeof(int) * nNo, hipMemcpyHostToDevice));
+    SVG_HIP(hipMemcpy(f.d_val, f.val.data(), sizeof(double) * dof * nNo, hipMemcpyHostToDevice));
+    SVG_HIP(hipMemset(f.d_valM, 0, sizeof(double) * dof * nNo));  // valM(dof, nNo) starts at 0 (bc.cpp:80)
+  }
   return SVG_OK;
 }
 
@@ -383,6 +396,11 @@ int svg_lhs_free(svg_lhs* L)
   dev_free(L->d_val32);
   dev_free(L->d_val16);
   dev_free(L->d_diag16);
+  for (auto& f : L->face) {
+    dev_free(f.d_glob);
+    dev_free(f.d_val);
+    dev_free(f.d_valM);
+  }
   auto& w = L->ns;
   for (double* p : {w.mK, w.mG, w.mD, w.mL, w.Gt, w.Rm, w.Rmi, w.Rc, w.Rci, w.U, w.MU, w.P, w.MP, w.cgX, w.cgP,
                     w.cgSP, w.cgGP, w.coef, w.dots, w.partial})

@@ -36,7 +36,9 @@ def _stream():
 
 
 class DeviceLhs:
-    """Device-resident FSILS lhs (fsils_lhs_create + fsils_bc_create)."""
+    """Device-resident FSILS lhs (fsils_lhs_create + fsils_bc_create). A face is
+    (name, nodes) — a Dirichlet face, val = 0 — or (name, nodes, "neu", val(nNo, 3))
+    — a Neumann face, coupled when fsils_solve is given a nonzero res for it."""
 
     def __init__(self, pipe: Pipe, faces=None):
         lib = _lib.load()
@@ -46,10 +48,12 @@ class DeviceLhs:
                                       len(faces), C.byref(h)))
         self.h = h
         self.nNo, self.nnz, self.nFaces = pipe.nNo, pipe.nnz, len(faces)
-        for i, (_, nodes) in enumerate(faces):
-            g = np.ascontiguousarray(nodes, dtype=np.int32)
-            val = np.zeros((len(g), 3))
-            _lib.check(lib.svg_lhs_bc_create(h, i, len(g), 3, _lib.BC_DIR, g.ctypes.data, val.ctypes.data))
+        for i, f in enumerate(faces):
+            g = np.ascontiguousarray(f[1], dtype=np.int32)
+            neu = len(f) > 2 and f[2] == "neu"
+            val = (np.ascontiguousarray(f[3], dtype=np.float64).reshape(len(g), 3) if neu else np.zeros((len(g), 3)))
+            _lib.check(lib.svg_lhs_bc_create(h, i, len(g), 3, _lib.BC_NEU if neu else _lib.BC_DIR, g.ctypes.data,
+                                             val.ctypes.data))
 
     def free(self):
         if self.h:

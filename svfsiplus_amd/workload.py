@@ -219,6 +219,25 @@ def face_arrays(p: Pipe):
     return nn, glob, val
 
 
+def face_normal_integrals(p: Pipe, faceIEN, inner):
+    """The FSILS face data of a coupled Neumann face: its nodes and
+    val(a) = integral of N_a n over the face (n outward, N_a the TRI3 shape
+    function: area n / 3 per triangle and vertex), the vector a resistance
+    boundary condition couples through (fsils_bc_create of the face with
+    BC_TYPE_Neu). Returns (nodes, val (nNo, 3))."""
+    nodes, inv = np.unique(np.asarray(faceIEN).ravel(), return_inverse=True)
+    x = p.x
+    a, b, c = (x[faceIEN[:, k]] for k in range(3))
+    cr = 0.5 * np.cross(b - a, c - a)                       # area-weighted normal
+    out = np.sign(np.einsum("ij,ij->i", cr, a - x[inner]))  # away from the interior node
+    cr = cr * out[:, None]
+    val = np.zeros((nodes.shape[0], 3))
+    np.add.at(val, inv.reshape(-1, 3)[:, 0], cr / 3.0)
+    np.add.at(val, inv.reshape(-1, 3)[:, 1], cr / 3.0)
+    np.add.at(val, inv.reshape(-1, 3)[:, 2], cr / 3.0)
+    return nodes.astype(np.int32), val
+
+
 def outlet_faces(p: Pipe):
     """TRI3 faces of the outlet (every tet face whose three nodes carry the
     outlet flag): faceIEN (nFe, 3) in the tet's node order, the parent tet

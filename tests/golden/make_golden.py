@@ -5,7 +5,7 @@ oracle/_ref/libsvref.so has been compiled from the reference sources by
 oracle/ref.mk). Each fixture is data: the seeded mesh, the inputs, and what
 the reference's construct_fluid and fsils_solve returned for them.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [names...]
 """
 from __future__ import annotations
 
@@ -55,8 +55,40 @@ def make(name, n, nz, seed, gm_tol, ns_tol):
     print(name, p.nNo, p.nEl, p.nnz, "gmres itr", og["itr"], "ns itr", on["itr"], on["GM_itr"], on["CG_itr"])
 
 
+def make_coupled(name, n, nz, res):
+    """A coupled (resistance) outlet: the first Newton system of the pipe with
+    the outlet registered as a Neumann face (val = integral of N_a n) and a
+    resistance `res` passed to fsils_solve, solved by the reference's GMRES."""
+    p = W.make_pipe(n, nz)
+    ph = W.Physics()
+    Ao, Yo, An, Yn = W.dirichlet_state(p, ph)
+    Ag, Yg = oracle.pici(ph, Ao, An, Yo, Yn)
+    R0, V0 = oracle.ref_construct_fluid(p, ph, Ag, Yg)
+    fi, _, inner = W.outlet_faces(p)
+    nodes, val = W.face_normal_integrals(p, fi, inner)
+    faces = list(p.faces) + [("outlet", nodes, "neu", val)]
+    rs = np.array([0.0, 0.0, res])
+    gls = oracle.ls_create(oracle.LS_GMRES, relTol=1e-6, absTol=1e-30, maxItr=1000, dimKry=50)
+    xg, Vg, og = oracle.ref_fsils_solve(p, oracle.LS_GMRES, oracle.ls_params(gls), R0, V0, faces=faces, res=rs)
+    np.savez_compressed(
+        os.path.join(OUT, name + ".npz"),
+        n=n, nz=nz, x=p.x, IEN=p.IEN, rowPtr=p.rowPtr, colPtr=p.colPtr, inlet=p.faces[0][1], wall=p.faces[1][1],
+        phys=np.array([ph.rho, ph.mu, ph.dt, ph.ro_inf, ph.peak]),
+        R0=R0, Val0=V0, outlet=nodes, outlet_val=val, res=rs, gm_tol=1e-6, gm_x=xg,
+        gm_out=np.array([og[k] for k in ("itr", "iNorm", "fNorm", "suc")]))
+    print(name, p.nNo, p.nEl, p.nnz, "coupled gmres itr", og["itr"])
+
+
+COUPLED = [("coupled_n4", 4, 8, 1000.0)]
+
+
 if __name__ == "__main__":
     if not oracle.ref_available():
         sys.exit("oracle/_ref/libsvref.so is not built (make -f oracle/ref.mk)")
+    only = sys.argv[1:]
     for c in CASES:
-        make(*c)
+        if not only or c[0] in only:
+            make(*c)
+    for c in COUPLED:
+        if not only or c[0] in only:
+            make_coupled(*c)

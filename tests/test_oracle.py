@@ -34,6 +34,60 @@ def test_gmres_matches_reference_golden(name):
     assert np.array_equal(x, d["gm_x"]) and np.array_equal(V, d["gm_Val"])
 
 
+def _coupled(d):
+    faces_ = [("inlet", d["inlet"]), ("wall", d["wall"]), ("outlet", d["outlet"], "neu", d["outlet_val"])]
+    return faces_, d["res"]
+
+
+def test_coupled_gmres_matches_reference_golden():
+    """GMRES with a coupled (resistance) outlet: add_bc_mul after every block
+    product and valM = val W, against the reference's own fsils_solve."""
+    p, ph, d = G.load("coupled_n4")
+    faces_, res = _coupled(d)
+    ls = oracle.ls_create(oracle.LS_GMRES, relTol=float(d["gm_tol"]), absTol=1e-30, maxItr=1000, dimKry=50)
+    x, V, ls = oracle.fsils_solve(p, ls, d["R0"], d["Val0"], faces=faces_, res=res)
+    itr, iNorm, fNorm, suc = d["gm_out"].tolist()
+    assert ls.RI.itr == itr and ls.RI.suc == suc
+    assert ls.RI.iNorm == iNorm and ls.RI.fNorm == fNorm
+    assert np.array_equal(x, d["gm_x"])
+
+
+def test_coupled_gmres_solves_the_coupled_system():
+    """The coupled solve's x satisfies K x + res v (v^T x) = R on the rows that
+    are not Dirichlet (v with its Dirichlet entries removed), checked with a
+    dense product; the uncoupled solution does not."""
+    p, ph, d = G.load("coupled_n4")
+    faces_, res = _coupled(d)
+    n = p.nNo
+    K = np.zeros((4 * n, 4 * n))
+    V0 = d["Val0"].reshape(-1, 4, 4)
+    for a in range(n):
+        for j in range(p.rowPtr[a], p.rowPtr[a + 1]):
+            b = p.colPtr[j]
+            K[4 * a:4 * a + 4, 4 * b:4 * b + 4] = V0[j]
+    fixed = np.zeros(4 * n, bool)
+    for nodes in (d["inlet"], d["wall"]):
+        for i in range(3):
+            fixed[4 * nodes + i] = True
+    v = np.zeros(4 * n)
+    for a, node in enumerate(d["outlet"]):
+        v[4 * node:4 * node + 3] = d["outlet_val"][a]
+    v[fixed] = 0.0
+    R = d["R0"].ravel()
+
+    def rel_residual(x):
+        r = R - K @ x - res[2] * v * (v @ x)
+        return np.linalg.norm(r[~fixed]) / np.linalg.norm(R[~fixed])
+
+    ls = oracle.ls_create(oracle.LS_GMRES, relTol=1e-10, absTol=1e-30, maxItr=1000, dimKry=50)
+    x, _, ls = oracle.fsils_solve(p, ls, d["R0"], d["Val0"], faces=faces_, res=res)
+    ls0 = oracle.ls_create(oracle.LS_GMRES, relTol=1e-10, absTol=1e-30, maxItr=1000, dimKry=50)
+    x0, _, _ = oracle.fsils_solve(p, ls0, d["R0"], d["Val0"], faces=faces_, res=np.zeros(3))
+    assert ls.RI.suc == 1
+    assert rel_residual(x.ravel()) < 1e-7
+    assert rel_residual(x0.ravel()) > 1e-3
+
+
 @pytest.mark.parametrize("name", G.NAMES)
 def test_ns_solver_matches_reference_golden(name):
     p, ph, d = G.load(name)
