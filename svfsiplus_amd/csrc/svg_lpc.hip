@@ -94,6 +94,30 @@ __device__ __forceinline__ void mma(const bf16* A, int sam, int sak, const bf16*
   }
 }
 
+// The same product for operands whose k index is contiguous (A(m,k) =
+// A[m*sam + k], B(k,n) = B[n*sbn + k]): each lane's 4 consecutive BF16 of A
+// and of B come in one 8-byte LDS read (rows 8-byte aligned).
+template <int M, int N, int K>
+__device__ __forceinline__ void mma_kc(const bf16* A, int sam, const bf16* B, int sbn, v4f* acc, int wave, int lane)
+{
+  constexpr int TN = N / 16, QW = (M / 16) * (N / 16) / 4;
+  const int r = lane & 15, g = lane >> 4;
+#pragma unroll
+  for (int qi = 0; qi < QW; qi++) {
+    const int q = wave + 4 * qi, tm = q / TN, tn = q % TN;
+    v4f c = acc[qi];
+    const bf16* Ar = A + (16 * tm + r) * sam + 4 * g;
+    const bf16* Br = B + (16 * tn + r) * sbn + 4 * g;
+#pragma unroll
+    for (int k0 = 0; k0 < K; k0 += 16) {
+      const v4s a = *reinterpret_cast<const v4s*>(Ar + k0);
+      const v4s b = *reinterpret_cast<const v4s*>(Br + k0);
+      c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+    }
+    acc[qi] = c;
+  }
+}
+
 // Row and column of accumulator element (qi, v) of wavefront `wave`.
 template <int N>
 __device__ __forceinline__ void acc_rc(int wave, int lane, int qi, int v, int& row, int& col)
@@ -174,8 +198,8 @@ __global__ __launch_bounds__(BS) void k_encode(int n, int rows, const double* __
                                                const bf16* __restrict__ W0, bf16* __restrict__ H,
                                                const int* __restrict__ skip, RowGate gate)
 {
-  __shared__ bf16 U[TILE * US];
-  __shared__ bf16 Ws[F * US];
+  __shared__ __align__(16) bf16 U[TILE * US];
+  __shared__ __align__(16) bf16 Ws[F * US];
   if (skip && *skip) return;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int ntiles = (rows + TILE - 1) / TILE;
@@ -188,7 +212,7 @@ __global__ __launch_bounds__(BS) void k_encode(int n, int rows, const double* __
     v4f acc[4];
 #pragma unroll
     for (int qi = 0; qi < 4; qi++) acc[qi] = v4f{0.f, 0.f, 0.f, 0.f};
-    mma<TILE, F, NIN>(U, US, 1, Ws, 1, US, acc, wave, lane);  // B(k,n) = W0[n][k]
+    mma_kc<TILE, F, NIN>(U, US, Ws, US, acc, wave, lane);  // B(k,n) = W0[n][k]
 #pragma unroll
     for (int qi = 0; qi < 4; qi++)
 #pragma unroll
@@ -333,7 +357,7 @@ __global__ __launch_bounds__(BS) void k_dense(int rows, const bf16* __restrict__
     v4f acc[4];
 #pragma unroll
     for (int qi = 0; qi < 4; qi++) acc[qi] = v4f{bv[qi], bv[qi], bv[qi], bv[qi]};
-    mma<TILE, F, 2 * F>(X, XS, 1, Ws, 1, XS, acc, wave, lane);  // B(k,n) = W[n][k]
+    mma_kc<TILE, F, 2 * F>(X, XS, Ws, XS, acc, wave, lane);  // B(k,n) = W[n][k]
     __syncthreads();
 #pragma unroll
     for (int qi = 0; qi < 4; qi++)
@@ -354,8 +378,22 @@ __global__ __launch_bounds__(BS) void k_dense(int rows, const bf16* __restrict__
     if (Wd) {
       const int node = t >> 2, d = t & 3, r = r0 + node;
       if (r < rows) {
+        // y = sum_f Wd[d][f] h[f] in f order, 8 BF16 per 16-byte load
+        const uint4* xr = reinterpret_cast<const uint4*>(X + node * XS);
+        const uint4* wr = reinterpret_cast<const uint4*>(Wd + d * F);
         float y = 0.f;
-        for (int f = 0; f < F; f++) y = fmaf(bf2f(Wd[d * F + f]), bf2f(X[node * XS + f]), y);
+#pragma unroll
+        for (int f8 = 0; f8 < F / 8; f8++) {
+          const uint4 xv = xr[f8], wv = wr[f8];
+          y = fmaf(lo(wv.x), lo(xv.x), y);
+          y = fmaf(hi(wv.x), hi(xv.x), y);
+          y = fmaf(lo(wv.y), lo(xv.y), y);
+          y = fmaf(hi(wv.y), hi(xv.y), y);
+          y = fmaf(lo(wv.z), lo(xv.z), y);
+          y = fmaf(hi(wv.z), hi(xv.z), y);
+          y = fmaf(lo(wv.w), lo(xv.w), y);
+          y = fmaf(hi(wv.w), hi(xv.w), y);
+        }
         const double z = V[(size_t)r * 4 + d] + (double)y * (nsc ? nsc[r].y : inv_sc);
         Z[(size_t)r * 4 + d] = z;
         if (Z32) Z32[(size_t)r * 4 + d] = (float)z;
@@ -732,7 +770,7 @@ static int lpc_forward(svg_lpc* N, const svg_lhs* L, int nvec, const double* V, 
     bf16* Hout = train ? N->H[l + 1] : N->H[(l + 1) & 1];
     bf16* Ml = train ? N->M[l] : N->M[0];
     // BF16 blocks + colPtr per block (per sample), rowPtr + h + m out per row
-    launch_b("lpc_msg", (32.0 + 4.0) * L->nnz * nvec + (4.0 + 2.0 * F) * rows, k_msg<2>, dim3(nm), dim3(256), 0, s, n,
+    launch_b("lpc_msg", (32.0 + 4.0) * L->nnz * nvec + (4.0 + 2.0 * F) * rows, k_msg<4>, dim3(nm), dim3(256), 0, s, n,
              rows, (const int*)L->d_rowPtr, (const int*)L->d_colPtr, (const bf16*)L->d_val16, Hin, Ml, skip, gate);
     // h, m in, h out (+ p for training; + V in, z out for the last layer)
     const double db = 6.0 * F * rows + (train ? 2.0 * F * rows : 0.0) + (last ? (64.0 + (Z32 ? 16.0 : 0.0)) * rows : 0.0);
