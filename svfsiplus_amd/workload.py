@@ -122,14 +122,22 @@ def make_pipe(n: int, nz: int | None = None, radius: float = 0.3, length: float 
     return p
 
 
-def _finish_mesh(x, IEN, flags, n, nz, radius, length) -> Pipe:
+def pattern(nNo: int, IEN: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    """Block-CSR pattern of a TET4 mesh (what lhsa builds, lhsa.cpp:153-380):
+    rowPtr (nNo+1, exclusive end) and the sorted colPtr."""
     lib = _lib.load()
-    nNo, nEl = x.shape[0], IEN.shape[0]
+    IEN = np.ascontiguousarray(IEN, dtype=np.int32)
+    nEl = IEN.shape[0]
     nnz = C.c_int()
     _lib.check(lib.svg_csr_pattern_count(nNo, nEl, 4, IEN.ctypes.data, C.byref(nnz)))
     rowPtr = np.empty(nNo + 1, dtype=np.int32)
     colPtr = np.empty(nnz.value, dtype=np.int32)
     _lib.check(lib.svg_csr_pattern(nNo, nEl, 4, IEN.ctypes.data, rowPtr.ctypes.data, colPtr.ctypes.data))
+    return rowPtr, colPtr
+
+
+def _finish_mesh(x, IEN, flags, n, nz, radius, length) -> Pipe:
+    rowPtr, colPtr = pattern(x.shape[0], IEN)
     inlet = np.nonzero(flags & _lib.NODE_INLET)[0].astype(np.int32)
     wall = np.nonzero(flags & _lib.NODE_WALL)[0].astype(np.int32)
     p = Pipe(n, nz, x, IEN, flags, rowPtr, colPtr, radius, length)
@@ -238,12 +246,12 @@ def face_normal_integrals(p: Pipe, faceIEN, inner):
     return nodes.astype(np.int32), val
 
 
-def outlet_faces(p: Pipe):
+def outlet_faces(p: Pipe, flag: int = _lib.NODE_OUTLET):
     """TRI3 faces of the outlet (every tet face whose three nodes carry the
-    outlet flag): faceIEN (nFe, 3) in the tet's node order, the parent tet
-    gE (nFe,) and its fourth node (nFe,), which orients the outward normal as
-    gnnb does."""
-    out = (p.flags & _lib.NODE_OUTLET) != 0
+    outlet flag, or `flag`): faceIEN (nFe, 3) in the tet's node order, the
+    parent tet gE (nFe,) and its fourth node (nFe,), which orients the outward
+    normal as gnnb does."""
+    out = (p.flags & flag) != 0
     faces, parent, inner = [], [], []
     for k in range(4):
         keep = [a for a in range(4) if a != k]

@@ -466,3 +466,31 @@ def test_coupled_gmres(torch_cuda):
         print(label, "itr", ls2.RI.itr, "residual", r2, "reference's", ref_res)
         assert ls2.RI.suc == 1
         assert r2 <= 10 * max(ref_res, 1e-9)
+
+
+def test_newton_step_on_a_loaded_model(torch_cuda, tmp_path, phys):
+    """The seeded pipe written as a SimVascular-style model (volume .vtu, face
+    .vtp files) and loaded back runs the same Newton step, record for record."""
+    from svfsiplus_amd import vtk_io as V
+    from svfsiplus_amd.newton import FluidNewton, ls_create
+    p = W.make_pipe(4)
+    V.write_vtu(str(tmp_path / "mesh.vtu"), p.x, p.IEN)
+    paths = {}
+    for name, bit in (("inlet", _lib.NODE_INLET), ("wall", _lib.NODE_WALL), ("outlet", _lib.NODE_OUTLET)):
+        fi, ge, _ = W.outlet_faces(p, bit)
+        nodes = np.unique(fi)
+        V.write_vtp(str(tmp_path / f"{name}.vtp"), p.x[nodes], np.searchsorted(nodes, fi),
+                    {"GlobalNodeID": (nodes + 1).astype(np.int32)}, {"GlobalElementID": (ge + 1).astype(np.int32)})
+        paths[name] = str(tmp_path / f"{name}.vtp")
+    q, faces = V.load_mesh(str(tmp_path / "mesh.vtu"), paths, dirichlet=["inlet", "wall"],
+                           node_flags={"outlet": _lib.NODE_OUTLET})
+    assert np.array_equal(q.faces[0][1], p.faces[0][1]) and np.array_equal(q.faces[1][1], p.faces[1][1])
+    mk = lambda: ls_create(_lib.LS_GMRES, relTol=1e-6, absTol=1e-30, maxItr=1000, dimKry=50)
+    h0 = FluidNewton(p, phys, mk()).time_step()
+    fn = FluidNewton(q, phys, mk())
+    h1 = fn.time_step()
+    for a, b in zip(h0, h1):
+        assert (a["ls_itr"], a["iNorm"], a["fNorm"]) == (b["ls_itr"], b["iNorm"], b["fNorm"])
+    V.write_result(str(tmp_path / "result_001.vtu"), q, fn.Yn, fn.An)
+    r = V.read(str(tmp_path / "result_001.vtu"))
+    assert np.array_equal(r.point_data["Velocity"], fn.Yn.cpu().numpy()[:, :3])
