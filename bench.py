@@ -156,6 +156,8 @@ def kernel_only(a, fn, pipe, lib):
         # the mixed-precision solve's FP32 SpMV on FP32 copies of the same matrix and vector
         kname = "spmv32"
         K32 = fn.Val.to(torch.float32)
+        scratch = torch.empty((pipe.nNo, 4), dtype=torch.float32, device="cuda")
+        _lib.check(lib.svg_spmv32_tune(fn.lhs.h, K32.data_ptr(), scratch.data_ptr(), KU.data_ptr(), None, _stream()))
         U32 = U.to(torch.float32)
         nbytes = pipe.nnz * (16 * 4 + 4) + pipe.nNo * (4 + 4 * 4 + 4 * 8)
         src = torch.empty(nbytes // 2 // 8, dtype=torch.float64, device="cuda").fill_(1.0)
@@ -193,6 +195,14 @@ def _bytes(lib, ktab):
             lib.svg_prof_get_bytes(k.encode(), C.byref(tb))
             ktab[k]["bytes"] = tb.value
     return ktab
+
+
+def spmv_variants(lib, lhs):
+    """The SpMV launch variants the library's autotuners picked for this lhs."""
+    import ctypes as C
+    v64, v32 = C.c_int(), C.c_int()
+    lib.svg_spmv_variants(lhs.h, C.byref(v64), C.byref(v32))
+    return {"fp64": v64.value, "fp32": v32.value}
 
 
 def timed_run(fn, lib, steps, warmup, prof, label):
@@ -315,6 +325,7 @@ def batch_run(a, log):
            "solver": "batched %sGMRES(%d), per-mesh Krylov spaces and convergence; each Newton iteration is three "
                      "replayed hipGraphs (pici + assembly + preconditioner, the restart cycle, the corrector); "
                      "block SpMV variant from the autotuner" % ("flexible " if net else "", a.krylov)}
+    out["spmv_variants"] = spmv_variants(_lib.load(), bn.fn.lhs)
     if net is not None:
         out["learned_pc"] = {"weights": os.path.relpath(a.lpc_weights, ROOT), "net_period": a.net_period,
                              "meshes_fallen_back": fallbacks}
@@ -430,7 +441,8 @@ def main():
                                f"{pipe.nEl} tets, {pipe.nNo} nodes, nnzb {pipe.nnz}; "
                                f"1 time step x 3 Newton iterations; GMRES({a.krylov}) rel-tol {a.tol:g} (FP64 "
                                f"residual of the scaled system)",
-                   "method": head, "solver": solver[head], "newton_records": R["hists"][-1]},
+                   "method": head, "solver": solver[head], "newton_records": R["hists"][-1],
+                   "spmv_variants": spmv_variants(lib, fn.lhs)},
         "roofline": roofline(R["kernels"], traffic),
         "cpu_baseline": cpu,
         "kernels": R["kernels_warmup"] or R["kernels"],

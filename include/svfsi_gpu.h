@@ -318,6 +318,13 @@ int svg_spmv_tune(svg_lhs* lhs, const double* d_K, int* best_variant, void* stre
  * multiply-adds in the block order of spar_mul.cpp:228-237, KU in FP64.
  * Asynchronous on the stream. */
 int svg_spmv32(const svg_lhs* lhs, const float* d_K32, const float* d_U32, double* d_KU, void* stream);
+/* Autotuner of the FP32 SpMV (no counterpart in the reference): times its
+ * launch variants (unroll depth, per-XCD row bands; identical results) on
+ * d_K32 and keeps the fastest for this lhs (run automatically at the first
+ * mixed-precision solve). d_U32 is zeroed and d_KU overwritten (scratch). */
+int svg_spmv32_tune(svg_lhs* lhs, const float* d_K32, float* d_U32, double* d_KU, int* best_variant, void* stream);
+/* The launch variants the autotuners picked for lhs (-1: not tuned yet). */
+int svg_spmv_variants(const svg_lhs* lhs, int* fp64_variant, int* fp32_variant);
 /* precond_diag (precond.cpp:122-257) with every registered face included. */
 int svg_precond_diag(svg_lhs* lhs, int dof, double* d_Val, double* d_R, double* d_W, void* stream);
 /* fsils_dot_v / fsi_ls_normv (dot.cpp:69-117, norm.cpp:67-112); blocking. */

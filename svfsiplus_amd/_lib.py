@@ -101,6 +101,8 @@ _SIGS = {
     "svg_spar_mul_vv": (_I, [_P, _I, _P, _P, _P, _P]),
     "svg_precond_diag": (_I, [_P, _I, _P, _P, _P, _P]),
     "svg_spmv32": (_I, [_P, _P, _P, _P, _P]),
+    "svg_spmv32_tune": (_I, [_P, _P, _P, _P, _IP, _P]),
+    "svg_spmv_variants": (_I, [_P, _IP, _IP]),
     "svg_spmv_num_variants": (_I, [_IP]),
     "svg_spmv_time_variant": (_I, [_P, _I, _P, _P, _P, _I, _DP, _P]),
     "svg_spmv_tune": (_I, [_P, _P, _IP, _P]),

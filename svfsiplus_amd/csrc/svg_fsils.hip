@@ -237,12 +237,38 @@ template <int UNR>
 __global__ __launch_bounds__(256) void k_spmv32(int nNo, const int* __restrict__ rowPtr,
                                                 const int* __restrict__ colPtr, const float* __restrict__ K,
                                                 const float* __restrict__ U, double* __restrict__ W,
-                                                const int* __restrict__ skip)
+                                                const int* __restrict__ skip, int xcd)
 {
   if (skip && *skip) return;
-  const size_t k = (size_t)blockIdx.x * 256 + threadIdx.x;
+  int bx = blockIdx.x;
+  if (xcd) {
+    bx = xcd_block(blockIdx.x, gridDim.x);
+    if (bx < 0) return;
+  }
+  const size_t k = (size_t)bx * 256 + threadIdx.x;
   const int row = (int)(k >> 2), i = (int)(k & 3);
   if (row >= nNo) return;
+  spmv32_row<UNR>(row, i, rowPtr, colPtr, K, U, W, k);
+}
+
+// The batched solve's FP32 SpMV: rows of systems that stopped iterating are skipped.
+template <int UNR>
+__global__ __launch_bounds__(256) void k_bspmv32(int nNo, const int* __restrict__ rowPtr,
+                                                 const int* __restrict__ colPtr, const float* __restrict__ K,
+                                                 const float* __restrict__ U, double* __restrict__ W,
+                                                 const int* __restrict__ sysOfNode, const GmState* __restrict__ st,
+                                                 int xcd)
+{
+  int bx = blockIdx.x;
+  if (xcd) {
+    bx = xcd_block(blockIdx.x, gridDim.x);
+    if (bx < 0) return;
+  }
+  const size_t k = (size_t)bx * 256 + threadIdx.x;
+  const int row = (int)(k >> 2), i = (int)(k & 3);
+  if (row >= nNo) return;
+  const GmState& S = st[sysOfNode[row]];
+  if (!S.act || S.done) return;
   spmv32_row<UNR>(row, i, rowPtr, colPtr, K, U, W, k);
 }
 
@@ -251,6 +277,77 @@ __global__ __launch_bounds__(256) void k_d2f(size_t n, const double* __restrict_
   size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   for (; k < n; k += stride) b[k] = (float)a[k];
+}
+
+// Launch variants of the FP32 block SpMV (unroll depth, per-XCD row bands),
+// chosen per lhs by spmv32_tune; every variant computes the same sums in the
+// same order. Variant 0 is the untuned default.
+struct Spmv32Variant {
+  int unroll;
+  bool xcd;
+};
+static const Spmv32Variant kSpmv32Variants[] = {{2, false}, {1, false}, {4, false}, {2, true}, {1, true}, {4, true}};
+constexpr int kNumSpmv32Variants = sizeof(kSpmv32Variants) / sizeof(kSpmv32Variants[0]);
+
+template <int UNR>
+static void spmv32_launch(const svg_lhs* L, const float* K32, const float* U32, double* W, const int* skip,
+                          const int* sysOfNode, const GmState* st, bool xcd, hipStream_t s, const char* name)
+{
+  unsigned nb = blocks_for((size_t)4 * L->nNo, 256);
+  const int use_xcd = (xcd && nb >= 64) ? 1 : 0;
+  if (use_xcd) nb = (nb + 7) & ~7u;
+  const double bytes = 68.0 * L->nnz + 36.0 * L->nNo;
+  if (sysOfNode)
+    launch_b(name, bytes, k_bspmv32<UNR>, dim3(nb), dim3(256), 0, s, L->nNo, (const int*)L->d_rowPtr,
+             (const int*)L->d_colPtr, K32, U32, W, sysOfNode, st, use_xcd);
+  else
+    launch_b(name, bytes, k_spmv32<UNR>, dim3(nb), dim3(256), 0, s, L->nNo, (const int*)L->d_rowPtr,
+             (const int*)L->d_colPtr, K32, U32, W, skip, use_xcd);
+}
+
+// W = K32 U32 with the lhs's tuned variant; sysOfNode/st: only the rows of
+// batched systems still iterating.
+static void spmv32(const svg_lhs* L, const float* K32, const float* U32, double* W, const int* skip,
+                   hipStream_t s, const char* name = "spmv32", const int* sysOfNode = nullptr,
+                   const GmState* st = nullptr, int variant = -1)
+{
+  const int v = variant >= 0 ? variant : (L->spmv32Variant >= 0 ? L->spmv32Variant : 0);
+  const Spmv32Variant& sv = kSpmv32Variants[v];
+  switch (sv.unroll) {
+    case 1: spmv32_launch<1>(L, K32, U32, W, skip, sysOfNode, st, sv.xcd, s, name); break;
+    case 4: spmv32_launch<4>(L, K32, U32, W, skip, sysOfNode, st, sv.xcd, s, name); break;
+    default: spmv32_launch<2>(L, K32, U32, W, skip, sysOfNode, st, sv.xcd, s, name); break;
+  }
+}
+
+// The autotuner of the FP32 SpMV: times every variant (one warm launch, then
+// 3) on K32 with the probe vector U32 (zeroed here) into W, keeps the fastest.
+static int spmv32_tune(svg_lhs* L, const float* K32, float* U32, double* W, hipStream_t s)
+{
+  SVG_HIP(hipMemsetAsync(U32, 0, sizeof(float) * 4 * L->nNo, s));
+  hipEvent_t a, b;
+  SVG_HIP(hipEventCreate(&a));
+  SVG_HIP(hipEventCreate(&b));
+  double best = 1e30;
+  int bv = 0;
+  for (int v = 0; v < kNumSpmv32Variants; v++) {
+    spmv32(L, K32, U32, W, nullptr, s, "spmv32_tune", nullptr, nullptr, v);
+    SVG_HIP(hipEventRecord(a, s));
+    for (int r = 0; r < 3; r++) spmv32(L, K32, U32, W, nullptr, s, "spmv32_tune", nullptr, nullptr, v);
+    SVG_HIP(hipEventRecord(b, s));
+    SVG_HIP(hipEventSynchronize(b));
+    float ms = 0.f;
+    SVG_HIP(hipEventElapsedTime(&ms, a, b));
+    if (ms < best) {
+      best = ms;
+      bv = v;
+    }
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  SVG_HIP(hipGetLastError());
+  L->spmv32Variant = bv;
+  return SVG_OK;
 }
 
 // ------------------------------------------------------------ reductions --
@@ -1162,10 +1259,7 @@ static int run_cycle(const Op& A, KrylovWork& k, int sD, FinArgs fa, double* X, 
       sin = k.z + k.ld * i;
     }
     if (mixed) {
-      const double bytes = 68.0 * A.L->nnz + 36.0 * A.L->nNo;
-      launch_b("spmv32", bytes, k_spmv32<2>, dim3(blocks_for((size_t)4 * A.L->nNo, 256)), dim3(256), 0, s,
-               A.L->nNo, (const int*)A.L->d_rowPtr, (const int*)A.L->d_colPtr, K32, (const float*)k.u32,
-               k.u + k.ld * (i + 1), (const int*)&k.st->done);
+      spmv32(A.L, K32, (const float*)k.u32, k.u + k.ld * (i + 1), (const int*)&k.st->done, s);
       add_bc_mul_dev(A.L, 4, sin, k.u + k.ld * (i + 1), (const int*)&k.st->done, s);  // FP64 rank-one term
     } else {
       op_apply(A, sin, k.u + k.ld * (i + 1), nullptr, &k.st->done, s);
@@ -1265,6 +1359,7 @@ static int gmres_v_mixed(svg_lhs* L, svg_subls* ls, const double* Val, double* R
   if (L->spmvVariant < 0 && (rc = svg_spmv_tune(L, Val, nullptr, s))) return rc;
   auto t0 = std::chrono::steady_clock::now();
   launch("d2f", k_d2f, dim3(8192), dim3(256), 0, s, (size_t)16 * L->nnz, Val, L->d_val32);
+  if (L->spmv32Variant < 0 && (rc = spmv32_tune(L, L->d_val32, k.u32, k.X, s))) return rc;
   FinArgs fa{};
   fa.sD = sD;
   fa.absTol = ls->absTol;
@@ -1336,7 +1431,10 @@ static int gmres_v_learned(svg_lhs* L, svg_subls* ls, const double* Val, double*
   if (L->spmvVariant < 0 && (rc = svg_spmv_tune(L, Val, nullptr, s))) return rc;
   auto t0 = std::chrono::steady_clock::now();
   if ((rc = lpc_prepare_dev(L, Val, s))) return rc;
-  if (mixed) launch("d2f", k_d2f, dim3(8192), dim3(256), 0, s, (size_t)16 * L->nnz, Val, L->d_val32);
+  if (mixed) {
+    launch("d2f", k_d2f, dim3(8192), dim3(256), 0, s, (size_t)16 * L->nnz, Val, L->d_val32);
+    if (L->spmv32Variant < 0 && (rc = spmv32_tune(L, L->d_val32, k.u32, k.X, s))) return rc;
+  }
   FinArgs fa{};
   fa.sD = sD;
   fa.absTol = ls->absTol;
@@ -1878,19 +1976,6 @@ __global__ __launch_bounds__(RED_BS) void k_bmdot(const int* __restrict__ spanSt
 
 // The FP32 SpMV of the batched solve: rows of systems that converged or
 // stopped skip (their basis vectors are no longer read).
-template <int UNR>
-__global__ __launch_bounds__(256) void k_bspmv32(int nNo, const int* __restrict__ rowPtr,
-                                                 const int* __restrict__ colPtr, const float* __restrict__ K,
-                                                 const float* __restrict__ U, double* __restrict__ W,
-                                                 const int* __restrict__ sysOfNode, const GmState* __restrict__ st)
-{
-  const size_t k = (size_t)blockIdx.x * 256 + threadIdx.x;
-  const int row = (int)(k >> 2), i = (int)(k & 3);
-  if (row >= nNo) return;
-  const GmState& S = st[sysOfNode[row]];
-  if (!S.act || S.done) return;
-  spmv32_row<UNR>(row, i, rowPtr, colPtr, K, U, W, k);
-}
 
 // One workgroup per system: the finaliser on the system's span partials.
 __global__ __launch_bounds__(FIN_BS) void k_bfin(FinArgs A, const int* __restrict__ sysSpanOff, int nsp,
@@ -2079,9 +2164,8 @@ static int batch_cycle(svg_batch* B, const double* Val, const double* R, const B
       sin = zi;
     }
     if (mixed)
-      hipLaunchKernelGGL(k_bspmv32<2>, dim3(blocks_for((size_t)4 * L->nNo, 256)), dim3(256), 0, s, L->nNo,
-                         (const int*)L->d_rowPtr, (const int*)L->d_colPtr, (const float*)L->d_val32,
-                         (const float*)B->u32, un, (const int*)B->d_sysOfNode, (const GmState*)B->st);
+      spmv32(L, (const float*)L->d_val32, (const float*)B->u32, un, nullptr, s, "bspmv32",
+             (const int*)B->d_sysOfNode, (const GmState*)B->st);
     else
       op_apply(A, sin, un, nullptr, nullptr, s);
     batch_mdot(B, un, B->u, B->ld, i + 2, 2, s);
@@ -2173,6 +2257,8 @@ static int batch_prepare(svg_batch* B, const double* Val, const BatchMode& md, h
   if (md.mixed) {
     if (!L->d_val32 && (rc = dev_alloc(&L->d_val32, (size_t)16 * L->nnz))) return rc;
     if (!B->u32 && (rc = dev_alloc(&B->u32, (size_t)B->n))) return rc;
+    // the FP32 SpMV variant for the union (timing only: the values do not matter)
+    if (L->spmv32Variant < 0 && (rc = spmv32_tune(L, L->d_val32, B->u32, B->X, caller))) return rc;
   }
   if (md.net) {
     if (!B->z && (rc = dev_alloc(&B->z, B->ld * B->sD))) return rc;
@@ -2215,7 +2301,7 @@ static int batch_cycle_graph(svg_batch* B, const double* R, const double* Val, c
   const int gi = md.mixed ? 1 : 0;
   GraphKey k;
   k.add(R).add(Val).add(fa.absTol).add(fa.relTol).add(fa.fbwin).add(B->L->d_val32).add(B->u32);
-  k.add(B->L->spmvVariant).add(md.net).add(md.period).add(B->z).add(B->d_nsc);
+  k.add(B->L->spmvVariant).add(B->L->spmv32Variant).add(md.net).add(md.period).add(B->z).add(B->d_nsc);
   if (B->graph[gi] && B->gKey[gi] == k.b) return SVG_OK;
   if (B->graph[gi]) (void)hipGraphExecDestroy(B->graph[gi]), B->graph[gi] = nullptr;
   hipStream_t s = B->cs_stream;
@@ -2754,11 +2840,25 @@ int svg_batch_newton_iteration(svg_batch* B, svg_msh* M, svg_face* Fa, const svg
 int svg_spmv32(const svg_lhs* L, const float* K32, const float* U32, double* KU, void* stream)
 {
   if (!L || !K32 || !U32 || !KU) return set_error(SVG_ERR_ARG, "svg_spmv32: null argument");
-  hipStream_t s = (hipStream_t)stream;
-  const double bytes = 68.0 * L->nnz + 36.0 * L->nNo;
-  launch_b("spmv32", bytes, k_spmv32<2>, dim3(blocks_for((size_t)4 * L->nNo, 256)), dim3(256), 0, s, L->nNo,
-           (const int*)L->d_rowPtr, (const int*)L->d_colPtr, K32, U32, KU, (const int*)nullptr);
+  spmv32(L, K32, U32, KU, nullptr, (hipStream_t)stream);
   SVG_HIP(hipGetLastError());
+  return SVG_OK;
+}
+
+int svg_spmv32_tune(svg_lhs* L, const float* K32, float* U32, double* KU, int* best_variant, void* stream)
+{
+  if (!L || !K32 || !U32 || !KU) return set_error(SVG_ERR_ARG, "svg_spmv32_tune: null argument");
+  int rc = spmv32_tune(L, K32, U32, KU, (hipStream_t)stream);
+  if (rc) return rc;
+  if (best_variant) *best_variant = L->spmv32Variant;
+  return SVG_OK;
+}
+
+int svg_spmv_variants(const svg_lhs* L, int* fp64_variant, int* fp32_variant)
+{
+  if (!L) return set_error(SVG_ERR_ARG, "svg_spmv_variants: null lhs");
+  if (fp64_variant) *fp64_variant = L->spmvVariant;
+  if (fp32_variant) *fp32_variant = L->spmv32Variant;
   return SVG_OK;
 }
 
