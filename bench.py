@@ -49,10 +49,16 @@ KERNELS = ["spmv_bcsr", "spmv32", "d2f", "mdot", "mdot_fin", "gs_update", "div",
 HBM_KERNELS = ["spmv_bcsr", "spmv32", "lpc_msg", "lpc_dense", "lpc_encode"]
 ROOFLINE_KERNELS = ["spmv_bcsr", "spmv32"]  # timed with events inside the timed region
 DEFAULT_WEIGHTS = os.path.join(ROOT, "svfsiplus_amd", "data", "lpc_weights.npz")
-# HBM bytes per launch on the 1.2M-tet pipe (n = 34), measured with rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE on `bench.py --kernel-only spmv|spmv32`, each scaled by the same counter's reading on a
-# device copy of known size (FETCH_SIZE reports half of a copy's reads, WRITE_SIZE all of its writes).
-MEASURED_TRAFFIC_N34 = {"spmv_bcsr": 448.1e6, "spmv32": 287.0e6}
+# HBM bytes per launch on the 1.2M-tet pipe (n = 34) for each launch variant of the two SpMV kernels
+# (the autotuners' variant indices), measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on
+# `bench.py --kernel-only spmv32` (which times every variant of both kernels), each scaled by the same
+# counter's reading on a device copy of known size (FETCH_SIZE reports half of a copy's reads,
+# WRITE_SIZE all of its writes).
+MEASURED_TRAFFIC_N34 = {
+    "spmv_bcsr": {0: 477.6e6, 1: 461.4e6, 2: 479.3e6, 3: 434.6e6, 4: 425.5e6, 5: 444.7e6, 6: 514.0e6, 7: 480.8e6,
+                  8: 478.4e6, 9: 450.3e6, 10: 459.9e6, 11: 429.0e6},
+    "spmv32": {0: 287.1e6, 1: 286.1e6, 2: 269.9e6, 3: 258.4e6, 4: 245.4e6, 5: 246.0e6},
+}
 
 
 def kernel_table(lib):
@@ -246,10 +252,11 @@ def timed_run(fn, lib, steps, warmup, prof, label):
             "kernels": ktab, "kernels_warmup": warm_tab, "Yn": fn.Yn.clone(), "An": fn.An.clone()}
 
 
-def roofline(ktab, measured=None):
+def roofline(ktab, measured=None, variants=None):
     """Roofline of the kernel with the largest total time among those with
     algorithmic byte counts: achieved = bytes per launch / average duration;
-    traffic = the counter-measured HBM bytes per launch where known."""
+    traffic = the counter-measured HBM bytes per launch of the launch variant
+    the autotuner picked, where known."""
     cands = [(v["total_ms"], k) for k, v in ktab.items() if k in HBM_KERNELS and v.get("bytes")]
     if not cands:
         return None
@@ -259,9 +266,16 @@ def roofline(ktab, measured=None):
     bpl = v["bytes"] / v["count"]
     ach = bpl / avg_s / 1e9
     return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": (measured or {}).get(k), "kernel": k,
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": _traffic(measured, variants, k), "kernel": k,
             "bytes_per_launch": round(bpl),
             "avg_us": round(avg_s * 1e6, 2)}
+
+
+def _traffic(measured, variants, k):
+    if not measured or not variants or k not in measured:
+        return None
+    v = variants.get({"spmv_bcsr": "fp64", "spmv32": "fp32"}[k], -1)
+    return measured[k].get(v)
 
 
 def get_network(a, log):
@@ -443,7 +457,7 @@ def main():
                                f"residual of the scaled system)",
                    "method": head, "solver": solver[head], "newton_records": R["hists"][-1],
                    "spmv_variants": spmv_variants(lib, fn.lhs)},
-        "roofline": roofline(R["kernels"], traffic),
+        "roofline": roofline(R["kernels"], traffic, spmv_variants(lib, fn.lhs)),
         "cpu_baseline": cpu,
         "kernels": R["kernels_warmup"] or R["kernels"],
         "kernels_note": "per-kernel event table of the untimed warmup step; the timed steps bracket only "
@@ -452,7 +466,7 @@ def main():
     if "plain" in runs and head != "plain":
         P = runs["plain"]
         out["plain"] = {"value": round(P["s_per_newton"], 6), "newton_records": P["hists"][-1],
-                        "roofline": roofline(P["kernels"], traffic)}
+                        "roofline": roofline(P["kernels"], traffic, spmv_variants(lib, fn.lhs))}
         out["speedup_vs_plain"] = round(P["s_per_newton"] / R["s_per_newton"], 3)
         # the time step's result against the FP64 plain path's (velocity, pressure, acceleration)
         def rel(x, ref):
