@@ -296,7 +296,7 @@ static void spmv32_launch(const svg_lhs* L, const float* K32, const float* U32, 
   unsigned nb = blocks_for((size_t)4 * L->nNo, 256);
   const int use_xcd = (xcd && nb >= 64) ? 1 : 0;
   if (use_xcd) nb = (nb + 7) & ~7u;
-  const double bytes = 68.0 * L->nnz + 36.0 * L->nNo;
+  const double bytes = 68.0 * L->nnz + 52.0 * L->nNo;  // block + col; rowPtr, U (FP32), W (FP64)
   if (sysOfNode)
     launch_b(name, bytes, k_bspmv32<UNR>, dim3(nb), dim3(256), 0, s, L->nNo, (const int*)L->d_rowPtr,
              (const int*)L->d_colPtr, K32, U32, W, sysOfNode, st, use_xcd);
