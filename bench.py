@@ -42,12 +42,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-KERNELS = ["spmv_bcsr", "spmv32", "d2f", "mdot", "mdot_fin", "gs_update", "div", "backsolve", "xupdate", "pc_w", "pc_val",
-           "fluid_tet4", "pici", "picc", "fill", "wc_scale", "set_int", "spmv_tune", "lpc_prep", "lpc_encode",
-           "lpc_msg", "lpc_dense"]
+KERNELS = ["spmv_bcsr", "spmv32", "d2f", "mdot", "mdot_fin", "gs_update", "zcopy", "div", "backsolve", "xupdate",
+           "pc_w", "pc_val", "fluid_tet4", "b_fluid", "pici", "picc", "fill", "wc_scale", "set_int", "spmv_tune",
+           "spmv32_tune", "lpc_prep", "lpc_encode", "lpc_msg", "lpc_dense", "add_bc_mul", "face_valM", "rsub"]
 # kernels whose launches carry algorithmic byte counts (roofline candidates)
 HBM_KERNELS = ["spmv_bcsr", "spmv32", "lpc_msg", "lpc_dense", "lpc_encode"]
 ROOFLINE_KERNELS = ["spmv_bcsr", "spmv32"]  # timed with events inside the timed region
+PROF_EVERY = 16  # ... one launch in 16 of each (events between launches cost time)
 DEFAULT_WEIGHTS = os.path.join(ROOT, "svfsiplus_amd", "data", "lpc_weights.npz")
 # HBM bytes per launch on the 1.2M-tet pipe (n = 34) for each launch variant of the two SpMV kernels
 # (the autotuners' variant indices), measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on
@@ -214,8 +215,9 @@ def spmv_variants(lib, lhs):
 def timed_run(fn, lib, steps, warmup, prof, label):
     """Warmup (every named kernel timed with events: the per-kernel table),
     then `steps` time steps timed with a device synchronisation on both sides,
-    in which events bracket only the block SpMV kernels the roofline reports
-    on (events between launches cost time; prof=False: none at all)."""
+    in which events bracket only one launch in PROF_EVERY of the block SpMV
+    kernels the roofline reports on (events between launches cost time;
+    prof=False: none at all)."""
     import torch
     warm_tab = {}
     for w in range(warmup):
@@ -230,6 +232,7 @@ def timed_run(fn, lib, steps, warmup, prof, label):
                                                  for x in h), file=sys.stderr, flush=True)
     lib.svg_prof_reset()
     lib.svg_prof_only(",".join(ROOFLINE_KERNELS).encode())
+    lib.svg_prof_sample(PROF_EVERY)
     lib.svg_prof_enable(1 if prof else 0)
     hists = []
     torch.cuda.synchronize()
@@ -241,6 +244,7 @@ def timed_run(fn, lib, steps, warmup, prof, label):
     t1 = time.perf_counter()
     lib.svg_prof_enable(0)
     lib.svg_prof_only(None)
+    lib.svg_prof_sample(1)
     ktab = _bytes(lib, kernel_table(lib))
     for k, h in enumerate(hists):
         print(f"# {label} step {k}: " + " ".join(
@@ -460,8 +464,8 @@ def main():
         "roofline": roofline(R["kernels"], traffic, spmv_variants(lib, fn.lhs)),
         "cpu_baseline": cpu,
         "kernels": R["kernels_warmup"] or R["kernels"],
-        "kernels_note": "per-kernel event table of the untimed warmup step; the timed steps bracket only "
-                        + ", ".join(ROOFLINE_KERNELS) + " with events",
+        "kernels_note": "per-kernel event table of the untimed warmup step; the timed steps bracket one launch "
+                        "in %d of %s with events (the roofline's durations)" % (PROF_EVERY, ", ".join(ROOFLINE_KERNELS)),
     }
     if "plain" in runs and head != "plain":
         P = runs["plain"]

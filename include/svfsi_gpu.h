@@ -363,6 +363,9 @@ int svg_prof_enable(int on);
  * list (NULL or "": every named kernel). Events cost time between launches,
  * so a timed run brackets only the kernels it reports on. */
 int svg_prof_only(const char* names);
+/* Brackets only one launch in `every` of each timed kernel (1: all); the
+ * count and summed duration then cover the sampled launches. */
+int svg_prof_sample(int every);
 int svg_prof_reset(void);
 int svg_prof_get(const char* name, int* count, double* total_ms);
 /* Summed algorithmic bytes of the named kernel's timed launches (where the

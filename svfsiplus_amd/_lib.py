@@ -115,6 +115,7 @@ _SIGS = {
     "svg_set_bc_dir": (_I, [_I, _P, _I, _I, _I, C.c_uint, _P, _D, _D, _P, _P, _P]),
     "svg_prof_enable": (_I, [_I]),
     "svg_prof_only": (_I, [C.c_char_p]),
+    "svg_prof_sample": (_I, [_I]),
     "svg_prof_reset": (_I, []),
     "svg_prof_get": (_I, [C.c_char_p, _IP, _DP]),
     "svg_prof_get_bytes": (_I, [C.c_char_p, _DP]),

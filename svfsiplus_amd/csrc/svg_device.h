@@ -45,6 +45,17 @@ inline void launch_b(const char* name, double bytes, K kernel, dim3 grid, dim3 b
 
 inline unsigned blocks_for(size_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
 
+// Workgroups are dispatched round-robin over the 8 XCDs; remap so that each
+// XCD walks one contiguous band of rows and its L2 keeps the entries its rows
+// gather (-1: a padding workgroup of a grid rounded up to a multiple of 8).
+__device__ __forceinline__ int xcd_block(int b, int nb)
+{
+  const int per = (nb + 7) >> 3;
+  const int xcd = b & 7, idx = b >> 3;
+  const int r = xcd * per + idx;
+  return r < nb ? r : -1;
+}
+
 __device__ __forceinline__ double wave_sum(double v)
 {
   // Fixed butterfly over the 64 lanes of a wavefront.
@@ -156,7 +167,7 @@ int lpc_apply_dev(svg_lpc* N, const svg_lhs* L, const double* V, double* Z, floa
                   hipStream_t s, const double2* nsc = nullptr, const int* sysOf = nullptr,
                   const GmState* st = nullptr);
 // Every buffer lpc_prepare_dev / lpc_apply_dev use on L, allocated ahead (graph capture).
-int lpc_reserve(svg_lpc* N, svg_lhs* L);
+int lpc_reserve(svg_lpc* N, svg_lhs* L, hipStream_t s);
 // construct_fluid split in two: the launches alone (no host synchronisation, so
 // they can be captured into a graph; the zero-Jacobian flag stays on the device)
 // and the check of that flag, which synchronises s.
@@ -191,6 +202,7 @@ struct svg_lhs {
   bool maskBuilt = false;
   int spmvVariant = -1;  // 4x4 SpMV launch variant picked by the autotuner (-1: not tuned)
   int spmv32Variant = -1;  // FP32 4x4 SpMV launch variant picked by the autotuner (-1: not tuned)
+  int msgVariant = -1;     // network message-gather launch: 0 round-robin, 1 XCD bands (-1: not tuned)
   float* d_val32 = nullptr;  // mixed precision: FP32 copy of the scaled Val
   unsigned short* d_val16 = nullptr;   // learned preconditioner: BF16 copy of the scaled Val
   unsigned short* d_diag16 = nullptr;  // learned preconditioner: BF16 diagonal blocks (16, nNo)

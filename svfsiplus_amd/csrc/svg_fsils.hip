@@ -39,15 +39,6 @@ constexpr int RED_SPAN = RED_BS * RED_EPT;
 constexpr int LOOKAHEAD = 3;  // Arnoldi steps enqueued ahead of the host's convergence check
 
 // ------------------------------------------------------------------ SpMV --
-// Blocks are dispatched round-robin over the 8 XCDs; remap so that each XCD
-// walks one contiguous band of rows and its L2 keeps the gathered U entries.
-__device__ __forceinline__ int xcd_block(int b, int nb)
-{
-  const int per = (nb + 7) >> 3;
-  const int xcd = b & 7, idx = b >> 3;
-  const int r = xcd * per + idx;
-  return r < nb ? r : -1;
-}
 
 // KU = K U (MODE 0) or KU = B - K U (MODE 1) for BR x BC blocks stored
 // row-major (K[BR*BC*j + BC*i + c]): vv is BR = BC = dof, sv is BR = dof,
@@ -587,7 +578,7 @@ __global__ __launch_bounds__(FIN_BS) void k_fin(FinArgs A, const double* __restr
 // u_{i+1} = (u_{i+1} - sum_j coef_j u_j) * scale, sequentially in j (omp_sum_v, omp_mul_v)
 __global__ __launch_bounds__(256) void k_gs_update(int n, double* __restrict__ u, size_t ld, int i,
                                                    const double* __restrict__ coef, const GmState* __restrict__ st,
-                                                   float* __restrict__ u32)
+                                                   float* __restrict__ u32, double* __restrict__ z)
 {
 #pragma clang fp contract(off)
   if (st->done) return;
@@ -599,6 +590,7 @@ __global__ __launch_bounds__(256) void k_gs_update(int n, double* __restrict__ u
   v = st->scale * v;
   w[k] = v;
   if (u32) u32[k] = (float)v;
+  if (z) z[k] = v;  // flexible solve: the next step is one the network does not precondition
 }
 
 // z_i = u_i (and its FP32 copy): an Arnoldi step of the flexible solve that
@@ -1252,10 +1244,10 @@ static int run_cycle(const Op& A, KrylovWork& k, int sD, FinArgs fa, double* X, 
       if (net_period <= 1 || i % net_period == 0) {
         int rc = lpc_apply_dev(net, A.L, k.u + k.ld * i, k.z + k.ld * i, mixed ? k.u32 : nullptr, &k.st->done, s);
         if (rc) return rc;
-      } else {
+      } else if (i == 0) {
         launch("zcopy", k_zcopy, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, (const double*)(k.u + k.ld * i),
                k.z + k.ld * i, mixed ? k.u32 : (float*)nullptr, (const GmState*)k.st);
-      }
+      }  // else: the previous step's Gram-Schmidt update wrote z_i = u_i (and u32)
       sin = k.z + k.ld * i;
     }
     if (mixed) {
@@ -1269,8 +1261,11 @@ static int run_cycle(const Op& A, KrylovWork& k, int sD, FinArgs fa, double* X, 
                           k.sn, k.err, k.coef, s, k.d_done);
       if (rc) return rc;
     }
+    // flexible: when the next step has no network, this update also writes z_{i+1} = u_{i+1}
+    const bool nextPlain = flex && i + 1 < sD && !(net_period <= 1 || (i + 1) % net_period == 0);
     launch("gs_update", k_gs_update, dim3(nbv), dim3(256), 0, s, n, k.u, k.ld, i, (const double*)k.coef,
-           (const GmState*)k.st, (mixed && !flex) ? k.u32 : (float*)nullptr);
+           (const GmState*)k.st, (mixed && (!flex || nextPlain)) ? k.u32 : (float*)nullptr,
+           nextPlain ? k.z + k.ld * (i + 1) : (double*)nullptr);
     SVG_HIP(hipEventRecord(k.ev[i % 8], s));
   }
   launch("backsolve", k_backsolve, dim3(1), dim3(256), 0, s, sD, (const double*)k.h, (const double*)k.err, k.y, k.st);
@@ -1429,6 +1424,7 @@ static int gmres_v_learned(svg_lhs* L, svg_subls* ls, const double* Val, double*
   }
   const Op A{L, Val, 4};
   if (L->spmvVariant < 0 && (rc = svg_spmv_tune(L, Val, nullptr, s))) return rc;
+  if ((rc = lpc_reserve(net, L, s))) return rc;
   auto t0 = std::chrono::steady_clock::now();
   if ((rc = lpc_prepare_dev(L, Val, s))) return rc;
   if (mixed) {
@@ -1995,7 +1991,8 @@ __global__ __launch_bounds__(FIN_BS) void k_bfin(FinArgs A, const int* __restric
 
 __global__ __launch_bounds__(256) void k_bgs_update(int n, double* __restrict__ u, size_t ld, int i, int sD,
                                                     const double* __restrict__ coef, const GmState* __restrict__ st,
-                                                    const int* __restrict__ sysOfNode, float* __restrict__ u32)
+                                                    const int* __restrict__ sysOfNode, float* __restrict__ u32,
+                                                    double* __restrict__ z)
 {
 #pragma clang fp contract(off)
   const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -2010,6 +2007,7 @@ __global__ __launch_bounds__(256) void k_bgs_update(int n, double* __restrict__ 
   v = S.scale * v;
   w[k] = v;
   if (u32) u32[k] = (float)v;
+  if (z) z[k] = v;  // flexible solve: the next step is one the network does not precondition
 }
 
 __global__ __launch_bounds__(256) void k_bdiv(int n, double* __restrict__ u, const GmState* __restrict__ st,
@@ -2157,10 +2155,10 @@ static int batch_cycle(svg_batch* B, const double* Val, const double* R, const B
         if (rc) return rc;
         hipLaunchKernelGGL(k_bzcopy, dim3(nbv), dim3(256), 0, s, n, (const double*)ui, zi, z32,
                            (const GmState*)B->st, (const int*)B->d_sysOfNode, 1);
-      } else {
+      } else if (i == 0) {
         hipLaunchKernelGGL(k_bzcopy, dim3(nbv), dim3(256), 0, s, n, (const double*)ui, zi, z32,
                            (const GmState*)B->st, (const int*)B->d_sysOfNode, 0);
-      }
+      }  // else: the previous step's Gram-Schmidt update wrote z_i = u_i (and u32)
       sin = zi;
     }
     if (mixed)
@@ -2172,8 +2170,10 @@ static int batch_cycle(svg_batch* B, const double* Val, const double* R, const B
     fa.mode = FIN_ARNOLDI;
     fa.i = i;
     batch_fin(B, fa, i + 2, s);
+    const bool nextPlain = flex && i + 1 < sD && !(md.period <= 1 || (i + 1) % md.period == 0);
     hipLaunchKernelGGL(k_bgs_update, dim3(nbv), dim3(256), 0, s, n, B->u, B->ld, i, sD, (const double*)B->coef,
-                       (const GmState*)B->st, (const int*)B->d_sysOfNode, u32w);
+                       (const GmState*)B->st, (const int*)B->d_sysOfNode, (nextPlain && mixed) ? B->u32 : u32w,
+                       nextPlain ? B->z + B->ld * (i + 1) : (double*)nullptr);
   }
   hipLaunchKernelGGL(k_bbacksolve, dim3(B->nsys), dim3(256), 0, s, sD, (const double*)B->h, (const double*)B->err,
                      B->y, B->st);
@@ -2271,7 +2271,7 @@ static int batch_prepare(svg_batch* B, const double* Val, const BatchMode& md, h
       if ((rc = dev_alloc(&B->d_nsc, (size_t)L->nNo))) return rc;
       SVG_HIP(hipMemcpy(B->d_nsc, sc.data(), sizeof(double2) * L->nNo, hipMemcpyHostToDevice));
     }
-    if ((rc = lpc_reserve(md.net, L))) return rc;
+    if ((rc = lpc_reserve(md.net, L, caller))) return rc;
   }
   return SVG_OK;
 }

@@ -235,10 +235,15 @@ template <int UNR>
 __global__ __launch_bounds__(256) void k_msg(int n, int rows, const int* __restrict__ rowPtr,
                                              const int* __restrict__ colPtr, const bf16* __restrict__ A16,
                                              const bf16* __restrict__ H, bf16* __restrict__ M,
-                                             const int* __restrict__ skip, RowGate gate)
+                                             const int* __restrict__ skip, RowGate gate, int xcd)
 {
   if (skip && *skip) return;
-  const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+  int bx = blockIdx.x;
+  if (xcd) {
+    bx = xcd_block(blockIdx.x, gridDim.x);
+    if (bx < 0) return;
+  }
+  const size_t idx = (size_t)bx * 256 + threadIdx.x;
   const int r = (int)(idx >> 2), g = (int)(idx & 3);
   if (r >= rows) return;
   const int b = r / n, i = r - b * n;
@@ -763,7 +768,9 @@ static int lpc_forward(svg_lpc* N, const svg_lhs* L, int nvec, const double* V, 
   // algorithmic bytes: V and the diagonal blocks read, h written
   launch_b("lpc_encode", (32.0 + 32.0 + 2.0 * F) * rows, k_encode, dim3(std::min(nt, 2048u)), dim3(BS), 0, s, n,
            rows, V, sc, nsc, (const bf16*)L->d_diag16, pb + OFF_W0, N->H[0], skip, gate);
-  const unsigned nm = blocks_for((size_t)rows * 4, 256);
+  unsigned nm = blocks_for((size_t)rows * 4, 256);
+  const int mx = (L->msgVariant == 1 && nm >= 64) ? 1 : 0;
+  if (mx) nm = (nm + 7) & ~7u;
   for (int l = 0; l < NL; l++) {
     const bool last = l == NL - 1;
     const bf16* Hin = train ? N->H[l] : N->H[l & 1];
@@ -771,7 +778,8 @@ static int lpc_forward(svg_lpc* N, const svg_lhs* L, int nvec, const double* V, 
     bf16* Ml = train ? N->M[l] : N->M[0];
     // BF16 blocks + colPtr per block (per sample), rowPtr + h + m out per row
     launch_b("lpc_msg", (32.0 + 4.0) * L->nnz * nvec + (4.0 + 2.0 * F) * rows, k_msg<4>, dim3(nm), dim3(256), 0, s, n,
-             rows, (const int*)L->d_rowPtr, (const int*)L->d_colPtr, (const bf16*)L->d_val16, Hin, Ml, skip, gate);
+             rows, (const int*)L->d_rowPtr, (const int*)L->d_colPtr, (const bf16*)L->d_val16, Hin, Ml, skip, gate,
+             mx);
     // h, m in, h out (+ p for training; + V in, z out for the last layer)
     const double db = 6.0 * F * rows + (train ? 2.0 * F * rows : 0.0) + (last ? (64.0 + (Z32 ? 16.0 : 0.0)) * rows : 0.0);
     launch_b("lpc_dense", db, k_dense, dim3(std::min(nt, 2048u)), dim3(BS), 0, s, rows, Hin, (const bf16*)Ml, Hout,
@@ -791,10 +799,44 @@ int lpc_apply_dev(svg_lpc* N, const svg_lhs* L, const double* V, double* Z, floa
   return lpc_forward(N, L, 1, V, Z, Z32, false, skip, s, nsc, sysOf, st);
 }
 
-int lpc_reserve(svg_lpc* N, svg_lhs* L)
+int lpc_reserve(svg_lpc* N, svg_lhs* L, hipStream_t s)
 {
   int rc = lhs_bf16_alloc(L);
-  return rc ? rc : lpc_ensure(N, L->nNo, false);
+  if (rc || (rc = lpc_ensure(N, L->nNo, false))) return rc;
+  if (L->msgVariant >= 0) return SVG_OK;
+  // the message gather's launch (round-robin workgroups or one row band per
+  // XCD; identical results), timed once per lhs on its buffers (values do not matter)
+  SVG_HIP(hipMemsetAsync(N->H[0], 0, sizeof(bf16) * (size_t)F * L->nNo, s));
+  SVG_HIP(hipMemsetAsync(L->d_val16, 0, sizeof(bf16) * 16 * (size_t)L->nnz, s));
+  hipEvent_t a, b;
+  SVG_HIP(hipEventCreate(&a));
+  SVG_HIP(hipEventCreate(&b));
+  const unsigned nm0 = blocks_for((size_t)L->nNo * 4, 256);
+  float best = 1e30f;
+  int bv = 0;
+  for (int v = 0; v < 2; v++) {
+    const int x = (v == 1 && nm0 >= 64) ? 1 : 0;
+    const unsigned nm = x ? ((nm0 + 7) & ~7u) : nm0;
+    for (int r = 0; r < 4; r++) {
+      if (r == 1) SVG_HIP(hipEventRecord(a, s));
+      launch("lpc_msg_tune", k_msg<4>, dim3(nm), dim3(256), 0, s, L->nNo, L->nNo, (const int*)L->d_rowPtr,
+             (const int*)L->d_colPtr, (const bf16*)L->d_val16, (const bf16*)N->H[0], N->M[0], (const int*)nullptr,
+             lpc::RowGate{nullptr, nullptr}, x);
+    }
+    SVG_HIP(hipEventRecord(b, s));
+    SVG_HIP(hipEventSynchronize(b));
+    float ms = 0.f;
+    SVG_HIP(hipEventElapsedTime(&ms, a, b));
+    if (ms < best) {
+      best = ms;
+      bv = v;
+    }
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  SVG_HIP(hipGetLastError());
+  L->msgVariant = bv;
+  return SVG_OK;
 }
 
 // Loss and gradient of L = (1/B) sum_b |R_b - A M(R_b)|^2 into N->g.

@@ -37,6 +37,8 @@ struct ProfPending {
 static std::mutex g_prof_mu;
 static bool g_prof_on = false;
 static std::vector<std::string> g_prof_only;  // empty: every named launch is timed
+static int g_prof_every = 1;                  // time one launch in this many of each kernel
+static std::map<std::string, long> g_prof_calls;
 
 bool prof_set(bool on)
 {
@@ -67,6 +69,7 @@ int prof_begin(const char* name, hipStream_t s, double bytes)
   if (!g_prof_only.empty() &&
       std::find(g_prof_only.begin(), g_prof_only.end(), std::string(name)) == g_prof_only.end())
     return -1;
+  if (g_prof_every > 1 && (g_prof_calls[name]++ % g_prof_every) != 0) return -1;
   ProfPending p;
   p.name = name;
   p.bytes = bytes;
@@ -201,6 +204,15 @@ int svg_prof_only(const char* names)
       cur += *c;
     }
   }
+  return SVG_OK;
+}
+
+int svg_prof_sample(int every)
+{
+  if (every < 1) return set_error(SVG_ERR_ARG, "svg_prof_sample: every must be >= 1");
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  g_prof_every = every;
+  g_prof_calls.clear();
   return SVG_OK;
 }
 
