@@ -42,7 +42,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-KERNELS = ["spmv_bcsr", "spmv32", "d2f", "mdot", "mdot_fin", "gs_update", "zcopy", "div", "backsolve", "xupdate",
+KERNELS = ["spmv_bcsr", "spmv32", "d2f", "mdot", "mdot_cb", "mdot_fin", "gs_update", "gs_update_cb", "zcopy", "div",
+           "backsolve", "xupdate", "xupdate_cb",
            "pc_w", "pc_val", "fluid_tet4", "b_fluid", "pici", "picc", "fill", "wc_scale", "set_int", "spmv_tune",
            "spmv32_tune", "lpc_prep", "lpc_encode", "lpc_msg", "lpc_dense", "add_bc_mul", "face_valM", "rsub"]
 # kernels whose launches carry algorithmic byte counts (roofline candidates)

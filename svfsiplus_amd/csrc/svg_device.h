@@ -135,7 +135,8 @@ struct KrylovWork {
   int* h_done = nullptr;      // mapped host flag written by the device tail
   int* d_done = nullptr;      // device view of h_done
   hipEvent_t ev[8] = {};      // per-step completion events (look-ahead ring)
-  float* u32 = nullptr;       // mixed precision: FP32 copy of the current SpMV input
+  float* u32 = nullptr;       // mixed precision, flexible: the network's FP32 z_i (the SpMV input)
+  float* v32 = nullptr;       // mixed precision: the Krylov basis in FP32, (sD+1) vectors at stride ld
   double* z = nullptr;        // flexible GMRES: preconditioned basis z_j = M(u_j), sD vectors
   double* scal = nullptr;     // scratch scalars for standalone dots
 };
@@ -224,7 +225,8 @@ struct svg_batch {
          *y = nullptr, *partial = nullptr;
   svg::GmState* st = nullptr;
   svg::GmState* h_st = nullptr;           // pinned mirror (nsys)
-  float* u32 = nullptr;
+  float* u32 = nullptr;                   // mixed, learned: the network's FP32 z_i (the SpMV input)
+  float* v32 = nullptr;                   // mixed precision: the Krylov basis in FP32 ((sD+1) vectors)
   double* z = nullptr;                    // learned preconditioner: flexible basis z_j (sD vectors)
   double2* d_nsc = nullptr;               // per node: (sqrt(nNo_q), 1 / sqrt(nNo_q)) of its system q
   std::vector<int> fbItr;                 // per system: iteration count at the fallback (-1: none)

@@ -358,6 +358,86 @@ struct FinArgs {
 constexpr int MAXM = 512;  // sums per reduction (Krylov dimension + 2)
 constexpr double FB_DROP = 1e-4;  // learned preconditioner: smallest relative decrease that counts
 
+// Pass 1 of the dots of w against m vectors base + j*ld (j < m) and, with
+// self, w.w as column m, over one span [k0, kend) of 1024 elements: each
+// lane sums its 4 elements (stride 256) in order, each wavefront butterflies,
+// and lane 0 of wavefront wv leaves column j's sum in lds[wv * mc + j].
+// Loads are unconditional (indices clamped into the span; the clamped values
+// are not used) and G columns are loaded and reduced together, so one memory
+// latency is spent per G columns (FP32 columns, half the bytes of FP64 ones,
+// need more in flight to keep HBM busy).
+template <int G, typename T>
+__device__ __forceinline__ void span_dots(const double* __restrict__ w, const T* __restrict__ base, size_t ld, int m,
+                                          bool self, size_t k0, size_t kend, double* lds, int mc)
+{
+#pragma clang fp contract(off)
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (kend == 0) {  // an empty vector (no element to clamp to): every sum is 0
+    if (lane == 0)
+      for (int j = 0; j < mc; j++) lds[wv * mc + j] = 0.0;
+    return;
+  }
+  size_t kc[RED_EPT];
+  bool in[RED_EPT];
+  double wr[RED_EPT];
+#pragma unroll
+  for (int r = 0; r < RED_EPT; r++) {
+    const size_t k = k0 + (size_t)r * RED_BS;
+    in[r] = k < kend;
+    kc[r] = in[r] ? k : kend - 1;
+    const double v = w[kc[r]];
+    wr[r] = in[r] ? v : 0.0;
+  }
+  int j = 0;
+  for (; G > 1 && j + G <= m; j += G) {
+    T c[G][RED_EPT];
+#pragma unroll
+    for (int q = 0; q < G; q++)
+#pragma unroll
+      for (int r = 0; r < RED_EPT; r++) c[q][r] = base[(size_t)(j + q) * ld + kc[r]];
+    double acc[G];
+#pragma unroll
+    for (int q = 0; q < G; q++) {
+      acc[q] = 0.0;
+#pragma unroll
+      for (int r = 0; r < RED_EPT; r++) {
+        const double v = in[r] ? (double)c[q][r] : 0.0;
+        acc[q] = acc[q] + v * wr[r];
+      }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+      for (int q = 0; q < G; q++) acc[q] += __shfl_xor(acc[q], o, 64);  // wave_sum's butterfly
+    if (lane == 0) {
+#pragma unroll
+      for (int q = 0; q < G; q++) lds[wv * mc + j + q] = acc[q];
+    }
+  }
+  for (; j < m; j++) {
+    T c[RED_EPT];
+#pragma unroll
+    for (int r = 0; r < RED_EPT; r++) c[r] = base[(size_t)j * ld + kc[r]];
+    double acc = 0.0;
+#pragma unroll
+    for (int r = 0; r < RED_EPT; r++) {
+      const double v = in[r] ? (double)c[r] : 0.0;
+      acc = acc + v * wr[r];
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) lds[wv * mc + j] = acc;
+  }
+  if (self) {
+    double acc = 0.0;
+#pragma unroll
+    for (int r = 0; r < RED_EPT; r++) acc = acc + wr[r] * wr[r];
+    acc = wave_sum(acc);
+    if (lane == 0) lds[wv * mc + m] = acc;
+  }
+}
+
+constexpr int MDOT_CB_G = 4;  // FP32 basis columns per load batch (4: 24.1 us, 8: 25.1 us on the 1.2M-tet pipe)
+
 // Dots of w against m vectors base + j*ld (j < m), pass 1: each workgroup
 // sums its 1024-element span (256 lanes x 4 elements at stride 256, in order),
 // butterflies inside each wavefront and adds the 4 wavefront sums in order;
@@ -369,31 +449,34 @@ __global__ __launch_bounds__(RED_BS) void k_mdot(int n, const double* __restrict
 #pragma clang fp contract(off)
   extern __shared__ double lds[];  // [RED_NW waves][m]
   if (skip && *skip) return;
-  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int t = threadIdx.x;
   const int nblk = gridDim.x;
-  const size_t k0 = (size_t)blockIdx.x * RED_SPAN + t;
-  double wr[RED_EPT];
-#pragma unroll
-  for (int r = 0; r < RED_EPT; r++) {
-    size_t k = k0 + (size_t)r * RED_BS;
-    wr[r] = k < (size_t)n ? w[k] : 0.0;
-  }
-  for (int j = 0; j < m; j++) {
-    const double* uj = base + (size_t)j * ld;
-    double acc = 0.0;
-#pragma unroll
-    for (int r = 0; r < RED_EPT; r++) {
-      size_t k = k0 + (size_t)r * RED_BS;
-      double v = k < (size_t)n ? uj[k] : 0.0;
-      acc = acc + v * wr[r];
-    }
-    acc = wave_sum(acc);
-    if (lane == 0) lds[wv * m + j] = acc;
-  }
+  span_dots<1>(w, base, ld, m, false, (size_t)blockIdx.x * RED_SPAN + t, (size_t)n, lds, m);
   __syncthreads();
   for (int j = t; j < m; j += RED_BS) {
     double v = lds[j];
     for (int q = 1; q < RED_NW; q++) v = v + lds[q * m + j];
+    partial[(size_t)j * nblk + blockIdx.x] = v;
+  }
+}
+
+// The mixed-precision solve keeps its Krylov basis in FP32 (a compressed
+// basis; the arithmetic stays FP64): dots of w against the m FP32 vectors
+// base + j*ld, then w.w as column m, in k_mdot's order; partial[j*nblk + b].
+__global__ __launch_bounds__(RED_BS) void k_mdot_cb(int n, const double* __restrict__ w, const float* __restrict__ base,
+                                                    size_t ld, int m, double* __restrict__ partial,
+                                                    const int* __restrict__ skip)
+{
+#pragma clang fp contract(off)
+  extern __shared__ double lds[];  // [RED_NW waves][m + 1]
+  if (skip && *skip) return;
+  const int t = threadIdx.x;
+  const int nblk = gridDim.x, mc = m + 1;
+  span_dots<MDOT_CB_G>(w, base, ld, m, true, (size_t)blockIdx.x * RED_SPAN + t, (size_t)n, lds, mc);
+  __syncthreads();
+  for (int j = t; j < mc; j += RED_BS) {
+    double v = lds[j];
+    for (int q = 1; q < RED_NW; q++) v = v + lds[q * mc + j];
     partial[(size_t)j * nblk + blockIdx.x] = v;
   }
 }
@@ -593,6 +676,32 @@ __global__ __launch_bounds__(256) void k_gs_update(int n, double* __restrict__ u
   if (z) z[k] = v;  // flexible solve: the next step is one the network does not precondition
 }
 
+// Compressed basis (mixed precision): u_{i+1} = (w - sum_j coef_j v_j) * scale
+// in FP64 with the FP32 basis vectors v_j, two elements per lane (n is even);
+// the result is stored as the FP32 basis vector v_{i+1} (the next SpMV input)
+// and, with keep64, in FP64 over w for a step that reads it (the network's
+// input, a coupled face's rank-one term).
+__global__ __launch_bounds__(256) void k_gs_update_cb(int n, double* __restrict__ w, float* __restrict__ v32,
+                                                      size_t ld, int i, const double* __restrict__ coef,
+                                                      const GmState* __restrict__ st, int keep64)
+{
+#pragma clang fp contract(off)
+  if (st->done) return;
+  const size_t k = 2 * ((size_t)blockIdx.x * blockDim.x + threadIdx.x);
+  if (k >= (size_t)n) return;
+  double2 v = *reinterpret_cast<const double2*>(w + k);
+  for (int j = 0; j <= i; j++) {
+    const float2 b = *reinterpret_cast<const float2*>(v32 + (size_t)j * ld + k);
+    const double c = -coef[j];
+    v.x = v.x + c * (double)b.x;
+    v.y = v.y + c * (double)b.y;
+  }
+  v.x = st->scale * v.x;
+  v.y = st->scale * v.y;
+  *reinterpret_cast<float2*>(v32 + (size_t)(i + 1) * ld + k) = make_float2((float)v.x, (float)v.y);
+  if (keep64) *reinterpret_cast<double2*>(w + k) = v;
+}
+
 // z_i = u_i (and its FP32 copy): an Arnoldi step of the flexible solve that
 // the network does not precondition.
 __global__ __launch_bounds__(256) void k_zcopy(int n, const double* __restrict__ u, double* __restrict__ z,
@@ -689,6 +798,33 @@ __global__ __launch_bounds__(256) void k_xupdate(int n, double* __restrict__ X, 
   double x = X[k];
   for (int j = 0; j <= last_i; j++) x = x + y[j] * u[(size_t)j * ld + k];
   X[k] = x;
+}
+
+// Compressed basis: X += sum_j y_j b_j, sequentially in j, with b_j = z_j
+// (FP64) on the steps the network preconditioned (z != null, j % period == 0)
+// and the FP32 basis vector v_j otherwise; two elements per lane.
+__global__ __launch_bounds__(256) void k_xupdate_cb(int n, double* __restrict__ X, const float* __restrict__ v32,
+                                                    const double* __restrict__ z, int period, size_t ld,
+                                                    const double* __restrict__ y, const GmState* __restrict__ st)
+{
+#pragma clang fp contract(off)
+  if (st->early || st->divzero || st->tconv) return;
+  const size_t k = 2 * ((size_t)blockIdx.x * blockDim.x + threadIdx.x);
+  if (k >= (size_t)n) return;
+  const int last_i = st->last_i;
+  double2 x = *reinterpret_cast<const double2*>(X + k);
+  for (int j = 0; j <= last_i; j++) {
+    double2 b;
+    if (z && (period <= 1 || j % period == 0)) {
+      b = *reinterpret_cast<const double2*>(z + (size_t)j * ld + k);
+    } else {
+      const float2 f = *reinterpret_cast<const float2*>(v32 + (size_t)j * ld + k);
+      b = make_double2((double)f.x, (double)f.y);
+    }
+    x.x = x.x + y[j] * b.x;
+    x.y = x.y + y[j] * b.y;
+  }
+  *reinterpret_cast<double2*>(X + k) = x;
 }
 
 // ------------------------------------------------------ preconditioning --
@@ -1048,6 +1184,7 @@ void krylov_free(KrylovWork& k)
   dev_free(k.st);
   dev_free(k.scal);
   dev_free(k.u32);
+  dev_free(k.v32);
   dev_free(k.z);
   if (k.h_st) (void)hipHostFree(k.h_st);
   k.h_st = nullptr;
@@ -1107,6 +1244,21 @@ static int dot_device(int n, const double* w, const double* base, size_t ld, int
          partial, skip);
   launch("mdot_fin", k_fin, dim3(1), dim3(FIN_BS), 0, s, fa, (const double*)partial, nblk, m, st, h, cs, sn, err,
          coef, out, (volatile int*)host_done);
+  return SVG_OK;
+}
+
+// Arnoldi dots of the compressed-basis solve: w against v_0..v_{m-1} (FP32)
+// and w.w, then the finaliser over the m + 1 sums.
+static int dot_device_cb(int n, const double* w, const float* base, size_t ld, int m, KrylovWork& k, FinArgs fa,
+                         hipStream_t s)
+{
+  int nblk = (n + RED_SPAN - 1) / RED_SPAN;
+  if (nblk < 1) nblk = 1;
+  if (m + 1 > MAXM) return set_error(SVG_ERR_ARG, "reduction: too many sums");
+  launch("mdot_cb", k_mdot_cb, dim3(nblk), dim3(RED_BS), (size_t)RED_NW * (m + 1) * sizeof(double), s, n, w, base,
+         ld, m, k.partial, (const int*)&k.st->done);
+  launch("mdot_fin", k_fin, dim3(1), dim3(FIN_BS), 0, s, fa, (const double*)k.partial, nblk, m + 1, k.st, k.h, k.cs,
+         k.sn, k.err, k.coef, k.scal, (volatile int*)k.d_done);
   return SVG_OK;
 }
 
@@ -1229,7 +1381,11 @@ static int run_cycle(const Op& A, KrylovWork& k, int sD, FinArgs fa, double* X, 
   const bool mixed = K32 != nullptr;
   const bool flex = net != nullptr;
   const int n = k.n;
-  const unsigned nbv = blocks_for(n, 256);
+  const unsigned nbv = blocks_for(n, 256), nbv2 = blocks_for(n / 2, 256);
+  // mixed precision keeps the basis in FP32 (k.v32); the FP64 u_i is kept
+  // only where a step reads it: the network's input, a coupled face's term
+  const bool coupled = mixed && has_coupled(A.L);
+  auto netStep = [&](int j) { return flex && (net_period <= 1 || j % net_period == 0); };
   *k.h_done = 0;
   for (int i = 0; i < sD; i++) {
     if (i >= LOOKAHEAD) {
@@ -1240,37 +1396,47 @@ static int run_cycle(const Op& A, KrylovWork& k, int sD, FinArgs fa, double* X, 
     fa.i = i;
     // flexible GMRES: the SpMV acts on z_i = M(u_i) (and its FP32 copy in u32)
     const double* sin = k.u + k.ld * i;
-    if (flex) {
-      if (net_period <= 1 || i % net_period == 0) {
-        int rc = lpc_apply_dev(net, A.L, k.u + k.ld * i, k.z + k.ld * i, mixed ? k.u32 : nullptr, &k.st->done, s);
-        if (rc) return rc;
-      } else if (i == 0) {
-        launch("zcopy", k_zcopy, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, (const double*)(k.u + k.ld * i),
-               k.z + k.ld * i, mixed ? k.u32 : (float*)nullptr, (const GmState*)k.st);
-      }  // else: the previous step's Gram-Schmidt update wrote z_i = u_i (and u32)
+    double* w = k.u + k.ld * (i + 1);
+    if (netStep(i)) {
+      int rc = lpc_apply_dev(net, A.L, k.u + k.ld * i, k.z + k.ld * i, mixed ? k.u32 : nullptr, &k.st->done, s);
+      if (rc) return rc;
+      sin = k.z + k.ld * i;
+    } else if (flex && !mixed) {
+      if (i == 0)
+        launch("zcopy", k_zcopy, dim3(nbv), dim3(256), 0, s, n, (const double*)(k.u + k.ld * i), k.z + k.ld * i,
+               (float*)nullptr, (const GmState*)k.st);
+      // else: the previous step's Gram-Schmidt update wrote z_i = u_i
       sin = k.z + k.ld * i;
     }
     if (mixed) {
-      spmv32(A.L, K32, (const float*)k.u32, k.u + k.ld * (i + 1), (const int*)&k.st->done, s);
-      add_bc_mul_dev(A.L, 4, sin, k.u + k.ld * (i + 1), (const int*)&k.st->done, s);  // FP64 rank-one term
-    } else {
-      op_apply(A, sin, k.u + k.ld * (i + 1), nullptr, &k.st->done, s);
-    }
-    {
-      int rc = dot_device(n, k.u + k.ld * (i + 1), k.u, k.ld, i + 2, k.partial, k.scal, k.st, fa, k.h, k.cs,
-                          k.sn, k.err, k.coef, s, k.d_done);
+      spmv32(A.L, K32, netStep(i) ? (const float*)k.u32 : (const float*)(k.v32 + k.ld * i), w,
+             (const int*)&k.st->done, s);
+      if (coupled) add_bc_mul_dev(A.L, 4, sin, w, (const int*)&k.st->done, s);  // FP64 rank-one term
+      int rc = dot_device_cb(n, w, k.v32, k.ld, i + 1, k, fa, s);
       if (rc) return rc;
+      const int keep64 = (coupled || (i + 1 < sD && netStep(i + 1))) ? 1 : 0;
+      launch("gs_update_cb", k_gs_update_cb, dim3(nbv2), dim3(256), 0, s, n, w, k.v32, k.ld, i,
+             (const double*)k.coef, (const GmState*)k.st, keep64);
+    } else {
+      op_apply(A, sin, w, nullptr, &k.st->done, s);
+      int rc = dot_device(n, w, k.u, k.ld, i + 2, k.partial, k.scal, k.st, fa, k.h, k.cs, k.sn, k.err, k.coef, s,
+                          k.d_done);
+      if (rc) return rc;
+      // flexible: when the next step has no network, this update also writes z_{i+1} = u_{i+1}
+      const bool nextPlain = flex && i + 1 < sD && !netStep(i + 1);
+      launch("gs_update", k_gs_update, dim3(nbv), dim3(256), 0, s, n, k.u, k.ld, i, (const double*)k.coef,
+             (const GmState*)k.st, (float*)nullptr, nextPlain ? k.z + k.ld * (i + 1) : (double*)nullptr);
     }
-    // flexible: when the next step has no network, this update also writes z_{i+1} = u_{i+1}
-    const bool nextPlain = flex && i + 1 < sD && !(net_period <= 1 || (i + 1) % net_period == 0);
-    launch("gs_update", k_gs_update, dim3(nbv), dim3(256), 0, s, n, k.u, k.ld, i, (const double*)k.coef,
-           (const GmState*)k.st, (mixed && (!flex || nextPlain)) ? k.u32 : (float*)nullptr,
-           nextPlain ? k.z + k.ld * (i + 1) : (double*)nullptr);
     SVG_HIP(hipEventRecord(k.ev[i % 8], s));
   }
   launch("backsolve", k_backsolve, dim3(1), dim3(256), 0, s, sD, (const double*)k.h, (const double*)k.err, k.y, k.st);
-  launch("xupdate", k_xupdate, dim3(nbv), dim3(256), 0, s, n, X, (const double*)(flex ? k.z : k.u), k.ld,
-         (const double*)k.y, (const GmState*)k.st);
+  if (mixed)
+    launch("xupdate_cb", k_xupdate_cb, dim3(nbv2), dim3(256), 0, s, n, X, (const float*)k.v32,
+           flex ? (const double*)k.z : (const double*)nullptr, net_period, k.ld, (const double*)k.y,
+           (const GmState*)k.st);
+  else
+    launch("xupdate", k_xupdate, dim3(nbv), dim3(256), 0, s, n, X, (const double*)(flex ? k.z : k.u), k.ld,
+           (const double*)k.y, (const GmState*)k.st);
   SVG_HIP(hipGetLastError());
   SVG_HIP(hipMemcpyAsync(k.h_st, k.st, sizeof(GmState), hipMemcpyDeviceToHost, s));
   SVG_HIP(hipStreamSynchronize(s));
@@ -1348,13 +1514,13 @@ static int gmres_v_mixed(svg_lhs* L, svg_subls* ls, const double* Val, double* R
   int rc = ensure_work(k, 4 * L->nNo, ls->sD);
   if (rc) return rc;
   const int n = k.n, sD = ls->sD;
-  if (!k.u32 && (rc = dev_alloc(&k.u32, (size_t)n))) return rc;
+  if (!k.v32 && (rc = dev_alloc(&k.v32, k.ld * (k.sD + 1)))) return rc;
   if (!L->d_val32 && (rc = dev_alloc(&L->d_val32, (size_t)16 * L->nnz))) return rc;
   const Op A{L, Val, 4};
   if (L->spmvVariant < 0 && (rc = svg_spmv_tune(L, Val, nullptr, s))) return rc;
   auto t0 = std::chrono::steady_clock::now();
   launch("d2f", k_d2f, dim3(8192), dim3(256), 0, s, (size_t)16 * L->nnz, Val, L->d_val32);
-  if (L->spmv32Variant < 0 && (rc = spmv32_tune(L, L->d_val32, k.u32, k.X, s))) return rc;
+  if (L->spmv32Variant < 0 && (rc = spmv32_tune(L, L->d_val32, k.v32, k.X, s))) return rc;
   FinArgs fa{};
   fa.sD = sD;
   fa.absTol = ls->absTol;
@@ -1384,7 +1550,7 @@ static int gmres_v_mixed(svg_lhs* L, svg_subls* ls, const double* Val, double* R
     fa.tcheck = 1;
     if ((rc = dot_device(n, k.u, k.u, 0, 1, k.partial, k.scal, k.st, fa, k.h, k.cs, k.sn, k.err, k.coef, s)))
       return rc;
-    launch("div", k_div, dim3(nbv), dim3(256), 0, s, n, k.u, (const GmState*)k.st, k.u32);
+    launch("div", k_div, dim3(nbv), dim3(256), 0, s, n, k.u, (const GmState*)k.st, k.v32);  // u_0 and v_0
     if ((rc = run_cycle(A, k, sD, fa, k.X, s, L->d_val32))) return rc;
     if (k.h_st->divzero) return set_error(SVG_ERR_ARG, "Array Divide by zero.");
     ls->itr = k.h_st->itr;
@@ -1420,6 +1586,7 @@ static int gmres_v_learned(svg_lhs* L, svg_subls* ls, const double* Val, double*
   if (!k.z && (rc = dev_alloc(&k.z, k.ld * sD))) return rc;
   if (mixed) {
     if (!k.u32 && (rc = dev_alloc(&k.u32, (size_t)n))) return rc;
+    if (!k.v32 && (rc = dev_alloc(&k.v32, k.ld * (k.sD + 1)))) return rc;
     if (!L->d_val32 && (rc = dev_alloc(&L->d_val32, (size_t)16 * L->nnz))) return rc;
   }
   const Op A{L, Val, 4};
@@ -1464,7 +1631,7 @@ static int gmres_v_learned(svg_lhs* L, svg_subls* ls, const double* Val, double*
     if ((rc = dot_device(n, k.u, k.u, 0, 1, k.partial, k.scal, k.st, fa, k.h, k.cs, k.sn, k.err, k.coef, s)))
       return rc;
     launch("div", k_div, dim3(nbv), dim3(256), 0, s, n, k.u, (const GmState*)k.st,
-           (mixed && !pc) ? k.u32 : (float*)nullptr);
+           mixed ? k.v32 : (float*)nullptr);
     if ((rc = run_cycle(A, k, sD, fa, k.X, s, mixed ? L->d_val32 : nullptr, pc, net_period))) return rc;
     if (k.h_st->divzero) return set_error(SVG_ERR_ARG, "Array Divide by zero.");
     ls->itr = k.h_st->itr;
@@ -1942,30 +2109,36 @@ __global__ __launch_bounds__(RED_BS) void k_bmdot(const int* __restrict__ spanSt
     const GmState& S = st[spanSys[b]];
     if (!S.act || (check == 2 && S.done)) return;
   }
-  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-  const size_t k0 = (size_t)spanStart[b] + t, kend = (size_t)spanEnd[b];
-  double wr[RED_EPT];
-#pragma unroll
-  for (int r = 0; r < RED_EPT; r++) {
-    size_t k = k0 + (size_t)r * RED_BS;
-    wr[r] = k < kend ? w[k] : 0.0;
-  }
-  for (int j = 0; j < m; j++) {
-    const double* uj = base + (size_t)j * ld;
-    double acc = 0.0;
-#pragma unroll
-    for (int r = 0; r < RED_EPT; r++) {
-      size_t k = k0 + (size_t)r * RED_BS;
-      double v = k < kend ? uj[k] : 0.0;
-      acc = acc + v * wr[r];
-    }
-    acc = wave_sum(acc);
-    if (lane == 0) lds[wv * m + j] = acc;
-  }
+  const int t = threadIdx.x;
+  span_dots<1>(w, base, ld, m, false, (size_t)spanStart[b] + t, (size_t)spanEnd[b], lds, m);
   __syncthreads();
   for (int j = t; j < m; j += RED_BS) {
     double v = lds[j];
     for (int q = 1; q < RED_NW; q++) v = v + lds[q * m + j];
+    partial[(size_t)j * nsp + b] = v;
+  }
+}
+
+// k_mdot_cb per span of the batched solve (FP32 basis, then w.w as column m).
+__global__ __launch_bounds__(RED_BS) void k_bmdot_cb(const int* __restrict__ spanStart,
+                                                     const int* __restrict__ spanEnd, const int* __restrict__ spanSys,
+                                                     int nsp, const double* __restrict__ w,
+                                                     const float* __restrict__ base, size_t ld, int m,
+                                                     double* __restrict__ partial, const GmState* __restrict__ st)
+{
+#pragma clang fp contract(off)
+  extern __shared__ double lds[];  // [RED_NW waves][m + 1]
+  const int b = blockIdx.x, mc = m + 1;
+  {
+    const GmState& S = st[spanSys[b]];
+    if (!S.act || S.done) return;
+  }
+  const int t = threadIdx.x;
+  span_dots<MDOT_CB_G>(w, base, ld, m, true, (size_t)spanStart[b] + t, (size_t)spanEnd[b], lds, mc);
+  __syncthreads();
+  for (int j = t; j < mc; j += RED_BS) {
+    double v = lds[j];
+    for (int q = 1; q < RED_NW; q++) v = v + lds[q * mc + j];
     partial[(size_t)j * nsp + b] = v;
   }
 }
@@ -2010,6 +2183,32 @@ __global__ __launch_bounds__(256) void k_bgs_update(int n, double* __restrict__ 
   if (z) z[k] = v;  // flexible solve: the next step is one the network does not precondition
 }
 
+// k_gs_update_cb per system of the batched solve (each node's 4 rows belong to one system).
+__global__ __launch_bounds__(256) void k_bgs_update_cb(int n, double* __restrict__ w, float* __restrict__ v32,
+                                                       size_t ld, int i, int sD, const double* __restrict__ coef,
+                                                       const GmState* __restrict__ st,
+                                                       const int* __restrict__ sysOfNode, int keep64)
+{
+#pragma clang fp contract(off)
+  const size_t k = 2 * ((size_t)blockIdx.x * blockDim.x + threadIdx.x);
+  if (k >= (size_t)n) return;
+  const int sy = sysOfNode[k >> 2];
+  const GmState& S = st[sy];
+  if (!S.act || S.done) return;
+  const double* c = coef + (size_t)sy * (sD + 2);
+  double2 v = *reinterpret_cast<const double2*>(w + k);
+  for (int j = 0; j <= i; j++) {
+    const float2 b = *reinterpret_cast<const float2*>(v32 + (size_t)j * ld + k);
+    const double cj = -c[j];
+    v.x = v.x + cj * (double)b.x;
+    v.y = v.y + cj * (double)b.y;
+  }
+  v.x = S.scale * v.x;
+  v.y = S.scale * v.y;
+  *reinterpret_cast<float2*>(v32 + (size_t)(i + 1) * ld + k) = make_float2((float)v.x, (float)v.y);
+  if (keep64) *reinterpret_cast<double2*>(w + k) = v;
+}
+
 __global__ __launch_bounds__(256) void k_bdiv(int n, double* __restrict__ u, const GmState* __restrict__ st,
                                               const int* __restrict__ sysOfNode, float* __restrict__ u32)
 {
@@ -2048,6 +2247,34 @@ __global__ __launch_bounds__(256) void k_bxupdate(int n, double* __restrict__ X,
   double x = X[k];
   for (int j = 0; j <= S.last_i; j++) x = x + ys[j] * u[(size_t)j * ld + k];
   X[k] = x;
+}
+
+// k_xupdate_cb per system of the batched solve.
+__global__ __launch_bounds__(256) void k_bxupdate_cb(int n, double* __restrict__ X, const float* __restrict__ v32,
+                                                     const double* __restrict__ z, int period, size_t ld, int sD,
+                                                     const double* __restrict__ y, const GmState* __restrict__ st,
+                                                     const int* __restrict__ sysOfNode)
+{
+#pragma clang fp contract(off)
+  const size_t k = 2 * ((size_t)blockIdx.x * blockDim.x + threadIdx.x);
+  if (k >= (size_t)n) return;
+  const int sy = sysOfNode[k >> 2];
+  const GmState& S = st[sy];
+  if (!S.act || S.early || S.divzero || S.tconv) return;
+  const double* ys = y + (size_t)sy * sD;
+  double2 x = *reinterpret_cast<const double2*>(X + k);
+  for (int j = 0; j <= S.last_i; j++) {
+    double2 b;
+    if (z && (period <= 1 || j % period == 0)) {
+      b = *reinterpret_cast<const double2*>(z + (size_t)j * ld + k);
+    } else {
+      const float2 f = *reinterpret_cast<const float2*>(v32 + (size_t)j * ld + k);
+      b = make_double2((double)f.x, (double)f.y);
+    }
+    x.x = x.x + ys[j] * b.x;
+    x.y = x.y + ys[j] * b.y;
+  }
+  *reinterpret_cast<double2*>(X + k) = x;
 }
 
 // After a cycle: systems that converged stop iterating (plain: the Givens
@@ -2107,7 +2334,9 @@ struct BatchMode {
 
 // z_j = u_j (and its FP32 copy) for the systems still iterating in the cycle;
 // after a network application (onlyFallback) only for the systems that fell
-// back to the diagonal preconditioner, whose flexible basis is then the plain one.
+// back to the diagonal preconditioner, whose flexible basis is then the plain
+// one: with the FP32 copy (mixed precision, FP32 basis) z_j is the FP32 basis
+// vector's value, as the solve without the network reads it.
 __global__ __launch_bounds__(256) void k_bzcopy(int n, const double* __restrict__ u, double* __restrict__ z,
                                                 float* __restrict__ z32, const GmState* __restrict__ st,
                                                 const int* __restrict__ sysOfNode, int onlyFallback)
@@ -2117,8 +2346,13 @@ __global__ __launch_bounds__(256) void k_bzcopy(int n, const double* __restrict_
   const GmState& S = st[sysOfNode[k >> 2]];
   if (!S.act || S.done || (onlyFallback && !S.fallback)) return;
   const double v = u[k];
-  z[k] = v;
-  if (z32) z32[k] = (float)v;
+  if (z32) {
+    const float f = (float)v;
+    z32[k] = f;
+    z[k] = (double)f;
+  } else {
+    z[k] = v;
+  }
 }
 
 // One restart cycle of every system still iterating (the body of gmres_v's
@@ -2140,45 +2374,62 @@ static int batch_cycle(svg_batch* B, const double* Val, const double* R, const B
   fa.first = 0;
   fa.tcheck = md.tcheck() ? 1 : 0;
   batch_fin(B, fa, 1, s);
-  float* u32w = (mixed && !flex) ? B->u32 : (float*)nullptr;  // flexible: z's copy feeds the SpMV
+  // mixed precision: the basis in FP32 (B->v32), u_0 and v_0 from the division
   hipLaunchKernelGGL(k_bdiv, dim3(nbv), dim3(256), 0, s, n, B->u, (const GmState*)B->st,
-                     (const int*)B->d_sysOfNode, u32w);
+                     (const int*)B->d_sysOfNode, mixed ? B->v32 : (float*)nullptr);
+  const unsigned nbv2 = blocks_for(n / 2, 256);
+  auto netStep = [&](int j) { return flex && (md.period <= 1 || j % md.period == 0); };
   for (int i = 0; i < sD; i++) {
     double* ui = B->u + B->ld * i;
     double* un = B->u + B->ld * (i + 1);
     const double* sin = ui;
-    if (flex) {
+    if (netStep(i)) {
       double* zi = B->z + B->ld * i;
       float* z32 = mixed ? B->u32 : (float*)nullptr;
-      if (md.period <= 1 || i % md.period == 0) {
-        int rc = lpc_apply_dev(md.net, L, ui, zi, z32, nullptr, s, B->d_nsc, B->d_sysOfNode, B->st);
-        if (rc) return rc;
-        hipLaunchKernelGGL(k_bzcopy, dim3(nbv), dim3(256), 0, s, n, (const double*)ui, zi, z32,
-                           (const GmState*)B->st, (const int*)B->d_sysOfNode, 1);
-      } else if (i == 0) {
-        hipLaunchKernelGGL(k_bzcopy, dim3(nbv), dim3(256), 0, s, n, (const double*)ui, zi, z32,
+      int rc = lpc_apply_dev(md.net, L, ui, zi, z32, nullptr, s, B->d_nsc, B->d_sysOfNode, B->st);
+      if (rc) return rc;
+      hipLaunchKernelGGL(k_bzcopy, dim3(nbv), dim3(256), 0, s, n, (const double*)ui, zi, z32,
+                         (const GmState*)B->st, (const int*)B->d_sysOfNode, 1);
+      sin = zi;
+    } else if (flex && !mixed) {
+      double* zi = B->z + B->ld * i;
+      if (i == 0)
+        hipLaunchKernelGGL(k_bzcopy, dim3(nbv), dim3(256), 0, s, n, (const double*)ui, zi, (float*)nullptr,
                            (const GmState*)B->st, (const int*)B->d_sysOfNode, 0);
-      }  // else: the previous step's Gram-Schmidt update wrote z_i = u_i (and u32)
+      // else: the previous step's Gram-Schmidt update wrote z_i = u_i
       sin = zi;
     }
-    if (mixed)
-      spmv32(L, (const float*)L->d_val32, (const float*)B->u32, un, nullptr, s, "bspmv32",
-             (const int*)B->d_sysOfNode, (const GmState*)B->st);
-    else
-      op_apply(A, sin, un, nullptr, nullptr, s);
-    batch_mdot(B, un, B->u, B->ld, i + 2, 2, s);
     fa.mode = FIN_ARNOLDI;
     fa.i = i;
-    batch_fin(B, fa, i + 2, s);
-    const bool nextPlain = flex && i + 1 < sD && !(md.period <= 1 || (i + 1) % md.period == 0);
-    hipLaunchKernelGGL(k_bgs_update, dim3(nbv), dim3(256), 0, s, n, B->u, B->ld, i, sD, (const double*)B->coef,
-                       (const GmState*)B->st, (const int*)B->d_sysOfNode, (nextPlain && mixed) ? B->u32 : u32w,
-                       nextPlain ? B->z + B->ld * (i + 1) : (double*)nullptr);
+    if (mixed) {
+      spmv32(L, (const float*)L->d_val32, netStep(i) ? (const float*)B->u32 : (const float*)(B->v32 + B->ld * i),
+             un, nullptr, s, "bspmv32", (const int*)B->d_sysOfNode, (const GmState*)B->st);
+      hipLaunchKernelGGL(k_bmdot_cb, dim3(B->nsp), dim3(RED_BS), (size_t)RED_NW * (i + 2) * sizeof(double), s,
+                         (const int*)B->d_spanStart, (const int*)B->d_spanEnd, (const int*)B->d_spanSys, B->nsp,
+                         (const double*)un, (const float*)B->v32, B->ld, i + 1, B->partial, (const GmState*)B->st);
+      batch_fin(B, fa, i + 2, s);
+      hipLaunchKernelGGL(k_bgs_update_cb, dim3(nbv2), dim3(256), 0, s, n, un, B->v32, B->ld, i, sD,
+                         (const double*)B->coef, (const GmState*)B->st, (const int*)B->d_sysOfNode,
+                         (i + 1 < sD && netStep(i + 1)) ? 1 : 0);
+    } else {
+      op_apply(A, sin, un, nullptr, nullptr, s);
+      batch_mdot(B, un, B->u, B->ld, i + 2, 2, s);
+      batch_fin(B, fa, i + 2, s);
+      const bool nextPlain = flex && i + 1 < sD && !netStep(i + 1);
+      hipLaunchKernelGGL(k_bgs_update, dim3(nbv), dim3(256), 0, s, n, B->u, B->ld, i, sD, (const double*)B->coef,
+                         (const GmState*)B->st, (const int*)B->d_sysOfNode, (float*)nullptr,
+                         nextPlain ? B->z + B->ld * (i + 1) : (double*)nullptr);
+    }
   }
   hipLaunchKernelGGL(k_bbacksolve, dim3(B->nsys), dim3(256), 0, s, sD, (const double*)B->h, (const double*)B->err,
                      B->y, B->st);
-  hipLaunchKernelGGL(k_bxupdate, dim3(nbv), dim3(256), 0, s, n, B->X, (const double*)(flex ? B->z : B->u), B->ld,
-                     sD, (const double*)B->y, (const GmState*)B->st, (const int*)B->d_sysOfNode);
+  if (mixed)
+    hipLaunchKernelGGL(k_bxupdate_cb, dim3(nbv2), dim3(256), 0, s, n, B->X, (const float*)B->v32,
+                       flex ? (const double*)B->z : (const double*)nullptr, md.period, B->ld, sD,
+                       (const double*)B->y, (const GmState*)B->st, (const int*)B->d_sysOfNode);
+  else
+    hipLaunchKernelGGL(k_bxupdate, dim3(nbv), dim3(256), 0, s, n, B->X, (const double*)(flex ? B->z : B->u), B->ld,
+                       sD, (const double*)B->y, (const GmState*)B->st, (const int*)B->d_sysOfNode);
   hipLaunchKernelGGL(k_bclose, dim3(blocks_for(B->nsys, 256)), dim3(256), 0, s, B->nsys, B->st,
                      md.tcheck() ? 1 : 0);
   SVG_HIP(hipGetLastError());
@@ -2257,6 +2508,7 @@ static int batch_prepare(svg_batch* B, const double* Val, const BatchMode& md, h
   if (md.mixed) {
     if (!L->d_val32 && (rc = dev_alloc(&L->d_val32, (size_t)16 * L->nnz))) return rc;
     if (!B->u32 && (rc = dev_alloc(&B->u32, (size_t)B->n))) return rc;
+    if (!B->v32 && (rc = dev_alloc(&B->v32, B->ld * (B->sD + 1)))) return rc;
     // the FP32 SpMV variant for the union (timing only: the values do not matter)
     if (L->spmv32Variant < 0 && (rc = spmv32_tune(L, L->d_val32, B->u32, B->X, caller))) return rc;
   }
@@ -2300,7 +2552,7 @@ static int batch_cycle_graph(svg_batch* B, const double* R, const double* Val, c
 {
   const int gi = md.mixed ? 1 : 0;
   GraphKey k;
-  k.add(R).add(Val).add(fa.absTol).add(fa.relTol).add(fa.fbwin).add(B->L->d_val32).add(B->u32);
+  k.add(R).add(Val).add(fa.absTol).add(fa.relTol).add(fa.fbwin).add(B->L->d_val32).add(B->u32).add(B->v32);
   k.add(B->L->spmvVariant).add(B->L->spmv32Variant).add(md.net).add(md.period).add(B->z).add(B->d_nsc);
   if (B->graph[gi] && B->gKey[gi] == k.b) return SVG_OK;
   if (B->graph[gi]) (void)hipGraphExecDestroy(B->graph[gi]), B->graph[gi] = nullptr;
@@ -2679,6 +2931,7 @@ int svg_batch_free(svg_batch* B)
     if (p) (void)hipFree(p);
   if (B->st) (void)hipFree(B->st);
   if (B->u32) (void)hipFree(B->u32);
+  if (B->v32) (void)hipFree(B->v32);
   if (B->z) (void)hipFree(B->z);
   if (B->d_nsc) (void)hipFree(B->d_nsc);
   if (B->h_st) (void)hipHostFree(B->h_st);
