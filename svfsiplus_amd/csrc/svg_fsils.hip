@@ -524,7 +524,20 @@ __device__ __forceinline__ void fin_body(const FinArgs& A, const double* __restr
   for (int j = wv; j < m; j += FIN_BS / 64) {
     const double* pj = partial + (size_t)j * pstride;
     double acc = 0.0;
-    for (int b = lane; b < nblk; b += 64) acc = acc + pj[b];
+    // the lane's partials b = lane, lane + 64, ... added in order; loaded 16
+    // at a time (indices clamped, clamped values unused) so that the loads
+    // overlap instead of costing one memory latency each
+    for (int b0 = 0; b0 < nblk; b0 += 16 * 64) {
+      double v[16];
+#pragma unroll
+      for (int q = 0; q < 16; q++) {
+        const int b = b0 + lane + 64 * q;
+        v[q] = pj[b < nblk ? b : nblk - 1];
+      }
+#pragma unroll
+      for (int q = 0; q < 16; q++)
+        if (b0 + lane + 64 * q < nblk) acc = acc + v[q];
+    }
     acc = wave_sum(acc);
     if (lane == 0) sums[j] = acc;
   }
