@@ -301,16 +301,47 @@ class FluidNewton:
         """To the next time step on the device: Ao = An, Yo = Yn (main.cpp:754-755),
         the predictor (picp, pic.cpp:591-715) and the strong Dirichlet values
         of every face (set_bc_dir, set_bc.cpp:794-1063; steady: dirA = 0, dirY = 1)."""
-        lib, s = _lib.load(), _stream()
         self.Ao.copy_(self.An)
         self.Yo.copy_(self.Yn)
+        self._predict()
+        self.step += 1
+
+    def _predict(self):
+        lib, s = _lib.load(), _stream()
         _lib.check(lib.svg_picp(self.pipe.nNo, self.phys.gam, _ptr(self.Ao), _ptr(self.Yo), _ptr(self.An),
                                 _ptr(self.Yn), s))
         for gN, gxn in self._dir:
             _lib.check(lib.svg_set_bc_dir(gN.shape[0], _ptr(gN), 4, 0, 3, 0, _ptr(gxn), 0.0, 1.0, _ptr(self.An),
                                           _ptr(self.Yn), s))
         self.history = []
-        self.step += 1
+
+    def restart_stamp(self) -> tuple:
+        """The stamp of this run's restart records (one rank, one equation, one mesh, no coupled-BC state)."""
+        from . import restart
+        return restart.make_stamp(self.pipe.nNo, tDof=4)
+
+    def save_restart(self, stFileName: str, replace: bool = False, cpu_time: float = 0.0) -> str:
+        """write_restart (output.cpp:202-360) after time step `step + 1`: Yn, An
+        and eq.iNorm, the file named for the time step with a `_last.bin` link."""
+        from . import restart
+        self.torch.cuda.synchronize()
+        cTS = self.step + 1
+        r = restart.Restart(self.restart_stamp(), cTS, cTS * self.phys.dt, cpu_time,
+                            np.array([self.eq.iNorm]), self.Yn.cpu().numpy(), self.An.cpu().numpy())
+        return restart.save(stFileName, r, replace)
+
+    def load_restart(self, fname: str):
+        """init_from_bin (initialize.cpp:78-241): Yo, Ao and eq.iNorm from the
+        record, then the predictor and Dirichlet values of the next time step
+        (cTS + 1) on the device, as `advance` leaves them."""
+        from . import restart
+        r = restart.read(fname, self.restart_stamp())
+        self.Yo.copy_(self.torch.from_numpy(r.Yn))
+        self.Ao.copy_(self.torch.from_numpy(r.An))
+        self.eq = EqState(self.eq.minItr, self.eq.maxItr, self.eq.tol, iNorm=float(r.iNorm[0]))
+        self.step = r.cTS
+        self._predict()
+        return r
 
     def time_steps(self, nsteps: int):
         """nsteps time steps from the current state; returns the Newton history of each."""

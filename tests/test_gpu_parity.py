@@ -227,6 +227,29 @@ def test_two_time_steps(torch_cuda, phys):
     assert np.linalg.norm(fn.An.cpu().numpy() - An0) <= 1e-6 * np.linalg.norm(An0)
 
 
+def test_restart_continues_bitwise(torch_cuda, phys, tmp_path):
+    """write_restart after time step 1, init_from_bin in a new driver, time step 2:
+    the same Newton records and the same Yn/An, bit for bit, as two steps run
+    without the restart (the record carries Yn, An and eq.iNorm)."""
+    from svfsiplus_amd import restart as RS
+    from svfsiplus_amd.newton import FluidNewton, ls_create
+    pipe = W.make_pipe(4)
+    mk = lambda: ls_create(_lib.LS_GMRES, relTol=1e-6, absTol=1e-30, maxItr=1000, dimKry=50)
+    fa = FluidNewton(pipe, phys, mk())
+    ha = fa.time_steps(2)
+    fb = FluidNewton(pipe, phys, mk())
+    fb.time_step()
+    f = fb.save_restart(str(tmp_path / "stFile"))
+    assert os.path.basename(f) == "stFile_001.bin" and os.path.samefile(RS.last(str(tmp_path / "stFile")), f)
+    fc = FluidNewton(pipe, phys, mk())
+    r = fc.load_restart(RS.last(str(tmp_path / "stFile")))
+    assert r.cTS == 1 and fc.step == 1
+    hc = fc.time_step()
+    print("continuous", ha[1], "\nrestarted ", hc)
+    assert [dict(h) for h in hc] == [dict(h) for h in ha[1]]
+    assert torch_cuda.equal(fc.Yn, fa.Yn) and torch_cuda.equal(fc.An, fa.An)
+
+
 @pytest.mark.parametrize("name", ["pipe_n2", "pipe_n4"])
 def test_gpu_against_reference_golden(torch_cuda, name):
     """The HIP path against the reference's own outputs (fixtures written from the reference build)."""
