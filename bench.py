@@ -348,7 +348,31 @@ def batch_run(a, log):
     if net is not None:
         out["learned_pc"] = {"weights": os.path.relpath(a.lpc_weights, ROOT), "net_period": a.net_period,
                              "meshes_fallen_back": fallbacks}
+    u = bn.pipe
     bn.free()
+    # the union's mesh setup (block pattern, colouring, colour order, slot map): host and device builds
+    from svfsiplus_amd.newton import DeviceLhs, DeviceMesh, device_arrays
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    W.pattern(u.nNo, u.IEN)
+    Lh = DeviceLhs(u)
+    Mh = DeviceMesh(Lh, u)
+    torch.cuda.synchronize()
+    th = time.perf_counter() - t
+    t = time.perf_counter()
+    d_ien, d_x = device_arrays(u)
+    Ld = DeviceLhs(u, d_ien=d_ien)
+    Md = DeviceMesh(Ld, u, d_ien=d_ien, d_x=d_x)
+    torch.cuda.synchronize()
+    td = time.perf_counter() - t
+    out["setup_s"] = {"host": round(th, 3), "device": round(td, 3),
+                      "colours": {"host": Mh.nColors, "device": Md.nColors},
+                      "note": "block pattern + colouring + colour order + slot map of the union (host: C++ on one "
+                              "core, then upload; device: upload of IEN and x, then svg_lhs_create_dev and "
+                              "svg_msh_create_dev)"}
+    log(f"# batch union setup: host {th:.3f}s, device {td:.3f}s")
+    for o in (Md, Ld, Mh, Lh):
+        o.free()
     return out
 
 

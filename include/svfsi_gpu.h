@@ -117,6 +117,15 @@ typedef struct svg_lhs svg_lhs;
 int svg_lhs_create(int nNo, int nnz, const int* rowPtr, const int* colPtr, int nFaces, svg_lhs** lhs);
 int svg_lhs_bc_create(svg_lhs* lhs, int faIn, int nNo, int dof, int bGrp, const int* gNodes, const double* val);
 int svg_lhs_free(svg_lhs* lhs);
+/* The lhs of a TET4 mesh whose connectivity d_IEN (4, nEl) is already in
+ * device memory: the block-CSR pattern lhsa builds (lhsa.cpp:153-380: the
+ * node pairs of every element and every node's diagonal, rows ascending,
+ * columns sorted and unique) computed on the device (one radix sort of the
+ * pairs and a unique pass), then fsils_lhs_create's structures as
+ * svg_lhs_create makes them. Synchronises `stream`. */
+int svg_lhs_create_dev(int nNo, int nEl, const int* d_IEN, int nFaces, svg_lhs** lhs, void* stream);
+int svg_lhs_nnz(const svg_lhs* lhs, int* nnz);
+int svg_lhs_pattern(const svg_lhs* lhs, int* rowPtr, int* colPtr); /* host copies: nNo+1 and nnz ints */
 
 /* ------------------------------------------------------------------------ */
 /* Element assembly: construct_fluid (fluid.cpp:464-708) for TET4 P1-P1 VMS  */
@@ -126,8 +135,17 @@ typedef struct svg_msh svg_msh;
 /* Uploads x and IEN, colours the elements and builds the element -> block
  * slot map over `lhs`'s pattern (the binary search of do_assem, lhsa.cpp:123-134). */
 int svg_msh_create(const svg_lhs* lhs, int nEl, const int* IEN, int nNo, const double* x, svg_msh** msh);
+/* The same from device-resident IEN (4, nEl) and x (3, nNo), every step on
+ * the device: Jones-Plassmann colouring (the sequential greedy colouring in
+ * a fixed pseudo-random element order; no two elements of one colour share a
+ * node; at most 256 colours), the stable colour order and the slot map.
+ * Synchronises `stream`. */
+int svg_msh_create_dev(const svg_lhs* lhs, int nEl, const int* d_IEN, const double* d_x, svg_msh** msh,
+                       void* stream);
 int svg_msh_free(svg_msh* msh);
 int svg_msh_num_colors(const svg_msh* msh, int* nColors);
+int svg_msh_colors(const svg_msh* msh, int* colorOff);  /* nColors+1 offsets into the colour order */
+int svg_msh_sorted_ien(const svg_msh* msh, int* ien);   /* (4, nEl) in the colour order */
 
 typedef struct {
   double rho;      /* fluid_density                                   */

@@ -91,6 +91,12 @@ _SIGS = {
     "svg_msh_create": (_I, [_P, _I, _P, _I, _P, C.POINTER(_P)]),
     "svg_msh_free": (_I, [_P]),
     "svg_msh_num_colors": (_I, [_P, _IP]),
+    "svg_lhs_create_dev": (_I, [_I, _I, _P, _I, C.POINTER(_P), _P]),
+    "svg_lhs_nnz": (_I, [_P, _IP]),
+    "svg_lhs_pattern": (_I, [_P, _P, _P]),
+    "svg_msh_create_dev": (_I, [_P, _I, _P, _P, C.POINTER(_P), _P]),
+    "svg_msh_colors": (_I, [_P, _P]),
+    "svg_msh_sorted_ien": (_I, [_P, _P]),
     "svg_construct_fluid": (_I, [_P, C.POINTER(FluidParams), _P, _P, _P, _P, _P, _P]),
     "svg_face_create": (_I, [_P, _I, _P, _P, C.POINTER(_P)]),
     "svg_face_free": (_I, [_P]),

@@ -49,7 +49,7 @@ class BatchNewton:
 
     def __init__(self, meshes: list[Pipe], phys: Physics, sD: int = 50, rel_tol: float = 1e-6,
                  precision: int = _lib.PRECISION_FP64, max_restarts: int = 1000, lpc=None, net_period: int = 5,
-                 fallback_window: int = 20):
+                 fallback_window: int = 20, device_setup: bool = False):
         self.meshes = meshes
         self.nsys = len(meshes)
         self.pipe, self.noff = union(meshes)
@@ -59,7 +59,7 @@ class BatchNewton:
         # the learned preconditioner (flexible GMRES, per-mesh fallback) when a network is given
         self.opts = solve_opts(precision, lpc, fallback_window, net_period)
         self.fn = FluidNewton(self.pipe, phys, ls_create(_lib.LS_GMRES, relTol=rel_tol, absTol=1e-30,
-                                                         maxItr=max_restarts, dimKry=sD))
+                                                         maxItr=max_restarts, dimKry=sD), device_setup=device_setup)
         h = C.c_void_p()
         _lib.check(_lib.load().svg_batch_create(self.fn.lhs.h, self.nsys, self.noff.ctypes.data, sD, C.byref(h)))
         self.h = h

@@ -40,14 +40,24 @@ class DeviceLhs:
     (name, nodes) — a Dirichlet face, val = 0 — or (name, nodes, "neu", val(nNo, 3))
     — a Neumann face, coupled when fsils_solve is given a nonzero res for it."""
 
-    def __init__(self, pipe: Pipe, faces=None):
+    def __init__(self, pipe: Pipe, faces=None, d_ien=None):
+        """With d_ien (a device tensor holding pipe.IEN) the block pattern is
+        built on the device (svg_lhs_create_dev) instead of taken from
+        pipe.rowPtr / pipe.colPtr."""
         lib = _lib.load()
         faces = pipe.faces if faces is None else faces
         h = C.c_void_p()
-        _lib.check(lib.svg_lhs_create(pipe.nNo, pipe.nnz, pipe.rowPtr.ctypes.data, pipe.colPtr.ctypes.data,
-                                      len(faces), C.byref(h)))
+        if d_ien is None:
+            _lib.check(lib.svg_lhs_create(pipe.nNo, pipe.nnz, pipe.rowPtr.ctypes.data, pipe.colPtr.ctypes.data,
+                                          len(faces), C.byref(h)))
+            nnz = pipe.nnz
+        else:
+            _lib.check(lib.svg_lhs_create_dev(pipe.nNo, pipe.nEl, _ptr(d_ien), len(faces), C.byref(h), _stream()))
+            c = C.c_int()
+            _lib.check(lib.svg_lhs_nnz(h, C.byref(c)))
+            nnz = c.value
         self.h = h
-        self.nNo, self.nnz, self.nFaces = pipe.nNo, pipe.nnz, len(faces)
+        self.nNo, self.nnz, self.nFaces = pipe.nNo, nnz, len(faces)
         for i, f in enumerate(faces):
             g = np.ascontiguousarray(f[1], dtype=np.int32)
             neu = len(f) > 2 and f[2] == "neu"
@@ -67,15 +77,28 @@ class DeviceLhs:
             pass
 
 
+def device_arrays(pipe: Pipe):
+    """pipe.IEN (int32) and pipe.x (float64) uploaded for the device setup."""
+    import torch
+    return (torch.as_tensor(np.ascontiguousarray(pipe.IEN, dtype=np.int32), device="cuda"),
+            torch.as_tensor(np.ascontiguousarray(pipe.x, dtype=np.float64), device="cuda"))
+
+
 class DeviceMesh:
     """Device-resident element data (coordinates, connectivity, colours, slot map)."""
 
-    def __init__(self, lhs: DeviceLhs, pipe: Pipe):
+    def __init__(self, lhs: DeviceLhs, pipe: Pipe, d_ien=None, d_x=None):
+        """With d_ien and d_x (device tensors holding pipe.IEN and pipe.x) the
+        colouring, colour order and slot map are built on the device
+        (svg_msh_create_dev)."""
         lib = _lib.load()
         h = C.c_void_p()
-        IEN = np.ascontiguousarray(pipe.IEN, dtype=np.int32)
-        x = np.ascontiguousarray(pipe.x)
-        _lib.check(lib.svg_msh_create(lhs.h, pipe.nEl, IEN.ctypes.data, pipe.nNo, x.ctypes.data, C.byref(h)))
+        if d_ien is None:
+            IEN = np.ascontiguousarray(pipe.IEN, dtype=np.int32)
+            x = np.ascontiguousarray(pipe.x)
+            _lib.check(lib.svg_msh_create(lhs.h, pipe.nEl, IEN.ctypes.data, pipe.nNo, x.ctypes.data, C.byref(h)))
+        else:
+            _lib.check(lib.svg_msh_create_dev(lhs.h, pipe.nEl, _ptr(d_ien), _ptr(d_x), C.byref(h), _stream()))
         self.h = h
         nc = C.c_int()
         _lib.check(lib.svg_msh_num_colors(h, C.byref(nc)))
@@ -210,7 +233,10 @@ class FluidNewton:
     """One fluid equation on one pipe, all state resident in HBM."""
 
     def __init__(self, pipe: Pipe, phys: Physics, ls: _lib.Ls, eq: EqState | None = None, device="cuda",
-                 precision: int = _lib.PRECISION_FP64, lpc=None, outlet: bool = True, net_period: int = 1):
+                 precision: int = _lib.PRECISION_FP64, lpc=None, outlet: bool = True, net_period: int = 1,
+                 device_setup: bool = False):
+        """device_setup: build the block pattern, colouring and slot map on the
+        device (svg_lhs_create_dev, svg_msh_create_dev) from IEN and x."""
         import torch
         if not torch.cuda.is_available():
             raise RuntimeError("FluidNewton needs a GPU (the HIP library has no CPU path)")
@@ -220,8 +246,13 @@ class FluidNewton:
         self.lpc = lpc
         self.net_period = net_period
         self.eq = eq or EqState()
-        self.lhs = DeviceLhs(pipe)
-        self.msh = DeviceMesh(self.lhs, pipe)
+        if device_setup:
+            d_ien, d_x = device_arrays(pipe)
+            self.lhs = DeviceLhs(pipe, d_ien=d_ien)
+            self.msh = DeviceMesh(self.lhs, pipe, d_ien=d_ien, d_x=d_x)
+        else:
+            self.lhs = DeviceLhs(pipe)
+            self.msh = DeviceMesh(self.lhs, pipe)
         # the zero-traction outlet is a Neumann face: backflow stabilisation (b_assem_neu_bc)
         self.outlet = None
         if outlet:
@@ -240,7 +271,7 @@ class FluidNewton:
         self.Yg = torch.empty((n, 4), dtype=f64, device=dev)
         self.R = torch.empty((n, 4), dtype=f64, device=dev)
         self._init = [t.clone() for t in (self.Ao, self.Yo, self.An, self.Yn)]
-        self.Val = torch.empty((pipe.nnz, 16), dtype=f64, device=dev)
+        self.Val = torch.empty((self.lhs.nnz, 16), dtype=f64, device=dev)
         self.history = []
         # strong Dirichlet faces for set_bc_dir at later time steps (node ids, gx nV)
         self._dir = [(torch.tensor(g, dtype=torch.int32, device=dev), torch.tensor(v, dtype=f64, device=dev))
