@@ -129,6 +129,40 @@ def cpu_baseline(pipe, phys, gpu_hist):
             "host_cpu": platform.processor() or platform.machine(), "nproc": os.cpu_count()}
 
 
+def kernel_only_lpc(a, fn, pipe, lib):
+    """Applications of the learned preconditioner alone (encoder, 3 x message
+    gather + dense layer, decoder) on the first Newton system of the workload,
+    diagonally scaled as fsils_solve hands it to GMRES (for counter runs)."""
+    import torch
+    from svfsiplus_amd import _lib, lpc as L
+    from svfsiplus_amd.newton import _stream, construct_fluid
+    ph = fn.phys
+    s = _stream()
+    _lib.check(lib.svg_pici(pipe.nNo, ph.am, ph.af, fn.Ao.data_ptr(), fn.An.data_ptr(), fn.Yo.data_ptr(),
+                            fn.Yn.data_ptr(), fn.Ag.data_ptr(), fn.Yg.data_ptr(), s))
+    fn.R.zero_()
+    fn.Val.zero_()
+    construct_fluid(fn.msh, ph, fn.Ag, fn.Yg, fn.R, fn.Val)
+    Wd = torch.empty((pipe.nNo, 4), dtype=torch.float64, device="cuda")
+    _lib.check(lib.svg_precond_diag(fn.lhs.h, 4, fn.Val.data_ptr(), fn.R.data_ptr(), Wd.data_ptr(), s))
+    _lib.check(lib.svg_lpc_prepare(fn.lhs.h, fn.Val.data_ptr(), s))
+    net = L.LearnedPC.load(a.lpc_weights)
+    V = torch.randn((pipe.nNo, 4), dtype=torch.float64, device="cuda")
+    V /= torch.linalg.vector_norm(V)
+    Z = torch.empty_like(V)
+    _lib.check(lib.svg_lpc_apply(net.h, fn.lhs.h, 1, V.data_ptr(), Z.data_ptr(), s))  # buffers and variant choice
+    torch.cuda.synchronize()
+    lib.svg_prof_reset()
+    lib.svg_prof_enable(1)
+    for _ in range(a.launches):
+        _lib.check(lib.svg_lpc_apply(net.h, fn.lhs.h, 1, V.data_ptr(), Z.data_ptr(), s))
+    torch.cuda.synchronize()
+    lib.svg_prof_enable(0)
+    tab = _bytes(lib, kernel_table(lib))
+    print(json.dumps({"kernel": "lpc", "applications": a.launches, "nNo": pipe.nNo, "nnzb": pipe.nnz,
+                      "kernels": {k: v for k, v in tab.items() if k.startswith("lpc_")}}))
+
+
 def kernel_only(a, fn, pipe, lib):
     """SpMV launches on the first Newton system of the workload (assembled on the device),
     and a torch device copy of the SpMV's algorithmic byte count for counter calibration."""
@@ -399,7 +433,7 @@ def main():
     ap.add_argument("--batch-method", choices=["plain", "learned"], default="learned",
                     help="batched solve: diagonal preconditioner, or the learned one (flexible GMRES)")
     ap.add_argument("--no-prof", action="store_true", help="time without per-kernel events")
-    ap.add_argument("--kernel-only", choices=["spmv", "spmv32"], default=None,
+    ap.add_argument("--kernel-only", choices=["spmv", "spmv32", "lpc"], default=None,
                     help="launch only the block SpMV on the assembled system (for counter runs), "
                          "plus a device copy of the same byte count as the calibration kernel")
     ap.add_argument("--launches", type=int, default=20)
@@ -434,6 +468,9 @@ def main():
     t_setup = time.perf_counter() - t_setup
     log(f"# setup {t_setup:.1f}s: nNo={pipe.nNo} nEl={pipe.nEl} nnzb={pipe.nnz} colours={fn.msh.nColors}")
 
+    if a.kernel_only == "lpc":
+        kernel_only_lpc(a, fn, pipe, lib)
+        return
     if a.kernel_only:
         kernel_only(a, fn, pipe, lib)
         return

@@ -203,7 +203,7 @@ struct svg_lhs {
   bool maskBuilt = false;
   int spmvVariant = -1;  // 4x4 SpMV launch variant picked by the autotuner (-1: not tuned)
   int spmv32Variant = -1;  // FP32 4x4 SpMV launch variant picked by the autotuner (-1: not tuned)
-  int msgVariant = -1;     // network message-gather launch: 0 round-robin, 1 XCD bands (-1: not tuned)
+  mutable int msgVariant = -1;  // network message-gather launch: 0 round-robin, 1 XCD bands (-1: not tuned)
   float* d_val32 = nullptr;  // mixed precision: FP32 copy of the scaled Val
   unsigned short* d_val16 = nullptr;   // learned preconditioner: BF16 copy of the scaled Val
   unsigned short* d_diag16 = nullptr;  // learned preconditioner: BF16 diagonal blocks (16, nNo)

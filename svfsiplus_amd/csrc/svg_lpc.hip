@@ -799,15 +799,16 @@ int lpc_apply_dev(svg_lpc* N, const svg_lhs* L, const double* V, double* Z, floa
   return lpc_forward(N, L, 1, V, Z, Z32, false, skip, s, nsc, sysOf, st);
 }
 
-int lpc_reserve(svg_lpc* N, svg_lhs* L, hipStream_t s)
+// The message gather's launch variant (0: round-robin workgroups, 1: one row
+// band per XCD, whose L2 then keeps the h rows its band gathers: 155 against
+// 278 MB of HBM reads per launch on the 1.2M-tet pipe; identical results),
+// timed once per lhs on its buffers: the block copy as it is (its values do
+// not change the time) and the network's scratch rows. (Non-temporal loads of
+// the streamed blocks raised the reads to 278-440 MB and are not a variant.)
+static int lpc_msg_tune(svg_lpc* N, const svg_lhs* L, hipStream_t s)
 {
-  int rc = lhs_bf16_alloc(L);
-  if (rc || (rc = lpc_ensure(N, L->nNo, false))) return rc;
   if (L->msgVariant >= 0) return SVG_OK;
-  // the message gather's launch (round-robin workgroups or one row band per
-  // XCD; identical results), timed once per lhs on its buffers (values do not matter)
   SVG_HIP(hipMemsetAsync(N->H[0], 0, sizeof(bf16) * (size_t)F * L->nNo, s));
-  SVG_HIP(hipMemsetAsync(L->d_val16, 0, sizeof(bf16) * 16 * (size_t)L->nnz, s));
   hipEvent_t a, b;
   SVG_HIP(hipEventCreate(&a));
   SVG_HIP(hipEventCreate(&b));
@@ -837,6 +838,13 @@ int lpc_reserve(svg_lpc* N, svg_lhs* L, hipStream_t s)
   SVG_HIP(hipGetLastError());
   L->msgVariant = bv;
   return SVG_OK;
+}
+
+int lpc_reserve(svg_lpc* N, svg_lhs* L, hipStream_t s)
+{
+  int rc = lhs_bf16_alloc(L);
+  if (rc || (rc = lpc_ensure(N, L->nNo, false))) return rc;
+  return lpc_msg_tune(N, L, s);
 }
 
 // Loss and gradient of L = (1/B) sum_b |R_b - A M(R_b)|^2 into N->g.
@@ -984,7 +992,7 @@ int svg_lpc_apply(svg_lpc* N, const svg_lhs* L, int nvec, const double* d_V, dou
   if ((long long)nvec * L->nNo > (1LL << 30)) return set_error(SVG_ERR_ARG, "svg_lpc_apply: too many rows");
   hipStream_t s = (hipStream_t)stream;
   int rc = lpc_ensure(N, nvec * L->nNo, false);
-  if (rc) return rc;
+  if (rc || (rc = lpc_msg_tune(N, L, s))) return rc;
   if ((rc = lpc_forward(N, L, nvec, d_V, d_Z, nullptr, false, nullptr, s))) return rc;
   SVG_HIP(hipStreamSynchronize(s));
   return SVG_OK;
