@@ -201,8 +201,11 @@ int svg_ls_create(svg_ls* ls, int LS_type);
  * Neumann face is registered). A Neumann face with res != 0 is coupled
  * (solve.cpp:96-111): with LS type GMRES every block product adds
  * res valM (valM^T x) (add_bc_mul.cpp:53-121, valM = val W from
- * precond_diag, precond.cpp:246-256); the NS solver with a coupled face
- * returns SVG_ERR_UNSUPPORTED. Synchronous: returns when Ri holds the result
+ * precond_diag, precond.cpp:246-256); the NS solver adds the same term to
+ * K U (ns_solver.cpp:334) and to its inner GMRES's products, and applies op
+ * PRE, Y += -res/(1 + res nS) valM (valM^T X) with nS = |valM|^2, to each
+ * inner Krylov vector (gmres.cpp:137-199) and to G P in the Schur CG
+ * (cgrad.cpp:101-108). Synchronous: returns when Ri holds the result
  * and ls holds the iteration record. */
 int svg_fsils_solve(svg_lhs* lhs, svg_ls* ls, int dof, double* d_Ri, double* d_Val, int prec, int nIncL,
                     const int* incL, const double* res, void* stream);

@@ -44,7 +44,7 @@ class Ls(C.Structure):
 class Face(C.Structure):
     _fields_ = [("nNo", C.c_int), ("dof", C.c_int), ("bGrp", C.c_int), ("incFlag", C.c_int),
                 ("coupledFlag", C.c_int), ("glob", C.c_void_p), ("val", C.c_void_p), ("res", C.c_double),
-                ("valM", C.c_void_p)]
+                ("valM", C.c_void_p), ("nS", C.c_double)]
 
 
 def face_spec(f):

@@ -77,6 +77,7 @@ typedef struct {
   const double* val;   /* (dof, nNo) */
   double res;          /* resistance of a coupled Neumann face (set by fsils_solve) */
   double* valM;        /* (dof, nNo): val scaled by the preconditioner (caller-owned, zero at creation) */
+  double nS;           /* |valM|^2 over the face (the NS solver's bc_pre, ns_solver.cpp:56-89) */
 } orc_face;
 
 typedef struct {
@@ -120,6 +121,9 @@ int orc_fluid_element(const double xl[3][4], const double al[4][4], const double
 void orc_spar_mul_vv(const orc_lhs* lhs, int dof, const double* K, const double* U, double* KU);
 /* add_bc_mul (add_bc_mul.cpp:53-121), op ADD: Y += res valM (valM^T X) for each coupled face. */
 void orc_add_bc_mul(const orc_lhs* lhs, int dof, const double* X, double* Y);
+/* add_bc_mul op PRE (add_bc_mul.cpp:60-64): Y += coef valM (valM^T X) with
+ * coef = -res / (1 + res nS), face after face; X may be Y (in place). */
+void orc_add_bc_pre(const orc_lhs* lhs, int dof, const double* X, double* Y);
 void orc_spar_mul_sv(const orc_lhs* lhs, int dof, const double* K, const double* U, double* KU);
 void orc_spar_mul_vs(const orc_lhs* lhs, int dof, const double* K, const double* U, double* KU);
 void orc_spar_mul_ss(const orc_lhs* lhs, const double* K, const double* U, double* KU);

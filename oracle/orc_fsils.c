@@ -16,14 +16,17 @@
  *   fsils_ls_create                liner_solver/ls.cpp:40-120
  *   fsils_solve                    liner_solver/solve.cpp:50-193
  *
- *   add_bc_mul (op ADD)            liner_solver/add_bc_mul.cpp:53-121
+ *   add_bc_mul (ops ADD and PRE)   liner_solver/add_bc_mul.cpp:53-121
  *
  * Coupled (resistance) Neumann faces: fsils_solve sets coupledFlag and res
  * (solve.cpp:96-111), precond_diag sets valM = val W (precond.cpp:246-256) and
  * gmres_v adds res valM (valM^T x) after every block product (gmres.cpp:510,
  * 539; its BCOP_TYPE_PRE branches never run: `flag` stays false). The NS
- * solver's coupled-face terms (ns_solver.cpp:334, cgrad.cpp:104-106) are not
- * restated: orc_fsils_solve returns an error for LS type NS with a coupled face.
+ * solver sets nS = |valM|^2 per face (bc_pre, ns_solver.cpp:56-89) and adds the
+ * same term to K U (ns_solver.cpp:334); its inner gmres adds it after every
+ * block product and applies op PRE in place to each new Krylov vector
+ * (gmres.cpp:137-150, 188-199); the Schur CG applies op PRE to G P, reading a
+ * copy of G P (cgrad.cpp:101-108).
  */
 #include "oracle.h"
 
@@ -262,19 +265,23 @@ static double face_dot(const orc_face* face, int nsd, int dof, const double* X)
   return S;
 }
 
-void orc_add_bc_mul(const orc_lhs* lhs, int dof, const double* X, double* Y)
+static void add_bc(const orc_lhs* lhs, int dof, const double* X, double* Y, int pre)
 {
   for (int f = 0; f < lhs->nFaces; f++) {
     const orc_face* face = &lhs->face[f];
     if (!face->coupledFlag) continue;
     int nsd = face->dof < dof ? face->dof : dof;
-    double S = face->res * face_dot(face, nsd, dof, X);
+    double coef = pre ? -face->res / (1.0 + (face->res * face->nS)) : face->res;
+    double S = coef * face_dot(face, nsd, dof, X);
     for (int a = 0; a < face->nNo; a++) {
       int Ac = face->glob[a];
       for (int i = 0; i < nsd; i++) Y[i + dof * Ac] = Y[i + dof * Ac] + face->valM[i + face->dof * a] * S;
     }
   }
 }
+
+void orc_add_bc_mul(const orc_lhs* lhs, int dof, const double* X, double* Y) { add_bc(lhs, dof, X, Y, 0); }
+void orc_add_bc_pre(const orc_lhs* lhs, int dof, const double* X, double* Y) { add_bc(lhs, dof, X, Y, 1); }
 
 void orc_precond_diag(const orc_lhs* lhs, int dof, double* Val, double* R, double* W)
 {
@@ -396,7 +403,7 @@ int orc_ge(int N, int ldA, const double* A, double* B)
 
 /* Arnoldi cycle shared by gmres and gmres_v: returns last_i, sets *suc. */
 static int arnoldi_cycle(const orc_lhs* lhs, orc_subls* ls, int dof, const double* Val, double* u,
-                         double* h, double* c, double* s, double* err, double eps)
+                         double* h, double* c, double* s, double* err, double eps, int pre)
 {
   int nNo = lhs->nNo;
   size_t n = (size_t)dof * nNo;
@@ -408,6 +415,7 @@ static int arnoldi_cycle(const orc_lhs* lhs, orc_subls* ls, int dof, const doubl
     last_i = i;
     orc_spar_mul_vv(lhs, dof, Val, u + n * i, u + n * (i + 1));
     orc_add_bc_mul(lhs, dof, u + n * i, u + n * (i + 1));
+    if (pre) orc_add_bc_pre(lhs, dof, u + n * (i + 1), u + n * (i + 1));
     for (int j = 0; j <= i + 1; j++) H(j, i) = orc_dot_v(dof, nNo, u + n * j, u + n * (i + 1));
     for (int j = 0; j <= i; j++) {
       omp_sum((int)n, -H(j, i), u + n * (i + 1), u + n * j);
@@ -480,7 +488,7 @@ void orc_gmres_v(const orc_lhs* lhs, orc_subls* ls, int dof, const double* Val, 
     for (size_t k = 0; k < n; k++) u[k] = R[k] - u[k];
     err[0] = orc_norm_v(dof, nNo, u);
     for (size_t k = 0; k < n; k++) u[k] = u[k] / err[0];
-    last_i = arnoldi_cycle(lhs, ls, dof, Val, u, h, c, s, err, eps);
+    last_i = arnoldi_cycle(lhs, ls, dof, Val, u, h, c, s, err, eps, 0);
     if (last_i >= sD) last_i = sD - 1;
     back_solve_update(dof, nNo, sD, last_i, h, err, y, u, X);
     ls->fNorm = fabs(err[last_i + 1]);
@@ -512,9 +520,11 @@ void orc_gmres(const orc_lhs* lhs, orc_subls* ls, int dof, const double* Val, co
       memcpy(u, R, n * sizeof(double));
     } else {
       orc_spar_mul_vv(lhs, dof, Val, X, u);
+      orc_add_bc_mul(lhs, dof, X, u);
       ls->itr = ls->itr + 1;
       for (size_t k = 0; k < n; k++) u[k] = R[k] - u[k];
     }
+    orc_add_bc_pre(lhs, dof, u, u);  /* in place; a no-op without a coupled face */
     err[0] = orc_norm_v(dof, nNo, u);
     if (l == 0) {
       eps = err[0];
@@ -529,7 +539,7 @@ void orc_gmres(const orc_lhs* lhs, orc_subls* ls, int dof, const double* Val, co
     }
     ls->dB = ls->fNorm;
     for (size_t k = 0; k < n; k++) u[k] = u[k] / err[0];
-    last_i = arnoldi_cycle(lhs, ls, dof, Val, u, h, c, s, err, eps);
+    last_i = arnoldi_cycle(lhs, ls, dof, Val, u, h, c, s, err, eps, 1);
     if (last_i >= sD) last_i = sD - 1;
     back_solve_update(dof, nNo, sD, last_i, h, err, y, u, X);
     ls->fNorm = fabs(err[last_i + 1]);
@@ -550,6 +560,7 @@ void orc_schur(const orc_lhs* lhs, orc_subls* ls, int dof, const double* D, cons
   double* SP = (double*)calloc(nNo, sizeof(double));
   double* DGP = (double*)calloc(nNo, sizeof(double));
   double* GP = (double*)calloc((size_t)dof * nNo, sizeof(double));
+  double* GP0 = (double*)calloc((size_t)dof * nNo, sizeof(double));
   ls->suc = 0;
   ls->iNorm = orc_norm_s(nNo, R);
   double eps = pow(fmax(ls->absTol, ls->relTol * ls->iNorm), 2.0);
@@ -565,6 +576,8 @@ void orc_schur(const orc_lhs* lhs, orc_subls* ls, int dof, const double* D, cons
     }
     errO = err;
     orc_spar_mul_sv(lhs, dof, G, P, GP);
+    memcpy(GP0, GP, (size_t)dof * nNo * sizeof(double));  /* unCondU = GP: a copy */
+    orc_add_bc_pre(lhs, dof, GP0, GP);
     orc_spar_mul_vs(lhs, dof, D, GP, DGP);
     orc_spar_mul_ss(lhs, L, P, SP);
     omp_sum(nNo, -1.0, SP, DGP);
@@ -582,7 +595,7 @@ void orc_schur(const orc_lhs* lhs, orc_subls* ls, int dof, const double* D, cons
   ls->fNorm = sqrt(err);
   ls->itr = ls->itr + last_i;
   if (errO < DBL_EPSILON) ls->dB = 0.0; else ls->dB = 5.0 * log(err / errO);
-  free(X); free(P); free(SP); free(DGP); free(GP);
+  free(X); free(P); free(SP); free(DGP); free(GP); free(GP0);
 }
 
 int orc_ns_solver(const orc_lhs* lhs, orc_ls* ls, int dof, const double* Val, double* Ri)
@@ -649,6 +662,15 @@ int orc_ns_solver(const orc_lhs* lhs, orc_ls* ls, int dof, const double* Val, do
         }
     }
 
+  /* bc_pre (ns_solver.cpp:56-89, one rank: no face is shared) */
+  for (int f = 0; f < lhs->nFaces; f++) {
+    orc_face* face = &lhs->face[f];
+    if (!face->coupledFlag) continue;
+    face->nS = 0.0;
+    for (int a = 0; a < face->nNo; a++)
+      for (int i = 0; i < nsd; i++) face->nS = face->nS + pow(face->valM[i + face->dof * a], 2.0);
+  }
+
   int iBB = 0, i_count = 0;
   for (int i = 0; i < ls->RI.mItr; i++) {
     int iB = 2 * i;
@@ -669,6 +691,7 @@ int orc_ns_solver(const orc_lhs* lhs, orc_ls* ls, int dof, const double* Val, do
     memcpy(Ut, MUiBB, nv * sizeof(double));
     orc_gmres(lhs, &ls->GM, nsd, mK, Ut, Ui);
     orc_spar_mul_vv(lhs, nsd, mK, Ui, MUiBB);
+    orc_add_bc_mul(lhs, nsd, Ui, MUiBB);
     orc_spar_mul_ss(lhs, mL, Pi, MP + (size_t)nNo * iB);
     orc_spar_mul_vs(lhs, nsd, mD, Ui, MP + (size_t)nNo * iBB);
 
@@ -809,10 +832,6 @@ int orc_fsils_solve(orc_lhs* lhs, orc_ls* ls, int dof, double* Ri, double* Val, 
       if (face->bGrp == 1 && res[f] != 0.0) {
         face->res = res[f];
         face->coupledFlag = 1;
-        if (ls->LS_type == ORC_LS_NS) {
-          snprintf(g_err2, sizeof g_err2, "coupled Neumann faces with the NS solver are not restated by the oracle");
-          return 8;
-        }
       }
     }
   }

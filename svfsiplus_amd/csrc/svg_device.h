@@ -149,6 +149,7 @@ struct NsWork {
   double *Rm = nullptr, *Rmi = nullptr, *Rc = nullptr, *Rci = nullptr;
   double *U = nullptr, *MU = nullptr, *P = nullptr, *MP = nullptr;  // cap, 2cap, cap, 2cap vectors
   double *cgX = nullptr, *cgP = nullptr, *cgSP = nullptr, *cgGP = nullptr;
+  double* cgGP0 = nullptr;  // copy of G P read by a coupled face's op PRE (cgrad.cpp:104-107)
   CgState* cg = nullptr;
   CgState* h_cg = nullptr;
   double* coef = nullptr;   // xB on the device
@@ -182,6 +183,7 @@ struct svg_face_rec {
   int nNo = 0, dof = 0, bGrp = SVG_BC_DIR;
   bool created = false, incFlag = false, coupledFlag = false;
   double res = 0.0;
+  double nS = 0.0;  // |valM|^2 (the NS solver's bc_pre, ns_solver.cpp:56-89)
   std::vector<int> glob;
   std::vector<double> val;
   // Neumann faces: device copies for add_bc_mul; valM = val W after precond_diag

@@ -1279,10 +1279,10 @@ static int dot_device_cb(int n, const double* w, const float* base, size_t ld, i
 // S = res valM^T X in one workgroup (lane t sums the entries q = t, t+256, ...
 // of q = a nsd + i in order, a butterfly per wavefront, the 4 wavefront sums
 // in order), then Y(i, glob(a)) += valM(i, a) S.
+// X may be Y (op PRE in place): every read of X precedes the barrier.
 __global__ __launch_bounds__(256) void k_add_bc_mul(int nNo, int fdof, int nsd, int dof, const int* __restrict__ glob,
-                                                    const double* __restrict__ valM, double res,
-                                                    const double* __restrict__ X, double* __restrict__ Y,
-                                                    const int* __restrict__ skip)
+                                                    const double* __restrict__ valM, double res, const double* X,
+                                                    double* Y, const int* __restrict__ skip)
 {
 #pragma clang fp contract(off)
   __shared__ double ws[4];
@@ -1337,6 +1337,19 @@ static bool has_coupled(const svg_lhs* L)
   return false;
 }
 
+// op PRE (add_bc_mul.cpp:60-64): Y += coef valM (valM^T X), coef = -res / (1 + res nS),
+// face after face (X may be Y: each face then reads the previous faces' update).
+static void add_bc_pre_dev(const svg_lhs* L, int dof, const double* X, double* Y, const int* skip, hipStream_t s)
+{
+  for (const auto& F : L->face) {
+    if (!F.coupledFlag) continue;
+    const int nsd = F.dof < dof ? F.dof : dof;
+    const double coef = -F.res / (1.0 + (F.res * F.nS));
+    launch("add_bc_pre", k_add_bc_mul, dim3(1), dim3(256), 0, s, F.nNo, F.dof, nsd, dof, (const int*)F.d_glob,
+           (const double*)F.d_valM, coef, X, Y, skip);
+  }
+}
+
 // Y += res valM (valM^T X) for every coupled face (a no-op without one).
 static void add_bc_mul_dev(const svg_lhs* L, int dof, const double* X, double* Y, const int* skip, hipStream_t s)
 {
@@ -1353,15 +1366,22 @@ struct Op {
   const svg_lhs* L;
   const double* K;
   int dof;
+  bool pre = false;  // coupled faces: op PRE after the operator (the NS solver's inner gmres, gmres.cpp:144-150)
 };
 
 static void op_apply(const Op& A, const double* x, double* y, const double* b, const int* skip, hipStream_t s)
 {
-  if (A.dof == 4 && has_coupled(A.L)) {
-    // K x + add_bc_mul (gmres.cpp:508-512, 537-539); the residual form as b - (K x + res v v^T x)
-    spmv<4, 4, 0>(A.L, A.K, x, y, nullptr, skip, s);
-    add_bc_mul_dev(A.L, 4, x, y, skip, s);
-    if (b) launch("rsub", k_rsub, dim3(blocks_for((size_t)4 * A.L->nNo, 256)), dim3(256), 0, s, 4 * A.L->nNo, b, y);
+  if (has_coupled(A.L)) {
+    // K x + add_bc_mul (gmres.cpp:508-512, 537-539; 134-139, 186-189); the residual form as
+    // b - (K x + res v v^T x); then, for the inner gmres, op PRE in place (gmres.cpp:142-150, 191-199)
+    if (A.dof == 4)
+      spmv<4, 4, 0>(A.L, A.K, x, y, nullptr, skip, s);
+    else
+      spmv<3, 3, 0>(A.L, A.K, x, y, nullptr, skip, s, "spmv_k3");
+    add_bc_mul_dev(A.L, A.dof, x, y, skip, s);
+    const int n = A.dof * A.L->nNo;
+    if (b) launch("rsub", k_rsub, dim3(blocks_for((size_t)n, 256)), dim3(256), 0, s, n, b, y);
+    if (A.pre) add_bc_pre_dev(A.L, A.dof, y, y, skip, s);
     return;
   }
   if (A.dof == 4) {
@@ -1687,8 +1707,9 @@ static int gmres_dev(const Op& A, KrylovWork& k, svg_subls* ls, const double* R,
   for (int l = 0; l < ls->mItr; l++) {
     if (l == 0) {
       SVG_HIP(hipMemcpyAsync(k.u, R, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+      if (A.pre) add_bc_pre_dev(A.L, A.dof, k.u, k.u, nullptr, s);  // gmres.cpp:142-150
     } else {
-      op_apply(A, X, k.u, R, nullptr, s);  // u0 = R - A X
+      op_apply(A, X, k.u, R, nullptr, s);  // u0 = R - A X (with op PRE)
     }
     fa.mode = FIN_RESTART;
     fa.inc = l == 0 ? 0 : 1;
@@ -1726,6 +1747,7 @@ static int gmres_dev(const Op& A, KrylovWork& k, svg_subls* ls, const double* R,
 static int schur_dev(svg_lhs* L, svg_subls* ls, double* Pio, hipStream_t s)
 {
   auto& w = L->ns;
+  const bool coupled = has_coupled(L);
   const int nNo = L->nNo;
   const int nb = (int)blocks_for(nNo, RED_SPAN);
   double* part1 = w.partial;
@@ -1748,6 +1770,10 @@ static int schur_dev(svg_lhs* L, svg_subls* ls, double* Pio, hipStream_t s)
       }
       // GP = G P (sv); SP = L P - D GP; alpha, X, R; err, P
       spmv<3, 1, 0>(L, w.mG, w.cgP, w.cgGP, nullptr, skip, s, "spmv_sv");
+      if (coupled) {  // op PRE on G P, reading a copy of it (cgrad.cpp:103-108)
+        SVG_HIP(hipMemcpyAsync(w.cgGP0, w.cgGP, sizeof(double) * 3 * nNo, hipMemcpyDeviceToDevice, s));
+        add_bc_pre_dev(L, 3, w.cgGP0, w.cgGP, skip, s);
+      }
       launch("cg_sp", k_cg_sp, dim3(nb), dim3(RED_BS), 0, s, nNo, (const int*)L->d_rowPtr, (const int*)L->d_colPtr,
              (const double*)w.Gt, (const double*)w.mL, (const double*)w.cgGP, (const double*)w.cgP, w.cgSP, part1,
              (const CgState*)w.cg);
@@ -1825,6 +1851,7 @@ static int ns_setup(svg_lhs* L)
       (rc = dev_alloc(&w.Gt, (size_t)3 * nnz)) || (rc = dev_alloc(&w.Rm, w.ld3)) || (rc = dev_alloc(&w.Rmi, w.ld3)) ||
       (rc = dev_alloc(&w.Rc, w.ld1)) || (rc = dev_alloc(&w.Rci, w.ld1)) || (rc = dev_alloc(&w.cgX, w.ld1)) ||
       (rc = dev_alloc(&w.cgP, w.ld1)) || (rc = dev_alloc(&w.cgSP, w.ld1)) || (rc = dev_alloc(&w.cgGP, w.ld3)) ||
+      (rc = dev_alloc(&w.cgGP0, w.ld3)) ||
       (rc = dev_alloc(&w.cg, 1)) || (rc = dev_alloc(&w.dots, MAXPAIRS)))
     return rc;
   w.partialCap = (int)blocks_for(3 * nNo, RED_SPAN) * MAXPAIRS + (int)blocks_for(nNo, RED_BS);
@@ -1946,7 +1973,18 @@ static int ns_solver_dev(svg_lhs* L, svg_ls* ls, const double* Val, double* Ri, 
   eps = fmax(ls->RI.absTol, ls->RI.relTol * eps);
   launch("depart", k_depart, dim3(blocks_for(nnz, 256)), dim3(256), 0, s, nnz, Val, (const int*)L->d_trans, w.mK,
          w.mG, w.mD, w.mL, w.Gt);
-  const Op K3{L, w.mK, 3};
+  // bc_pre (ns_solver.cpp:56-89; one rank, no face is shared): nS = |valM|^2 in the
+  // reference's order, on the host from the device valM
+  for (auto& F : L->face) {
+    if (!F.coupledFlag) continue;
+    std::vector<double> vm((size_t)F.dof * F.nNo);
+    SVG_HIP(hipMemcpyAsync(vm.data(), F.d_valM, sizeof(double) * vm.size(), hipMemcpyDeviceToHost, s));
+    SVG_HIP(hipStreamSynchronize(s));
+    F.nS = 0.0;
+    for (int a = 0; a < F.nNo; a++)
+      for (int i = 0; i < 3; i++) F.nS = F.nS + std::pow(vm[i + (size_t)F.dof * a], 2.0);
+  }
+  const Op K3{L, w.mK, 3, true};
   std::vector<double> A((size_t)nB * nB, 0.0), B(nB, 0.0), xB(nB, 0.0), oldxB(nB, 0.0), tmp;
   int iBB = 0, i_count = 0;
   for (int i = 0; i < mItr; i++) {
@@ -1970,6 +2008,7 @@ static int ns_solver_dev(svg_lhs* L, svg_ls* ls, const double* Val, double* Ri, 
     launch("sub", k_sub, dim3(nb3), dim3(256), 0, s, 3 * nNo, (const double*)w.Rm, (const double*)MUiB, MUiBB);
     if ((rc = gmres_dev(K3, L->kw3, &ls->GM, MUiBB, Ui, s))) return rc;
     spmv<3, 3, 0>(L, w.mK, Ui, MUiBB, nullptr, nullptr, s, "spmv_k3");
+    add_bc_mul_dev(L, 3, Ui, MUiBB, nullptr, s);  // ns_solver.cpp:334
     // MP_iB = L P_i; MP_iBB = D U_i
     spmv<1, 1, 0>(L, w.mL, Pi, MPiB, nullptr, nullptr, s, "spmv_ss");
     spmv<1, 3, 0>(L, w.mD, Ui, MPiBB, nullptr, nullptr, s, "spmv_vs");
@@ -2822,9 +2861,9 @@ int svg_fsils_solve_ex(svg_lhs* L, svg_ls* ls, int dof, double* Ri, double* Val,
       if (F.bGrp == SVG_BC_NEU && res[f] != 0.0) {
         F.res = res[f];
         F.coupledFlag = true;
-        if (ls->LS_type != SVG_LS_GMRES)
+        if (ls->LS_type != SVG_LS_GMRES && ls->LS_type != SVG_LS_NS)
           return set_error(SVG_ERR_UNSUPPORTED,
-                           "fsils_solve: coupled (resistance) Neumann faces are built for LS type GMRES");
+                           "fsils_solve: coupled (resistance) Neumann faces are built for LS types GMRES and NS");
       }
     }
   }

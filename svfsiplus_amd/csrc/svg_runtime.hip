@@ -415,7 +415,7 @@ int svg_lhs_free(svg_lhs* L)
   }
   auto& w = L->ns;
   for (double* p : {w.mK, w.mG, w.mD, w.mL, w.Gt, w.Rm, w.Rmi, w.Rc, w.Rci, w.U, w.MU, w.P, w.MP, w.cgX, w.cgP,
-                    w.cgSP, w.cgGP, w.coef, w.dots, w.partial})
+                    w.cgSP, w.cgGP, w.cgGP0, w.coef, w.dots, w.partial})
     if (p) (void)hipFree(p);
   if (w.cg) (void)hipFree(w.cg);
   if (w.h_cg) (void)hipHostFree(w.h_cg);

@@ -55,7 +55,7 @@ def make(name, n, nz, seed, gm_tol, ns_tol):
     print(name, p.nNo, p.nEl, p.nnz, "gmres itr", og["itr"], "ns itr", on["itr"], on["GM_itr"], on["CG_itr"])
 
 
-def make_coupled(name, n, nz, res):
+def make_coupled(name, n, nz, res, ns_res=100.0, ns_tol=1e-3):
     """A coupled (resistance) outlet: the first Newton system of the pipe with
     the outlet registered as a Neumann face (val = integral of N_a n) and a
     resistance `res` passed to fsils_solve, solved by the reference's GMRES."""
@@ -70,13 +70,23 @@ def make_coupled(name, n, nz, res):
     rs = np.array([0.0, 0.0, res])
     gls = oracle.ls_create(oracle.LS_GMRES, relTol=1e-6, absTol=1e-30, maxItr=1000, dimKry=50)
     xg, Vg, og = oracle.ref_fsils_solve(p, oracle.LS_GMRES, oracle.ls_params(gls), R0, V0, faces=faces, res=rs)
+    # the NS solver with the coupled outlet (bc_pre's nS, add_bc_mul ADD in the
+    # momentum products, op PRE in its inner GMRES and Schur CG); at res = 1000 the
+    # reference's NS solver stops on its ill-conditioning check, so a smaller one
+    rn = np.array([0.0, 0.0, ns_res])
+    nls = oracle.ls_create(oracle.LS_NS, relTol=ns_tol, absTol=1e-30, maxItr=1000, dimKry=50,
+                           relTolIn=(1e-2, 1e-2), absTolIn=(1e-30, 1e-30), maxItrIn=(1000, 1000))
+    xn, Vn, on = oracle.ref_fsils_solve(p, oracle.LS_NS, oracle.ls_params(nls), R0, V0, faces=faces, res=rn)
     np.savez_compressed(
         os.path.join(OUT, name + ".npz"),
         n=n, nz=nz, x=p.x, IEN=p.IEN, rowPtr=p.rowPtr, colPtr=p.colPtr, inlet=p.faces[0][1], wall=p.faces[1][1],
         phys=np.array([ph.rho, ph.mu, ph.dt, ph.ro_inf, ph.peak]),
         R0=R0, Val0=V0, outlet=nodes, outlet_val=val, res=rs, gm_tol=1e-6, gm_x=xg,
-        gm_out=np.array([og[k] for k in ("itr", "iNorm", "fNorm", "suc")]))
-    print(name, p.nNo, p.nEl, p.nnz, "coupled gmres itr", og["itr"])
+        gm_out=np.array([og[k] for k in ("itr", "iNorm", "fNorm", "suc")]),
+        ns_res=rn, ns_tol=ns_tol, ns_x=xn,
+        ns_out=np.array([on[k] for k in ("itr", "iNorm", "fNorm", "suc", "GM_itr", "CG_itr", "Resm", "Resc")]))
+    print(name, p.nNo, p.nEl, p.nnz, "coupled gmres itr", og["itr"], "coupled ns itr", on["itr"], "GM", on["GM_itr"],
+          "CG", on["CG_itr"])
 
 
 COUPLED = [("coupled_n4", 4, 8, 1000.0)]

@@ -285,6 +285,40 @@ def test_gpu_against_reference_golden(torch_cuda, name):
     assert np.linalg.norm(x - d["gm_x"]) <= 10 * tol * np.linalg.norm(d["gm_x"])
 
 
+def test_coupled_ns_solver(torch_cuda):
+    """The NS solver with a coupled (resistance) outlet on the device: nS, add_bc_mul
+    after the momentum products, op PRE in the inner GMRES and the Schur CG —
+    bit for bit against the device-order oracle, and within the tolerance of the
+    reference's own solution (golden coupled_n4, res = 100)."""
+    import _golden as G
+    from svfsiplus_amd.newton import DeviceLhs, fsils_solve, ls_create
+    torch = torch_cuda
+    p, ph, d = G.load("coupled_n4")
+    faces = [("inlet", d["inlet"]), ("wall", d["wall"]), ("outlet", d["outlet"], "neu", d["outlet_val"])]
+    res = d["ns_res"]
+    tol = float(d["ns_tol"])
+    L = DeviceLhs(p, faces)
+    ls = ls_create(_lib.LS_NS, relTol=tol, absTol=1e-30, maxItr=1000, dimKry=50, ns_gm=(1000, 1e-2),
+                   ns_cg=(1000, 1e-2))
+    dR, dV = _dev(torch, d["R0"]), _dev(torch, d["Val0"])
+    fsils_solve(L, ls, 4, dR, dV, res=res)
+    itr, iNorm, fNorm, suc, gmi, cgi, resm, resc = d["ns_out"].tolist()
+    x = dR.cpu().numpy()
+    rel = np.linalg.norm(x - d["ns_x"]) / np.linalg.norm(d["ns_x"])
+    with oracle.device_order():
+        xd, _, ols = oracle.fsils_solve(
+            p, oracle.ls_create(oracle.LS_NS, relTol=tol, absTol=1e-30, maxItr=1000, dimKry=50, relTolIn=(1e-2, 1e-2),
+                                absTolIn=(1e-30, 1e-30), maxItrIn=(1000, 1000)), d["R0"], d["Val0"],
+            faces=faces, res=res)
+    print("coupled NS gpu RI/GM/CG", ls.RI.itr, ls.GM.itr, ls.CG.itr, "Resm/Resc", ls.Resm, ls.Resc,
+          "| reference", itr, gmi, cgi, resm, resc, "| oracle(device order)", ols.RI.itr, ols.GM.itr, ols.CG.itr,
+          "| rel to reference", rel, "max|gpu-device-order|", np.abs(x - xd).max())
+    assert (ls.RI.itr, ls.GM.itr, ls.CG.itr, ls.Resm, ls.Resc) == (ols.RI.itr, ols.GM.itr, ols.CG.itr, ols.Resm, ols.Resc)
+    assert np.array_equal(x, xd)
+    assert ls.RI.suc == suc and abs(ls.RI.itr - itr) <= 1
+    assert rel <= 1e-2
+
+
 @pytest.mark.parametrize("name", ["pipe_n2", "pipe_n4"])
 def test_ns_solver_against_reference_golden(torch_cuda, name):
     """Bipartitioned NS solver (ns_solver.cpp) on the device against the reference's record."""

@@ -88,6 +88,24 @@ def test_coupled_gmres_solves_the_coupled_system():
     assert rel_residual(x0.ravel()) > 1e-3
 
 
+def test_coupled_ns_solver_matches_reference_golden():
+    """The NS solver with a coupled (resistance) outlet — bc_pre's nS, add_bc_mul
+    ADD after the momentum products, op PRE in place in the inner GMRES and on
+    a copy of G P in the Schur CG — against the reference's own fsils_solve, bit
+    for bit; and the coupling changes the result."""
+    p, ph, d = G.load("coupled_n4")
+    faces_, _ = _coupled(d)
+    mk = lambda: oracle.ls_create(oracle.LS_NS, relTol=float(d["ns_tol"]), absTol=1e-30, maxItr=1000, dimKry=50,
+                                  relTolIn=(1e-2, 1e-2), absTolIn=(1e-30, 1e-30), maxItrIn=(1000, 1000))
+    x, V, ls = oracle.fsils_solve(p, mk(), d["R0"], d["Val0"], faces=faces_, res=d["ns_res"])
+    itr, iNorm, fNorm, suc, gmi, cgi, resm, resc = d["ns_out"].tolist()
+    assert (ls.RI.itr, ls.GM.itr, ls.CG.itr, ls.Resm, ls.Resc) == (itr, gmi, cgi, resm, resc)
+    assert ls.RI.iNorm == iNorm and ls.RI.fNorm == fNorm
+    assert np.array_equal(x, d["ns_x"])
+    x0, _, _ = oracle.fsils_solve(p, mk(), d["R0"], d["Val0"], faces=faces_, res=np.zeros(3))
+    assert not np.array_equal(x0, x)
+
+
 @pytest.mark.parametrize("name", G.NAMES)
 def test_ns_solver_matches_reference_golden(name):
     p, ph, d = G.load(name)
