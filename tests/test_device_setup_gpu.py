@@ -100,3 +100,17 @@ def test_device_setup_newton_step(torch_cuda, phys):
     for a, b in zip(h0, h1):
         assert abs(a["ls_itr"] - b["ls_itr"]) <= 2
         assert abs(a["iNorm"] - b["iNorm"]) <= 1e-6 * a["iNorm"]
+
+
+def test_device_setup_rejects_bad_connectivity(torch_cuda):
+    """A node id out of range is an error (checked on the device before any use),
+    as the host pattern build reports it."""
+    from svfsiplus_amd.newton import DeviceLhs
+    p = W.make_pipe(2)
+    ien = np.ascontiguousarray(p.IEN, dtype=np.int32).copy()
+    ien[5, 2] = p.nNo  # one past the last node
+    d_ien = torch_cuda.as_tensor(ien, device="cuda")
+    with pytest.raises(RuntimeError, match="IEN entry out of range"):
+        DeviceLhs(p, d_ien=d_ien)
+    with pytest.raises(RuntimeError, match="IEN entry out of range"):
+        W.pattern(p.nNo, ien)
