@@ -2,6 +2,7 @@
 // handle types behind the C ABI, error/launch helpers and event timing.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <string>
@@ -20,27 +21,29 @@ int hip_check(hipError_t e, const char* what);
     if (_e != hipSuccess) return ::svg::hip_check(_e, #call); \
   } while (0)
 
-// Event timing of named kernels on the launching stream, with the
-// algorithmic bytes of each launch (for achieved-bandwidth reporting).
-int prof_begin(const char* name, hipStream_t s, double bytes = 0.0);
+// Event timing of named kernels, with the algorithmic bytes of each launch
+// (for achieved-bandwidth reporting). A timed launch goes through
+// hipExtLaunchKernelGGL with the slot's start and stop events, which the
+// dispatch records at the kernel's own start and end (the duration rocprof
+// reports), not around the launch call with the gap before it.
+int prof_begin(const char* name, hipStream_t s, double bytes, hipEvent_t* start, hipEvent_t* stop);
 bool prof_set(bool on);  // returns the previous setting (event timing off under graph capture)
-void prof_end(int slot, hipStream_t s);
-
-template <typename K, typename... Args>
-inline void launch(const char* name, K kernel, dim3 grid, dim3 block, size_t shmem, hipStream_t s, Args... args)
-{
-  int slot = prof_begin(name, s);
-  hipLaunchKernelGGL(kernel, grid, block, shmem, s, args...);
-  prof_end(slot, s);
-}
 
 template <typename K, typename... Args>
 inline void launch_b(const char* name, double bytes, K kernel, dim3 grid, dim3 block, size_t shmem, hipStream_t s,
                      Args... args)
 {
-  int slot = prof_begin(name, s, bytes);
-  hipLaunchKernelGGL(kernel, grid, block, shmem, s, args...);
-  prof_end(slot, s);
+  hipEvent_t a = nullptr, b = nullptr;
+  if (prof_begin(name, s, bytes, &a, &b) >= 0)
+    hipExtLaunchKernelGGL(kernel, grid, block, (uint32_t)shmem, s, a, b, 0u, args...);
+  else
+    hipLaunchKernelGGL(kernel, grid, block, shmem, s, args...);
+}
+
+template <typename K, typename... Args>
+inline void launch(const char* name, K kernel, dim3 grid, dim3 block, size_t shmem, hipStream_t s, Args... args)
+{
+  launch_b(name, 0.0, kernel, grid, block, shmem, s, args...);
 }
 
 inline unsigned blocks_for(size_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }

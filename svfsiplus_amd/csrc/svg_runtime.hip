@@ -62,8 +62,9 @@ static hipEvent_t take_event()
   return e;
 }
 
-int prof_begin(const char* name, hipStream_t s, double bytes)
+int prof_begin(const char* name, hipStream_t s, double bytes, hipEvent_t* start, hipEvent_t* stop)
 {
+  (void)s;
   if (!g_prof_on) return -1;
   std::lock_guard<std::mutex> lk(g_prof_mu);
   if (!g_prof_only.empty() &&
@@ -76,16 +77,10 @@ int prof_begin(const char* name, hipStream_t s, double bytes)
   p.a = take_event();
   p.b = take_event();
   if (!p.a || !p.b) return -1;
-  (void)hipEventRecord(p.a, s);
+  *start = p.a;
+  *stop = p.b;
   g_pending.push_back(p);
   return (int)g_pending.size() - 1;
-}
-
-void prof_end(int slot, hipStream_t s)
-{
-  if (slot < 0) return;
-  std::lock_guard<std::mutex> lk(g_prof_mu);
-  (void)hipEventRecord(g_pending[slot].b, s);
 }
 
 static void prof_flush()
