@@ -92,7 +92,7 @@ def build_library(verbose: bool = False) -> Path:
 
 def build_oracle() -> Path:
     oracle = ROOT / "oracle"
-    _run(["make", "-s", "-C", str(oracle)])
+    _run(["make", "-s", "-B", "-C", str(oracle)])  # always rebuilt (a few seconds)
     return oracle / "liboracle.so"
 
 
