@@ -7,7 +7,7 @@
 // with picc's convergence test (pic.cpp:236-281) on the host; the state stays
 // on the device throughout.
 //
-//   svg_newton_step <n> [relTol] [krylov] [maxItr] [steps]
+//   svg_newton_step <n> [relTol] [krylov] [maxItr] [steps] [stFileName]
 //
 // prints one JSON record per Newton iteration (step, itr, ls_itr, iNorm, fNorm, suc).
 // The workload: pipe radius 0.3, length 3, n x n cross-section, 5n axial,
@@ -45,6 +45,8 @@ int main(int argc, char** argv)
   const int sD = argc > 3 ? std::atoi(argv[3]) : 50;
   const int mItr = argc > 4 ? std::atoi(argv[4]) : 1000;
   const int nSteps = argc > 5 ? std::atoi(argv[5]) : 1;
+  // restart files after every time step (Increment_in_saving_restart_files = 1), when a name is given
+  const std::string stFileName = argc > 6 ? argv[6] : "";
   if (n < 2 || n % 2 || nSteps < 1) {
     std::fprintf(stderr, "usage: %s <n> [relTol] [krylov] [maxItr] [steps]  (n even >= 2, steps >= 1)\n", argv[0]);
     return 2;
@@ -159,6 +161,8 @@ int main(int argc, char** argv)
         std::fflush(stdout);
         if (ok) break;
       }
+      if (!stFileName.empty())  // main.cpp:695-708
+        write_restart(stFileName, step, step * p.dt, 0.0, eqINorm, nNo, dYn.data(), dAn.data());
       // Ao = An, Yo = Yn (main.cpp:754-755)
       dAo.copy_from(dAn);
       dYo.copy_from(dYn);

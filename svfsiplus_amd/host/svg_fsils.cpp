@@ -1,6 +1,10 @@
 // C++ host API of the fluid Newton step over the C ABI (see svg_fsils.hpp).
 #include "svg_fsils.hpp"
 
+#include <unistd.h>
+
+#include <cstdio>
+
 namespace svg_host {
 
 void check(int rc)
@@ -126,6 +130,34 @@ void set_bc_dir(const DirichletFace& f, int tDof, int s, double dirA, double dir
                 hipStream_t stream)
 {
   check(svg_set_bc_dir(f.nNo, f.gN.data(), tDof, s, f.lDof, f.eDir, f.gxn.data(), dirA, dirY, A, Y, stream));
+}
+
+std::string write_restart(const std::string& stFileName, int cTS, double time, double cpuTime, double iNorm,
+                          int nNo, const double* d_Yn, const double* d_An, bool replace)
+{
+  const int stamp[7] = {1, 1, 1, nNo, 0, 4, 0};  // ranks, nEq, nMsh, tnNo, nX, tDof, dFlag (initialize.cpp:497)
+  std::vector<double> Y((size_t)4 * nNo), A((size_t)4 * nNo);
+  if (hipMemcpy(Y.data(), d_Yn, sizeof(double) * Y.size(), hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(A.data(), d_An, sizeof(double) * A.size(), hipMemcpyDeviceToHost) != hipSuccess)
+    throw std::runtime_error("write_restart: device copy failed");
+  char num[32];
+  std::snprintf(num, sizeof num, cTS >= 1000 ? "%d" : "%03d", cTS);
+  const std::string last = stFileName + "_last.bin";
+  const std::string fName = replace ? last : stFileName + "_" + num + ".bin";
+  std::FILE* f = std::fopen(fName.c_str(), "wb");
+  if (!f) throw std::runtime_error("write_restart: cannot open " + fName);
+  const double td[2] = {time, cpuTime};
+  bool ok = std::fwrite(stamp, sizeof(int), 7, f) == 7 && std::fwrite(&cTS, sizeof(int), 1, f) == 1 &&
+            std::fwrite(td, sizeof(double), 2, f) == 2 && std::fwrite(&iNorm, sizeof(double), 1, f) == 1 &&
+            std::fwrite(Y.data(), sizeof(double), Y.size(), f) == Y.size() &&
+            std::fwrite(A.data(), sizeof(double), A.size(), f) == A.size();
+  ok = (std::fclose(f) == 0) && ok;
+  if (!ok) throw std::runtime_error("write_restart: short write to " + fName);
+  if (!replace) {  // output.cpp:353-358: ln -f <file> <stFileName>_last.bin
+    std::remove(last.c_str());
+    if (link(fName.c_str(), last.c_str()) != 0) throw std::runtime_error("write_restart: cannot link " + last);
+  }
+  return fName;
 }
 
 }  // namespace svg_host

@@ -171,4 +171,13 @@ void dirichlet_face_create(DirichletFace& f, const std::vector<int>& gN, int lDo
 void set_bc_dir(const DirichletFace& f, int tDof, int s, double dirA, double dirY, double* A, double* Y,
                 hipStream_t stream = nullptr);
 
+// write_restart (output.cpp:202-360, header output.cpp:365-376) for one rank
+// and the fluid equation: stamp {1, 1, 1, nNo, 0, 4, 0}, cTS, time, CPU time,
+// eq.iNorm, Yn, An (4 x nNo each, dof fastest, copied from the device), into
+// <stFileName>_<cTS>.bin (cTS as %03d below 1000) with the hard link
+// <stFileName>_last.bin, or into <stFileName>_last.bin alone with replace.
+// Returns the file name.
+std::string write_restart(const std::string& stFileName, int cTS, double time, double cpuTime, double iNorm,
+                          int nNo, const double* d_Yn, const double* d_An, bool replace = false);
+
 }  // namespace svg_host

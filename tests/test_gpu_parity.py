@@ -380,21 +380,26 @@ def test_mixed_precision_solve(torch_cuda, phys, n):
     assert err <= 20 * tol
 
 
-def test_cpp_driver_matches_python_driver(torch_cuda):
+def test_cpp_driver_matches_python_driver(torch_cuda, tmp_path):
     """The C++ host driver (svg_newton_step, the reference's time loop over
     the C++ host API) and the Python driver give the same Newton records over
-    two time steps."""
+    two time steps, and write byte-identical restart files."""
     import json
     import os
     import subprocess
     exe = os.path.join(os.path.dirname(_lib.LIB_PATH), "svg_newton_step")
-    r = subprocess.run([exe, "4", "1e-6", "50", "1000", "2"], capture_output=True, text=True, timeout=120)
+    cpp = str(tmp_path / "cpp")
+    r = subprocess.run([exe, "4", "1e-6", "50", "1000", "2", cpp], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     recs = [json.loads(line) for line in r.stdout.splitlines()[1:]]
     from svfsiplus_amd.newton import FluidNewton, ls_create
     pipe = W.make_pipe(4)
     ls = ls_create(_lib.LS_GMRES, relTol=1e-6, absTol=1e-30, maxItr=1000, dimKry=50)
-    hist = [h for step in FluidNewton(pipe, W.Physics(), ls).time_steps(2) for h in step]
+    fn = FluidNewton(pipe, W.Physics(), ls)
+    hist = [h for step in fn.time_steps(2) for h in step]
+    py = fn.save_restart(str(tmp_path / "py"))
+    assert os.path.exists(cpp + "_001.bin") and os.path.samefile(cpp + "_last.bin", cpp + "_002.bin")
+    assert open(cpp + "_002.bin", "rb").read() == open(py, "rb").read()
     assert [x["step"] for x in recs] == [1, 1, 1, 2, 2, 2]
     print("C++ driver", [(x["itr"], x["ls_itr"], x["iNorm"]) for x in recs])
     print("Python driver", [(x["itr"], x["ls_itr"], x["iNorm"]) for x in hist])
