@@ -89,6 +89,13 @@ def test_dot_norm(torch_cuda, pipe4):
         assert abs(out.value - exact) <= 1e-14 * np.abs(U * V).sum() + 1e-300
         _lib.call("svg_norm_v", 1, n, dU.data_ptr(), C.byref(out), _stream())
         assert abs(out.value - np.sqrt(np.sum(U.astype(np.longdouble) ** 2))) <= 1e-14 * out.value
+    # an empty vector (nNo = 0): the sums are 0, as the reference's loops give
+    out = C.c_double(1.0)
+    _lib.call("svg_dot_v", 4, 0, dU.data_ptr(), dV.data_ptr(), C.byref(out), _stream())
+    assert out.value == 0.0
+    out = C.c_double(1.0)
+    _lib.call("svg_norm_v", 4, 0, dU.data_ptr(), C.byref(out), _stream())
+    assert out.value == 0.0
 
 
 @pytest.mark.parametrize("n", [2, 4, 8])
