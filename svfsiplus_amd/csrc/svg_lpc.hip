@@ -329,15 +329,18 @@ __global__ __launch_bounds__(BS) void k_dense(int rows, const bf16* __restrict__
                                               RowGate gate)
 {
   __shared__ __align__(16) bf16 X[TILE * XS];
-  __shared__ __align__(16) bf16 Ws[F * XS];
   if (skip && *skip) return;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int ntiles = (rows + TILE - 1) / TILE;
-  // the layer's weights once per workgroup; the workgroup then walks tiles
-  for (int q = t; q < F * 2 * F / 8; q += BS) {  // 8 bf16 per load
-    const int o = (q >> 4), k8 = (q & 15) * 8;
-    *reinterpret_cast<uint4*>(Ws + o * XS + k8) = reinterpret_cast<const uint4*>(W)[q];
-  }
+  // Wavefront w computes output columns 16w..16w+15 of every row tile: its MFMA
+  // B operands (W rows 16w + lane%16, k = 4 (lane/16) + 16 kk + 0..3) are held
+  // in registers for all of the workgroup's tiles, so LDS holds only [h | m]
+  // (17 KB) and twice as many workgroups fit on a CU as with W staged there.
+  const int fr = lane & 15, fg = lane >> 4;
+  v4s bw[2 * F / 16];
+#pragma unroll
+  for (int kk = 0; kk < 2 * F / 16; kk++)
+    bw[kk] = *reinterpret_cast<const v4s*>(W + (size_t)(16 * wave + fr) * (2 * F) + 4 * fg + 16 * kk);
   float bv[4];
 #pragma unroll
   for (int qi = 0; qi < 4; qi++) {
@@ -362,7 +365,16 @@ __global__ __launch_bounds__(BS) void k_dense(int rows, const bf16* __restrict__
     v4f acc[4];
 #pragma unroll
     for (int qi = 0; qi < 4; qi++) acc[qi] = v4f{bv[qi], bv[qi], bv[qi], bv[qi]};
-    mma_kc<TILE, F, 2 * F>(X, XS, Ws, XS, acc, wave, lane);  // B(k,n) = W[n][k]
+    // the 64 x 128 x 64 product in mma_kc's order (row tile qi, column tile = wave)
+#pragma unroll
+    for (int qi = 0; qi < 4; qi++) {
+      v4f c = acc[qi];
+      const bf16* Ar = X + (16 * qi + fr) * XS + 4 * fg;
+#pragma unroll
+      for (int kk = 0; kk < 2 * F / 16; kk++)
+        c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(*reinterpret_cast<const v4s*>(Ar + 16 * kk), bw[kk], c, 0, 0, 0);
+      acc[qi] = c;
+    }
     __syncthreads();
 #pragma unroll
     for (int qi = 0; qi < 4; qi++)
