@@ -74,59 +74,68 @@ def kernel_table(lib):
     return out
 
 
-def cpu_baseline(pipe, phys, gpu_hist):
-    """The reference's own CPU path, timed on this host on a bounded sample of
-    the same workload: its construct_fluid (oracle/_ref, compiled from the
-    reference's sources) on the first eighth of the elements (scaled to all),
-    and its fsils_solve for one 50-step GMRES cycle with the diagonal
-    preconditioner on the full assembled system; per Newton step
-    T_asm + T_pc+cycle-derived per-iteration time x the plain GPU run's GMRES
-    iterations per Newton step. Falls back to the oracle restatement
-    ("port") where the reference build is not present."""
+def host_cpu():
+    """CPU model and clock of this host (from /proc/cpuinfo)."""
+    model, mhz = platform.processor() or platform.machine(), None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k = k.strip()
+                if k == "model name" and model in ("", "x86_64", platform.processor()):
+                    model = v.strip()
+                elif k == "cpu MHz" and mhz is None:
+                    mhz = float(v)
+    except OSError:
+        pass
+    return {"model": model, "mhz": mhz, "nproc": os.cpu_count()}
+
+
+def cpu_baseline(pipe, phys, gpu_hist, cycles=4):
+    """The reference's own CPU path, timed on this host: its construct_fluid
+    (oracle/_ref, compiled from the reference's sources) over every element of
+    the workload (measured), then its fsils_solve on that assembled system for
+    `cycles` GMRES(50) restart cycles with the diagonal preconditioner
+    (measured); the Newton step is the measured assembly plus the measured
+    per-iteration solve time x the plain GPU run's GMRES iterations per Newton
+    step (that product is the extrapolated part). Falls back to the oracle
+    restatement ("port") where the reference build is not present."""
     import numpy as np
     import oracle
-    from svfsiplus_amd.workload import Pipe, dirichlet_state
+    from svfsiplus_amd.workload import dirichlet_state
     Ao, Yo, An, Yn = dirichlet_state(pipe, phys)
     Ag, Yg = oracle.pici(phys, Ao, An, Yo, Yn)
     itr_per_newton = float(np.mean([h["ls_itr"] for h in gpu_hist]))
-    R, V = oracle.construct_fluid(pipe, phys, Ag, Yg)  # the system the solve sample runs on
     sD = 50
+    cpu = host_cpu()
+    kind, asm, solve = "port", oracle.construct_fluid, None
     if oracle.ref_available():
-        try:
-            q = max(1, pipe.nEl // 8)
-            sub = Pipe(pipe.n, pipe.nz, pipe.x, np.ascontiguousarray(pipe.IEN[:q]), pipe.flags, pipe.rowPtr,
-                       pipe.colPtr, pipe.radius, pipe.length)
-            t0 = time.perf_counter()
-            oracle.ref_construct_fluid(sub, phys, Ag, Yg)
-            t_asm = (time.perf_counter() - t0) * pipe.nEl / q
-            ls = oracle.ls_create(oracle.LS_GMRES, relTol=1e-30, absTol=1e-300, maxItr=1, dimKry=sD)
-            t0 = time.perf_counter()
-            _, _, rec = oracle.ref_fsils_solve(pipe, oracle.LS_GMRES, oracle.ls_params(ls), R, V)
-            t_solve = time.perf_counter() - t0
-            t_iter = t_solve / max(rec["itr"], 1)
-            t_newton = t_asm + itr_per_newton * t_iter
-            return {"value": round(t_newton, 3), "unit": "s/Newton-step", "cores": 1, "kind": "reference",
-                    "sample": (f"the reference's construct_fluid and fsils_solve (compiled from its sources, 1 "
-                               f"thread) on the same pipe ({pipe.nEl} tets): assembly of {q} elements -> {t_asm:.2f}s for "
-                               f"all {pipe.nEl}; diagonal preconditioner + one {int(rec['itr'])}-SpMV GMRES cycle "
-                               f"{t_solve:.2f}s -> {1e3 * t_iter:.1f} ms/iteration x {itr_per_newton:.1f} "
-                               f"iterations per Newton step (the plain GPU run's count)"),
-                    "host_cpu": platform.processor() or platform.machine(), "nproc": os.cpu_count()}
-        except Exception as e:  # noqa: BLE001 - report the port instead
-            print(f"# reference CPU path unavailable ({e}); timing the oracle restatement", file=sys.stderr)
+        kind, asm = "reference", oracle.ref_construct_fluid
     t0 = time.perf_counter()
-    oracle.construct_fluid(pipe, phys, Ag, Yg)
+    R, V = asm(pipe, phys, Ag, Yg)
     t_asm = time.perf_counter() - t0
-    ls = oracle.ls_create(oracle.LS_GMRES, relTol=1e-30, absTol=1e-300, maxItr=1, dimKry=sD)
+    ls = oracle.ls_create(oracle.LS_GMRES, relTol=1e-30, absTol=1e-300, maxItr=cycles, dimKry=sD)
     t0 = time.perf_counter()
-    oracle.fsils_solve(pipe, ls, R, V)
-    t_iter = (time.perf_counter() - t0) / max(ls.RI.itr, 1)
+    if kind == "reference":
+        _, _, rec = oracle.ref_fsils_solve(pipe, oracle.LS_GMRES, oracle.ls_params(ls), R, V)
+        itr = int(rec["itr"])
+    else:
+        oracle.fsils_solve(pipe, ls, R, V)
+        itr = int(ls.RI.itr)
+    t_solve = time.perf_counter() - t0
+    t_iter = t_solve / max(itr, 1)
     t_newton = t_asm + itr_per_newton * t_iter
-    return {"value": round(t_newton, 3), "unit": "s/Newton-step", "cores": 1, "kind": "port",
-            "sample": (f"oracle (C restatement of the reference, 1 thread) on the same pipe ({pipe.nEl} tets): full "
-                       f"assembly {t_asm:.2f}s, one {ls.RI.itr}-SpMV GMRES cycle -> {1e3 * t_iter:.1f} ms/iteration "
-                       f"x {itr_per_newton:.1f} iterations per Newton step (the plain GPU run's count)"),
-            "host_cpu": platform.processor() or platform.machine(), "nproc": os.cpu_count()}
+    who = ("the reference's construct_fluid and fsils_solve (compiled from its sources)" if kind == "reference"
+           else "the oracle (C restatement of the reference)")
+    return {"value": round(t_newton, 3), "unit": "s/Newton-step", "cores": 1, "kind": kind,
+            "sample": (f"{who}, 1 thread, on the same pipe ({pipe.nEl} tets, {pipe.nNo} nodes): assembly of every "
+                       f"element measured ({t_asm:.2f}s); diagonal preconditioner + {itr} GMRES iterations "
+                       f"({cycles} restart cycles of {sD}) measured ({t_solve:.2f}s, {1e3 * t_iter:.2f} ms/iteration); "
+                       f"the solve is extrapolated to the plain GPU run's {itr_per_newton:.1f} iterations per "
+                       f"Newton step"),
+            "assembly_s": round(t_asm, 3), "solve_s_per_iteration": round(t_iter, 6),
+            "solve_iterations_measured": itr, "solve_extrapolated": True,
+            "host_cpu": cpu["model"], "host_mhz": cpu["mhz"], "nproc": cpu["nproc"]}
 
 
 def kernel_only_lpc(a, fn, pipe, lib):
@@ -287,8 +296,10 @@ def timed_run(fn, lib, steps, warmup, prof, label):
             + (f" fallback@{x['fallback_itr']}" if x.get('fallback_itr', -1) >= 0 else "") + "]" for x in h),
             file=sys.stderr, flush=True)
     n_newton = sum(len(h) for h in hists)
+    ok = bool(all(x["suc"] for h in hists for x in h))
     return {"elapsed": t1 - t0, "s_per_newton": (t1 - t0) / n_newton, "n_newton": n_newton, "hists": hists,
-            "kernels": ktab, "kernels_warmup": warm_tab, "Yn": fn.Yn.clone(), "An": fn.An.clone()}
+            "kernels": ktab, "kernels_warmup": warm_tab, "Yn": fn.Yn.clone(), "An": fn.An.clone(),
+            "all_converged": ok, "steps": steps, "warmup": warmup}
 
 
 def roofline(ktab, measured=None, variants=None):
@@ -334,6 +345,55 @@ def get_network(a, log):
     return net, info
 
 
+def mixed_draws(a, fn, lib, log):
+    """The mixed-precision workload: seeded draws of viscosity (log-uniform in
+    [0.02, 0.08]) and inflow peak (uniform in [10, 50]) on the same pipe; per
+    draw one time step with the plain FP64 GMRES and one with the FP32-SpMV /
+    FP32-basis GMRES refined in FP64 (diagonal preconditioner, true-residual
+    restarts), each timed; the mixed result's error against the FP64 one."""
+    import torch
+    from svfsiplus_amd import _lib, workload as W
+    rng = np.random.default_rng(a.c3_seed)
+    draws = []
+    for d in range(a.c3_draws):
+        mu = float(np.exp(rng.uniform(np.log(0.02), np.log(0.08))))
+        peak = float(rng.uniform(10.0, 50.0))
+        fn.set_physics(W.Physics(mu=mu, peak=peak))
+        res = {}
+        for label, prec in (("fp64", _lib.PRECISION_FP64), ("mixed", _lib.PRECISION_MIXED)):
+            fn.precision, fn.lpc = prec, None
+            fn.reset()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            h = [dict(x) for x in fn.time_step()]
+            torch.cuda.synchronize()
+            res[label] = (time.perf_counter() - t0, h, fn.Yn.clone(), fn.An.clone())
+        (t64, h64, Y64, A64), (t32, h32, Y32, A32) = res["fp64"], res["mixed"]
+
+        def rel(x, ref):
+            return float(torch.linalg.vector_norm(x - ref) / torch.linalg.vector_norm(ref))
+        rec = {"mu": round(mu, 6), "peak": round(peak, 4),
+               "s_per_newton": round(t32 / len(h32), 5), "fp64_s_per_newton": round(t64 / len(h64), 5),
+               "speedup_vs_fp64": round((t64 / len(h64)) / (t32 / len(h32)), 3),
+               "linear_iterations": [x["ls_itr"] for x in h32], "fp64_linear_iterations": [x["ls_itr"] for x in h64],
+               "all_converged": bool(all(x["suc"] for x in h32 + h64)),
+               "error_vs_fp64": {"velocity": rel(Y32[:, :3], Y64[:, :3]), "pressure": rel(Y32[:, 3], Y64[:, 3]),
+                                 "acceleration": rel(A32[:, :3], A64[:, :3]),
+                                 "final_rel_residual": [x["fNorm"] / x["iNorm"] for x in h32],
+                                 "final_rel_residual_fp64": [x["fNorm"] / x["iNorm"] for x in h64]}}
+        log(f"# mixed draw {d}: mu {mu:.4f} peak {peak:.2f}: {rec['s_per_newton']} s/Newton (fp64 "
+            f"{rec['fp64_s_per_newton']}), itr {rec['linear_iterations']} vs {rec['fp64_linear_iterations']}, "
+            f"err u {rec['error_vs_fp64']['velocity']:.2e} p {rec['error_vs_fp64']['pressure']:.2e}")
+        draws.append(rec)
+    fn.set_physics(W.Physics())
+    return {"draws": draws, "seed": a.c3_seed,
+            "s_per_newton_mean": round(float(np.mean([d["s_per_newton"] for d in draws])), 5),
+            "all_converged": bool(all(d["all_converged"] for d in draws)),
+            "solver": "GMRES(%d) with FP32 block SpMV and FP32 Krylov basis, FP64 Gram-Schmidt/Givens and FP64 "
+                      "true-residual restarts (convergence decided on the true residual), diagonal "
+                      "preconditioner; one time step x 3 Newton iterations per draw" % a.krylov}
+
+
 def batch_run(a, log):
     """The batched workload: independent bifurcating-pipe meshes (tet counts
     log-uniform in 5k-80k, daughter angles 25-45 degrees, Murray radii) solved
@@ -354,25 +414,27 @@ def batch_run(a, log):
     bn = B.BatchNewton(meshes, phys, sD=a.krylov, rel_tol=a.tol, precision=prec, lpc=net, net_period=a.net_period)
     log(f"# batch setup {time.perf_counter() - t0:.1f}s: {len(meshes)} meshes, {bn.pipe.nEl} tets, "
         f"{bn.pipe.nNo} nodes, nnzb {bn.pipe.nnz}")
-    for w in range(a.warmup):
+    for w in range(a.batch_warmup):
         bn.reset()
         bn.time_step(3)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     recs = []
-    for _ in range(a.steps):
+    for _ in range(a.batch_steps):
         bn.reset()
         recs.append(bn.time_step(3))
     torch.cuda.synchronize()
     el = time.perf_counter() - t1
-    newton = len(meshes) * 3 * a.steps
+    newton = len(meshes) * 3 * a.batch_steps
     itr = [[r["itr"] for r in it] for it in recs[-1]]
     ok = all(r["suc"] for it in recs[-1] for r in it)
     fallbacks = [sum(1 for r in it if r.get("fallback_itr", -1) >= 0) for it in recs[-1]]
     log(f"# batch: {newton} Newton steps in {el:.2f}s; max linear iterations per Newton iteration "
         f"{[max(x) for x in itr]}, all converged {ok}")
+    ok = ok and all(r["suc"] for rr in recs for it in rr for r in it)
     out = {"meshes": len(meshes), "tets": int(bn.pipe.nEl), "nodes": int(bn.pipe.nNo), "nnzb": int(bn.pipe.nnz),
-           "newton_steps_per_s": round(newton / el, 2), "s_per_time_step": round(el / a.steps, 4),
+           "newton_steps_per_s": round(newton / el, 2), "s_per_time_step": round(el / a.batch_steps, 4),
+           "steps": a.batch_steps, "warmup": a.batch_warmup,
            "precision": a.batch_precision, "method": a.batch_method, "all_converged": ok,
            "linear_iterations": {"max": [max(x) for x in itr], "mean": [round(float(np.mean(x)), 1) for x in itr]},
            "solver": "batched %sGMRES(%d), per-mesh Krylov spaces and convergence; each Newton iteration is three "
@@ -428,7 +490,14 @@ def main():
     ap.add_argument("--train-draws", type=int, default=40)
     ap.add_argument("--train-steps", type=int, default=1500)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--plain-steps", type=int, default=2, help="timed time steps of the plain FP64 leg")
+    ap.add_argument("--plain-warmup", type=int, default=1)
+    ap.add_argument("--c3-draws", type=int, default=3,
+                    help="mixed-precision leg: seeded (mu, inflow peak) draws on the same pipe (0: skip it)")
+    ap.add_argument("--c3-seed", type=int, default=20261203)
     ap.add_argument("--batch", type=int, default=256, help="meshes of the batched workload (0: skip it)")
+    ap.add_argument("--batch-steps", type=int, default=5, help="timed time steps of the batched workload")
+    ap.add_argument("--batch-warmup", type=int, default=1)
     ap.add_argument("--batch-precision", choices=["fp64", "mixed"], default="mixed")
     ap.add_argument("--batch-method", choices=["plain", "learned"], default="learned",
                     help="batched solve: diagonal preconditioner, or the learned one (flexible GMRES)")
@@ -479,7 +548,8 @@ def main():
     net, net_info = None, None
     if a.method in ("compare", "plain"):
         fn.precision, fn.lpc = _lib.PRECISION_FP64, None
-        runs["plain"] = timed_run(fn, lib, a.steps, a.warmup, not a.no_prof, "plain")
+        ps, pw = (a.steps, a.warmup) if a.method == "plain" else (a.plain_steps, a.plain_warmup)
+        runs["plain"] = timed_run(fn, lib, ps, pw, not a.no_prof, "plain")
     if a.method == "mixed":
         fn.precision, fn.lpc = _lib.PRECISION_MIXED, None
         runs["mixed"] = timed_run(fn, lib, a.steps, a.warmup, not a.no_prof, "mixed")
@@ -517,6 +587,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if head == "plain" else "f64+f32+bf16",
         "data": "synthetic (seeded pipe generator)",
+        "all_converged": R["all_converged"],
         "config": {"workload": f"VMS pipe n={a.n}{f' (stenosis {a.stenosis:g})' if a.stenosis > 0 else ''}: "
                                f"{pipe.nEl} tets, {pipe.nNo} nodes, nnzb {pipe.nnz}; "
                                f"1 time step x 3 Newton iterations; GMRES({a.krylov}) rel-tol {a.tol:g} (FP64 "
@@ -532,6 +603,7 @@ def main():
     if "plain" in runs and head != "plain":
         P = runs["plain"]
         out["plain"] = {"value": round(P["s_per_newton"], 6), "newton_records": P["hists"][-1],
+                        "steps": P["steps"], "warmup": P["warmup"], "all_converged": P["all_converged"],
                         "roofline": roofline(P["kernels"], traffic, spmv_variants(lib, fn.lhs))}
         out["speedup_vs_plain"] = round(P["s_per_newton"] / R["s_per_newton"], 3)
         # the time step's result against the FP64 plain path's (velocity, pressure, acceleration)
@@ -544,9 +616,14 @@ def main():
                                 "final_fNorm_fp64": P["hists"][-1][-1]["fNorm"]}
     if net_info:
         out["learned_pc"] = net_info
+    if a.c3_draws > 0 and a.method == "compare":
+        out["mixed"] = mixed_draws(a, fn, lib, log)
     if a.batch > 0 and a.method == "compare":
         out["batch"] = batch_run(a, log)
-    print(json.dumps(out))
+    print(json.dumps(out), flush=True)
+    if not R["all_converged"]:
+        log("# the headline run's linear solves did not all converge: the value is not a valid measurement")
+        sys.exit(3)
 
 
 if __name__ == "__main__":

@@ -173,6 +173,9 @@ int lpc_apply_dev(svg_lpc* N, const svg_lhs* L, const double* V, double* Z, floa
                   const GmState* st = nullptr);
 // Every buffer lpc_prepare_dev / lpc_apply_dev use on L, allocated ahead (graph capture).
 int lpc_reserve(svg_lpc* N, svg_lhs* L, hipStream_t s);
+// Differs between networks and changes when a network's activation buffers
+// are reallocated (a graph holding its forward pass must then be recaptured).
+unsigned long long lpc_generation(const svg_lpc* N);
 // construct_fluid split in two: the launches alone (no host synchronisation, so
 // they can be captured into a graph; the zero-Jacobian flag stays on the device)
 // and the check of that flag, which synchronises s.

@@ -1616,7 +1616,9 @@ static int gmres_v_learned(svg_lhs* L, svg_subls* ls, const double* Val, double*
   int rc = ensure_work(k, 4 * L->nNo, ls->sD);
   if (rc) return rc;
   const int n = k.n, sD = ls->sD;
-  if (!k.z && (rc = dev_alloc(&k.z, k.ld * sD))) return rc;
+  // z is sized for the workspace's capacity k.sD (>= sD), like v32: the
+  // workspace outlives this call and a later solve may use a larger sD.
+  if (!k.z && (rc = dev_alloc(&k.z, k.ld * k.sD))) return rc;
   if (mixed) {
     if (!k.u32 && (rc = dev_alloc(&k.u32, (size_t)n))) return rc;
     if (!k.v32 && (rc = dev_alloc(&k.v32, k.ld * (k.sD + 1)))) return rc;
@@ -2606,6 +2608,8 @@ static int batch_cycle_graph(svg_batch* B, const double* R, const double* Val, c
   GraphKey k;
   k.add(R).add(Val).add(fa.absTol).add(fa.relTol).add(fa.fbwin).add(B->L->d_val32).add(B->u32).add(B->v32);
   k.add(B->L->spmvVariant).add(B->L->spmv32Variant).add(md.net).add(md.period).add(B->z).add(B->d_nsc);
+  // the network's activation buffers are baked into the captured forward pass
+  k.add(lpc_generation(md.net)).add(B->L->msgVariant);
   if (B->graph[gi] && B->gKey[gi] == k.b) return SVG_OK;
   if (B->graph[gi]) (void)hipGraphExecDestroy(B->graph[gi]), B->graph[gi] = nullptr;
   hipStream_t s = B->cs_stream;

@@ -20,6 +20,7 @@
 // Training minimises L = (1/B) sum_b |r_b - A z_b|^2 with z_b = M(r_b), by
 // hand-written backward kernels (the same MFMA GEMMs transposed, the transposed
 // message passing via the block-transpose map) and Adam on FP32 master weights.
+#include <atomic>
 #include <cfloat>
 #include <cmath>
 #include <cstring>
@@ -681,9 +682,17 @@ struct svg_lpc {
   float* part = nullptr;  // weight-gradient partials
   double* sq = nullptr;   // loss partials
   double* h_sq = nullptr;
+  // Changes whenever the buffers a captured forward pass bakes in (the
+  // activations) are reallocated, and differs between networks: graphs that
+  // contain the forward pass are keyed on it.
+  unsigned long long gen = 0;
 };
 
 namespace svg {
+
+static std::atomic<unsigned long long> g_lpc_gen{0};
+
+unsigned long long lpc_generation(const svg_lpc* N) { return N ? N->gen : 0ull; }
 
 static void lpc_free_acts(svg_lpc* N)
 {
@@ -721,6 +730,7 @@ static int lpc_ensure(svg_lpc* N, int rows, bool train)
       return rc;
   }
   N->cap = cap;
+  N->gen = ++g_lpc_gen;
   return SVG_OK;
 }
 
@@ -948,6 +958,7 @@ int svg_lpc_create(svg_lpc** out)
     svg_lpc_free(N);
     return hip_check(e, "hipMemset");
   }
+  N->gen = ++g_lpc_gen;
   *out = N;
   return SVG_OK;
 }

@@ -278,6 +278,19 @@ class FluidNewton:
                      for g, v in dirichlet_faces(pipe, phys)]
         self.step = 0
 
+    def set_physics(self, phys: Physics):
+        """Another (rho, mu, inflow peak, ...) on the same mesh: the initial
+        state and the strong Dirichlet values follow the new inflow; the
+        pattern, colouring and device buffers are kept."""
+        torch = self.torch
+        self.phys = phys
+        init = dirichlet_state(self.pipe, phys)
+        self._init = [torch.tensor(a, dtype=torch.float64, device=self.Ao.device) for a in init]
+        self._dir = [(torch.tensor(g, dtype=torch.int32, device=self.Ao.device),
+                      torch.tensor(v, dtype=torch.float64, device=self.Ao.device))
+                     for g, v in dirichlet_faces(self.pipe, phys)]
+        self.reset()
+
     def reset(self):
         """Back to the initial state of the time step (device-to-device copies)."""
         for t, a in zip((self.Ao, self.Yo, self.An, self.Yn), self._init):

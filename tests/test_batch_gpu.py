@@ -167,3 +167,36 @@ def test_batch_learned_equals_individual_learned(torch_cuda, precision):
             assert bn.out[q].suc == 1
         if case == "harmful":
             assert (fb >= window).all(), fb
+
+
+def test_batch_graph_recaptured_after_network_reallocation(torch_cuda):
+    """The batched restart-cycle graph holds the network's activation buffers.
+    A training step on a larger system reallocates them; the next batched time
+    step recaptures the graph and gives, bit for bit, what a batch created
+    afresh with the same (trained) network gives."""
+    import os
+    from svfsiplus_amd import lpc as L
+    torch = torch_cuda
+    net = L.LearnedPC.load(os.path.join(os.path.dirname(_lib.LIB_PATH), "data", "lpc_weights.npz"))
+    meshes = _meshes(3, seed=17)
+    phys = W.Physics()
+    bn = B.BatchNewton(meshes, phys, precision=_lib.PRECISION_MIXED, lpc=net, net_period=5)
+    first = bn.time_step(1)
+    # training on a system larger than any mesh of the batch: new activation buffers
+    big = L.ScaledSystem(W.make_pipe(10), phys)
+    Rt = (big.R / torch.linalg.vector_norm(big.R)).reshape(1, big.n, 4).repeat(2, 1, 1).contiguous()
+    loss = C.c_double()
+    _lib.check(_lib.load().svg_lpc_train_step(net.h, big.lhs.h, big.Val.data_ptr(), 2, Rt.data_ptr(),
+                                              C.c_float(1e-4), C.byref(loss), None))
+    torch.cuda.synchronize()
+    bn.reset()
+    again = bn.time_step(1)
+    fresh = B.BatchNewton(meshes, phys, precision=_lib.PRECISION_MIXED, lpc=net, net_period=5)
+    ref = fresh.time_step(1)
+    print("before training", [r["itr"] for r in first[0]], "after (reused batch)", [r["itr"] for r in again[0]],
+          "after (fresh batch)", [r["itr"] for r in ref[0]])
+    assert again == ref
+    assert torch.equal(bn.fn.Yn, fresh.fn.Yn) and torch.equal(bn.fn.An, fresh.fn.An)
+    assert all(r["suc"] == 1 for r in again[0])
+    fresh.free()
+    bn.free()

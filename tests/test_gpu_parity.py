@@ -375,9 +375,9 @@ def test_mixed_precision_solve(torch_cuda, phys, n):
     dR, dV = _dev(torch, R), _dev(torch, V)
     fsils_solve(L, ls, 4, dR, dV, precision=_lib.PRECISION_MIXED)
     x1 = dR.cpu().numpy()
-    y = x1 / Wd                                   # unscaled back to the scaled system's unknown
     mask = Wd != 0
-    y[~mask] = 0.0
+    # unscaled back to the scaled system's unknown (0 on the Dirichlet rows, where W = 0)
+    y = np.divide(x1, Wd, out=np.zeros_like(x1), where=mask)
     res = Rs - oracle.spar_mul_vv(pipe, 4, Vs, y)
     rel_res = np.linalg.norm(res) / np.linalg.norm(Rs)
     err = np.linalg.norm(x1 - x0) / np.linalg.norm(x0)

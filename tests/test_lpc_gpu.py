@@ -3,6 +3,7 @@ backward kernels against the CPU oracle (oracle/lpc.py), training, the
 learned-preconditioned flexible GMRES (FP64 and mixed precision), its fallback
 to the diagonal preconditioner, and what a trained network gains over it."""
 import ctypes as C
+import os
 
 import numpy as np
 import pytest
@@ -57,7 +58,9 @@ def test_lpc_forward_matches_oracle(torch_cuda):
     rel = np.linalg.norm(yg - yo) / np.linalg.norm(yo)
     print(f"lpc forward: |y| {np.linalg.norm(yo):.3e} rel err gpu vs oracle {rel:.2e}")
     assert np.linalg.norm(yo) > 0.1
-    assert rel < 2e-2
+    # the kernels round to BF16 where the oracle does; what remains is the FP32
+    # accumulation order (observed ~8e-7)
+    assert rel < 1e-5
 
 
 def test_lpc_gradient_matches_oracle(torch_cuda):
@@ -72,7 +75,7 @@ def test_lpc_gradient_matches_oracle(torch_cuda):
     Val, rp, cp, dp = _host(sys)
     lo, go, _ = O.loss_grad(net.params(), Val, rp, cp, dp, R.cpu().numpy())
     print(f"lpc loss gpu {loss.value:.6e} oracle {lo:.6e}")
-    assert abs(loss.value - lo) <= 1e-2 * lo
+    assert abs(loss.value - lo) <= 1e-5 * lo
     blocks = [("W0", O.OFF_W0, O.OFF_L)]
     for l in range(O.NL):
         o = O.OFF_L + l * O.LSZ
@@ -81,7 +84,7 @@ def test_lpc_gradient_matches_oracle(torch_cuda):
     for name, a, b in blocks:
         rel = np.linalg.norm(g[a:b] - go[a:b]) / max(np.linalg.norm(go[a:b]), 1e-30)
         print(f"  grad {name}: |g| {np.linalg.norm(go[a:b]):.3e} rel err {rel:.2e}")
-        assert rel < 5e-2, name
+        assert rel < 1e-5, name  # observed <= 3e-7
 
 
 def test_lpc_training_reduces_loss(torch_cuda):
@@ -154,3 +157,34 @@ def test_trained_network_gain(torch_cuda):
               f"itr {ls1.RI.itr} (fallback {fb}) rel {rel0:.2e} / {rel1:.2e}")
         assert ls1.RI.suc == 1 and rel1 < 1e-6
         assert ls1.RI.itr < ls0.RI.itr
+
+
+def test_learned_after_larger_then_smaller_krylov_dimension(torch_cuda):
+    """One lhs solved with the plain GMRES at Krylov dimension 100, then the
+    learned flexible GMRES at 30 and at 60: the workspace kept from the first
+    solve serves the later ones (the flexible basis is sized for its capacity).
+    Each meets the FP64 true residual, and the last equals, bit for bit, the
+    same learned solve on a freshly created lhs."""
+    import torch
+    from svfsiplus_amd.newton import fsils_solve, ls_create
+    sys = _sys(torch_cuda, n=6)
+    net = L.LearnedPC.load(os.path.join(os.path.dirname(_lib.LIB_PATH), "data", "lpc_weights.npz"))
+
+    def solve(s, sD, lpc):
+        x = s.R.clone()
+        ls = ls_create(_lib.LS_GMRES, relTol=1e-6, absTol=1e-30, maxItr=1000, dimKry=sD)
+        fsils_solve(s.lhs, ls, 4, x, s.Val, prec=_lib.PREC_NONE, precision=_lib.PRECISION_MIXED, lpc=lpc,
+                    fallback_window=20, net_period=5)
+        rel = float(torch.linalg.vector_norm(s.residual(x)) / torch.linalg.vector_norm(s.R))
+        return x, ls, rel
+
+    out = {}
+    for sD, lpc in ((100, None), (30, net), (60, net)):
+        x, ls, rel = solve(sys, sD, lpc)
+        print(f"sD {sD} {'learned' if lpc else 'plain'}: itr {ls.RI.itr} true rel residual {rel:.2e}")
+        assert ls.RI.suc == 1 and rel <= 1e-6
+        out[sD] = (x, ls.RI.itr)
+    fresh = _sys(torch_cuda, n=6)
+    x, ls, rel = solve(fresh, 60, net)
+    assert ls.RI.itr == out[60][1]
+    assert torch.equal(x, out[60][0])
