@@ -351,6 +351,16 @@ int svg_precond_diag(svg_lhs* lhs, int dof, double* d_Val, double* d_R, double* 
 /* fsils_dot_v / fsi_ls_normv (dot.cpp:69-117, norm.cpp:67-112); blocking. */
 int svg_dot_v(int dof, int nNo, const double* d_U, const double* d_V, double* out, void* stream);
 int svg_norm_v(int dof, int nNo, const double* d_U, double* out, void* stream);
+/* svg_dot_v without any allocation. No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (tight solver or serving
+ * loops, harnesses that forbid hipMalloc). Same kernels, same order and a
+ * bit-identical result as svg_dot_v, but the reduction's partial sums live in
+ * a 2 MiB static device array that each GPU holds until the process ends, so
+ * dof*nNo is limited to 2^28 entries (larger sizes return SVG_ERR_ARG). Every
+ * call on one GPU shares that array: calls that overlap on one GPU (other
+ * host threads, or other streams) race and must be serialised by the caller.
+ * Blocking, like svg_dot_v. */
+int svg_dot_v_noalloc(int dof, int nNo, const double* d_U, const double* d_V, double* out, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Generalized-alpha pieces of a Newton iteration (pic.cpp).                 */

@@ -114,6 +114,7 @@ _SIGS = {
     "svg_spmv_tune": (_I, [_P, _P, _IP, _P]),
     "svg_dot_v": (_I, [_I, _I, _P, _P, _DP, _P]),
     "svg_norm_v": (_I, [_I, _I, _P, _DP, _P]),
+    "svg_dot_v_noalloc": (_I, [_I, _I, _P, _P, _DP, _P]),
     "svg_pici": (_I, [_I, _D, _D, _P, _P, _P, _P, _P, _P, _P]),
     "svg_picc_update": (_I, [_I, _D, _D, _P, _P, _P, _P]),
     "svg_fill": (_I, [_P, C.c_size_t, _D, _P]),

@@ -29,6 +29,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 namespace svg {
 
@@ -1244,6 +1245,40 @@ struct DotScratch {
   double* out = nullptr;
 };
 static DotScratch g_dot;
+
+// Scratch of svg_dot_v_noalloc: a file-scope device array (each GPU holds its
+// own copy for the life of the process), so that entry point never allocates.
+// It holds one partial sum per reduction workgroup (RED_SPAN entries each) of
+// a vector of at most kNoAllocMaxN entries, then the result slot. Every slot
+// a call reads was written earlier in the same call, so nothing is cleared.
+constexpr long kNoAllocMaxN = 1L << 28;
+constexpr int kNoAllocPartials = (int)((kNoAllocMaxN + RED_SPAN - 1) / RED_SPAN);
+constexpr int kNoAllocSlots = kNoAllocPartials + 32;  // result at kNoAllocPartials (256-byte aligned)
+static __device__ __align__(256) double scratch[kNoAllocSlots];
+
+// Host code never uses `scratch` itself (its host-side name is no device
+// address): each device's address is looked up once, with that device
+// current, and kept here.
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratchMu;
+static double* g_scratchPtr[kMaxDevices] = {};
+
+static int noalloc_scratch(double** out)
+{
+  int dev = -1;
+  SVG_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return set_error(SVG_ERR_ARG, "svg_dot_v_noalloc: device index out of range");
+  std::lock_guard<std::mutex> g(g_scratchMu);
+  if (!g_scratchPtr[dev]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_check(e, "hipGetSymbolAddress(scratch)");
+    if (!p) return set_error(SVG_ERR_HIP, "hipGetSymbolAddress(scratch) returned a null address");
+    g_scratchPtr[dev] = (double*)p;
+  }
+  *out = g_scratchPtr[dev];
+  return SVG_OK;
+}
 
 static int dot_device(int n, const double* w, const double* base, size_t ld, int m, double* partial, double* out,
                       GmState* st, FinArgs fa, double* h, double* cs, double* sn, double* err, double* coef,
@@ -2821,6 +2856,28 @@ int svg_norm_v(int dof, int nNo, const double* U, double* out, void* stream)
   int rc = dot_api(dof * nNo, U, U, &d, (hipStream_t)stream);
   if (rc) return rc;
   *out = sqrt(d);
+  return SVG_OK;
+}
+
+int svg_dot_v_noalloc(int dof, int nNo, const double* U, const double* V, double* out, void* stream)
+{
+  if (!U || !V || !out || dof <= 0 || nNo < 0) return set_error(SVG_ERR_ARG, "svg_dot_v_noalloc: bad argument");
+  const long n = (long)dof * (long)nNo;
+  if (n > kNoAllocMaxN)
+    return set_error(SVG_ERR_ARG, "svg_dot_v_noalloc: dof*nNo exceeds 2^28 entries, the most its static scratch holds");
+  double* sp = nullptr;
+  int rc = noalloc_scratch(&sp);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  FinArgs fa{};
+  fa.mode = FIN_DOT;
+  // the kernels and their order are svg_dot_v's: the result is bit-identical
+  double* res = sp + kNoAllocPartials;
+  if ((rc = dot_device((int)n, V, U, 0, 1, sp, res, nullptr, fa, nullptr, nullptr, nullptr, nullptr, nullptr, s)))
+    return rc;
+  SVG_HIP(hipGetLastError());
+  SVG_HIP(hipMemcpyAsync(out, res, sizeof(double), hipMemcpyDeviceToHost, s));
+  SVG_HIP(hipStreamSynchronize(s));
   return SVG_OK;
 }
 
