@@ -185,6 +185,50 @@ def ref_construct_fluid(pipe, phys, Ag, Yg, Bf=None):
     return R, Val
 
 
+def ref_b_assem_neu_bc(pipe, phys, Yg, faceIEN, gE, R, Val, hg=None):
+    """The reference's own b_assem_neu_bc (oracle/_ref: eq_assem.cpp:58-172 with
+    gnnb and b_fluid) on TRI3 faces faceIEN (nFe, 3) with parent elements gE:
+    accumulates into R (nNo, 4) and Val (nnz, 16) in place."""
+    lib = load_ref()
+    prm = np.array([phys.rho, phys.mu, phys.f[0], phys.f[1], phys.f[2], phys.K_darcy, phys.af, phys.am, phys.gam,
+                    phys.dt])
+    x = np.ascontiguousarray(pipe.x)
+    IEN = np.ascontiguousarray(pipe.IEN, dtype=np.int32)
+    fi = np.ascontiguousarray(faceIEN, dtype=np.int32)
+    ge = np.ascontiguousarray(gE, dtype=np.int32)
+    Yg = np.ascontiguousarray(Yg, dtype=np.float64)
+    h = None if hg is None else np.ascontiguousarray(hg, dtype=np.float64)
+    assert R.flags["C_CONTIGUOUS"] and Val.flags["C_CONTIGUOUS"]
+    rc = lib.ref_b_assem_neu_bc(C.c_int(pipe.nEl), C.c_void_p(_p(IEN)), C.c_int(pipe.nNo), C.c_void_p(_p(x)),
+                                C.c_int(fi.shape[0]), C.c_void_p(_p(fi)), C.c_void_p(_p(ge)),
+                                C.c_void_p(None if h is None else _p(h)), C.c_void_p(_p(Yg)), C.c_void_p(_p(prm)),
+                                C.c_double(phys.beta), C.c_double(phys.qmTRI3), C.c_int(pipe.nnz),
+                                C.c_void_p(_p(pipe.rowPtr)), C.c_void_p(_p(pipe.colPtr)), C.c_void_p(_p(R)),
+                                C.c_void_p(_p(Val)))
+    if rc:
+        raise RuntimeError(lib.ref_last_error().decode())
+    return R, Val
+
+
+REF_LS_UNSET = -7777.0  # ref_ls_create's value for a field fsils_ls_create leaves unset
+
+
+def ref_ls_create(ls_type):
+    """The fields the reference's fsils_ls_create (ls.cpp:40-120, no optional
+    argument) sets: {"RI"|"GM"|"CG": {relTol, absTol, mItr, sD}, "LS_type", "foC"};
+    a field it leaves unset reads REF_LS_UNSET."""
+    lib = load_ref()
+    out = np.zeros(16)
+    rc = lib.ref_ls_create(C.c_int(ls_type), C.c_void_p(_p(out)))
+    if rc:
+        raise RuntimeError(lib.ref_last_error().decode())
+    d = {}
+    for k, name in enumerate(("RI", "GM", "CG")):
+        d[name] = dict(zip(("relTol", "absTol", "mItr", "sD"), out[4 * k:4 * k + 4].tolist()))
+    d["LS_type"], d["foC"] = int(out[12]), bool(out[13])
+    return d
+
+
 class OracleLhs:
     """Holds the numpy arrays behind an orc_lhs."""
 

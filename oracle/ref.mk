@@ -28,7 +28,8 @@ CXXFLAGS = -std=c++17 -O2 -fPIC -ffunction-sections -fdata-sections -w \
 
 LS_SRC     := $(wildcard $(LS)/*.cpp)
 SOLVER_TUS := fluid nn fs lhsa utils Array Array3 Vector all_fun ComMod CmMod consts CepMod \
-              mat_fun Tensor4 FsilsLinearAlgebra LinearAlgebra PetscLinearAlgebra TrilinosLinearAlgebra
+              mat_fun Tensor4 FsilsLinearAlgebra LinearAlgebra PetscLinearAlgebra TrilinosLinearAlgebra \
+              eq_assem heats heatf l_elas cep
 SOLVER_SRC := $(addprefix $(SOLVER)/,$(addsuffix .cpp,$(SOLVER_TUS)))
 
 LS_OBJ     := $(patsubst $(LS)/%.cpp,$(OBJ)/ls_%.o,$(LS_SRC))

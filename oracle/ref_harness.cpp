@@ -12,6 +12,7 @@
 //                          then the <LS> overrides of read_files.cpp:2056-2090
 #include "ComMod.h"
 #include "FsilsLinearAlgebra.h"
+#include "eq_assem.h"
 #include "fluid.h"
 #include "fs.h"
 #include "fsils_api.hpp"
@@ -192,4 +193,143 @@ extern "C" int ref_fsils_solve(int nNo, int nnz, const int* rowPtr, const int* c
 {
   return ref_fsils_solve_bc(nNo, nnz, rowPtr, colPtr, nFaces, faceNNo, faceGlob, faceVal, nullptr, nullptr, LS_type,
                             ls_prm, dof, Ri, Val, prec, out);
+}
+
+// fsils_ls_create (ls.cpp:40-120) with no optional argument: the fields it sets
+// for LS_type. Each field starts at the sentinel -7777 (integers) or -7777.0,
+// so a field the reference leaves unset comes back as the sentinel.
+// out (16 doubles): RI.{relTol, absTol, mItr, sD}, GM.{...}, CG.{...},
+//                   then LS_type, foC
+extern "C" int ref_ls_create(int LS_type, double* out)
+{
+  using namespace fsi_linear_solver;
+  try {
+    FSILS_lsType ls;
+    for (FSILS_subLsType* s : {&ls.RI, &ls.GM, &ls.CG}) {
+      s->relTol = -7777.0;
+      s->absTol = -7777.0;
+      s->mItr = -7777;
+      s->sD = -7777;
+    }
+    fsils_ls_create(ls, static_cast<LinearSolverType>(LS_type));
+    int k = 0;
+    for (const FSILS_subLsType* s : {&ls.RI, &ls.GM, &ls.CG}) {
+      out[k++] = s->relTol;
+      out[k++] = s->absTol;
+      out[k++] = s->mItr;
+      out[k++] = s->sD;
+    }
+    out[k++] = static_cast<int>(ls.LS_type);
+    out[k++] = ls.foC ? 1.0 : 0.0;
+  } catch (const std::exception& ex) {
+    g_ref_err = ex.what();
+    return 1;
+  }
+  return 0;
+}
+
+// The fluid equation's Neumann-face assembly through the reference's own
+// eq_assem::b_assem_neu_bc (eq_assem.cpp:58-172: nn::gnnb for the outward
+// normal, the TRI3 face rule of nn_elem_gip.h, fluid::b_fluid fluid.cpp:46-133
+// and assembly through FsilsLinearAlgebra / do_assem) on one face of TRI3
+// elements of a TET4 mesh. faceIEN(3, nFe) node ids, gE(nFe) the parent
+// element of each face element (index into IEN), hg(nNo) the Neumann value
+// per node (null: zero), Yg(4, nNo). prm as for ref_construct_fluid; beta the
+// backflow stabilisation coefficient, qmTRI3 the face quadrature modifier.
+// R(4, nNo) and Val(16, nnz) are accumulated into.
+extern "C" int ref_b_assem_neu_bc(int nEl, const int* IEN, int nNo, const double* x, int nFe, const int* faceIEN,
+                                  const int* gE, const double* hg, const double* Yg, const double* prm, double beta,
+                                  double qmTRI3, int nnz, const int* rowPtr, const int* colPtr, double* R,
+                                  double* Val)
+{
+  using namespace consts;
+  try {
+    ComMod com_mod;
+    com_mod.nsd = 3;
+    com_mod.nsymd = 6;
+    com_mod.dof = 4;
+    com_mod.tDof = 4;
+    com_mod.tnNo = nNo;
+    com_mod.cEq = 0;
+    com_mod.mvMsh = false;
+    com_mod.dt = prm[9];
+    com_mod.x.resize(3, nNo);
+    for (int a = 0; a < nNo; a++)
+      for (int i = 0; i < 3; i++) com_mod.x(i, a) = x[i + 3 * a];
+    com_mod.rowPtr.resize(nNo + 1);
+    com_mod.colPtr.resize(nnz);
+    for (int i = 0; i <= nNo; i++) com_mod.rowPtr(i) = rowPtr[i];
+    for (int i = 0; i < nnz; i++) com_mod.colPtr(i) = colPtr[i];
+    com_mod.R.resize(4, nNo);
+    com_mod.Val.resize(16, nnz);
+    std::memcpy(com_mod.R.data(), R, sizeof(double) * 4 * nNo);
+    std::memcpy(com_mod.Val.data(), Val, sizeof(double) * 16 * (size_t)nnz);
+
+    com_mod.eq.resize(1);
+    auto& eq = com_mod.eq[0];
+    eq.af = prm[6];
+    eq.am = prm[7];
+    eq.gam = prm[8];
+    eq.dof = 4;
+    eq.nDmn = 1;
+    eq.dmn.resize(1);
+    auto& dmn = eq.dmn[0];
+    dmn.Id = -1;
+    dmn.phys = EquationType::phys_fluid;
+    dmn.prop[PhysicalProperyType::fluid_density] = prm[0];
+    dmn.prop[PhysicalProperyType::backflow_stab] = beta;
+    dmn.fluid_visc.viscType = FluidViscosityModelType::viscType_Const;
+    dmn.fluid_visc.mu_i = prm[1];
+    FsilsLinearAlgebra la;
+    eq.linear_algebra = &la;
+
+    com_mod.nMsh = 1;
+    com_mod.msh.resize(1);
+    mshType& msh = com_mod.msh[0];
+    msh.name = "volume";
+    msh.nEl = nEl;
+    msh.gnEl = nEl;
+    msh.eNoN = 4;
+    msh.nFs = 1;
+    msh.IEN.resize(4, nEl);
+    for (int e = 0; e < nEl; e++)
+      for (int a = 0; a < 4; a++) msh.IEN(a, e) = IEN[a + 4 * e];
+    nn::select_ele(com_mod, msh);
+    fs::init_fs_msh(com_mod, msh);
+
+    // the face as select_eleb sets it up for a 3-noded face of a 3D mesh
+    // (nn_elem_props.h: TRI3, nG = 3); get_gip/get_gnn of a face use only the face
+    faceType lFa;
+    lFa.name = "outlet";
+    lFa.iM = 0;
+    lFa.nEl = nFe;
+    lFa.gnEl = nFe;
+    lFa.eNoN = 3;
+    lFa.eType = ElementType::TRI3;
+    lFa.nG = 3;
+    lFa.qmTRI3 = qmTRI3;
+    lFa.IEN.resize(3, nFe);
+    lFa.gE.resize(nFe);
+    for (int e = 0; e < nFe; e++) {
+      for (int a = 0; a < 3; a++) lFa.IEN(a, e) = faceIEN[a + 3 * e];
+      lFa.gE(e) = gE[e];
+    }
+    lFa.w = Vector<double>(lFa.nG);
+    lFa.xi = Array<double>(2, lFa.nG);
+    nn::get_gip(nullptr, lFa);
+    lFa.N = Array<double>(lFa.eNoN, lFa.nG);
+    lFa.Nx = Array3<double>(2, lFa.eNoN, lFa.nG);
+    for (int g = 0; g < lFa.nG; g++) nn::get_gnn(nullptr, g, lFa);
+
+    Vector<double> vhg(nNo);
+    for (int a = 0; a < nNo; a++) vhg(a) = hg ? hg[a] : 0.0;
+    eq_assem::b_assem_neu_bc(com_mod, lFa, vhg, Array<double>(4, nNo, const_cast<double*>(Yg)));
+    std::memcpy(R, com_mod.R.data(), sizeof(double) * 4 * nNo);
+    std::memcpy(Val, com_mod.Val.data(), sizeof(double) * 16 * (size_t)nnz);
+    eq.linear_algebra = nullptr;
+  } catch (const std::exception& ex) {
+    g_ref_err = ex.what();
+    return 1;
+  }
+  return 0;
 }

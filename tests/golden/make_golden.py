@@ -92,6 +92,45 @@ def make_coupled(name, n, nz, res, ns_res=100.0, ns_tol=1e-3):
 COUPLED = [("coupled_n4", 4, 8, 1000.0)]
 
 
+def make_backflow(name, n, nz, seed):
+    """The outlet's Neumann-face assembly by the reference's own
+    b_assem_neu_bc (gnnb, the TRI3 face rule, b_fluid, do_assem) on zero R and
+    Val, for a velocity that flows back through part of the outlet (u_z drawn
+    in [-20, 20], transverse components nonzero) and a nonzero Neumann value."""
+    p = W.make_pipe(n, nz)
+    ph = W.Physics()
+    rng = np.random.default_rng(seed)
+    Yg = np.zeros((p.nNo, 4))
+    Yg[:, :2] = rng.uniform(-3.0, 3.0, (p.nNo, 2))
+    Yg[:, 2] = rng.uniform(-20.0, 20.0, p.nNo)
+    Yg[:, 3] = rng.standard_normal(p.nNo)
+    hg = rng.standard_normal(p.nNo)
+    fi, ge, inner = W.outlet_faces(p)
+    R = np.zeros((p.nNo, 4))
+    V = np.zeros((p.nnz, 16))
+    oracle.ref_b_assem_neu_bc(p, ph, Yg, fi, ge, R, V, hg=hg)
+    np.savez_compressed(
+        os.path.join(OUT, name + ".npz"),
+        n=n, nz=nz, x=p.x, IEN=p.IEN, rowPtr=p.rowPtr, colPtr=p.colPtr, inlet=p.faces[0][1], wall=p.faces[1][1],
+        phys=np.array([ph.rho, ph.mu, ph.dt, ph.ro_inf, ph.peak]), beta=ph.beta, qmTRI3=ph.qmTRI3,
+        faceIEN=fi, gE=ge, inner=inner, Yg=Yg, hg=hg, R=R, Val=V)
+    back = int(np.sum(Yg[np.unique(fi), 2] > 0)), int(np.sum(Yg[np.unique(fi), 2] < 0))
+    print(name, p.nNo, p.nEl, p.nnz, "outlet faces", len(fi), "outlet nodes out/back", back)
+
+
+def make_ls_defaults(name):
+    """fsils_ls_create's defaults (ls.cpp:40-120) for every LS type, from the
+    reference's own function; fields it leaves unset read -7777."""
+    import json
+    d = {str(t): oracle.ref_ls_create(t) for t in (oracle.LS_NS, oracle.LS_GMRES, oracle.LS_CG, 795)}
+    with open(os.path.join(OUT, name + ".json"), "w") as f:
+        json.dump(d, f, indent=1, sort_keys=True)
+    print(name, sorted(d))
+
+
+BACKFLOW = [("backflow_n4", 4, 8, 21)]
+
+
 if __name__ == "__main__":
     if not oracle.ref_available():
         sys.exit("oracle/_ref/libsvref.so is not built (make -f oracle/ref.mk)")
@@ -102,3 +141,8 @@ if __name__ == "__main__":
     for c in COUPLED:
         if not only or c[0] in only:
             make_coupled(*c)
+    for c in BACKFLOW:
+        if not only or c[0] in only:
+            make_backflow(*c)
+    if not only or "ls_defaults" in only:
+        make_ls_defaults("ls_defaults")

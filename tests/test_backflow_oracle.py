@@ -90,3 +90,43 @@ def test_face_node_order_does_not_matter():
     R2, V2 = _run(p, ph, np.ascontiguousarray(fi[:, ::-1]), inner, Yg)
     assert np.linalg.norm(R1 - R2) <= 1e-13 * np.linalg.norm(R1)
     assert np.linalg.norm(V1 - V2) <= 1e-13 * np.linalg.norm(V1)
+
+
+def _golden_backflow():
+    from tests import _golden as G
+    p, ph, d = G.load("backflow_n4")
+    ph.beta, ph.qmTRI3 = float(d["beta"]), float(d["qmTRI3"])
+    return p, ph, d
+
+
+def test_oracle_equals_reference_golden():
+    """The oracle against the reference's own b_assem_neu_bc (golden fixture
+    written by tests/golden/make_golden.py): a velocity flowing back through
+    part of the outlet with transverse components, and a nonzero Neumann value.
+    Bit for bit."""
+    p, ph, d = _golden_backflow()
+    assert np.any(d["Yg"][np.unique(d["faceIEN"]), 2] > 0) and np.any(d["Yg"][np.unique(d["faceIEN"]), 2] < 0)
+    R, V = _run(p, ph, d["faceIEN"], d["inner"], d["Yg"], hg=d["hg"])
+    assert np.linalg.norm(d["R"]) > 0 and np.linalg.norm(d["Val"]) > 0
+    assert np.array_equal(R, d["R"])
+    assert np.array_equal(V, d["Val"])
+
+
+def test_oracle_equals_live_reference():
+    """The same against the reference compiled here (oracle/_ref), on other
+    seeded states and on a second mesh size, accumulated onto nonzero R, Val."""
+    import pytest
+    if not oracle.ref_available():
+        pytest.skip("oracle/_ref is not built (no reference sources here)")
+    for n, seed in ((2, 3), (4, 4), (6, 5)):
+        p = W.make_pipe(n)
+        ph = W.Physics()
+        rng = np.random.default_rng(seed)
+        Yg = rng.uniform(-15.0, 15.0, (p.nNo, 4))
+        hg = rng.standard_normal(p.nNo)
+        fi, ge, inner = W.outlet_faces(p)
+        R0 = rng.standard_normal((p.nNo, 4))
+        V0 = rng.standard_normal((p.nnz, 16))
+        R1, V1 = oracle.ref_b_assem_neu_bc(p, ph, Yg, fi, ge, R0.copy(), V0.copy(), hg=hg)
+        R2, V2 = oracle.b_assem_neu_fluid(p, ph, Yg, fi, inner, R0.copy(), V0.copy(), hg=hg)
+        assert np.array_equal(R1, R2) and np.array_equal(V1, V2), n

@@ -447,6 +447,35 @@ def test_b_assem_neu_bc_matches_oracle(torch_cuda, pipe4, phys):
     assert np.linalg.norm(V1 - V0) <= 1e-12 * np.linalg.norm(V0)
 
 
+def test_b_assem_neu_bc_against_reference_golden(torch_cuda):
+    """The device's outlet Neumann/backflow assembly against what the
+    reference's own b_assem_neu_bc (gnnb, b_fluid, do_assem) gave on the same
+    mesh, velocity (flowing back through part of the outlet) and Neumann value
+    (tests/golden/backflow_n4.npz): norm-wise and entry-wise to 1e-12."""
+    import _golden as G
+    from svfsiplus_amd.newton import DeviceFace, DeviceLhs, DeviceMesh, b_assem_neu_bc
+    torch = torch_cuda
+    p, ph, d = G.load("backflow_n4")
+    ph.beta, ph.qmTRI3 = float(d["beta"]), float(d["qmTRI3"])
+    L = DeviceLhs(p)
+    M = DeviceMesh(L, p)
+    Fa = DeviceFace(M, d["faceIEN"], d["gE"])
+    dR = torch.zeros((p.nNo, 4), dtype=torch.float64, device="cuda")
+    dV = torch.zeros((p.nnz, 16), dtype=torch.float64, device="cuda")
+    b_assem_neu_bc(Fa, ph, _dev(torch, d["Yg"]), dR, dV, hg=_dev(torch, d["hg"]))
+    torch.cuda.synchronize()
+    R1, V1 = dR.cpu().numpy(), dV.cpu().numpy()
+    eR = np.linalg.norm(R1 - d["R"]) / np.linalg.norm(d["R"])
+    eV = np.linalg.norm(V1 - d["Val"]) / np.linalg.norm(d["Val"])
+    print(f"backflow vs reference golden: R rel {eR:.2e} (max {np.abs(R1 - d['R']).max():.2e}), "
+          f"Val rel {eV:.2e} (max {np.abs(V1 - d['Val']).max():.2e})")
+    assert eR <= 1e-12 and eV <= 1e-12
+    assert np.abs(R1 - d["R"]).max() <= 1e-12 * np.abs(d["R"]).max()
+    assert np.abs(V1 - d["Val"]).max() <= 1e-12 * np.abs(d["Val"]).max()
+    for o in (Fa, M, L):
+        o.free()
+
+
 
 def test_spmv32_matches_fp64_product(torch_cuda, pipe4):
     """The FP32 block SpMV of the mixed-precision solve against the FP64 product

@@ -148,3 +148,32 @@ def test_bench_refuses_more_than_one_gpu():
     r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
                        text=True, timeout=120)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def _ls_fields(ls):
+    return {name: {f: getattr(getattr(ls, name), f) for f in ("relTol", "absTol", "mItr", "sD")}
+            for name in ("RI", "GM", "CG")}
+
+
+def test_ls_create_defaults_match_reference():
+    """svg_ls_create (host code, no device call) against the reference's own
+    fsils_ls_create (ls.cpp:40-120) for every LS type: the fixture
+    tests/golden/ls_defaults.json written from it, and the reference compiled
+    here when present. Fields the reference leaves unset (-7777 in the fixture)
+    are 0 here."""
+    import json
+    import oracle
+    with open(os.path.join(ROOT, "tests", "golden", "ls_defaults.json")) as f:
+        golden = json.load(f)
+    live = {t: oracle.ref_ls_create(int(t)) for t in golden} if oracle.ref_available() else {}
+    assert set(golden) == {str(t) for t in (_lib.LS_NS, _lib.LS_GMRES, _lib.LS_CG, _lib.LS_BICGS)}
+    for t, ref in golden.items():
+        if t in live:
+            assert live[t] == ref, t
+        ls = _lib.Ls()
+        _lib.call("svg_ls_create", C.byref(ls), int(t))
+        assert ls.LS_type == ref["LS_type"] == int(t)
+        got = _ls_fields(ls)
+        for name in ("RI", "GM", "CG"):
+            for f, r in ref[name].items():
+                assert got[name][f] == (0 if r == -7777.0 else r), (t, name, f)
