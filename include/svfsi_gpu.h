@@ -241,6 +241,12 @@ int svg_lpc_loss_grad(svg_lpc* net, svg_lhs* lhs, const double* d_Val, int nvec,
 /* One Adam step (beta 0.9/0.999) with learning rate lr on that loss. */
 int svg_lpc_train_step(svg_lpc* net, svg_lhs* lhs, const double* d_Val, int nvec, const double* d_R, float lr,
                        double* loss, void* stream);
+/* The same step on the loss (1/nvec) sum_b |T_b - A M(V_b)|^2 with network
+ * input d_V and target d_T (d_T NULL: T = V, svg_lpc_train_step). The network
+ * as the smoother of the two-level preconditioner is trained with V = the
+ * block-Jacobi output and T = the coarse-corrected residual. */
+int svg_lpc_train_step_ex(svg_lpc* net, svg_lhs* lhs, const double* d_Val, int nvec, const double* d_V,
+                          const double* d_T, float lr, double* loss, void* stream);
 
 /* Solve options beyond the reference (no counterpart in fsils_solve):
  *   precision SVG_PRECISION_MIXED (LS type GMRES): Arnoldi cycles use an FP32
@@ -267,12 +273,39 @@ typedef struct {
   int fallback_itr; /* out */
   int net_period;   /* learned: the network preconditions Arnoldi steps i with i % net_period == 0, the
                        others use the diagonal scaling alone (0 or 1: every step) */
-  int reserved[3];
+  int coarse;       /* learned: > 0 adds the two-level preconditioner on every Arnoldi step, with node
+                       aggregates of about `coarse` nodes (0: none); `lpc` may then be NULL */
+  int reserved[2];
   svg_lpc* lpc;
 } svg_solve_opts;
 
 int svg_fsils_solve_ex(svg_lhs* lhs, svg_ls* ls, int dof, double* d_Ri, double* d_Val, int prec, int nIncL,
                        const int* incL, const double* res, svg_solve_opts* opts, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Two-level preconditioner (no counterpart in the reference; it stands      */
+/* beside precond_diag, precond.cpp:122-257, and is used by the flexible     */
+/* GMRES through svg_solve_opts.coarse). On the diagonally scaled system A:  */
+/*   M(v) = P zc + B (v - A P zc),  zc = Ac^-1 P^T v,  Ac = P^T A P           */
+/* P: piecewise-constant prolongation of the pressure (dof 3) over node      */
+/* aggregates grown breadth-first over the lhs's block pattern (about aggSize */
+/* nodes each; aggregates under aggSize/4 nodes join the neighbour they share */
+/* most edges with); Ac: dense (at most 8192 aggregates), inverted in FP64 by */
+/* blocked Gauss-Jordan elimination and applied in FP32; B: the inverses of  */
+/* the 4x4 diagonal blocks (FP32).                                            */
+/* svg_coarse_prepare builds the aggregates (once per lhs and aggSize) and    */
+/* this system's operators from the scaled d_Val (synchronises stream).       */
+/* svg_coarse_apply on nvec stacked vectors (nvec, nNo, 4): mode              */
+/* SVG_COARSE_APPLY gives Z = M(V); SVG_COARSE_RESIDUAL gives the coarse-     */
+/* corrected residual Z = V - A P zc (what a smoother other than B receives); */
+/* SVG_COARSE_SMOOTH gives Z = B (V - A P zc).                                */
+/* svg_coarse_info: the number of aggregates and (agg != NULL) each node's.   */
+#define SVG_COARSE_APPLY 0
+#define SVG_COARSE_RESIDUAL 1
+#define SVG_COARSE_SMOOTH 2 /* Z = B (V - A P zc): the block-Jacobi smoothing without P zc */
+int svg_coarse_prepare(svg_lhs* lhs, int aggSize, const double* d_Val, void* stream);
+int svg_coarse_apply(const svg_lhs* lhs, int mode, int nvec, const double* d_V, double* d_Z, void* stream);
+int svg_coarse_info(const svg_lhs* lhs, int* nAgg, int* agg);
 
 /* ------------------------------------------------------------------------ */
 /* Batched GMRES (no counterpart in the reference): many independent systems  */

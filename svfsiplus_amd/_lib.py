@@ -58,10 +58,12 @@ class SubLs(C.Structure):
 
 class SolveOpts(C.Structure):
     _fields_ = [("precision", C.c_int), ("method", C.c_int), ("fallback_window", C.c_int),
-                ("fallback_itr", C.c_int), ("net_period", C.c_int), ("reserved", C.c_int * 3), ("lpc", C.c_void_p)]
+                ("fallback_itr", C.c_int), ("net_period", C.c_int), ("coarse", C.c_int), ("reserved", C.c_int * 2),
+                ("lpc", C.c_void_p)]
 
 
 PRECISION_FP64, PRECISION_MIXED = 0, 1
+COARSE_APPLY, COARSE_RESIDUAL, COARSE_SMOOTH = 0, 1, 2
 METHOD_PLAIN, METHOD_LEARNED = 0, 1
 
 
@@ -115,6 +117,9 @@ _SIGS = {
     "svg_dot_v": (_I, [_I, _I, _P, _P, _DP, _P]),
     "svg_norm_v": (_I, [_I, _I, _P, _DP, _P]),
     "svg_dot_v_noalloc": (_I, [_I, _I, _P, _P, _DP, _P]),
+    "svg_coarse_prepare": (_I, [_P, _I, _P, _P]),
+    "svg_coarse_apply": (_I, [_P, _I, _I, _P, _P, _P]),
+    "svg_coarse_info": (_I, [_P, _IP, _P]),
     "svg_pici": (_I, [_I, _D, _D, _P, _P, _P, _P, _P, _P, _P]),
     "svg_picc_update": (_I, [_I, _D, _D, _P, _P, _P, _P]),
     "svg_fill": (_I, [_P, C.c_size_t, _D, _P]),
@@ -144,6 +149,7 @@ _SIGS = {
     "svg_lpc_apply": (_I, [_P, _P, _I, _P, _P, _P]),
     "svg_lpc_loss_grad": (_I, [_P, _P, _P, _I, _P, _DP, _P, _P]),
     "svg_lpc_train_step": (_I, [_P, _P, _P, _I, _P, C.c_float, _DP, _P]),
+    "svg_lpc_train_step_ex": (_I, [_P, _P, _P, _I, _P, _P, C.c_float, _DP, _P]),
 }
 
 EXPORTED = tuple(_SIGS)

@@ -20,7 +20,8 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libsvgpu.so"
 
-HIP_SOURCES = ["svg_runtime.hip", "svg_fsils.hip", "svg_assembly.hip", "svg_lpc.hip", "svg_setup.hip"]
+HIP_SOURCES = ["svg_runtime.hip", "svg_fsils.hip", "svg_assembly.hip", "svg_lpc.hip", "svg_setup.hip",
+               "svg_coarse.hip"]
 CPP_SOURCES = ["host_setup.cpp"]
 HOST_DIR = PKG / "host"
 HOST_SOURCES = ["svg_fsils.cpp"]            # C++ host API (into the library)

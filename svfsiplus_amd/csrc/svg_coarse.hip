@@ -1,0 +1,693 @@
+// Two-level preconditioner of the diagonally scaled fluid system (no
+// counterpart in the reference; it stands beside precond_diag,
+// liner_solver/precond.cpp:122-257, and is used only by the flexible GMRES of
+// svg_fsils_solve_ex):
+//
+//   M(v) = P zc + B (v - A P zc),   zc = Ac^-1 P^T v,   Ac = P^T A P
+//
+//   P   piecewise-constant prolongation of the pressure (dof 3) over node
+//       aggregates: the global pressure modes that a local smoother cannot
+//       reach (the Laplacian-like continuity block of the VMS P1-P1 system,
+//       fluid.cpp:1662-1688) are solved on the aggregates
+//   Ac  dense nAgg x nAgg; on the diagonally scaled system it is symmetric and
+//       negative definite, so it is inverted by blocked Gauss-Jordan elimination
+//       without pivoting (FP64), and applied as an FP32 matrix-vector product
+//   B   block Jacobi: the inverses of the 4x4 diagonal blocks of A (FP32)
+//
+// Aggregates are grown breadth-first over the lhs's block pattern (a greedy
+// aggregation that needs no coordinates, so it sees only what fsils_solve
+// sees), with undersized ones merged into the neighbour they share most
+// edges with. Every reduction runs in a fixed order, so the preconditioner is
+// deterministic.
+#include "svg_device.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <numeric>
+#include <vector>
+
+namespace svg {
+
+constexpr int GJB = 64;              // Gauss-Jordan block size
+constexpr int COARSE_MAX = 8192;     // most aggregates (dense FP64 Ac: 512 MB)
+
+struct CoarseLevel {
+  int aggSize = 0;    // target nodes per aggregate this level was built for
+  int nAgg = 0, N = 0, nAP = 0, nSeg = 0, nNo = 0;
+  int *d_agg = nullptr, *d_aggPtr = nullptr, *d_aggNodes = nullptr;
+  int *d_apPtr = nullptr, *d_apAgg = nullptr, *d_apOf = nullptr;
+  int *d_segPtr = nullptr, *d_segIdx = nullptr, *d_segPerm = nullptr;
+  float* d_apVal = nullptr;   // (4, nAP): column 3 of A summed over each AP entry's blocks
+  double* d_apPP = nullptr;   // (nAP): the pressure-pressure entry of the same sums, FP64
+  float* d_dinv = nullptr;    // (16, nNo): inverses of the diagonal blocks, row-major
+  double* d_Ac = nullptr;     // N x N, inverted in place
+  float* d_Ai = nullptr;      // N x N FP32 copy of Ac^-1
+  double *d_piv = nullptr, *d_cp = nullptr, *d_rp = nullptr;  // Gauss-Jordan panels
+  float *d_rc = nullptr, *d_zc = nullptr;                     // restricted vector, coarse solution
+  int* d_flag = nullptr;
+  int* h_flag = nullptr;
+};
+
+void coarse_free(CoarseLevel*& c)
+{
+  if (!c) return;
+  for (int* p : {c->d_agg, c->d_aggPtr, c->d_aggNodes, c->d_apPtr, c->d_apAgg, c->d_apOf, c->d_segPtr, c->d_segIdx,
+                 c->d_segPerm, c->d_flag})
+    if (p) (void)hipFree(p);
+  for (float* p : {c->d_apVal, c->d_dinv, c->d_Ai, c->d_rc, c->d_zc})
+    if (p) (void)hipFree(p);
+  for (double* p : {c->d_apPP, c->d_Ac, c->d_piv, c->d_cp, c->d_rp})
+    if (p) (void)hipFree(p);
+  if (c->h_flag) (void)hipHostFree(c->h_flag);
+  delete c;
+  c = nullptr;
+}
+
+// ------------------------------------------------------------- host setup --
+// Breadth-first aggregation of the node graph: seeds in node order, each
+// aggregate grows over unaggregated neighbours until it holds `target` nodes;
+// then every aggregate under target/4 nodes joins the neighbouring aggregate
+// (of at least target/4 nodes) it shares most edges with (ties: the lowest
+// id). Labels are renumbered in order of first node.
+static void aggregate(const svg_lhs* L, int target, std::vector<int>& agg, int& nAgg)
+{
+  const int nNo = L->nNo;
+  const auto& rp = L->rowPtr;
+  const auto& cp = L->colPtr;
+  agg.assign(nNo, -1);
+  std::vector<int> size, q;
+  int na = 0;
+  for (int s = 0; s < nNo; s++) {
+    if (agg[s] >= 0) continue;
+    q.clear();
+    q.push_back(s);
+    agg[s] = na;
+    int cnt = 1;
+    for (size_t h = 0; h < q.size() && cnt < target; h++) {
+      const int v = q[h];
+      for (int k = rp[v]; k < rp[v + 1] && cnt < target; k++) {
+        const int w = cp[k];
+        if (agg[w] < 0) {
+          agg[w] = na;
+          q.push_back(w);
+          cnt++;
+        }
+      }
+    }
+    size.push_back(cnt);
+    na++;
+  }
+  const int small = std::max(1, target / 4);
+  std::vector<int> dest(na);
+  std::iota(dest.begin(), dest.end(), 0);
+  // nodes of each aggregate
+  std::vector<int> off(na + 1, 0), nodes(nNo);
+  for (int a = 0; a < nNo; a++) off[agg[a] + 1]++;
+  for (int a = 0; a < na; a++) off[a + 1] += off[a];
+  {
+    std::vector<int> pos(off.begin(), off.end() - 1);
+    for (int a = 0; a < nNo; a++) nodes[pos[agg[a]]++] = a;
+  }
+  std::map<int, int> edges;
+  for (int a = 0; a < na; a++) {
+    if (size[a] >= small) continue;
+    edges.clear();
+    for (int t = off[a]; t < off[a + 1]; t++) {
+      const int v = nodes[t];
+      for (int k = rp[v]; k < rp[v + 1]; k++) {
+        const int b = agg[cp[k]];
+        if (b != a && size[b] >= small) edges[b]++;
+      }
+    }
+    int best = -1, bc = 0;
+    for (const auto& e : edges)
+      if (e.second > bc) best = e.first, bc = e.second;
+    if (best >= 0) dest[a] = best;
+  }
+  std::vector<int> label(na, -1);
+  nAgg = 0;
+  for (int a = 0; a < nNo; a++) {
+    const int d = dest[agg[a]];
+    if (label[d] < 0) label[d] = nAgg++;
+    agg[a] = label[d];
+  }
+}
+
+template <typename T>
+static int upload(T** d, const std::vector<T>& h)
+{
+  int rc = dev_alloc(d, h.size());
+  if (rc) return rc;
+  if (!h.empty()) SVG_HIP(hipMemcpy(*d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+  return SVG_OK;
+}
+
+static int coarse_build(const svg_lhs* L, int target, CoarseLevel** out)
+{
+  std::vector<int> agg;
+  int nAgg = 0;
+  aggregate(L, target, agg, nAgg);
+  if (nAgg > COARSE_MAX)
+    return set_error(SVG_ERR_ARG, "two-level preconditioner: " + std::to_string(nAgg) +
+                                      " aggregates exceed the dense coarse limit of " + std::to_string(COARSE_MAX) +
+                                      "; use larger aggregates");
+  const int nNo = L->nNo, nnz = L->nnz;
+  const auto& rp = L->rowPtr;
+  const auto& cp = L->colPtr;
+  // restriction lists: nodes of each aggregate, ascending
+  std::vector<int> aggPtr(nAgg + 1, 0), aggNodes(nNo);
+  for (int a = 0; a < nNo; a++) aggPtr[agg[a] + 1]++;
+  for (int a = 0; a < nAgg; a++) aggPtr[a + 1] += aggPtr[a];
+  {
+    std::vector<int> pos(aggPtr.begin(), aggPtr.end() - 1);
+    for (int a = 0; a < nNo; a++) aggNodes[pos[agg[a]]++] = a;
+  }
+  // A P: per row, the distinct aggregates of its column nodes (ascending);
+  // apOf[k] = the entry of block k
+  std::vector<int> apPtr(nNo + 1, 0), apAgg, apOf(nnz), row;
+  apAgg.reserve((size_t)nNo * 4);
+  for (int i = 0; i < nNo; i++) {
+    row.clear();
+    for (int k = rp[i]; k < rp[i + 1]; k++) row.push_back(agg[cp[k]]);
+    std::sort(row.begin(), row.end());
+    row.erase(std::unique(row.begin(), row.end()), row.end());
+    for (int k = rp[i]; k < rp[i + 1]; k++)
+      apOf[k] = apPtr[i] + (int)(std::lower_bound(row.begin(), row.end(), agg[cp[k]]) - row.begin());
+    apAgg.insert(apAgg.end(), row.begin(), row.end());
+    apPtr[i + 1] = (int)apAgg.size();
+  }
+  const int nAP = (int)apAgg.size();
+  // dense Ac entries: the AP entries ordered by (aggregate of the row, column
+  // aggregate, entry index); one segment per coarse entry
+  std::vector<int> apRowAgg(nAP);
+  for (int i = 0; i < nNo; i++)
+    for (int e = apPtr[i]; e < apPtr[i + 1]; e++) apRowAgg[e] = agg[i];
+  std::vector<int> perm(nAP);
+  std::iota(perm.begin(), perm.end(), 0);
+  std::stable_sort(perm.begin(), perm.end(), [&](int x, int y) {
+    return apRowAgg[x] != apRowAgg[y] ? apRowAgg[x] < apRowAgg[y] : apAgg[x] < apAgg[y];
+  });
+  const int N = (nAgg + GJB - 1) / GJB * GJB;
+  std::vector<int> segPtr, segIdx;
+  for (int q = 0; q < nAP; q++) {
+    const int e = perm[q];
+    if (q == 0 || apRowAgg[e] != apRowAgg[perm[q - 1]] || apAgg[e] != apAgg[perm[q - 1]]) {
+      segPtr.push_back(q);
+      segIdx.push_back(apRowAgg[e] * N + apAgg[e]);
+    }
+  }
+  segPtr.push_back(nAP);
+
+  auto* c = new CoarseLevel;
+  c->aggSize = target;
+  c->nAgg = nAgg;
+  c->N = N;
+  c->nAP = nAP;
+  c->nSeg = (int)segIdx.size();
+  c->nNo = nNo;
+  int rc;
+  if ((rc = upload(&c->d_agg, agg)) || (rc = upload(&c->d_aggPtr, aggPtr)) || (rc = upload(&c->d_aggNodes, aggNodes)) ||
+      (rc = upload(&c->d_apPtr, apPtr)) || (rc = upload(&c->d_apAgg, apAgg)) || (rc = upload(&c->d_apOf, apOf)) ||
+      (rc = upload(&c->d_segPtr, segPtr)) || (rc = upload(&c->d_segIdx, segIdx)) ||
+      (rc = upload(&c->d_segPerm, perm)) || (rc = dev_alloc(&c->d_apVal, (size_t)4 * nAP)) ||
+      (rc = dev_alloc(&c->d_apPP, (size_t)nAP)) || (rc = dev_alloc(&c->d_dinv, (size_t)16 * nNo)) ||
+      (rc = dev_alloc(&c->d_Ac, (size_t)N * N)) || (rc = dev_alloc(&c->d_Ai, (size_t)N * N)) ||
+      (rc = dev_alloc(&c->d_piv, (size_t)GJB * GJB)) || (rc = dev_alloc(&c->d_cp, (size_t)N * GJB)) ||
+      (rc = dev_alloc(&c->d_rp, (size_t)GJB * N)) || (rc = dev_alloc(&c->d_rc, (size_t)N)) ||
+      (rc = dev_alloc(&c->d_zc, (size_t)N)) || (rc = dev_alloc(&c->d_flag, 1))) {
+    coarse_free(c);
+    return rc;
+  }
+  hipError_t e = hipHostMalloc((void**)&c->h_flag, sizeof(int), hipHostMallocDefault);
+  if (e != hipSuccess) {
+    coarse_free(c);
+    return hip_check(e, "hipHostMalloc");
+  }
+  *out = c;
+  return SVG_OK;
+}
+
+// ------------------------------------------------ per-system set-up kernels --
+// A P values of row i: apVal(d, e) = sum over the row's blocks k of entry e, in
+// CSR order, of A_k(d, 3); apPP(e) the same sum of A_k(3, 3) in FP64.
+__global__ __launch_bounds__(256) void k_cl_ap(int nNo, const int* __restrict__ rowPtr, const int* __restrict__ apPtr,
+                                               const int* __restrict__ apOf, const double* __restrict__ Val,
+                                               float* __restrict__ apVal, double* __restrict__ apPP)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nNo) return;
+  const int k0 = rowPtr[i], k1 = rowPtr[i + 1];
+  for (int e = apPtr[i]; e < apPtr[i + 1]; e++) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    for (int k = k0; k < k1; k++) {
+      if (apOf[k] != e) continue;
+      const double* b = Val + (size_t)16 * k;
+      s0 += b[3];
+      s1 += b[7];
+      s2 += b[11];
+      s3 += b[15];
+    }
+    *reinterpret_cast<float4*>(apVal + (size_t)4 * e) = make_float4((float)s0, (float)s1, (float)s2, (float)s3);
+    apPP[e] = s3;
+  }
+}
+
+// Inverse of node i's 4x4 diagonal block (Gauss-Jordan with partial pivoting in
+// registers); a zero diagonal entry (a Dirichlet row and column, zeroed by the
+// scaling) becomes 1 first; a singular block gives the identity.
+__global__ __launch_bounds__(256) void k_cl_dinv(int nNo, const int* __restrict__ diagPtr,
+                                                 const double* __restrict__ Val, float* __restrict__ dinv)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nNo) return;
+  const double* b = Val + (size_t)16 * diagPtr[i];
+  double m[4][8];
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      m[r][c] = b[4 * r + c];
+      m[r][4 + c] = r == c ? 1.0 : 0.0;
+    }
+    if (m[r][r] == 0.0) m[r][r] = 1.0;
+  }
+  bool ok = true;
+#pragma unroll
+  for (int p = 0; p < 4; p++) {
+    int piv = p;
+    double best = fabs(m[p][p]);
+#pragma unroll
+    for (int r = p + 1; r < 4; r++)
+      if (fabs(m[r][p]) > best) best = fabs(m[r][p]), piv = r;
+    if (!(best > 0.0)) ok = false;
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+      if (r == piv && piv != p) {
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+          const double t = m[p][c];
+          m[p][c] = m[r][c];
+          m[r][c] = t;
+        }
+      }
+    const double inv = best > 0.0 ? 1.0 / m[p][p] : 0.0;
+#pragma unroll
+    for (int c = 0; c < 8; c++) m[p][c] *= inv;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      if (r == p) continue;
+      const double f = m[r][p];
+#pragma unroll
+      for (int c = 0; c < 8; c++) m[r][c] -= f * m[p][c];
+    }
+  }
+  float* o = dinv + (size_t)16 * i;
+#pragma unroll
+  for (int r = 0; r < 4; r++)
+    *reinterpret_cast<float4*>(o + 4 * r) =
+        ok ? make_float4((float)m[r][4], (float)m[r][5], (float)m[r][6], (float)m[r][7])
+           : make_float4(r == 0 ? 1.f : 0.f, r == 1 ? 1.f : 0.f, r == 2 ? 1.f : 0.f, r == 3 ? 1.f : 0.f);
+}
+
+// Ac = 0 except the identity on the padding rows nAgg..N-1.
+__global__ __launch_bounds__(256) void k_cl_acinit(int nAgg, int N, double* __restrict__ Ac, int* __restrict__ flag)
+{
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q == 0) *flag = 0;
+  if (q >= (size_t)N * N) return;
+  const int r = (int)(q / N), c = (int)(q - (size_t)r * N);
+  Ac[q] = (r == c && r >= nAgg) ? 1.0 : 0.0;
+}
+
+// One coarse entry per thread: the sum of its AP entries in segment order.
+__global__ __launch_bounds__(256) void k_cl_ac(int nSeg, const int* __restrict__ segPtr, const int* __restrict__ segIdx,
+                                               const int* __restrict__ perm, const double* __restrict__ apPP,
+                                               double* __restrict__ Ac)
+{
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nSeg) return;
+  double s = 0.0;
+  for (int q = segPtr[g]; q < segPtr[g + 1]; q++) s += apPP[perm[q]];
+  Ac[segIdx[g]] = s;
+}
+
+// --------------------------------------------- blocked Gauss-Jordan inverse --
+// In-place inversion of the row-major N x N matrix A, one GJB-wide block
+// column K per step:
+//   piv = A[K,K]^-1                       (k_gj_pivot, one workgroup)
+//   rp  = piv A[K,:],  cp = A[:,K]         (k_gj_panels)
+//   A[I,J] -= cp[I] rp[J]  (I, J != K);  A[I,K] = -cp[I] piv;  A[K,J] = rp[J];  A[K,K] = piv
+// No pivoting: the matrix is definite (every pivot a leading principal minor's
+// ratio, nonzero); a zero or non-finite pivot raises *flag.
+__global__ __launch_bounds__(1024) void k_gj_pivot(const double* __restrict__ A, int N, int K, double* __restrict__ piv,
+                                                   int* __restrict__ flag)
+{
+  __shared__ double S[GJB][GJB + 1];
+  const int t = threadIdx.x;
+  const size_t base = (size_t)K * GJB * N + (size_t)K * GJB;
+  for (int e = t; e < GJB * GJB; e += 1024) S[e / GJB][e % GJB] = A[base + (size_t)(e / GJB) * N + e % GJB];
+  __syncthreads();
+  for (int p = 0; p < GJB; p++) {
+    const double pv = S[p][p];
+    double nv[GJB * GJB / 1024];
+#pragma unroll
+    for (int q = 0; q < GJB * GJB / 1024; q++) {
+      const int e = t + 1024 * q, i = e / GJB, j = e % GJB;
+      const double sij = S[i][j], sip = S[i][p], spj = S[p][j];
+      const double inv = 1.0 / pv;
+      nv[q] = (i == p) ? ((j == p) ? inv : spj * inv) : ((j == p) ? -sip * inv : sij - sip * (spj * inv));
+    }
+    if (t == 0 && !(fabs(pv) > 0.0 && isfinite(pv))) *flag = 1 + K * GJB + p;
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < GJB * GJB / 1024; q++) {
+      const int e = t + 1024 * q;
+      S[e / GJB][e % GJB] = nv[q];
+    }
+    __syncthreads();
+  }
+  for (int e = t; e < GJB * GJB; e += 1024) piv[e] = S[e / GJB][e % GJB];
+}
+
+// Workgroup J: rp[:, J] = piv A[K, J] (J != K) and cp[J rows, :] = A[J rows, K].
+__global__ __launch_bounds__(256) void k_gj_panels(const double* __restrict__ A, int N, int K,
+                                                   const double* __restrict__ piv, double* __restrict__ rp,
+                                                   double* __restrict__ cp)
+{
+  // the left operand is staged transposed (PT[k][r] = piv[r][k]) so that both
+  // operands are read along rows of LDS
+  __shared__ double PT[GJB][GJB];
+  __shared__ double B[GJB][GJB];
+  const int J = blockIdx.x, t = threadIdx.x;
+  for (int e = t; e < GJB * GJB; e += 256) {
+    const int r = e / GJB, c = e % GJB;
+    cp[(size_t)(J * GJB + r) * GJB + c] = A[(size_t)(J * GJB + r) * N + K * GJB + c];
+  }
+  if (J == K) return;
+  for (int e = t; e < GJB * GJB; e += 256) {
+    const int r = e / GJB, c = e % GJB;
+    PT[c][r] = piv[e];
+    B[r][c] = A[(size_t)(K * GJB + r) * N + J * GJB + c];
+  }
+  __syncthreads();
+  const int r0 = 4 * (t / 16), c0 = 4 * (t % 16);
+  double acc[4][4] = {};
+  for (int k = 0; k < GJB; k++) {
+    double a[4], b[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) a[u] = PT[k][r0 + u], b[u] = B[k][c0 + u];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) acc[u][v] += a[u] * b[v];
+  }
+#pragma unroll
+  for (int u = 0; u < 4; u++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) rp[(size_t)(r0 + u) * N + J * GJB + c0 + v] = acc[u][v];
+}
+
+// Tile (I, J) of the update; blockIdx.x = J, blockIdx.y = I.
+__global__ __launch_bounds__(256) void k_gj_update(double* __restrict__ A, int N, int K, const double* __restrict__ piv,
+                                                   const double* __restrict__ rp, const double* __restrict__ cp)
+{
+  __shared__ double CT[GJB][GJB];  // CT[k][r] = cp[I rows r][k]
+  __shared__ double B[GJB][GJB];
+  const int J = blockIdx.x, I = blockIdx.y, t = threadIdx.x;
+  const int r0 = 4 * (t / 16), c0 = 4 * (t % 16);
+  double* At = A + (size_t)I * GJB * N + (size_t)J * GJB;
+  if (I == K) {
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+      for (int v = 0; v < 4; v++)
+        At[(size_t)(r0 + u) * N + c0 + v] =
+            (J == K) ? piv[(r0 + u) * GJB + c0 + v] : rp[(size_t)(r0 + u) * N + J * GJB + c0 + v];
+    return;
+  }
+  for (int e = t; e < GJB * GJB; e += 256) {
+    const int r = e / GJB, c = e % GJB;
+    CT[c][r] = cp[(size_t)(I * GJB + r) * GJB + c];
+    B[r][c] = (J == K) ? piv[e] : rp[(size_t)r * N + J * GJB + c];
+  }
+  __syncthreads();
+  double acc[4][4] = {};
+  for (int k = 0; k < GJB; k++) {
+    double a[4], b[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) a[u] = CT[k][r0 + u], b[u] = B[k][c0 + u];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) acc[u][v] += a[u] * b[v];
+  }
+#pragma unroll
+  for (int u = 0; u < 4; u++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      double* a = At + (size_t)(r0 + u) * N + c0 + v;
+      *a = (J == K) ? -acc[u][v] : *a - acc[u][v];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_cl_d2f(size_t n, const double* __restrict__ a, float* __restrict__ b)
+{
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+    b[q] = (float)a[q];
+}
+
+// ----------------------------------------------------- application kernels --
+// rc(a) = sum of v(3, i) over the nodes i of aggregate a (one wavefront per
+// aggregate: lanes at stride 64 in node order, then a fixed butterfly).
+template <typename T>
+__global__ __launch_bounds__(256) void k_cl_restrict(int nAgg, const int* __restrict__ aggPtr,
+                                                     const int* __restrict__ aggNodes, const T* __restrict__ v,
+                                                     float* __restrict__ rc, const int* __restrict__ skip)
+{
+  if (skip && *skip) return;
+  const int a = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (a >= nAgg) return;
+  double s = 0.0;
+  for (int t = aggPtr[a] + lane; t < aggPtr[a + 1]; t += 64) s += (double)v[(size_t)4 * aggNodes[t] + 3];
+  s = wave_sum(s);
+  if (lane == 0) rc[a] = (float)s;
+}
+
+// zc = Ai rc over the nAgg x nAgg leading block of the N x N FP32 inverse (one
+// wavefront per row; rc staged in LDS).
+__global__ __launch_bounds__(256) void k_cl_gemv(int nAgg, int N, const float* __restrict__ Ai,
+                                                 const float* __restrict__ rc, float* __restrict__ zc,
+                                                 const int* __restrict__ skip)
+{
+  extern __shared__ float x[];
+  if (skip && *skip) return;
+  for (int c = threadIdx.x; c < N; c += 256) x[c] = c < nAgg ? rc[c] : 0.f;
+  __syncthreads();
+  const int r = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (r >= nAgg) return;
+  const float4* row = reinterpret_cast<const float4*>(Ai + (size_t)r * N);
+  float s = 0.f;
+  for (int q = lane; q < N / 4; q += 64) {
+    const float4 a = row[q];
+    const float4 b = *reinterpret_cast<const float4*>(x + 4 * q);
+    s += a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if (lane == 0) zc[r] = s;
+}
+
+// With r2_i = v_i - sum_e AP(:, e) zc(apAgg(e)) (the coarse-corrected residual):
+//   mode 0  z_i = B_i r2_i + (0, 0, 0, zc(agg(i)))   (the two-level preconditioner)
+//   mode 1  z_i = r2_i
+//   mode 2  z_i = B_i r2_i
+// written in FP64 and/or FP32.
+template <typename T>
+__global__ __launch_bounds__(256) void k_cl_apply(int nNo, const T* __restrict__ v, const int* __restrict__ apPtr,
+                                                  const int* __restrict__ apAgg, const float* __restrict__ apVal,
+                                                  const float* __restrict__ zc, const int* __restrict__ agg,
+                                                  const float* __restrict__ dinv, double* __restrict__ z64,
+                                                  float* __restrict__ z32, const int* __restrict__ skip, int mode)
+{
+  if (skip && *skip) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nNo) return;
+  float r[4];
+#pragma unroll
+  for (int d = 0; d < 4; d++) r[d] = (float)v[(size_t)4 * i + d];
+  for (int e = apPtr[i]; e < apPtr[i + 1]; e++) {
+    const float c = zc[apAgg[e]];
+    const float4 a = *reinterpret_cast<const float4*>(apVal + (size_t)4 * e);
+    r[0] -= a.x * c;
+    r[1] -= a.y * c;
+    r[2] -= a.z * c;
+    r[3] -= a.w * c;
+  }
+  float z[4];
+  if (mode == 1) {
+#pragma unroll
+    for (int d = 0; d < 4; d++) z[d] = r[d];
+  } else {
+    const float4* D = reinterpret_cast<const float4*>(dinv + (size_t)16 * i);
+#pragma unroll
+    for (int d = 0; d < 4; d++) {
+      const float4 m = D[d];
+      z[d] = m.x * r[0] + m.y * r[1] + m.z * r[2] + m.w * r[3];
+    }
+    if (mode == 0) z[3] += zc[agg[i]];
+  }
+  if (z32) *reinterpret_cast<float4*>(z32 + (size_t)4 * i) = make_float4(z[0], z[1], z[2], z[3]);
+  if (z64) {
+    double2* o = reinterpret_cast<double2*>(z64 + (size_t)4 * i);
+    o[0] = make_double2(z[0], z[1]);
+    o[1] = make_double2(z[2], z[3]);
+  }
+}
+
+// z(3, i) += zc(agg(i)) in FP64 and FP32 (the coarse correction added to
+// another smoother's output).
+__global__ __launch_bounds__(256) void k_cl_addc(int nNo, const float* __restrict__ zc, const int* __restrict__ agg,
+                                                 double* __restrict__ z64, float* __restrict__ z32,
+                                                 const int* __restrict__ skip)
+{
+  if (skip && *skip) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nNo) return;
+  const float c = zc[agg[i]];
+  if (z64) z64[(size_t)4 * i + 3] += (double)c;
+  if (z32) z32[(size_t)4 * i + 3] += c;
+}
+
+// ------------------------------------------------------------- interfaces --
+// The coarse level of L for aggregates of about aggSize nodes (built once and
+// kept on L), then this system's A P, block-Jacobi inverses, Ac and Ac^-1 from
+// the scaled Val. Synchronises s (to check the elimination's pivots).
+int coarse_prepare_dev(svg_lhs* L, const double* Val, int aggSize, hipStream_t s)
+{
+  if (aggSize < 2) return set_error(SVG_ERR_ARG, "two-level preconditioner: aggregates need at least 2 nodes");
+  CoarseLevel*& c = L->coarse;
+  if (c && (c->aggSize != aggSize || c->nNo != L->nNo)) coarse_free(c);
+  int rc;
+  if (!c && (rc = coarse_build(L, aggSize, &c))) return rc;
+  const int nNo = L->nNo, N = c->N;
+  launch("cl_ap", k_cl_ap, dim3(blocks_for(nNo, 256)), dim3(256), 0, s, nNo, (const int*)L->d_rowPtr,
+         (const int*)c->d_apPtr, (const int*)c->d_apOf, Val, c->d_apVal, c->d_apPP);
+  launch("cl_dinv", k_cl_dinv, dim3(blocks_for(nNo, 256)), dim3(256), 0, s, nNo, (const int*)L->d_diagPtr, Val,
+         c->d_dinv);
+  launch("cl_acinit", k_cl_acinit, dim3(blocks_for((size_t)N * N, 256)), dim3(256), 0, s, c->nAgg, N, c->d_Ac,
+         c->d_flag);
+  launch("cl_ac", k_cl_ac, dim3(blocks_for(c->nSeg, 256)), dim3(256), 0, s, c->nSeg, (const int*)c->d_segPtr,
+         (const int*)c->d_segIdx, (const int*)c->d_segPerm, (const double*)c->d_apPP, c->d_Ac);
+  const int nb = N / GJB;
+  for (int K = 0; K < nb; K++) {
+    launch("gj_pivot", k_gj_pivot, dim3(1), dim3(1024), 0, s, (const double*)c->d_Ac, N, K, c->d_piv, c->d_flag);
+    launch("gj_panels", k_gj_panels, dim3(nb), dim3(256), 0, s, (const double*)c->d_Ac, N, K,
+           (const double*)c->d_piv, c->d_rp, c->d_cp);
+    launch("gj_update", k_gj_update, dim3(nb, nb), dim3(256), 0, s, c->d_Ac, N, K, (const double*)c->d_piv,
+           (const double*)c->d_rp, (const double*)c->d_cp);
+  }
+  launch("cl_d2f", k_cl_d2f, dim3(2048), dim3(256), 0, s, (size_t)N * N, (const double*)c->d_Ac, c->d_Ai);
+  SVG_HIP(hipGetLastError());
+  SVG_HIP(hipMemcpyAsync(c->h_flag, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, s));
+  SVG_HIP(hipStreamSynchronize(s));
+  if (*c->h_flag)
+    return set_error(SVG_ERR_SINGULAR, "two-level preconditioner: zero pivot " + std::to_string(*c->h_flag - 1) +
+                                           " in the coarse matrix");
+  return SVG_OK;
+}
+
+template <typename T>
+static void coarse_apply_t(const svg_lhs* L, const T* v, double* z64, float* z32, const int* skip, hipStream_t s,
+                           int mode = 0)
+{
+  const CoarseLevel* c = L->coarse;
+  const int nAgg = c->nAgg;
+  launch("cl_restrict", k_cl_restrict<T>, dim3(blocks_for(nAgg, 4)), dim3(256), 0, s, nAgg,
+         (const int*)c->d_aggPtr, (const int*)c->d_aggNodes, v, c->d_rc, skip);
+  launch("cl_gemv", k_cl_gemv, dim3(blocks_for(nAgg, 4)), dim3(256), sizeof(float) * c->N, s, nAgg, c->N,
+         (const float*)c->d_Ai, (const float*)c->d_rc, c->d_zc, skip);
+  launch("cl_apply", k_cl_apply<T>, dim3(blocks_for(L->nNo, 256)), dim3(256), 0, s, L->nNo, v,
+         (const int*)c->d_apPtr, (const int*)c->d_apAgg, (const float*)c->d_apVal, (const float*)c->d_zc,
+         (const int*)c->d_agg, (const float*)c->d_dinv, z64, z32, skip, mode);
+}
+
+void coarse_apply_dev(const svg_lhs* L, const float* v, double* z64, float* z32, const int* skip, hipStream_t s)
+{
+  coarse_apply_t<float>(L, v, z64, z32, skip, s);
+}
+
+void coarse_apply_dev(const svg_lhs* L, const double* v, double* z64, float* z32, const int* skip, hipStream_t s)
+{
+  coarse_apply_t<double>(L, v, z64, z32, skip, s);
+}
+
+void coarse_residual_dev(const svg_lhs* L, const float* v, double* r2, const int* skip, hipStream_t s)
+{
+  coarse_apply_t<float>(L, v, r2, nullptr, skip, s, 1);
+}
+
+void coarse_residual_dev(const svg_lhs* L, const double* v, double* r2, const int* skip, hipStream_t s)
+{
+  coarse_apply_t<double>(L, v, r2, nullptr, skip, s, 1);
+}
+
+void coarse_smooth_dev(const svg_lhs* L, const float* v, double* y, const int* skip, hipStream_t s)
+{
+  coarse_apply_t<float>(L, v, y, nullptr, skip, s, 2);
+}
+
+void coarse_smooth_dev(const svg_lhs* L, const double* v, double* y, const int* skip, hipStream_t s)
+{
+  coarse_apply_t<double>(L, v, y, nullptr, skip, s, 2);
+}
+
+void coarse_add_dev(const svg_lhs* L, double* z64, float* z32, const int* skip, hipStream_t s)
+{
+  const CoarseLevel* c = L->coarse;
+  launch("cl_addc", k_cl_addc, dim3(blocks_for(L->nNo, 256)), dim3(256), 0, s, L->nNo, (const float*)c->d_zc,
+         (const int*)c->d_agg, z64, z32, skip);
+}
+
+int coarse_num_aggregates(const svg_lhs* L) { return L->coarse ? L->coarse->nAgg : 0; }
+
+}  // namespace svg
+
+using namespace svg;
+
+extern "C" {
+
+int svg_coarse_prepare(svg_lhs* L, int aggSize, const double* Val, void* stream)
+{
+  if (!L || !Val) return set_error(SVG_ERR_ARG, "svg_coarse_prepare: null argument");
+  return coarse_prepare_dev(L, Val, aggSize, (hipStream_t)stream);
+}
+
+int svg_coarse_apply(const svg_lhs* L, int mode, int nvec, const double* V, double* Z, void* stream)
+{
+  if (!L || !V || !Z || nvec < 0) return set_error(SVG_ERR_ARG, "svg_coarse_apply: bad argument");
+  if (!L->coarse) return set_error(SVG_ERR_ARG, "svg_coarse_apply: svg_coarse_prepare has not run on this lhs");
+  if (mode != SVG_COARSE_APPLY && mode != SVG_COARSE_RESIDUAL && mode != SVG_COARSE_SMOOTH)
+    return set_error(SVG_ERR_ARG, "svg_coarse_apply: unknown mode");
+  if (V == Z) return set_error(SVG_ERR_ARG, "svg_coarse_apply: V and Z must not overlap");
+  hipStream_t s = (hipStream_t)stream;
+  const size_t n = (size_t)4 * L->nNo;
+  for (int b = 0; b < nvec; b++) {
+    coarse_apply_t<double>(L, V + b * n, Z + b * n, nullptr, nullptr, s, mode);
+  }
+  SVG_HIP(hipGetLastError());
+  SVG_HIP(hipStreamSynchronize(s));
+  return SVG_OK;
+}
+
+int svg_coarse_info(const svg_lhs* L, int* nAgg, int* agg)
+{
+  if (!L || !nAgg) return set_error(SVG_ERR_ARG, "svg_coarse_info: null argument");
+  const CoarseLevel* c = L->coarse;
+  *nAgg = c ? c->nAgg : 0;
+  if (c && agg) SVG_HIP(hipMemcpy(agg, c->d_agg, sizeof(int) * c->nNo, hipMemcpyDeviceToHost));
+  return SVG_OK;
+}
+
+}  // extern "C"

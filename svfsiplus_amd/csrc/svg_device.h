@@ -141,6 +141,8 @@ struct KrylovWork {
   float* u32 = nullptr;       // mixed precision, flexible: the network's FP32 z_i (the SpMV input)
   float* v32 = nullptr;       // mixed precision: the Krylov basis in FP32, (sD+1) vectors at stride ld
   double* z = nullptr;        // flexible GMRES: preconditioned basis z_j = M(u_j), sD vectors
+  float* z32 = nullptr;       // mixed precision, two-level preconditioner: z_j in FP32, sD vectors
+  double* r2 = nullptr;       // two-level with the network as smoother: its input B (u - A P zc)
   double* scal = nullptr;     // scratch scalars for standalone dots
 };
 
@@ -161,6 +163,25 @@ struct NsWork {
   double* partial = nullptr;
   int partialCap = 0;
 };
+
+// Two-level (aggregation coarse level + block Jacobi) preconditioner of an lhs.
+struct CoarseLevel;
+void coarse_free(CoarseLevel*& c);
+// Builds the coarse level of L for aggregates of about aggSize nodes (once per
+// lhs and size), then this system's operators from the scaled Val (synchronises s).
+int coarse_prepare_dev(svg_lhs* L, const double* Val, int aggSize, hipStream_t s);
+// z = M(v) in FP64 (z64) and/or FP32 (z32); skip: kernels return at once when *skip.
+void coarse_apply_dev(const svg_lhs* L, const float* v, double* z64, float* z32, const int* skip, hipStream_t s);
+void coarse_apply_dev(const svg_lhs* L, const double* v, double* z64, float* z32, const int* skip, hipStream_t s);
+// With another smoother: r2 = v - A P zc (FP64; the coarse solution zc is kept),
+// then, after the smoother wrote z from r2, z(3, i) += zc(agg(i)).
+void coarse_residual_dev(const svg_lhs* L, const float* v, double* r2, const int* skip, hipStream_t s);
+void coarse_residual_dev(const svg_lhs* L, const double* v, double* r2, const int* skip, hipStream_t s);
+// y = B (v - A P zc) (block Jacobi on the coarse-corrected residual, without P zc).
+void coarse_smooth_dev(const svg_lhs* L, const float* v, double* y, const int* skip, hipStream_t s);
+void coarse_smooth_dev(const svg_lhs* L, const double* v, double* y, const int* skip, hipStream_t s);
+void coarse_add_dev(const svg_lhs* L, double* z64, float* z32, const int* skip, hipStream_t s);
+int coarse_num_aggregates(const svg_lhs* L);
 
 // Shared between the translation units.
 void krylov_free(KrylovWork& k);   // every buffer, pinned mirror and event of a Krylov workspace
@@ -215,6 +236,7 @@ struct svg_lhs {
   float* d_val32 = nullptr;  // mixed precision: FP32 copy of the scaled Val
   unsigned short* d_val16 = nullptr;   // learned preconditioner: BF16 copy of the scaled Val
   unsigned short* d_diag16 = nullptr;  // learned preconditioner: BF16 diagonal blocks (16, nNo)
+  svg::CoarseLevel* coarse = nullptr;  // two-level preconditioner (built on first use)
   svg::KrylovWork kw;    // gmres_v on the 4x4 system
   svg::KrylovWork kw3;   // inner GMRES of the NS solver on the 3x3 momentum block
   svg::NsWork ns;

@@ -28,6 +28,7 @@
 #include <chrono>
 #include <cfloat>
 #include <cmath>
+#include <cstddef>
 #include <cstring>
 #include <mutex>
 
@@ -814,12 +815,14 @@ __global__ __launch_bounds__(256) void k_xupdate(int n, double* __restrict__ X, 
   X[k] = x;
 }
 
-// Compressed basis: X += sum_j y_j b_j, sequentially in j, with b_j = z_j
-// (FP64) on the steps the network preconditioned (z != null, j % period == 0)
-// and the FP32 basis vector v_j otherwise; two elements per lane.
+// Compressed basis: X += sum_j y_j b_j, sequentially in j, with b_j = z32_j
+// (FP32) when the whole flexible basis is FP32 (z32 != null), else z_j (FP64)
+// on the steps the network preconditioned (z != null, j % period == 0) and the
+// FP32 basis vector v_j otherwise; two elements per lane.
 __global__ __launch_bounds__(256) void k_xupdate_cb(int n, double* __restrict__ X, const float* __restrict__ v32,
                                                     const double* __restrict__ z, int period, size_t ld,
-                                                    const double* __restrict__ y, const GmState* __restrict__ st)
+                                                    const double* __restrict__ y, const GmState* __restrict__ st,
+                                                    const float* __restrict__ z32)
 {
 #pragma clang fp contract(off)
   if (st->early || st->divzero || st->tconv) return;
@@ -829,7 +832,10 @@ __global__ __launch_bounds__(256) void k_xupdate_cb(int n, double* __restrict__ 
   double2 x = *reinterpret_cast<const double2*>(X + k);
   for (int j = 0; j <= last_i; j++) {
     double2 b;
-    if (z && (period <= 1 || j % period == 0)) {
+    if (z32) {
+      const float2 f = *reinterpret_cast<const float2*>(z32 + (size_t)j * ld + k);
+      b = make_double2((double)f.x, (double)f.y);
+    } else if (z && (period <= 1 || j % period == 0)) {
       b = *reinterpret_cast<const double2*>(z + (size_t)j * ld + k);
     } else {
       const float2 f = *reinterpret_cast<const float2*>(v32 + (size_t)j * ld + k);
@@ -1200,6 +1206,8 @@ void krylov_free(KrylovWork& k)
   dev_free(k.u32);
   dev_free(k.v32);
   dev_free(k.z);
+  dev_free(k.z32);
+  dev_free(k.r2);
   if (k.h_st) (void)hipHostFree(k.h_st);
   k.h_st = nullptr;
   if (k.h_done) {
@@ -1442,18 +1450,21 @@ int spmv_residual(const svg_lhs* L, const double* K, const double* U, const doub
 // One restart cycle (gmres.cpp:529-612 / :533-592): up to sD Arnoldi steps with
 // at most LOOKAHEAD steps enqueued beyond the host's last convergence check,
 // then the back substitution and X += sum y_j u_j. Returns after the cycle's
-// state has been copied to the host.
+// state has been copied to the host. twolevel: every step is preconditioned by
+// the lhs's two-level preconditioner, z_i = P zc + B(u_i - A P zc) with B the
+// block Jacobi smoother, and on the network's steps z_i = P zc + N(B(u_i - A P zc)).
 static int run_cycle(const Op& A, KrylovWork& k, int sD, FinArgs fa, double* X, hipStream_t s,
-                     const float* K32 = nullptr, svg_lpc* net = nullptr, int net_period = 1)
+                     const float* K32 = nullptr, svg_lpc* net = nullptr, int net_period = 1, bool twolevel = false)
 {
   const bool mixed = K32 != nullptr;
-  const bool flex = net != nullptr;
+  const bool tl = twolevel;
+  const bool flex = net != nullptr || tl;
   const int n = k.n;
   const unsigned nbv = blocks_for(n, 256), nbv2 = blocks_for(n / 2, 256);
   // mixed precision keeps the basis in FP32 (k.v32); the FP64 u_i is kept
   // only where a step reads it: the network's input, a coupled face's term
   const bool coupled = mixed && has_coupled(A.L);
-  auto netStep = [&](int j) { return flex && (net_period <= 1 || j % net_period == 0); };
+  auto netStep = [&](int j) { return net && (net_period <= 1 || j % net_period == 0); };
   *k.h_done = 0;
   for (int i = 0; i < sD; i++) {
     if (i >= LOOKAHEAD) {
@@ -1465,7 +1476,28 @@ static int run_cycle(const Op& A, KrylovWork& k, int sD, FinArgs fa, double* X, 
     // flexible GMRES: the SpMV acts on z_i = M(u_i) (and its FP32 copy in u32)
     const double* sin = k.u + k.ld * i;
     double* w = k.u + k.ld * (i + 1);
-    if (netStep(i)) {
+    if (tl && netStep(i)) {
+      // the network corrects the block-Jacobi smoothing of the coarse-corrected
+      // residual, z = N(B r2); then z += P zc
+      const int* skip = (const int*)&k.st->done;
+      if (mixed)
+        coarse_smooth_dev(A.L, (const float*)(k.v32 + k.ld * i), k.r2, skip, s);
+      else
+        coarse_smooth_dev(A.L, (const double*)(k.u + k.ld * i), k.r2, skip, s);
+      int rc = lpc_apply_dev(net, A.L, k.r2, k.z + k.ld * i, mixed ? k.z32 + k.ld * i : nullptr, skip, s);
+      if (rc) return rc;
+      coarse_add_dev(A.L, k.z + k.ld * i, mixed ? k.z32 + k.ld * i : nullptr, skip, s);
+      sin = k.z + k.ld * i;
+    } else if (tl) {
+      // z_i = M(u_i): from the FP32 basis vector into the FP32 flexible basis
+      // (and FP64 where a coupled face's term reads it), or FP64 throughout
+      if (mixed)
+        coarse_apply_dev(A.L, (const float*)(k.v32 + k.ld * i), coupled ? k.z + k.ld * i : nullptr,
+                         k.z32 + k.ld * i, (const int*)&k.st->done, s);
+      else
+        coarse_apply_dev(A.L, (const double*)(k.u + k.ld * i), k.z + k.ld * i, nullptr, (const int*)&k.st->done, s);
+      sin = k.z + k.ld * i;
+    } else if (netStep(i)) {
       int rc = lpc_apply_dev(net, A.L, k.u + k.ld * i, k.z + k.ld * i, mixed ? k.u32 : nullptr, &k.st->done, s);
       if (rc) return rc;
       sin = k.z + k.ld * i;
@@ -1477,12 +1509,14 @@ static int run_cycle(const Op& A, KrylovWork& k, int sD, FinArgs fa, double* X, 
       sin = k.z + k.ld * i;
     }
     if (mixed) {
-      spmv32(A.L, K32, netStep(i) ? (const float*)k.u32 : (const float*)(k.v32 + k.ld * i), w,
-             (const int*)&k.st->done, s);
+      spmv32(A.L, K32,
+             tl ? (const float*)(k.z32 + k.ld * i)
+                : (netStep(i) ? (const float*)k.u32 : (const float*)(k.v32 + k.ld * i)),
+             w, (const int*)&k.st->done, s);
       if (coupled) add_bc_mul_dev(A.L, 4, sin, w, (const int*)&k.st->done, s);  // FP64 rank-one term
       int rc = dot_device_cb(n, w, k.v32, k.ld, i + 1, k, fa, s);
       if (rc) return rc;
-      const int keep64 = (coupled || (i + 1 < sD && netStep(i + 1))) ? 1 : 0;
+      const int keep64 = (!tl && (coupled || (i + 1 < sD && netStep(i + 1)))) ? 1 : 0;
       launch("gs_update_cb", k_gs_update_cb, dim3(nbv2), dim3(256), 0, s, n, w, k.v32, k.ld, i,
              (const double*)k.coef, (const GmState*)k.st, keep64);
     } else {
@@ -1491,7 +1525,7 @@ static int run_cycle(const Op& A, KrylovWork& k, int sD, FinArgs fa, double* X, 
                           k.d_done);
       if (rc) return rc;
       // flexible: when the next step has no network, this update also writes z_{i+1} = u_{i+1}
-      const bool nextPlain = flex && i + 1 < sD && !netStep(i + 1);
+      const bool nextPlain = flex && !tl && i + 1 < sD && !netStep(i + 1);
       launch("gs_update", k_gs_update, dim3(nbv), dim3(256), 0, s, n, k.u, k.ld, i, (const double*)k.coef,
              (const GmState*)k.st, (float*)nullptr, nextPlain ? k.z + k.ld * (i + 1) : (double*)nullptr);
     }
@@ -1500,8 +1534,8 @@ static int run_cycle(const Op& A, KrylovWork& k, int sD, FinArgs fa, double* X, 
   launch("backsolve", k_backsolve, dim3(1), dim3(256), 0, s, sD, (const double*)k.h, (const double*)k.err, k.y, k.st);
   if (mixed)
     launch("xupdate_cb", k_xupdate_cb, dim3(nbv2), dim3(256), 0, s, n, X, (const float*)k.v32,
-           flex ? (const double*)k.z : (const double*)nullptr, net_period, k.ld, (const double*)k.y,
-           (const GmState*)k.st);
+           (flex && !tl) ? (const double*)k.z : (const double*)nullptr, net_period, k.ld, (const double*)k.y,
+           (const GmState*)k.st, tl ? (const float*)k.z32 : (const float*)nullptr);
   else
     launch("xupdate", k_xupdate, dim3(nbv), dim3(256), 0, s, n, X, (const double*)(flex ? k.z : k.u), k.ld,
            (const double*)k.y, (const GmState*)k.st);
@@ -1644,7 +1678,7 @@ static int gmres_v_mixed(svg_lhs* L, svg_subls* ls, const double* Val, double* R
 // solve goes on from the current X with the plain (diagonal-only) GMRES
 // restarts of the same precision.
 static int gmres_v_learned(svg_lhs* L, svg_subls* ls, const double* Val, double* R, bool mixed, svg_lpc* net,
-                           int window, int net_period, int* fallback_itr, hipStream_t s)
+                           int window, int net_period, int coarse, int* fallback_itr, hipStream_t s)
 {
   if (ls->sD < 1 || ls->sD > 250) return set_error(SVG_ERR_ARG, "gmres_v: Krylov dimension must be in [1, 250]");
   auto& k = L->kw;
@@ -1658,12 +1692,15 @@ static int gmres_v_learned(svg_lhs* L, svg_subls* ls, const double* Val, double*
     if (!k.u32 && (rc = dev_alloc(&k.u32, (size_t)n))) return rc;
     if (!k.v32 && (rc = dev_alloc(&k.v32, k.ld * (k.sD + 1)))) return rc;
     if (!L->d_val32 && (rc = dev_alloc(&L->d_val32, (size_t)16 * L->nnz))) return rc;
+    if (coarse > 0 && !k.z32 && (rc = dev_alloc(&k.z32, k.ld * k.sD))) return rc;
   }
+  if (coarse > 0 && net && !k.r2 && (rc = dev_alloc(&k.r2, (size_t)n))) return rc;
   const Op A{L, Val, 4};
   if (L->spmvVariant < 0 && (rc = svg_spmv_tune(L, Val, nullptr, s))) return rc;
-  if ((rc = lpc_reserve(net, L, s))) return rc;
+  if (net && (rc = lpc_reserve(net, L, s))) return rc;
   auto t0 = std::chrono::steady_clock::now();
-  if ((rc = lpc_prepare_dev(L, Val, s))) return rc;
+  if (net && (rc = lpc_prepare_dev(L, Val, s))) return rc;
+  if (coarse > 0 && (rc = coarse_prepare_dev(L, Val, coarse, s))) return rc;
   if (mixed) {
     launch("d2f", k_d2f, dim3(8192), dim3(256), 0, s, (size_t)16 * L->nnz, Val, L->d_val32);
     if (L->spmv32Variant < 0 && (rc = spmv32_tune(L, L->d_val32, k.u32, k.X, s))) return rc;
@@ -1690,6 +1727,7 @@ static int gmres_v_learned(svg_lhs* L, svg_subls* ls, const double* Val, double*
   const unsigned nbv = blocks_for(n, 256);
   double dB = ls->fNorm;
   svg_lpc* pc = net;
+  bool tl = coarse > 0;
   fa.fbwin = window;
   for (int l = 0; l < ls->mItr; l++) {
     dB = ls->fNorm;
@@ -1702,7 +1740,7 @@ static int gmres_v_learned(svg_lhs* L, svg_subls* ls, const double* Val, double*
       return rc;
     launch("div", k_div, dim3(nbv), dim3(256), 0, s, n, k.u, (const GmState*)k.st,
            mixed ? k.v32 : (float*)nullptr);
-    if ((rc = run_cycle(A, k, sD, fa, k.X, s, mixed ? L->d_val32 : nullptr, pc, net_period))) return rc;
+    if ((rc = run_cycle(A, k, sD, fa, k.X, s, mixed ? L->d_val32 : nullptr, pc, net_period, tl))) return rc;
     if (k.h_st->divzero) return set_error(SVG_ERR_ARG, "Array Divide by zero.");
     ls->itr = k.h_st->itr;
     ls->fNorm = k.h_st->fNorm;
@@ -1710,10 +1748,21 @@ static int gmres_v_learned(svg_lhs* L, svg_subls* ls, const double* Val, double*
       ls->suc = 1;
       break;
     }
-    if (pc && k.h_st->fallback) {
+    if ((pc || tl) && k.h_st->fallback) {
+      // stagnation: the network is dropped first (the two-level preconditioner
+      // with block Jacobi continues and may fall back in turn), then the
+      // two-level preconditioner (the diagonally preconditioned GMRES continues)
       *fallback_itr = ls->itr;
-      pc = nullptr;
-      fa.fbwin = 0;
+      if (pc && tl) {
+        pc = nullptr;
+        // re-arm the stagnation check (fallback, nondec) for the two-level stage
+        static_assert(offsetof(GmState, nondec) == offsetof(GmState, fallback) + sizeof(int), "GmState layout");
+        SVG_HIP(hipMemsetAsync(&k.st->fallback, 0, 2 * sizeof(int), s));
+      } else {
+        pc = nullptr;
+        tl = false;
+        fa.fbwin = 0;
+      }
     }
   }
   SVG_HIP(hipMemcpyAsync(R, k.X, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
@@ -2895,8 +2944,9 @@ int svg_fsils_solve_ex(svg_lhs* L, svg_ls* ls, int dof, double* Ri, double* Val,
   const bool learned = opts && opts->method == SVG_METHOD_LEARNED;
   if (opts && opts->method != SVG_METHOD_PLAIN && !learned)
     return set_error(SVG_ERR_ARG, "fsils_solve: unknown method");
-  if (learned && (!opts->lpc || (ls && ls->LS_type != SVG_LS_GMRES)))
-    return set_error(SVG_ERR_UNSUPPORTED, "fsils_solve: the learned preconditioner needs a network and LS type GMRES");
+  if (learned && ((!opts->lpc && opts->coarse <= 0) || (ls && ls->LS_type != SVG_LS_GMRES)))
+    return set_error(SVG_ERR_UNSUPPORTED,
+                     "fsils_solve: the learned preconditioner needs a network or a coarse level, and LS type GMRES");
   if (opts) opts->fallback_itr = -1;
   if (!L || !ls || !Ri || !Val) return set_error(SVG_ERR_ARG, "svg_fsils_solve: null argument");
   hipStream_t s = (hipStream_t)stream;
@@ -2952,7 +3002,7 @@ int svg_fsils_solve_ex(svg_lhs* L, svg_ls* ls, int dof, double* Ri, double* Val,
       if (learned)
         rc = gmres_v_learned(L, &ls->RI, Val, Ri, opts->precision == SVG_PRECISION_MIXED, opts->lpc,
                              opts->fallback_window > 0 ? opts->fallback_window : 20, opts->net_period,
-                             &opts->fallback_itr, s);
+                             opts->coarse, &opts->fallback_itr, s);
       else
         rc = (opts && opts->precision == SVG_PRECISION_MIXED) ? gmres_v_mixed(L, &ls->RI, Val, Ri, s)
                                                                : gmres_v_dev(L, &ls->RI, dof, Val, Ri, s);

@@ -870,8 +870,10 @@ int lpc_reserve(svg_lpc* N, svg_lhs* L, hipStream_t s)
 }
 
 // Loss and gradient of L = (1/B) sum_b |R_b - A M(R_b)|^2 into N->g.
-static int lpc_loss_grad(svg_lpc* N, svg_lhs* L, const double* Val, int nvec, const double* R, double* loss,
-                         hipStream_t s)
+// Loss (1/nvec) sum_b |T_b - A M(R_b)|^2 and its gradient: R the network's
+// input, T the target (null: T = R).
+static int lpc_loss_grad(svg_lpc* N, svg_lhs* L, const double* Val, int nvec, const double* R, const double* T,
+                         double* loss, hipStream_t s)
 {
   const int n = L->nNo, rows = nvec * n;
   int rc;
@@ -886,8 +888,8 @@ static int lpc_loss_grad(svg_lpc* N, svg_lhs* L, const double* Val, int nvec, co
   const unsigned nbv = blocks_for((size_t)4 * n, 256);
   for (int b = 0; b < nvec; b++) {
     const size_t o = (size_t)b * 4 * n;
-    // Res = R - A Z (FP64, the solver's SpMV), GZ = -(2/B) A^T Res
-    if ((rc = spmv_residual(L, Val, N->Z + o, R + o, N->Res + o, s))) return rc;
+    // Res = T - A Z (FP64, the solver's SpMV), GZ = -(2/B) A^T Res
+    if ((rc = spmv_residual(L, Val, N->Z + o, (T ? T : R) + o, N->Res + o, s))) return rc;
     launch("lpc_spmvT", k_spmvT, dim3(nbv), dim3(256), 0, s, n, (const int*)L->d_rowPtr, (const int*)L->d_colPtr,
            (const int*)L->d_trans, Val, (const double*)(N->Res + o), N->GZ + o, -2.0 / nvec);
   }
@@ -1028,7 +1030,7 @@ int svg_lpc_loss_grad(svg_lpc* N, svg_lhs* L, const double* d_Val, int nvec, con
   if (!L->d_val16) return set_error(SVG_ERR_ARG, "svg_lpc_loss_grad: svg_lpc_prepare was not called on this lhs");
   if ((long long)nvec * L->nNo > (1LL << 28)) return set_error(SVG_ERR_ARG, "svg_lpc_loss_grad: too many rows");
   hipStream_t s = (hipStream_t)stream;
-  int rc = lpc_loss_grad(N, L, d_Val, nvec, d_R, loss, s);
+  int rc = lpc_loss_grad(N, L, d_Val, nvec, d_R, nullptr, loss, s);
   if (rc) return rc;
   if (grad) SVG_HIP(hipMemcpy(grad, N->g, sizeof(float) * NPARAM, hipMemcpyDeviceToHost));
   return SVG_OK;
@@ -1037,11 +1039,17 @@ int svg_lpc_loss_grad(svg_lpc* N, svg_lhs* L, const double* d_Val, int nvec, con
 int svg_lpc_train_step(svg_lpc* N, svg_lhs* L, const double* d_Val, int nvec, const double* d_R, float lr,
                        double* loss, void* stream)
 {
-  if (!N || !L || !d_Val || !d_R || !loss || nvec < 1) return set_error(SVG_ERR_ARG, "svg_lpc_train_step: bad argument");
+  return svg_lpc_train_step_ex(N, L, d_Val, nvec, d_R, nullptr, lr, loss, stream);
+}
+
+int svg_lpc_train_step_ex(svg_lpc* N, svg_lhs* L, const double* d_Val, int nvec, const double* d_V, const double* d_T,
+                          float lr, double* loss, void* stream)
+{
+  if (!N || !L || !d_Val || !d_V || !loss || nvec < 1) return set_error(SVG_ERR_ARG, "svg_lpc_train_step: bad argument");
   if (!L->d_val16) return set_error(SVG_ERR_ARG, "svg_lpc_train_step: svg_lpc_prepare was not called on this lhs");
   if ((long long)nvec * L->nNo > (1LL << 28)) return set_error(SVG_ERR_ARG, "svg_lpc_train_step: too many rows");
   hipStream_t s = (hipStream_t)stream;
-  int rc = lpc_loss_grad(N, L, d_Val, nvec, d_R, loss, s);
+  int rc = lpc_loss_grad(N, L, d_Val, nvec, d_V, d_T, loss, s);
   if (rc) return rc;
   N->step++;
   const float b1 = 0.9f, b2 = 0.999f;

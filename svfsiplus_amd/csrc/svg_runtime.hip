@@ -400,6 +400,7 @@ int svg_lhs_free(svg_lhs* L)
   dev_free(L->d_mask);
   svg::krylov_free(L->kw);
   svg::krylov_free(L->kw3);
+  svg::coarse_free(L->coarse);
   dev_free(L->d_val32);
   dev_free(L->d_val16);
   dev_free(L->d_diag16);

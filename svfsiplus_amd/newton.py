@@ -179,29 +179,34 @@ def construct_fluid(msh: DeviceMesh, phys: Physics, Ag, Yg, R, Val, Bf=None):
 
 
 def solve_opts(precision: int = _lib.PRECISION_FP64, lpc=None, fallback_window: int = 0,
-               net_period: int = 1) -> _lib.SolveOpts:
-    """The svg_solve_opts of a solve: precision, and the learned method when a network is given."""
+               net_period: int = 1, coarse: int = 0) -> _lib.SolveOpts:
+    """The svg_solve_opts of a solve: precision, and the learned method when a
+    network is given or coarse > 0 (the two-level preconditioner with node
+    aggregates of about `coarse` nodes)."""
     opts = _lib.SolveOpts()
     opts.precision = precision
-    if lpc is not None:
+    if lpc is not None or coarse > 0:
         opts.method = _lib.METHOD_LEARNED
-        opts.lpc = lpc.h.value if hasattr(lpc.h, "value") else lpc.h
+        if lpc is not None:
+            opts.lpc = lpc.h.value if hasattr(lpc.h, "value") else lpc.h
         opts.fallback_window = fallback_window
         opts.net_period = net_period
+        opts.coarse = coarse
     return opts
 
 
 def fsils_solve(lhs: DeviceLhs, ls: _lib.Ls, dof: int, Ri, Val, prec: int = _lib.PREC_FSILS, incL=None, res=None,
                 precision: int = _lib.PRECISION_FP64, lpc=None, fallback_window: int = 0, net_period: int = 1,
-                info: dict | None = None):
+                info: dict | None = None, coarse: int = 0):
     """Reference signature: fsils_solve(lhs, ls, dof, Ri, Val, prec, incL, res) (solve.cpp:50).
     precision = PRECISION_MIXED selects the FP32-SpMV / FP64-refined GMRES; a
-    network `lpc` (LearnedPC) selects the learned-preconditioned flexible GMRES
+    network `lpc` (LearnedPC) or coarse > 0 (the two-level preconditioner,
+    aggregates of about `coarse` nodes) selects the flexible GMRES
     (info["fallback_itr"] = iteration count at the fallback, -1 if none)."""
     nf = lhs.nFaces
     inc = np.ones(max(1, nf), dtype=np.int32) if incL is None else np.ascontiguousarray(incL, dtype=np.int32)
     rs = None if res is None else np.ascontiguousarray(res, dtype=np.float64)
-    opts = solve_opts(precision, lpc, fallback_window, net_period)
+    opts = solve_opts(precision, lpc, fallback_window, net_period, coarse)
     _lib.check(_lib.load().svg_fsils_solve_ex(lhs.h, C.byref(ls), dof, _ptr(Ri), _ptr(Val), prec, nf,
                                               inc.ctypes.data, None if rs is None else rs.ctypes.data, C.byref(opts),
                                               _stream()))
@@ -234,7 +239,7 @@ class FluidNewton:
 
     def __init__(self, pipe: Pipe, phys: Physics, ls: _lib.Ls, eq: EqState | None = None, device="cuda",
                  precision: int = _lib.PRECISION_FP64, lpc=None, outlet: bool = True, net_period: int = 1,
-                 device_setup: bool = False):
+                 device_setup: bool = False, coarse: int = 0):
         """device_setup: build the block pattern, colouring and slot map on the
         device (svg_lhs_create_dev, svg_msh_create_dev) from IEN and x."""
         import torch
@@ -245,6 +250,7 @@ class FluidNewton:
         self.precision = precision
         self.lpc = lpc
         self.net_period = net_period
+        self.coarse = coarse
         self.eq = eq or EqState()
         if device_setup:
             d_ien, d_x = device_arrays(pipe)
@@ -312,7 +318,7 @@ class FluidNewton:
             b_assem_neu_bc(self.outlet, ph, self.Yg, self.R, self.Val)
         info = {}
         fsils_solve(self.lhs, ls, 4, self.R, self.Val, precision=self.precision, lpc=self.lpc,
-                    net_period=self.net_period, info=info)
+                    net_period=self.net_period, info=info, coarse=self.coarse)
         _lib.check(lib.svg_picc_update(self.pipe.nNo, ph.gam, ph.dt, _ptr(self.R), _ptr(self.An), _ptr(self.Yn), s))
         # Newton convergence, pic.cpp:236-281
         eps = np.finfo(np.float64).eps
