@@ -8,15 +8,23 @@ step of 3 Newton iterations; GMRES (Krylov dimension 50) to a 1e-6 relative
 FP64 residual (absolute tolerance 1e-30 so the relative test binds).
 
 Methods: "plain" = the reference's FSILS GMRES with the diagonal
-preconditioner in FP64; "learned" = flexible GMRES with the learned
-preconditioner and the FP32 block SpMV (FP64 Gram-Schmidt, Givens and
-true-residual restarts, fallback to the diagonal preconditioner); "mixed" =
-plain GMRES with the FP32 SpMV. The default, "compare", times plain and
-learned back to back: `value` is the learned method's seconds per Newton step,
-`plain` and `speedup_vs_plain` stand beside it. The network is loaded from
-svfsiplus_amd/data/lpc_weights.npz (trained on the GPU by
-`python -m svfsiplus_amd.lpc`), or trained here within a bounded budget
-(outside the timed region) when that file is absent.
+preconditioner in FP64; "twolevel" = flexible GMRES with the FP32 block SpMV
+(FP64 Gram-Schmidt, Givens and true-residual restarts) preconditioned by the
+two-level preconditioner (pressure aggregation coarse level + block Jacobi);
+"learned" = the same with the learned network as the smoother on every
+net-period-th Arnoldi step (fallback: to the two-level preconditioner with
+block Jacobi, then to the diagonal one); "mixed" = plain GMRES with the FP32
+SpMV. The default, "compare", times plain, twolevel and learned back to back:
+`value` is the learned method's seconds per Newton step, `plain`, `twolevel`,
+`speedup_vs_plain` and `network_gain` stand beside it. The network is loaded
+from svfsiplus_amd/data/lpc_smoother.npz (trained on the GPU by
+`python -m svfsiplus_amd.lpc --coarse 128`), or trained here within a bounded
+budget (outside the timed region) when that file is absent.
+
+Further legs of the default run: the mixed-precision draws of viscosity and
+inflow (diagonal preconditioner, C3), the 4.2M-tet stenosed pipe (C4: twolevel
+and learned timed, plain FP64 on its first Newton system bounded by a restart
+budget), the CPU baseline and the batched bifurcations (C5).
 
 One bench "step" = one time step from the same initial state (reset by
 device-to-device copies, inside the timed region); seconds per Newton step =
@@ -45,12 +53,15 @@ HBM_PEAK_GBS = 8000.0
 KERNELS = ["spmv_bcsr", "spmv32", "d2f", "mdot", "mdot_cb", "mdot_fin", "gs_update", "gs_update_cb", "zcopy", "div",
            "backsolve", "xupdate", "xupdate_cb",
            "pc_w", "pc_val", "fluid_tet4", "b_fluid", "pici", "picc", "fill", "wc_scale", "set_int", "spmv_tune",
-           "spmv32_tune", "lpc_prep", "lpc_encode", "lpc_msg", "lpc_dense", "add_bc_mul", "face_valM", "rsub"]
+           "spmv32_tune", "lpc_prep", "lpc_encode", "lpc_msg", "lpc_dense", "lpc_layer", "add_bc_mul", "face_valM",
+           "rsub", "cl_ap", "cl_dinv", "cl_acinit", "cl_ac", "gj_pivot", "gj_panels", "gj_update", "cl_d2f",
+           "cl_restrict", "cl_gemv", "cl_apply", "cl_addc", "rearm"]
 # kernels whose launches carry algorithmic byte counts (roofline candidates)
-HBM_KERNELS = ["spmv_bcsr", "spmv32", "lpc_msg", "lpc_dense", "lpc_encode"]
+HBM_KERNELS = ["spmv_bcsr", "spmv32", "lpc_msg", "lpc_dense", "lpc_layer", "lpc_encode"]
 ROOFLINE_KERNELS = ["spmv_bcsr", "spmv32"]  # timed with events inside the timed region
 PROF_EVERY = 16  # ... one launch in 16 of each (events between launches cost time)
-DEFAULT_WEIGHTS = os.path.join(ROOT, "svfsiplus_amd", "data", "lpc_weights.npz")
+DEFAULT_WEIGHTS = os.path.join(ROOT, "svfsiplus_amd", "data", "lpc_weights.npz")     # the network alone
+DEFAULT_SMOOTHER = os.path.join(ROOT, "svfsiplus_amd", "data", "lpc_smoother.npz")   # the two-level smoother
 # HBM bytes per launch on the 1.2M-tet pipe (n = 34) for each launch variant of the two SpMV kernels
 # (the autotuners' variant indices), measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on
 # `bench.py --kernel-only spmv32` (which times every variant of both kernels), each scaled by the same
@@ -155,7 +166,7 @@ def kernel_only_lpc(a, fn, pipe, lib):
     Wd = torch.empty((pipe.nNo, 4), dtype=torch.float64, device="cuda")
     _lib.check(lib.svg_precond_diag(fn.lhs.h, 4, fn.Val.data_ptr(), fn.R.data_ptr(), Wd.data_ptr(), s))
     _lib.check(lib.svg_lpc_prepare(fn.lhs.h, fn.Val.data_ptr(), s))
-    net = L.LearnedPC.load(a.lpc_weights)
+    net = L.LearnedPC.load(a.lpc_weights or (DEFAULT_SMOOTHER if a.coarse > 0 else DEFAULT_WEIGHTS))
     V = torch.randn((pipe.nNo, 4), dtype=torch.float64, device="cuda")
     V /= torch.linalg.vector_norm(V)
     Z = torch.empty_like(V)
@@ -328,16 +339,21 @@ def _traffic(measured, variants, k):
     return measured[k].get(v)
 
 
-def get_network(a, log):
-    """The learned preconditioner: committed weights, or a bounded training run."""
+def get_network(a, log, path=None, coarse=None):
+    """The learned preconditioner: committed weights, or a bounded training run
+    (as the two-level smoother when coarse > 0)."""
     from svfsiplus_amd import lpc as L
-    if a.lpc_weights and os.path.exists(a.lpc_weights):
-        return L.LearnedPC.load(a.lpc_weights), {"source": os.path.relpath(a.lpc_weights, ROOT)}
+    path = path or a.lpc_weights or (DEFAULT_SMOOTHER if a.coarse > 0 else DEFAULT_WEIGHTS)
+    coarse = a.coarse if coarse is None else coarse
+    if os.path.exists(path):
+        with np.load(path, allow_pickle=False) as d:
+            meta = {k: d[k].item() for k in d.files if k != "params"}
+        return L.LearnedPC.load(path), {"source": os.path.relpath(path, ROOT), **meta}
     t0 = time.perf_counter()
-    data = L.build_training_set(a.train_draws, seed=11, k_max=8, log=log)
+    data = L.build_training_set(a.train_draws, seed=11, k_max=8, log=log, coarse=coarse)
     net = L.LearnedPC(seed=0)
-    hist = L.train(net, data, steps=a.train_steps, lr=2e-3, batch=8, log=log, log_every=500)
-    info = {"source": "trained in this run", "draws": a.train_draws,
+    hist = L.train(net, data, steps=a.train_steps, lr=2e-3, batch=8, log=log, log_every=500, smoother=coarse > 0)
+    info = {"source": "trained in this run", "draws": a.train_draws, "coarse": coarse,
             "triples": int(sum(r.shape[0] for _, r, _ in data)), "steps": a.train_steps,
             "final_loss": round(float(sum(hist[-100:]) / max(len(hist[-100:]), 1)), 5),
             "train_s": round(time.perf_counter() - t0, 1)}
@@ -361,7 +377,7 @@ def mixed_draws(a, fn, lib, log):
         fn.set_physics(W.Physics(mu=mu, peak=peak))
         res = {}
         for label, prec in (("fp64", _lib.PRECISION_FP64), ("mixed", _lib.PRECISION_MIXED)):
-            fn.precision, fn.lpc = prec, None
+            fn.precision, fn.lpc, fn.coarse = prec, None, 0
             fn.reset()
             torch.cuda.synchronize()
             t0 = time.perf_counter()
@@ -394,6 +410,76 @@ def mixed_draws(a, fn, lib, log):
                       "preconditioner; one time step x 3 Newton iterations per draw" % a.krylov}
 
 
+def configure(fn, method, a, net=None):
+    """The solver options of a method on a FluidNewton."""
+    from svfsiplus_amd import _lib
+    fn.precision = _lib.PRECISION_FP64 if method == "plain" else _lib.PRECISION_MIXED
+    fn.coarse = a.coarse if method in ("twolevel", "learned") else 0
+    fn.lpc = net if method == "learned" else None
+    fn.net_period = a.net_period
+
+
+def c4_run(a, lib, net, log):
+    """The 4.2M-tet stenosed pipe (n = 52 by default, stenosis depth 0.5 over z in
+    [1.25, 1.75]): one time step of 3 Newton iterations with the twolevel and the
+    learned methods (each timed after one warm-up step), then the plain FP64
+    GMRES on the first Newton system alone with a budget of c4_plain_restarts
+    restart cycles (timed; where it does not reach the tolerance within the
+    budget, its time is a lower bound of what it needs)."""
+    import torch
+    from svfsiplus_amd import _lib, workload as W
+    from svfsiplus_amd.newton import FluidNewton, ls_create
+    t0 = time.perf_counter()
+    pipe = W.make_pipe(a.c4_n, sten_depth=a.c4_sten, sten_z0=1.25, sten_len=0.5)
+    fn = FluidNewton(pipe, W.Physics(), ls_create(_lib.LS_GMRES, relTol=a.tol, absTol=1e-30, maxItr=1000,
+                                                  dimKry=a.krylov), device_setup=True)
+    log(f"# c4 setup {time.perf_counter() - t0:.1f}s: {pipe.nEl} tets, {pipe.nNo} nodes, nnzb {pipe.nnz}")
+    out = {"workload": f"stenosed VMS pipe n={a.c4_n} (depth {a.c4_sten:g} over z in [1.25, 1.75]): {pipe.nEl} tets, "
+                       f"{pipe.nNo} nodes, nnzb {pipe.nnz}; 1 time step x 3 Newton iterations; GMRES({a.krylov}) "
+                       f"rel-tol {a.tol:g}"}
+    for method in ("twolevel", "learned"):
+        configure(fn, method, a, net)
+        els = []
+        for _ in range(1 + a.c4_steps):
+            fn.reset()
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            h = [dict(x) for x in fn.time_step()]
+            torch.cuda.synchronize()
+            els.append(time.perf_counter() - t)
+        el = float(np.mean(els[1:]))
+        out[method] = {"s_per_newton": round(el / len(h), 5), "linear_iterations": [x["ls_itr"] for x in h],
+                       "all_converged": bool(all(x["suc"] for x in h)),
+                       "final_rel_residual": [x["fNorm"] / x["iNorm"] for x in h],
+                       "fallback_itr": [x.get("fallback_itr", -1) for x in h], "steps": a.c4_steps, "warmup": 1}
+        log(f"# c4 {method}: {out[method]['s_per_newton']} s/Newton, itr {out[method]['linear_iterations']}")
+    Y_learned = fn.Yn.clone()
+    # plain FP64 on the first Newton system within the restart budget
+    configure(fn, "plain", a)
+    fn.ls = ls_create(_lib.LS_GMRES, relTol=a.tol, absTol=1e-30, maxItr=a.c4_plain_restarts, dimKry=a.krylov)
+    fn.reset()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    fn.eq.itr = 0
+    fn.newton_iteration()
+    torch.cuda.synchronize()
+    tp = time.perf_counter() - t
+    hp = fn.history[-1]
+    first = out["learned"]["s_per_newton"]
+    out["plain_first_newton"] = {
+        "restart_budget": a.c4_plain_restarts, "linear_iterations": hp["ls_itr"], "s": round(tp, 4),
+        "converged": bool(hp["suc"]), "rel_residual_reached": hp["fNorm"] / hp["iNorm"],
+        "note": "the plain FP64 GMRES on the first Newton system with a budget of %d restart cycles; when it has not "
+                "converged, s is a lower bound of its time for that Newton iteration alone" % a.c4_plain_restarts}
+    out["speedup_vs_plain_lower_bound"] = round(tp / first, 2) if not hp["suc"] else None
+    out["speedup_vs_plain"] = round(tp / first, 2) if hp["suc"] else None
+    log(f"# c4 plain (first Newton system, {a.c4_plain_restarts} restarts): {tp:.2f}s, itr {hp['ls_itr']}, "
+        f"rel res {hp['fNorm'] / hp['iNorm']:.2e}, converged {bool(hp['suc'])}")
+    del Y_learned
+    fn = None
+    return out
+
+
 def batch_run(a, log):
     """The batched workload: independent bifurcating-pipe meshes (tet counts
     log-uniform in 5k-80k, daughter angles 25-45 degrees, Murray radii) solved
@@ -410,7 +496,7 @@ def batch_run(a, log):
     net = None
     if a.batch_method == "learned":
         from svfsiplus_amd import lpc as L
-        net = L.LearnedPC.load(a.lpc_weights)
+        net = L.LearnedPC.load(DEFAULT_WEIGHTS)
     bn = B.BatchNewton(meshes, phys, sD=a.krylov, rel_tol=a.tol, precision=prec, lpc=net, net_period=a.net_period)
     log(f"# batch setup {time.perf_counter() - t0:.1f}s: {len(meshes)} meshes, {bn.pipe.nEl} tets, "
         f"{bn.pipe.nNo} nodes, nnzb {bn.pipe.nnz}")
@@ -442,7 +528,7 @@ def batch_run(a, log):
                      "block SpMV variant from the autotuner" % ("flexible " if net else "", a.krylov)}
     out["spmv_variants"] = spmv_variants(_lib.load(), bn.fn.lhs)
     if net is not None:
-        out["learned_pc"] = {"weights": os.path.relpath(a.lpc_weights, ROOT), "net_period": a.net_period,
+        out["learned_pc"] = {"weights": os.path.relpath(DEFAULT_WEIGHTS, ROOT), "net_period": a.net_period,
                              "meshes_fallen_back": fallbacks}
     u = bn.pipe
     bn.free()
@@ -482,11 +568,21 @@ def main():
                     help="sinusoidal stenosis depth (fraction of the radius) over z in [1.25, 1.75]")
     ap.add_argument("--krylov", type=int, default=50)
     ap.add_argument("--tol", type=float, default=1e-6)
-    ap.add_argument("--method", choices=["compare", "plain", "mixed", "learned"], default="compare")
-    ap.add_argument("--lpc-weights", default=DEFAULT_WEIGHTS)
+    ap.add_argument("--method", choices=["compare", "plain", "mixed", "twolevel", "learned"], default="compare")
+    ap.add_argument("--lpc-weights", default=None,
+                    help="network weights (default: the two-level smoother's, or the stand-alone network's when "
+                         "--coarse 0)")
+    ap.add_argument("--coarse", type=int, default=128,
+                    help="two-level preconditioner of twolevel/learned: node aggregates of about this many nodes "
+                         "(0: learned is the stand-alone network after the diagonal scaling)")
     ap.add_argument("--net-period", type=int, default=5,
                     help="learned: the network preconditions every net-period-th Arnoldi step, the others the "
-                         "diagonal scaling alone")
+                         "two-level preconditioner with block Jacobi (or, with --coarse 0, the diagonal scaling)")
+    ap.add_argument("--c4", type=int, default=1, help="run the stenosed 4.2M-tet leg (0: skip it)")
+    ap.add_argument("--c4-n", type=int, default=52)
+    ap.add_argument("--c4-sten", type=float, default=0.5)
+    ap.add_argument("--c4-steps", type=int, default=1)
+    ap.add_argument("--c4-plain-restarts", type=int, default=300)
     ap.add_argument("--train-draws", type=int, default=40)
     ap.add_argument("--train-steps", type=int, default=1500)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -547,15 +643,18 @@ def main():
     runs = {}
     net, net_info = None, None
     if a.method in ("compare", "plain"):
-        fn.precision, fn.lpc = _lib.PRECISION_FP64, None
+        configure(fn, "plain", a)
         ps, pw = (a.steps, a.warmup) if a.method == "plain" else (a.plain_steps, a.plain_warmup)
         runs["plain"] = timed_run(fn, lib, ps, pw, not a.no_prof, "plain")
     if a.method == "mixed":
-        fn.precision, fn.lpc = _lib.PRECISION_MIXED, None
+        configure(fn, "mixed", a)
         runs["mixed"] = timed_run(fn, lib, a.steps, a.warmup, not a.no_prof, "mixed")
+    if a.method in ("compare", "twolevel") and a.coarse > 0:
+        configure(fn, "twolevel", a)
+        runs["twolevel"] = timed_run(fn, lib, a.steps, a.warmup, not a.no_prof, "twolevel")
     if a.method in ("compare", "learned"):
         net, net_info = get_network(a, log)
-        fn.precision, fn.lpc, fn.net_period = _lib.PRECISION_MIXED, net, a.net_period
+        configure(fn, "learned", a, net)
         runs["learned"] = timed_run(fn, lib, a.steps, a.warmup, not a.no_prof, "learned")
     head = "learned" if "learned" in runs else next(iter(runs))
     R = runs[head]
@@ -567,13 +666,21 @@ def main():
         base = runs.get("plain", R)
         cpu = cpu_baseline(pipe, phys, [x for h in base["hists"] for x in h])
 
+    tl_text = ("two-level preconditioner (pressure aggregation coarse level, aggregates of about %d nodes, dense "
+               "Galerkin coarse matrix inverted on the device; block-Jacobi smoother)" % a.coarse)
     solver = {"plain": "FSILS GMRES (gmres_v), diagonal preconditioner, FP64",
               "mixed": "GMRES with FP32 block SpMV, FP64 Gram-Schmidt/Givens, FP64 true-residual restarts, "
                        "diagonal preconditioner",
-              "learned": "flexible GMRES, learned GNN preconditioner (3 layers, width 64, BF16/MFMA) on every "
-                         "%d-th Arnoldi step after the diagonal scaling, FP32 block SpMV, FP64 Gram-Schmidt/Givens "
-                         "and true-residual restarts, fallback to the diagonal preconditioner after 20 "
-                         "non-decreasing iterations" % a.net_period}
+              "twolevel": "flexible GMRES with FP32 block SpMV, FP64 Gram-Schmidt/Givens and true-residual "
+                          "restarts, " + tl_text,
+              "learned": ("flexible GMRES with FP32 block SpMV, FP64 Gram-Schmidt/Givens and true-residual restarts, "
+                          + tl_text + ", the learned GNN (3 layers, width 64, BF16/MFMA) as the smoother on every "
+                          "%d-th Arnoldi step; fallback after 20 non-decreasing iterations to the block-Jacobi "
+                          "smoother, then to the diagonal preconditioner" % a.net_period) if a.coarse > 0 else
+                         ("flexible GMRES, learned GNN preconditioner (3 layers, width 64, BF16/MFMA) on every "
+                          "%d-th Arnoldi step after the diagonal scaling, FP32 block SpMV, FP64 Gram-Schmidt/Givens "
+                          "and true-residual restarts, fallback to the diagonal preconditioner after 20 "
+                          "non-decreasing iterations" % a.net_period)}
     out = {
         "metric": "Newton-step s at FP64 rel-res 1e-6, 1.2M-tet VMS pipe: learned-PC vs plain",
         "value": round(R["s_per_newton"], 6),
@@ -614,8 +721,20 @@ def main():
                                 "acceleration": rel(R["An"][:, :3], P["An"][:, :3]),
                                 "final_fNorm": R["hists"][-1][-1]["fNorm"],
                                 "final_fNorm_fp64": P["hists"][-1][-1]["fNorm"]}
+    if "twolevel" in runs and head != "twolevel":
+        T = runs["twolevel"]
+        out["twolevel"] = {"value": round(T["s_per_newton"], 6), "newton_records": T["hists"][-1],
+                           "steps": T["steps"], "warmup": T["warmup"], "all_converged": T["all_converged"],
+                           "solver": solver["twolevel"]}
+        # what the network gains over the same method without it
+        it_t = sum(x["ls_itr"] for x in T["hists"][-1])
+        it_l = sum(x["ls_itr"] for x in R["hists"][-1])
+        out["network_gain"] = {"time": round(T["s_per_newton"] / R["s_per_newton"], 3),
+                               "linear_iterations": round(it_t / max(it_l, 1), 3)}
     if net_info:
-        out["learned_pc"] = net_info
+        out["learned_pc"] = {**net_info, "net_period": a.net_period, "coarse": a.coarse}
+    if a.c4 and a.method == "compare":
+        out["c4"] = c4_run(a, lib, net, log)
     if a.c3_draws > 0 and a.method == "compare":
         out["mixed"] = mixed_draws(a, fn, lib, log)
     if a.batch > 0 and a.method == "compare":

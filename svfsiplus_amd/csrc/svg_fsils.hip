@@ -28,7 +28,6 @@
 #include <chrono>
 #include <cfloat>
 #include <cmath>
-#include <cstddef>
 #include <cstring>
 #include <mutex>
 
@@ -1366,6 +1365,17 @@ __global__ __launch_bounds__(256) void k_face_valM(int nNo, int fdof, int nsd, i
   valM[i + (size_t)fdof * a] = val[i + (size_t)fdof * a] * W[i + (size_t)dof * glob[a]];
 }
 
+// The learned preconditioner's stagnation check armed again: no fallback yet,
+// no non-decreasing iterations, no best residual estimate.
+__global__ __launch_bounds__(64) void k_fb_rearm(GmState* st)
+{
+  if (threadIdx.x == 0) {
+    st->fallback = 0;
+    st->nondec = 0;
+    st->best = DBL_MAX;
+  }
+}
+
 // y = b - y
 __global__ __launch_bounds__(256) void k_rsub(int n, const double* __restrict__ b, double* __restrict__ y)
 {
@@ -1755,9 +1765,10 @@ static int gmres_v_learned(svg_lhs* L, svg_subls* ls, const double* Val, double*
       *fallback_itr = ls->itr;
       if (pc && tl) {
         pc = nullptr;
-        // re-arm the stagnation check (fallback, nondec) for the two-level stage
-        static_assert(offsetof(GmState, nondec) == offsetof(GmState, fallback) + sizeof(int), "GmState layout");
-        SVG_HIP(hipMemsetAsync(&k.st->fallback, 0, 2 * sizeof(int), s));
+        // re-arm the stagnation check for the two-level stage, measured from
+        // its own first cycle (the best estimate of the network stage came
+        // from FP32 products and may lie below the true residual restarted from)
+        launch("rearm", k_fb_rearm, dim3(1), dim3(64), 0, s, k.st);
       } else {
         pc = nullptr;
         tl = false;

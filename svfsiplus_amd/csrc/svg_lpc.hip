@@ -226,55 +226,38 @@ __global__ __launch_bounds__(BS) void k_encode(int n, int rows, const double* __
   }
 }
 
-// Messages of one layer: M[r] = sum_{j in row i} A_ij (x) H[j] through the
-// BF16 block matrix. One lane per (row, group of 4 channels) computing 16
-// outputs, so a wavefront keeps 16 rows in flight; neighbours in row order
-// (FP32 accumulation, per output the same sequence of FMAs as a per-channel
-// lane), the next batch's column indices loaded while the current batch is
-// consumed. No LDS, so the gather runs at full occupancy.
-template <int UNR>
-__global__ __launch_bounds__(256) void k_msg(int n, int rows, const int* __restrict__ rowPtr,
-                                             const int* __restrict__ colPtr, const bf16* __restrict__ A16,
-                                             const bf16* __restrict__ H, bf16* __restrict__ M,
-                                             const int* __restrict__ skip, RowGate gate, int xcd)
+typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
+
+// m[c][d] += sum_q A[d][q] h[c][q] for one block: the block's row d is the
+// packed BF16 pairs (a[2d], a[2d+1]), channel c of h the pairs (h[2c], h[2c+1]);
+// two packed dot products (FP32 accumulation) per output, q = 0,1 then 2,3.
+__device__ __forceinline__ void msg_block(float (&m)[4][4], const uint4& a0, const uint4& a1, const uint4& h0,
+                                          const uint4& h1)
 {
-  if (skip && *skip) return;
-  int bx = blockIdx.x;
-  if (xcd) {
-    bx = xcd_block(blockIdx.x, gridDim.x);
-    if (bx < 0) return;
-  }
-  const size_t idx = (size_t)bx * 256 + threadIdx.x;
-  const int r = (int)(idx >> 2), g = (int)(idx & 3);
-  if (r >= rows) return;
-  const int b = r / n, i = r - b * n;
-  if (gate.sysOf && !gate.live(i)) return;
-  const bf16* Hb = H + (size_t)b * n * F + 16 * g;
-  const int s = rowPtr[i], e = rowPtr[i + 1];
-  float m[4][4];
+  const unsigned A[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+  const unsigned H[8] = {h0.x, h0.y, h0.z, h0.w, h1.x, h1.y, h1.z, h1.w};
+#pragma unroll
+  for (int c = 0; c < 4; c++)
+#pragma unroll
+    for (int d = 0; d < 4; d++) {
+      float t = __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(v2bf, A[2 * d]),
+                                               __builtin_bit_cast(v2bf, H[2 * c]), m[c][d], false);
+      m[c][d] = __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(v2bf, A[2 * d + 1]),
+                                               __builtin_bit_cast(v2bf, H[2 * c + 1]), t, false);
+    }
+}
+
+// The messages of node i for this lane's 4 channels (16 outputs): m = sum over
+// the row's blocks k, in CSR order, of A_k (x) h[colPtr[k]], the next UNR
+// blocks' column indices loaded while the current ones are consumed.
+template <int UNR>
+__device__ __forceinline__ void gather_row(float (&m)[4][4], int s, int e, const int* __restrict__ colPtr,
+                                           const bf16* __restrict__ A16, const bf16* __restrict__ Hb)
+{
 #pragma unroll
   for (int c = 0; c < 4; c++)
 #pragma unroll
     for (int d = 0; d < 4; d++) m[c][d] = 0.f;
-  auto acc = [&](const uint4& a0, const uint4& a1, const uint4& h0, const uint4& h1) {
-    const float A[4][4] = {{lo(a0.x), hi(a0.x), lo(a0.y), hi(a0.y)},
-                           {lo(a0.z), hi(a0.z), lo(a0.w), hi(a0.w)},
-                           {lo(a1.x), hi(a1.x), lo(a1.y), hi(a1.y)},
-                           {lo(a1.z), hi(a1.z), lo(a1.w), hi(a1.w)}};
-    const float hv[4][4] = {{lo(h0.x), hi(h0.x), lo(h0.y), hi(h0.y)},
-                            {lo(h0.z), hi(h0.z), lo(h0.w), hi(h0.w)},
-                            {lo(h1.x), hi(h1.x), lo(h1.y), hi(h1.y)},
-                            {lo(h1.z), hi(h1.z), lo(h1.w), hi(h1.w)}};
-#pragma unroll
-    for (int c = 0; c < 4; c++)
-#pragma unroll
-      for (int d = 0; d < 4; d++) {
-        float t = m[c][d];
-#pragma unroll
-        for (int q = 0; q < 4; q++) t = fmaf(A[d][q], hv[c][q], t);
-        m[c][d] = t;
-      }
-  };
   int k = s;
   int jn[UNR];
 #pragma unroll
@@ -294,7 +277,7 @@ __global__ __launch_bounds__(256) void k_msg(int n, int rows, const int* __restr
 #pragma unroll
     for (int q = 0; q < UNR; q++) jn[q] = k + UNR + q < e ? colPtr[k + UNR + q] : 0;
 #pragma unroll
-    for (int q = 0; q < UNR; q++) acc(a0[q], a1[q], h0[q], h1[q]);
+    for (int q = 0; q < UNR; q++) msg_block(m, a0[q], a1[q], h0[q], h1[q]);
   }
   for (int q = 0; k < e; k++, q++) {
     const int jj = jn[q];
@@ -302,9 +285,13 @@ __global__ __launch_bounds__(256) void k_msg(int n, int rows, const int* __restr
     const uint4 a1 = reinterpret_cast<const uint4*>(A16 + 16 * (size_t)k)[1];
     const uint4 h0 = reinterpret_cast<const uint4*>(Hb + (size_t)jj * F)[0];
     const uint4 h1 = reinterpret_cast<const uint4*>(Hb + (size_t)jj * F)[1];
-    acc(a0, a1, h0, h1);
+    msg_block(m, a0, a1, h0, h1);
   }
-  uint4 o0, o1;
+}
+
+// 16 messages rounded to BF16, packed as two 16-byte words (channel c, dof d at 4c + d).
+__device__ __forceinline__ void pack_msg(const float (&m)[4][4], uint4& o0, uint4& o1)
+{
   o0.x = (unsigned)f2bf(m[0][0]) | ((unsigned)f2bf(m[0][1]) << 16);
   o0.y = (unsigned)f2bf(m[0][2]) | ((unsigned)f2bf(m[0][3]) << 16);
   o0.z = (unsigned)f2bf(m[1][0]) | ((unsigned)f2bf(m[1][1]) << 16);
@@ -313,6 +300,34 @@ __global__ __launch_bounds__(256) void k_msg(int n, int rows, const int* __restr
   o1.y = (unsigned)f2bf(m[2][2]) | ((unsigned)f2bf(m[2][3]) << 16);
   o1.z = (unsigned)f2bf(m[3][0]) | ((unsigned)f2bf(m[3][1]) << 16);
   o1.w = (unsigned)f2bf(m[3][2]) | ((unsigned)f2bf(m[3][3]) << 16);
+}
+
+// Messages of one layer: M[r] = sum_{j in row i} A_ij (x) H[j] through the
+// BF16 block matrix. One lane per (row, group of 4 channels) computing 16
+// outputs, so a wavefront keeps 16 rows in flight. No LDS, so the gather runs
+// at full occupancy. (The training forward pass; inference fuses the gather
+// into the dense layer, k_layer, with the same arithmetic.)
+template <int UNR>
+__global__ __launch_bounds__(256) void k_msg(int n, int rows, const int* __restrict__ rowPtr,
+                                             const int* __restrict__ colPtr, const bf16* __restrict__ A16,
+                                             const bf16* __restrict__ H, bf16* __restrict__ M,
+                                             const int* __restrict__ skip, RowGate gate, int xcd)
+{
+  if (skip && *skip) return;
+  int bx = blockIdx.x;
+  if (xcd) {
+    bx = xcd_block(blockIdx.x, gridDim.x);
+    if (bx < 0) return;
+  }
+  const size_t idx = (size_t)bx * 256 + threadIdx.x;
+  const int r = (int)(idx >> 2), g = (int)(idx & 3);
+  if (r >= rows) return;
+  const int b = r / n, i = r - b * n;
+  if (gate.sysOf && !gate.live(i)) return;
+  float m[4][4];
+  gather_row<UNR>(m, rowPtr[i], rowPtr[i + 1], colPtr, A16, H + (size_t)b * n * F + 16 * g);
+  uint4 o0, o1;
+  pack_msg(m, o0, o1);
   uint4* out = reinterpret_cast<uint4*>(M + (size_t)r * F + 16 * g);
   out[0] = o0;
   out[1] = o1;
@@ -417,6 +432,115 @@ __global__ __launch_bounds__(BS) void k_dense(int rows, const bf16* __restrict__
         if (Z32) Z32[(size_t)r * 4 + d] = (float)z;
       }
       __syncthreads();
+    }
+  }
+}
+
+// One whole layer for inference on a tile of 64 node rows (one tile per
+// workgroup): the rows' messages gathered into LDS beside their h (k_msg's
+// arithmetic, rounded to BF16 as k_msg stores them), then k_dense's product,
+// residual ReLU update and (last layer) decoder, so the messages never go
+// through HBM. Bit for bit what k_msg followed by k_dense computes.
+template <int UNR>
+__global__ __launch_bounds__(BS) void k_layer(int n, int rows, const int* __restrict__ rowPtr,
+                                              const int* __restrict__ colPtr, const bf16* __restrict__ A16,
+                                              const bf16* __restrict__ Hin, bf16* __restrict__ Hout,
+                                              const bf16* __restrict__ W, const float* __restrict__ bias,
+                                              const bf16* __restrict__ Wd, const double* __restrict__ V,
+                                              double* __restrict__ Z, float* __restrict__ Z32, double inv_sc,
+                                              const double2* __restrict__ nsc, const int* __restrict__ skip,
+                                              RowGate gate, int xcd)
+{
+  __shared__ __align__(16) bf16 X[TILE * XS];
+  if (skip && *skip) return;
+  int bx = blockIdx.x;
+  if (xcd) {
+    bx = xcd_block(blockIdx.x, gridDim.x);
+    if (bx < 0) return;
+  }
+  const int r0 = bx * TILE;
+  if (r0 >= rows) return;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  if (gate.sysOf && !tile_live(gate, r0, rows, rows, t)) return;
+  // gather: thread t = 4 row + g computes channels 4g..4g+3 of its row
+  {
+    const int row = t >> 2, g = t & 3, r = r0 + row;
+    uint4 m0 = {0u, 0u, 0u, 0u}, m1 = {0u, 0u, 0u, 0u}, h0 = {0u, 0u, 0u, 0u}, h1 = {0u, 0u, 0u, 0u};
+    if (r < rows) {
+      const int b = r / n, i = r - b * n;
+      h0 = reinterpret_cast<const uint4*>(Hin + (size_t)r * F + 16 * g)[0];
+      h1 = reinterpret_cast<const uint4*>(Hin + (size_t)r * F + 16 * g)[1];
+      if (!gate.sysOf || gate.live(i)) {
+        float m[4][4];
+        gather_row<UNR>(m, rowPtr[i], rowPtr[i + 1], colPtr, A16, Hin + (size_t)b * n * F + 16 * g);
+        pack_msg(m, m0, m1);
+      }
+    }
+    uint4* xr = reinterpret_cast<uint4*>(X + row * XS + 16 * g);
+    xr[0] = h0;
+    xr[1] = h1;
+    uint4* xm = reinterpret_cast<uint4*>(X + row * XS + F + 16 * g);
+    xm[0] = m0;
+    xm[1] = m1;
+  }
+  const int fr = lane & 15, fg = lane >> 4;
+  v4s bw[2 * F / 16];
+#pragma unroll
+  for (int kk = 0; kk < 2 * F / 16; kk++)
+    bw[kk] = *reinterpret_cast<const v4s*>(W + (size_t)(16 * wave + fr) * (2 * F) + 4 * fg + 16 * kk);
+  v4f acc[4];
+#pragma unroll
+  for (int qi = 0; qi < 4; qi++) {
+    int row, col;
+    acc_rc<F>(wave, lane, qi, 0, row, col);
+    const float b0 = bias[col];
+    acc[qi] = v4f{b0, b0, b0, b0};
+  }
+  __syncthreads();
+#pragma unroll
+  for (int qi = 0; qi < 4; qi++) {
+    v4f c = acc[qi];
+    const bf16* Ar = X + (16 * qi + fr) * XS + 4 * fg;
+#pragma unroll
+    for (int kk = 0; kk < 2 * F / 16; kk++)
+      c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(*reinterpret_cast<const v4s*>(Ar + 16 * kk), bw[kk], c, 0, 0, 0);
+    acc[qi] = c;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int qi = 0; qi < 4; qi++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      int row, col;
+      acc_rc<F>(wave, lane, qi, v, row, col);
+      const float p = acc[qi][v];
+      const bf16 ho = f2bf(bf2f(X[row * XS + col]) + fmaxf(p, 0.f));
+      X[row * XS + col] = ho;
+      const int r = r0 + row;
+      if (r < rows) Hout[(size_t)r * F + col] = ho;
+    }
+  if (Wd) {
+    __syncthreads();
+    const int node = t >> 2, d = t & 3, r = r0 + node;
+    if (r < rows) {
+      const uint4* xr = reinterpret_cast<const uint4*>(X + node * XS);
+      const uint4* wr = reinterpret_cast<const uint4*>(Wd + d * F);
+      float y = 0.f;
+#pragma unroll
+      for (int f8 = 0; f8 < F / 8; f8++) {
+        const uint4 xv = xr[f8], wv = wr[f8];
+        y = fmaf(lo(wv.x), lo(xv.x), y);
+        y = fmaf(hi(wv.x), hi(xv.x), y);
+        y = fmaf(lo(wv.y), lo(xv.y), y);
+        y = fmaf(hi(wv.y), hi(xv.y), y);
+        y = fmaf(lo(wv.z), lo(xv.z), y);
+        y = fmaf(hi(wv.z), hi(xv.z), y);
+        y = fmaf(lo(wv.w), lo(xv.w), y);
+        y = fmaf(hi(wv.w), hi(xv.w), y);
+      }
+      const double z = V[(size_t)r * 4 + d] + (double)y * (nsc ? nsc[r].y : inv_sc);
+      Z[(size_t)r * 4 + d] = z;
+      if (Z32) Z32[(size_t)r * 4 + d] = (float)z;
     }
   }
 }
@@ -793,6 +917,24 @@ static int lpc_forward(svg_lpc* N, const svg_lhs* L, int nvec, const double* V, 
   unsigned nm = blocks_for((size_t)rows * 4, 256);
   const int mx = (L->msgVariant == 1 && nm >= 64) ? 1 : 0;
   if (mx) nm = (nm + 7) & ~7u;
+  if (!train) {
+    // inference: one fused launch per layer, one 64-row tile per workgroup
+    unsigned nl = nt;
+    if (mx) nl = (nl + 7) & ~7u;
+    for (int l = 0; l < NL; l++) {
+      const bool last = l == NL - 1;
+      // BF16 blocks + colPtr per block, rowPtr + h in + h out per row (+ V in, z out for the last layer)
+      const double db = (32.0 + 4.0) * L->nnz * nvec + (4.0 + 4.0 * F) * rows +
+                        (last ? (64.0 + (Z32 ? 16.0 : 0.0)) * rows : 0.0);
+      launch_b("lpc_layer", db, k_layer<4>, dim3(nl), dim3(BS), 0, s, n, rows, (const int*)L->d_rowPtr,
+               (const int*)L->d_colPtr, (const bf16*)L->d_val16, (const bf16*)N->H[l & 1], N->H[(l + 1) & 1],
+               pb + OFF_L + l * LSZ, (const float*)(N->p + OFF_L + l * LSZ + F * 2 * F),
+               last ? pb + OFF_WD : (const bf16*)nullptr, V, last ? Z : (double*)nullptr,
+               last ? Z32 : (float*)nullptr, 1.0 / sc, nsc, skip, gate, mx);
+    }
+    SVG_HIP(hipGetLastError());
+    return SVG_OK;
+  }
   for (int l = 0; l < NL; l++) {
     const bool last = l == NL - 1;
     const bf16* Hin = train ? N->H[l] : N->H[l & 1];
