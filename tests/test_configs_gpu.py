@@ -1,0 +1,336 @@
+"""GPU checks on the benchmark workloads themselves, at their full sizes:
+
+  C1  the 15k-tet pipe (n = 8), the NS solver (the fluid default LS type,
+      ls.cpp:51-60, ns_solver.cpp:167-499) over 3 Newton iterations against the
+      device-order oracle;
+  C2  the 1.2M-tet pipe (n = 34): construct_fluid (+ the outlet's backflow
+      assembly) against the oracle to 1e-12; the plain FP64 GMRES(50) of the
+      first Newton system (gmres.cpp:450-629) to the FP64 true residual, its
+      iNorm against the oracle's scaled residual; the learned method (two-level
+      preconditioner + network smoother, FP32 SpMV) on the same system;
+  C3  seeded (mu, inflow peak) draws on the 1.2M-tet pipe with the
+      mixed-precision GMRES: FP64 true residual and error against the FP64 solve;
+  C4  the 4.2M-tet stenosed pipe (n = 52): the learned method on the first
+      Newton system to the FP64 true residual without falling back;
+  C5  the 256-mesh bifurcation batch: the batched FP64 GMRES of the first
+      Newton system, 4 sampled meshes bit for bit against the device-order
+      oracle's fsils_solve of the mesh alone; the batched learned
+      mixed-precision solve with every mesh at its FP64 true residual.
+
+The true residual is that of the diagonally scaled system fsils_solve hands
+to GMRES: |W R - (W A W) y| / |W R| with y = x / W on the free dofs.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from svfsiplus_amd import _lib, workload as W
+
+pytestmark = pytest.mark.gpu
+
+DATA = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "svfsiplus_amd", "data")
+
+
+def _first_system(torch, pipe, phys, device_setup=False):
+    """The first Newton system of the time step, assembled on the device
+    (pici, construct_fluid, the outlet's b_assem_neu_bc): (FluidNewton, R, Val)."""
+    from svfsiplus_amd.newton import FluidNewton, _stream, b_assem_neu_bc, construct_fluid, ls_create
+    fn = FluidNewton(pipe, phys, ls_create(_lib.LS_GMRES, relTol=1e-6, absTol=1e-30, maxItr=1000, dimKry=50),
+                     device_setup=device_setup)
+    lib = _lib.load()
+    _lib.check(lib.svg_pici(pipe.nNo, phys.am, phys.af, fn.Ao.data_ptr(), fn.An.data_ptr(), fn.Yo.data_ptr(),
+                            fn.Yn.data_ptr(), fn.Ag.data_ptr(), fn.Yg.data_ptr(), _stream()))
+    fn.R.zero_()
+    fn.Val.zero_()
+    construct_fluid(fn.msh, phys, fn.Ag, fn.Yg, fn.R, fn.Val)
+    if fn.outlet is not None:
+        b_assem_neu_bc(fn.outlet, phys, fn.Yg, fn.R, fn.Val)
+    torch.cuda.synchronize()
+    return fn, fn.R.clone(), fn.Val.clone()
+
+
+def _scaled(torch, lhs, R, V):
+    """(W R, W A W, W) by the library's precond_diag on copies."""
+    from svfsiplus_amd.newton import _stream
+    Rs, Vs = R.clone(), V.clone()
+    Wd = torch.empty_like(R)
+    _lib.check(_lib.load().svg_precond_diag(lhs.h, 4, Vs.data_ptr(), Rs.data_ptr(), Wd.data_ptr(), _stream()))
+    return Rs, Vs, Wd
+
+
+def _true_rel_res(torch, lhs, Rs, Vs, Wd, x, rows=None):
+    """|Rs - As y| / |Rs| (per row range in `rows`: a list of (a, b) node ranges)."""
+    from svfsiplus_amd.newton import _stream
+    y = torch.where(Wd != 0, x / torch.where(Wd != 0, Wd, torch.ones_like(Wd)), torch.zeros_like(x))
+    Ay = torch.empty_like(y)
+    _lib.check(_lib.load().svg_spar_mul_vv(lhs.h, 4, Vs.data_ptr(), y.data_ptr(), Ay.data_ptr(), _stream()))
+    r = Rs - Ay
+    if rows is None:
+        return float(torch.linalg.vector_norm(r) / torch.linalg.vector_norm(Rs))
+    return [float(torch.linalg.vector_norm(r[a:b]) / torch.linalg.vector_norm(Rs[a:b])) for a, b in rows]
+
+
+def _smoother():
+    from svfsiplus_amd import lpc as L
+    return L.LearnedPC.load(os.path.join(DATA, "lpc_smoother.npz"))
+
+
+@pytest.fixture(scope="module")
+def pipe34():
+    return W.make_pipe(34)
+
+
+def test_c2_assembly_matches_oracle(torch_cuda, pipe34):
+    """construct_fluid and the outlet's b_assem_neu_bc of the 1.2M-tet pipe's
+    first Newton iteration against the oracle: <= 1e-12 norm-wise and in the
+    max norm (the sums differ only in element order)."""
+    torch = torch_cuda
+    phys = W.Physics()
+    fn, R1, V1 = _first_system(torch, pipe34, phys)
+    Ao, Yo, An, Yn = W.dirichlet_state(pipe34, phys)
+    Ag, Yg = oracle.pici(phys, Ao, An, Yo, Yn)
+    R0, V0 = oracle.construct_fluid(pipe34, phys, Ag, Yg)
+    fi, ge, inner = W.outlet_faces(pipe34)
+    oracle.b_assem_neu_fluid(pipe34, phys, Yg, fi, inner, R0, V0)
+    R1, V1 = R1.cpu().numpy(), V1.cpu().numpy()
+    eR = np.linalg.norm(R1 - R0) / np.linalg.norm(R0)
+    eV = np.linalg.norm(V1 - V0) / np.linalg.norm(V0)
+    mR = np.abs(R1 - R0).max() / np.abs(R0).max()
+    mV = np.abs(V1 - V0).max() / np.abs(V0).max()
+    print(f"C2 assembly ({pipe34.nEl} tets): |dR|/|R| {eR:.2e} |dV|/|V| {eV:.2e} max {mR:.2e} {mV:.2e}")
+    assert eR <= 1e-12 and eV <= 1e-12 and mR <= 1e-12 and mV <= 1e-12
+
+
+def test_c2_plain_and_learned_first_newton_system(torch_cuda, pipe34):
+    """The plain FP64 GMRES(50) on the first Newton system: converged, FP64
+    true residual at the tolerance (the reference stops on the Givens estimate,
+    so 1.05e-6 is allowed), iNorm equal to the oracle's |W R| to 1e-12. The
+    learned method on the same system: the FP64 true residual below 1e-6
+    (it stops on it), the solution within 1e-4 of the plain one, and fewer
+    than a twentieth of its iterations."""
+    from svfsiplus_amd.newton import fsils_solve, ls_create
+    torch = torch_cuda
+    phys = W.Physics()
+    fn, R, V = _first_system(torch, pipe34, phys)
+    Rs, Vs, Wd = _scaled(torch, fn.lhs, R, V)
+    # the oracle's scaled residual norm (its precond_diag on the same assembled system)
+    V2, R2, _ = oracle.precond_diag(pipe34, V.cpu().numpy(), R.cpu().numpy())
+    iN0 = float(np.linalg.norm(R2))
+    ls = ls_create(_lib.LS_GMRES, relTol=1e-6, absTol=1e-30, maxItr=1000, dimKry=50)
+    x, Vc = R.clone(), V.clone()
+    fsils_solve(fn.lhs, ls, 4, x, Vc)
+    rel = _true_rel_res(torch, fn.lhs, Rs, Vs, Wd, x)
+    print(f"C2 plain: itr {ls.RI.itr} suc {ls.RI.suc} iNorm {ls.RI.iNorm:.12e} oracle {iN0:.12e} true rel res {rel:.3e}")
+    assert ls.RI.suc == 1 and rel <= 1.05e-6
+    assert abs(ls.RI.iNorm - iN0) <= 1e-12 * iN0
+    net = _smoother()
+    lsl = ls_create(_lib.LS_GMRES, relTol=1e-6, absTol=1e-30, maxItr=1000, dimKry=50)
+    xl, Vl = R.clone(), V.clone()
+    info = {}
+    fsils_solve(fn.lhs, lsl, 4, xl, Vl, precision=_lib.PRECISION_MIXED, lpc=net, fallback_window=20, net_period=5,
+                coarse=128, info=info)
+    rell = _true_rel_res(torch, fn.lhs, Rs, Vs, Wd, xl)
+    err = float(torch.linalg.vector_norm(xl - x) / torch.linalg.vector_norm(x))
+    print(f"C2 learned: itr {lsl.RI.itr} suc {lsl.RI.suc} fallback {info['fallback_itr']} true rel res {rell:.3e} "
+          f"|x - x_plain|/|x_plain| {err:.2e}")
+    assert lsl.RI.suc == 1 and rell <= 1e-6
+    assert err <= 1e-4
+    assert 20 * lsl.RI.itr < ls.RI.itr
+
+
+def test_c3_mixed_precision_draws(torch_cuda, pipe34):
+    """Three seeded draws of viscosity (log-uniform in [0.02, 0.08]) and inflow
+    peak (uniform in [10, 50]) on the 1.2M-tet pipe (the bench's seed): the
+    first Newton system solved by the plain FP64 GMRES and by the
+    mixed-precision GMRES (FP32 SpMV and basis, FP64 true-residual restarts,
+    diagonal preconditioner). The mixed solve stops on the FP64 true residual
+    (<= 1e-6) and its solution is within 1e-4 of the FP64 one."""
+    from svfsiplus_amd.newton import fsils_solve, ls_create
+    torch = torch_cuda
+    rng = np.random.default_rng(20261203)
+    for d in range(3):
+        mu = float(np.exp(rng.uniform(np.log(0.02), np.log(0.08))))
+        peak = float(rng.uniform(10.0, 50.0))
+        phys = W.Physics(mu=mu, peak=peak)
+        fn, R, V = _first_system(torch, pipe34, phys)
+        Rs, Vs, Wd = _scaled(torch, fn.lhs, R, V)
+        out = {}
+        for label, prec in (("fp64", _lib.PRECISION_FP64), ("mixed", _lib.PRECISION_MIXED)):
+            ls = ls_create(_lib.LS_GMRES, relTol=1e-6, absTol=1e-30, maxItr=1000, dimKry=50)
+            x, Vc = R.clone(), V.clone()
+            fsils_solve(fn.lhs, ls, 4, x, Vc, precision=prec)
+            out[label] = (x, ls.RI.itr, ls.RI.suc, _true_rel_res(torch, fn.lhs, Rs, Vs, Wd, x))
+        x64, x32 = out["fp64"][0], out["mixed"][0]
+        e_u = float(torch.linalg.vector_norm(x32[:, :3] - x64[:, :3]) / torch.linalg.vector_norm(x64[:, :3]))
+        e_p = float(torch.linalg.vector_norm(x32[:, 3] - x64[:, 3]) / torch.linalg.vector_norm(x64[:, 3]))
+        print(f"C3 draw {d}: mu {mu:.4f} peak {peak:.2f}: itr fp64 {out['fp64'][1]} mixed {out['mixed'][1]}; "
+              f"true rel res {out['fp64'][3]:.3e} / {out['mixed'][3]:.3e}; err u {e_u:.2e} p {e_p:.2e}")
+        assert out["fp64"][2] == 1 and out["mixed"][2] == 1
+        assert out["mixed"][3] <= 1e-6
+        assert e_u <= 1e-4 and e_p <= 1e-4
+        fn = None
+
+
+def test_c4_learned_converges(torch_cuda):
+    """The 4.2M-tet stenosed pipe (n = 52, depth 0.5 over z in [1.25, 1.75]):
+    the learned method on the first Newton system reaches the FP64 true
+    residual 1e-6 without falling back (the plain FP64 GMRES does not reach it
+    within hundreds of restart cycles there)."""
+    from svfsiplus_amd.newton import fsils_solve, ls_create
+    torch = torch_cuda
+    pipe = W.make_pipe(52, sten_depth=0.5, sten_z0=1.25, sten_len=0.5)
+    phys = W.Physics()
+    fn, R, V = _first_system(torch, pipe, phys, device_setup=True)
+    Rs, Vs, Wd = _scaled(torch, fn.lhs, R, V)
+    ls = ls_create(_lib.LS_GMRES, relTol=1e-6, absTol=1e-30, maxItr=1000, dimKry=50)
+    x, Vc = R.clone(), V.clone()
+    info = {}
+    fsils_solve(fn.lhs, ls, 4, x, Vc, precision=_lib.PRECISION_MIXED, lpc=_smoother(), fallback_window=20,
+                net_period=5, coarse=128, info=info)
+    rel = _true_rel_res(torch, fn.lhs, Rs, Vs, Wd, x)
+    print(f"C4 ({pipe.nEl} tets) learned: itr {ls.RI.itr} suc {ls.RI.suc} fallback {info['fallback_itr']} "
+          f"true rel res {rel:.3e}")
+    assert ls.RI.suc == 1 and rel <= 1e-6
+    assert info["fallback_itr"] == -1
+
+
+@pytest.fixture(scope="module")
+def batch256():
+    from svfsiplus_amd import batch as B
+    return B.make_batch(W.bifurcation_batch(256))
+
+
+def _assembled_batch(torch, meshes, phys, **kw):
+    from svfsiplus_amd import batch as B
+    from svfsiplus_amd.newton import _stream, b_assem_neu_bc, construct_fluid
+    bn = B.BatchNewton(meshes, phys, **kw)
+    fn = bn.fn
+    lib = _lib.load()
+    _lib.check(lib.svg_pici(bn.pipe.nNo, phys.am, phys.af, fn.Ao.data_ptr(), fn.An.data_ptr(), fn.Yo.data_ptr(),
+                            fn.Yn.data_ptr(), fn.Ag.data_ptr(), fn.Yg.data_ptr(), _stream()))
+    fn.R.zero_()
+    fn.Val.zero_()
+    construct_fluid(fn.msh, phys, fn.Ag, fn.Yg, fn.R, fn.Val)
+    b_assem_neu_bc(fn.outlet, phys, fn.Yg, fn.R, fn.Val)
+    torch.cuda.synchronize()
+    return bn
+
+
+def test_c5_batch_fp64_matches_oracle_per_mesh(torch_cuda, batch256):
+    """The batched FP64 GMRES(50) of the 256 meshes' first Newton systems:
+    every mesh converges, and 4 sampled meshes give, bit for bit, the
+    device-order oracle's fsils_solve of that mesh alone (same iteration count,
+    same increment)."""
+    import ctypes as C
+    torch = torch_cuda
+    meshes = batch256
+    phys = W.Physics()
+    bn = _assembled_batch(torch, meshes, phys)
+    R0, V0 = bn.fn.R.clone(), bn.fn.Val.clone()
+    Rb, Vb = R0.clone(), V0.clone()
+    _lib.check(_lib.load().svg_batch_fsils_solve(bn.h, C.byref(bn.proto), Rb.data_ptr(), Vb.data_ptr(),
+                                                 _lib.PRECISION_FP64, bn.out, None))
+    torch.cuda.synchronize()
+    assert all(o.suc == 1 for o in bn.out)
+    boff = np.concatenate([[0], np.cumsum([m.nnz for m in meshes])])
+    rng = np.random.default_rng(256)
+    small = [q for q, m in enumerate(meshes) if m.nNo <= 6000]
+    picks = sorted(rng.choice(small, 4, replace=False).tolist())
+    for q in picks:
+        m = meshes[q]
+        a, b = int(bn.noff[q]), int(bn.noff[q + 1])
+        Rq = R0[a:b].cpu().numpy()
+        Vq = V0[boff[q]:boff[q + 1]].cpu().numpy()
+        with oracle.device_order():
+            xq, _, lsq = oracle.fsils_solve(m, oracle.ls_create(oracle.LS_GMRES, relTol=1e-6, absTol=1e-30,
+                                                                maxItr=1000, dimKry=50), Rq, Vq)
+        got = Rb[a:b].cpu().numpy()
+        print(f"C5 mesh {q}: nNo {m.nNo} itr batch {bn.out[q].itr} oracle {lsq.RI.itr} max|dx| "
+              f"{np.abs(got - xq).max():.2e}")
+        assert bn.out[q].itr == lsq.RI.itr
+        assert np.array_equal(got, xq)
+    bn.free()
+
+
+def test_c5_batch_learned_mixed_true_residuals(torch_cuda, batch256):
+    """The bench's batched configuration (mixed precision, the learned network
+    every 5th step, per-mesh fallback) on the 256 meshes' first Newton
+    systems: every mesh converges with its FP64 true residual below 1e-6."""
+    import ctypes as C
+    from svfsiplus_amd import lpc as L
+    torch = torch_cuda
+    meshes = batch256
+    phys = W.Physics()
+    net = L.LearnedPC.load(os.path.join(DATA, "lpc_weights.npz"))
+    bn = _assembled_batch(torch, meshes, phys, precision=_lib.PRECISION_MIXED, lpc=net, net_period=5)
+    R0, V0 = bn.fn.R.clone(), bn.fn.Val.clone()
+    Rs, Vs, Wd = _scaled(torch, bn.fn.lhs, R0, V0)
+    Rb, Vb = R0.clone(), V0.clone()
+    _lib.check(_lib.load().svg_batch_fsils_solve_ex(bn.h, C.byref(bn.proto), Rb.data_ptr(), Vb.data_ptr(),
+                                                    C.byref(bn.opts), bn.out, None))
+    torch.cuda.synchronize()
+    rows = [(int(bn.noff[q]), int(bn.noff[q + 1])) for q in range(len(meshes))]
+    rel = _true_rel_res(torch, bn.fn.lhs, Rs, Vs, Wd, Rb, rows=rows)
+    print(f"C5 learned mixed: max itr {max(o.itr for o in bn.out)}, max true rel res {max(rel):.3e}, "
+          f"all converged {all(o.suc == 1 for o in bn.out)}")
+    assert all(o.suc == 1 for o in bn.out)
+    assert max(rel) <= 1e-6
+    bn.free()
+
+
+def test_c1_ns_solver_newton_time_step(torch_cuda):
+    """The NS solver (the fluid default LS type) with fsils_ls_create's NS
+    defaults (ls.cpp:51-60: RI 0.4 / 10 outer iterations / Krylov 100, GM 1e-2 /
+    2, CG 0.2 / 500) over the 3 Newton iterations of the 15k-tet pipe's time
+    step, in lock step with the oracle: each iteration's system is assembled on
+    the device (pici, construct_fluid, the outlet's b_assem_neu_bc) and solved
+    by the device and by the device-order oracle from the same bytes; the
+    increments, the RI/GM/CG iteration counts, iNorm, fNorm and Resm/Resc agree
+    bit for bit; the device's increment then advances the Newton iterate (picc).
+    (With this solver's inexact inner solves, a 1e-16 difference in the
+    assembled sums changes the outer residual in the third digit, so the two
+    Newton loops are not run independently.)"""
+    import ctypes as C
+    from svfsiplus_amd.newton import DeviceFace, DeviceLhs, DeviceMesh, _stream, b_assem_neu_bc, construct_fluid, fsils_solve
+    torch = torch_cuda
+    pipe = W.make_pipe(8)
+    phys = W.Physics()
+    mk = lambda: oracle.ls_create(oracle.LS_NS, absTol=1e-30)
+    fi, ge, inner = W.outlet_faces(pipe)
+    L = DeviceLhs(pipe)
+    M = DeviceMesh(L, pipe)
+    Fa = DeviceFace(M, fi, ge)
+    Ao, Yo, An, Yn = (torch.tensor(a, dtype=torch.float64, device="cuda") for a in W.dirichlet_state(pipe, phys))
+    Ag, Yg = torch.empty_like(Ao), torch.empty_like(Yo)
+    lib = _lib.load()
+    for it in range(3):
+        _lib.check(lib.svg_pici(pipe.nNo, phys.am, phys.af, Ao.data_ptr(), An.data_ptr(), Yo.data_ptr(), Yn.data_ptr(),
+                                Ag.data_ptr(), Yg.data_ptr(), _stream()))
+        R = torch.zeros_like(Ao)
+        V = torch.zeros((pipe.nnz, 16), dtype=torch.float64, device="cuda")
+        construct_fluid(M, phys, Ag, Yg, R, V)
+        b_assem_neu_bc(Fa, phys, Yg, R, V)
+        torch.cuda.synchronize()
+        Rh, Vh = R.cpu().numpy(), V.cpu().numpy()
+        ols = mk()
+        ls = _lib.Ls()  # the same record, field for field (FSILS_lsType)
+        C.memmove(C.byref(ls), C.byref(ols), C.sizeof(ls))
+        assert (ls.RI.mItr, ls.GM.mItr, ls.CG.mItr, ls.RI.sD, ls.GM.sD) == (10, 2, 500, 100, 100)
+        fsils_solve(L, ls, 4, R, V)
+        with oracle.device_order():
+            x0, _, ols = oracle.fsils_solve(pipe, ols, Rh, Vh)
+        x1 = R.cpu().numpy()
+        print(f"NS Newton {it + 1}: RI itr {ols.RI.itr}/{ls.RI.itr} GM {ols.GM.itr}/{ls.GM.itr} CG {ols.CG.itr}/"
+              f"{ls.CG.itr} iNorm {ols.RI.iNorm:.12e}/{ls.RI.iNorm:.12e} fNorm {ols.RI.fNorm:.12e}/{ls.RI.fNorm:.12e} "
+              f"Resm/Resc {ols.Resm}/{ols.Resc} {ls.Resm}/{ls.Resc} suc {ols.RI.suc}/{ls.RI.suc} "
+              f"max|dx| {np.abs(x1 - x0).max():.2e}")
+        assert (ls.RI.itr, ls.GM.itr, ls.CG.itr) == (ols.RI.itr, ols.GM.itr, ols.CG.itr)
+        assert ls.RI.iNorm == ols.RI.iNorm and ls.RI.fNorm == ols.RI.fNorm and ls.RI.suc == ols.RI.suc
+        assert (ls.Resm, ls.Resc) == (ols.Resm, ols.Resc)
+        assert np.array_equal(x1, x0)
+        _lib.check(lib.svg_picc_update(pipe.nNo, phys.gam, phys.dt, R.data_ptr(), An.data_ptr(), Yn.data_ptr(),
+                                       _stream()))
+    for o in (Fa, M, L):
+        o.free()
