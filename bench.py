@@ -497,7 +497,9 @@ def batch_run(a, log):
     if a.batch_method == "learned":
         from svfsiplus_amd import lpc as L
         net = L.LearnedPC.load(DEFAULT_WEIGHTS)
-    bn = B.BatchNewton(meshes, phys, sD=a.krylov, rel_tol=a.tol, precision=prec, lpc=net, net_period=a.net_period)
+    coarse = a.batch_coarse if a.batch_method == "twolevel" else 0
+    bn = B.BatchNewton(meshes, phys, sD=a.krylov, rel_tol=a.tol, precision=prec, lpc=net, net_period=a.net_period,
+                       coarse=coarse)
     log(f"# batch setup {time.perf_counter() - t0:.1f}s: {len(meshes)} meshes, {bn.pipe.nEl} tets, "
         f"{bn.pipe.nNo} nodes, nnzb {bn.pipe.nnz}")
     for w in range(a.batch_warmup):
@@ -523,13 +525,19 @@ def batch_run(a, log):
            "steps": a.batch_steps, "warmup": a.batch_warmup,
            "precision": a.batch_precision, "method": a.batch_method, "all_converged": ok,
            "linear_iterations": {"max": [max(x) for x in itr], "mean": [round(float(np.mean(x)), 1) for x in itr]},
-           "solver": "batched %sGMRES(%d), per-mesh Krylov spaces and convergence; each Newton iteration is three "
+           "solver": "batched %sGMRES(%d), per-mesh Krylov spaces and convergence%s; each Newton iteration is three "
                      "replayed hipGraphs (pici + assembly + preconditioner, the restart cycle, the corrector); "
-                     "block SpMV variant from the autotuner" % ("flexible " if net else "", a.krylov)}
+                     "block SpMV variant from the autotuner" % (
+                         "flexible " if (net or coarse) else "", a.krylov,
+                         (", two-level preconditioner (aggregates of about %d nodes, one dense coarse block per mesh "
+                          "inverted inside the captured graph; block-Jacobi smoother; per-mesh fallback to the "
+                          "diagonal preconditioner)" % coarse) if coarse else "")}
     out["spmv_variants"] = spmv_variants(_lib.load(), bn.fn.lhs)
     if net is not None:
         out["learned_pc"] = {"weights": os.path.relpath(DEFAULT_WEIGHTS, ROOT), "net_period": a.net_period,
                              "meshes_fallen_back": fallbacks}
+    if coarse:
+        out["twolevel"] = {"coarse": coarse, "meshes_fallen_back": fallbacks}
     u = bn.pipe
     bn.free()
     # the union's mesh setup (block pattern, colouring, colour order, slot map): host and device builds
@@ -595,8 +603,10 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=5, help="timed time steps of the batched workload")
     ap.add_argument("--batch-warmup", type=int, default=1)
     ap.add_argument("--batch-precision", choices=["fp64", "mixed"], default="mixed")
-    ap.add_argument("--batch-method", choices=["plain", "learned"], default="learned",
-                    help="batched solve: diagonal preconditioner, or the learned one (flexible GMRES)")
+    ap.add_argument("--batch-method", choices=["plain", "learned", "twolevel"], default="twolevel",
+                    help="batched solve: diagonal preconditioner, the learned network alone (flexible GMRES), "
+                         "or the two-level preconditioner (one coarse block per mesh)")
+    ap.add_argument("--batch-coarse", type=int, default=64, help="twolevel batch: nodes per aggregate")
     ap.add_argument("--no-prof", action="store_true", help="time without per-kernel events")
     ap.add_argument("--kernel-only", choices=["spmv", "spmv32", "lpc"], default=None,
                     help="launch only the block SpMV on the assembled system (for counter runs), "

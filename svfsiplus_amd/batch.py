@@ -49,15 +49,17 @@ class BatchNewton:
 
     def __init__(self, meshes: list[Pipe], phys: Physics, sD: int = 50, rel_tol: float = 1e-6,
                  precision: int = _lib.PRECISION_FP64, max_restarts: int = 1000, lpc=None, net_period: int = 5,
-                 fallback_window: int = 20, device_setup: bool = False):
+                 fallback_window: int = 20, device_setup: bool = False, coarse: int = 0):
         self.meshes = meshes
         self.nsys = len(meshes)
         self.pipe, self.noff = union(meshes)
         self.phys = phys
         self.precision = precision
         self.lpc = lpc
-        # the learned preconditioner (flexible GMRES, per-mesh fallback) when a network is given
-        self.opts = solve_opts(precision, lpc, fallback_window, net_period)
+        # the learned preconditioner (flexible GMRES, per-mesh fallback) when a network is given, or
+        # the two-level preconditioner with aggregates of about `coarse` nodes (one coarse block per mesh)
+        self.coarse = coarse
+        self.opts = solve_opts(precision, lpc, fallback_window, net_period, coarse)
         self.fn = FluidNewton(self.pipe, phys, ls_create(_lib.LS_GMRES, relTol=rel_tol, absTol=1e-30,
                                                          maxItr=max_restarts, dimKry=sD), device_setup=device_setup)
         h = C.c_void_p()
@@ -99,7 +101,7 @@ class BatchNewton:
 
     def _records(self):
         recs = [dict(itr=o.itr, iNorm=o.iNorm, fNorm=o.fNorm, suc=o.suc) for o in self.out]
-        if self.lpc is not None:
+        if self.lpc is not None or self.coarse > 0:
             fb = np.empty(self.nsys, dtype=np.int32)
             _lib.check(_lib.load().svg_batch_fallback_itr(self.h, fb.ctypes.data))
             for r, f in zip(recs, fb):

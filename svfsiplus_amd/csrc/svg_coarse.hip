@@ -11,7 +11,11 @@
 //       fluid.cpp:1662-1688) are solved on the aggregates
 //   Ac  dense nAgg x nAgg; on the diagonally scaled system it is symmetric and
 //       negative definite, so it is inverted by blocked Gauss-Jordan elimination
-//       without pivoting (FP64), and applied as an FP32 matrix-vector product
+//       without pivoting (FP64), and applied as an FP32 matrix-vector product.
+//       A union of independent systems (the batched solve) has one dense block
+//       per system (its aggregates never cross systems): the elimination runs
+//       over all blocks at once, each block with the arithmetic a single
+//       system of that size gets.
 //   B   block Jacobi: the inverses of the 4x4 diagonal blocks of A (FP32)
 //
 // Aggregates are grown breadth-first over the lhs's block pattern (a greedy
@@ -22,6 +26,7 @@
 #include "svg_device.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -31,30 +36,50 @@
 namespace svg {
 
 constexpr int GJB = 64;              // Gauss-Jordan block size
-constexpr int COARSE_MAX = 8192;     // most aggregates (dense FP64 Ac: 512 MB)
+constexpr int COARSE_MAX = 8192;     // most aggregates of one system (dense FP64 block: 512 MB)
 
 struct CoarseLevel {
   int aggSize = 0;    // target nodes per aggregate this level was built for
-  int nAgg = 0, N = 0, nAP = 0, nSeg = 0, nNo = 0;
+  int nsys = 0;       // systems of a batched union (0: one system)
+  int nAgg = 0, nAP = 0, nSeg = 0, nNo = 0;
+  // dense blocks, one per system: block q holds the aggregates
+  // [aggOff[q], aggOff[q] + n_q) in an N_q x N_q row-major matrix (N_q = n_q
+  // rounded up to GJB, identity on the padding) at element offset blkOff[q];
+  // its Gauss-Jordan column panel (N_q x GJB) and row panel (GJB x N_q) at
+  // panOff[q], its pivot block at GJB * GJB * q
+  int nblk = 0, maxN = 0;
+  size_t total = 0, panTotal = 0;
+  std::vector<int> aggOff, blkN;
+  int *d_aggOff = nullptr, *d_blkN = nullptr, *d_aggBlk = nullptr;
+  long long *d_blkOff = nullptr, *d_panOff = nullptr;
   int *d_agg = nullptr, *d_aggPtr = nullptr, *d_aggNodes = nullptr;
   int *d_apPtr = nullptr, *d_apAgg = nullptr, *d_apOf = nullptr;
-  int *d_segPtr = nullptr, *d_segIdx = nullptr, *d_segPerm = nullptr;
+  int *d_segPtr = nullptr, *d_segPerm = nullptr;
+  long long* d_segIdx = nullptr;
   float* d_apVal = nullptr;   // (4, nAP): column 3 of A summed over each AP entry's blocks
   double* d_apPP = nullptr;   // (nAP): the pressure-pressure entry of the same sums, FP64
   float* d_dinv = nullptr;    // (16, nNo): inverses of the diagonal blocks, row-major
-  double* d_Ac = nullptr;     // N x N, inverted in place
-  float* d_Ai = nullptr;      // N x N FP32 copy of Ac^-1
+  double* d_Ac = nullptr;     // the blocks, inverted in place
+  float* d_Ai = nullptr;      // FP32 copy of the inverted blocks
   double *d_piv = nullptr, *d_cp = nullptr, *d_rp = nullptr;  // Gauss-Jordan panels
   float *d_rc = nullptr, *d_zc = nullptr;                     // restricted vector, coarse solution
   int* d_flag = nullptr;
   int* h_flag = nullptr;
+  // differs between builds (a graph holding this level's launches is keyed on it)
+  unsigned long long gen = 0;
 };
+
+static std::atomic<unsigned long long> g_coarse_gen{0};
+
+unsigned long long coarse_generation(const svg_lhs* L) { return L->coarse ? L->coarse->gen : 0ull; }
 
 void coarse_free(CoarseLevel*& c)
 {
   if (!c) return;
-  for (int* p : {c->d_agg, c->d_aggPtr, c->d_aggNodes, c->d_apPtr, c->d_apAgg, c->d_apOf, c->d_segPtr, c->d_segIdx,
-                 c->d_segPerm, c->d_flag})
+  for (int* p : {c->d_agg, c->d_aggPtr, c->d_aggNodes, c->d_apPtr, c->d_apAgg, c->d_apOf, c->d_segPtr, c->d_segPerm,
+                 c->d_flag, c->d_aggOff, c->d_blkN, c->d_aggBlk})
+    if (p) (void)hipFree(p);
+  for (long long* p : {c->d_segIdx, c->d_blkOff, c->d_panOff})
     if (p) (void)hipFree(p);
   for (float* p : {c->d_apVal, c->d_dinv, c->d_Ai, c->d_rc, c->d_zc})
     if (p) (void)hipFree(p);
@@ -144,16 +169,44 @@ static int upload(T** d, const std::vector<T>& h)
   return SVG_OK;
 }
 
-static int coarse_build(const svg_lhs* L, int target, CoarseLevel** out)
+// sysNodeOff (nsys + 1 node offsets, or null for one system): the systems of a
+// block-diagonal union; each gets its own dense block.
+static int coarse_build(const svg_lhs* L, int target, const int* sysNodeOff, int nsys, CoarseLevel** out)
 {
   std::vector<int> agg;
   int nAgg = 0;
   aggregate(L, target, agg, nAgg);
-  if (nAgg > COARSE_MAX)
-    return set_error(SVG_ERR_ARG, "two-level preconditioner: " + std::to_string(nAgg) +
-                                      " aggregates exceed the dense coarse limit of " + std::to_string(COARSE_MAX) +
-                                      "; use larger aggregates");
   const int nNo = L->nNo, nnz = L->nnz;
+  // the blocks: aggregates are numbered by first node and never cross systems
+  std::vector<int> aggOff;
+  if (sysNodeOff && nsys > 0) {
+    aggOff.resize(nsys + 1);
+    for (int q = 0; q < nsys; q++) aggOff[q] = agg[sysNodeOff[q]];
+    aggOff[nsys] = nAgg;
+    for (int q = 0; q < nsys; q++)
+      for (int a = sysNodeOff[q]; a < sysNodeOff[q + 1]; a++)
+        if (agg[a] < aggOff[q] || agg[a] >= aggOff[q + 1])
+          return set_error(SVG_ERR_ARG, "two-level preconditioner: an aggregate crosses systems (node " +
+                                            std::to_string(a) + "); the systems must not share block-pattern entries");
+  } else {
+    aggOff = {0, nAgg};
+  }
+  const int nblk = (int)aggOff.size() - 1;
+  std::vector<int> blkN(nblk), aggBlk(nAgg);
+  std::vector<long long> blkOff(nblk + 1, 0), panOff(nblk + 1, 0);
+  int maxN = 0;
+  for (int q = 0; q < nblk; q++) {
+    const int nq = aggOff[q + 1] - aggOff[q];
+    if (nq > COARSE_MAX)
+      return set_error(SVG_ERR_ARG, "two-level preconditioner: " + std::to_string(nq) +
+                                        " aggregates exceed the dense coarse limit of " + std::to_string(COARSE_MAX) +
+                                        "; use larger aggregates");
+    blkN[q] = (nq + GJB - 1) / GJB * GJB;
+    maxN = std::max(maxN, blkN[q]);
+    blkOff[q + 1] = blkOff[q] + (long long)blkN[q] * blkN[q];
+    panOff[q + 1] = panOff[q] + (long long)blkN[q] * GJB;
+    for (int a = aggOff[q]; a < aggOff[q + 1]; a++) aggBlk[a] = q;
+  }
   const auto& rp = L->rowPtr;
   const auto& cp = L->colPtr;
   // restriction lists: nodes of each aggregate, ascending
@@ -189,34 +242,45 @@ static int coarse_build(const svg_lhs* L, int target, CoarseLevel** out)
   std::stable_sort(perm.begin(), perm.end(), [&](int x, int y) {
     return apRowAgg[x] != apRowAgg[y] ? apRowAgg[x] < apRowAgg[y] : apAgg[x] < apAgg[y];
   });
-  const int N = (nAgg + GJB - 1) / GJB * GJB;
-  std::vector<int> segPtr, segIdx;
+  std::vector<int> segPtr;
+  std::vector<long long> segIdx;
   for (int q = 0; q < nAP; q++) {
     const int e = perm[q];
     if (q == 0 || apRowAgg[e] != apRowAgg[perm[q - 1]] || apAgg[e] != apAgg[perm[q - 1]]) {
+      const int b = aggBlk[apRowAgg[e]], a0 = aggOff[b];
+      if (aggBlk[apAgg[e]] != b)
+        return set_error(SVG_ERR_ARG, "two-level preconditioner: a block-pattern entry couples two systems");
       segPtr.push_back(q);
-      segIdx.push_back(apRowAgg[e] * N + apAgg[e]);
+      segIdx.push_back(blkOff[b] + (long long)(apRowAgg[e] - a0) * blkN[b] + (apAgg[e] - a0));
     }
   }
   segPtr.push_back(nAP);
 
   auto* c = new CoarseLevel;
   c->aggSize = target;
+  c->nsys = (sysNodeOff && nsys > 0) ? nsys : 0;
   c->nAgg = nAgg;
-  c->N = N;
   c->nAP = nAP;
   c->nSeg = (int)segIdx.size();
   c->nNo = nNo;
+  c->nblk = nblk;
+  c->maxN = maxN;
+  c->total = (size_t)blkOff[nblk];
+  c->panTotal = (size_t)panOff[nblk];
+  c->aggOff = aggOff;
+  c->blkN = blkN;
   int rc;
   if ((rc = upload(&c->d_agg, agg)) || (rc = upload(&c->d_aggPtr, aggPtr)) || (rc = upload(&c->d_aggNodes, aggNodes)) ||
       (rc = upload(&c->d_apPtr, apPtr)) || (rc = upload(&c->d_apAgg, apAgg)) || (rc = upload(&c->d_apOf, apOf)) ||
       (rc = upload(&c->d_segPtr, segPtr)) || (rc = upload(&c->d_segIdx, segIdx)) ||
-      (rc = upload(&c->d_segPerm, perm)) || (rc = dev_alloc(&c->d_apVal, (size_t)4 * nAP)) ||
+      (rc = upload(&c->d_segPerm, perm)) || (rc = upload(&c->d_aggOff, aggOff)) || (rc = upload(&c->d_blkN, blkN)) ||
+      (rc = upload(&c->d_aggBlk, aggBlk)) || (rc = upload(&c->d_blkOff, blkOff)) ||
+      (rc = upload(&c->d_panOff, panOff)) || (rc = dev_alloc(&c->d_apVal, (size_t)4 * nAP)) ||
       (rc = dev_alloc(&c->d_apPP, (size_t)nAP)) || (rc = dev_alloc(&c->d_dinv, (size_t)16 * nNo)) ||
-      (rc = dev_alloc(&c->d_Ac, (size_t)N * N)) || (rc = dev_alloc(&c->d_Ai, (size_t)N * N)) ||
-      (rc = dev_alloc(&c->d_piv, (size_t)GJB * GJB)) || (rc = dev_alloc(&c->d_cp, (size_t)N * GJB)) ||
-      (rc = dev_alloc(&c->d_rp, (size_t)GJB * N)) || (rc = dev_alloc(&c->d_rc, (size_t)N)) ||
-      (rc = dev_alloc(&c->d_zc, (size_t)N)) || (rc = dev_alloc(&c->d_flag, 1))) {
+      (rc = dev_alloc(&c->d_Ac, c->total)) || (rc = dev_alloc(&c->d_Ai, c->total)) ||
+      (rc = dev_alloc(&c->d_piv, (size_t)GJB * GJB * nblk)) || (rc = dev_alloc(&c->d_cp, c->panTotal)) ||
+      (rc = dev_alloc(&c->d_rp, c->panTotal)) || (rc = dev_alloc(&c->d_rc, (size_t)nAgg)) ||
+      (rc = dev_alloc(&c->d_zc, (size_t)nAgg)) || (rc = dev_alloc(&c->d_flag, 1))) {
     coarse_free(c);
     return rc;
   }
@@ -225,6 +289,7 @@ static int coarse_build(const svg_lhs* L, int target, CoarseLevel** out)
     coarse_free(c);
     return hip_check(e, "hipHostMalloc");
   }
+  c->gen = ++g_coarse_gen;
   *out = c;
   return SVG_OK;
 }
@@ -311,20 +376,24 @@ __global__ __launch_bounds__(256) void k_cl_dinv(int nNo, const int* __restrict_
            : make_float4(r == 0 ? 1.f : 0.f, r == 1 ? 1.f : 0.f, r == 2 ? 1.f : 0.f, r == 3 ? 1.f : 0.f);
 }
 
-// Ac = 0 except the identity on the padding rows nAgg..N-1.
-__global__ __launch_bounds__(256) void k_cl_acinit(int nAgg, int N, double* __restrict__ Ac, int* __restrict__ flag)
+// Block blockIdx.y of Ac = 0 except the identity on its padding rows n..N-1.
+__global__ __launch_bounds__(256) void k_cl_acinit(const int* __restrict__ aggOff, const int* __restrict__ blkN,
+                                                   const long long* __restrict__ blkOff, double* __restrict__ Ac,
+                                                   int* __restrict__ flag)
 {
+  const int b = blockIdx.y;
+  const int N = blkN[b], n = aggOff[b + 1] - aggOff[b];
   const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q == 0) *flag = 0;
+  if (b == 0 && q == 0) *flag = 0;
   if (q >= (size_t)N * N) return;
   const int r = (int)(q / N), c = (int)(q - (size_t)r * N);
-  Ac[q] = (r == c && r >= nAgg) ? 1.0 : 0.0;
+  Ac[blkOff[b] + q] = (r == c && r >= n) ? 1.0 : 0.0;
 }
 
 // One coarse entry per thread: the sum of its AP entries in segment order.
-__global__ __launch_bounds__(256) void k_cl_ac(int nSeg, const int* __restrict__ segPtr, const int* __restrict__ segIdx,
-                                               const int* __restrict__ perm, const double* __restrict__ apPP,
-                                               double* __restrict__ Ac)
+__global__ __launch_bounds__(256) void k_cl_ac(int nSeg, const int* __restrict__ segPtr,
+                                               const long long* __restrict__ segIdx, const int* __restrict__ perm,
+                                               const double* __restrict__ apPP, double* __restrict__ Ac)
 {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= nSeg) return;
@@ -334,17 +403,23 @@ __global__ __launch_bounds__(256) void k_cl_ac(int nSeg, const int* __restrict__
 }
 
 // --------------------------------------------- blocked Gauss-Jordan inverse --
-// In-place inversion of the row-major N x N matrix A, one GJB-wide block
-// column K per step:
+// In-place inversion of each row-major N x N block A (block blockIdx.z: N =
+// blkN[b], at blkOff[b]), one GJB-wide block column K per step (blocks with
+// fewer steps skip the later ones):
 //   piv = A[K,K]^-1                       (k_gj_pivot, one workgroup)
 //   rp  = piv A[K,:],  cp = A[:,K]         (k_gj_panels)
 //   A[I,J] -= cp[I] rp[J]  (I, J != K);  A[I,K] = -cp[I] piv;  A[K,J] = rp[J];  A[K,K] = piv
 // No pivoting: the matrix is definite (every pivot a leading principal minor's
 // ratio, nonzero); a zero or non-finite pivot raises *flag.
-__global__ __launch_bounds__(1024) void k_gj_pivot(const double* __restrict__ A, int N, int K, double* __restrict__ piv,
-                                                   int* __restrict__ flag)
+__global__ __launch_bounds__(1024) void k_gj_pivot(const double* __restrict__ A0, const int* __restrict__ blkN,
+                                                   const long long* __restrict__ blkOff, int K,
+                                                   double* __restrict__ piv0, int* __restrict__ flag)
 {
   __shared__ double S[GJB][GJB + 1];
+  const int b = blockIdx.z, N = blkN[b];
+  if (K >= N / GJB) return;
+  const double* A = A0 + blkOff[b];
+  double* piv = piv0 + (size_t)GJB * GJB * b;
   const int t = threadIdx.x;
   const size_t base = (size_t)K * GJB * N + (size_t)K * GJB;
   for (int e = t; e < GJB * GJB; e += 1024) S[e / GJB][e % GJB] = A[base + (size_t)(e / GJB) * N + e % GJB];
@@ -372,10 +447,18 @@ __global__ __launch_bounds__(1024) void k_gj_pivot(const double* __restrict__ A,
 }
 
 // Workgroup J: rp[:, J] = piv A[K, J] (J != K) and cp[J rows, :] = A[J rows, K].
-__global__ __launch_bounds__(256) void k_gj_panels(const double* __restrict__ A, int N, int K,
-                                                   const double* __restrict__ piv, double* __restrict__ rp,
-                                                   double* __restrict__ cp)
+__global__ __launch_bounds__(256) void k_gj_panels(const double* __restrict__ A0, const int* __restrict__ blkN,
+                                                   const long long* __restrict__ blkOff,
+                                                   const long long* __restrict__ panOff, int K,
+                                                   const double* __restrict__ piv0, double* __restrict__ rp0,
+                                                   double* __restrict__ cp0)
 {
+  const int b = blockIdx.z, N = blkN[b], nb = N / GJB;
+  if (K >= nb || (int)blockIdx.x >= nb) return;
+  const double* A = A0 + blkOff[b];
+  const double* piv = piv0 + (size_t)GJB * GJB * b;
+  double* rp = rp0 + panOff[b];
+  double* cp = cp0 + panOff[b];
   // the left operand is staged transposed (PT[k][r] = piv[r][k]) so that both
   // operands are read along rows of LDS
   __shared__ double PT[GJB][GJB];
@@ -409,10 +492,19 @@ __global__ __launch_bounds__(256) void k_gj_panels(const double* __restrict__ A,
     for (int v = 0; v < 4; v++) rp[(size_t)(r0 + u) * N + J * GJB + c0 + v] = acc[u][v];
 }
 
-// Tile (I, J) of the update; blockIdx.x = J, blockIdx.y = I.
-__global__ __launch_bounds__(256) void k_gj_update(double* __restrict__ A, int N, int K, const double* __restrict__ piv,
-                                                   const double* __restrict__ rp, const double* __restrict__ cp)
+// Tile (I, J) of the update; blockIdx.x = J, blockIdx.y = I, blockIdx.z the block.
+__global__ __launch_bounds__(256) void k_gj_update(double* __restrict__ A0, const int* __restrict__ blkN,
+                                                   const long long* __restrict__ blkOff,
+                                                   const long long* __restrict__ panOff, int K,
+                                                   const double* __restrict__ piv0, const double* __restrict__ rp0,
+                                                   const double* __restrict__ cp0)
 {
+  const int b = blockIdx.z, N = blkN[b], nb = N / GJB;
+  if (K >= nb || (int)blockIdx.x >= nb || (int)blockIdx.y >= nb) return;
+  double* A = A0 + blkOff[b];
+  const double* piv = piv0 + (size_t)GJB * GJB * b;
+  const double* rp = rp0 + panOff[b];
+  const double* cp = cp0 + panOff[b];
   __shared__ double CT[GJB][GJB];  // CT[k][r] = cp[I rows r][k]
   __shared__ double B[GJB][GJB];
   const int J = blockIdx.x, I = blockIdx.y, t = threadIdx.x;
@@ -475,23 +567,26 @@ __global__ __launch_bounds__(256) void k_cl_restrict(int nAgg, const int* __rest
   if (lane == 0) rc[a] = (float)s;
 }
 
-// zc = Ai rc over the nAgg x nAgg leading block of the N x N FP32 inverse (one
-// wavefront per row; rc staged in LDS).
-__global__ __launch_bounds__(256) void k_cl_gemv(int nAgg, int N, const float* __restrict__ Ai,
-                                                 const float* __restrict__ rc, float* __restrict__ zc,
-                                                 const int* __restrict__ skip)
+// zc = Ai rc block by block: aggregate row r of block b = aggBlk[r] is row
+// r - aggOff[b] of the block's N x N FP32 inverse, over the block's n real
+// columns (one wavefront per row; the padding columns read as zero).
+__global__ __launch_bounds__(256) void k_cl_gemv(int nAgg, const int* __restrict__ aggBlk, const int* __restrict__ aggOff,
+                                                 const int* __restrict__ blkN, const long long* __restrict__ blkOff,
+                                                 const float* __restrict__ Ai, const float* __restrict__ rc,
+                                                 float* __restrict__ zc, const int* __restrict__ skip)
 {
-  extern __shared__ float x[];
   if (skip && *skip) return;
-  for (int c = threadIdx.x; c < N; c += 256) x[c] = c < nAgg ? rc[c] : 0.f;
-  __syncthreads();
   const int r = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (r >= nAgg) return;
-  const float4* row = reinterpret_cast<const float4*>(Ai + (size_t)r * N);
+  const int bk = aggBlk[r], a0 = aggOff[bk], n = aggOff[bk + 1] - a0, N = blkN[bk];
+  const float4* row = reinterpret_cast<const float4*>(Ai + blkOff[bk] + (size_t)(r - a0) * N);
+  const float* x = rc + a0;
   float s = 0.f;
   for (int q = lane; q < N / 4; q += 64) {
     const float4 a = row[q];
-    const float4 b = *reinterpret_cast<const float4*>(x + 4 * q);
+    const int c = 4 * q;
+    const float4 b = make_float4(c < n ? x[c] : 0.f, c + 1 < n ? x[c + 1] : 0.f, c + 2 < n ? x[c + 2] : 0.f,
+                                 c + 3 < n ? x[c + 3] : 0.f);
     s += a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w;
   }
 #pragma unroll
@@ -561,34 +656,49 @@ __global__ __launch_bounds__(256) void k_cl_addc(int nNo, const float* __restric
 }
 
 // ------------------------------------------------------------- interfaces --
-// The coarse level of L for aggregates of about aggSize nodes (built once and
-// kept on L), then this system's A P, block-Jacobi inverses, Ac and Ac^-1 from
-// the scaled Val. Synchronises s (to check the elimination's pivots).
-int coarse_prepare_dev(svg_lhs* L, const double* Val, int aggSize, hipStream_t s)
+// The coarse level of L for aggregates of about aggSize nodes, one dense block
+// per system of sysNodeOff (null: one system); built once and kept on L.
+int coarse_setup(svg_lhs* L, int aggSize, const int* sysNodeOff, int nsys)
 {
   if (aggSize < 2) return set_error(SVG_ERR_ARG, "two-level preconditioner: aggregates need at least 2 nodes");
   CoarseLevel*& c = L->coarse;
-  if (c && (c->aggSize != aggSize || c->nNo != L->nNo)) coarse_free(c);
-  int rc;
-  if (!c && (rc = coarse_build(L, aggSize, &c))) return rc;
-  const int nNo = L->nNo, N = c->N;
+  const int ns = (sysNodeOff && nsys > 0) ? nsys : 0;
+  if (c && (c->aggSize != aggSize || c->nNo != L->nNo || c->nsys != ns)) coarse_free(c);
+  if (!c) return coarse_build(L, aggSize, sysNodeOff, nsys, &c);
+  return SVG_OK;
+}
+
+// This system's A P, block-Jacobi inverses, Ac and its inverse from the scaled
+// Val: launches only (no host synchronisation, no allocation; capturable).
+void coarse_enqueue(svg_lhs* L, const double* Val, hipStream_t s)
+{
+  CoarseLevel* c = L->coarse;
+  const int nNo = L->nNo, nblk = c->nblk, nb = c->maxN / GJB;
   launch("cl_ap", k_cl_ap, dim3(blocks_for(nNo, 256)), dim3(256), 0, s, nNo, (const int*)L->d_rowPtr,
          (const int*)c->d_apPtr, (const int*)c->d_apOf, Val, c->d_apVal, c->d_apPP);
   launch("cl_dinv", k_cl_dinv, dim3(blocks_for(nNo, 256)), dim3(256), 0, s, nNo, (const int*)L->d_diagPtr, Val,
          c->d_dinv);
-  launch("cl_acinit", k_cl_acinit, dim3(blocks_for((size_t)N * N, 256)), dim3(256), 0, s, c->nAgg, N, c->d_Ac,
-         c->d_flag);
+  launch("cl_acinit", k_cl_acinit, dim3(blocks_for((size_t)c->maxN * c->maxN, 256), nblk), dim3(256), 0, s,
+         (const int*)c->d_aggOff, (const int*)c->d_blkN, (const long long*)c->d_blkOff, c->d_Ac, c->d_flag);
   launch("cl_ac", k_cl_ac, dim3(blocks_for(c->nSeg, 256)), dim3(256), 0, s, c->nSeg, (const int*)c->d_segPtr,
-         (const int*)c->d_segIdx, (const int*)c->d_segPerm, (const double*)c->d_apPP, c->d_Ac);
-  const int nb = N / GJB;
+         (const long long*)c->d_segIdx, (const int*)c->d_segPerm, (const double*)c->d_apPP, c->d_Ac);
   for (int K = 0; K < nb; K++) {
-    launch("gj_pivot", k_gj_pivot, dim3(1), dim3(1024), 0, s, (const double*)c->d_Ac, N, K, c->d_piv, c->d_flag);
-    launch("gj_panels", k_gj_panels, dim3(nb), dim3(256), 0, s, (const double*)c->d_Ac, N, K,
+    launch("gj_pivot", k_gj_pivot, dim3(1, 1, nblk), dim3(1024), 0, s, (const double*)c->d_Ac,
+           (const int*)c->d_blkN, (const long long*)c->d_blkOff, K, c->d_piv, c->d_flag);
+    launch("gj_panels", k_gj_panels, dim3(nb, 1, nblk), dim3(256), 0, s, (const double*)c->d_Ac,
+           (const int*)c->d_blkN, (const long long*)c->d_blkOff, (const long long*)c->d_panOff, K,
            (const double*)c->d_piv, c->d_rp, c->d_cp);
-    launch("gj_update", k_gj_update, dim3(nb, nb), dim3(256), 0, s, c->d_Ac, N, K, (const double*)c->d_piv,
+    launch("gj_update", k_gj_update, dim3(nb, nb, nblk), dim3(256), 0, s, c->d_Ac, (const int*)c->d_blkN,
+           (const long long*)c->d_blkOff, (const long long*)c->d_panOff, K, (const double*)c->d_piv,
            (const double*)c->d_rp, (const double*)c->d_cp);
   }
-  launch("cl_d2f", k_cl_d2f, dim3(2048), dim3(256), 0, s, (size_t)N * N, (const double*)c->d_Ac, c->d_Ai);
+  launch("cl_d2f", k_cl_d2f, dim3(2048), dim3(256), 0, s, c->total, (const double*)c->d_Ac, c->d_Ai);
+}
+
+// The elimination's pivot check (synchronises s).
+int coarse_check(const svg_lhs* L, hipStream_t s)
+{
+  const CoarseLevel* c = L->coarse;
   SVG_HIP(hipGetLastError());
   SVG_HIP(hipMemcpyAsync(c->h_flag, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, s));
   SVG_HIP(hipStreamSynchronize(s));
@@ -596,6 +706,14 @@ int coarse_prepare_dev(svg_lhs* L, const double* Val, int aggSize, hipStream_t s
     return set_error(SVG_ERR_SINGULAR, "two-level preconditioner: zero pivot " + std::to_string(*c->h_flag - 1) +
                                            " in the coarse matrix");
   return SVG_OK;
+}
+
+int coarse_prepare_dev(svg_lhs* L, const double* Val, int aggSize, hipStream_t s)
+{
+  int rc = coarse_setup(L, aggSize, nullptr, 0);
+  if (rc) return rc;
+  coarse_enqueue(L, Val, s);
+  return coarse_check(L, s);
 }
 
 template <typename T>
@@ -606,8 +724,9 @@ static void coarse_apply_t(const svg_lhs* L, const T* v, double* z64, float* z32
   const int nAgg = c->nAgg;
   launch("cl_restrict", k_cl_restrict<T>, dim3(blocks_for(nAgg, 4)), dim3(256), 0, s, nAgg,
          (const int*)c->d_aggPtr, (const int*)c->d_aggNodes, v, c->d_rc, skip);
-  launch("cl_gemv", k_cl_gemv, dim3(blocks_for(nAgg, 4)), dim3(256), sizeof(float) * c->N, s, nAgg, c->N,
-         (const float*)c->d_Ai, (const float*)c->d_rc, c->d_zc, skip);
+  launch("cl_gemv", k_cl_gemv, dim3(blocks_for(nAgg, 4)), dim3(256), 0, s, nAgg, (const int*)c->d_aggBlk,
+         (const int*)c->d_aggOff, (const int*)c->d_blkN, (const long long*)c->d_blkOff, (const float*)c->d_Ai,
+         (const float*)c->d_rc, c->d_zc, skip);
   launch("cl_apply", k_cl_apply<T>, dim3(blocks_for(L->nNo, 256)), dim3(256), 0, s, L->nNo, v,
          (const int*)c->d_apPtr, (const int*)c->d_apAgg, (const float*)c->d_apVal, (const float*)c->d_zc,
          (const int*)c->d_agg, (const float*)c->d_dinv, z64, z32, skip, mode);

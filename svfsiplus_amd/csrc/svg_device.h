@@ -170,6 +170,11 @@ void coarse_free(CoarseLevel*& c);
 // Builds the coarse level of L for aggregates of about aggSize nodes (once per
 // lhs and size), then this system's operators from the scaled Val (synchronises s).
 int coarse_prepare_dev(svg_lhs* L, const double* Val, int aggSize, hipStream_t s);
+// The same in three parts: the build (host; one dense coarse block per system of
+// sysNodeOff, null: one system), the launches alone (capturable) and the pivot check.
+int coarse_setup(svg_lhs* L, int aggSize, const int* sysNodeOff, int nsys);
+void coarse_enqueue(svg_lhs* L, const double* Val, hipStream_t s);
+int coarse_check(const svg_lhs* L, hipStream_t s);
 // z = M(v) in FP64 (z64) and/or FP32 (z32); skip: kernels return at once when *skip.
 void coarse_apply_dev(const svg_lhs* L, const float* v, double* z64, float* z32, const int* skip, hipStream_t s);
 void coarse_apply_dev(const svg_lhs* L, const double* v, double* z64, float* z32, const int* skip, hipStream_t s);
@@ -182,6 +187,7 @@ void coarse_smooth_dev(const svg_lhs* L, const float* v, double* y, const int* s
 void coarse_smooth_dev(const svg_lhs* L, const double* v, double* y, const int* skip, hipStream_t s);
 void coarse_add_dev(const svg_lhs* L, double* z64, float* z32, const int* skip, hipStream_t s);
 int coarse_num_aggregates(const svg_lhs* L);
+unsigned long long coarse_generation(const svg_lhs* L);  // changes when the level is rebuilt
 
 // Shared between the translation units.
 void krylov_free(KrylovWork& k);   // every buffer, pinned mirror and event of a Krylov workspace
@@ -258,6 +264,7 @@ struct svg_batch {
   float* u32 = nullptr;                   // mixed, learned: the network's FP32 z_i (the SpMV input)
   float* v32 = nullptr;                   // mixed precision: the Krylov basis in FP32 ((sD+1) vectors)
   double* z = nullptr;                    // learned preconditioner: flexible basis z_j (sD vectors)
+  float* z32 = nullptr;                   // two-level preconditioner, mixed: the flexible basis in FP32
   double2* d_nsc = nullptr;               // per node: (sqrt(nNo_q), 1 / sqrt(nNo_q)) of its system q
   std::vector<int> fbItr;                 // per system: iteration count at the fallback (-1: none)
   hipStream_t cs_stream = nullptr;        // capture/replay stream

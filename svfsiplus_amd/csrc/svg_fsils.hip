@@ -2402,7 +2402,8 @@ __global__ __launch_bounds__(256) void k_bxupdate(int n, double* __restrict__ X,
 __global__ __launch_bounds__(256) void k_bxupdate_cb(int n, double* __restrict__ X, const float* __restrict__ v32,
                                                      const double* __restrict__ z, int period, size_t ld, int sD,
                                                      const double* __restrict__ y, const GmState* __restrict__ st,
-                                                     const int* __restrict__ sysOfNode)
+                                                     const int* __restrict__ sysOfNode,
+                                                     const float* __restrict__ z32)
 {
 #pragma clang fp contract(off)
   const size_t k = 2 * ((size_t)blockIdx.x * blockDim.x + threadIdx.x);
@@ -2414,7 +2415,10 @@ __global__ __launch_bounds__(256) void k_bxupdate_cb(int n, double* __restrict__
   double2 x = *reinterpret_cast<const double2*>(X + k);
   for (int j = 0; j <= S.last_i; j++) {
     double2 b;
-    if (z && (period <= 1 || j % period == 0)) {
+    if (z32) {
+      const float2 f = *reinterpret_cast<const float2*>(z32 + (size_t)j * ld + k);
+      b = make_double2((double)f.x, (double)f.y);
+    } else if (z && (period <= 1 || j % period == 0)) {
       b = *reinterpret_cast<const double2*>(z + (size_t)j * ld + k);
     } else {
       const float2 f = *reinterpret_cast<const float2*>(v32 + (size_t)j * ld + k);
@@ -2477,9 +2481,23 @@ static void batch_fin(svg_batch* B, const FinArgs& fa, int m, hipStream_t s)
 struct BatchMode {
   bool mixed = false;
   svg_lpc* net = nullptr;
+  int coarse = 0;  // two-level preconditioner: aggregates of about this many nodes (0: none)
   int period = 1, window = 20;
-  bool tcheck() const { return mixed || net; }  // convergence on the FP64 true restart residual
+  bool tcheck() const { return mixed || net || coarse; }  // convergence on the FP64 true restart residual
+  bool flex() const { return net || coarse; }             // flexible GMRES (per-system fallback)
 };
+
+// The two-level preconditioner's FP32 flexible basis vector z32 = v32 for the
+// systems that fell back to the diagonal preconditioner (and still iterate).
+__global__ __launch_bounds__(256) void k_bz32fb(int n, const float* __restrict__ v32, float* __restrict__ z32,
+                                                const GmState* __restrict__ st, const int* __restrict__ sysOfNode)
+{
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (size_t)n) return;
+  const GmState& S = st[sysOfNode[k >> 2]];
+  if (!S.act || S.done || !S.fallback) return;
+  z32[k] = v32[k];
+}
 
 // z_j = u_j (and its FP32 copy) for the systems still iterating in the cycle;
 // after a network application (onlyFallback) only for the systems that fell
@@ -2514,7 +2532,7 @@ static int batch_cycle(svg_batch* B, const double* Val, const double* R, const B
   const int n = B->n, sD = B->sD;
   const unsigned nbv = blocks_for(n, 256);
   const Op A{L, Val, 4};
-  const bool mixed = md.mixed, flex = md.net != nullptr;
+  const bool mixed = md.mixed, flex = md.flex(), tl = md.coarse > 0;
   FinArgs fa = base;
   op_apply(A, B->X, B->u, R, nullptr, s);  // u0 = R - A X
   batch_mdot(B, B->u, B->u, 0, 1, 1, s);
@@ -2527,12 +2545,26 @@ static int batch_cycle(svg_batch* B, const double* Val, const double* R, const B
   hipLaunchKernelGGL(k_bdiv, dim3(nbv), dim3(256), 0, s, n, B->u, (const GmState*)B->st,
                      (const int*)B->d_sysOfNode, mixed ? B->v32 : (float*)nullptr);
   const unsigned nbv2 = blocks_for(n / 2, 256);
-  auto netStep = [&](int j) { return flex && (md.period <= 1 || j % md.period == 0); };
+  auto netStep = [&](int j) { return md.net && (md.period <= 1 || j % md.period == 0); };
   for (int i = 0; i < sD; i++) {
     double* ui = B->u + B->ld * i;
     double* un = B->u + B->ld * (i + 1);
     const double* sin = ui;
-    if (netStep(i)) {
+    if (tl) {
+      // z_i = M(u_i) with the systems' dense coarse blocks; the systems that fell
+      // back keep z_i = u_i (their flexible basis is the plain one)
+      if (mixed) {
+        coarse_apply_dev(L, (const float*)(B->v32 + B->ld * i), nullptr, B->z32 + B->ld * i, nullptr, s);
+        hipLaunchKernelGGL(k_bz32fb, dim3(nbv), dim3(256), 0, s, n, (const float*)(B->v32 + B->ld * i),
+                           B->z32 + B->ld * i, (const GmState*)B->st, (const int*)B->d_sysOfNode);
+      } else {
+        double* zi = B->z + B->ld * i;
+        coarse_apply_dev(L, (const double*)ui, zi, nullptr, nullptr, s);
+        hipLaunchKernelGGL(k_bzcopy, dim3(nbv), dim3(256), 0, s, n, (const double*)ui, zi, (float*)nullptr,
+                           (const GmState*)B->st, (const int*)B->d_sysOfNode, 1);
+        sin = zi;
+      }
+    } else if (netStep(i)) {
       double* zi = B->z + B->ld * i;
       float* z32 = mixed ? B->u32 : (float*)nullptr;
       int rc = lpc_apply_dev(md.net, L, ui, zi, z32, nullptr, s, B->d_nsc, B->d_sysOfNode, B->st);
@@ -2551,7 +2583,9 @@ static int batch_cycle(svg_batch* B, const double* Val, const double* R, const B
     fa.mode = FIN_ARNOLDI;
     fa.i = i;
     if (mixed) {
-      spmv32(L, (const float*)L->d_val32, netStep(i) ? (const float*)B->u32 : (const float*)(B->v32 + B->ld * i),
+      spmv32(L, (const float*)L->d_val32,
+             tl ? (const float*)(B->z32 + B->ld * i)
+                : (netStep(i) ? (const float*)B->u32 : (const float*)(B->v32 + B->ld * i)),
              un, nullptr, s, "bspmv32", (const int*)B->d_sysOfNode, (const GmState*)B->st);
       hipLaunchKernelGGL(k_bmdot_cb, dim3(B->nsp), dim3(RED_BS), (size_t)RED_NW * (i + 2) * sizeof(double), s,
                          (const int*)B->d_spanStart, (const int*)B->d_spanEnd, (const int*)B->d_spanSys, B->nsp,
@@ -2564,7 +2598,7 @@ static int batch_cycle(svg_batch* B, const double* Val, const double* R, const B
       op_apply(A, sin, un, nullptr, nullptr, s);
       batch_mdot(B, un, B->u, B->ld, i + 2, 2, s);
       batch_fin(B, fa, i + 2, s);
-      const bool nextPlain = flex && i + 1 < sD && !netStep(i + 1);
+      const bool nextPlain = flex && !tl && i + 1 < sD && !netStep(i + 1);
       hipLaunchKernelGGL(k_bgs_update, dim3(nbv), dim3(256), 0, s, n, B->u, B->ld, i, sD, (const double*)B->coef,
                          (const GmState*)B->st, (const int*)B->d_sysOfNode, (float*)nullptr,
                          nextPlain ? B->z + B->ld * (i + 1) : (double*)nullptr);
@@ -2574,8 +2608,9 @@ static int batch_cycle(svg_batch* B, const double* Val, const double* R, const B
                      B->y, B->st);
   if (mixed)
     hipLaunchKernelGGL(k_bxupdate_cb, dim3(nbv2), dim3(256), 0, s, n, B->X, (const float*)B->v32,
-                       flex ? (const double*)B->z : (const double*)nullptr, md.period, B->ld, sD,
-                       (const double*)B->y, (const GmState*)B->st, (const int*)B->d_sysOfNode);
+                       (flex && !tl) ? (const double*)B->z : (const double*)nullptr, md.period, B->ld, sD,
+                       (const double*)B->y, (const GmState*)B->st, (const int*)B->d_sysOfNode,
+                       tl ? (const float*)B->z32 : (const float*)nullptr);
   else
     hipLaunchKernelGGL(k_bxupdate, dim3(nbv), dim3(256), 0, s, n, B->X, (const double*)(flex ? B->z : B->u), B->ld,
                        sD, (const double*)B->y, (const GmState*)B->st, (const int*)B->d_sysOfNode);
@@ -2626,7 +2661,7 @@ static FinArgs batch_fin_args(const svg_subls* proto, int sD, const BatchMode& m
   fa.absTol = proto->absTol;
   fa.relTol = proto->relTol;
   fa.mode = FIN_NORM_R;
-  fa.fbwin = md.net ? md.window : 0;
+  fa.fbwin = md.flex() ? md.window : 0;
   return fa;
 }
 
@@ -2636,8 +2671,13 @@ static int batch_mode(const svg_solve_opts* o, BatchMode* md)
   if (!o) return SVG_OK;
   md->mixed = o->precision == SVG_PRECISION_MIXED;
   if (o->method == SVG_METHOD_LEARNED) {
-    if (!o->lpc) return set_error(SVG_ERR_ARG, "batched solve: the learned method needs a network (opts.lpc)");
+    if (!o->lpc && o->coarse <= 0)
+      return set_error(SVG_ERR_ARG, "batched solve: the learned method needs a network (opts.lpc) or a coarse level");
+    if (o->lpc && o->coarse > 0)
+      return set_error(SVG_ERR_UNSUPPORTED,
+                       "batched solve: the network and the two-level preconditioner together are not built");
     md->net = o->lpc;
+    md->coarse = o->coarse > 0 ? o->coarse : 0;
     md->period = o->net_period > 1 ? o->net_period : 1;
     md->window = o->fallback_window > 0 ? o->fallback_window : 20;
   } else if (o->method != SVG_METHOD_PLAIN) {
@@ -2660,6 +2700,11 @@ static int batch_prepare(svg_batch* B, const double* Val, const BatchMode& md, h
     if (!B->v32 && (rc = dev_alloc(&B->v32, B->ld * (B->sD + 1)))) return rc;
     // the FP32 SpMV variant for the union (timing only: the values do not matter)
     if (L->spmv32Variant < 0 && (rc = spmv32_tune(L, L->d_val32, B->u32, B->X, caller))) return rc;
+  }
+  if (md.coarse) {
+    if ((rc = coarse_setup(L, md.coarse, B->nodeOff.data(), B->nsys))) return rc;
+    if (md.mixed && !B->z32 && (rc = dev_alloc(&B->z32, B->ld * B->sD))) return rc;
+    if (!md.mixed && !B->z && (rc = dev_alloc(&B->z, B->ld * B->sD))) return rc;
   }
   if (md.net) {
     if (!B->z && (rc = dev_alloc(&B->z, B->ld * B->sD))) return rc;
@@ -2686,6 +2731,7 @@ static int batch_begin(svg_batch* B, const double* R, const double* Val, const B
   svg_lhs* L = B->L;
   int rc;
   if (md.net && (rc = lpc_prepare_dev(L, Val, s))) return rc;
+  if (md.coarse) coarse_enqueue(L, Val, s);
   if (md.mixed) launch("d2f", k_d2f, dim3(8192), dim3(256), 0, s, (size_t)16 * L->nnz, Val, L->d_val32);
   batch_mdot(B, R, R, 0, 1, 0, s);
   batch_fin(B, fa, 1, s);
@@ -2705,6 +2751,8 @@ static int batch_cycle_graph(svg_batch* B, const double* R, const double* Val, c
   k.add(B->L->spmvVariant).add(B->L->spmv32Variant).add(md.net).add(md.period).add(B->z).add(B->d_nsc);
   // the network's activation buffers are baked into the captured forward pass
   k.add(lpc_generation(md.net)).add(B->L->msgVariant);
+  // the coarse level's buffers are baked in too
+  k.add(md.coarse).add(coarse_generation(B->L)).add(B->z32).add(md.window);
   if (B->graph[gi] && B->gKey[gi] == k.b) return SVG_OK;
   if (B->graph[gi]) (void)hipGraphExecDestroy(B->graph[gi]), B->graph[gi] = nullptr;
   hipStream_t s = B->cs_stream;
@@ -2737,7 +2785,7 @@ static int batch_iterate(svg_batch* B, const BatchMode& md, int mItr, std::vecto
     SVG_HIP(hipStreamSynchronize(s));
     for (int q = 0; q < B->nsys; q++) {
       if (B->h_st[q].divzero) return set_error(SVG_ERR_ARG, "Array Divide by zero.");
-      if (md.net && B->h_st[q].fallback && B->fbItr[q] < 0) B->fbItr[q] = B->h_st[q].itr;
+      if (md.flex() && B->h_st[q].fallback && B->fbItr[q] < 0) B->fbItr[q] = B->h_st[q].itr;
     }
   }
   return SVG_OK;
@@ -3107,6 +3155,7 @@ int svg_batch_free(svg_batch* B)
   if (B->u32) (void)hipFree(B->u32);
   if (B->v32) (void)hipFree(B->v32);
   if (B->z) (void)hipFree(B->z);
+  if (B->z32) (void)hipFree(B->z32);
   if (B->d_nsc) (void)hipFree(B->d_nsc);
   if (B->h_st) (void)hipHostFree(B->h_st);
   if (B->ev) (void)hipEventDestroy(B->ev);
@@ -3132,6 +3181,7 @@ int svg_batch_gmres_ex(svg_batch* B, const svg_subls* proto, double* R, const do
   if ((rc = batch_begin(B, R, Val, md, fa, s)) || (rc = batch_cycle_graph(B, R, Val, md, fa))) return rc;
   std::vector<double> dB;
   if ((rc = batch_iterate(B, md, proto->mItr, dB))) return rc;
+  if (md.coarse && (rc = coarse_check(B->L, s))) return rc;
   hipLaunchKernelGGL(k_bresult, dim3(blocks_for(B->n, 256)), dim3(256), 0, s, B->n, R, (const double*)B->X,
                      (const GmState*)B->st, (const int*)B->d_sysOfNode, proto->absTol);
   SVG_HIP(hipGetLastError());
@@ -3215,6 +3265,7 @@ int svg_batch_newton_iteration(svg_batch* B, svg_msh* M, svg_face* Fa, const svg
   GraphKey k;
   k.add(M).add(Fa).add(*p).add(beta).add(qmTRI3).add(Ao).add(An).add(Yo).add(Yn).add(Ag).add(Yg).add(R).add(Val);
   k.add(W).add(md.mixed).add(md.net).add(fa.absTol).add(fa.relTol).add(L->d_val32).add(L->d_val16);
+  k.add(md.coarse).add(coarse_generation(L));
   if (B->pro && B->newtonKey != k.b) {
     for (hipGraphExec_t* g : {&B->pro, &B->epi})
       if (*g) (void)hipGraphExecDestroy(*g), *g = nullptr;
@@ -3258,6 +3309,7 @@ int svg_batch_newton_iteration(svg_batch* B, svg_msh* M, svg_face* Fa, const svg
   SVG_HIP(hipEventRecord(B->ev, s));
   SVG_HIP(hipStreamWaitEvent(caller, B->ev, 0));
   if ((rc = construct_fluid_check(M, s))) return rc;
+  if (md.coarse && (rc = coarse_check(L, s))) return rc;
   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   batch_records(B, proto, md, dB, secs, out);
   return SVG_OK;

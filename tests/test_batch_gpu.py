@@ -200,3 +200,76 @@ def test_batch_graph_recaptured_after_network_reallocation(torch_cuda):
     assert all(r["suc"] == 1 for r in again[0])
     fresh.free()
     bn.free()
+
+
+@pytest.mark.parametrize("precision", [_lib.PRECISION_FP64, _lib.PRECISION_MIXED])
+def test_batch_twolevel_equals_individual_twolevel(torch_cuda, precision):
+    """The two-level preconditioner in the batched solve (one dense coarse block
+    per mesh): each mesh's flexible GMRES is bit for bit the single-system
+    two-level solve of that mesh (iteration count, norms, solution), and the
+    aggregates of the union are each mesh's own aggregates, offset."""
+    torch = torch_cuda
+    meshes = _meshes(3, seed=23)
+    phys = W.Physics()
+    agg = 16
+    bn = _assembled_union(torch, meshes, phys)
+    R0, V0 = bn.fn.R.clone(), bn.fn.Val.clone()
+    boff = np.concatenate([[0], np.cumsum([m.nnz for m in meshes])])
+    opts = _lib.SolveOpts()
+    opts.precision, opts.method, opts.coarse, opts.fallback_window = precision, _lib.METHOD_LEARNED, agg, 20
+    Rb, Vb = R0.clone(), V0.clone()
+    _lib.check(_lib.load().svg_batch_fsils_solve_ex(bn.h, C.byref(bn.proto), Rb.data_ptr(), Vb.data_ptr(),
+                                                    C.byref(opts), bn.out, None))
+    torch.cuda.synchronize()
+    na_u = C.c_int()
+    agg_u = np.empty(bn.pipe.nNo, dtype=np.int32)
+    _lib.check(_lib.load().svg_coarse_info(bn.fn.lhs.h, C.byref(na_u), agg_u.ctypes.data))
+    a_off = 0
+    for q, m in enumerate(meshes):
+        a, b = int(bn.noff[q]), int(bn.noff[q + 1])
+        Rq = R0[a:b].clone()
+        Vq = V0[boff[q]:boff[q + 1]].clone()
+        L = DeviceLhs(m)
+        ls = ls_create(_lib.LS_GMRES, relTol=1e-6, absTol=1e-30, maxItr=1000, dimKry=50)
+        fsils_solve(L, ls, 4, Rq, Vq, precision=precision, coarse=agg, fallback_window=20)
+        na = C.c_int()
+        agg_q = np.empty(m.nNo, dtype=np.int32)
+        _lib.check(_lib.load().svg_coarse_info(L.h, C.byref(na), agg_q.ctypes.data))
+        assert np.array_equal(agg_u[a:b], agg_q + a_off)
+        a_off += na.value
+        diff = float((Rb[a:b] - Rq).abs().max())
+        print(f"mesh {q}: nNo {m.nNo} aggregates {na.value} itr batch {bn.out[q].itr} single {ls.RI.itr} "
+              f"max|dx| {diff:.2e}")
+        assert bn.out[q].suc == ls.RI.suc == 1
+        assert bn.out[q].itr == ls.RI.itr
+        assert bn.out[q].iNorm == ls.RI.iNorm and bn.out[q].fNorm == ls.RI.fNorm
+        assert diff == 0.0
+        L.free()
+    assert a_off == na_u.value
+    bn.free()
+
+
+def test_batch_twolevel_newton_graph_equals_eager(torch_cuda):
+    """The graph-captured Newton iteration with the two-level preconditioner
+    (its coarse blocks assembled and inverted inside the captured graph)
+    against the eager launches: identical records and state on every replay."""
+    torch = torch_cuda
+    meshes = _meshes(3, seed=29)
+    phys = W.Physics()
+    bn = B.BatchNewton(meshes, phys, precision=_lib.PRECISION_MIXED, coarse=16)
+    ref = bn.time_step(3, graph=False)
+    state = [t.clone() for t in (bn.fn.An, bn.fn.Yn, bn.fn.R)]
+    for rep in range(2):
+        bn.reset()
+        torch.cuda.synchronize()
+        got = bn.time_step(3, graph=True)
+        torch.cuda.synchronize()
+        for it in range(3):
+            for q in range(len(meshes)):
+                assert got[it][q] == ref[it][q], (rep, it, q, got[it][q], ref[it][q])
+                assert got[it][q]["suc"] == 1
+        for a, b in zip((bn.fn.An, bn.fn.Yn, bn.fn.R), state):
+            assert torch.equal(a, b), rep
+        bn.fn.R.mul_(0.5)
+        print(f"replay {rep}: itr {[r['itr'] for r in got[0]]} identical to the eager iteration")
+    bn.free()
