@@ -273,8 +273,10 @@ typedef struct {
   int fallback_itr; /* out */
   int net_period;   /* learned: the network preconditions Arnoldi steps i with i % net_period == 0, the
                        others use the diagonal scaling alone (0 or 1: every step) */
-  int coarse;       /* learned: > 0 adds the two-level preconditioner on every Arnoldi step, with node
-                       aggregates of about `coarse` nodes (0: none); `lpc` may then be NULL */
+  int coarse;       /* > 0: the two-level preconditioner on every Arnoldi step (flexible GMRES, LS type
+                       GMRES), with node aggregates of about `coarse` nodes (0: none); method PLAIN
+                       smooths with block Jacobi, method LEARNED with the network `lpc` on its steps
+                       (lpc may then be NULL: block Jacobi) */
   int reserved[2];
   svg_lpc* lpc;
 } svg_solve_opts;

@@ -10,7 +10,15 @@
 # in by Simulation-based helpers) are removed by section garbage collection
 # under a version script that exports only the harness's ref_* functions.
 #
-#   make -f oracle/ref.mk -j8
+# The second target, oracle/_ref/libsvbind.so, compiles the drop-in back-end
+# svfsiplus_amd/binding/GpuFsilsLinearAlgebra.cpp against the reference's own
+# headers (LinearAlgebra.h, ComMod.h, ...) and links it, with
+# oracle/bind_harness.cpp and the reference's solver/ls.cpp (ls_alloc,
+# ls_solve), against the objects above and svfsiplus_amd/libsvgpu.so (built
+# first by svfsiplus_amd/build.py); HIP's host runtime API from $(ROCM).
+#
+#   make -f oracle/ref.mk -j8            # both
+#   make -f oracle/ref.mk -j8 bind
 
 REF     ?= /root/reference
 HERE    := $(dir $(abspath $(lastword $(MAKEFILE_LIST))))
@@ -36,7 +44,16 @@ LS_OBJ     := $(patsubst $(LS)/%.cpp,$(OBJ)/ls_%.o,$(LS_SRC))
 SOLVER_OBJ := $(patsubst $(SOLVER)/%.cpp,$(OBJ)/sv_%.o,$(SOLVER_SRC))
 HARN_OBJ   := $(OBJ)/ref_harness.o
 
-all: $(OUT)/libsvref.so
+REPO    := $(abspath $(HERE)..)
+BIND    := $(REPO)/svfsiplus_amd/binding
+SVGLIB  := $(REPO)/svfsiplus_amd/libsvgpu.so
+ROCM    ?= /opt/rocm
+BIND_FLAGS = $(CXXFLAGS) -I$(BIND) -I$(REPO)/include -I$(ROCM)/include -D__HIP_PLATFORM_AMD__
+BIND_OBJ := $(OBJ)/bind_GpuFsilsLinearAlgebra.o $(OBJ)/bind_harness.o $(OBJ)/sv_ls.o
+
+all: $(OUT)/libsvref.so $(OUT)/libsvbind.so
+
+bind: $(OUT)/libsvbind.so
 
 $(OBJ):
 	mkdir -p $(OBJ)
@@ -50,6 +67,24 @@ $(OBJ)/sv_%.o: $(SOLVER)/%.cpp | $(OBJ)
 $(HARN_OBJ): $(HERE)ref_harness.cpp | $(OBJ)
 	$(CXX) $(CXXFLAGS) -c $< -o $@
 
+$(OBJ)/bind_GpuFsilsLinearAlgebra.o: $(BIND)/GpuFsilsLinearAlgebra.cpp $(BIND)/GpuFsilsLinearAlgebra.h \
+    $(REPO)/include/svfsi_gpu.h | $(OBJ)
+	$(CXX) $(BIND_FLAGS) -c $< -o $@
+
+$(OBJ)/bind_harness.o: $(HERE)bind_harness.cpp $(BIND)/GpuFsilsLinearAlgebra.h | $(OBJ)
+	$(CXX) $(BIND_FLAGS) -c $< -o $@
+
+$(OUT)/bind_exports.map: | $(OBJ)
+	printf '{ global: bind_*; local: *; };\n' > $@
+
+$(OUT)/libsvbind.so: $(BIND_OBJ) $(LS_OBJ) $(SOLVER_OBJ) $(SVGLIB) $(OUT)/bind_exports.map
+	$(CXX) -shared -o $@ $(BIND_OBJ) $(LS_OBJ) $(SOLVER_OBJ) \
+	  -Wl,--gc-sections -Wl,--version-script=$(OUT)/bind_exports.map -Wl,--no-undefined \
+	  -static-libstdc++ -static-libgcc \
+	  -L$(REPO)/svfsiplus_amd -lsvgpu -L$(ROCM)/lib -lamdhip64 \
+	  $(CONDA)/lib/libmpi.so $(CONDA)/lib/libmkl_rt.so \
+	  -Wl,-rpath,'$$ORIGIN/../../svfsiplus_amd' -Wl,-rpath,$(ROCM)/lib -Wl,-rpath,$(CONDA)/lib
+
 $(OUT)/exports.map: | $(OBJ)
 	printf '{ global: ref_*; local: *; };\n' > $@
 
@@ -59,4 +94,4 @@ $(OUT)/libsvref.so: $(LS_OBJ) $(SOLVER_OBJ) $(HARN_OBJ) $(OUT)/exports.map
 	  -static-libstdc++ -static-libgcc \
 	  $(CONDA)/lib/libmpi.so $(CONDA)/lib/libmkl_rt.so -Wl,-rpath,$(CONDA)/lib
 
-.PHONY: all
+.PHONY: all bind

@@ -2682,6 +2682,10 @@ static int batch_mode(const svg_solve_opts* o, BatchMode* md)
     md->window = o->fallback_window > 0 ? o->fallback_window : 20;
   } else if (o->method != SVG_METHOD_PLAIN) {
     return set_error(SVG_ERR_ARG, "batched solve: unknown method");
+  } else if (o->coarse > 0) {
+    // the two-level preconditioner with block-Jacobi smoothing
+    md->coarse = o->coarse;
+    md->window = o->fallback_window > 0 ? o->fallback_window : 20;
   }
   return SVG_OK;
 }
@@ -3003,9 +3007,13 @@ int svg_fsils_solve_ex(svg_lhs* L, svg_ls* ls, int dof, double* Ri, double* Val,
   const bool learned = opts && opts->method == SVG_METHOD_LEARNED;
   if (opts && opts->method != SVG_METHOD_PLAIN && !learned)
     return set_error(SVG_ERR_ARG, "fsils_solve: unknown method");
-  if (learned && ((!opts->lpc && opts->coarse <= 0) || (ls && ls->LS_type != SVG_LS_GMRES)))
+  if (learned && !opts->lpc && opts->coarse <= 0)
+    return set_error(SVG_ERR_UNSUPPORTED, "fsils_solve: the learned preconditioner needs a network or a coarse level");
+  // the flexible GMRES: the network, the two-level preconditioner, or both
+  const bool flex = learned || (opts && opts->coarse > 0);
+  if (flex && ls && ls->LS_type != SVG_LS_GMRES)
     return set_error(SVG_ERR_UNSUPPORTED,
-                     "fsils_solve: the learned preconditioner needs a network or a coarse level, and LS type GMRES");
+                     "fsils_solve: the learned and two-level preconditioners are built for LS type GMRES");
   if (opts) opts->fallback_itr = -1;
   if (!L || !ls || !Ri || !Val) return set_error(SVG_ERR_ARG, "svg_fsils_solve: null argument");
   hipStream_t s = (hipStream_t)stream;
@@ -3058,8 +3066,8 @@ int svg_fsils_solve_ex(svg_lhs* L, svg_ls* ls, int dof, double* Ri, double* Val,
   }
   switch (ls->LS_type) {
     case SVG_LS_GMRES:
-      if (learned)
-        rc = gmres_v_learned(L, &ls->RI, Val, Ri, opts->precision == SVG_PRECISION_MIXED, opts->lpc,
+      if (flex)
+        rc = gmres_v_learned(L, &ls->RI, Val, Ri, opts->precision == SVG_PRECISION_MIXED, learned ? opts->lpc : nullptr,
                              opts->fallback_window > 0 ? opts->fallback_window : 20, opts->net_period,
                              opts->coarse, &opts->fallback_itr, s);
       else

@@ -180,17 +180,18 @@ def construct_fluid(msh: DeviceMesh, phys: Physics, Ag, Yg, R, Val, Bf=None):
 
 def solve_opts(precision: int = _lib.PRECISION_FP64, lpc=None, fallback_window: int = 0,
                net_period: int = 1, coarse: int = 0) -> _lib.SolveOpts:
-    """The svg_solve_opts of a solve: precision, and the learned method when a
-    network is given or coarse > 0 (the two-level preconditioner with node
-    aggregates of about `coarse` nodes)."""
+    """The svg_solve_opts of a solve: precision; the learned method when a
+    network is given; coarse > 0 adds the two-level preconditioner with node
+    aggregates of about `coarse` nodes (smoothed by block Jacobi, or by the
+    network on its steps)."""
     opts = _lib.SolveOpts()
     opts.precision = precision
-    if lpc is not None or coarse > 0:
+    if lpc is not None:
         opts.method = _lib.METHOD_LEARNED
-        if lpc is not None:
-            opts.lpc = lpc.h.value if hasattr(lpc.h, "value") else lpc.h
-        opts.fallback_window = fallback_window
+        opts.lpc = lpc.h.value if hasattr(lpc.h, "value") else lpc.h
         opts.net_period = net_period
+    if lpc is not None or coarse > 0:
+        opts.fallback_window = fallback_window
         opts.coarse = coarse
     return opts
 
