@@ -1,8 +1,22 @@
 // TET4 P1-P1 VMS fluid assembly on the device: construct_fluid
 // (Code/Source/solver/fluid.cpp:464-708) with fluid_3d_m (:1697-2139),
-// fluid_3d_c (:1389-1689) and do_assem (lhsa.cpp:97-141) fused into one
-// kernel, one element per lane, launched once per element colour so that the
-// read-modify-write of Val and R blocks never races.
+// fluid_3d_c (:1389-1689) and do_assem (lhsa.cpp:97-141), in two kernels:
+//
+//   k_fluid_elem    one element per lane over the whole mesh in one launch:
+//                   geometry, the Gauss-point scalars (tauM, tauC, tauB,
+//                   u.Nx, u'.Nx) and the element residual, stored as one
+//                   640-byte record per element;
+//   k_fluid_gather  one lane per 4x4 block of the pattern: the tangent
+//                   contributions of the elements that share the block's
+//                   node pair, formed from their records and added in
+//                   increasing element order; the lane of a diagonal block
+//                   adds the residual rows of its node the same way.
+//
+// Each block and residual row receives its element contributions in element
+// order, as the reference's element loop adds them, and every operation is
+// rounded separately (no contraction into fused multiply-adds, as in the
+// reference's build), so Val and R equal the reference's bit for bit. No
+// colouring and no atomics: each lane owns the block (row) it writes.
 //
 // Specialisations, each exact for this element and material (the values
 // dropped are signed zeros, which leave every sum unchanged):
@@ -15,9 +29,7 @@
 //     terms of the tangent and the viscosity-gradient terms of updu vanish and
 //     updu(i,j,b) = delta_ij * (-rho uNx_b - mu K N_b);
 //   * all four TET4 Gauss weights are 1/24, so w and wl are per element.
-// Per-entry summation over Gauss points keeps the reference order g = 0..3;
-// element contributions are summed into Val in colour order instead of
-// element order (the only intended difference: rounding of the sums).
+// Per-entry summation over Gauss points keeps the reference order g = 0..3.
 #include "svg_device.h"
 
 #include <cfloat>
@@ -36,53 +48,35 @@ struct FluidConst {
   double N[4][4];  // N[g][a]
 };
 
-__device__ __forceinline__ bool is_zero_dev(double v)
+// Element record (kTet4Rec doubles per element, caller's element order), laid
+// out so that it is written in 16-byte pairs as the values are produced:
+//   [0] wl  [1] 0  [2+2g] tauM(g)  [3+2g] tauC(g)  [10+g] tauB(g)  [14..15] 0
+//   node chunk a at 16 + 16 a:
+//     [0..2] Nx(:,a)  [3] 0  [4+2g] uNx(g,a)  [5+2g] upNx(g,a)  [12..15] lR(0..3,a)
+constexpr int kRecNode = 16;
+static_assert(kTet4Rec == 16 + 4 * kRecNode, "element record layout");
+constexpr int kRecTauM = 2, kRecTauB = 10, kRecU = 4, kRecR = 12;
+
+__host__ __device__ __forceinline__ void put2(double* __restrict__ p, int k, double x, double y)
+{
+  *reinterpret_cast<double2*>(p + k) = make_double2(x, y);
+}
+
+__host__ __device__ __forceinline__ bool tet4_is_zero(double v)
 {
   double a = fabs(v);
   double nrm = fmax(a, DBL_EPSILON);
   return (a / nrm) < 10.0 * DBL_EPSILON;
 }
 
-__global__ __launch_bounds__(128) void k_fluid_tet4(int q0, int qn, const int4* __restrict__ ien,
-                                                     const int* __restrict__ slot, int nElPad,
-                                                     const double* __restrict__ X, const double* __restrict__ Ag,
-                                                     const double* __restrict__ Yg, const double* __restrict__ Bf,
-                                                     FluidConst C, double* __restrict__ R, double* __restrict__ Val,
-                                                     int* __restrict__ err)
+// One element: returns 1 (and writes nothing) on a zero Jacobian, else fills
+// its record. xl(3,4) coordinates, al(3,4) acceleration minus body force,
+// yl(4,4) velocity and pressure, as the reference's local arrays.
+__host__ __device__ __forceinline__ int tet4_element(const double (&xl)[3][4], const double (&al)[3][4],
+                                                     const double (&yl)[4][4], const FluidConst& C,
+                                                     double* __restrict__ rec)
 {
-  int q = q0 + blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= q0 + qn) return;
-  const int4 nd4 = ien[q];
-  const int nd[4] = {nd4.x, nd4.y, nd4.z, nd4.w};
-
-  double xl[3][4], al[3][4], yl[4][4];
-#pragma unroll
-  for (int a = 0; a < 4; a++) {
-    const double* xp = X + 3 * (size_t)nd[a];
-    xl[0][a] = xp[0];
-    xl[1][a] = xp[1];
-    xl[2][a] = xp[2];
-    const double4 ya = *reinterpret_cast<const double4*>(Yg + 4 * (size_t)nd[a]);
-    yl[0][a] = ya.x;
-    yl[1][a] = ya.y;
-    yl[2][a] = ya.z;
-    yl[3][a] = ya.w;
-    const double4 aa = *reinterpret_cast<const double4*>(Ag + 4 * (size_t)nd[a]);
-    al[0][a] = aa.x;
-    al[1][a] = aa.y;
-    al[2][a] = aa.z;
-    if (Bf) {
-      const double* bp = Bf + 3 * (size_t)nd[a];
-      al[0][a] = al[0][a] - bp[0];
-      al[1][a] = al[1][a] - bp[1];
-      al[2][a] = al[2][a] - bp[2];
-    } else {
-      al[0][a] = al[0][a] - 0.0;
-      al[1][a] = al[1][a] - 0.0;
-      al[2][a] = al[2][a] - 0.0;
-    }
-  }
-
+#pragma clang fp contract(off)
   // ---- gnn, nn.cpp:503-540 (TET4 parent gradients are 0/1/-1: xXi = x_j - x_3)
   double xXi[3][3];
 #pragma unroll
@@ -92,10 +86,7 @@ __global__ __launch_bounds__(128) void k_fluid_tet4(int q0, int qn, const int4* 
   const double Jac = xXi[0][0] * xXi[1][1] * xXi[2][2] + xXi[0][1] * xXi[1][2] * xXi[2][0] +
                      xXi[0][2] * xXi[1][0] * xXi[2][1] - xXi[0][0] * xXi[1][2] * xXi[2][1] -
                      xXi[0][1] * xXi[1][0] * xXi[2][2] - xXi[0][2] * xXi[1][1] * xXi[2][0];
-  if (is_zero_dev(Jac)) {
-    atomicMin(err, q);
-    return;
-  }
+  if (tet4_is_zero(Jac)) return 1;
   double xiX[3][3];
   xiX[0][0] = (xXi[1][1] * xXi[2][2] - xXi[1][2] * xXi[2][1]) / Jac;
   xiX[0][1] = (xXi[2][1] * xXi[0][2] - xXi[2][2] * xXi[0][1]) / Jac;
@@ -125,11 +116,11 @@ __global__ __launch_bounds__(128) void k_fluid_tet4(int q0, int qn, const int4* 
     Nx[i][3] = -xiX[0][i] - xiX[1][i] - xiX[2][i];
   }
 
-  const double rho = C.rho, mu = C.mu, K = C.K;
+  const double rho = C.rho, mu = C.mu;
   const double w = C.wg * Jac;
   const double wl = w * C.T1;
   const double wr = w * rho;
-  const double muK = mu * K;
+  const double muK = mu * C.K;
 
   // Element-constant kinematics (P1 gradients).
   double ux[3][3];
@@ -165,9 +156,16 @@ __global__ __launch_bounds__(128) void k_fluid_tet4(int q0, int qn, const int4* 
 #pragma unroll
     for (int a = 0; a < 4; a++) lR[i][a] = 0.0;
 
-  // Per-Gauss-point scalars kept for the tangent loop.
-  double gTauM[4], gTauC[4], gTauB[4], gUNx[4][4], gUpNx[4][4];
+  put2(rec, 14, 0.0, 0.0);
+#pragma unroll
+  for (int a = 0; a < 4; a++) {
+    double* A = rec + 16 + kRecNode * a;
+    put2(A, 0, Nx[0][a], Nx[1][a]);
+    put2(A, 2, Nx[2][a], 0.0);
+  }
 
+  double gUpNx[4][4];
+  double tauB_prev = 0.0;
 #pragma unroll
   for (int g = 0; g < 4; g++) {
     const double* N = C.N[g];
@@ -201,7 +199,7 @@ __global__ __launch_bounds__(128) void k_fluid_tet4(int q0, int qn, const int4* 
     double tauB = up[0] * up[0] * Kxi[0][0] + up[1] * up[0] * Kxi[1][0] + up[2] * up[0] * Kxi[2][0] +
                   up[0] * up[1] * Kxi[0][1] + up[1] * up[1] * Kxi[1][1] + up[2] * up[1] * Kxi[2][1] +
                   up[0] * up[2] * Kxi[0][2] + up[1] * up[2] * Kxi[1][2] + up[2] * up[2] * Kxi[2][2];
-    if (is_zero_dev(tauB)) tauB = DBL_EPSILON;
+    if (tet4_is_zero(tauB)) tauB = DBL_EPSILON;
     tauB = rho / sqrt(tauB);
     double ua[3] = {u[0] + up[0], u[1] + up[1], u[2] + up[2]};
     const double pa = p - tauC * divU;
@@ -225,101 +223,211 @@ __global__ __launch_bounds__(128) void k_fluid_tet4(int q0, int qn, const int4* 
       for (int i = 0; i < 3; i++) {
         lR[i][a] = lR[i][a] + wr * N[a] * rV3[i] + w * (Nx[0][a] * rM[0][i] + Nx[1][a] * rM[1][i] + Nx[2][a] * rM[2][i]);
       }
-      gUNx[g][a] = u[0] * Nx[0][a] + u[1] * Nx[1][a] + u[2] * Nx[2][a];
       gUpNx[g][a] = up[0] * Nx[0][a] + up[1] * Nx[1][a] + up[2] * Nx[2][a];
+      put2(rec + 16 + kRecNode * a, kRecU + 2 * g, u[0] * Nx[0][a] + u[1] * Nx[1][a] + u[2] * Nx[2][a],
+           gUpNx[g][a]);
     }
     // Brinkman term (fluid.cpp:2132-2138)
 #pragma unroll
     for (int a = 0; a < 4; a++)
 #pragma unroll
       for (int i = 0; i < 3; i++) lR[i][a] = lR[i][a] + muK * w * N[a] * (u[i] + up[i]);
-    gTauM[g] = tauM;
-    gTauC[g] = tauC;
-    gTauB[g] = tauB;
+    if (g == 0)
+      put2(rec, 0, wl, 0.0);
+    put2(rec, kRecTauM + 2 * g, tauM, tauC);
+    if (g & 1)
+      put2(rec, kRecTauB + g - 1, tauB_prev, tauB);
+    else
+      tauB_prev = tauB;
   }
   // Continuity residual (fluid_3d_c, fluid.cpp:1655-1658), Gauss points in order.
 #pragma unroll
   for (int g = 0; g < 4; g++)
 #pragma unroll
     for (int a = 0; a < 4; a++) lR[3][a] = lR[3][a] + w * (C.N[g][a] * divU - gUpNx[g][a]);
-
-  // ---- R scatter (do_assem, lhsa.cpp:113-115)
 #pragma unroll
   for (int a = 0; a < 4; a++) {
-    double4* rp = reinterpret_cast<double4*>(R + 4 * (size_t)nd[a]);
-    double4 r = *rp;
-    r.x = r.x + lR[0][a];
-    r.y = r.y + lR[1][a];
-    r.z = r.z + lR[2][a];
-    r.w = r.w + lR[3][a];
-    *rp = r;
+    double* A = rec + 16 + kRecNode * a;
+    put2(A, kRecR, lR[0][a], lR[1][a]);
+    put2(A, kRecR + 2, lR[2][a], lR[3][a]);
   }
+  return 0;
+}
 
-  // ---- tangent blocks, one (a,b) pair at a time, accumulated over g in order
+// The 4x4 tangent block (a, b) of one element from its record: E the element
+// part, A and B the node chunks of a and b, Na/Nb the shape functions of a and
+// b at the four Gauss points; acc(4i+j) as lK(4i+j, a, b), summed over g in
+// order (fluid.cpp:2058-2130 momentum and pressure columns, 1662-1688
+// continuity rows).
+__host__ __device__ __forceinline__ void tet4_block(const double* __restrict__ E, const double* __restrict__ A,
+                                                    const double* __restrict__ B, const double (&Na)[4],
+                                                    const double (&Nb)[4], const FluidConst& C, double (&acc)[16])
+{
+#pragma clang fp contract(off)
+  const double rho = C.rho, mu = C.mu, muK = mu * C.K;
   const double rho_amd = rho * C.amd;
-#pragma unroll 1
-  for (int ab = 0; ab < 16; ab++) {
-    const int a = ab >> 2, b = ab & 3;
-    const int s = slot[(size_t)(a * 4 + b) * nElPad + q];
-    double Nxa[3], Nxb[3];
+  const double wl = E[0];
+  const double Nxa[3] = {A[0], A[1], A[2]}, Nxb[3] = {B[0], B[1], B[2]};
+  const double NxNx = Nxa[0] * Nxb[0] + Nxa[1] * Nxb[1] + Nxa[2] * Nxb[2];
 #pragma unroll
-    for (int i = 0; i < 3; i++) {
-      // dynamic a/b index into register arrays: select through a small switch
-      Nxa[i] = (a == 0) ? Nx[i][0] : (a == 1) ? Nx[i][1] : (a == 2) ? Nx[i][2] : Nx[i][3];
-      Nxb[i] = (b == 0) ? Nx[i][0] : (b == 1) ? Nx[i][1] : (b == 2) ? Nx[i][2] : Nx[i][3];
-    }
-    const double NxNx = Nxa[0] * Nxb[0] + Nxa[1] * Nxb[1] + Nxa[2] * Nxb[2];
-    double acc[16];
+  for (int k = 0; k < 16; k++) acc[k] = 0.0;
 #pragma unroll
-    for (int k = 0; k < 16; k++) acc[k] = 0.0;
+  for (int g = 0; g < 4; g++) {
+    const double uNxa = A[kRecU + 2 * g], uNxb = B[kRecU + 2 * g];
+    const double upNxa = A[kRecU + 2 * g + 1], upNxb = B[kRecU + 2 * g + 1];
+    const double tauM = E[kRecTauM + 2 * g], tauC = E[kRecTauM + 2 * g + 1], tauB = E[kRecTauB + g];
+    const double uaNxa = uNxa + upNxa;
+    const double T1ub = -rho * uNxb - muK * Nb[g];  // updu(i,i,b)
+    const double rtu = rho * tauM * uaNxa;
+    // momentum, fluid.cpp:2058-2113
+    const double T1 = mu * NxNx + rho_amd * Nb[g] * (Na[g] + rho * tauM * uaNxa) + rho * Na[g] * (uNxb + upNxb) +
+                      tauB * upNxa * upNxb;
 #pragma unroll
-    for (int g = 0; g < 4; g++) {
-      const double Na = C.N[g][a], Nb = C.N[g][b];
-      const double uNxa = (a == 0) ? gUNx[g][0] : (a == 1) ? gUNx[g][1] : (a == 2) ? gUNx[g][2] : gUNx[g][3];
-      const double uNxb = (b == 0) ? gUNx[g][0] : (b == 1) ? gUNx[g][1] : (b == 2) ? gUNx[g][2] : gUNx[g][3];
-      const double upNxa = (a == 0) ? gUpNx[g][0] : (a == 1) ? gUpNx[g][1] : (a == 2) ? gUpNx[g][2] : gUpNx[g][3];
-      const double upNxb = (b == 0) ? gUpNx[g][0] : (b == 1) ? gUpNx[g][1] : (b == 2) ? gUpNx[g][2] : gUpNx[g][3];
-      const double tauM = gTauM[g], tauC = gTauC[g], tauB = gTauB[g];
-      const double uaNxa = uNxa + upNxa;
-      const double T1ub = -rho * uNxb - muK * Nb;  // updu(i,i,b)
-      const double rtu = rho * tauM * uaNxa;
-      // momentum, fluid.cpp:2058-2113
-      const double T1 = mu * NxNx + rho_amd * Nb * (Na + rho * tauM * uaNxa) + rho * Na * (uNxb + upNxb) +
-                        tauB * upNxa * upNxb;
-#pragma unroll
-      for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-          if (i == j) {
-            const double T2 = (mu + tauC) * (Nxa[i] * Nxb[i]) - rtu * T1ub;
-            acc[4 * i + j] = acc[4 * i + j] + wl * (T2 + T1);
-            acc[4 * i + j] = acc[4 * i + j] + muK * wl * Nb * Na;
-          } else {
-            const double T2 = mu * (Nxa[j] * Nxb[i]) + tauC * (Nxa[i] * Nxb[j]);
-            acc[4 * i + j] = acc[4 * i + j] + wl * T2;
-          }
-        }
-      // pressure columns, fluid.cpp:2117-2130
-#pragma unroll
-      for (int i = 0; i < 3; i++) acc[4 * i + 3] = acc[4 * i + 3] - wl * (Nxa[i] * Nb - Nxb[i] * rtu);
-      // continuity rows, fluid.cpp:1662-1688
-      const double T1c = rho_amd * Nb;
+    for (int i = 0; i < 3; i++)
 #pragma unroll
       for (int j = 0; j < 3; j++) {
-        const double T2 = Nxa[j] * (T1ub - T1c);
-        acc[12 + j] = acc[12 + j] + wl * (Na * Nxb[j] - tauM * T2);
+        if (i == j) {
+          const double T2 = (mu + tauC) * (Nxa[i] * Nxb[i]) - rtu * T1ub;
+          acc[4 * i + j] = acc[4 * i + j] + wl * (T2 + T1);
+          acc[4 * i + j] = acc[4 * i + j] + muK * wl * Nb[g] * Na[g];
+        } else {
+          const double T2 = mu * (Nxa[j] * Nxb[i]) + tauC * (Nxa[i] * Nxb[j]);
+          acc[4 * i + j] = acc[4 * i + j] + wl * T2;
+        }
       }
-      acc[15] = acc[15] + wl * tauM * NxNx;
+    // pressure columns, fluid.cpp:2117-2130
+#pragma unroll
+    for (int i = 0; i < 3; i++) acc[4 * i + 3] = acc[4 * i + 3] - wl * (Nxa[i] * Nb[g] - Nxb[i] * rtu);
+    // continuity rows, fluid.cpp:1662-1688
+    const double T1c = rho_amd * Nb[g];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      const double T2 = Nxa[j] * (T1ub - T1c);
+      acc[12 + j] = acc[12 + j] + wl * (Na[g] * Nxb[j] - tauM * T2);
     }
-    double2* vp = reinterpret_cast<double2*>(Val + 16 * (size_t)s);
+    acc[15] = acc[15] + wl * tauM * NxNx;
+  }
+}
+
+// Gathers an element's local arrays (construct_fluid, fluid.cpp:575-598).
+__host__ __device__ __forceinline__ void tet4_gather(const int (&nd)[4], const double* __restrict__ X,
+                                                    const double* __restrict__ Ag, const double* __restrict__ Yg,
+                                                    const double* __restrict__ Bf, double (&xl)[3][4],
+                                                    double (&al)[3][4], double (&yl)[4][4])
+{
+#pragma unroll
+  for (int a = 0; a < 4; a++) {
+    const double* xp = X + 3 * (size_t)nd[a];
+    xl[0][a] = xp[0];
+    xl[1][a] = xp[1];
+    xl[2][a] = xp[2];
+    const double* yp = Yg + 4 * (size_t)nd[a];
+    const double* ap = Ag + 4 * (size_t)nd[a];
+#pragma unroll
+    for (int i = 0; i < 4; i++) yl[i][a] = yp[i];
+#pragma unroll
+    for (int i = 0; i < 3; i++) al[i][a] = ap[i] - (Bf ? Bf[3 * (size_t)nd[a] + i] : 0.0);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_fluid_elem(int nEl, const int4* __restrict__ ien,
+                                                    const double* __restrict__ X, const double* __restrict__ Ag,
+                                                    const double* __restrict__ Yg, const double* __restrict__ Bf,
+                                                    FluidConst C, double* __restrict__ rec, int* __restrict__ err)
+{
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nEl) return;
+  const int4 v = ien[e];
+  const int nd[4] = {v.x, v.y, v.z, v.w};
+  double xl[3][4], al[3][4], yl[4][4];
+  tet4_gather(nd, X, Ag, Yg, Bf, xl, al, yl);
+  if (tet4_element(xl, al, yl, C, rec + (size_t)kTet4Rec * e)) atomicMin(err, e);
+}
+
+__device__ __forceinline__ double sel4(const double (&v)[4], int a)
+{
+  return a == 0 ? v[0] : a == 1 ? v[1] : a == 2 ? v[2] : v[3];
+}
+
+// One lane per block slot s: Val(:, s) += the tangent blocks of its element
+// entries in element order; the lane of a diagonal block also adds the
+// residual rows of its node, R(:, i) += lR(:, a), in the same order.
+__global__ __launch_bounds__(256) void k_fluid_gather(int nnz, const int* __restrict__ sorder,
+                                                      const int* __restrict__ cptr, const int* __restrict__ clist,
+                                                      const int4* __restrict__ ien, const double* __restrict__ rec,
+                                                      FluidConst C, double* __restrict__ R, double* __restrict__ Val)
+{
+#pragma clang fp contract(off)
+  // each XCD walks one contiguous band of the lane order (its L2 keeps the
+  // element records of the band's rows)
+  const int bx = xcd_block(blockIdx.x, gridDim.x);
+  if (bx < 0) return;
+  const int t = bx * blockDim.x + threadIdx.x;
+  if (t >= nnz) return;
+  const int s = sorder[t];
+  const int c0 = cptr[s], c1 = cptr[s + 1];
+  if (c0 == c1) return;
+  double v[16];
+  double2* vp = reinterpret_cast<double2*>(Val + 16 * (size_t)s);
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const double2 t = vp[k];
+    v[2 * k] = t.x;
+    v[2 * k + 1] = t.y;
+  }
+  double r[4] = {0.0, 0.0, 0.0, 0.0};
+  int node = -1;
+  for (int c = c0; c < c1; c++) {
+    const int code = clist[c];
+    const int e = code >> 4, a = (code >> 2) & 3, b = code & 3;
+    const double* Er = rec + (size_t)kTet4Rec * e;
+    double E[14], A[16], B[12];
+#pragma unroll
+    for (int k = 0; k < 7; k++) {
+      const double2 t = reinterpret_cast<const double2*>(Er)[k];
+      E[2 * k] = t.x;
+      E[2 * k + 1] = t.y;
+    }
 #pragma unroll
     for (int k = 0; k < 8; k++) {
-      double2 v = vp[k];
-      v.x = v.x + acc[2 * k];
-      v.y = v.y + acc[2 * k + 1];
-      vp[k] = v;
+      const double2 u = reinterpret_cast<const double2*>(Er + 16 + kRecNode * a)[k];
+      A[2 * k] = u.x;
+      A[2 * k + 1] = u.y;
+    }
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+      const double2 u = reinterpret_cast<const double2*>(Er + 16 + kRecNode * b)[k];
+      B[2 * k] = u.x;
+      B[2 * k + 1] = u.y;
+    }
+    double Na[4], Nb[4];
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+      const double Ng[4] = {C.N[g][0], C.N[g][1], C.N[g][2], C.N[g][3]};
+      Na[g] = sel4(Ng, a);
+      Nb[g] = sel4(Ng, b);
+    }
+    double acc[16];
+    tet4_block(E, A, B, Na, Nb, C, acc);
+#pragma unroll
+    for (int k = 0; k < 16; k++) v[k] = v[k] + acc[k];
+    if (a == b) {
+      if (node < 0) {
+        const int4 nv = ien[e];
+        node = a == 0 ? nv.x : a == 1 ? nv.y : a == 2 ? nv.z : nv.w;
+        const double4 t = reinterpret_cast<const double4*>(R)[node];
+        r[0] = t.x;
+        r[1] = t.y;
+        r[2] = t.z;
+        r[3] = t.w;
+      }
+#pragma unroll
+      for (int i = 0; i < 4; i++) r[i] = r[i] + A[kRecR + i];
     }
   }
+#pragma unroll
+  for (int k = 0; k < 8; k++) vp[k] = make_double2(v[2 * k], v[2 * k + 1]);
+  if (node >= 0) reinterpret_cast<double4*>(R)[node] = make_double4(r[0], r[1], r[2], r[3]);
 }
 
 __global__ void k_set_int(int* p, int v) { *p = v; }
@@ -428,14 +536,15 @@ int construct_fluid_enqueue(svg_msh* M, const svg_fluid_params* p, const double*
     C.N[g][3] = 1.0 - xi[g][0] - xi[g][1] - xi[g][2];
   }
   launch("set_int", k_set_int, dim3(1), dim3(1), 0, s, M->d_err, INT_MAX);
-  const int bs = 128;
-  for (int c = 0; c < M->nColors; c++) {
-    int q0 = M->colorOff[c], qn = M->colorOff[c + 1] - q0;
-    if (qn <= 0) continue;
-    launch("fluid_tet4", k_fluid_tet4, dim3(blocks_for(qn, bs)), dim3(bs), 0, s, q0, qn,
-           reinterpret_cast<const int4*>(M->d_ien), (const int*)M->d_slot, M->nElPad, (const double*)M->d_x, Ag, Yg,
-           Bf, C, R, Val, M->d_err);
-  }
+  const svg_lhs* L = M->lhs;
+  launch("fluid_elem", k_fluid_elem, dim3(blocks_for(M->nEl, 256)), dim3(256), 0, s, M->nEl,
+         reinterpret_cast<const int4*>(M->d_ien0), (const double*)M->d_x, Ag, Yg, Bf, C, M->d_rec, M->d_err);
+  // a zero Jacobian leaves its record unwritten; the gather still runs and
+  // construct_fluid_check reports the element (Val and R are then undefined,
+  // as the reference stops with an exception)
+  launch("fluid_gather", k_fluid_gather, dim3((blocks_for(L->nnz, 256) + 7) & ~7u), dim3(256), 0, s, L->nnz,
+         (const int*)M->d_sorder, (const int*)M->d_cptr, (const int*)M->d_clist, reinterpret_cast<const int4*>(M->d_ien0),
+         (const double*)M->d_rec, C, R, Val);
   SVG_HIP(hipGetLastError());
   return SVG_OK;
 }
@@ -445,8 +554,8 @@ int construct_fluid_check(svg_msh* M, hipStream_t s)
   SVG_HIP(hipMemcpyAsync(M->h_err, M->d_err, sizeof(int), hipMemcpyDeviceToHost, s));
   SVG_HIP(hipStreamSynchronize(s));
   if (*M->h_err != INT_MAX) {
-    return set_error(SVG_ERR_JACOBIAN, "[construct_fluid] Jacobian for element (colour-sorted index " +
-                                           std::to_string(*M->h_err) + ") is < 0.");
+    return set_error(SVG_ERR_JACOBIAN,
+                     "[construct_fluid] Jacobian for element " + std::to_string(*M->h_err) + " is < 0.");
   }
   return SVG_OK;
 }

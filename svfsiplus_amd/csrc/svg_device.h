@@ -286,7 +286,23 @@ struct svg_msh {
   int* h_err = nullptr;        // pinned mirror
   int nElPad = 0;
   std::vector<int> ien;        // (4, nEl) in the caller's element order (face -> parent lookups)
+  // Element-order assembly (construct_fluid): per block slot s of the pattern,
+  // the element pairs that add to it, clist[cptr[s] .. cptr[s+1]) = e*16 + 4a + b
+  // in increasing element order e, and the per-element record the gather reads.
+  int* d_ien0 = nullptr;       // (4, nEl) in the caller's element order (int4 per element)
+  int* d_cptr = nullptr;       // nnz + 1
+  int* d_clist = nullptr;      // 16 nEl
+  int* d_sorder = nullptr;     // the nnz slots ordered by their entry count (the gather's lane order)
+  double* d_rec = nullptr;     // kTet4Rec * nEl
 };
+
+// Doubles per element in the element-order assembly's record.
+constexpr int kTet4Rec = 80;
+// Builds d_ien0, d_cptr, d_clist and d_rec from the caller-order IEN on the
+// device (d_IEN: 4 nEl ints, 16-byte aligned); synchronises s.
+namespace svg {
+int msh_gather_build(svg_msh* M, const int* d_IEN, hipStream_t s);
+}
 
 // A TRI3 Neumann face of a fluid mesh (faceType: IEN, gE), colour-sorted.
 struct svg_face {

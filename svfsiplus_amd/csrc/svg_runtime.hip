@@ -467,6 +467,19 @@ int svg_msh_create(const svg_lhs* L, int nEl, const int* IEN, int nNo, const dou
     svg_msh_free(M);
     return hip_check(e, "svg_msh_create: upload");
   }
+  // the element-order assembly map, built on the device from the caller-order IEN
+  int* d_IEN = nullptr;
+  if ((rc = dev_alloc(&d_IEN, (size_t)4 * nEl))) {
+    svg_msh_free(M);
+    return rc;
+  }
+  e = hipMemcpy(d_IEN, IEN, sizeof(int) * 4 * (size_t)nEl, hipMemcpyHostToDevice);
+  rc = e == hipSuccess ? msh_gather_build(M, d_IEN, nullptr) : hip_check(e, "svg_msh_create: upload");
+  dev_free(d_IEN);
+  if (rc) {
+    svg_msh_free(M);
+    return rc;
+  }
   *out = M;
   return SVG_OK;
 }
@@ -478,6 +491,11 @@ int svg_msh_free(svg_msh* M)
   dev_free(M->d_ien);
   dev_free(M->d_slot);
   dev_free(M->d_err);
+  dev_free(M->d_ien0);
+  dev_free(M->d_cptr);
+  dev_free(M->d_clist);
+  dev_free(M->d_sorder);
+  dev_free(M->d_rec);
   if (M->h_err) (void)hipHostFree(M->h_err);
   delete M;
   return SVG_OK;

@@ -198,6 +198,76 @@ struct Scratch {
   }
 };
 
+// Element-order assembly map: entry k = 16 e + 4 a + b of element e (caller
+// order) adds to the block slot (ien[a], ien[b]); key[k] = that slot.
+__global__ __launch_bounds__(256) void k_contrib_keys(int nEl, const int4* __restrict__ ien,
+                                                      const int* __restrict__ rowPtr, const int* __restrict__ colPtr,
+                                                      int* __restrict__ key, int* __restrict__ val,
+                                                      int* __restrict__ err)
+{
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= 16LL * nEl) return;
+  const int e = (int)(k >> 4), a = (int)((k >> 2) & 3), b = (int)(k & 3);
+  const int4 v = ien[e];
+  const int nd[4] = {v.x, v.y, v.z, v.w};
+  const int row = a == 0 ? nd[0] : a == 1 ? nd[1] : a == 2 ? nd[2] : nd[3];
+  const int col = b == 0 ? nd[0] : b == 1 ? nd[1] : b == 2 ? nd[2] : nd[3];
+  const int s = rowPtr[row], t = rowPtr[row + 1];
+  int lo = s, hi = t;  // first position with colPtr >= col
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (colPtr[mid] < col)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  if (lo == t || colPtr[lo] != col) {
+    atomicOr(err, 4);
+    lo = s;
+  }
+  key[k] = lo;
+  val[k] = (int)k;
+}
+
+// key[s] = (window of slot s's row) * 256 + its entry count (clamped to 255),
+// val[s] = s: the gather's lanes walk windows of kGatherRows rows in order and,
+// inside a window, the slots by entry count.
+constexpr int kGatherRows = 1024;
+__global__ __launch_bounds__(256) void k_slot_count(int nnz, const int* __restrict__ cptr,
+                                                    const int* __restrict__ clist, const int4* __restrict__ ien,
+                                                    int* __restrict__ key, int* __restrict__ val)
+{
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nnz) return;
+  const int c0 = cptr[s], n = cptr[s + 1] - c0;
+  int row = 0;
+  if (n > 0) {
+    const int code = clist[c0];
+    const int4 v = ien[code >> 4];
+    const int a = (code >> 2) & 3;
+    row = a == 0 ? v.x : a == 1 ? v.y : a == 2 ? v.z : v.w;
+  }
+  key[s] = (row / kGatherRows) * 256 + min(n, 255);
+  val[s] = s;
+}
+
+// cptr[s] = first position of slot s in the sorted keys (lower bound), s in [0, nnz].
+__global__ __launch_bounds__(256) void k_contrib_ptr(int nnz, long long n, const int* __restrict__ sorted,
+                                                     int* __restrict__ cptr)
+{
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s > nnz) return;
+  long long lo = 0, hi = n;
+  while (lo < hi) {
+    const long long mid = (lo + hi) >> 1;
+    if (sorted[mid] < s)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  cptr[s] = (int)lo;
+}
+
 int check_ien(long long n, const int* d_ien, int nNo, int* d_err, hipStream_t s, const char* who)
 {
   SVG_HIP(hipMemsetAsync(d_err, 0, sizeof(int), s));
@@ -210,6 +280,68 @@ int check_ien(long long n, const int* d_ien, int nNo, int* d_err, hipStream_t s,
 }
 
 }  // namespace
+
+// The element-order assembly's map and record storage for M (see svg_msh):
+// every (element, a, b) entry keyed by its block slot, sorted stably by slot
+// (so each slot's entries stay in element order), and the slot offsets.
+int msh_gather_build(svg_msh* M, const int* d_IEN, hipStream_t s)
+{
+  const svg_lhs* L = M->lhs;
+  const int nEl = M->nEl, nnz = L->nnz;
+  if (nEl >= (1 << 27)) return set_error(SVG_ERR_ARG, "element-order assembly: more than 2^27 elements");
+  if (reinterpret_cast<uintptr_t>(d_IEN) & 15)
+    return set_error(SVG_ERR_ARG, "element-order assembly: IEN must be 16-byte aligned (read as int4)");
+  const long long n = 16LL * nEl;
+  Scratch S;
+  int rc;
+  int *key = nullptr, *sorted = nullptr, *val = nullptr, *d_err = nullptr;
+  if ((rc = dev_alloc(&M->d_ien0, (size_t)4 * nEl)) || (rc = dev_alloc(&M->d_cptr, (size_t)nnz + 1)) ||
+      (rc = dev_alloc(&M->d_clist, (size_t)n)) || (rc = dev_alloc(&M->d_rec, (size_t)kTet4Rec * nEl)))
+    return rc;
+  if ((rc = S.get(&key, (size_t)n)) || (rc = S.get(&sorted, (size_t)n)) || (rc = S.get(&val, (size_t)n)) ||
+      (rc = S.get(&d_err, 1)))
+    return rc;
+  SVG_HIP(hipMemcpyAsync(M->d_ien0, d_IEN, sizeof(int) * 4 * (size_t)nEl, hipMemcpyDeviceToDevice, s));
+  SVG_HIP(hipMemsetAsync(d_err, 0, sizeof(int), s));
+  hipLaunchKernelGGL(k_contrib_keys, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, nEl,
+                     reinterpret_cast<const int4*>(M->d_ien0), (const int*)L->d_rowPtr, (const int*)L->d_colPtr, key,
+                     val, d_err);
+  SVG_HIP(hipGetLastError());
+  const int endBit = bits_for((unsigned long long)(nnz > 0 ? nnz - 1 : 0));
+  size_t tb = 0;
+  SVG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, (const int*)key, sorted, (const int*)val, M->d_clist,
+                                             (int)n, 0, endBit, s));
+  unsigned char* tmp = nullptr;
+  if ((rc = S.get(&tmp, tb))) return rc;
+  SVG_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, (const int*)key, sorted, (const int*)val, M->d_clist, (int)n,
+                                             0, endBit, s));
+  hipLaunchKernelGGL(k_contrib_ptr, dim3(blocks_for((size_t)nnz + 1, 256)), dim3(256), 0, s, nnz, n,
+                     (const int*)sorted, M->d_cptr);
+  SVG_HIP(hipGetLastError());
+  // The gather's lane order: windows of rows in order and, inside a window,
+  // slots sorted (stably) by their entry count, so that the lanes of a
+  // wavefront loop over equally many entries (diagonal blocks have about four
+  // times the entries of the others) while the element records a window
+  // reads stay few enough for one XCD's L2.
+  if ((rc = dev_alloc(&M->d_sorder, (size_t)nnz))) return rc;
+  hipLaunchKernelGGL(k_slot_count, dim3(blocks_for((size_t)nnz, 256)), dim3(256), 0, s, nnz,
+                     (const int*)M->d_cptr, (const int*)M->d_clist, reinterpret_cast<const int4*>(M->d_ien0), key,
+                     val);
+  SVG_HIP(hipGetLastError());
+  const int endBit2 = bits_for((unsigned long long)(L->nNo / kGatherRows) * 256 + 255);
+  size_t tb2 = 0;
+  SVG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb2, (const int*)key, sorted, (const int*)val, M->d_sorder,
+                                             nnz, 0, endBit2, s));
+  unsigned char* tmp2 = tmp;
+  if (tb2 > tb && (rc = S.get(&tmp2, tb2))) return rc;
+  SVG_HIP(hipcub::DeviceRadixSort::SortPairs(tmp2, tb2, (const int*)key, sorted, (const int*)val, M->d_sorder, nnz,
+                                             0, endBit2, s));
+  int herr = 0;
+  SVG_HIP(hipMemcpyAsync(&herr, d_err, sizeof(int), hipMemcpyDeviceToHost, s));
+  SVG_HIP(hipStreamSynchronize(s));
+  if (herr & 4) return set_error(SVG_ERR_ARG, "slot_map: pattern does not contain an element pair");
+  return SVG_OK;
+}
 
 }  // namespace svg
 
@@ -388,6 +520,10 @@ int svg_msh_create_dev(const svg_lhs* L, int nEl, const int* d_IEN, const double
   M->colorOff.assign(nColors + 1, 0);
   for (int q = 0; q < nEl; q++) M->colorOff[hc[q] + 1]++;
   for (int c = 0; c < nColors; c++) M->colorOff[c + 1] += M->colorOff[c];
+  if ((rc = msh_gather_build(M, d_IEN, s))) {
+    svg_msh_free(M);
+    return rc;
+  }
   *out = M;
   return SVG_OK;
 }

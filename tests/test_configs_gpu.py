@@ -4,7 +4,8 @@
       ls.cpp:51-60, ns_solver.cpp:167-499) over 3 Newton iterations against the
       device-order oracle;
   C2  the 1.2M-tet pipe (n = 34): construct_fluid (+ the outlet's backflow
-      assembly) against the oracle to 1e-12; the plain FP64 GMRES(50) of the
+      assembly) against the oracle bit for bit (1e-12 with the outlet's face);
+      the plain FP64 GMRES(50) of the
       first Newton system (gmres.cpp:450-629) to the FP64 true residual, its
       iNorm against the oracle's scaled residual; the learned method (two-level
       preconditioner + network smoother, FP32 SpMV) on the same system;
@@ -83,15 +84,21 @@ def pipe34():
 
 
 def test_c2_assembly_matches_oracle(torch_cuda, pipe34):
-    """construct_fluid and the outlet's b_assem_neu_bc of the 1.2M-tet pipe's
-    first Newton iteration against the oracle: <= 1e-12 norm-wise and in the
-    max norm (the sums differ only in element order)."""
+    """construct_fluid of the 1.2M-tet pipe's first Newton iteration against
+    the oracle bit for bit, then with the outlet's b_assem_neu_bc: <= 1e-12
+    norm-wise and in the max norm (the face sums differ in face order)."""
+    from svfsiplus_amd.newton import construct_fluid
     torch = torch_cuda
     phys = W.Physics()
     fn, R1, V1 = _first_system(torch, pipe34, phys)
     Ao, Yo, An, Yn = W.dirichlet_state(pipe34, phys)
     Ag, Yg = oracle.pici(phys, Ao, An, Yo, Yn)
     R0, V0 = oracle.construct_fluid(pipe34, phys, Ag, Yg)
+    fn.R.zero_()
+    fn.Val.zero_()
+    construct_fluid(fn.msh, phys, fn.Ag, fn.Yg, fn.R, fn.Val)
+    torch.cuda.synchronize()
+    assert np.array_equal(fn.R.cpu().numpy(), R0) and np.array_equal(fn.Val.cpu().numpy(), V0)
     fi, ge, inner = W.outlet_faces(pipe34)
     oracle.b_assem_neu_fluid(pipe34, phys, Yg, fi, inner, R0, V0)
     R1, V1 = R1.cpu().numpy(), V1.cpu().numpy()

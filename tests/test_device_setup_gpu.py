@@ -80,10 +80,11 @@ def test_device_setup_assembly_matches_host(torch_cuda, phys):
         torch.cuda.synchronize()
         out.append((R.cpu().numpy(), V.cpu().numpy()))
     (R0, V0), (R1, V1) = out
-    eR = np.abs(R1 - R0).max() / np.abs(R0).max()
-    eV = np.abs(V1 - V0).max() / np.abs(V0).max()
-    print(f"device vs host setup: R {eR:.2e}, Val {eV:.2e} (max abs / max)")
-    assert eR <= 1e-13 and eV <= 1e-13
+    # the assembly adds element contributions in element order whatever the
+    # colouring, so both set-ups give the oracle's sums bit for bit
+    Rr, Vr = oracle.construct_fluid(p, phys, Ag, Yg)
+    assert np.array_equal(R0, Rr) and np.array_equal(V0, Vr)
+    assert np.array_equal(R1, R0) and np.array_equal(V1, V0)
 
 
 def test_device_setup_newton_step(torch_cuda, phys):
@@ -94,9 +95,9 @@ def test_device_setup_newton_step(torch_cuda, phys):
     fd = FluidNewton(p, phys, mk(), device_setup=True)
     h1 = fd.time_step()
     print("host setup  ", h0, "\ndevice setup", h1)
-    # the first residual differs by the assembly's rounding only; the later ones
-    # follow linear solves stopped at a 1e-6 relative residual
-    assert abs(h0[0]["iNorm"] - h1[0]["iNorm"]) <= 1e-12 * h0[0]["iNorm"]
+    # the first residual is the same (the assembly does not depend on the
+    # set-up); the later ones follow linear solves stopped at a 1e-6 relative residual
+    assert h0[0]["iNorm"] == h1[0]["iNorm"]
     for a, b in zip(h0, h1):
         assert abs(a["ls_itr"] - b["ls_itr"]) <= 2
         assert abs(a["iNorm"] - b["iNorm"]) <= 1e-6 * a["iNorm"]

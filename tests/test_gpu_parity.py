@@ -1,10 +1,9 @@
 """GPU parity of the HIP path against the CPU oracle (same seeded inputs).
 
-Bars: bit-exact for the block SpMV and the diagonal preconditioner (same
-arithmetic, same order); <= 1e-12 norm-wise relative for the assembled Val and
-R (only the summation order of element contributions differs); GMRES
-iteration counts within +-1 of the oracle's and solutions within the solver
-tolerance; Newton iteration records matching the oracle's.
+Bars: bit-exact for the block SpMV, the diagonal preconditioner and the
+assembled Val and R (same arithmetic, element contributions added in element
+order); GMRES iteration counts within +-1 of the oracle's and solutions within
+the solver tolerance; Newton iteration records matching the oracle's.
 """
 import numpy as np
 import pytest
@@ -112,12 +111,8 @@ def test_construct_fluid(torch_cuda, phys, n):
     construct_fluid(M, phys, _dev(torch, Ag), _dev(torch, Yg), dR, dV)
     torch.cuda.synchronize()
     R1, V1 = dR.cpu().numpy(), dV.cpu().numpy()
-    eR = np.linalg.norm(R1 - R0) / np.linalg.norm(R0)
-    eV = np.linalg.norm(V1 - V0) / np.linalg.norm(V0)
-    mR = np.abs(R1 - R0).max() / np.abs(R0).max()
-    mV = np.abs(V1 - V0).max() / np.abs(V0).max()
-    print(f"n={n} colours={M.nColors} |dR|/|R|={eR:.2e} |dV|/|V|={eV:.2e} max {mR:.2e} {mV:.2e}")
-    assert eR <= 1e-12 and eV <= 1e-12 and mR <= 1e-12 and mV <= 1e-12
+    print(f"n={n} R differs in {(R1 != R0).sum()} of {R0.size}, Val in {(V1 != V0).sum()} of {V0.size}")
+    assert np.array_equal(R1, R0) and np.array_equal(V1, V0)
 
 
 @pytest.mark.parametrize("n", [4, 8])
@@ -270,9 +265,8 @@ def test_gpu_against_reference_golden(torch_cuda, name):
     dV = torch.zeros((p.nnz, 16), dtype=torch.float64, device="cuda")
     construct_fluid(M, ph, _dev(torch, d["Ag_p"]), _dev(torch, d["Yg_p"]), dR, dV)
     torch.cuda.synchronize()
-    for got, ref in ((dR.cpu().numpy(), d["R_p"]), (dV.cpu().numpy(), d["Val_p"])):
-        assert np.linalg.norm(got - ref) <= 1e-12 * np.linalg.norm(ref)
-        assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+    # the reference's own construct_fluid output, bit for bit
+    assert np.array_equal(dR.cpu().numpy(), d["R_p"]) and np.array_equal(dV.cpu().numpy(), d["Val_p"])
     tol = float(d["gm_tol"])
     ls = ls_create(_lib.LS_GMRES, relTol=tol, absTol=1e-30, maxItr=1000, dimKry=50)
     dR, dV = _dev(torch, d["R0"]), _dev(torch, d["Val0"])
