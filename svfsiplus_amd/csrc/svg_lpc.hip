@@ -228,13 +228,22 @@ __global__ __launch_bounds__(BS) void k_encode(int n, int rows, const double* __
 
 typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
 
+// Lane k of each group of four lanes (quad) broadcast to the quad (DPP quad_perm).
+template <int K>
+__device__ __forceinline__ unsigned quad_bcast(unsigned v)
+{
+  return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, K * 0x55, 0xF, 0xF, false);
+}
+
 // m[c][d] += sum_q A[d][q] h[c][q] for one block: the block's row d is the
 // packed BF16 pairs (a[2d], a[2d+1]), channel c of h the pairs (h[2c], h[2c+1]);
 // two packed dot products (FP32 accumulation) per output, q = 0,1 then 2,3.
-__device__ __forceinline__ void msg_block(float (&m)[4][4], const uint4& a0, const uint4& a1, const uint4& h0,
-                                          const uint4& h1)
+// The four lanes of a node row (a quad) each load one block row, aq = row
+// (lane & 3), and exchange them by DPP, so the block is read once per row.
+__device__ __forceinline__ void msg_block(float (&m)[4][4], const uint2& aq, const uint4& h0, const uint4& h1)
 {
-  const unsigned A[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+  const unsigned A[8] = {quad_bcast<0>(aq.x), quad_bcast<0>(aq.y), quad_bcast<1>(aq.x), quad_bcast<1>(aq.y),
+                         quad_bcast<2>(aq.x), quad_bcast<2>(aq.y), quad_bcast<3>(aq.x), quad_bcast<3>(aq.y)};
   const unsigned H[8] = {h0.x, h0.y, h0.z, h0.w, h1.x, h1.y, h1.z, h1.w};
 #pragma unroll
   for (int c = 0; c < 4; c++)
@@ -249,9 +258,11 @@ __device__ __forceinline__ void msg_block(float (&m)[4][4], const uint4& a0, con
 
 // The messages of node i for this lane's 4 channels (16 outputs): m = sum over
 // the row's blocks k, in CSR order, of A_k (x) h[colPtr[k]], the next UNR
-// blocks' column indices loaded while the current ones are consumed.
+// blocks' column indices loaded while the current ones are consumed. The four
+// lanes of the row (g = lane & 3) run this together: they must form one quad
+// and all take part (DPP exchange of the block rows).
 template <int UNR>
-__device__ __forceinline__ void gather_row(float (&m)[4][4], int s, int e, const int* __restrict__ colPtr,
+__device__ __forceinline__ void gather_row(float (&m)[4][4], int s, int e, int g, const int* __restrict__ colPtr,
                                            const bf16* __restrict__ A16, const bf16* __restrict__ Hb)
 {
 #pragma unroll
@@ -266,26 +277,25 @@ __device__ __forceinline__ void gather_row(float (&m)[4][4], int s, int e, const
     int j[UNR];
 #pragma unroll
     for (int q = 0; q < UNR; q++) j[q] = jn[q];
-    uint4 a0[UNR], a1[UNR], h0[UNR], h1[UNR];
+    uint2 aq[UNR];
+    uint4 h0[UNR], h1[UNR];
 #pragma unroll
     for (int q = 0; q < UNR; q++) {
-      a0[q] = reinterpret_cast<const uint4*>(A16 + 16 * (size_t)(k + q))[0];
-      a1[q] = reinterpret_cast<const uint4*>(A16 + 16 * (size_t)(k + q))[1];
+      aq[q] = reinterpret_cast<const uint2*>(A16 + 16 * (size_t)(k + q))[g];
       h0[q] = reinterpret_cast<const uint4*>(Hb + (size_t)j[q] * F)[0];
       h1[q] = reinterpret_cast<const uint4*>(Hb + (size_t)j[q] * F)[1];
     }
 #pragma unroll
     for (int q = 0; q < UNR; q++) jn[q] = k + UNR + q < e ? colPtr[k + UNR + q] : 0;
 #pragma unroll
-    for (int q = 0; q < UNR; q++) msg_block(m, a0[q], a1[q], h0[q], h1[q]);
+    for (int q = 0; q < UNR; q++) msg_block(m, aq[q], h0[q], h1[q]);
   }
   for (int q = 0; k < e; k++, q++) {
     const int jj = jn[q];
-    const uint4 a0 = reinterpret_cast<const uint4*>(A16 + 16 * (size_t)k)[0];
-    const uint4 a1 = reinterpret_cast<const uint4*>(A16 + 16 * (size_t)k)[1];
+    const uint2 aq = reinterpret_cast<const uint2*>(A16 + 16 * (size_t)k)[g];
     const uint4 h0 = reinterpret_cast<const uint4*>(Hb + (size_t)jj * F)[0];
     const uint4 h1 = reinterpret_cast<const uint4*>(Hb + (size_t)jj * F)[1];
-    msg_block(m, a0, a1, h0, h1);
+    msg_block(m, aq, h0, h1);
   }
 }
 
@@ -325,7 +335,7 @@ __global__ __launch_bounds__(256) void k_msg(int n, int rows, const int* __restr
   const int b = r / n, i = r - b * n;
   if (gate.sysOf && !gate.live(i)) return;
   float m[4][4];
-  gather_row<UNR>(m, rowPtr[i], rowPtr[i + 1], colPtr, A16, H + (size_t)b * n * F + 16 * g);
+  gather_row<UNR>(m, rowPtr[i], rowPtr[i + 1], g, colPtr, A16, H + (size_t)b * n * F + 16 * g);
   uint4 o0, o1;
   pack_msg(m, o0, o1);
   uint4* out = reinterpret_cast<uint4*>(M + (size_t)r * F + 16 * g);
@@ -472,7 +482,7 @@ __global__ __launch_bounds__(BS) void k_layer(int n, int rows, const int* __rest
       h1 = reinterpret_cast<const uint4*>(Hin + (size_t)r * F + 16 * g)[1];
       if (!gate.sysOf || gate.live(i)) {
         float m[4][4];
-        gather_row<UNR>(m, rowPtr[i], rowPtr[i + 1], colPtr, A16, Hin + (size_t)b * n * F + 16 * g);
+        gather_row<UNR>(m, rowPtr[i], rowPtr[i + 1], g, colPtr, A16, Hin + (size_t)b * n * F + 16 * g);
         pack_msg(m, m0, m1);
       }
     }
