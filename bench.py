@@ -415,6 +415,7 @@ def configure(fn, method, a, net=None):
     from svfsiplus_amd import _lib
     fn.precision = _lib.PRECISION_FP64 if method == "plain" else _lib.PRECISION_MIXED
     fn.coarse = a.coarse if method in ("twolevel", "learned") else 0
+    fn.coarse_keep = bool(a.coarse_keep) and fn.coarse > 0
     fn.lpc = net if method == "learned" else None
     fn.net_period = a.net_period
 
@@ -506,6 +507,7 @@ def batch_run(a, log):
         bn.reset()
         bn.time_step(3)
     torch.cuda.synchronize()
+    r0 = bn.replays()
     t1 = time.perf_counter()
     recs = []
     for _ in range(a.batch_steps):
@@ -513,6 +515,7 @@ def batch_run(a, log):
         recs.append(bn.time_step(3))
     torch.cuda.synchronize()
     el = time.perf_counter() - t1
+    syncs = (bn.replays() - r0) / a.batch_steps
     newton = len(meshes) * 3 * a.batch_steps
     itr = [[r["itr"] for r in it] for it in recs[-1]]
     ok = all(r["suc"] for it in recs[-1] for r in it)
@@ -525,9 +528,12 @@ def batch_run(a, log):
            "steps": a.batch_steps, "warmup": a.batch_warmup,
            "precision": a.batch_precision, "method": a.batch_method, "all_converged": ok,
            "linear_iterations": {"max": [max(x) for x in itr], "mean": [round(float(np.mean(x)), 1) for x in itr]},
-           "solver": "batched %sGMRES(%d), per-mesh Krylov spaces and convergence%s; each Newton iteration is three "
-                     "replayed hipGraphs (pici + assembly + preconditioner, the restart cycle, the corrector); "
-                     "block SpMV variant from the autotuner" % (
+           "restart_cycles_max": [int(np.ceil(max(x) / a.krylov)) for x in itr],
+           "host_syncs_per_time_step": syncs,
+           "solver": "batched %sGMRES(%d), per-mesh Krylov spaces and convergence%s; each Newton iteration is "
+                     "replayed hipGraphs (pici + assembly + preconditioner; the restart cycles, four per replay while "
+                     "more than an eighth of the meshes iterate, then one; the corrector), the host reading the "
+                     "meshes' states once per replay; block SpMV variant from the autotuner" % (
                          "flexible " if (net or coarse) else "", a.krylov,
                          (", two-level preconditioner (aggregates of about %d nodes, one dense coarse block per mesh "
                           "inverted inside the captured graph; block-Jacobi smoother; per-mesh fallback to the "
@@ -583,6 +589,9 @@ def main():
     ap.add_argument("--coarse", type=int, default=128,
                     help="two-level preconditioner of twolevel/learned: node aggregates of about this many nodes "
                          "(0: learned is the stand-alone network after the diagonal scaling)")
+    ap.add_argument("--coarse-keep", type=int, default=1,
+                    help="1: Newton iterations 2.. of a time step keep the coarse inverse of the first "
+                         "(A P and block Jacobi rebuilt every iteration); 0: rebuild it every iteration")
     ap.add_argument("--net-period", type=int, default=5,
                     help="learned: the network preconditions every net-period-th Arnoldi step, the others the "
                          "two-level preconditioner with block Jacobi (or, with --coarse 0, the diagonal scaling)")
@@ -677,7 +686,9 @@ def main():
         cpu = cpu_baseline(pipe, phys, [x for h in base["hists"] for x in h])
 
     tl_text = ("two-level preconditioner (pressure aggregation coarse level, aggregates of about %d nodes, dense "
-               "Galerkin coarse matrix inverted on the device; block-Jacobi smoother)" % a.coarse)
+               "Galerkin coarse matrix inverted on the device%s; block-Jacobi smoother)"
+               % (a.coarse, " in the first Newton iteration of the time step and kept for the others"
+                  if a.coarse_keep else " every Newton iteration"))
     solver = {"plain": "FSILS GMRES (gmres_v), diagonal preconditioner, FP64",
               "mixed": "GMRES with FP32 block SpMV, FP64 Gram-Schmidt/Givens, FP64 true-residual restarts, "
                        "diagonal preconditioner",

@@ -132,8 +132,10 @@ int svg_lhs_pattern(const svg_lhs* lhs, int* rowPtr, int* colPtr); /* host copie
 /* with constant viscosity, one fluid domain, fixed mesh (mvMsh false).       */
 typedef struct svg_msh svg_msh;
 
-/* Uploads x and IEN, colours the elements and builds the element -> block
- * slot map over `lhs`'s pattern (the binary search of do_assem, lhsa.cpp:123-134). */
+/* Uploads x and IEN, colours the elements, builds the element -> block slot
+ * map over `lhs`'s pattern (the binary search of do_assem, lhsa.cpp:123-134)
+ * and, for svg_construct_fluid, its inverse: per block slot the element
+ * entries (e, a, b) that add to it, in element order. */
 int svg_msh_create(const svg_lhs* lhs, int nEl, const int* IEN, int nNo, const double* x, svg_msh** msh);
 /* The same from device-resident IEN (4, nEl) and x (3, nNo), every step on
  * the device: Jones-Plassmann colouring (the sequential greedy colouring in
@@ -158,7 +160,10 @@ typedef struct {
 } svg_fluid_params;
 
 /* R(4,nNo) += element residuals, Val(16,nnz) += element tangents (the
- * caller zeroes them first, as ls_alloc does). d_Bf may be NULL (zero). */
+ * caller zeroes them first, as ls_alloc does). d_Bf may be NULL (zero).
+ * Each block and residual row receives its element contributions in element
+ * order with every operation rounded separately, as the reference's element
+ * loop adds them: the result equals the reference's bit for bit. */
 int svg_construct_fluid(svg_msh* msh, const svg_fluid_params* p, const double* d_Ag, const double* d_Yg,
                         const double* d_Bf, double* d_R, double* d_Val, void* stream);
 
@@ -277,7 +282,10 @@ typedef struct {
                        GMRES), with node aggregates of about `coarse` nodes (0: none); method PLAIN
                        smooths with block Jacobi, method LEARNED with the network `lpc` on its steps
                        (lpc may then be NULL: block Jacobi) */
-  int reserved[2];
+  int coarse_keep;  /* with coarse > 0: 1 keeps the coarse inverse of this lhs's last two-level set-up
+                       (only A P and the block-Jacobi inverses are rebuilt from Val), e.g. for the Newton
+                       iterations after the first of a time step; 0 rebuilds it */
+  int reserved;
   svg_lpc* lpc;
 } svg_solve_opts;
 
@@ -315,8 +323,13 @@ int svg_coarse_info(const svg_lhs* lhs, int* nAgg, int* agg);
 /* [nodeOff[q], nodeOff[q+1])) are solved together by gmres_v's algorithm    */
 /* (gmres.cpp:450-629) with per-system Krylov spaces, dot products, Givens    */
 /* rotations and convergence; each system's arithmetic is that of a solve of  */
-/* the system alone. One restart cycle of all systems is captured into a      */
-/* hipGraph and replayed until every system has converged or mItr cycles ran. */
+/* the system alone. The restart cycle of all systems is captured into two   */
+/* hipGraphs, one cycle and four cycles per replay (systems that stop inside  */
+/* a replay are skipped by every kernel, and mItr is enforced per system on   */
+/* the device); the four-cycle graph is replayed while more than an eighth  */
+/* of the systems iterate, the one-cycle graph after that, until every system */
+/* has converged or ran mItr cycles. The host reads the systems' states once  */
+/* per replay (svg_batch_replays counts the replays).                         */
 /* d_R, d_Val: the union's (already diagonally scaled) right-hand side and    */
 /* matrix; d_R receives the per-system solutions; out[nsys] the records.      */
 /* precision SVG_PRECISION_MIXED: FP32 SpMV, FP64 true-residual restarts.     */
@@ -355,6 +368,8 @@ int svg_batch_gmres_ex(svg_batch* batch, const svg_subls* proto, double* d_R, co
 int svg_batch_fsils_solve_ex(svg_batch* batch, const svg_subls* proto, double* d_R, double* d_Val,
                              const svg_solve_opts* opts, svg_subls* out, void* stream);
 int svg_batch_fallback_itr(const svg_batch* batch, int* itr);
+/* Restart-cycle graph replays (one host synchronisation each) since the batch was created. */
+int svg_batch_replays(const svg_batch* batch, long long* replays);
 
 /* ------------------------------------------------------------------------ */
 /* Kernels of the solve, exposed for unit parity tests.                      */

@@ -58,7 +58,7 @@ class SubLs(C.Structure):
 
 class SolveOpts(C.Structure):
     _fields_ = [("precision", C.c_int), ("method", C.c_int), ("fallback_window", C.c_int),
-                ("fallback_itr", C.c_int), ("net_period", C.c_int), ("coarse", C.c_int), ("reserved", C.c_int * 2),
+                ("fallback_itr", C.c_int), ("net_period", C.c_int), ("coarse", C.c_int), ("coarse_keep", C.c_int), ("reserved", C.c_int),
                 ("lpc", C.c_void_p)]
 
 
@@ -140,6 +140,7 @@ _SIGS = {
     "svg_batch_gmres_ex": (_I, [_P, C.POINTER(SubLs), _P, _P, C.POINTER(SolveOpts), C.POINTER(SubLs), _P]),
     "svg_batch_fsils_solve_ex": (_I, [_P, C.POINTER(SubLs), _P, _P, C.POINTER(SolveOpts), C.POINTER(SubLs), _P]),
     "svg_batch_fallback_itr": (_I, [_P, _P]),
+    "svg_batch_replays": (_I, [_P, _P]),
     "svg_lpc_num_params": (_I, [_IP]),
     "svg_lpc_create": (_I, [C.POINTER(_P)]),
     "svg_lpc_free": (_I, [_P]),

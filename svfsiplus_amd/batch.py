@@ -108,6 +108,12 @@ class BatchNewton:
                 r["fallback_itr"] = int(f)
         return recs
 
+    def replays(self) -> int:
+        """Restart-cycle graph replays (host synchronisations) since the batch was created."""
+        n = C.c_longlong()
+        _lib.check(_lib.load().svg_batch_replays(self.h, C.byref(n)))
+        return n.value
+
     def time_step(self, newton_iterations: int = 3, graph: bool = True):
         return [self.newton_iteration(graph) for _ in range(newton_iterations)]
 

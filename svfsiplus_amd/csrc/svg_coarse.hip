@@ -67,6 +67,7 @@ struct CoarseLevel {
   int* h_flag = nullptr;
   // differs between builds (a graph holding this level's launches is keyed on it)
   unsigned long long gen = 0;
+  bool inverted = false;  // d_Ai holds the inverse of a system's Ac (a keep set-up may reuse it)
 };
 
 static std::atomic<unsigned long long> g_coarse_gen{0};
@@ -670,7 +671,7 @@ int coarse_setup(svg_lhs* L, int aggSize, const int* sysNodeOff, int nsys)
 
 // This system's A P, block-Jacobi inverses, Ac and its inverse from the scaled
 // Val: launches only (no host synchronisation, no allocation; capturable).
-void coarse_enqueue(svg_lhs* L, const double* Val, hipStream_t s)
+void coarse_enqueue(svg_lhs* L, const double* Val, hipStream_t s, bool invert)
 {
   CoarseLevel* c = L->coarse;
   const int nNo = L->nNo, nblk = c->nblk, nb = c->maxN / GJB;
@@ -678,6 +679,8 @@ void coarse_enqueue(svg_lhs* L, const double* Val, hipStream_t s)
          (const int*)c->d_apPtr, (const int*)c->d_apOf, Val, c->d_apVal, c->d_apPP);
   launch("cl_dinv", k_cl_dinv, dim3(blocks_for(nNo, 256)), dim3(256), 0, s, nNo, (const int*)L->d_diagPtr, Val,
          c->d_dinv);
+  if (!invert) return;
+  c->inverted = true;
   launch("cl_acinit", k_cl_acinit, dim3(blocks_for((size_t)c->maxN * c->maxN, 256), nblk), dim3(256), 0, s,
          (const int*)c->d_aggOff, (const int*)c->d_blkN, (const long long*)c->d_blkOff, c->d_Ac, c->d_flag);
   launch("cl_ac", k_cl_ac, dim3(blocks_for(c->nSeg, 256)), dim3(256), 0, s, c->nSeg, (const int*)c->d_segPtr,
@@ -708,12 +711,13 @@ int coarse_check(const svg_lhs* L, hipStream_t s)
   return SVG_OK;
 }
 
-int coarse_prepare_dev(svg_lhs* L, const double* Val, int aggSize, hipStream_t s)
+int coarse_prepare_dev(svg_lhs* L, const double* Val, int aggSize, hipStream_t s, bool keep)
 {
   int rc = coarse_setup(L, aggSize, nullptr, 0);
   if (rc) return rc;
-  coarse_enqueue(L, Val, s);
-  return coarse_check(L, s);
+  const bool invert = !(keep && L->coarse->inverted);
+  coarse_enqueue(L, Val, s, invert);
+  return invert ? coarse_check(L, s) : SVG_OK;
 }
 
 template <typename T>

@@ -97,11 +97,14 @@ struct GmState {
   int fallback; // learned preconditioner: no new residual minimum for the fallback window
   int nondec;   // iterations since the last new minimum of |err|
   int act;      // batched solve: the system still iterates (not converged before this cycle)
+  int cyc;      // batched solve: restart cycles the system ran
+  int fbitr;    // batched solve: itr after the cycle in which the system fell back (-1: none)
   int pad;
   double best;  // smallest |err(i+1)| of the solve
   double eps;
   double scale;  // err(0) after a restart, 1 / h(i+1,i) after an Arnoldi step
   double iNorm, fNorm;
+  double fStart;  // batched solve: fNorm at the start of the system's last cycle
 };
 
 // Device state of the Schur-complement CG (cgrad.cpp:50-160). errO is double
@@ -169,11 +172,13 @@ struct CoarseLevel;
 void coarse_free(CoarseLevel*& c);
 // Builds the coarse level of L for aggregates of about aggSize nodes (once per
 // lhs and size), then this system's operators from the scaled Val (synchronises s).
-int coarse_prepare_dev(svg_lhs* L, const double* Val, int aggSize, hipStream_t s);
+// keep: reuse the coarse inverse of the lhs's last set-up when there is one
+// (A P and the block-Jacobi inverses are rebuilt from Val either way).
+int coarse_prepare_dev(svg_lhs* L, const double* Val, int aggSize, hipStream_t s, bool keep = false);
 // The same in three parts: the build (host; one dense coarse block per system of
 // sysNodeOff, null: one system), the launches alone (capturable) and the pivot check.
 int coarse_setup(svg_lhs* L, int aggSize, const int* sysNodeOff, int nsys);
-void coarse_enqueue(svg_lhs* L, const double* Val, hipStream_t s);
+void coarse_enqueue(svg_lhs* L, const double* Val, hipStream_t s, bool invert = true);
 int coarse_check(const svg_lhs* L, hipStream_t s);
 // z = M(v) in FP64 (z64) and/or FP32 (z32); skip: kernels return at once when *skip.
 void coarse_apply_dev(const svg_lhs* L, const float* v, double* z64, float* z32, const int* skip, hipStream_t s);
@@ -268,8 +273,10 @@ struct svg_batch {
   double2* d_nsc = nullptr;               // per node: (sqrt(nNo_q), 1 / sqrt(nNo_q)) of its system q
   std::vector<int> fbItr;                 // per system: iteration count at the fallback (-1: none)
   hipStream_t cs_stream = nullptr;        // capture/replay stream
-  hipGraphExec_t graph[2] = {nullptr, nullptr};  // restart cycle: FP64 / mixed
-  std::vector<unsigned char> gKey[2];            // the arguments each cycle graph was captured with
+  // restart cycles: [FP64 / mixed][one cycle / kBatchCycles cycles per replay]
+  hipGraphExec_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+  std::vector<unsigned char> gKey[2][2];  // the arguments each cycle graph was captured with
+  long long replays = 0;                  // restart-cycle graph replays (host synchronisations) so far
   hipGraphExec_t pro = nullptr, epi = nullptr;   // Newton iteration: pici..initial norms / result..picc
   std::vector<unsigned char> newtonKey;          // the arguments pro and epi were captured with
   hipEvent_t ev = nullptr;

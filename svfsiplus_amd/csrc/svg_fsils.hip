@@ -587,6 +587,9 @@ __device__ __forceinline__ void fin_body(const FinArgs& A, const double* __restr
       st->nondec = 0;
       st->best = DBL_MAX;
       st->act = nrm > A.absTol ? 1 : 0;
+      st->cyc = 0;
+      st->fbitr = -1;
+      st->fStart = nrm;
     }
     return;
   }
@@ -1705,7 +1708,7 @@ static int gmres_v_mixed(svg_lhs* L, svg_subls* ls, const double* Val, double* R
 // solve goes on from the current X with the plain (diagonal-only) GMRES
 // restarts of the same precision.
 static int gmres_v_learned(svg_lhs* L, svg_subls* ls, const double* Val, double* R, bool mixed, svg_lpc* net,
-                           int window, int net_period, int coarse, int* fallback_itr, hipStream_t s)
+                           int window, int net_period, int coarse, bool coarse_keep, int* fallback_itr, hipStream_t s)
 {
   if (ls->sD < 1 || ls->sD > 250) return set_error(SVG_ERR_ARG, "gmres_v: Krylov dimension must be in [1, 250]");
   auto& k = L->kw;
@@ -1727,7 +1730,7 @@ static int gmres_v_learned(svg_lhs* L, svg_subls* ls, const double* Val, double*
   if (net && (rc = lpc_reserve(net, L, s))) return rc;
   auto t0 = std::chrono::steady_clock::now();
   if (net && (rc = lpc_prepare_dev(L, Val, s))) return rc;
-  if (coarse > 0 && (rc = coarse_prepare_dev(L, Val, coarse, s))) return rc;
+  if (coarse > 0 && (rc = coarse_prepare_dev(L, Val, coarse, s, coarse_keep))) return rc;
   if (mixed) {
     launch("d2f", k_d2f, dim3(8192), dim3(256), 0, s, (size_t)16 * L->nnz, Val, L->d_val32);
     if (L->spmv32Variant < 0 && (rc = spmv32_tune(L, L->d_val32, k.u32, k.X, s))) return rc;
@@ -2449,13 +2452,33 @@ __global__ __launch_bounds__(256) void k_bxupdate_cb(int n, double* __restrict__
 
 // After a cycle: systems that converged stop iterating (plain: the Givens
 // estimate, gmres.cpp:585-589; mixed: the FP64 true residual at the restart).
-__global__ void k_bclose(int nsys, GmState* st, int tcheck)
+// End of a restart cycle: a system that converged, hit a zero restart norm or
+// ran mItr cycles stops; the cycle count and the iteration count after the
+// cycle in which the system fell back are kept in its state (the host reads
+// the states only between replays, which may hold several cycles).
+__global__ void k_bclose(int nsys, GmState* st, int tcheck, int mItr)
 {
   const int sy = blockIdx.x * blockDim.x + threadIdx.x;
   if (sy >= nsys) return;
   GmState& S = st[sy];
+  const bool ran = S.act != 0;
   if (S.act && (tcheck ? S.tconv : S.suc)) S.act = 0;
   if (S.divzero) S.act = 0;
+  if (ran) {
+    S.cyc = S.cyc + 1;
+    if (S.fallback && S.fbitr < 0) S.fbitr = S.itr;
+    if (S.cyc >= mItr) S.act = 0;
+  }
+}
+
+// Start of a restart cycle: the residual norm the cycle starts from (dB of
+// the records is taken against the last cycle's).
+__global__ void k_bcycle_start(int nsys, GmState* st)
+{
+  const int sy = blockIdx.x * blockDim.x + threadIdx.x;
+  if (sy >= nsys) return;
+  GmState& S = st[sy];
+  if (S.act) S.fStart = S.fNorm;
 }
 
 // R = X for the systems that iterated; the others keep R (gmres.cpp:489-493).
@@ -2472,8 +2495,9 @@ __global__ __launch_bounds__(256) void k_bresult(int n, double* __restrict__ R, 
 
 static void batch_graph_free(svg_batch* B)
 {
-  for (auto& g : B->graph)
-    if (g) (void)hipGraphExecDestroy(g), g = nullptr;
+  for (auto& gg : B->graph)
+    for (auto& g : gg)
+      if (g) (void)hipGraphExecDestroy(g), g = nullptr;
   for (hipGraphExec_t* g : {&B->pro, &B->epi})
     if (*g) (void)hipGraphExecDestroy(*g), *g = nullptr;
   B->newtonKey.clear();
@@ -2543,7 +2567,7 @@ __global__ __launch_bounds__(256) void k_bzcopy(int n, const double* __restrict_
 // restart loop, gmres.cpp:529-612; flexible with the learned preconditioner as
 // the single-system learned solve), enqueued on s (captured into a graph).
 static int batch_cycle(svg_batch* B, const double* Val, const double* R, const BatchMode& md, const FinArgs& base,
-                       hipStream_t s)
+                       int mItr, hipStream_t s)
 {
   svg_lhs* L = B->L;
   const int n = B->n, sD = B->sD;
@@ -2551,6 +2575,7 @@ static int batch_cycle(svg_batch* B, const double* Val, const double* R, const B
   const Op A{L, Val, 4};
   const bool mixed = md.mixed, flex = md.flex(), tl = md.coarse > 0;
   FinArgs fa = base;
+  hipLaunchKernelGGL(k_bcycle_start, dim3(blocks_for(B->nsys, 256)), dim3(256), 0, s, B->nsys, B->st);
   op_apply(A, B->X, B->u, R, nullptr, s);  // u0 = R - A X
   batch_mdot(B, B->u, B->u, 0, 1, 1, s);
   fa.mode = FIN_RESTART;
@@ -2632,7 +2657,7 @@ static int batch_cycle(svg_batch* B, const double* Val, const double* R, const B
     hipLaunchKernelGGL(k_bxupdate, dim3(nbv), dim3(256), 0, s, n, B->X, (const double*)(flex ? B->z : B->u), B->ld,
                        sD, (const double*)B->y, (const GmState*)B->st, (const int*)B->d_sysOfNode);
   hipLaunchKernelGGL(k_bclose, dim3(blocks_for(B->nsys, 256)), dim3(256), 0, s, B->nsys, B->st,
-                     md.tcheck() ? 1 : 0);
+                     md.tcheck() ? 1 : 0, mItr);
   SVG_HIP(hipGetLastError());
   return SVG_OK;
 }
@@ -2762,24 +2787,44 @@ static int batch_begin(svg_batch* B, const double* R, const double* Val, const B
   return SVG_OK;
 }
 
-// The restart cycle as one graph (R, Val, the tolerances and the form are
-// baked in); graph[0] serves the FP64 forms, graph[1] the mixed ones.
-static int batch_cycle_graph(svg_batch* B, const double* R, const double* Val, const BatchMode& md, const FinArgs& fa)
+// Restart cycles per replay of the many-cycle graph: the host reads the
+// systems' states once per replay. Cycles of systems that have stopped cost
+// only their launches (every kernel skips them), so the one-cycle graph takes
+// over once no more than an eighth of the systems iterate (on the 256-mesh
+// batch: 14 replays per time step instead of 42 cycles, at the same speed;
+// four cycles per replay to the end: 12 replays, 2 % slower).
+constexpr int kBatchCycles = 4;
+
+// The restart cycle as graphs (R, Val, the tolerances, mItr and the form are
+// baked in): graph[p][0] one cycle, graph[p][1] kBatchCycles cycles; p = 0
+// serves the FP64 forms, p = 1 the mixed ones.
+static int batch_cycle_graph(svg_batch* B, const double* R, const double* Val, const BatchMode& md, const FinArgs& fa,
+                             int mItr)
 {
   const int gi = md.mixed ? 1 : 0;
   GraphKey k;
+  k.add(mItr);
   k.add(R).add(Val).add(fa.absTol).add(fa.relTol).add(fa.fbwin).add(B->L->d_val32).add(B->u32).add(B->v32);
   k.add(B->L->spmvVariant).add(B->L->spmv32Variant).add(md.net).add(md.period).add(B->z).add(B->d_nsc);
   // the network's activation buffers are baked into the captured forward pass
   k.add(lpc_generation(md.net)).add(B->L->msgVariant);
   // the coarse level's buffers are baked in too
   k.add(md.coarse).add(coarse_generation(B->L)).add(B->z32).add(md.window);
-  if (B->graph[gi] && B->gKey[gi] == k.b) return SVG_OK;
-  if (B->graph[gi]) (void)hipGraphExecDestroy(B->graph[gi]), B->graph[gi] = nullptr;
   hipStream_t s = B->cs_stream;
-  int rc = capture(s, &B->graph[gi], [&]() -> int { return batch_cycle(B, Val, R, md, fa, s); });
-  if (rc) return rc;
-  B->gKey[gi] = k.b;
+  for (int m = 0; m < 2; m++) {
+    if (B->graph[gi][m] && B->gKey[gi][m] == k.b) continue;
+    if (B->graph[gi][m]) (void)hipGraphExecDestroy(B->graph[gi][m]), B->graph[gi][m] = nullptr;
+    const int ncyc = m ? kBatchCycles : 1;
+    int rc = capture(s, &B->graph[gi][m], [&]() -> int {
+      for (int c = 0; c < ncyc; c++) {
+        int r = batch_cycle(B, Val, R, md, fa, mItr, s);
+        if (r) return r;
+      }
+      return SVG_OK;
+    });
+    if (rc) return rc;
+    B->gKey[gi][m] = k.b;
+  }
   return SVG_OK;
 }
 
@@ -2790,24 +2835,27 @@ static int batch_cycle_graph(svg_batch* B, const double* R, const double* Val, c
 static int batch_iterate(svg_batch* B, const BatchMode& md, int mItr, std::vector<double>& dB)
 {
   hipStream_t s = B->cs_stream;
+  const int gi = md.mixed ? 1 : 0;
   SVG_HIP(hipMemcpyAsync(B->h_st, B->st, sizeof(GmState) * B->nsys, hipMemcpyDeviceToHost, s));
   SVG_HIP(hipStreamSynchronize(s));
-  dB.assign(B->nsys, 0.0);
-  B->fbItr.assign(B->nsys, -1);
-  for (int q = 0; q < B->nsys; q++) dB[q] = B->h_st[q].fNorm;
+  // every replay runs at least one cycle of each system still iterating, and a
+  // system stops after mItr cycles (k_bclose), so mItr replays suffice
   for (int l = 0; l < mItr; l++) {
     int active = 0;
     for (int q = 0; q < B->nsys; q++) active += B->h_st[q].act;
     if (!active) break;
-    for (int q = 0; q < B->nsys; q++)
-      if (B->h_st[q].act) dB[q] = B->h_st[q].fNorm;
-    SVG_HIP(hipGraphLaunch(B->graph[md.mixed ? 1 : 0], s));
+    SVG_HIP(hipGraphLaunch(B->graph[gi][8 * active > B->nsys ? 1 : 0], s));
     SVG_HIP(hipMemcpyAsync(B->h_st, B->st, sizeof(GmState) * B->nsys, hipMemcpyDeviceToHost, s));
     SVG_HIP(hipStreamSynchronize(s));
-    for (int q = 0; q < B->nsys; q++) {
+    B->replays++;
+    for (int q = 0; q < B->nsys; q++)
       if (B->h_st[q].divzero) return set_error(SVG_ERR_ARG, "Array Divide by zero.");
-      if (md.flex() && B->h_st[q].fallback && B->fbItr[q] < 0) B->fbItr[q] = B->h_st[q].itr;
-    }
+  }
+  dB.assign(B->nsys, 0.0);
+  B->fbItr.assign(B->nsys, -1);
+  for (int q = 0; q < B->nsys; q++) {
+    dB[q] = B->h_st[q].fStart;
+    if (md.flex()) B->fbItr[q] = B->h_st[q].fbitr;
   }
   return SVG_OK;
 }
@@ -3083,7 +3131,7 @@ int svg_fsils_solve_ex(svg_lhs* L, svg_ls* ls, int dof, double* Ri, double* Val,
       if (flex)
         rc = gmres_v_learned(L, &ls->RI, Val, Ri, opts->precision == SVG_PRECISION_MIXED, learned ? opts->lpc : nullptr,
                              opts->fallback_window > 0 ? opts->fallback_window : 20, opts->net_period,
-                             opts->coarse, &opts->fallback_itr, s);
+                             opts->coarse, opts->coarse_keep != 0, &opts->fallback_itr, s);
       else
         rc = (opts && opts->precision == SVG_PRECISION_MIXED) ? gmres_v_mixed(L, &ls->RI, Val, Ri, s)
                                                                : gmres_v_dev(L, &ls->RI, dof, Val, Ri, s);
@@ -3200,7 +3248,8 @@ int svg_batch_gmres_ex(svg_batch* B, const svg_subls* proto, double* R, const do
   SVG_HIP(hipEventRecord(B->ev, caller));
   SVG_HIP(hipStreamWaitEvent(s, B->ev, 0));
   const FinArgs fa = batch_fin_args(proto, B->sD, md);
-  if ((rc = batch_begin(B, R, Val, md, fa, s)) || (rc = batch_cycle_graph(B, R, Val, md, fa))) return rc;
+  if ((rc = batch_begin(B, R, Val, md, fa, s)) || (rc = batch_cycle_graph(B, R, Val, md, fa, proto->mItr)))
+    return rc;
   std::vector<double> dB;
   if ((rc = batch_iterate(B, md, proto->mItr, dB))) return rc;
   if (md.coarse && (rc = coarse_check(B->L, s))) return rc;
@@ -3256,6 +3305,13 @@ int svg_batch_fallback_itr(const svg_batch* B, int* itr)
 {
   if (!B || !itr) return set_error(SVG_ERR_ARG, "svg_batch_fallback_itr: null argument");
   for (int q = 0; q < B->nsys; q++) itr[q] = q < (int)B->fbItr.size() ? B->fbItr[q] : -1;
+  return SVG_OK;
+}
+
+int svg_batch_replays(const svg_batch* B, long long* replays)
+{
+  if (!B || !replays) return set_error(SVG_ERR_ARG, "svg_batch_replays: null argument");
+  *replays = B->replays;
   return SVG_OK;
 }
 
@@ -3321,7 +3377,7 @@ int svg_batch_newton_iteration(svg_batch* B, svg_msh* M, svg_face* Fa, const svg
     }
     B->newtonKey = k.b;
   }
-  if ((rc = batch_cycle_graph(B, R, Val, md, fa))) return rc;
+  if ((rc = batch_cycle_graph(B, R, Val, md, fa, proto->mItr))) return rc;
   SVG_HIP(hipEventRecord(B->ev, caller));
   SVG_HIP(hipStreamWaitEvent(s, B->ev, 0));
   SVG_HIP(hipGraphLaunch(B->pro, s));

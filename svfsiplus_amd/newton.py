@@ -179,7 +179,7 @@ def construct_fluid(msh: DeviceMesh, phys: Physics, Ag, Yg, R, Val, Bf=None):
 
 
 def solve_opts(precision: int = _lib.PRECISION_FP64, lpc=None, fallback_window: int = 0,
-               net_period: int = 1, coarse: int = 0) -> _lib.SolveOpts:
+               net_period: int = 1, coarse: int = 0, coarse_keep: bool = False) -> _lib.SolveOpts:
     """The svg_solve_opts of a solve: precision; the learned method when a
     network is given; coarse > 0 adds the two-level preconditioner with node
     aggregates of about `coarse` nodes (smoothed by block Jacobi, or by the
@@ -193,21 +193,23 @@ def solve_opts(precision: int = _lib.PRECISION_FP64, lpc=None, fallback_window: 
     if lpc is not None or coarse > 0:
         opts.fallback_window = fallback_window
         opts.coarse = coarse
+        opts.coarse_keep = 1 if (coarse > 0 and coarse_keep) else 0
     return opts
 
 
 def fsils_solve(lhs: DeviceLhs, ls: _lib.Ls, dof: int, Ri, Val, prec: int = _lib.PREC_FSILS, incL=None, res=None,
                 precision: int = _lib.PRECISION_FP64, lpc=None, fallback_window: int = 0, net_period: int = 1,
-                info: dict | None = None, coarse: int = 0):
+                info: dict | None = None, coarse: int = 0, coarse_keep: bool = False):
     """Reference signature: fsils_solve(lhs, ls, dof, Ri, Val, prec, incL, res) (solve.cpp:50).
     precision = PRECISION_MIXED selects the FP32-SpMV / FP64-refined GMRES; a
     network `lpc` (LearnedPC) or coarse > 0 (the two-level preconditioner,
     aggregates of about `coarse` nodes) selects the flexible GMRES
-    (info["fallback_itr"] = iteration count at the fallback, -1 if none)."""
+    (info["fallback_itr"] = iteration count at the fallback, -1 if none);
+    coarse_keep reuses the coarse inverse of the lhs's last two-level set-up."""
     nf = lhs.nFaces
     inc = np.ones(max(1, nf), dtype=np.int32) if incL is None else np.ascontiguousarray(incL, dtype=np.int32)
     rs = None if res is None else np.ascontiguousarray(res, dtype=np.float64)
-    opts = solve_opts(precision, lpc, fallback_window, net_period, coarse)
+    opts = solve_opts(precision, lpc, fallback_window, net_period, coarse, coarse_keep)
     _lib.check(_lib.load().svg_fsils_solve_ex(lhs.h, C.byref(ls), dof, _ptr(Ri), _ptr(Val), prec, nf,
                                               inc.ctypes.data, None if rs is None else rs.ctypes.data, C.byref(opts),
                                               _stream()))
@@ -240,9 +242,11 @@ class FluidNewton:
 
     def __init__(self, pipe: Pipe, phys: Physics, ls: _lib.Ls, eq: EqState | None = None, device="cuda",
                  precision: int = _lib.PRECISION_FP64, lpc=None, outlet: bool = True, net_period: int = 1,
-                 device_setup: bool = False, coarse: int = 0):
+                 device_setup: bool = False, coarse: int = 0, coarse_keep: bool = False):
         """device_setup: build the block pattern, colouring and slot map on the
-        device (svg_lhs_create_dev, svg_msh_create_dev) from IEN and x."""
+        device (svg_lhs_create_dev, svg_msh_create_dev) from IEN and x;
+        coarse_keep: the Newton iterations after the first of a time step keep
+        the coarse inverse of the first (the two-level preconditioner)."""
         import torch
         if not torch.cuda.is_available():
             raise RuntimeError("FluidNewton needs a GPU (the HIP library has no CPU path)")
@@ -252,6 +256,7 @@ class FluidNewton:
         self.lpc = lpc
         self.net_period = net_period
         self.coarse = coarse
+        self.coarse_keep = coarse_keep
         self.eq = eq or EqState()
         if device_setup:
             d_ien, d_x = device_arrays(pipe)
@@ -319,7 +324,8 @@ class FluidNewton:
             b_assem_neu_bc(self.outlet, ph, self.Yg, self.R, self.Val)
         info = {}
         fsils_solve(self.lhs, ls, 4, self.R, self.Val, precision=self.precision, lpc=self.lpc,
-                    net_period=self.net_period, info=info, coarse=self.coarse)
+                    net_period=self.net_period, info=info, coarse=self.coarse,
+                    coarse_keep=self.coarse_keep and eq.itr > 1)
         _lib.check(lib.svg_picc_update(self.pipe.nNo, ph.gam, ph.dt, _ptr(self.R), _ptr(self.An), _ptr(self.Yn), s))
         # Newton convergence, pic.cpp:236-281
         eps = np.finfo(np.float64).eps

@@ -266,3 +266,50 @@ def test_batch_twolevel_newton_graph_equals_eager(torch_cuda):
         bn.fn.R.mul_(0.5)
         print(f"replay {rep}: itr {[r['itr'] for r in got[0]]} identical to the eager iteration")
     bn.free()
+
+
+@pytest.mark.parametrize("mItr", [3, 1000])
+def test_batch_cycles_per_replay(torch_cuda, mItr):
+    """Several restart cycles per graph replay: every mesh's count, norms and
+    solution equal its single-mesh solve bit for bit, also when the restart
+    budget mItr cuts the solves short (the budget is enforced per mesh on the
+    device, in the middle of a replay), and the host synchronises once per
+    replay, fewer times than the cycles the slowest mesh runs."""
+    torch = torch_cuda
+    meshes = _meshes(4, seed=13)
+    phys = W.Physics()
+    bn = _assembled_union(torch, meshes, phys)
+    R0, V0 = bn.fn.R.clone(), bn.fn.Val.clone()
+    boff = np.concatenate([[0], np.cumsum([m.nnz for m in meshes])])
+    tol, sD = 1e-9, 20
+    singles = []
+    for q, m in enumerate(meshes):
+        a, b = int(bn.noff[q]), int(bn.noff[q + 1])
+        Rq, Vq = R0[a:b].clone(), V0[boff[q]:boff[q + 1]].clone()
+        ls = ls_create(_lib.LS_GMRES, relTol=tol, absTol=1e-30, maxItr=mItr, dimKry=sD)
+        fsils_solve(DeviceLhs(m), ls, 4, Rq, Vq)
+        singles.append((Rq, ls.RI.itr, ls.RI.iNorm, ls.RI.fNorm, ls.RI.suc))
+    h = C.c_void_p()
+    _lib.check(_lib.load().svg_batch_create(bn.fn.lhs.h, bn.nsys, bn.noff.ctypes.data, sD, C.byref(h)))
+    proto = _lib.SubLs()
+    proto.mItr, proto.sD, proto.absTol, proto.relTol = mItr, sD, 1e-30, tol
+    out = (_lib.SubLs * bn.nsys)()
+    Rb, Vb = R0.clone(), V0.clone()
+    r0 = C.c_longlong()
+    _lib.check(_lib.load().svg_batch_replays(h, C.byref(r0)))
+    _lib.check(_lib.load().svg_batch_fsils_solve(h, C.byref(proto), Rb.data_ptr(), Vb.data_ptr(),
+                                                 _lib.PRECISION_FP64, out, None))
+    torch.cuda.synchronize()
+    r1 = C.c_longlong()
+    _lib.check(_lib.load().svg_batch_replays(h, C.byref(r1)))
+    cycles = []
+    for q, (Rq, itr, iN, fN, suc) in enumerate(singles):
+        a, b = int(bn.noff[q]), int(bn.noff[q + 1])
+        print(f"mesh {q}: itr batch {out[q].itr} single {itr} fNorm {out[q].fNorm:.6e}/{fN:.6e} suc {out[q].suc}/{suc}")
+        assert (out[q].itr, out[q].iNorm, out[q].fNorm, out[q].suc) == (itr, iN, fN, suc)
+        assert torch.equal(Rb[a:b], Rq)
+        cycles.append(-(-itr // (sD + 1)))
+    replays = r1.value - r0.value
+    print(f"mItr {mItr}: {replays} replays, the slowest mesh ran about {max(cycles)} cycles")
+    assert replays < max(cycles) or max(cycles) <= 1
+    _lib.load().svg_batch_free(h)

@@ -142,3 +142,41 @@ def test_two_level_gmres_solve(torch_cuda, precision):
     fsils_solve(L, plain, 4, dR, dV, precision=precision)
     assert 3 * out[0][1] <= plain.RI.itr
     L.free()
+
+
+@pytest.mark.parametrize("precision", [_lib.PRECISION_FP64, _lib.PRECISION_MIXED])
+def test_two_level_keeps_coarse_inverse(torch_cuda, precision):
+    """coarse_keep: a first solve with coarse_keep set but no inverse on the lhs
+    builds it (bit for bit the solve without the flag); a second system on the
+    same lhs (the next Newton iteration's, a perturbed state) solved with the
+    kept inverse reaches the FP64 true residual 1e-6 of its own scaled system,
+    and its iteration count stays within 25 % of the rebuilt one's."""
+    from svfsiplus_amd.newton import DeviceLhs, fsils_solve, ls_create
+    torch = torch_cuda
+    p = W.make_pipe(8)
+    ph = W.Physics()
+    Ao, Yo, An, Yn = W.dirichlet_state(p, ph)
+    Ag, Yg = oracle.pici(ph, Ao, An, Yo, Yn)
+    R1, V1 = oracle.construct_fluid(p, ph, Ag, Yg)
+    rng = np.random.default_rng(3)
+    R2, V2 = oracle.construct_fluid(p, ph, Ag + rng.standard_normal(Ag.shape), Yg + 0.2 * rng.standard_normal(Yg.shape))
+    mk = lambda: ls_create(_lib.LS_GMRES, relTol=1e-6, absTol=1e-30, maxItr=1000, dimKry=50)
+    res = {}
+    for keep in (False, True):
+        L = DeviceLhs(p)
+        out = []
+        for R, V in ((R1, V1), (R2, V2)):
+            ls = mk()
+            dR, dV = _dev(torch, R), _dev(torch, V)
+            fsils_solve(L, ls, 4, dR, dV, precision=precision, coarse=32, coarse_keep=keep)
+            x = dR.cpu().numpy()
+            Vs, Rs, Wd = oracle.precond_diag(p, V, R)
+            y = np.divide(x, Wd, out=np.zeros_like(x), where=Wd != 0)
+            rel = np.linalg.norm(Rs - oracle.spar_mul_vv(p, 4, Vs, y)) / np.linalg.norm(Rs)
+            print(f"keep {keep}: itr {ls.RI.itr} true rel res {rel:.2e}")
+            assert ls.RI.suc == 1 and rel <= 1e-6
+            out.append((x, ls.RI.itr))
+        res[keep] = out
+        L.free()
+    assert np.array_equal(res[True][0][0], res[False][0][0]) and res[True][0][1] == res[False][0][1]
+    assert res[True][1][1] <= 1.25 * res[False][1][1] + 2
