@@ -59,6 +59,9 @@ KERNELS = ["spmv_bcsr", "spmv32", "d2f", "mdot", "mdot_cb", "mdot_fin", "gs_upda
 # kernels whose launches carry algorithmic byte counts (roofline candidates)
 HBM_KERNELS = ["spmv_bcsr", "spmv32", "lpc_msg", "lpc_dense", "lpc_layer", "lpc_encode"]
 ROOFLINE_KERNELS = ["spmv_bcsr", "spmv32"]  # timed with events inside the timed region
+# the kernels of one Arnoldi step of the GMRES (SpMV, Gram-Schmidt dots, finaliser, update), FP64 / FP32 basis
+ARNOLDI_KERNELS = {"fp64": ["spmv_bcsr", "mdot", "mdot_fin", "gs_update"],
+                   "fp32": ["spmv32", "mdot_cb", "mdot_fin", "gs_update_cb"]}
 PROF_EVERY = 16  # ... one launch in 16 of each (events between launches cost time)
 DEFAULT_WEIGHTS = os.path.join(ROOT, "svfsiplus_amd", "data", "lpc_weights.npz")     # the network alone
 DEFAULT_SMOOTHER = os.path.join(ROOT, "svfsiplus_amd", "data", "lpc_smoother.npz")   # the two-level smoother
@@ -251,7 +254,7 @@ def kernel_only(a, fn, pipe, lib):
 
 def _bytes(lib, ktab):
     import ctypes as C
-    for k in HBM_KERNELS:
+    for k in HBM_KERNELS + ARNOLDI_KERNELS["fp64"] + ARNOLDI_KERNELS["fp32"]:
         if k in ktab:
             tb = C.c_double()
             lib.svg_prof_get_bytes(k.encode(), C.byref(tb))
@@ -330,6 +333,23 @@ def roofline(ktab, measured=None, variants=None):
             "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": _traffic(measured, variants, k), "kernel": k,
             "bytes_per_launch": round(bpl),
             "avg_us": round(avg_s * 1e6, 2)}
+
+
+def arnoldi_step(ktab, basis):
+    """Roofline of the whole Arnoldi step (SpMV + Gram-Schmidt dots + finaliser
+    + update) from a kernel table in which every launch was timed: the
+    algorithmic bytes of those launches over their summed durations, per
+    Arnoldi step (= per update launch)."""
+    ks = ARNOLDI_KERNELS[basis]
+    if not all(k in ktab and ktab[k].get("count") for k in ks):
+        return None
+    steps = ktab[ks[3]]["count"]
+    t = sum(ktab[k]["total_ms"] for k in ks) * 1e-3
+    b = sum(ktab[k].get("bytes", 0.0) for k in ks)
+    return {"kernels": ks, "steps": steps, "us_per_step": round(t / steps * 1e6, 2),
+            "bytes_per_step": round(b / steps), "achieved": round(b / t / 1e9, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(b / t / 1e9 / HBM_PEAK_GBS, 4),
+            "us_per_step_at_peak": round(b / steps / (HBM_PEAK_GBS * 1e9) * 1e6, 2)}
 
 
 def _traffic(measured, variants, k):
@@ -418,6 +438,79 @@ def configure(fn, method, a, net=None):
     fn.coarse_keep = bool(a.coarse_keep) and fn.coarse > 0
     fn.lpc = net if method == "learned" else None
     fn.net_period = a.net_period
+
+
+def ns_run(a, log, with_cpu=True):
+    """The 15k-tet pipe (n = 8) with the fluid equation's default linear solver,
+    FSILS NS (ns_solver.cpp:167-499) with fsils_ls_create's NS defaults
+    (ls.cpp:51-60), one time step of 3 Newton iterations (outlet backflow face
+    included): seconds per Newton step on the device (a warm-up step, then
+    a.ns_steps timed) and the RI/GM/CG iteration counts per Newton iteration;
+    then the reference's own CPU path on the same three systems (its
+    construct_fluid, b_assem_neu_bc and fsils_solve from oracle/_ref, 1 thread),
+    each assembled from the device's Newton iterates, timed in full."""
+    import torch
+    import oracle
+    from svfsiplus_amd import _lib, workload as W
+    from svfsiplus_amd.newton import FluidNewton, ls_create
+    pipe = W.make_pipe(8)
+    phys = W.Physics()
+    fn = FluidNewton(pipe, phys, ls_create(_lib.LS_NS, absTol=1e-30))
+    states, recs = [], []
+    els = []
+    for k in range(1 + a.ns_steps):
+        fn.reset()
+        fn.eq.itr = 0
+        fn.eq.ok = False
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        rec = []
+        while True:
+            if k == 0:
+                states.append(None)
+            done = fn.newton_iteration()
+            ls = fn.ls
+            rec.append({"RI": ls.RI.itr, "GM": ls.GM.itr, "CG": ls.CG.itr, "Resm": ls.Resm, "Resc": ls.Resc,
+                        "iNorm": ls.RI.iNorm, "fNorm": ls.RI.fNorm, "suc": ls.RI.suc})
+            if k == 0:
+                states[-1] = (fn.Ag.cpu().numpy(), fn.Yg.cpu().numpy())
+            if done:
+                break
+        torch.cuda.synchronize()
+        els.append(time.perf_counter() - t)
+        recs.append(rec)
+    el = float(np.mean(els[1:]))
+    out = {"workload": f"VMS pipe n=8: {pipe.nEl} tets, {pipe.nNo} nodes, nnzb {pipe.nnz}; 1 time step x "
+                       f"{len(recs[-1])} Newton iterations; FSILS NS solver, fsils_ls_create's NS defaults",
+           "s_per_newton": round(el / len(recs[-1]), 6), "newton_records": recs[-1],
+           "reached_relTol": bool(all(r["suc"] for rr in recs for r in rr)), "steps": a.ns_steps, "warmup": 1,
+           "note": "with fsils_ls_create's NS defaults (RI relTol 0.4, 10 outer iterations) the outer iteration "
+                   "stops on its iteration limit in every Newton iteration of this time step (suc 0), as the "
+                   "reference's own solve of the same systems does (cpu_reference.ri_suc)"}
+    log(f"# ns C1: {out['s_per_newton']} s/Newton, RI/GM/CG {[(r['RI'], r['GM'], r['CG']) for r in recs[-1]]}")
+    if with_cpu and oracle.ref_available():
+        fi, ge, _ = W.outlet_faces(pipe)
+        prm = oracle.ls_params(oracle.ls_create(oracle.LS_NS, absTol=1e-30))
+        t_cpu, crec = 0.0, []
+        for Ag, Yg in states:
+            t = time.perf_counter()
+            R, V = oracle.ref_construct_fluid(pipe, phys, Ag, Yg)
+            oracle.ref_b_assem_neu_bc(pipe, phys, Yg, fi, ge, R, V)
+            _, _, r = oracle.ref_fsils_solve(pipe, oracle.LS_NS, prm, R, V)
+            t_cpu += time.perf_counter() - t
+            crec.append((int(r["itr"]), int(r["GM_itr"]), int(r["CG_itr"]), int(r["suc"])))
+        cpu = host_cpu()
+        out["cpu_reference"] = {"value": round(t_cpu / len(states), 4), "unit": "s/Newton-step", "cores": 1,
+                                "kind": "reference", "ri_iterations": [c[0] for c in crec],
+                                "gm_iterations": [c[1] for c in crec], "cg_iterations": [c[2] for c in crec],
+                                "ri_suc": [c[3] for c in crec], "host_cpu": cpu["model"],
+                                "host_mhz": cpu["mhz"],
+                                "sample": "the reference's construct_fluid, b_assem_neu_bc and fsils_solve (NS) "
+                                          "compiled from its sources, 1 thread, on the three Newton systems of the "
+                                          "time step (assembled from the device's iterates), every one timed in full"}
+        out["speedup_vs_cpu_reference"] = round(t_cpu / len(states) / out["s_per_newton"], 1)
+        log(f"# ns C1 cpu reference: {out['cpu_reference']['value']} s/Newton")
+    return out
 
 
 def c4_run(a, lib, net, log):
@@ -596,6 +689,8 @@ def main():
                     help="learned: the network preconditions every net-period-th Arnoldi step, the others the "
                          "two-level preconditioner with block Jacobi (or, with --coarse 0, the diagonal scaling)")
     ap.add_argument("--c4", type=int, default=1, help="run the stenosed 4.2M-tet leg (0: skip it)")
+    ap.add_argument("--ns", type=int, default=1, help="run the NS-solver leg on the 15k-tet pipe (0: skip it)")
+    ap.add_argument("--ns-steps", type=int, default=3)
     ap.add_argument("--c4-n", type=int, default=52)
     ap.add_argument("--c4-sten", type=float, default=0.5)
     ap.add_argument("--c4-steps", type=int, default=1)
@@ -723,6 +818,7 @@ def main():
                    "method": head, "solver": solver[head], "newton_records": R["hists"][-1],
                    "spmv_variants": spmv_variants(lib, fn.lhs)},
         "roofline": roofline(R["kernels"], traffic, spmv_variants(lib, fn.lhs)),
+        "arnoldi_step": arnoldi_step(R["kernels_warmup"] or R["kernels"], "fp64" if head == "plain" else "fp32"),
         "cpu_baseline": cpu,
         "kernels": R["kernels_warmup"] or R["kernels"],
         "kernels_note": "per-kernel event table of the untimed warmup step; the timed steps bracket one launch "
@@ -732,7 +828,8 @@ def main():
         P = runs["plain"]
         out["plain"] = {"value": round(P["s_per_newton"], 6), "newton_records": P["hists"][-1],
                         "steps": P["steps"], "warmup": P["warmup"], "all_converged": P["all_converged"],
-                        "roofline": roofline(P["kernels"], traffic, spmv_variants(lib, fn.lhs))}
+                        "roofline": roofline(P["kernels"], traffic, spmv_variants(lib, fn.lhs)),
+                        "arnoldi_step": arnoldi_step(P["kernels_warmup"] or P["kernels"], "fp64")}
         out["speedup_vs_plain"] = round(P["s_per_newton"] / R["s_per_newton"], 3)
         # the time step's result against the FP64 plain path's (velocity, pressure, acceleration)
         def rel(x, ref):
@@ -758,6 +855,8 @@ def main():
         out["c4"] = c4_run(a, lib, net, log)
     if a.c3_draws > 0 and a.method == "compare":
         out["mixed"] = mixed_draws(a, fn, lib, log)
+    if a.ns and a.method == "compare":
+        out["ns_c1"] = ns_run(a, log, with_cpu=not a.no_cpu)
     if a.batch > 0 and a.method == "compare":
         out["batch"] = batch_run(a, log)
     print(json.dumps(out), flush=True)

@@ -1315,10 +1315,11 @@ static int dot_device(int n, const double* w, const double* base, size_t ld, int
   if (nblk < 1) nblk = 1;
   if (m > MAXM) return set_error(SVG_ERR_ARG, "reduction: too many sums");
   const int* skip = (fa.mode == FIN_ARNOLDI) ? &st->done : nullptr;
-  launch("mdot", k_mdot, dim3(nblk), dim3(RED_BS), (size_t)RED_NW * m * sizeof(double), s, n, w, base, ld, m,
-         partial, skip);
-  launch("mdot_fin", k_fin, dim3(1), dim3(FIN_BS), 0, s, fa, (const double*)partial, nblk, m, st, h, cs, sn, err,
-         coef, out, (volatile int*)host_done);
+  // algorithmic bytes: w and the m - 1 basis vectors read, the partials written; the finaliser reads them
+  launch_b("mdot", 8.0 * n * m + 8.0 * m * nblk, k_mdot, dim3(nblk), dim3(RED_BS), (size_t)RED_NW * m * sizeof(double),
+           s, n, w, base, ld, m, partial, skip);
+  launch_b("mdot_fin", 8.0 * m * nblk, k_fin, dim3(1), dim3(FIN_BS), 0, s, fa, (const double*)partial, nblk, m, st,
+           h, cs, sn, err, coef, out, (volatile int*)host_done);
   return SVG_OK;
 }
 
@@ -1330,10 +1331,11 @@ static int dot_device_cb(int n, const double* w, const float* base, size_t ld, i
   int nblk = (n + RED_SPAN - 1) / RED_SPAN;
   if (nblk < 1) nblk = 1;
   if (m + 1 > MAXM) return set_error(SVG_ERR_ARG, "reduction: too many sums");
-  launch("mdot_cb", k_mdot_cb, dim3(nblk), dim3(RED_BS), (size_t)RED_NW * (m + 1) * sizeof(double), s, n, w, base,
-         ld, m, k.partial, (const int*)&k.st->done);
-  launch("mdot_fin", k_fin, dim3(1), dim3(FIN_BS), 0, s, fa, (const double*)k.partial, nblk, m + 1, k.st, k.h, k.cs,
-         k.sn, k.err, k.coef, k.scal, (volatile int*)k.d_done);
+  // algorithmic bytes: w (FP64) and the m FP32 basis vectors read, the partials written
+  launch_b("mdot_cb", 8.0 * n + 4.0 * n * m + 8.0 * (m + 1) * nblk, k_mdot_cb, dim3(nblk), dim3(RED_BS),
+           (size_t)RED_NW * (m + 1) * sizeof(double), s, n, w, base, ld, m, k.partial, (const int*)&k.st->done);
+  launch_b("mdot_fin", 8.0 * (m + 1) * nblk, k_fin, dim3(1), dim3(FIN_BS), 0, s, fa, (const double*)k.partial, nblk,
+           m + 1, k.st, k.h, k.cs, k.sn, k.err, k.coef, k.scal, (volatile int*)k.d_done);
   return SVG_OK;
 }
 
@@ -1547,8 +1549,9 @@ static int run_cycle(const Op& A, KrylovWork& k, int sD, FinArgs fa, double* X, 
       int rc = dot_device_cb(n, w, k.v32, k.ld, i + 1, k, fa, s);
       if (rc) return rc;
       const int keep64 = (!tl && (coupled || (i + 1 < sD && netStep(i + 1)))) ? 1 : 0;
-      launch("gs_update_cb", k_gs_update_cb, dim3(nbv2), dim3(256), 0, s, n, w, k.v32, k.ld, i,
-             (const double*)k.coef, (const GmState*)k.st, keep64);
+      // algorithmic bytes: w and v_0..v_i read, v_{i+1} (FP32) written, w rewritten with keep64
+      launch_b("gs_update_cb", 8.0 * n + 4.0 * n * (i + 2) + (keep64 ? 8.0 * n : 0.0), k_gs_update_cb, dim3(nbv2),
+               dim3(256), 0, s, n, w, k.v32, k.ld, i, (const double*)k.coef, (const GmState*)k.st, keep64);
     } else {
       op_apply(A, sin, w, nullptr, &k.st->done, s);
       int rc = dot_device(n, w, k.u, k.ld, i + 2, k.partial, k.scal, k.st, fa, k.h, k.cs, k.sn, k.err, k.coef, s,
@@ -1556,8 +1559,10 @@ static int run_cycle(const Op& A, KrylovWork& k, int sD, FinArgs fa, double* X, 
       if (rc) return rc;
       // flexible: when the next step has no network, this update also writes z_{i+1} = u_{i+1}
       const bool nextPlain = flex && !tl && i + 1 < sD && !netStep(i + 1);
-      launch("gs_update", k_gs_update, dim3(nbv), dim3(256), 0, s, n, k.u, k.ld, i, (const double*)k.coef,
-             (const GmState*)k.st, (float*)nullptr, nextPlain ? k.z + k.ld * (i + 1) : (double*)nullptr);
+      // algorithmic bytes: w and u_0..u_i read, u_{i+1} written (and z_{i+1})
+      launch_b("gs_update", 8.0 * n * (i + 3) + (nextPlain ? 8.0 * n : 0.0), k_gs_update, dim3(nbv), dim3(256), 0, s,
+               n, k.u, k.ld, i, (const double*)k.coef, (const GmState*)k.st, (float*)nullptr,
+               nextPlain ? k.z + k.ld * (i + 1) : (double*)nullptr);
     }
     SVG_HIP(hipEventRecord(k.ev[i % 8], s));
   }

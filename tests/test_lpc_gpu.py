@@ -188,3 +188,20 @@ def test_learned_after_larger_then_smaller_krylov_dimension(torch_cuda):
     x, ls, rel = solve(fresh, 60, net)
     assert ls.RI.itr == out[60][1]
     assert torch.equal(x, out[60][0])
+
+
+def test_lpc_apply_bitwise_regression(torch_cuda):
+    """svg_lpc_apply on seeded inputs against the stored output of the same
+    kernels (tests/golden/lpc_apply_n4.npz, written by
+    tests/golden/make_lpc_golden.py): bit for bit, so a change to the network
+    kernels meant to keep their arithmetic is checked to keep it."""
+    import sys as _sys
+    golden = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    _sys.path.insert(0, golden)
+    import make_lpc_golden as G
+    d = np.load(os.path.join(golden, "lpc_apply_n4.npz"))
+    p, s, V = G.inputs()
+    assert np.array_equal(p, d["params"]) and np.array_equal(V, d["V"])
+    Z = G.apply(p, s, V)
+    print(f"lpc apply: {(Z != d['Z']).sum()} of {Z.size} entries differ from the stored output")
+    assert np.array_equal(Z, d["Z"])
