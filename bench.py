@@ -52,7 +52,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 KERNELS = ["spmv_bcsr", "spmv32", "d2f", "mdot", "mdot_cb", "mdot_fin", "gs_update", "gs_update_cb", "zcopy", "div",
            "backsolve", "xupdate", "xupdate_cb",
-           "pc_w", "pc_val", "fluid_tet4", "b_fluid", "pici", "picc", "fill", "wc_scale", "set_int", "spmv_tune",
+           "pc_w", "pc_val", "fluid_elem", "fluid_gather", "b_fluid", "pici", "picc", "fill", "wc_scale", "set_int", "spmv_tune",
            "spmv32_tune", "lpc_prep", "lpc_encode", "lpc_msg", "lpc_dense", "lpc_layer", "add_bc_mul", "face_valM",
            "rsub", "cl_ap", "cl_dinv", "cl_acinit", "cl_ac", "gj_pivot", "gj_panels", "gj_update", "cl_d2f",
            "cl_restrict", "cl_gemv", "cl_apply", "cl_addc", "rearm"]
@@ -184,6 +184,32 @@ def kernel_only_lpc(a, fn, pipe, lib):
     tab = _bytes(lib, kernel_table(lib))
     print(json.dumps({"kernel": "lpc", "applications": a.launches, "nNo": pipe.nNo, "nnzb": pipe.nnz,
                       "kernels": {k: v for k, v in tab.items() if k.startswith("lpc_")}}))
+
+
+def kernel_only_assembly(a, fn, pipe, lib):
+    """construct_fluid alone (the element kernel and the element-order gather)
+    on the first Newton iteration's state of the workload, a.launches times
+    (for counter runs), with the per-kernel event table."""
+    import torch
+    from svfsiplus_amd import _lib
+    from svfsiplus_amd.newton import _stream, construct_fluid
+    ph = fn.phys
+    _lib.check(lib.svg_pici(pipe.nNo, ph.am, ph.af, fn.Ao.data_ptr(), fn.An.data_ptr(), fn.Yo.data_ptr(),
+                            fn.Yn.data_ptr(), fn.Ag.data_ptr(), fn.Yg.data_ptr(), _stream()))
+    construct_fluid(fn.msh, ph, fn.Ag, fn.Yg, fn.R, fn.Val)
+    torch.cuda.synchronize()
+    lib.svg_prof_reset()
+    lib.svg_prof_enable(1)
+    for _ in range(a.launches):
+        fn.R.zero_()
+        fn.Val.zero_()
+        construct_fluid(fn.msh, ph, fn.Ag, fn.Yg, fn.R, fn.Val)
+    torch.cuda.synchronize()
+    lib.svg_prof_enable(0)
+    tab = kernel_table(lib)
+    print(json.dumps({"kernel": "assembly", "launches": a.launches, "nEl": pipe.nEl, "nNo": pipe.nNo,
+                      "nnzb": pipe.nnz,
+                      "kernels": {k: v for k, v in tab.items() if k in ("fluid_elem", "fluid_gather")}}))
 
 
 def kernel_only(a, fn, pipe, lib):
@@ -441,21 +467,27 @@ def configure(fn, method, a, net=None):
 
 
 def ns_run(a, log, with_cpu=True):
-    """The 15k-tet pipe (n = 8) with the fluid equation's default linear solver,
-    FSILS NS (ns_solver.cpp:167-499) with fsils_ls_create's NS defaults
-    (ls.cpp:51-60), one time step of 3 Newton iterations (outlet backflow face
+    """The 15k-tet pipe (n = 8, the C1 case) with the fluid equation's default
+    linear solver, FSILS NS (ns_solver.cpp:167-499), set up as the reference's
+    read_ls does for <LS type="NS"> with Tolerance a.ns_tol and the other
+    <Linear_solver> parameters at their defaults (read_files.cpp:2060-2090,
+    Parameters.cpp:2466-2479: RI Max_iterations 1000, Absolute_tolerance
+    1e-10, Krylov dimension 100 (fsils_ls_create's), NS_GM 1000 / 1e-2, NS_CG
+    1000 / 1e-2); one time step of 3 Newton iterations (outlet backflow face
     included): seconds per Newton step on the device (a warm-up step, then
     a.ns_steps timed) and the RI/GM/CG iteration counts per Newton iteration;
     then the reference's own CPU path on the same three systems (its
-    construct_fluid, b_assem_neu_bc and fsils_solve from oracle/_ref, 1 thread),
-    each assembled from the device's Newton iterates, timed in full."""
+    construct_fluid, b_assem_neu_bc and fsils_solve from oracle/_ref, 1 thread,
+    the same parameters), each assembled from the device's Newton iterates,
+    timed in full."""
     import torch
     import oracle
     from svfsiplus_amd import _lib, workload as W
     from svfsiplus_amd.newton import FluidNewton, ls_create
     pipe = W.make_pipe(8)
     phys = W.Physics()
-    fn = FluidNewton(pipe, phys, ls_create(_lib.LS_NS, absTol=1e-30))
+    ls0 = ls_create(_lib.LS_NS, relTol=a.ns_tol, absTol=1e-10, maxItr=1000)
+    fn = FluidNewton(pipe, phys, ls0)
     states, recs = [], []
     els = []
     for k in range(1 + a.ns_steps):
@@ -472,6 +504,7 @@ def ns_run(a, log, with_cpu=True):
             ls = fn.ls
             rec.append({"RI": ls.RI.itr, "GM": ls.GM.itr, "CG": ls.CG.itr, "Resm": ls.Resm, "Resc": ls.Resc,
                         "iNorm": ls.RI.iNorm, "fNorm": ls.RI.fNorm, "suc": ls.RI.suc})
+            log(f"# ns C1 step {k} Newton {len(rec)}: RI/GM/CG {ls.RI.itr}/{ls.GM.itr}/{ls.CG.itr} suc {ls.RI.suc}")
             if k == 0:
                 states[-1] = (fn.Ag.cpu().numpy(), fn.Yg.cpu().numpy())
             if done:
@@ -481,16 +514,16 @@ def ns_run(a, log, with_cpu=True):
         recs.append(rec)
     el = float(np.mean(els[1:]))
     out = {"workload": f"VMS pipe n=8: {pipe.nEl} tets, {pipe.nNo} nodes, nnzb {pipe.nnz}; 1 time step x "
-                       f"{len(recs[-1])} Newton iterations; FSILS NS solver, fsils_ls_create's NS defaults",
+                       f"{len(recs[-1])} Newton iterations; FSILS NS solver, RI relTol {a.ns_tol:g} / absTol "
+                       f"1e-10 / 1000 iterations / Krylov {ls0.RI.sD}, GM {ls0.GM.mItr} / {ls0.GM.relTol:g}, "
+                       f"CG {ls0.CG.mItr} / {ls0.CG.relTol:g} (read_ls with <LS type=NS> defaults)",
            "s_per_newton": round(el / len(recs[-1]), 6), "newton_records": recs[-1],
-           "reached_relTol": bool(all(r["suc"] for rr in recs for r in rr)), "steps": a.ns_steps, "warmup": 1,
-           "note": "with fsils_ls_create's NS defaults (RI relTol 0.4, 10 outer iterations) the outer iteration "
-                   "stops on its iteration limit in every Newton iteration of this time step (suc 0), as the "
-                   "reference's own solve of the same systems does (cpu_reference.ri_suc)"}
+           "reached_relTol": bool(all(r["suc"] for rr in recs for r in rr)), "steps": a.ns_steps, "warmup": 1}
     log(f"# ns C1: {out['s_per_newton']} s/Newton, RI/GM/CG {[(r['RI'], r['GM'], r['CG']) for r in recs[-1]]}")
     if with_cpu and oracle.ref_available():
         fi, ge, _ = W.outlet_faces(pipe)
-        prm = oracle.ls_params(oracle.ls_create(oracle.LS_NS, absTol=1e-30))
+        prm = (ls0.RI.relTol, ls0.RI.absTol, ls0.RI.mItr, ls0.RI.sD, ls0.GM.relTol, ls0.GM.absTol, ls0.GM.mItr,
+               ls0.GM.sD, ls0.CG.relTol, ls0.CG.absTol, ls0.CG.mItr)
         t_cpu, crec = 0.0, []
         for Ag, Yg in states:
             t = time.perf_counter()
@@ -499,6 +532,7 @@ def ns_run(a, log, with_cpu=True):
             _, _, r = oracle.ref_fsils_solve(pipe, oracle.LS_NS, prm, R, V)
             t_cpu += time.perf_counter() - t
             crec.append((int(r["itr"]), int(r["GM_itr"]), int(r["CG_itr"]), int(r["suc"])))
+            log(f"# ns C1 cpu reference Newton {len(crec)}: RI/GM/CG {crec[-1][:3]}")
         cpu = host_cpu()
         out["cpu_reference"] = {"value": round(t_cpu / len(states), 4), "unit": "s/Newton-step", "cores": 1,
                                 "kind": "reference", "ri_iterations": [c[0] for c in crec],
@@ -690,7 +724,9 @@ def main():
                          "two-level preconditioner with block Jacobi (or, with --coarse 0, the diagonal scaling)")
     ap.add_argument("--c4", type=int, default=1, help="run the stenosed 4.2M-tet leg (0: skip it)")
     ap.add_argument("--ns", type=int, default=1, help="run the NS-solver leg on the 15k-tet pipe (0: skip it)")
-    ap.add_argument("--ns-steps", type=int, default=3)
+    ap.add_argument("--ns-steps", type=int, default=1)
+    ap.add_argument("--ns-tol", type=float, default=0.5,
+                    help="NS leg: the <LS> Tolerance (RI relTol; 0.5 is the <Linear_solver> default)")
     ap.add_argument("--c4-n", type=int, default=52)
     ap.add_argument("--c4-sten", type=float, default=0.5)
     ap.add_argument("--c4-steps", type=int, default=1)
@@ -712,7 +748,7 @@ def main():
                          "or the two-level preconditioner (one coarse block per mesh)")
     ap.add_argument("--batch-coarse", type=int, default=64, help="twolevel batch: nodes per aggregate")
     ap.add_argument("--no-prof", action="store_true", help="time without per-kernel events")
-    ap.add_argument("--kernel-only", choices=["spmv", "spmv32", "lpc"], default=None,
+    ap.add_argument("--kernel-only", choices=["spmv", "spmv32", "lpc", "assembly"], default=None,
                     help="launch only the block SpMV on the assembled system (for counter runs), "
                          "plus a device copy of the same byte count as the calibration kernel")
     ap.add_argument("--launches", type=int, default=20)
@@ -749,6 +785,9 @@ def main():
 
     if a.kernel_only == "lpc":
         kernel_only_lpc(a, fn, pipe, lib)
+        return
+    if a.kernel_only == "assembly":
+        kernel_only_assembly(a, fn, pipe, lib)
         return
     if a.kernel_only:
         kernel_only(a, fn, pipe, lib)
