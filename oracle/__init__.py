@@ -210,6 +210,24 @@ def ref_b_assem_neu_bc(pipe, phys, Yg, faceIEN, gE, R, Val, hg=None):
     return R, Val
 
 
+def ref_write_restart(stFileName, stFileRepl, cTS, time, stamp, iNorm, xn, Yn, An, recLn):
+    """The reference's own write_restart (oracle/_ref: output.cpp:202-376) on one
+    rank for the fluid equation; Yn/An (tnNo, tDof). Writes the files the
+    reference writes (the numbered file and its _last.bin link, or _last.bin)."""
+    lib = load_ref()
+    st = np.ascontiguousarray(stamp, dtype=np.int32)
+    iN = np.ascontiguousarray(iNorm, dtype=np.float64)
+    x = np.ascontiguousarray(xn, dtype=np.float64)
+    Y = np.ascontiguousarray(Yn, dtype=np.float64)
+    A = np.ascontiguousarray(An, dtype=np.float64)
+    rc = lib.ref_write_restart(stFileName.encode(), C.c_int(1 if stFileRepl else 0), C.c_int(cTS), C.c_double(time),
+                               C.c_void_p(_p(st)), C.c_int(len(iN)), C.c_void_p(_p(iN)), C.c_int(len(x)),
+                               C.c_void_p(_p(x) if len(x) else None), C.c_int(Y.shape[0]), C.c_int(Y.shape[1]),
+                               C.c_void_p(_p(Y)), C.c_void_p(_p(A)), C.c_int(recLn))
+    if rc:
+        raise RuntimeError(lib.ref_last_error().decode())
+
+
 REF_LS_UNSET = -7777.0  # ref_ls_create's value for a field fsils_ls_create leaves unset
 
 

@@ -37,11 +37,13 @@ CXXFLAGS = -std=c++17 -O2 -fPIC -ffunction-sections -fdata-sections -w \
 LS_SRC     := $(wildcard $(LS)/*.cpp)
 SOLVER_TUS := fluid nn fs lhsa utils Array Array3 Vector all_fun ComMod CmMod consts CepMod \
               mat_fun Tensor4 FsilsLinearAlgebra LinearAlgebra PetscLinearAlgebra TrilinosLinearAlgebra \
-              eq_assem heats heatf l_elas cep
+              eq_assem heats heatf l_elas cep output Simulation Parameters CepModAp CepModBo CepModFn \
+              CepModTtp ChnlMod
 SOLVER_SRC := $(addprefix $(SOLVER)/,$(addsuffix .cpp,$(SOLVER_TUS)))
 
 LS_OBJ     := $(patsubst $(LS)/%.cpp,$(OBJ)/ls_%.o,$(LS_SRC))
-SOLVER_OBJ := $(patsubst $(SOLVER)/%.cpp,$(OBJ)/sv_%.o,$(SOLVER_SRC))
+SOLVER_OBJ := $(patsubst $(SOLVER)/%.cpp,$(OBJ)/sv_%.o,$(SOLVER_SRC)) $(OBJ)/tinyxml2.o
+TINYXML    := $(REF)/Code/ThirdParty/tinyxml/simvascular_tinyxml/tinyxml2.cpp
 HARN_OBJ   := $(OBJ)/ref_harness.o
 
 REPO    := $(abspath $(HERE)..)
@@ -62,6 +64,9 @@ $(OBJ)/ls_%.o: $(LS)/%.cpp | $(OBJ)
 	$(CXX) $(CXXFLAGS) -c $< -o $@
 
 $(OBJ)/sv_%.o: $(SOLVER)/%.cpp | $(OBJ)
+	$(CXX) $(CXXFLAGS) -c $< -o $@
+
+$(OBJ)/tinyxml2.o: $(TINYXML) | $(OBJ)
 	$(CXX) $(CXXFLAGS) -c $< -o $@
 
 $(HARN_OBJ): $(HERE)ref_harness.cpp | $(OBJ)

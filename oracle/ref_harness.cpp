@@ -12,12 +12,17 @@
 //                          then the <LS> overrides of read_files.cpp:2056-2090
 #include "ComMod.h"
 #include "FsilsLinearAlgebra.h"
+#include "Simulation.h"
 #include "eq_assem.h"
 #include "fluid.h"
 #include "fs.h"
 #include "fsils_api.hpp"
 #include "lhs.h"
 #include "nn.h"
+#include "output.h"
+#include "utils.h"
+
+#include <mpi.h>
 
 #include <cstdio>
 #include <cstring>
@@ -327,6 +332,54 @@ extern "C" int ref_b_assem_neu_bc(int nEl, const int* IEN, int nNo, const double
     std::memcpy(R, com_mod.R.data(), sizeof(double) * 4 * nNo);
     std::memcpy(Val, com_mod.Val.data(), sizeof(double) * 16 * (size_t)nnz);
     eq.linear_algebra = nullptr;
+  } catch (const std::exception& ex) {
+    g_ref_err = ex.what();
+    return 1;
+  }
+  return 0;
+}
+
+// The reference's own write_restart (output.cpp:202-360, with
+// write_restart_header :361-376) for the fluid equation on one rank: a
+// Simulation whose ComMod holds the given stamp, time step, time, per-equation
+// iNorm, cplBC.xn and Yn/An (tDof, tnNo), no moving mesh, no structure,
+// prestress, CEP or immersed-body fields. Writes <stFileName>_<cTS>.bin and its
+// _last.bin link (or _last.bin alone with stFileRepl), as the reference does.
+// The header's CPU-time field is utils::cput() - timeP[0] at the write.
+extern "C" int ref_write_restart(const char* stFileName, int stFileRepl, int cTS, double time, const int* stamp,
+                                 int nEq, const double* iNorm, int nX, const double* xn, int tnNo, int tDof,
+                                 const double* Yn, const double* An, int recLn)
+{
+  try {
+    int inited = 0;
+    MPI_Initialized(&inited);
+    if (!inited) MPI_Init(nullptr, nullptr);
+    Simulation sim;
+    ComMod& com = sim.com_mod;
+    com.stFileName = stFileName;
+    com.stFileRepl = stFileRepl != 0;
+    com.cTS = cTS;
+    com.time = time;
+    for (int i = 0; i < 7; i++) com.stamp[i] = stamp[i];
+    com.recLn = recLn;
+    com.eq.resize(nEq);
+    for (int i = 0; i < nEq; i++) com.eq[i].iNorm = iNorm[i];
+    com.cplBC.xn.resize(nX);
+    for (int i = 0; i < nX; i++) com.cplBC.xn(i) = xn[i];
+    com.Yn.resize(tDof, tnNo);
+    com.An.resize(tDof, tnNo);
+    for (int a = 0; a < tnNo; a++)
+      for (int i = 0; i < tDof; i++) {
+        com.Yn(i, a) = Yn[i + tDof * a];
+        com.An(i, a) = An[i + tDof * a];
+      }
+    com.ibFlag = false;
+    com.dFlag = false;
+    com.sstEq = false;
+    com.pstEq = false;
+    sim.cep_mod.cepEq = false;
+    std::array<double, 3> timeP{utils::cput(), 0.0, 0.0};
+    output::write_restart(&sim, timeP);
   } catch (const std::exception& ex) {
     g_ref_err = ex.what();
     return 1;

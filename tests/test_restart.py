@@ -2,8 +2,9 @@
 reference's writer and reader fix (output.cpp:202-376, initialize.cpp:78-241,
 the record length of initialize.cpp:503-521): offsets parsed independently
 with struct, round trips, multi-rank records, the stamp check and the file
-naming. The reference ships no restart file, so the layout is pinned by these
-offsets; a GPU test continues a run from a restart file bit for bit."""
+naming, and the bytes and file names of the reference's own write_restart
+(compiled from its sources into oracle/_ref) for the same record; a GPU test
+continues a run from a restart file bit for bit."""
 import os
 import struct
 
@@ -102,3 +103,37 @@ def test_shape_checks():
     r.Yn = r.Yn[:, :3]
     with pytest.raises(ValueError, match="Yn must be"):
         RS.write(os.devnull, r)
+
+
+@pytest.mark.parametrize("replace,cTS,nX", [(False, 7, 0), (False, 1234, 2), (True, 12, 0)])
+def test_bytes_equal_the_reference_writer(tmp_path, replace, cTS, nX):
+    """The reference's own write_restart (compiled from its sources into
+    oracle/_ref) and restart.save, given the same record: the same file names
+    (the numbered file and its _last.bin hard link, or _last.bin alone), the
+    same length and the same bytes everywhere but the CPU-time field (bytes
+    40-47: utils::cput() at the write in the reference); read back by
+    restart.read field for field."""
+    import oracle
+    if not oracle.ref_available():
+        pytest.skip("the reference build (oracle/_ref) is not present")
+    r = _record(tnNo=9, nX=nX, cTS=cTS, seed=cTS)
+    a, b = tmp_path / "ref", tmp_path / "ours"
+    a.mkdir()
+    b.mkdir()
+    oracle.ref_write_restart(str(a / "stFile"), replace, r.cTS, r.time, r.stamp, r.iNorm, r.xn, r.Yn, r.An,
+                             RS.record_length(r.stamp))
+    RS.save(str(b / "stFile"), r, replace=replace)
+    names = sorted(os.listdir(a))
+    assert names == sorted(os.listdir(b))
+    assert names == (["stFile_last.bin"] if replace else sorted([os.path.basename(RS.file_name("x", cTS))
+                                                                  .replace("x", "stFile"), "stFile_last.bin"]))
+    if not replace:
+        for d in (a, b):
+            assert os.path.samefile(d / names[0], d / names[1])  # a hard link
+    ra, rb = (open(d / "stFile_last.bin", "rb").read() for d in (a, b))
+    assert len(ra) == len(rb) == RS.record_length(r.stamp)
+    assert ra[:40] == rb[:40] and ra[48:] == rb[48:]
+    back = RS.read(str(a / "stFile_last.bin"), r.stamp)
+    assert back.cTS == r.cTS and back.time == r.time
+    assert np.array_equal(back.iNorm, r.iNorm) and np.array_equal(back.xn, r.xn)
+    assert np.array_equal(back.Yn, r.Yn) and np.array_equal(back.An, r.An)
