@@ -46,7 +46,7 @@ struct CoarseLevel {
   // [aggOff[q], aggOff[q] + n_q) in an N_q x N_q row-major matrix (N_q = n_q
   // rounded up to GJB, identity on the padding) at element offset blkOff[q];
   // its Gauss-Jordan column panel (N_q x GJB) and row panel (GJB x N_q) at
-  // panOff[q], its pivot block at GJB * GJB * q
+  // panOff[q], its two pivot-block slots at GJB * GJB * 2q
   int nblk = 0, maxN = 0;
   size_t total = 0, panTotal = 0;
   std::vector<int> aggOff, blkN;
@@ -279,7 +279,7 @@ static int coarse_build(const svg_lhs* L, int target, const int* sysNodeOff, int
       (rc = upload(&c->d_panOff, panOff)) || (rc = dev_alloc(&c->d_apVal, (size_t)4 * nAP)) ||
       (rc = dev_alloc(&c->d_apPP, (size_t)nAP)) || (rc = dev_alloc(&c->d_dinv, (size_t)16 * nNo)) ||
       (rc = dev_alloc(&c->d_Ac, c->total)) || (rc = dev_alloc(&c->d_Ai, c->total)) ||
-      (rc = dev_alloc(&c->d_piv, (size_t)GJB * GJB * nblk)) || (rc = dev_alloc(&c->d_cp, c->panTotal)) ||
+      (rc = dev_alloc(&c->d_piv, (size_t)2 * GJB * GJB * nblk)) || (rc = dev_alloc(&c->d_cp, c->panTotal)) ||
       (rc = dev_alloc(&c->d_rp, c->panTotal)) || (rc = dev_alloc(&c->d_rc, (size_t)nAgg)) ||
       (rc = dev_alloc(&c->d_zc, (size_t)nAgg)) || (rc = dev_alloc(&c->d_flag, 1))) {
     coarse_free(c);
@@ -412,42 +412,98 @@ __global__ __launch_bounds__(256) void k_cl_ac(int nSeg, const int* __restrict__
 //   A[I,J] -= cp[I] rp[J]  (I, J != K);  A[I,K] = -cp[I] piv;  A[K,J] = rp[J];  A[K,K] = piv
 // No pivoting: the matrix is definite (every pivot a leading principal minor's
 // ratio, nonzero); a zero or non-finite pivot raises *flag.
-__global__ __launch_bounds__(1024) void k_gj_pivot(const double* __restrict__ A0, const int* __restrict__ blkN,
-                                                   const long long* __restrict__ blkOff, int K,
-                                                   double* __restrict__ piv0, int* __restrict__ flag)
+
+// The pivot block by unpivoted Gauss-Jordan in registers: lane (w, i) of the
+// 256 holds row i, columns 16w..16w+15 (v); step p publishes row p and column
+// p through LDS, double buffered by the step's parity (a thread writing step
+// p + 1's buffer has passed step p's barrier, so every thread has read that
+// buffer at step p - 1), one barrier per step. Block column K of the
+// elimination: a zero or non-finite pivot raises *flag.
+__device__ __forceinline__ void gj_pivot_regs(double (&v)[16], int K, double (&rowb)[2][GJB],
+                                              double (&colb)[2][GJB], int* __restrict__ flag)
 {
-  __shared__ double S[GJB][GJB + 1];
-  const int b = blockIdx.z, N = blkN[b];
-  if (K >= N / GJB) return;
-  const double* A = A0 + blkOff[b];
-  double* piv = piv0 + (size_t)GJB * GJB * b;
-  const int t = threadIdx.x;
-  const size_t base = (size_t)K * GJB * N + (size_t)K * GJB;
-  for (int e = t; e < GJB * GJB; e += 1024) S[e / GJB][e % GJB] = A[base + (size_t)(e / GJB) * N + e % GJB];
-  __syncthreads();
-  for (int p = 0; p < GJB; p++) {
-    const double pv = S[p][p];
-    double nv[GJB * GJB / 1024];
+  static_assert(GJB == 64, "gj_pivot_regs: one row per lane");
+  const int t = threadIdx.x, i = t & 63, w = t >> 6, j0 = 16 * w;
 #pragma unroll
-    for (int q = 0; q < GJB * GJB / 1024; q++) {
-      const int e = t + 1024 * q, i = e / GJB, j = e % GJB;
-      const double sij = S[i][j], sip = S[i][p], spj = S[p][j];
-      const double inv = 1.0 / pv;
-      nv[q] = (i == p) ? ((j == p) ? inv : spj * inv) : ((j == p) ? -sip * inv : sij - sip * (spj * inv));
+  for (int p = 0; p < GJB; p++) {
+    const int buf = p & 1;
+    if (i == p) {
+#pragma unroll
+      for (int jj = 0; jj < 16; jj++) rowb[buf][j0 + jj] = v[jj];
+    }
+    if (w == p / 16) colb[buf][i] = v[p % 16];
+    __syncthreads();
+    const double pv = rowb[buf][p];
+    const double inv = 1.0 / pv;
+    const double sip = colb[buf][i];
+#pragma unroll
+    for (int jj = 0; jj < 16; jj++) {
+      const int j = j0 + jj;
+      const double spj = rowb[buf][j];
+      v[jj] = (i == p) ? ((j == p) ? inv : spj * inv) : ((j == p) ? -sip * inv : v[jj] - sip * (spj * inv));
     }
     if (t == 0 && !(fabs(pv) > 0.0 && isfinite(pv))) *flag = 1 + K * GJB + p;
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < GJB * GJB / 1024; q++) {
-      const int e = t + 1024 * q;
-      S[e / GJB][e % GJB] = nv[q];
-    }
-    __syncthreads();
   }
-  for (int e = t; e < GJB * GJB; e += 1024) piv[e] = S[e / GJB][e % GJB];
 }
 
-// Workgroup J: rp[:, J] = piv A[K, J] (J != K) and cp[J rows, :] = A[J rows, K].
+// Pivot blocks are double buffered by the step's parity: step K's panels and
+// update read slot K & 1 while the update's first workgroup writes step K + 1's.
+__device__ __forceinline__ double* gj_piv_slot(double* piv0, int b, int K)
+{
+  return piv0 + (size_t)GJB * GJB * (2 * b + (K & 1));
+}
+
+// The first step's pivot block (later steps' are inverted by the update).
+__global__ __launch_bounds__(256) void k_gj_pivot(const double* __restrict__ A0, const int* __restrict__ blkN,
+                                                  const long long* __restrict__ blkOff, int K,
+                                                  double* __restrict__ piv0, int* __restrict__ flag)
+{
+  __shared__ double rowb[2][GJB], colb[2][GJB];
+  const int b = blockIdx.z, N = blkN[b];
+  if (K >= N / GJB) return;
+  const double* A = A0 + blkOff[b] + (size_t)K * GJB * N + (size_t)K * GJB;
+  double* piv = gj_piv_slot(piv0, b, K);
+  const int t = threadIdx.x, i = t & 63, j0 = 16 * (t >> 6);
+  double v[16];
+#pragma unroll
+  for (int jj = 0; jj < 16; jj++) v[jj] = A[(size_t)i * N + j0 + jj];
+  gj_pivot_regs(v, K, rowb, colb, flag);
+#pragma unroll
+  for (int jj = 0; jj < 16; jj++) piv[i * GJB + j0 + jj] = v[jj];
+}
+
+typedef double gj_v4d __attribute__((ext_vector_type(4)));
+
+// C (64 x 64) = L R on the FP64 matrix cores, L(r, k) = Lp[r * ls + k * lk_],
+// R(k, c) = Rp[k * rs + c]: wavefront w takes the 32 x 32 quarter (w / 2,
+// w % 2) as 2 x 2 16 x 16 blocks over 16 steps of k = 4 (v_mfma_f64_16x16x4:
+// lane l holds L[l % 16][l / 16], R[l / 16][l % 16] and C[4 r + l / 16][l % 16],
+// r < 4); acc[u][v][r] is C(rb + 16 u + 4 r + l / 16, cb + 16 v + l % 16).
+__device__ __forceinline__ void gj_mfma64(const double* __restrict__ Lp, size_t ls, size_t lk_,
+                                          const double* __restrict__ Rp, size_t rs, gj_v4d (&acc)[2][2])
+{
+  const int t = threadIdx.x, w = t >> 6, l = t & 63, lr = l & 15, lk = l >> 4;
+  const int rb = 32 * (w >> 1), cb = 32 * (w & 1);
+  const double* ap = Lp + (size_t)(rb + lr) * ls + (size_t)lk * lk_;
+  const double* bp = Rp + (size_t)lk * rs + cb + lr;
+#pragma unroll
+  for (int u = 0; u < 2; u++)
+#pragma unroll
+    for (int v = 0; v < 2; v++) acc[u][v] = gj_v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int ks = 0; ks < GJB / 4; ks++) {
+    const double a0 = ap[(size_t)4 * ks * lk_], a1 = ap[(size_t)4 * ks * lk_ + 16 * ls];
+    const double b0 = bp[(size_t)4 * ks * rs], b1 = bp[(size_t)4 * ks * rs + 16];
+    acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+    acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+    acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+    acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+  }
+}
+
+// Workgroup J: the column panel block cpT[:, J rows] = A[J rows, K]^T (the
+// panel stored transposed, GJB x N row-major, so that the update reads it
+// along rows) and rp[:, J] = piv A[K, J] (J != K).
 __global__ __launch_bounds__(256) void k_gj_panels(const double* __restrict__ A0, const int* __restrict__ blkN,
                                                    const long long* __restrict__ blkOff,
                                                    const long long* __restrict__ panOff, int K,
@@ -457,61 +513,50 @@ __global__ __launch_bounds__(256) void k_gj_panels(const double* __restrict__ A0
   const int b = blockIdx.z, N = blkN[b], nb = N / GJB;
   if (K >= nb || (int)blockIdx.x >= nb) return;
   const double* A = A0 + blkOff[b];
-  const double* piv = piv0 + (size_t)GJB * GJB * b;
+  const double* piv = gj_piv_slot(const_cast<double*>(piv0), b, K);
   double* rp = rp0 + panOff[b];
-  double* cp = cp0 + panOff[b];
-  // the left operand is staged transposed (PT[k][r] = piv[r][k]) so that both
-  // operands are read along rows of LDS
-  __shared__ double PT[GJB][GJB];
-  __shared__ double B[GJB][GJB];
+  double* cpT = cp0 + panOff[b];
   const int J = blockIdx.x, t = threadIdx.x;
   for (int e = t; e < GJB * GJB; e += 256) {
-    const int r = e / GJB, c = e % GJB;
-    cp[(size_t)(J * GJB + r) * GJB + c] = A[(size_t)(J * GJB + r) * N + K * GJB + c];
+    const int k = e / GJB, r = e % GJB;
+    cpT[(size_t)k * N + J * GJB + r] = A[(size_t)(J * GJB + r) * N + K * GJB + k];
   }
   if (J == K) return;
-  for (int e = t; e < GJB * GJB; e += 256) {
-    const int r = e / GJB, c = e % GJB;
-    PT[c][r] = piv[e];
-    B[r][c] = A[(size_t)(K * GJB + r) * N + J * GJB + c];
-  }
-  __syncthreads();
-  const int r0 = 4 * (t / 16), c0 = 4 * (t % 16);
-  double acc[4][4] = {};
-  for (int k = 0; k < GJB; k++) {
-    double a[4], b[4];
+  gj_v4d acc[2][2];
+  gj_mfma64(piv, GJB, 1, A + (size_t)K * GJB * N + (size_t)J * GJB, N, acc);
+  const int w = t >> 6, l = t & 63, lr = l & 15, lk = l >> 4, rb = 32 * (w >> 1), cb = 32 * (w & 1);
 #pragma unroll
-    for (int u = 0; u < 4; u++) a[u] = PT[k][r0 + u], b[u] = B[k][c0 + u];
+  for (int u = 0; u < 2; u++)
 #pragma unroll
-    for (int u = 0; u < 4; u++)
+    for (int v = 0; v < 2; v++)
 #pragma unroll
-      for (int v = 0; v < 4; v++) acc[u][v] += a[u] * b[v];
-  }
-#pragma unroll
-  for (int u = 0; u < 4; u++)
-#pragma unroll
-    for (int v = 0; v < 4; v++) rp[(size_t)(r0 + u) * N + J * GJB + c0 + v] = acc[u][v];
+      for (int r = 0; r < 4; r++)
+        rp[(size_t)(rb + 16 * u + 4 * r + lk) * N + J * GJB + cb + 16 * v + lr] = acc[u][v][r];
 }
 
-// Tile (I, J) of the update; blockIdx.x = J, blockIdx.y = I, blockIdx.z the block.
+// Tile (I, J) of the update, the tiles rotated so that workgroup (0, 0) takes
+// (K + 1, K + 1): it finishes its tile first and then inverts it as step
+// K + 1's pivot block while the other tiles are updated. The product cp[I]
+// rp[J] (or cp[I] piv for J = K) runs on the FP64 matrix cores, both operands
+// read straight from the panels (16 consecutive entries per 16 lanes).
 __global__ __launch_bounds__(256) void k_gj_update(double* __restrict__ A0, const int* __restrict__ blkN,
                                                    const long long* __restrict__ blkOff,
                                                    const long long* __restrict__ panOff, int K,
-                                                   const double* __restrict__ piv0, const double* __restrict__ rp0,
-                                                   const double* __restrict__ cp0)
+                                                   double* __restrict__ piv0, const double* __restrict__ rp0,
+                                                   const double* __restrict__ cp0, int* __restrict__ flag)
 {
+  __shared__ double T[GJB][GJB + 1];
+  __shared__ double rowb[2][GJB], colb[2][GJB];
   const int b = blockIdx.z, N = blkN[b], nb = N / GJB;
   if (K >= nb || (int)blockIdx.x >= nb || (int)blockIdx.y >= nb) return;
   double* A = A0 + blkOff[b];
-  const double* piv = piv0 + (size_t)GJB * GJB * b;
+  const double* piv = gj_piv_slot(piv0, b, K);
   const double* rp = rp0 + panOff[b];
-  const double* cp = cp0 + panOff[b];
-  __shared__ double CT[GJB][GJB];  // CT[k][r] = cp[I rows r][k]
-  __shared__ double B[GJB][GJB];
-  const int J = blockIdx.x, I = blockIdx.y, t = threadIdx.x;
-  const int r0 = 4 * (t / 16), c0 = 4 * (t % 16);
+  const double* cpT = cp0 + panOff[b];
+  const int J = (blockIdx.x + K + 1) % nb, I = (blockIdx.y + K + 1) % nb, t = threadIdx.x;
   double* At = A + (size_t)I * GJB * N + (size_t)J * GJB;
   if (I == K) {
+    const int r0 = 4 * (t / 16), c0 = 4 * (t % 16);
 #pragma unroll
     for (int u = 0; u < 4; u++)
 #pragma unroll
@@ -520,29 +565,33 @@ __global__ __launch_bounds__(256) void k_gj_update(double* __restrict__ A0, cons
             (J == K) ? piv[(r0 + u) * GJB + c0 + v] : rp[(size_t)(r0 + u) * N + J * GJB + c0 + v];
     return;
   }
-  for (int e = t; e < GJB * GJB; e += 256) {
-    const int r = e / GJB, c = e % GJB;
-    CT[c][r] = cp[(size_t)(I * GJB + r) * GJB + c];
-    B[r][c] = (J == K) ? piv[e] : rp[(size_t)r * N + J * GJB + c];
-  }
+  const bool diag = (J == K);
+  gj_v4d acc[2][2];
+  gj_mfma64(cpT + (size_t)I * GJB, 1, N, diag ? piv : rp + (size_t)J * GJB, diag ? GJB : N, acc);
+  const int w = t >> 6, l = t & 63, lr = l & 15, lk = l >> 4, rb = 32 * (w >> 1), cb = 32 * (w & 1);
+  const bool next = (I == K + 1 && J == K + 1);
+#pragma unroll
+  for (int u = 0; u < 2; u++)
+#pragma unroll
+    for (int v = 0; v < 2; v++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int row = rb + 16 * u + 4 * r + lk, col = cb + 16 * v + lr;
+        double* a = At + (size_t)row * N + col;
+        const double x = diag ? -acc[u][v][r] : *a - acc[u][v][r];
+        *a = x;
+        if (next) T[row][col] = x;
+      }
+  if (!next) return;
   __syncthreads();
-  double acc[4][4] = {};
-  for (int k = 0; k < GJB; k++) {
-    double a[4], b[4];
+  const int i = t & 63, j0 = 16 * (t >> 6);
+  double vr[16];
 #pragma unroll
-    for (int u = 0; u < 4; u++) a[u] = CT[k][r0 + u], b[u] = B[k][c0 + u];
+  for (int jj = 0; jj < 16; jj++) vr[jj] = T[i][j0 + jj];
+  gj_pivot_regs(vr, K + 1, rowb, colb, flag);
+  double* pn = gj_piv_slot(piv0, b, K + 1);
 #pragma unroll
-    for (int u = 0; u < 4; u++)
-#pragma unroll
-      for (int v = 0; v < 4; v++) acc[u][v] += a[u] * b[v];
-  }
-#pragma unroll
-  for (int u = 0; u < 4; u++)
-#pragma unroll
-    for (int v = 0; v < 4; v++) {
-      double* a = At + (size_t)(r0 + u) * N + c0 + v;
-      *a = (J == K) ? -acc[u][v] : *a - acc[u][v];
-    }
+  for (int jj = 0; jj < 16; jj++) pn[i * GJB + j0 + jj] = vr[jj];
 }
 
 __global__ __launch_bounds__(256) void k_cl_d2f(size_t n, const double* __restrict__ a, float* __restrict__ b)
@@ -568,31 +617,49 @@ __global__ __launch_bounds__(256) void k_cl_restrict(int nAgg, const int* __rest
   if (lane == 0) rc[a] = (float)s;
 }
 
-// zc = Ai rc block by block: aggregate row r of block b = aggBlk[r] is row
-// r - aggOff[b] of the block's N x N FP32 inverse, over the block's n real
-// columns (one wavefront per row; the padding columns read as zero).
-__global__ __launch_bounds__(256) void k_cl_gemv(int nAgg, const int* __restrict__ aggBlk, const int* __restrict__ aggOff,
-                                                 const int* __restrict__ blkN, const long long* __restrict__ blkOff,
-                                                 const float* __restrict__ Ai, const float* __restrict__ rc,
-                                                 float* __restrict__ zc, const int* __restrict__ skip)
+// zc = Ai rc block by block: workgroup (x, y) takes rows 4x..4x+3 of block y
+// (one wavefront per row; row r of the block's N x N FP32 inverse), with the
+// block's rc staged in LDS and zero on the padding columns. Each lane adds its
+// float4 chunks q = lane, lane + 64, ... in order (loaded eight at a time, so
+// that a row costs one or two memory latencies), then a fixed butterfly.
+__global__ __launch_bounds__(256) void k_cl_gemv(const int* __restrict__ aggOff, const int* __restrict__ blkN,
+                                                 const long long* __restrict__ blkOff, const float* __restrict__ Ai,
+                                                 const float* __restrict__ rc, float* __restrict__ zc,
+                                                 const int* __restrict__ skip)
 {
+  extern __shared__ float xs[];
   if (skip && *skip) return;
+  const int bk = blockIdx.y, a0 = aggOff[bk], n = aggOff[bk + 1] - a0, N = blkN[bk];
+  if ((int)blockIdx.x * 4 >= n) return;
+  for (int c0 = threadIdx.x; c0 < N; c0 += 256 * 8) {
+    float xv[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) xv[u] = c0 + 256 * u < n ? rc[a0 + c0 + 256 * u] : 0.f;
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+      if (c0 + 256 * u < N) xs[c0 + 256 * u] = xv[u];
+  }
+  __syncthreads();
   const int r = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-  if (r >= nAgg) return;
-  const int bk = aggBlk[r], a0 = aggOff[bk], n = aggOff[bk + 1] - a0, N = blkN[bk];
-  const float4* row = reinterpret_cast<const float4*>(Ai + blkOff[bk] + (size_t)(r - a0) * N);
-  const float* x = rc + a0;
+  if (r >= n) return;
+  const float4* row = reinterpret_cast<const float4*>(Ai + blkOff[bk] + (size_t)r * N);
+  const float4* x = reinterpret_cast<const float4*>(xs);
+  const int nq = N / 4;
   float s = 0.f;
-  for (int q = lane; q < N / 4; q += 64) {
-    const float4 a = row[q];
-    const int c = 4 * q;
-    const float4 b = make_float4(c < n ? x[c] : 0.f, c + 1 < n ? x[c + 1] : 0.f, c + 2 < n ? x[c + 2] : 0.f,
-                                 c + 3 < n ? x[c + 3] : 0.f);
-    s += a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w;
+  for (int q0 = lane; q0 < nq; q0 += 512) {
+    float4 a[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) a[u] = (q0 + 64 * u < nq) ? row[q0 + 64 * u] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+      if (q0 + 64 * u < nq) {
+        const float4 bb = x[q0 + 64 * u];
+        s += a[u].x * bb.x + a[u].y * bb.y + a[u].z * bb.z + a[u].w * bb.w;
+      }
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-  if (lane == 0) zc[r] = s;
+  if (lane == 0) zc[a0 + r] = s;
 }
 
 // With r2_i = v_i - sum_e AP(:, e) zc(apAgg(e)) (the coarse-corrected residual):
@@ -613,26 +680,50 @@ __global__ __launch_bounds__(256) void k_cl_apply(int nNo, const T* __restrict__
   float r[4];
 #pragma unroll
   for (int d = 0; d < 4; d++) r[d] = (float)v[(size_t)4 * i + d];
-  for (int e = apPtr[i]; e < apPtr[i + 1]; e++) {
-    const float c = zc[apAgg[e]];
-    const float4 a = *reinterpret_cast<const float4*>(apVal + (size_t)4 * e);
-    r[0] -= a.x * c;
-    r[1] -= a.y * c;
-    r[2] -= a.z * c;
-    r[3] -= a.w * c;
+  // the block-Jacobi inverse and the own aggregate's entry do not depend on
+  // the loop: their loads are issued before it
+  float4 D[4];
+  const float4* Dp = reinterpret_cast<const float4*>(dinv + (size_t)16 * i);
+  if (mode != 1) {
+#pragma unroll
+    for (int d = 0; d < 4; d++) D[d] = Dp[d];
+  }
+  const float zown = mode == 0 ? zc[agg[i]] : 0.f;
+  // the AP entries in order, their loads issued four at a time
+  const int eb = apPtr[i], ee = apPtr[i + 1];
+  for (int e = eb; e < ee; e += 4) {
+    int g[4];
+    float4 a[4];
+    float c[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (e + u < ee) {
+        g[u] = apAgg[e + u];
+        a[u] = *reinterpret_cast<const float4*>(apVal + (size_t)4 * (e + u));
+      }
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (e + u < ee) c[u] = zc[g[u]];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (e + u < ee) {
+        r[0] -= a[u].x * c[u];
+        r[1] -= a[u].y * c[u];
+        r[2] -= a[u].z * c[u];
+        r[3] -= a[u].w * c[u];
+      }
   }
   float z[4];
   if (mode == 1) {
 #pragma unroll
     for (int d = 0; d < 4; d++) z[d] = r[d];
   } else {
-    const float4* D = reinterpret_cast<const float4*>(dinv + (size_t)16 * i);
 #pragma unroll
     for (int d = 0; d < 4; d++) {
       const float4 m = D[d];
       z[d] = m.x * r[0] + m.y * r[1] + m.z * r[2] + m.w * r[3];
     }
-    if (mode == 0) z[3] += zc[agg[i]];
+    if (mode == 0) z[3] += zown;
   }
   if (z32) *reinterpret_cast<float4*>(z32 + (size_t)4 * i) = make_float4(z[0], z[1], z[2], z[3]);
   if (z64) {
@@ -685,15 +776,15 @@ void coarse_enqueue(svg_lhs* L, const double* Val, hipStream_t s, bool invert)
          (const int*)c->d_aggOff, (const int*)c->d_blkN, (const long long*)c->d_blkOff, c->d_Ac, c->d_flag);
   launch("cl_ac", k_cl_ac, dim3(blocks_for(c->nSeg, 256)), dim3(256), 0, s, c->nSeg, (const int*)c->d_segPtr,
          (const long long*)c->d_segIdx, (const int*)c->d_segPerm, (const double*)c->d_apPP, c->d_Ac);
+  launch("gj_pivot", k_gj_pivot, dim3(1, 1, nblk), dim3(256), 0, s, (const double*)c->d_Ac, (const int*)c->d_blkN,
+         (const long long*)c->d_blkOff, 0, c->d_piv, c->d_flag);
   for (int K = 0; K < nb; K++) {
-    launch("gj_pivot", k_gj_pivot, dim3(1, 1, nblk), dim3(1024), 0, s, (const double*)c->d_Ac,
-           (const int*)c->d_blkN, (const long long*)c->d_blkOff, K, c->d_piv, c->d_flag);
     launch("gj_panels", k_gj_panels, dim3(nb, 1, nblk), dim3(256), 0, s, (const double*)c->d_Ac,
            (const int*)c->d_blkN, (const long long*)c->d_blkOff, (const long long*)c->d_panOff, K,
            (const double*)c->d_piv, c->d_rp, c->d_cp);
     launch("gj_update", k_gj_update, dim3(nb, nb, nblk), dim3(256), 0, s, c->d_Ac, (const int*)c->d_blkN,
-           (const long long*)c->d_blkOff, (const long long*)c->d_panOff, K, (const double*)c->d_piv,
-           (const double*)c->d_rp, (const double*)c->d_cp);
+           (const long long*)c->d_blkOff, (const long long*)c->d_panOff, K, c->d_piv, (const double*)c->d_rp,
+           (const double*)c->d_cp, c->d_flag);
   }
   launch("cl_d2f", k_cl_d2f, dim3(2048), dim3(256), 0, s, c->total, (const double*)c->d_Ac, c->d_Ai);
 }
@@ -728,7 +819,7 @@ static void coarse_apply_t(const svg_lhs* L, const T* v, double* z64, float* z32
   const int nAgg = c->nAgg;
   launch("cl_restrict", k_cl_restrict<T>, dim3(blocks_for(nAgg, 4)), dim3(256), 0, s, nAgg,
          (const int*)c->d_aggPtr, (const int*)c->d_aggNodes, v, c->d_rc, skip);
-  launch("cl_gemv", k_cl_gemv, dim3(blocks_for(nAgg, 4)), dim3(256), 0, s, nAgg, (const int*)c->d_aggBlk,
+  launch("cl_gemv", k_cl_gemv, dim3(blocks_for(c->maxN, 4), c->nblk), dim3(256), sizeof(float) * c->maxN, s,
          (const int*)c->d_aggOff, (const int*)c->d_blkN, (const long long*)c->d_blkOff, (const float*)c->d_Ai,
          (const float*)c->d_rc, c->d_zc, skip);
   launch("cl_apply", k_cl_apply<T>, dim3(blocks_for(L->nNo, 256)), dim3(256), 0, s, L->nNo, v,

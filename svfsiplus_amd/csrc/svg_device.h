@@ -28,6 +28,11 @@ int hip_check(hipError_t e, const char* what);
 // reports), not around the launch call with the gap before it.
 int prof_begin(const char* name, hipStream_t s, double bytes, hipEvent_t* start, hipEvent_t* stop);
 bool prof_set(bool on);  // returns the previous setting (event timing off under graph capture)
+// Position in the list of timed launches not yet read back, and removal of the
+// timed launches between two positions (launches that found their solve
+// finished and exited at once: their durations are not the kernel's).
+size_t prof_mark();
+void prof_drop(size_t from, size_t to);
 
 template <typename K, typename... Args>
 inline void launch_b(const char* name, double bytes, K kernel, dim3 grid, dim3 block, size_t shmem, hipStream_t s,

@@ -605,6 +605,10 @@ __device__ __forceinline__ void fin_body(const FinArgs& A, const double* __restr
         if (e0 <= A.absTol) {
           st->early = 1;
           st->done = 1;
+          if (host_done) {
+            *host_done = 1;
+            __threadfence_system();
+          }
           return;
         }
         st->iNorm = e0;
@@ -617,11 +621,15 @@ __device__ __forceinline__ void fin_body(const FinArgs& A, const double* __restr
         st->suc = 1;
         st->done = 1;
         st->fNorm = e0;
-        return;
-      }
-      if (e0 == 0.0) {
+      } else if (e0 == 0.0) {
         st->divzero = 1;
         st->done = 1;
+      }
+      // a cycle that starts done is skipped by every launch: the host stops
+      // enqueueing it at its first look at the flag
+      if (st->done && host_done) {
+        *host_done = 1;
+        __threadfence_system();
       }
     }
     return;
@@ -1497,7 +1505,10 @@ static int run_cycle(const Op& A, KrylovWork& k, int sD, FinArgs fa, double* X, 
   // only where a step reads it: the network's input, a coupled face's term
   const bool coupled = mixed && has_coupled(A.L);
   auto netStep = [&](int j) { return net && (net_period <= 1 || j % net_period == 0); };
-  *k.h_done = 0;
+  // k.h_done was cleared by the caller before its restart check, which raises
+  // it when the cycle starts done (converged true residual, zero residual)
+  const size_t mark0 = prof_mark();
+  size_t postFin[MAXM];  // timed-launch positions before each step's Gram-Schmidt update
   for (int i = 0; i < sD; i++) {
     if (i >= LOOKAHEAD) {
       SVG_HIP(hipEventSynchronize(k.ev[(i - LOOKAHEAD) % 8]));
@@ -1548,6 +1559,7 @@ static int run_cycle(const Op& A, KrylovWork& k, int sD, FinArgs fa, double* X, 
       if (coupled) add_bc_mul_dev(A.L, 4, sin, w, (const int*)&k.st->done, s);  // FP64 rank-one term
       int rc = dot_device_cb(n, w, k.v32, k.ld, i + 1, k, fa, s);
       if (rc) return rc;
+      postFin[i] = prof_mark();
       const int keep64 = (!tl && (coupled || (i + 1 < sD && netStep(i + 1)))) ? 1 : 0;
       // algorithmic bytes: w and v_0..v_i read, v_{i+1} (FP32) written, w rewritten with keep64
       launch_b("gs_update_cb", 8.0 * n + 4.0 * n * (i + 2) + (keep64 ? 8.0 * n : 0.0), k_gs_update_cb, dim3(nbv2),
@@ -1557,6 +1569,7 @@ static int run_cycle(const Op& A, KrylovWork& k, int sD, FinArgs fa, double* X, 
       int rc = dot_device(n, w, k.u, k.ld, i + 2, k.partial, k.scal, k.st, fa, k.h, k.cs, k.sn, k.err, k.coef, s,
                           k.d_done);
       if (rc) return rc;
+      postFin[i] = prof_mark();
       // flexible: when the next step has no network, this update also writes z_{i+1} = u_{i+1}
       const bool nextPlain = flex && !tl && i + 1 < sD && !netStep(i + 1);
       // algorithmic bytes: w and u_0..u_i read, u_{i+1} written (and z_{i+1})
@@ -1566,6 +1579,7 @@ static int run_cycle(const Op& A, KrylovWork& k, int sD, FinArgs fa, double* X, 
     }
     SVG_HIP(hipEventRecord(k.ev[i % 8], s));
   }
+  const size_t markEnd = prof_mark();
   launch("backsolve", k_backsolve, dim3(1), dim3(256), 0, s, sD, (const double*)k.h, (const double*)k.err, k.y, k.st);
   if (mixed)
     launch("xupdate_cb", k_xupdate_cb, dim3(nbv2), dim3(256), 0, s, n, X, (const float*)k.v32,
@@ -1577,6 +1591,14 @@ static int run_cycle(const Op& A, KrylovWork& k, int sD, FinArgs fa, double* X, 
   SVG_HIP(hipGetLastError());
   SVG_HIP(hipMemcpyAsync(k.h_st, k.st, sizeof(GmState), hipMemcpyDeviceToHost, s));
   SVG_HIP(hipStreamSynchronize(s));
+  // timed launches that exited at once are not kept: the whole loop when the
+  // cycle started done, else those from the converging (or falling back)
+  // step's update on
+  const GmState& h = *k.h_st;
+  if (h.tconv || h.early || h.divzero)
+    prof_drop(mark0, markEnd);
+  else if (h.done && h.last_i >= 0 && h.last_i < sD)
+    prof_drop(postFin[h.last_i], markEnd);
   return SVG_OK;
 }
 
@@ -1616,11 +1638,13 @@ static int gmres_v_dev(svg_lhs* L, svg_subls* ls, int dof, const double* Val, do
   double dB = ls->fNorm;
   for (int l = 0; l < ls->mItr; l++) {
     dB = ls->fNorm;
+    *k.h_done = 0;
     op_apply(A, k.X, k.u, R, nullptr, s);  // u0 = R - A X
     fa.mode = FIN_RESTART;
     fa.inc = 1;
     fa.first = 0;
-    if ((rc = dot_device(n, k.u, k.u, 0, 1, k.partial, k.scal, k.st, fa, k.h, k.cs, k.sn, k.err, k.coef, s)))
+    if ((rc = dot_device(n, k.u, k.u, 0, 1, k.partial, k.scal, k.st, fa, k.h, k.cs, k.sn, k.err, k.coef, s,
+                         k.d_done)))
       return rc;
     launch("div", k_div, dim3(nbv), dim3(256), 0, s, n, k.u, (const GmState*)k.st, (float*)nullptr);
     if ((rc = run_cycle(A, k, sD, fa, k.X, s))) return rc;
@@ -1680,12 +1704,14 @@ static int gmres_v_mixed(svg_lhs* L, svg_subls* ls, const double* Val, double* R
   double dB = ls->fNorm;
   for (int l = 0; l < ls->mItr; l++) {
     dB = ls->fNorm;
+    *k.h_done = 0;
     op_apply(A, k.X, k.u, R, nullptr, s);  // FP64 true residual
     fa.mode = FIN_RESTART;
     fa.inc = 1;
     fa.first = 0;
     fa.tcheck = 1;
-    if ((rc = dot_device(n, k.u, k.u, 0, 1, k.partial, k.scal, k.st, fa, k.h, k.cs, k.sn, k.err, k.coef, s)))
+    if ((rc = dot_device(n, k.u, k.u, 0, 1, k.partial, k.scal, k.st, fa, k.h, k.cs, k.sn, k.err, k.coef, s,
+                         k.d_done)))
       return rc;
     launch("div", k_div, dim3(nbv), dim3(256), 0, s, n, k.u, (const GmState*)k.st, k.v32);  // u_0 and v_0
     if ((rc = run_cycle(A, k, sD, fa, k.X, s, L->d_val32))) return rc;
@@ -1766,12 +1792,14 @@ static int gmres_v_learned(svg_lhs* L, svg_subls* ls, const double* Val, double*
   fa.fbwin = window;
   for (int l = 0; l < ls->mItr; l++) {
     dB = ls->fNorm;
+    *k.h_done = 0;
     op_apply(A, k.X, k.u, R, nullptr, s);  // FP64 true residual
     fa.mode = FIN_RESTART;
     fa.inc = 1;
     fa.first = 0;
     fa.tcheck = 1;
-    if ((rc = dot_device(n, k.u, k.u, 0, 1, k.partial, k.scal, k.st, fa, k.h, k.cs, k.sn, k.err, k.coef, s)))
+    if ((rc = dot_device(n, k.u, k.u, 0, 1, k.partial, k.scal, k.st, fa, k.h, k.cs, k.sn, k.err, k.coef, s,
+                         k.d_done)))
       return rc;
     launch("div", k_div, dim3(nbv), dim3(256), 0, s, n, k.u, (const GmState*)k.st,
            mixed ? k.v32 : (float*)nullptr);
@@ -1827,6 +1855,7 @@ static int gmres_dev(const Op& A, KrylovWork& k, svg_subls* ls, const double* R,
   launch("reset_state", k_reset_state, dim3(1), dim3(64), 0, s, k.st);
   int itr0 = ls->itr;
   for (int l = 0; l < ls->mItr; l++) {
+    *k.h_done = 0;
     if (l == 0) {
       SVG_HIP(hipMemcpyAsync(k.u, R, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
       if (A.pre) add_bc_pre_dev(A.L, A.dof, k.u, k.u, nullptr, s);  // gmres.cpp:142-150
@@ -1836,7 +1865,8 @@ static int gmres_dev(const Op& A, KrylovWork& k, svg_subls* ls, const double* R,
     fa.mode = FIN_RESTART;
     fa.inc = l == 0 ? 0 : 1;
     fa.first = l == 0 ? 1 : 0;
-    if ((rc = dot_device(n, k.u, k.u, 0, 1, k.partial, k.scal, k.st, fa, k.h, k.cs, k.sn, k.err, k.coef, s)))
+    if ((rc = dot_device(n, k.u, k.u, 0, 1, k.partial, k.scal, k.st, fa, k.h, k.cs, k.sn, k.err, k.coef, s,
+                         k.d_done)))
       return rc;
     launch("div", k_div, dim3(nbv), dim3(256), 0, s, n, k.u, (const GmState*)k.st, (float*)nullptr);
     double dBprev = ls->fNorm;

@@ -83,6 +83,24 @@ int prof_begin(const char* name, hipStream_t s, double bytes, hipEvent_t* start,
   return (int)g_pending.size() - 1;
 }
 
+size_t prof_mark()
+{
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  return g_pending.size();
+}
+
+void prof_drop(size_t from, size_t to)
+{
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  if (to > g_pending.size()) to = g_pending.size();
+  if (from >= to) return;
+  for (size_t q = from; q < to; q++) {
+    g_event_pool.push_back(g_pending[q].a);
+    g_event_pool.push_back(g_pending[q].b);
+  }
+  g_pending.erase(g_pending.begin() + from, g_pending.begin() + to);
+}
+
 static void prof_flush()
 {
   std::lock_guard<std::mutex> lk(g_prof_mu);
