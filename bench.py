@@ -19,7 +19,11 @@ SpMV. The default, "compare", times plain, twolevel and learned back to back:
 `speedup_vs_plain` and `network_gain` stand beside it. The network is loaded
 from svfsiplus_amd/data/lpc_smoother.npz (trained on the GPU by
 `python -m svfsiplus_amd.lpc --coarse 128`), or trained here within a bounded
-budget (outside the timed region) when that file is absent.
+budget (outside the timed region) when that file is absent. On the 1.2M-tet
+pipe twolevel and learned use aggregates of about 64 nodes and learned applies
+the network on every 8th step (the fastest of a sweep over 32-256 nodes and
+periods 5-16); the stenosed 4.2M-tet leg uses 128 nodes (64 would give more
+aggregates than the dense coarse block allows) and every 5th step.
 
 Further legs of the default run: the mixed-precision draws of viscosity and
 inflow (diagonal preconditioner, C3), the 4.2M-tet stenosed pipe (C4: twolevel
@@ -568,8 +572,11 @@ def c4_run(a, lib, net, log):
     out = {"workload": f"stenosed VMS pipe n={a.c4_n} (depth {a.c4_sten:g} over z in [1.25, 1.75]): {pipe.nEl} tets, "
                        f"{pipe.nNo} nodes, nnzb {pipe.nnz}; 1 time step x 3 Newton iterations; GMRES({a.krylov}) "
                        f"rel-tol {a.tol:g}"}
+    ac = argparse.Namespace(**vars(a))
+    ac.coarse, ac.net_period = a.c4_coarse, a.c4_net_period
+    out["coarse"], out["net_period"] = ac.coarse, ac.net_period
     for method in ("twolevel", "learned"):
-        configure(fn, method, a, net)
+        configure(fn, method, ac, net)
         els = []
         for _ in range(1 + a.c4_steps):
             fn.reset()
@@ -716,13 +723,13 @@ def main():
     ap.add_argument("--lpc-weights", default=None,
                     help="network weights (default: the two-level smoother's, or the stand-alone network's when "
                          "--coarse 0)")
-    ap.add_argument("--coarse", type=int, default=128,
+    ap.add_argument("--coarse", type=int, default=64,
                     help="two-level preconditioner of twolevel/learned: node aggregates of about this many nodes "
                          "(0: learned is the stand-alone network after the diagonal scaling)")
     ap.add_argument("--coarse-keep", type=int, default=1,
                     help="1: Newton iterations 2.. of a time step keep the coarse inverse of the first "
                          "(A P and block Jacobi rebuilt every iteration); 0: rebuild it every iteration")
-    ap.add_argument("--net-period", type=int, default=5,
+    ap.add_argument("--net-period", type=int, default=8,
                     help="learned: the network preconditions every net-period-th Arnoldi step, the others the "
                          "two-level preconditioner with block Jacobi (or, with --coarse 0, the diagonal scaling)")
     ap.add_argument("--c4", type=int, default=1, help="run the stenosed 4.2M-tet leg (0: skip it)")
@@ -732,6 +739,9 @@ def main():
     ap.add_argument("--c4-sten", type=float, default=0.5)
     ap.add_argument("--c4-steps", type=int, default=1)
     ap.add_argument("--c4-plain-restarts", type=int, default=300)
+    ap.add_argument("--c4-coarse", type=int, default=128,
+                    help="stenosed leg: nodes per aggregate (733k nodes: 64 would exceed the dense coarse limit)")
+    ap.add_argument("--c4-net-period", type=int, default=5)
     ap.add_argument("--train-draws", type=int, default=40)
     ap.add_argument("--train-steps", type=int, default=1500)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
