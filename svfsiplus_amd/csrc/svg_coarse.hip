@@ -617,11 +617,21 @@ __global__ __launch_bounds__(256) void k_cl_restrict(int nAgg, const int* __rest
   if (lane == 0) rc[a] = (float)s;
 }
 
+// The streams read once per application (the coarse inverse, the
+// block-Jacobi inverses) use non-temporal loads, leaving the memory-side cache
+// to the block matrix the Arnoldi step's SpMV reads again.
+typedef float cl_f4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ cl_f4 nt_f4(const float4* p)
+{
+  return __builtin_nontemporal_load(reinterpret_cast<const cl_f4*>(p));
+}
+
 // zc = Ai rc block by block: workgroup (x, y) takes rows 4x..4x+3 of block y
 // (one wavefront per row; row r of the block's N x N FP32 inverse), with the
 // block's rc staged in LDS and zero on the padding columns. Each lane adds its
-// float4 chunks q = lane, lane + 64, ... in order (loaded eight at a time, so
-// that a row costs one or two memory latencies), then a fixed butterfly.
+// float4 chunks q = lane, lane + 64, ... in order, then a fixed butterfly. The
+// row's first 16 chunks per lane (N <= 4096) are loaded before the staging,
+// so that their latency overlaps it; longer rows continue eight at a time.
 __global__ __launch_bounds__(256) void k_cl_gemv(const int* __restrict__ aggOff, const int* __restrict__ blkN,
                                                  const long long* __restrict__ blkOff, const float* __restrict__ Ai,
                                                  const float* __restrict__ rc, float* __restrict__ zc,
@@ -631,6 +641,13 @@ __global__ __launch_bounds__(256) void k_cl_gemv(const int* __restrict__ aggOff,
   if (skip && *skip) return;
   const int bk = blockIdx.y, a0 = aggOff[bk], n = aggOff[bk + 1] - a0, N = blkN[bk];
   if ((int)blockIdx.x * 4 >= n) return;
+  const int r = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const bool live = r < n;
+  const float4* row = reinterpret_cast<const float4*>(Ai + blkOff[bk] + (size_t)(live ? r : 0) * N);
+  const int nq = N / 4;
+  cl_f4 a[16];
+#pragma unroll
+  for (int u = 0; u < 16; u++) a[u] = (live && lane + 64 * u < nq) ? nt_f4(row + lane + 64 * u) : cl_f4{0.f, 0.f, 0.f, 0.f};
   for (int c0 = threadIdx.x; c0 < N; c0 += 256 * 8) {
     float xv[8];
 #pragma unroll
@@ -640,21 +657,24 @@ __global__ __launch_bounds__(256) void k_cl_gemv(const int* __restrict__ aggOff,
       if (c0 + 256 * u < N) xs[c0 + 256 * u] = xv[u];
   }
   __syncthreads();
-  const int r = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-  if (r >= n) return;
-  const float4* row = reinterpret_cast<const float4*>(Ai + blkOff[bk] + (size_t)r * N);
+  if (!live) return;
   const float4* x = reinterpret_cast<const float4*>(xs);
-  const int nq = N / 4;
   float s = 0.f;
-  for (int q0 = lane; q0 < nq; q0 += 512) {
-    float4 a[8];
 #pragma unroll
-    for (int u = 0; u < 8; u++) a[u] = (q0 + 64 * u < nq) ? row[q0 + 64 * u] : make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int u = 0; u < 16; u++)
+    if (lane + 64 * u < nq) {
+      const float4 bb = x[lane + 64 * u];
+      s += a[u].x * bb.x + a[u].y * bb.y + a[u].z * bb.z + a[u].w * bb.w;
+    }
+  for (int q0 = lane + 1024; q0 < nq; q0 += 512) {
+    cl_f4 c[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) c[u] = (q0 + 64 * u < nq) ? nt_f4(row + q0 + 64 * u) : cl_f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int u = 0; u < 8; u++)
       if (q0 + 64 * u < nq) {
         const float4 bb = x[q0 + 64 * u];
-        s += a[u].x * bb.x + a[u].y * bb.y + a[u].z * bb.z + a[u].w * bb.w;
+        s += c[u].x * bb.x + c[u].y * bb.y + c[u].z * bb.z + c[u].w * bb.w;
       }
   }
 #pragma unroll
@@ -682,11 +702,11 @@ __global__ __launch_bounds__(256) void k_cl_apply(int nNo, const T* __restrict__
   for (int d = 0; d < 4; d++) r[d] = (float)v[(size_t)4 * i + d];
   // the block-Jacobi inverse and the own aggregate's entry do not depend on
   // the loop: their loads are issued before it
-  float4 D[4];
+  cl_f4 D[4];
   const float4* Dp = reinterpret_cast<const float4*>(dinv + (size_t)16 * i);
   if (mode != 1) {
 #pragma unroll
-    for (int d = 0; d < 4; d++) D[d] = Dp[d];
+    for (int d = 0; d < 4; d++) D[d] = nt_f4(Dp + d);
   }
   const float zown = mode == 0 ? zc[agg[i]] : 0.f;
   // the AP entries in order, their loads issued four at a time
@@ -720,7 +740,7 @@ __global__ __launch_bounds__(256) void k_cl_apply(int nNo, const T* __restrict__
   } else {
 #pragma unroll
     for (int d = 0; d < 4; d++) {
-      const float4 m = D[d];
+      const cl_f4 m = D[d];
       z[d] = m.x * r[0] + m.y * r[1] + m.z * r[2] + m.w * r[3];
     }
     if (mode == 0) z[3] += zown;

@@ -384,6 +384,15 @@ constexpr double FB_DROP = 1e-4;  // learned preconditioner: smallest relative d
 // are not used) and G columns are loaded and reduced together, so one memory
 // latency is spent per G columns (FP32 columns, half the bytes of FP64 ones,
 // need more in flight to keep HBM busy).
+// Basis entries of the dots. The FP32 (compressed) basis of the mixed and
+// flexible solves is streamed with non-temporal loads (and written with
+// non-temporal stores by the update): its passes then leave the memory-side
+// cache to the FP32 block matrix, which the next Arnoldi step's SpMV reads
+// again (in-solve SpMV 45 -> 39 us on the 1.2M-tet pipe). The FP64 basis of
+// the plain solve keeps ordinary loads (its matrix does not fit that cache).
+__device__ __forceinline__ float basis_load(const float* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ double basis_load(const double* p) { return *p; }
+
 template <int G, typename T>
 __device__ __forceinline__ void span_dots(const double* __restrict__ w, const T* __restrict__ base, size_t ld, int m,
                                           bool self, size_t k0, size_t kend, double* lds, int mc)
@@ -412,7 +421,7 @@ __device__ __forceinline__ void span_dots(const double* __restrict__ w, const T*
 #pragma unroll
     for (int q = 0; q < G; q++)
 #pragma unroll
-      for (int r = 0; r < RED_EPT; r++) c[q][r] = base[(size_t)(j + q) * ld + kc[r]];
+      for (int r = 0; r < RED_EPT; r++) c[q][r] = basis_load(base + (size_t)(j + q) * ld + kc[r]);
     double acc[G];
 #pragma unroll
     for (int q = 0; q < G; q++) {
@@ -435,7 +444,7 @@ __device__ __forceinline__ void span_dots(const double* __restrict__ w, const T*
   for (; j < m; j++) {
     T c[RED_EPT];
 #pragma unroll
-    for (int r = 0; r < RED_EPT; r++) c[r] = base[(size_t)j * ld + kc[r]];
+    for (int r = 0; r < RED_EPT; r++) c[r] = basis_load(base + (size_t)j * ld + kc[r]);
     double acc = 0.0;
 #pragma unroll
     for (int r = 0; r < RED_EPT; r++) {
@@ -732,15 +741,20 @@ __global__ __launch_bounds__(256) void k_gs_update_cb(int n, double* __restrict_
   const size_t k = 2 * ((size_t)blockIdx.x * blockDim.x + threadIdx.x);
   if (k >= (size_t)n) return;
   double2 v = *reinterpret_cast<const double2*>(w + k);
+  // the FP32 basis streamed non-temporally (see basis_load)
   for (int j = 0; j <= i; j++) {
-    const float2 b = *reinterpret_cast<const float2*>(v32 + (size_t)j * ld + k);
+    const unsigned long long bb =
+        __builtin_nontemporal_load(reinterpret_cast<const unsigned long long*>(v32 + (size_t)j * ld + k));
+    const float2 b = make_float2(__uint_as_float((unsigned)bb), __uint_as_float((unsigned)(bb >> 32)));
     const double c = -coef[j];
     v.x = v.x + c * (double)b.x;
     v.y = v.y + c * (double)b.y;
   }
   v.x = st->scale * v.x;
   v.y = st->scale * v.y;
-  *reinterpret_cast<float2*>(v32 + (size_t)(i + 1) * ld + k) = make_float2((float)v.x, (float)v.y);
+  const unsigned long long o = (unsigned long long)__float_as_uint((float)v.x) |
+                               ((unsigned long long)__float_as_uint((float)v.y) << 32);
+  __builtin_nontemporal_store(o, reinterpret_cast<unsigned long long*>(v32 + (size_t)(i + 1) * ld + k));
   if (keep64) *reinterpret_cast<double2*>(w + k) = v;
 }
 
