@@ -720,7 +720,15 @@ __global__ __launch_bounds__(256) void k_gs_update(int n, double* __restrict__ u
   if (k >= (size_t)n) return;
   double* w = u + (size_t)(i + 1) * ld;
   double v = w[k];
-  for (int j = 0; j <= i; j++) v = v + (-coef[j]) * u[(size_t)j * ld + k];
+  // eight vectors' loads issued before their (sequential) updates
+  for (int j0 = 0; j0 <= i; j0 += 8) {
+    double b[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) b[q] = j0 + q <= i ? u[(size_t)(j0 + q) * ld + k] : 0.0;
+#pragma unroll
+    for (int q = 0; q < 8; q++)
+      if (j0 + q <= i) v = v + (-coef[j0 + q]) * b[q];
+  }
   v = st->scale * v;
   w[k] = v;
   if (u32) u32[k] = (float)v;
@@ -741,14 +749,22 @@ __global__ __launch_bounds__(256) void k_gs_update_cb(int n, double* __restrict_
   const size_t k = 2 * ((size_t)blockIdx.x * blockDim.x + threadIdx.x);
   if (k >= (size_t)n) return;
   double2 v = *reinterpret_cast<const double2*>(w + k);
-  // the FP32 basis streamed non-temporally (see basis_load)
-  for (int j = 0; j <= i; j++) {
-    const unsigned long long bb =
-        __builtin_nontemporal_load(reinterpret_cast<const unsigned long long*>(v32 + (size_t)j * ld + k));
-    const float2 b = make_float2(__uint_as_float((unsigned)bb), __uint_as_float((unsigned)(bb >> 32)));
-    const double c = -coef[j];
-    v.x = v.x + c * (double)b.x;
-    v.y = v.y + c * (double)b.y;
+  // the FP32 basis streamed non-temporally (see basis_load), eight vectors'
+  // loads issued before their (sequential) updates
+  for (int j0 = 0; j0 <= i; j0 += 8) {
+    unsigned long long bb[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++)
+      bb[q] = j0 + q <= i ? __builtin_nontemporal_load(
+                                reinterpret_cast<const unsigned long long*>(v32 + (size_t)(j0 + q) * ld + k))
+                          : 0ull;
+#pragma unroll
+    for (int q = 0; q < 8; q++)
+      if (j0 + q <= i) {
+        const double c = -coef[j0 + q];
+        v.x = v.x + c * (double)__uint_as_float((unsigned)bb[q]);
+        v.y = v.y + c * (double)__uint_as_float((unsigned)(bb[q] >> 32));
+      }
   }
   v.x = st->scale * v.x;
   v.y = st->scale * v.y;

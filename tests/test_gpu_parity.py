@@ -57,6 +57,36 @@ def test_spar_mul_vv_bitwise(torch_cuda, pipe4):
         assert np.array_equal(got, ref), f"dof={dof}"
 
 
+@pytest.mark.parametrize("n", [4, 8])
+def test_every_spmv_variant_bitwise(torch_cuda, n):
+    """Every launch variant the FP64 SpMV autotuner may pick (unroll depth,
+    non-temporal streaming, XCD row bands) gives the oracle's product bit for
+    bit: the tuner chooses on time alone, so any of them may run the solve."""
+    import ctypes as C
+    from svfsiplus_amd.newton import DeviceLhs, _stream
+    torch = torch_cuda
+    pipe = W.make_pipe(n)
+    rng = np.random.default_rng(11)
+    L = DeviceLhs(pipe)
+    K = rng.standard_normal((pipe.nnz, 16))
+    U = rng.standard_normal((pipe.nNo, 4))
+    ref = oracle.spar_mul_vv(pipe, 4, K, U)
+    dK, dU = _dev(torch, K), _dev(torch, U)
+    lib = _lib.load()
+    nv = C.c_int()
+    _lib.check(lib.svg_spmv_num_variants(C.byref(nv)))
+    assert nv.value >= 12
+    for v in range(nv.value):
+        dKU = torch.full((pipe.nNo, 4), float("nan"), dtype=torch.float64, device="cuda")
+        us = C.c_double()
+        _lib.check(lib.svg_spmv_time_variant(L.h, v, dK.data_ptr(), dU.data_ptr(), dKU.data_ptr(), 1, C.byref(us),
+                                             _stream()))
+        torch.cuda.synchronize()
+        got = dKU.cpu().numpy()
+        assert np.array_equal(got, ref), f"variant {v}: max diff {np.nanmax(np.abs(got - ref))}"
+    L.free()
+
+
 def test_precond_diag_bitwise(torch_cuda, pipe4, phys):
     from svfsiplus_amd.newton import DeviceLhs, _stream
     torch = torch_cuda
