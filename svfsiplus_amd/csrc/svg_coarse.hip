@@ -434,7 +434,12 @@ __device__ __forceinline__ void gj_pivot_regs(double (&v)[16], int K, double (&r
     if (w == p / 16) colb[buf][i] = v[p % 16];
     __syncthreads();
     const double pv = rowb[buf][p];
-    const double inv = 1.0 / pv;
+    // 1 / pv: the hardware reciprocal refined by two Newton steps (a shorter
+    // dependent chain than the correctly rounded division; this is the
+    // elimination's critical path)
+    double inv = __builtin_amdgcn_rcp(pv);
+    inv = fma(inv, fma(-pv, inv, 1.0), inv);
+    inv = fma(inv, fma(-pv, inv, 1.0), inv);
     const double sip = colb[buf][i];
 #pragma unroll
     for (int jj = 0; jj < 16; jj++) {
