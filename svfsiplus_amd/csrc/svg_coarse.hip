@@ -608,14 +608,24 @@ __global__ __launch_bounds__(256) void k_cl_d2f(size_t n, const double* __restri
 // ----------------------------------------------------- application kernels --
 // rc(a) = sum of v(3, i) over the nodes i of aggregate a (one wavefront per
 // aggregate: lanes at stride 64 in node order, then a fixed butterfly).
+// The batched solve's gate (st non-null): the work of a system that does not
+// iterate in this cycle (converged before it, or during it) is skipped.
+__device__ __forceinline__ bool sys_idle(const GmState* st, int sys)
+{
+  const GmState& S = st[sys];
+  return !S.act || S.done;
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_cl_restrict(int nAgg, const int* __restrict__ aggPtr,
                                                      const int* __restrict__ aggNodes, const T* __restrict__ v,
-                                                     float* __restrict__ rc, const int* __restrict__ skip)
+                                                     float* __restrict__ rc, const int* __restrict__ skip,
+                                                     const GmState* __restrict__ st, const int* __restrict__ aggBlk)
 {
   if (skip && *skip) return;
   const int a = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (a >= nAgg) return;
+  if (st && sys_idle(st, aggBlk[a])) return;
   double s = 0.0;
   for (int t = aggPtr[a] + lane; t < aggPtr[a + 1]; t += 64) s += (double)v[(size_t)4 * aggNodes[t] + 3];
   s = wave_sum(s);
@@ -640,12 +650,13 @@ __device__ __forceinline__ cl_f4 nt_f4(const float4* p)
 __global__ __launch_bounds__(256) void k_cl_gemv(const int* __restrict__ aggOff, const int* __restrict__ blkN,
                                                  const long long* __restrict__ blkOff, const float* __restrict__ Ai,
                                                  const float* __restrict__ rc, float* __restrict__ zc,
-                                                 const int* __restrict__ skip)
+                                                 const int* __restrict__ skip, const GmState* __restrict__ st)
 {
   extern __shared__ float xs[];
   if (skip && *skip) return;
   const int bk = blockIdx.y, a0 = aggOff[bk], n = aggOff[bk + 1] - a0, N = blkN[bk];
   if ((int)blockIdx.x * 4 >= n) return;
+  if (st && sys_idle(st, bk)) return;  // block bk is system bk's
   const int r = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   const bool live = r < n;
   const float4* row = reinterpret_cast<const float4*>(Ai + blkOff[bk] + (size_t)(live ? r : 0) * N);
@@ -697,11 +708,13 @@ __global__ __launch_bounds__(256) void k_cl_apply(int nNo, const T* __restrict__
                                                   const int* __restrict__ apAgg, const float* __restrict__ apVal,
                                                   const float* __restrict__ zc, const int* __restrict__ agg,
                                                   const float* __restrict__ dinv, double* __restrict__ z64,
-                                                  float* __restrict__ z32, const int* __restrict__ skip, int mode)
+                                                  float* __restrict__ z32, const int* __restrict__ skip, int mode,
+                                                  const GmState* __restrict__ st, const int* __restrict__ sysOfNode)
 {
   if (skip && *skip) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nNo) return;
+  if (st && sys_idle(st, sysOfNode[i])) return;
   float r[4];
 #pragma unroll
   for (int d = 0; d < 4; d++) r[d] = (float)v[(size_t)4 * i + d];
@@ -838,18 +851,30 @@ int coarse_prepare_dev(svg_lhs* L, const double* Val, int aggSize, hipStream_t s
 
 template <typename T>
 static void coarse_apply_t(const svg_lhs* L, const T* v, double* z64, float* z32, const int* skip, hipStream_t s,
-                           int mode = 0)
+                           int mode = 0, const GmState* st = nullptr, const int* sysOfNode = nullptr)
 {
   const CoarseLevel* c = L->coarse;
   const int nAgg = c->nAgg;
   launch("cl_restrict", k_cl_restrict<T>, dim3(blocks_for(nAgg, 4)), dim3(256), 0, s, nAgg,
-         (const int*)c->d_aggPtr, (const int*)c->d_aggNodes, v, c->d_rc, skip);
+         (const int*)c->d_aggPtr, (const int*)c->d_aggNodes, v, c->d_rc, skip, st, (const int*)c->d_aggBlk);
   launch("cl_gemv", k_cl_gemv, dim3(blocks_for(c->maxN, 4), c->nblk), dim3(256), sizeof(float) * c->maxN, s,
          (const int*)c->d_aggOff, (const int*)c->d_blkN, (const long long*)c->d_blkOff, (const float*)c->d_Ai,
-         (const float*)c->d_rc, c->d_zc, skip);
+         (const float*)c->d_rc, c->d_zc, skip, st);
   launch("cl_apply", k_cl_apply<T>, dim3(blocks_for(L->nNo, 256)), dim3(256), 0, s, L->nNo, v,
          (const int*)c->d_apPtr, (const int*)c->d_apAgg, (const float*)c->d_apVal, (const float*)c->d_zc,
-         (const int*)c->d_agg, (const float*)c->d_dinv, z64, z32, skip, mode);
+         (const int*)c->d_agg, (const float*)c->d_dinv, z64, z32, skip, mode, st, sysOfNode);
+}
+
+void coarse_apply_batch_dev(const svg_lhs* L, const float* v, double* z64, float* z32, const GmState* st,
+                            const int* sysOfNode, hipStream_t s)
+{
+  coarse_apply_t<float>(L, v, z64, z32, nullptr, s, 0, st, sysOfNode);
+}
+
+void coarse_apply_batch_dev(const svg_lhs* L, const double* v, double* z64, float* z32, const GmState* st,
+                            const int* sysOfNode, hipStream_t s)
+{
+  coarse_apply_t<double>(L, v, z64, z32, nullptr, s, 0, st, sysOfNode);
 }
 
 void coarse_apply_dev(const svg_lhs* L, const float* v, double* z64, float* z32, const int* skip, hipStream_t s)

@@ -188,6 +188,12 @@ int coarse_check(const svg_lhs* L, hipStream_t s);
 // z = M(v) in FP64 (z64) and/or FP32 (z32); skip: kernels return at once when *skip.
 void coarse_apply_dev(const svg_lhs* L, const float* v, double* z64, float* z32, const int* skip, hipStream_t s);
 void coarse_apply_dev(const svg_lhs* L, const double* v, double* z64, float* z32, const int* skip, hipStream_t s);
+// The batched solve's form (one coarse block per system): the systems that do
+// not iterate in the cycle (per st and sysOfNode) are skipped.
+void coarse_apply_batch_dev(const svg_lhs* L, const float* v, double* z64, float* z32, const GmState* st,
+                            const int* sysOfNode, hipStream_t s);
+void coarse_apply_batch_dev(const svg_lhs* L, const double* v, double* z64, float* z32, const GmState* st,
+                            const int* sysOfNode, hipStream_t s);
 // With another smoother: r2 = v - A P zc (FP64; the coarse solution zc is kept),
 // then, after the smoother wrote z from r2, z(3, i) += zc(agg(i)).
 void coarse_residual_dev(const svg_lhs* L, const float* v, double* r2, const int* skip, hipStream_t s);

@@ -2661,12 +2661,14 @@ static int batch_cycle(svg_batch* B, const double* Val, const double* R, const B
       // z_i = M(u_i) with the systems' dense coarse blocks; the systems that fell
       // back keep z_i = u_i (their flexible basis is the plain one)
       if (mixed) {
-        coarse_apply_dev(L, (const float*)(B->v32 + B->ld * i), nullptr, B->z32 + B->ld * i, nullptr, s);
+        coarse_apply_batch_dev(L, (const float*)(B->v32 + B->ld * i), nullptr, B->z32 + B->ld * i,
+                               (const GmState*)B->st, (const int*)B->d_sysOfNode, s);
         hipLaunchKernelGGL(k_bz32fb, dim3(nbv), dim3(256), 0, s, n, (const float*)(B->v32 + B->ld * i),
                            B->z32 + B->ld * i, (const GmState*)B->st, (const int*)B->d_sysOfNode);
       } else {
         double* zi = B->z + B->ld * i;
-        coarse_apply_dev(L, (const double*)ui, zi, nullptr, nullptr, s);
+        coarse_apply_batch_dev(L, (const double*)ui, zi, nullptr, (const GmState*)B->st,
+                               (const int*)B->d_sysOfNode, s);
         hipLaunchKernelGGL(k_bzcopy, dim3(nbv), dim3(256), 0, s, n, (const double*)ui, zi, (float*)nullptr,
                            (const GmState*)B->st, (const int*)B->d_sysOfNode, 1);
         sin = zi;
