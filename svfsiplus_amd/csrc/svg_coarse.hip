@@ -609,7 +609,8 @@ __global__ __launch_bounds__(256) void k_cl_d2f(size_t n, const double* __restri
 // rc(a) = sum of v(3, i) over the nodes i of aggregate a (one wavefront per
 // aggregate: lanes at stride 64 in node order, then a fixed butterfly).
 // The batched solve's gate (st non-null): the work of a system that does not
-// iterate in this cycle (converged before it, or during it) is skipped.
+// iterate in this cycle (converged before it, or during it) is skipped; the
+// apply gives a system that fell back to the diagonal preconditioner z = v.
 __device__ __forceinline__ bool sys_idle(const GmState* st, int sys)
 {
   const GmState& S = st[sys];
@@ -714,7 +715,20 @@ __global__ __launch_bounds__(256) void k_cl_apply(int nNo, const T* __restrict__
   if (skip && *skip) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nNo) return;
-  if (st && sys_idle(st, sysOfNode[i])) return;
+  if (st) {
+    const GmState& S = st[sysOfNode[i]];
+    if (!S.act || S.done) return;
+    if (S.fallback) {
+      // a system that fell back to the diagonal preconditioner: its flexible
+      // basis vector is the plain one, z_i = v_i
+#pragma unroll
+      for (int d = 0; d < 4; d++) {
+        if (z32) z32[(size_t)4 * i + d] = (float)v[(size_t)4 * i + d];
+        if (z64) z64[(size_t)4 * i + d] = (double)v[(size_t)4 * i + d];
+      }
+      return;
+    }
+  }
   float r[4];
 #pragma unroll
   for (int d = 0; d < 4; d++) r[d] = (float)v[(size_t)4 * i + d];

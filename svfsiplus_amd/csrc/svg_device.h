@@ -189,7 +189,8 @@ int coarse_check(const svg_lhs* L, hipStream_t s);
 void coarse_apply_dev(const svg_lhs* L, const float* v, double* z64, float* z32, const int* skip, hipStream_t s);
 void coarse_apply_dev(const svg_lhs* L, const double* v, double* z64, float* z32, const int* skip, hipStream_t s);
 // The batched solve's form (one coarse block per system): the systems that do
-// not iterate in the cycle (per st and sysOfNode) are skipped.
+// not iterate in the cycle (per st and sysOfNode) are skipped, and the
+// systems that fell back to the diagonal preconditioner get z = v.
 void coarse_apply_batch_dev(const svg_lhs* L, const float* v, double* z64, float* z32, const GmState* st,
                             const int* sysOfNode, hipStream_t s);
 void coarse_apply_batch_dev(const svg_lhs* L, const double* v, double* z64, float* z32, const GmState* st,

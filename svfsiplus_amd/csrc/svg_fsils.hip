@@ -2593,18 +2593,6 @@ struct BatchMode {
   bool flex() const { return net || coarse; }             // flexible GMRES (per-system fallback)
 };
 
-// The two-level preconditioner's FP32 flexible basis vector z32 = v32 for the
-// systems that fell back to the diagonal preconditioner (and still iterate).
-__global__ __launch_bounds__(256) void k_bz32fb(int n, const float* __restrict__ v32, float* __restrict__ z32,
-                                                const GmState* __restrict__ st, const int* __restrict__ sysOfNode)
-{
-  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= (size_t)n) return;
-  const GmState& S = st[sysOfNode[k >> 2]];
-  if (!S.act || S.done || !S.fallback) return;
-  z32[k] = v32[k];
-}
-
 // z_j = u_j (and its FP32 copy) for the systems still iterating in the cycle;
 // after a network application (onlyFallback) only for the systems that fell
 // back to the diagonal preconditioner, whose flexible basis is then the plain
@@ -2659,18 +2647,15 @@ static int batch_cycle(svg_batch* B, const double* Val, const double* R, const B
     const double* sin = ui;
     if (tl) {
       // z_i = M(u_i) with the systems' dense coarse blocks; the systems that fell
-      // back keep z_i = u_i (their flexible basis is the plain one)
+      // back keep z_i = u_i (their flexible basis is the plain one; the apply
+      // writes it)
       if (mixed) {
         coarse_apply_batch_dev(L, (const float*)(B->v32 + B->ld * i), nullptr, B->z32 + B->ld * i,
                                (const GmState*)B->st, (const int*)B->d_sysOfNode, s);
-        hipLaunchKernelGGL(k_bz32fb, dim3(nbv), dim3(256), 0, s, n, (const float*)(B->v32 + B->ld * i),
-                           B->z32 + B->ld * i, (const GmState*)B->st, (const int*)B->d_sysOfNode);
       } else {
         double* zi = B->z + B->ld * i;
         coarse_apply_batch_dev(L, (const double*)ui, zi, nullptr, (const GmState*)B->st,
                                (const int*)B->d_sysOfNode, s);
-        hipLaunchKernelGGL(k_bzcopy, dim3(nbv), dim3(256), 0, s, n, (const double*)ui, zi, (float*)nullptr,
-                           (const GmState*)B->st, (const int*)B->d_sysOfNode, 1);
         sin = zi;
       }
     } else if (netStep(i)) {
