@@ -150,6 +150,7 @@ struct KrylovWork {
   float* v32 = nullptr;       // mixed precision: the Krylov basis in FP32, (sD+1) vectors at stride ld
   double* z = nullptr;        // flexible GMRES: preconditioned basis z_j = M(u_j), sD vectors
   float* z32 = nullptr;       // mixed precision, two-level preconditioner: z_j in FP32, sD vectors
+  float* w32 = nullptr;       // mixed precision, two-level preconditioner: the FP32 SpMV's output A z_i
   double* r2 = nullptr;       // two-level with the network as smoother: its input B (u - A P zc)
   double* scal = nullptr;     // scratch scalars for standalone dots
 };

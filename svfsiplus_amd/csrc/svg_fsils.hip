@@ -185,10 +185,10 @@ static void spmv(const svg_lhs* L, const double* K, const double* U, double* KU,
 // SpMV of the mixed-precision solve: W = K32 U32 with FP32 copies of the
 // scaled matrix and of the current Arnoldi vector, FP32 accumulation in the
 // reference's block order, one lane per (row, block-row i), W stored in FP64.
-template <int UNR>
+template <int UNR, typename TW = double>
 __device__ __forceinline__ void spmv32_row(int row, int i, const int* __restrict__ rowPtr,
                                            const int* __restrict__ colPtr, const float* __restrict__ K,
-                                           const float* __restrict__ U, double* __restrict__ W, size_t k)
+                                           const float* __restrict__ U, TW* __restrict__ W, size_t k)
 {
   const int s = rowPtr[row], e = rowPtr[row + 1];
   float acc = 0.0f;
@@ -219,16 +219,18 @@ __device__ __forceinline__ void spmv32_row(int row, int i, const int* __restrict
     acc = __builtin_fmaf(a.z, u.z, acc);
     acc = __builtin_fmaf(a.w, u.w, acc);
   }
-  W[k] = (double)acc;
+  W[k] = (TW)acc;
 }
 
 // SpMV of the mixed-precision solve: W = K32 U32 with FP32 copies of the
 // scaled matrix and of the current Arnoldi vector, FP32 accumulation in the
-// reference's block order, one lane per (row, block-row i), W stored in FP64.
-template <int UNR>
+// reference's block order, one lane per (row, block-row i), W stored in FP64
+// or, where no FP64 term is added to it, in FP32 (the same value: the sum is
+// an FP32 number).
+template <int UNR, typename TW = double>
 __global__ __launch_bounds__(256) void k_spmv32(int nNo, const int* __restrict__ rowPtr,
                                                 const int* __restrict__ colPtr, const float* __restrict__ K,
-                                                const float* __restrict__ U, double* __restrict__ W,
+                                                const float* __restrict__ U, TW* __restrict__ W,
                                                 const int* __restrict__ skip, int xcd)
 {
   if (skip && *skip) return;
@@ -240,7 +242,7 @@ __global__ __launch_bounds__(256) void k_spmv32(int nNo, const int* __restrict__
   const size_t k = (size_t)bx * 256 + threadIdx.x;
   const int row = (int)(k >> 2), i = (int)(k & 3);
   if (row >= nNo) return;
-  spmv32_row<UNR>(row, i, rowPtr, colPtr, K, U, W, k);
+  spmv32_row<UNR, TW>(row, i, rowPtr, colPtr, K, U, W, k);
 }
 
 // The batched solve's FP32 SpMV: rows of systems that stopped iterating are skipped.
@@ -309,6 +311,30 @@ static void spmv32(const svg_lhs* L, const float* K32, const float* U32, double*
     case 1: spmv32_launch<1>(L, K32, U32, W, skip, sysOfNode, st, sv.xcd, s, name); break;
     case 4: spmv32_launch<4>(L, K32, U32, W, skip, sysOfNode, st, sv.xcd, s, name); break;
     default: spmv32_launch<2>(L, K32, U32, W, skip, sysOfNode, st, sv.xcd, s, name); break;
+  }
+}
+
+// W = K32 U32 with an FP32 W (the lhs's tuned variant).
+static void spmv32_f(const svg_lhs* L, const float* K32, const float* U32, float* W, const int* skip, hipStream_t s)
+{
+  const Spmv32Variant& sv = kSpmv32Variants[L->spmv32Variant >= 0 ? L->spmv32Variant : 0];
+  unsigned nb = blocks_for((size_t)4 * L->nNo, 256);
+  const int use_xcd = (sv.xcd && nb >= 64) ? 1 : 0;
+  if (use_xcd) nb = (nb + 7) & ~7u;
+  const double bytes = 68.0 * L->nnz + 36.0 * L->nNo;  // block + col; rowPtr, U (FP32), W (FP32)
+  switch (sv.unroll) {
+    case 1:
+      launch_b("spmv32", bytes, k_spmv32<1, float>, dim3(nb), dim3(256), 0, s, L->nNo, (const int*)L->d_rowPtr,
+               (const int*)L->d_colPtr, K32, U32, W, skip, use_xcd);
+      break;
+    case 4:
+      launch_b("spmv32", bytes, k_spmv32<4, float>, dim3(nb), dim3(256), 0, s, L->nNo, (const int*)L->d_rowPtr,
+               (const int*)L->d_colPtr, K32, U32, W, skip, use_xcd);
+      break;
+    default:
+      launch_b("spmv32", bytes, k_spmv32<2, float>, dim3(nb), dim3(256), 0, s, L->nNo, (const int*)L->d_rowPtr,
+               (const int*)L->d_colPtr, K32, U32, W, skip, use_xcd);
+      break;
   }
 }
 
@@ -393,8 +419,8 @@ constexpr double FB_DROP = 1e-4;  // learned preconditioner: smallest relative d
 __device__ __forceinline__ float basis_load(const float* p) { return __builtin_nontemporal_load(p); }
 __device__ __forceinline__ double basis_load(const double* p) { return *p; }
 
-template <int G, typename T>
-__device__ __forceinline__ void span_dots(const double* __restrict__ w, const T* __restrict__ base, size_t ld, int m,
+template <int G, typename T, typename TWv = double>
+__device__ __forceinline__ void span_dots(const TWv* __restrict__ w, const T* __restrict__ base, size_t ld, int m,
                                           bool self, size_t k0, size_t kend, double* lds, int mc)
 {
 #pragma clang fp contract(off)
@@ -412,7 +438,7 @@ __device__ __forceinline__ void span_dots(const double* __restrict__ w, const T*
     const size_t k = k0 + (size_t)r * RED_BS;
     in[r] = k < kend;
     kc[r] = in[r] ? k : kend - 1;
-    const double v = w[kc[r]];
+    const double v = (double)w[kc[r]];
     wr[r] = in[r] ? v : 0.0;
   }
   int j = 0;
@@ -490,7 +516,8 @@ __global__ __launch_bounds__(RED_BS) void k_mdot(int n, const double* __restrict
 // The mixed-precision solve keeps its Krylov basis in FP32 (a compressed
 // basis; the arithmetic stays FP64): dots of w against the m FP32 vectors
 // base + j*ld, then w.w as column m, in k_mdot's order; partial[j*nblk + b].
-__global__ __launch_bounds__(RED_BS) void k_mdot_cb(int n, const double* __restrict__ w, const float* __restrict__ base,
+template <typename TWv>
+__global__ __launch_bounds__(RED_BS) void k_mdot_cb(int n, const TWv* __restrict__ w, const float* __restrict__ base,
                                                     size_t ld, int m, double* __restrict__ partial,
                                                     const int* __restrict__ skip)
 {
@@ -740,7 +767,8 @@ __global__ __launch_bounds__(256) void k_gs_update(int n, double* __restrict__ u
 // the result is stored as the FP32 basis vector v_{i+1} (the next SpMV input)
 // and, with keep64, in FP64 over w for a step that reads it (the network's
 // input, a coupled face's rank-one term).
-__global__ __launch_bounds__(256) void k_gs_update_cb(int n, double* __restrict__ w, float* __restrict__ v32,
+template <typename TWv>
+__global__ __launch_bounds__(256) void k_gs_update_cb(int n, TWv* __restrict__ w, float* __restrict__ v32,
                                                       size_t ld, int i, const double* __restrict__ coef,
                                                       const GmState* __restrict__ st, int keep64)
 {
@@ -748,7 +776,7 @@ __global__ __launch_bounds__(256) void k_gs_update_cb(int n, double* __restrict_
   if (st->done) return;
   const size_t k = 2 * ((size_t)blockIdx.x * blockDim.x + threadIdx.x);
   if (k >= (size_t)n) return;
-  double2 v = *reinterpret_cast<const double2*>(w + k);
+  double2 v = make_double2((double)w[k], (double)w[k + 1]);
   // the FP32 basis streamed non-temporally (see basis_load), eight vectors'
   // loads issued before their (sequential) updates
   for (int j0 = 0; j0 <= i; j0 += 8) {
@@ -771,7 +799,8 @@ __global__ __launch_bounds__(256) void k_gs_update_cb(int n, double* __restrict_
   const unsigned long long o = (unsigned long long)__float_as_uint((float)v.x) |
                                ((unsigned long long)__float_as_uint((float)v.y) << 32);
   __builtin_nontemporal_store(o, reinterpret_cast<unsigned long long*>(v32 + (size_t)(i + 1) * ld + k));
-  if (keep64) *reinterpret_cast<double2*>(w + k) = v;
+  if constexpr (sizeof(TWv) == sizeof(double))
+    if (keep64) *reinterpret_cast<double2*>(w + k) = v;
 }
 
 // z_i = u_i (and its FP32 copy): an Arnoldi step of the flexible solve that
@@ -1264,6 +1293,7 @@ void krylov_free(KrylovWork& k)
   dev_free(k.v32);
   dev_free(k.z);
   dev_free(k.z32);
+  dev_free(k.w32);
   dev_free(k.r2);
   if (k.h_st) (void)hipHostFree(k.h_st);
   k.h_st = nullptr;
@@ -1363,15 +1393,17 @@ static int dot_device(int n, const double* w, const double* base, size_t ld, int
 
 // Arnoldi dots of the compressed-basis solve: w against v_0..v_{m-1} (FP32)
 // and w.w, then the finaliser over the m + 1 sums.
-static int dot_device_cb(int n, const double* w, const float* base, size_t ld, int m, KrylovWork& k, FinArgs fa,
+template <typename TWv>
+static int dot_device_cb(int n, const TWv* w, const float* base, size_t ld, int m, KrylovWork& k, FinArgs fa,
                          hipStream_t s)
 {
   int nblk = (n + RED_SPAN - 1) / RED_SPAN;
   if (nblk < 1) nblk = 1;
   if (m + 1 > MAXM) return set_error(SVG_ERR_ARG, "reduction: too many sums");
-  // algorithmic bytes: w (FP64) and the m FP32 basis vectors read, the partials written
-  launch_b("mdot_cb", 8.0 * n + 4.0 * n * m + 8.0 * (m + 1) * nblk, k_mdot_cb, dim3(nblk), dim3(RED_BS),
-           (size_t)RED_NW * (m + 1) * sizeof(double), s, n, w, base, ld, m, k.partial, (const int*)&k.st->done);
+  // algorithmic bytes: w (FP64 or FP32) and the m FP32 basis vectors read, the partials written
+  launch_b("mdot_cb", (double)sizeof(TWv) * n + 4.0 * n * m + 8.0 * (m + 1) * nblk, k_mdot_cb<TWv>, dim3(nblk),
+           dim3(RED_BS), (size_t)RED_NW * (m + 1) * sizeof(double), s, n, w, base, ld, m, k.partial,
+           (const int*)&k.st->done);
   launch_b("mdot_fin", 8.0 * (m + 1) * nblk, k_fin, dim3(1), dim3(FIN_BS), 0, s, fa, (const double*)k.partial, nblk,
            m + 1, k.st, k.h, k.cs, k.sn, k.err, k.coef, k.scal, (volatile int*)k.d_done);
   return SVG_OK;
@@ -1581,7 +1613,17 @@ static int run_cycle(const Op& A, KrylovWork& k, int sD, FinArgs fa, double* X, 
       // else: the previous step's Gram-Schmidt update wrote z_i = u_i
       sin = k.z + k.ld * i;
     }
-    if (mixed) {
+    if (mixed && tl && !coupled && k.w32) {
+      // the SpMV's output in FP32 (no FP64 term is added to it here), read by
+      // the dots and the update as FP64 values: the same arithmetic, half the bytes
+      spmv32_f(A.L, K32, (const float*)(k.z32 + k.ld * i), k.w32, (const int*)&k.st->done, s);
+      int rc = dot_device_cb(n, (const float*)k.w32, k.v32, k.ld, i + 1, k, fa, s);
+      if (rc) return rc;
+      postFin[i] = prof_mark();
+      // algorithmic bytes: w (FP32) and v_0..v_i read, v_{i+1} (FP32) written
+      launch_b("gs_update_cb", 4.0 * n + 4.0 * n * (i + 2), k_gs_update_cb<float>, dim3(nbv2), dim3(256), 0, s, n,
+               k.w32, k.v32, k.ld, i, (const double*)k.coef, (const GmState*)k.st, 0);
+    } else if (mixed) {
       spmv32(A.L, K32,
              tl ? (const float*)(k.z32 + k.ld * i)
                 : (netStep(i) ? (const float*)k.u32 : (const float*)(k.v32 + k.ld * i)),
@@ -1592,8 +1634,8 @@ static int run_cycle(const Op& A, KrylovWork& k, int sD, FinArgs fa, double* X, 
       postFin[i] = prof_mark();
       const int keep64 = (!tl && (coupled || (i + 1 < sD && netStep(i + 1)))) ? 1 : 0;
       // algorithmic bytes: w and v_0..v_i read, v_{i+1} (FP32) written, w rewritten with keep64
-      launch_b("gs_update_cb", 8.0 * n + 4.0 * n * (i + 2) + (keep64 ? 8.0 * n : 0.0), k_gs_update_cb, dim3(nbv2),
-               dim3(256), 0, s, n, w, k.v32, k.ld, i, (const double*)k.coef, (const GmState*)k.st, keep64);
+      launch_b("gs_update_cb", 8.0 * n + 4.0 * n * (i + 2) + (keep64 ? 8.0 * n : 0.0), k_gs_update_cb<double>,
+               dim3(nbv2), dim3(256), 0, s, n, w, k.v32, k.ld, i, (const double*)k.coef, (const GmState*)k.st, keep64);
     } else {
       op_apply(A, sin, w, nullptr, &k.st->done, s);
       int rc = dot_device(n, w, k.u, k.ld, i + 2, k.partial, k.scal, k.st, fa, k.h, k.cs, k.sn, k.err, k.coef, s,
@@ -1784,6 +1826,7 @@ static int gmres_v_learned(svg_lhs* L, svg_subls* ls, const double* Val, double*
     if (!k.v32 && (rc = dev_alloc(&k.v32, k.ld * (k.sD + 1)))) return rc;
     if (!L->d_val32 && (rc = dev_alloc(&L->d_val32, (size_t)16 * L->nnz))) return rc;
     if (coarse > 0 && !k.z32 && (rc = dev_alloc(&k.z32, k.ld * k.sD))) return rc;
+    if (coarse > 0 && !k.w32 && (rc = dev_alloc(&k.w32, (size_t)n))) return rc;
   }
   if (coarse > 0 && net && !k.r2 && (rc = dev_alloc(&k.r2, (size_t)n))) return rc;
   const Op A{L, Val, 4};
