@@ -524,13 +524,16 @@ __global__ __launch_bounds__(BS) void k_layer(int n, int rows, const int* __rest
       int row, col;
       acc_rc<F>(wave, lane, qi, v, row, col);
       const float p = acc[qi][v];
-      const bf16 ho = f2bf(bf2f(X[row * XS + col]) + fmaxf(p, 0.f));
-      X[row * XS + col] = ho;
-      const int r = r0 + row;
-      if (r < rows) Hout[(size_t)r * F + col] = ho;
+      X[row * XS + col] = f2bf(bf2f(X[row * XS + col]) + fmaxf(p, 0.f));
     }
+  __syncthreads();
+  // the tile's updated h rows leave LDS as 16-byte stores (the accumulator
+  // layout would give 2-byte ones)
+  for (int q = t; q < TILE * F / 8; q += BS) {
+    const int row = q / (F / 8), c8 = q % (F / 8), r = r0 + row;
+    if (r < rows) reinterpret_cast<uint4*>(Hout + (size_t)r * F)[c8] = reinterpret_cast<const uint4*>(X + row * XS)[c8];
+  }
   if (Wd) {
-    __syncthreads();
     const int node = t >> 2, d = t & 3, r = r0 + node;
     if (r < rows) {
       const uint4* xr = reinterpret_cast<const uint4*>(X + node * XS);
