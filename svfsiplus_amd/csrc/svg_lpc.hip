@@ -199,8 +199,10 @@ __global__ __launch_bounds__(BS) void k_encode(int n, int rows, const double* __
                                                const bf16* __restrict__ W0, bf16* __restrict__ H,
                                                const int* __restrict__ skip, RowGate gate)
 {
+  constexpr int HS = F + 8;  // LDS row stride of the output tile (bf16; 16-byte aligned rows)
   __shared__ __align__(16) bf16 U[TILE * US];
   __shared__ __align__(16) bf16 Ws[F * US];
+  __shared__ __align__(16) bf16 Ho[TILE * HS];
   if (skip && *skip) return;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int ntiles = (rows + TILE - 1) / TILE;
@@ -220,8 +222,15 @@ __global__ __launch_bounds__(BS) void k_encode(int n, int rows, const double* __
       for (int v = 0; v < 4; v++) {
         int row, col;
         acc_rc<F>(wave, lane, qi, v, row, col);
-        if (r0 + row < rows) H[(size_t)(r0 + row) * F + col] = f2bf(acc[qi][v]);
+        Ho[row * HS + col] = f2bf(acc[qi][v]);
       }
+    __syncthreads();
+    // the tile's h rows leave LDS as 16-byte stores
+    for (int q = t; q < TILE * F / 8; q += BS) {
+      const int row = q / (F / 8), c8 = q % (F / 8);
+      if (r0 + row < rows)
+        reinterpret_cast<uint4*>(H + (size_t)(r0 + row) * F)[c8] = reinterpret_cast<const uint4*>(Ho + row * HS)[c8];
+    }
     __syncthreads();
   }
 }
