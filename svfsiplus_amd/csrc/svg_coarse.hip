@@ -699,13 +699,86 @@ __global__ __launch_bounds__(256) void k_cl_gemv(const int* __restrict__ aggOff,
   if (lane == 0) zc[a0 + r] = s;
 }
 
+// Lane K of each quad's value, broadcast to the quad (DPP quad_perm).
+template <int K>
+__device__ __forceinline__ float quad_bcast_f32(float v)
+{
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), K * 0x55, 0xF, 0xF, false));
+}
+
 // With r2_i = v_i - sum_e AP(:, e) zc(apAgg(e)) (the coarse-corrected residual):
 //   mode 0  z_i = B_i r2_i + (0, 0, 0, zc(agg(i)))   (the two-level preconditioner)
 //   mode 1  z_i = r2_i
 //   mode 2  z_i = B_i r2_i
-// written in FP64 and/or FP32.
+// written in FP64 and/or FP32; a quad of lanes per node (lane d computes
+// component d of r2 and of z), so that four times as many wavefronts cover the
+// loads' latency.
 template <typename T>
 __global__ __launch_bounds__(256) void k_cl_apply(int nNo, const T* __restrict__ v, const int* __restrict__ apPtr,
+                                                  const int* __restrict__ apAgg, const float* __restrict__ apVal,
+                                                  const float* __restrict__ zc, const int* __restrict__ agg,
+                                                  const float* __restrict__ dinv, double* __restrict__ z64,
+                                                  float* __restrict__ z32, const int* __restrict__ skip, int mode,
+                                                  const GmState* __restrict__ st, const int* __restrict__ sysOfNode)
+{
+  if (skip && *skip) return;
+  // a quad of lanes per node, lane d for dof d (whole quads leave together)
+  const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = (int)(gid >> 2), d = (int)(gid & 3);
+  if (i >= nNo) return;
+  const size_t k = (size_t)4 * i + d;
+  if (st) {
+    const GmState& S = st[sysOfNode[i]];
+    if (!S.act || S.done) return;
+    if (S.fallback) {
+      // a system that fell back to the diagonal preconditioner: its flexible
+      // basis vector is the plain one, z_i = v_i
+      if (z32) z32[k] = (float)v[k];
+      if (z64) z64[k] = (double)v[k];
+      return;
+    }
+  }
+  float rd = (float)v[k];
+  // row d of the block-Jacobi inverse and the own aggregate's entry do not
+  // depend on the loop: their loads are issued before it
+  cl_f4 D = {0.f, 0.f, 0.f, 0.f};
+  if (mode != 1) D = nt_f4(reinterpret_cast<const float4*>(dinv + (size_t)16 * i) + d);
+  const float zown = (mode == 0 && d == 3) ? zc[agg[i]] : 0.f;
+  // component d of r2 over the AP entries in order, their loads issued four at a time
+  const int eb = apPtr[i], ee = apPtr[i + 1];
+  for (int e = eb; e < ee; e += 4) {
+    int g[4];
+    float a[4], c[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (e + u < ee) {
+        g[u] = apAgg[e + u];
+        a[u] = apVal[(size_t)4 * (e + u) + d];
+      }
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (e + u < ee) c[u] = zc[g[u]];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (e + u < ee) rd -= a[u] * c[u];
+  }
+  float z = rd;
+  if (mode != 1) {
+    // the quad's four components of r2 by DPP
+    const float r0 = quad_bcast_f32<0>(rd), r1 = quad_bcast_f32<1>(rd), r2c = quad_bcast_f32<2>(rd),
+                r3 = quad_bcast_f32<3>(rd);
+    z = D.x * r0 + D.y * r1 + D.z * r2c + D.w * r3;
+    if (mode == 0 && d == 3) z += zown;
+  }
+  if (z32) z32[k] = z;
+  if (z64) z64[k] = (double)z;
+}
+
+// The same with one lane per node (the batched solve's gated form: most of a
+// batch's nodes belong to meshes that stopped iterating, whose lanes leave at
+// once, so fewer lanes per node launch fewer idle workgroups).
+template <typename T>
+__global__ __launch_bounds__(256) void k_cl_apply_node(int nNo, const T* __restrict__ v, const int* __restrict__ apPtr,
                                                   const int* __restrict__ apAgg, const float* __restrict__ apVal,
                                                   const float* __restrict__ zc, const int* __restrict__ agg,
                                                   const float* __restrict__ dinv, double* __restrict__ z64,
@@ -874,9 +947,14 @@ static void coarse_apply_t(const svg_lhs* L, const T* v, double* z64, float* z32
   launch("cl_gemv", k_cl_gemv, dim3(blocks_for(c->maxN, 4), c->nblk), dim3(256), sizeof(float) * c->maxN, s,
          (const int*)c->d_aggOff, (const int*)c->d_blkN, (const long long*)c->d_blkOff, (const float*)c->d_Ai,
          (const float*)c->d_rc, c->d_zc, skip, st);
-  launch("cl_apply", k_cl_apply<T>, dim3(blocks_for(L->nNo, 256)), dim3(256), 0, s, L->nNo, v,
-         (const int*)c->d_apPtr, (const int*)c->d_apAgg, (const float*)c->d_apVal, (const float*)c->d_zc,
-         (const int*)c->d_agg, (const float*)c->d_dinv, z64, z32, skip, mode, st, sysOfNode);
+  if (st)
+    launch("cl_apply", k_cl_apply_node<T>, dim3(blocks_for(L->nNo, 256)), dim3(256), 0, s, L->nNo, v,
+           (const int*)c->d_apPtr, (const int*)c->d_apAgg, (const float*)c->d_apVal, (const float*)c->d_zc,
+           (const int*)c->d_agg, (const float*)c->d_dinv, z64, z32, skip, mode, st, sysOfNode);
+  else
+    launch("cl_apply", k_cl_apply<T>, dim3(blocks_for((size_t)4 * L->nNo, 256)), dim3(256), 0, s, L->nNo, v,
+           (const int*)c->d_apPtr, (const int*)c->d_apAgg, (const float*)c->d_apVal, (const float*)c->d_zc,
+           (const int*)c->d_agg, (const float*)c->d_dinv, z64, z32, skip, mode, st, sysOfNode);
 }
 
 void coarse_apply_batch_dev(const svg_lhs* L, const float* v, double* z64, float* z32, const GmState* st,
