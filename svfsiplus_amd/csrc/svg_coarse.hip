@@ -441,11 +441,14 @@ __device__ __forceinline__ void gj_pivot_regs(double (&v)[16], int K, double (&r
     inv = fma(inv, fma(-pv, inv, 1.0), inv);
     inv = fma(inv, fma(-pv, inv, 1.0), inv);
     const double sip = colb[buf][i];
+    // the general entry, then column p (one wavefront, a fixed register) and
+    // row p (one lane) overwritten: no per-entry selects
 #pragma unroll
-    for (int jj = 0; jj < 16; jj++) {
-      const int j = j0 + jj;
-      const double spj = rowb[buf][j];
-      v[jj] = (i == p) ? ((j == p) ? inv : spj * inv) : ((j == p) ? -sip * inv : v[jj] - sip * (spj * inv));
+    for (int jj = 0; jj < 16; jj++) v[jj] = v[jj] - sip * (rowb[buf][j0 + jj] * inv);
+    if (w == p / 16) v[p % 16] = -sip * inv;
+    if (i == p) {
+#pragma unroll
+      for (int jj = 0; jj < 16; jj++) v[jj] = (j0 + jj == p) ? inv : rowb[buf][j0 + jj] * inv;
     }
     if (t == 0 && !(fabs(pv) > 0.0 && isfinite(pv))) *flag = 1 + K * GJB + p;
   }
