@@ -99,7 +99,8 @@ def build_oracle() -> Path:
 
 def build_reference_harness() -> Path | None:
     """Compiles oracle/_ref/libsvref.so from the reference sources (only
-    where /root/reference exists; the GPU box uses the prebuilt file)."""
+    where /root/reference exists; a machine without the sources uses the
+    prebuilt file)."""
     if not Path("/root/reference/Code/Source/solver/fluid.cpp").exists():
         return None
     _run(["make", "-s", "-j8", "-f", str(ROOT / "oracle" / "ref.mk")])
