@@ -180,3 +180,44 @@ def test_two_level_keeps_coarse_inverse(torch_cuda, precision):
         L.free()
     assert np.array_equal(res[True][0][0], res[False][0][0]) and res[True][0][1] == res[False][0][1]
     assert res[True][1][1] <= 1.25 * res[False][1][1] + 2
+
+
+def test_timed_launches_are_the_solve_iterations(torch_cuda):
+    """With every launch timed, a mixed-precision two-level solve keeps exactly
+    its real work: the FP32 SpMV samples number its Arnoldi steps and the FP64
+    SpMV samples its restart checks, together ls.itr. The launches that return
+    at once because the solve is already done (the steps the host enqueues past
+    the converging one, and the cycle after a restart check that met the
+    tolerance) are dropped; a mistake in either would break the identity."""
+    import ctypes as C
+    from svfsiplus_amd.newton import DeviceLhs, fsils_solve, ls_create
+    torch = torch_cuda
+    p = W.make_pipe(8)
+    ph = W.Physics()
+    Ao, Yo, An, Yn = W.dirichlet_state(p, ph)
+    Ag, Yg = oracle.pici(ph, Ao, An, Yo, Yn)
+    R, V = oracle.construct_fluid(p, ph, Ag, Yg)
+    lib = _lib.load()
+    L = DeviceLhs(p)
+    # a first solve tunes the SpMV launch variants (untimed)
+    fsils_solve(L, ls_create(_lib.LS_GMRES, relTol=1e-6, absTol=1e-30, maxItr=1000, dimKry=50), 4, _dev(torch, R),
+                _dev(torch, V), precision=_lib.PRECISION_MIXED, coarse=32)
+    _lib.check(lib.svg_prof_sample(1))
+    _lib.check(lib.svg_prof_only(None))
+    _lib.check(lib.svg_prof_reset())
+    _lib.check(lib.svg_prof_enable(1))
+    try:
+        ls = ls_create(_lib.LS_GMRES, relTol=1e-6, absTol=1e-30, maxItr=1000, dimKry=50)
+        fsils_solve(L, ls, 4, _dev(torch, R), _dev(torch, V), precision=_lib.PRECISION_MIXED, coarse=32)
+    finally:
+        lib.svg_prof_enable(0)
+    counts = {}
+    for name in ("spmv32", "spmv_bcsr"):
+        cnt, ms = C.c_int(), C.c_double()
+        _lib.check(lib.svg_prof_get(name.encode(), C.byref(cnt), C.byref(ms)))
+        counts[name] = cnt.value
+    print(f"itr {ls.RI.itr}, timed launches {counts}")
+    assert ls.RI.suc == 1
+    assert counts["spmv_bcsr"] >= 2  # the restart checks, the last one finding the tolerance met
+    assert counts["spmv32"] + counts["spmv_bcsr"] == ls.RI.itr
+    L.free()
